@@ -1,0 +1,162 @@
+/*
+ * mumaxplus_amd.h -- C ABI of the MI355X micromagnetic time-stepping core.
+ *
+ * Every entry point returns 0 on success or a non-zero error code
+ * (MPA_ERR_*); mpa_last_error() gives the message of the last failure on the
+ * calling thread.  No entry point exits the process.  Arrays are plain
+ * pointers; field data is structure-of-arrays float32 with index
+ * c*ncells + x + nx*(y + ny*z) (the reference's Field layout,
+ * src/core/field.hpp:15-94 and src/bindings/fieldToArray.hpp:37-46).
+ * Host arrays are copied; nothing keeps a pointer to caller memory.
+ *
+ * Each entry point names the reference interface it replaces
+ * (dirkpetersen/mumax-plus, file:line).  The reference binds these classes to
+ * Python through pybind11 (src/bindings/main.cpp:3-27); this library exposes
+ * the same object model through handles.
+ */
+#ifndef MUMAXPLUS_AMD_H
+#define MUMAXPLUS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPA_OK 0
+#define MPA_ERR_INVALID 1 /* invalid argument (the reference throws std::invalid_argument) */
+#define MPA_ERR_RUNTIME 2 /* runtime error (std::runtime_error)                          */
+#define MPA_ERR_HIP 3     /* HIP / device failure (the reference exits, cudaerror.hpp:9-18) */
+#define MPA_ERR_INDEX 4   /* index out of range                                          */
+
+typedef struct mpa_world mpa_world;
+typedef struct mpa_ferromagnet mpa_ferromagnet;
+typedef struct mpa_demag_plan mpa_demag_plan;
+
+/* Message of the last error on this thread ("" if none). */
+const char* mpa_last_error(void);
+/* ABI version of this library. */
+int mpa_abi_version(void);
+/* 1 if a HIP device is usable, 0 otherwise (no other call needed first). */
+int mpa_device_available(void);
+
+/* ---------------------------------------------------------------- World -- */
+
+/* World(cellsize, mastergrid, pbc_repetitions): wrap_world.cpp:40-45,
+ * MumaxWorld ctor mumaxworld.cpp:28-31, World::World world.cpp:9-19. */
+int mpa_world_create(const double cellsize[3], const int mastergrid_size[3], const int mastergrid_origin[3],
+                     const int pbc_repetitions[3], mpa_world** out);
+int mpa_world_destroy(mpa_world* w);
+/* World.cellsize (wrap_world.cpp:46-47) */
+int mpa_world_get_cellsize(const mpa_world* w, double out[3]);
+/* World.bias_magnetic_field (wrap_world.cpp:48-49) */
+int mpa_world_set_bias_magnetic_field(mpa_world* w, const double b[3]);
+int mpa_world_get_bias_magnetic_field(const mpa_world* w, double out[3]);
+/* World.add_ferromagnet(grid, geometry_array, regions_array, name): wrap_world.cpp:52-64,
+ * MumaxWorld::addFerromagnet mumaxworld.cpp:57-80.  geometry: ncells bytes or NULL. */
+int mpa_world_add_ferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
+                              const char* name, mpa_ferromagnet** out);
+/* World.ferromagnets (wrap_world.cpp:87-88) */
+int mpa_world_num_ferromagnets(const mpa_world* w, int* out);
+int mpa_world_ferromagnet_at(mpa_world* w, int index, mpa_ferromagnet** out);
+/* World.get_ferromagnet(name) (wrap_world.cpp:79-81); *out = NULL when absent. */
+int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet** out);
+/* World.set_pbc(mastergrid, reps) / set_pbc(reps) / unset_pbc / mastergrid / pbc_repetitions
+ * (wrap_world.cpp:100-113, mumaxworld.cpp:288-342). */
+int mpa_world_set_pbc(mpa_world* w, const int mastergrid_size[3], const int mastergrid_origin[3],
+                      const int reps[3]);
+int mpa_world_set_pbc_auto(mpa_world* w, const int reps[3]);
+int mpa_world_unset_pbc(mpa_world* w);
+int mpa_world_get_pbc(const mpa_world* w, int mastergrid_size[3], int mastergrid_origin[3], int reps[3]);
+/* World.bounding_grid (wrap_world.cpp:97-99, mumaxworld.cpp:274-286) */
+int mpa_world_bounding_grid(const mpa_world* w, int size[3], int origin[3]);
+/* The HIP stream (hipStream_t) all work of this world is ordered on. */
+int mpa_world_stream(mpa_world* w, void** stream);
+
+/* ----------------------------------------------------------- Ferromagnet -- */
+
+/* Magnet.name / system.grid / geometry (wrap_magnet.cpp:17-24, wrap_system.cpp:40-51) */
+int mpa_fm_name(const mpa_ferromagnet* m, char* buf, int buflen);
+int mpa_fm_grid(const mpa_ferromagnet* m, int size[3], int origin[3]);
+int mpa_fm_geometry(const mpa_ferromagnet* m, uint8_t* out); /* ncells bytes */
+int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsingeo system.cpp:80-83 */
+/* Ferromagnet.enable_demag / enable_openbc (wrap_ferromagnet.cpp:33-38). flag: "enable_demag",
+ * "enable_openbc". */
+int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
+int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
+/* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,
+ * alpha, ku1, ku2, kc1, kc2, kc3, frozen_spins, anisU, anisC1, anisC2, bias_magnetic_field,
+ * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz. */
+int mpa_fm_param_info(const mpa_ferromagnet* m, const char* key, int* ncomp, int* is_uniform, char* name,
+                      int namelen, char* unit, int unitlen);
+int mpa_fm_param_set_uniform(mpa_ferromagnet* m, const char* key, const double* value /* ncomp */);
+int mpa_fm_param_set_cells(mpa_ferromagnet* m, const char* key, const float* values /* ncomp*ncells */);
+int mpa_fm_param_get(const mpa_ferromagnet* m, const char* key, float* out /* ncomp*ncells */);
+int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* out /* ncomp */);
+/* Variable.set / get of the magnetisation (wrap_variable.cpp:8-16, wrap_field.cpp:19-68,
+ * NormalizedVariable::set variable.cpp:76-79): count 3 (uniform) or 3*ncells. */
+int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count);
+int mpa_fm_get_magnetization(mpa_ferromagnet* m, float* out);
+/* FieldQuantity.eval / average of the quantity factories (wrap_ferromagnet.cpp:75-128,
+ * wrap_fieldquantity.cpp:8-22).  q: magnetization, exchange_field, anisotropy_field,
+ * external_field, dmi_field, demag_field, effective_field, torque, llg_torque, damping_torque,
+ * exchange_energy_density, anisotropy_energy_density, zeeman_energy_density,
+ * dmi_energy_density, demag_energy_density, total_energy_density. */
+int mpa_fm_quantity_ncomp(const mpa_ferromagnet* m, const char* q, int* ncomp);
+int mpa_fm_eval(mpa_ferromagnet* m, const char* q, float* out /* ncomp*ncells */);
+int mpa_fm_average(mpa_ferromagnet* m, const char* q, double* out /* ncomp */);
+/* ScalarQuantity.eval: max_torque, exchange_energy, anisotropy_energy, zeeman_energy,
+ * dmi_energy, demag_energy, total_energy (wrap_scalarquantity.cpp). */
+int mpa_fm_scalar(mpa_ferromagnet* m, const char* q, double* out);
+/* _demag_kernel(magnet) (wrap_magnet.cpp:56-59): reference kernel-grid layout (6, Kz, Ky, Kx).
+ * Call with out = NULL to get ksize only. */
+int mpa_fm_demag_kernel(mpa_ferromagnet* m, int ksize[3], float* out);
+
+/* ------------------------------------------------------------ TimeSolver -- */
+
+/* TimeSolver properties (wrap_timesolver.cpp:15-44, timesolver.hpp:74-82).  key: time, timestep,
+ * adaptive_timestep, max_error, headroom, lower_bound, upper_bound, sensible_factor,
+ * sensible_timestep_default; read-only counters nsteps, nrejected, nrhs. */
+int mpa_ts_get(const mpa_world* w, const char* key, double* out);
+int mpa_ts_set(mpa_world* w, const char* key, double value);
+/* TimeSolver.sensible_timestep (timesolver.cpp:37-47) */
+int mpa_ts_sensible_timestep(mpa_world* w, double* out);
+/* TimeSolver.set_method (timesolver.cpp:23-31; names of butchertableau.hpp:17-25) */
+int mpa_ts_set_method(mpa_world* w, const char* name);
+int mpa_ts_get_method(const mpa_world* w, char* buf, int buflen);
+/* TimeSolver.step / steps / run (timesolver.cpp:77-111) */
+int mpa_ts_step(mpa_world* w);
+int mpa_ts_steps(mpa_world* w, long long n);
+int mpa_ts_run(mpa_world* w, double duration);
+
+/* ------------------------------------------------------- Demag operator -- */
+
+/* StrayFieldFFTExecutor (strayfieldfft.cu:149-246) + StrayFieldKernel (strayfieldkernel.cu:11-99)
+ * as a stand-alone operator on device pointers. */
+int mpa_demag_plan_create(const int size[3], const double cellsize[3], const int mastergrid_size[3],
+                          const int pbc_repetitions[3], mpa_demag_plan** out);
+int mpa_demag_plan_destroy(mpa_demag_plan* p);
+/* h = -mu0 N * (Ms m): m, h device arrays of 3*ncells floats; msat_cells device array of ncells
+ * floats or NULL (then msat_uniform is used); geometry device array of ncells bytes or NULL;
+ * stream: hipStream_t or NULL. */
+int mpa_demag_plan_exec(mpa_demag_plan* p, const float* m, const float* msat_cells, float msat_uniform,
+                        const uint8_t* geometry, float* h, void* stream);
+/* FFT lengths, algorithmic HBM bytes per exec and device bytes held by the plan. */
+int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algorithmic_bytes,
+                        double* device_bytes);
+
+/* --------------------------------------------------------------- Timing -- */
+
+/* Time the kernels of one RK stage of world w (stage combine, demag passes, local field +
+ * torque) with HIP events on the world stream, averaged over `reps` repetitions.  Fills up
+ * to `cap` entries of names (each 32 bytes), avg_ms and algorithmic bytes per launch;
+ * *count receives the number of kernels. */
+int mpa_world_profile_stage(mpa_world* w, int reps, int cap, char* names, double* avg_ms, double* bytes,
+                            int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUMAXPLUS_AMD_H */

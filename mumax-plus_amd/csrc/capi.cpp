@@ -1,0 +1,578 @@
+// extern "C" boundary of the library; declarations in include/mumaxplus_amd.h.
+#include "../../include/mumaxplus_amd.h"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "world.hpp"
+
+using namespace mpa;
+
+struct mpa_world {
+  World w;
+  mpa_world(const double cs[3], const int mgs[3], const int mgo[3], const int reps[3]) : w(cs, mgs, mgo, reps) {}
+};
+struct mpa_ferromagnet {};  // opaque: the pointer is the Ferromagnet itself
+struct mpa_demag_plan {
+  DemagPlan plan;
+  explicit mpa_demag_plan(const DemagGeometry& g) : plan(g) {}
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return MPA_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "out of host memory";
+    return MPA_ERR_RUNTIME;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MPA_ERR_RUNTIME;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return MPA_ERR_RUNTIME;
+  }
+}
+
+void need(const void* p, const char* what) {
+  if (!p) throw Error(MPA_ERR_INVALID, std::string(what) + " must not be NULL");
+}
+
+Ferromagnet* FM(mpa_ferromagnet* m) {
+  need(m, "ferromagnet");
+  return reinterpret_cast<Ferromagnet*>(m);
+}
+const Ferromagnet* FM(const mpa_ferromagnet* m) {
+  need(m, "ferromagnet");
+  return reinterpret_cast<const Ferromagnet*>(m);
+}
+mpa_ferromagnet* H(Ferromagnet* f) { return reinterpret_cast<mpa_ferromagnet*>(f); }
+
+void copy_str(const std::string& s, char* buf, int buflen) {
+  if (!buf || buflen <= 0) return;
+  const size_t n = std::min<size_t>(s.size(), (size_t)buflen - 1);
+  std::memcpy(buf, s.data(), n);
+  buf[n] = 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char* mpa_last_error(void) { return g_last_error.c_str(); }
+int mpa_abi_version(void) { return 1; }
+
+int mpa_device_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- World
+int mpa_world_create(const double cellsize[3], const int mastergrid_size[3], const int mastergrid_origin[3],
+                     const int pbc_repetitions[3], mpa_world** out) {
+  return guard([&] {
+    need(cellsize, "cellsize");
+    need(out, "out");
+    const int zero[3] = {0, 0, 0};
+    *out = new mpa_world(cellsize, mastergrid_size ? mastergrid_size : zero,
+                         mastergrid_origin ? mastergrid_origin : zero, pbc_repetitions ? pbc_repetitions : zero);
+  });
+}
+
+int mpa_world_destroy(mpa_world* w) {
+  return guard([&] { delete w; });
+}
+
+int mpa_world_get_cellsize(const mpa_world* w, double out[3]) {
+  return guard([&] {
+    need(w, "world");
+    for (int a = 0; a < 3; ++a) out[a] = w->w.cellsize[a];
+  });
+}
+
+int mpa_world_set_bias_magnetic_field(mpa_world* w, const double b[3]) {
+  return guard([&] {
+    need(w, "world");
+    for (int a = 0; a < 3; ++a) w->w.bias[a] = b[a];
+    w->w.touch();
+  });
+}
+
+int mpa_world_get_bias_magnetic_field(const mpa_world* w, double out[3]) {
+  return guard([&] {
+    need(w, "world");
+    for (int a = 0; a < 3; ++a) out[a] = w->w.bias[a];
+  });
+}
+
+int mpa_world_add_ferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
+                              const char* name, mpa_ferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    need(size, "size");
+    const int zero[3] = {0, 0, 0};
+    Ferromagnet* f = w->w.add_ferromagnet(size, origin ? origin : zero, geometry, name);
+    if (out) *out = H(f);
+  });
+}
+
+int mpa_world_num_ferromagnets(const mpa_world* w, int* out) {
+  return guard([&] {
+    need(w, "world");
+    *out = (int)w->w.magnets.size();
+  });
+}
+
+int mpa_world_ferromagnet_at(mpa_world* w, int index, mpa_ferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    if (index < 0 || index >= (int)w->w.magnets.size()) throw Error(MPA_ERR_INDEX, "ferromagnet index out of range");
+    *out = H(w->w.magnets[index].get());
+  });
+}
+
+int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    need(name, "name");
+    *out = H(w->w.find(name));
+  });
+}
+
+int mpa_world_set_pbc(mpa_world* w, const int mastergrid_size[3], const int mastergrid_origin[3], const int reps[3]) {
+  return guard([&] {
+    need(w, "world");
+    const int zero[3] = {0, 0, 0};
+    w->w.set_pbc(mastergrid_size, mastergrid_origin ? mastergrid_origin : zero, reps);
+  });
+}
+
+int mpa_world_set_pbc_auto(mpa_world* w, const int reps[3]) {
+  return guard([&] {
+    need(w, "world");
+    w->w.set_pbc_auto(reps);
+  });
+}
+
+int mpa_world_unset_pbc(mpa_world* w) {
+  return guard([&] {
+    need(w, "world");
+    w->w.unset_pbc();
+  });
+}
+
+int mpa_world_get_pbc(const mpa_world* w, int mastergrid_size[3], int mastergrid_origin[3], int reps[3]) {
+  return guard([&] {
+    need(w, "world");
+    for (int a = 0; a < 3; ++a) {
+      if (mastergrid_size) mastergrid_size[a] = w->w.mastergrid_size[a];
+      if (mastergrid_origin) mastergrid_origin[a] = w->w.mastergrid_origin[a];
+      if (reps) reps[a] = w->w.pbc[a];
+    }
+  });
+}
+
+int mpa_world_bounding_grid(const mpa_world* w, int size[3], int origin[3]) {
+  return guard([&] {
+    need(w, "world");
+    w->w.bounding_grid(size, origin);
+  });
+}
+
+int mpa_world_stream(mpa_world* w, void** stream) {
+  return guard([&] {
+    need(w, "world");
+    *stream = (void*)w->w.stream;
+  });
+}
+
+// ----------------------------------------------------------- Ferromagnet
+int mpa_fm_name(const mpa_ferromagnet* m, char* buf, int buflen) {
+  return guard([&] { copy_str(FM(m)->name, buf, buflen); });
+}
+
+int mpa_fm_grid(const mpa_ferromagnet* m, int size[3], int origin[3]) {
+  return guard([&] {
+    const Ferromagnet* f = FM(m);
+    for (int a = 0; a < 3; ++a) {
+      if (size) size[a] = f->n[a];
+      if (origin) origin[a] = f->origin[a];
+    }
+  });
+}
+
+int mpa_fm_geometry(const mpa_ferromagnet* m, uint8_t* out) {
+  return guard([&] {
+    const Ferromagnet* f = FM(m);
+    if (f->geom_host.empty()) std::memset(out, 1, f->N);
+    else std::memcpy(out, f->geom_host.data(), f->N);
+  });
+}
+
+int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out) {
+  return guard([&] { *out = FM(m)->cellsingeo; });
+}
+
+int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(flag, "flag");
+    const std::string k = flag;
+    if (k == "enable_demag") f->enable_demag = value != 0;
+    else if (k == "enable_openbc") f->openbc = value != 0;
+    else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
+    f->world->touch();
+  });
+}
+
+int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
+  return guard([&] {
+    const Ferromagnet* f = FM(m);
+    need(flag, "flag");
+    const std::string k = flag;
+    if (k == "enable_demag") *out = f->enable_demag;
+    else if (k == "enable_openbc") *out = f->openbc;
+    else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
+  });
+}
+
+int mpa_fm_param_info(const mpa_ferromagnet* m, const char* key, int* ncomp, int* is_uniform, char* name,
+                      int namelen, char* unit, int unitlen) {
+  return guard([&] {
+    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
+    need(key, "key");
+    const Param& p = f->param(key);
+    if (ncomp) *ncomp = p.ncomp;
+    if (is_uniform) *is_uniform = p.is_uniform();
+    copy_str(p.name, name, namelen);
+    copy_str(p.unit, unit, unitlen);
+  });
+}
+
+int mpa_fm_param_set_uniform(mpa_ferromagnet* m, const char* key, const double* value) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(key, "key");
+    need(value, "value");
+    Param& p = f->param(key);
+    p.cells.release();
+    for (int c = 0; c < 3; ++c) p.uniform[c] = c < p.ncomp ? (float)value[c] : 0.f;
+    f->world->touch();
+  });
+}
+
+int mpa_fm_param_set_cells(mpa_ferromagnet* m, const char* key, const float* values) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(key, "key");
+    need(values, "values");
+    Param& p = f->param(key);
+    const size_t n = (size_t)p.ncomp * f->N;
+    if (p.cells.n != n) p.cells.allocate(n);
+    HIP_CHECK(hipMemcpy(p.cells.get(), values, n * sizeof(float), hipMemcpyHostToDevice));
+    for (int c = 0; c < 3; ++c) p.uniform[c] = 0.f;
+    f->world->touch();
+  });
+}
+
+int mpa_fm_param_get(const mpa_ferromagnet* m, const char* key, float* out) {
+  return guard([&] {
+    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
+    need(key, "key");
+    need(out, "out");
+    const Param& p = f->param(key);
+    const size_t n = (size_t)p.ncomp * f->N;
+    if (p.is_uniform()) {
+      for (int c = 0; c < p.ncomp; ++c)
+        for (int i = 0; i < f->N; ++i) out[(size_t)c * f->N + i] = p.uniform[c];
+    } else {
+      HIP_CHECK(hipMemcpy(out, p.cells.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* out) {
+  return guard([&] {
+    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
+    need(key, "key");
+    const Param& p = f->param(key);
+    if (!p.is_uniform()) throw Error(MPA_ERR_RUNTIME, "parameter '" + std::string(key) + "' is not uniform");
+    for (int c = 0; c < p.ncomp; ++c) out[c] = p.uniform[c];
+  });
+}
+
+int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(values, "values");
+    f->set_magnetization(values, count, f->world->stream);
+  });
+}
+
+int mpa_fm_get_magnetization(mpa_ferromagnet* m, float* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(out, "out");
+    f->get_magnetization(out, f->world->stream);
+  });
+}
+
+int mpa_fm_quantity_ncomp(const mpa_ferromagnet* m, const char* q, int* ncomp) {
+  return guard([&] {
+    need(q, "quantity");
+    *ncomp = FM(m)->quantity_ncomp(q);
+  });
+}
+
+int mpa_fm_eval(mpa_ferromagnet* m, const char* q, float* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(q, "quantity");
+    need(out, "out");
+    f->eval_quantity(q, out, f->world->stream);
+  });
+}
+
+int mpa_fm_average(mpa_ferromagnet* m, const char* q, double* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(q, "quantity");
+    need(out, "out");
+    f->average_quantity(q, out, f->world->stream);
+  });
+}
+
+int mpa_fm_scalar(mpa_ferromagnet* m, const char* q, double* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(q, "quantity");
+    need(out, "out");
+    *out = f->scalar_quantity(q, f->world->stream);
+  });
+}
+
+int mpa_fm_demag_kernel(mpa_ferromagnet* m, int ksize[3], float* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    DemagGeometry g;
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = f->n[a];
+      g.cs[a] = f->world->cellsize[a];
+      g.mastergrid[a] = f->world->mastergrid_size[a];
+      g.pbc[a] = f->world->pbc[a];
+    }
+    if (!out) reference_kernel_size(g, ksize);
+    else reference_kernel(g, ksize, out);
+  });
+}
+
+// ------------------------------------------------------------ TimeSolver
+int mpa_ts_get(const mpa_world* w, const char* key, double* out) {
+  return guard([&] {
+    need(w, "world");
+    need(key, "key");
+    const TimeSolver& t = w->w.solver;
+    const std::string k = key;
+    if (k == "time") *out = t.time;
+    else if (k == "timestep") *out = t.dt;
+    else if (k == "adaptive_timestep") *out = t.adaptive ? 1.0 : 0.0;
+    else if (k == "max_error") *out = t.max_error;
+    else if (k == "headroom") *out = t.headroom;
+    else if (k == "lower_bound") *out = t.lower_bound;
+    else if (k == "upper_bound") *out = t.upper_bound;
+    else if (k == "sensible_factor") *out = t.sensible_factor;
+    else if (k == "sensible_timestep_default") *out = t.sensible_default;
+    else if (k == "nsteps") *out = (double)t.nsteps;
+    else if (k == "nrejected") *out = (double)t.nrejected;
+    else if (k == "nrhs") *out = (double)t.nrhs;
+    else throw Error(MPA_ERR_INVALID, "unknown time solver property '" + k + "'");
+  });
+}
+
+int mpa_ts_set(mpa_world* w, const char* key, double value) {
+  return guard([&] {
+    need(w, "world");
+    need(key, "key");
+    TimeSolver& t = w->w.solver;
+    const std::string k = key;
+    if (k == "time") { t.time = value; w->w.touch(); }
+    else if (k == "timestep") t.dt = value;
+    else if (k == "adaptive_timestep") t.adaptive = value != 0.0;
+    else if (k == "max_error") t.max_error = value;
+    else if (k == "headroom") t.headroom = value;
+    else if (k == "lower_bound") t.lower_bound = value;
+    else if (k == "upper_bound") t.upper_bound = value;
+    else if (k == "sensible_factor") t.sensible_factor = value;
+    else if (k == "sensible_timestep_default") {
+      if (value < 0) throw Error(MPA_ERR_RUNTIME, "The sensible timestep should be larger than zero.");
+      t.sensible_default = value;
+    } else throw Error(MPA_ERR_INVALID, "unknown or read-only time solver property '" + k + "'");
+  });
+}
+
+int mpa_ts_sensible_timestep(mpa_world* w, double* out) {
+  return guard([&] {
+    need(w, "world");
+    *out = w->w.solver.sensible_timestep();
+  });
+}
+
+int mpa_ts_set_method(mpa_world* w, const char* name) {
+  return guard([&] {
+    need(w, "world");
+    need(name, "method name");
+    w->w.solver.set_method(name);
+  });
+}
+
+int mpa_ts_get_method(const mpa_world* w, char* buf, int buflen) {
+  return guard([&] {
+    need(w, "world");
+    copy_str(w->w.solver.method, buf, buflen);
+  });
+}
+
+int mpa_ts_step(mpa_world* w) {
+  return guard([&] {
+    need(w, "world");
+    w->w.solver.step();
+    HIP_CHECK(hipStreamSynchronize(w->w.stream));
+  });
+}
+
+int mpa_ts_steps(mpa_world* w, long long n) {
+  return guard([&] {
+    need(w, "world");
+    if (n < 0) throw Error(MPA_ERR_INVALID, "number of steps must be non-negative");
+    w->w.solver.steps(n);
+  });
+}
+
+int mpa_ts_run(mpa_world* w, double duration) {
+  return guard([&] {
+    need(w, "world");
+    w->w.solver.run(duration);
+  });
+}
+
+// -------------------------------------------------------- Demag operator
+int mpa_demag_plan_create(const int size[3], const double cellsize[3], const int mastergrid_size[3],
+                          const int pbc_repetitions[3], mpa_demag_plan** out) {
+  return guard([&] {
+    need(size, "size");
+    need(cellsize, "cellsize");
+    need(out, "out");
+    DemagGeometry g;
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = size[a];
+      g.cs[a] = cellsize[a];
+      g.mastergrid[a] = mastergrid_size ? mastergrid_size[a] : 0;
+      g.pbc[a] = pbc_repetitions ? pbc_repetitions[a] : 0;
+      if ((g.mastergrid[a] == 0) != (g.pbc[a] == 0))
+        throw Error(MPA_ERR_INVALID, "0 in size of mastergrid should match 0 in pbcRepetitions.");
+    }
+    *out = new mpa_demag_plan(g);
+  });
+}
+
+int mpa_demag_plan_destroy(mpa_demag_plan* p) {
+  return guard([&] { delete p; });
+}
+
+int mpa_demag_plan_exec(mpa_demag_plan* p, const float* m, const float* msat_cells, float msat_uniform,
+                        const uint8_t* geometry, float* h, void* stream) {
+  return guard([&] {
+    need(p, "plan");
+    need(m, "m");
+    need(h, "h");
+    p->plan.exec(m, PS{msat_cells, msat_uniform}, geometry, h, (hipStream_t)stream);
+  });
+}
+
+int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algorithmic_bytes, double* device_bytes) {
+  return guard([&] {
+    need(p, "plan");
+    for (int a = 0; a < 3; ++a)
+      if (fft_size) fft_size[a] = p->plan.K(a);
+    if (algorithmic_bytes) *algorithmic_bytes = p->plan.algorithmic_bytes();
+    if (device_bytes) *device_bytes = (double)p->plan.device_bytes();
+  });
+}
+
+// ---------------------------------------------------------------- Timing
+int mpa_world_profile_stage(mpa_world* w, int reps, int cap, char* names, double* avg_ms, double* bytes, int* count) {
+  return guard([&] {
+    need(w, "world");
+    need(count, "count");
+    World& W = w->w;
+    if (W.magnets.empty()) throw Error(MPA_ERR_RUNTIME, "no magnets to profile");
+    if (reps < 1) reps = 1;
+    Ferromagnet& f = *W.magnets[0];
+    hipStream_t s = W.stream;
+    f.ensure_stage_buffers(2);
+    HIP_CHECK(hipMemcpyAsync(f.x0.get(), f.m.get(), (size_t)3 * f.N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    const unsigned terms = f.active_terms();
+    const bool dem = (terms & TERM_DEMAG) != 0;
+    const int npass = dem ? f.demag().npasses() : 0;
+    const int nk = npass + 2;  // combine, demag passes, local field + torque
+    std::vector<hipEvent_t> ev(nk + 1);
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    std::vector<double> acc(nk, 0.0);
+    const double N = f.N;
+    for (int r = 0; r < reps; ++r) {
+      HIP_CHECK(hipEventRecord(ev[0], s));
+      Combo co{};
+      co.nk = 1;
+      co.k[0] = f.k[0].get();
+      co.c[0] = 0.f;
+      launch_combine(f.m.get(), f.x0.get(), co, 3, f.N, f.geom_ptr(), true, s);
+      if (dem) f.demag().exec_timed(f.m.get(), f.params.at("msat").ps(), f.geom_ptr(), f.hdemag.get(), s, &ev[1]);
+      else HIP_CHECK(hipEventRecord(ev[1], s));
+      launch_field(f.desc(terms), f.m.get(), f.hdemag.get(), f.k[1].get(), MODE_LLG, s);
+      HIP_CHECK(hipEventRecord(ev[nk], s));
+      HIP_CHECK(hipEventSynchronize(ev[nk]));
+      // ev[1] marks the end of the combine; ev[i+1] the end of kernel i
+      for (int i = 0; i < nk; ++i) {
+        float ms = 0.f;
+        const int a = (i == 0) ? 0 : i;
+        const int b = (i == 0) ? 1 : i + 1;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        acc[i] += ms;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *count = nk;
+    for (int i = 0; i < nk && i < cap; ++i) {
+      std::string nm;
+      double b = 0.0;
+      if (i == 0) {
+        nm = "k_combine";
+        b = 12.0 * 3 * N;  // x0, one k, write m (the stage-1 combination)
+      } else if (i <= npass) {
+        nm = f.demag().pass_name(i - 1);
+        b = f.demag().pass_bytes(i - 1);
+      } else {
+        nm = "k_field";
+        b = 12.0 * N * (dem ? 3 : 2);  // m, h_demag, k
+      }
+      if (names) copy_str(nm, names + 32 * i, 32);
+      if (avg_ms) avg_ms[i] = acc[i] / reps;
+      if (bytes) bytes[i] = b;
+    }
+    // restore the state
+    HIP_CHECK(hipMemcpyAsync(f.m.get(), f.x0.get(), (size_t)3 * f.N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    W.touch();
+  });
+}
+
+}  // extern "C"

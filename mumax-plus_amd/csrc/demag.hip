@@ -1,0 +1,662 @@
+// Demag convolution on MI355X: FP64 Newell spectrum build plus five
+// hand-written FFT passes (three for a single-layer grid).
+//
+// Data path of one exec() for an Nx x Ny x Nz magnet padded to Kx x Ky x Kz
+// (Hx = Kx/2+1, Hy = Ky/2+1, Hz = Kz/2+1), all complex buffers FP32:
+//
+//   xfwd : m (x-lines)          -> A[c][z][kx][y]   R2C of two real lines at a time,
+//                                                   Ms*m and the zero pad fused in
+//   yfwd : A (y-lines)          -> B[c][kx][ky][z]  only Ny inputs per line are non-zero
+//   zpass: B (z-lines)          -> B                forward z, 3x3 real symmetric kernel
+//                                                   multiply, inverse z; keeps Nz outputs
+//   yinv : B                    -> A                keeps Ny outputs
+//   xinv : A                    -> h (x-lines)      C2R of two lines at a time, crop and
+//                                                   geometry mask fused in
+//
+// For Kz == 1 the y pass does forward y, the kernel multiply and inverse y in
+// LDS (ypass2d) and B is not used.  Each pass reads or writes whole lines
+// contiguously on one side and 128-byte column segments on the other; the
+// transposition happens in LDS.
+//
+// Kernel spectrum: the Newell kernel on offsets -(N-1)..(N-1) placed
+// circularly is even (xx, yy, zz) or odd (xy: x,y; xz: x,z; yz: y,z) along
+// each axis, so its DFT is real and only the octant kx<Hx, ky<Hy, kz<Hz is
+// stored (6 floats per entry), with the sign of odd components flipped for
+// ky > Ky/2 and kz > Kz/2.  It is built in FP64 as separable cosine / sine
+// sums of the FP64 octant of the real-space kernel, pre-scaled by
+// -mu0/(Kx Ky Kz) (the unnormalised inverse), and rounded to FP32 once.
+#include "demag.hpp"
+#include "fft.hpp"
+#include "newell.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace mpa {
+
+namespace {
+
+constexpr double MU0 = 1.25663706212e-6;  // src/physics/constants.hpp:22
+constexpr int BLOCK = 256;
+
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+struct DD {
+  int nx, ny, nz;
+  int Kx, Ky, Kz;
+  int Hx, Hy, Hz;
+  int l2x, l2y, l2z;
+  int nlines;  // 3*ny*nz real x-lines
+};
+
+// ---------------------------------------------------------------- kernels
+
+__device__ __forceinline__ float load_M(const float* __restrict__ m, PS msat, const uint8_t* __restrict__ geom,
+                                        const DD& d, int line, int x) {
+  const int nyz = d.ny * d.nz;
+  const int cell = (line % nyz) * d.nx + x;
+  if (geom && !geom[cell]) return 0.0f;
+  return msat.at(cell) * m[(size_t)line * d.nx + x];
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_xfwd(DD d, const float* __restrict__ m, PS msat,
+                                               const uint8_t* __restrict__ geom, float2* __restrict__ A,
+                                               const float2* __restrict__ tw, int PP) {
+  extern __shared__ float2 lds[];
+  const int ls = d.Kx + 1;
+  const int p0 = blockIdx.x * PP;
+  for (int q = threadIdx.x; q < PP * d.Kx; q += blockDim.x) {
+    const int p = q >> d.l2x, x = q & (d.Kx - 1);
+    const int la = 2 * (p0 + p), lb = la + 1;
+    float2 v = make_float2(0.f, 0.f);
+    if (x < d.nx) {
+      if (la < d.nlines) v.x = load_M(m, msat, geom, d, la, x);
+      if (lb < d.nlines) v.y = load_M(m, msat, geom, d, lb, x);
+    }
+    lds[p * ls + x] = v;
+  }
+  __syncthreads();
+  lds_fft<E, false>(lds, d.l2x, PP, ls, tw);
+  const int nl = 2 * PP;
+  for (int q = threadIdx.x; q < d.Hx * nl; q += blockDim.x) {
+    const int kx = q / nl, li = q - kx * nl;
+    const int l = 2 * p0 + li;
+    if (l >= d.nlines) continue;
+    const int p = li >> 1;
+    const float2 z1 = lds[p * ls + kx];
+    const float2 z2 = lds[p * ls + ((d.Kx - kx) & (d.Kx - 1))];
+    float2 X;
+    if (!(li & 1)) {
+      X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+    } else {
+      X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+    }
+    const int yz = l / d.ny;
+    A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)] = X;
+  }
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B,
+                                               const float2* __restrict__ tw, int PY) {
+  extern __shared__ float2 lds[];
+  const int ls = d.Ky + 1;
+  const int nq = 3 * d.Hx * d.nz;
+  const int q0 = blockIdx.x * PY;
+  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
+    const int li = e >> d.l2y, y = e & (d.Ky - 1);
+    const int q = q0 + li;
+    float2 v = make_float2(0.f, 0.f);
+    if (q < nq && y < d.ny) {
+      const int ckx = q / d.nz, z = q - ckx * d.nz;
+      const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
+      v = A[((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny + y];
+    }
+    lds[li * ls + y] = v;
+  }
+  __syncthreads();
+  lds_fft<E, false>(lds, d.l2y, PY, ls, tw);
+  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
+    const int ky = e / PY, li = e - ky * PY;
+    const int q = q0 + li;
+    if (q >= nq) continue;
+    const int ckx = q / d.nz, z = q - ckx * d.nz;
+    B[((size_t)ckx * d.Ky + ky) * d.nz + z] = lds[li * ls + ky];
+  }
+}
+
+// h = T . m at one spectral point; T components are xx, yy, zz, xy, xz, yz.
+__device__ __forceinline__ void kernel_multiply(const float* __restrict__ T, const DD& d, int kx, int ky, int kz,
+                                                float2& a, float2& b, float2& c) {
+  int kyp = ky, kzp = kz;
+  float sy = 1.f, sz = 1.f;
+  if (ky > d.Ky / 2) { kyp = d.Ky - ky; sy = -1.f; }
+  if (kz > d.Kz / 2) { kzp = d.Kz - kz; sz = -1.f; }
+  const size_t stride = (size_t)d.Hx * d.Hy * d.Hz;
+  const size_t base = ((size_t)kx * d.Hy + kyp) * d.Hz + kzp;
+  const float nxx = T[base];
+  const float nyy = T[stride + base];
+  const float nzz = T[2 * stride + base];
+  const float nxy = sy * T[3 * stride + base];
+  const bool flat = d.Kz == 1;  // single layer: xz and yz vanish
+  const float nxz = flat ? 0.f : sz * T[4 * stride + base];
+  const float nyz = flat ? 0.f : sy * sz * T[5 * stride + base];
+  const float2 ma = a, mb = b, mc = c;
+  a = make_float2(nxx * ma.x + nxy * mb.x + nxz * mc.x, nxx * ma.y + nxy * mb.y + nxz * mc.y);
+  b = make_float2(nxy * ma.x + nyy * mb.x + nyz * mc.x, nxy * ma.y + nyy * mb.y + nyz * mc.y);
+  c = make_float2(nxz * ma.x + nyz * mb.x + nzz * mc.x, nxz * ma.y + nyz * mb.y + nzz * mc.y);
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ T,
+                                                const float2* __restrict__ tw, int PZ) {
+  extern __shared__ float2 lds[];
+  const int ls = d.Kz + 1;
+  const int npt = d.Hx * d.Ky;
+  const int pt0 = blockIdx.x * PZ;
+  const size_t cstride = (size_t)d.Hx * d.Ky;  // points per component
+  for (int e = threadIdx.x; e < 3 * PZ * d.Kz; e += blockDim.x) {
+    const int li = e >> d.l2z, z = e & (d.Kz - 1);
+    const int c = li / PZ, pp = li - c * PZ;
+    const int pt = pt0 + pp;
+    float2 v = make_float2(0.f, 0.f);
+    if (pt < npt && z < d.nz) v = B[((size_t)c * cstride + pt) * d.nz + z];
+    lds[li * ls + z] = v;
+  }
+  __syncthreads();
+  lds_fft<E, false>(lds, d.l2z, 3 * PZ, ls, tw);
+  for (int e = threadIdx.x; e < PZ * d.Kz; e += blockDim.x) {
+    const int pp = e >> d.l2z, kz = e & (d.Kz - 1);
+    const int pt = pt0 + pp;
+    if (pt >= npt) continue;
+    const int kx = pt / d.Ky, ky = pt - kx * d.Ky;
+    float2& a = lds[(0 * PZ + pp) * ls + kz];
+    float2& b = lds[(1 * PZ + pp) * ls + kz];
+    float2& c = lds[(2 * PZ + pp) * ls + kz];
+    kernel_multiply(T, d, kx, ky, kz, a, b, c);
+  }
+  __syncthreads();
+  lds_fft<E, true>(lds, d.l2z, 3 * PZ, ls, tw);
+  for (int e = threadIdx.x; e < 3 * PZ * d.Kz; e += blockDim.x) {
+    const int li = e >> d.l2z, z = e & (d.Kz - 1);
+    const int c = li / PZ, pp = li - c * PZ;
+    const int pt = pt0 + pp;
+    if (pt < npt && z < d.nz) B[((size_t)c * cstride + pt) * d.nz + z] = lds[li * ls + z];
+  }
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A,
+                                               const float2* __restrict__ tw, int PY) {
+  extern __shared__ float2 lds[];
+  const int ls = d.Ky + 1;
+  const int nq = 3 * d.Hx * d.nz;
+  const int q0 = blockIdx.x * PY;
+  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
+    const int ky = e / PY, li = e - ky * PY;
+    const int q = q0 + li;
+    float2 v = make_float2(0.f, 0.f);
+    if (q < nq) {
+      const int ckx = q / d.nz, z = q - ckx * d.nz;
+      v = B[((size_t)ckx * d.Ky + ky) * d.nz + z];
+    }
+    lds[li * ls + ky] = v;
+  }
+  __syncthreads();
+  lds_fft<E, true>(lds, d.l2y, PY, ls, tw);
+  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
+    const int li = e >> d.l2y, y = e & (d.Ky - 1);
+    const int q = q0 + li;
+    if (q >= nq || y >= d.ny) continue;
+    const int ckx = q / d.nz, z = q - ckx * d.nz;
+    const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
+    A[((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny + y] = lds[li * ls + y];
+  }
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ T,
+                                                  const float2* __restrict__ tw, int PQ) {
+  extern __shared__ float2 lds[];
+  const int ls = d.Ky + 1;
+  const int kx0 = blockIdx.x * PQ;
+  for (int e = threadIdx.x; e < 3 * PQ * d.Ky; e += blockDim.x) {
+    const int li = e >> d.l2y, y = e & (d.Ky - 1);
+    const int c = li / PQ, pp = li - c * PQ;
+    const int kx = kx0 + pp;
+    float2 v = make_float2(0.f, 0.f);
+    if (kx < d.Hx && y < d.ny) v = A[((size_t)c * d.Hx + kx) * d.ny + y];
+    lds[li * ls + y] = v;
+  }
+  __syncthreads();
+  lds_fft<E, false>(lds, d.l2y, 3 * PQ, ls, tw);
+  for (int e = threadIdx.x; e < PQ * d.Ky; e += blockDim.x) {
+    const int pp = e >> d.l2y, ky = e & (d.Ky - 1);
+    const int kx = kx0 + pp;
+    if (kx >= d.Hx) continue;
+    float2& a = lds[(0 * PQ + pp) * ls + ky];
+    float2& b = lds[(1 * PQ + pp) * ls + ky];
+    float2& c = lds[(2 * PQ + pp) * ls + ky];
+    kernel_multiply(T, d, kx, ky, 0, a, b, c);
+  }
+  __syncthreads();
+  lds_fft<E, true>(lds, d.l2y, 3 * PQ, ls, tw);
+  for (int e = threadIdx.x; e < 3 * PQ * d.Ky; e += blockDim.x) {
+    const int li = e >> d.l2y, y = e & (d.Ky - 1);
+    const int c = li / PQ, pp = li - c * PQ;
+    const int kx = kx0 + pp;
+    if (kx < d.Hx && y < d.ny) A[((size_t)c * d.Hx + kx) * d.ny + y] = lds[li * ls + y];
+  }
+}
+
+template <int E>
+__global__ void __launch_bounds__(BLOCK) k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom,
+                                               float* __restrict__ h, const float2* __restrict__ tw, int PP) {
+  extern __shared__ float2 lds[];
+  const int xs = d.Hx + 1;
+  const int nl = 2 * PP;
+  float2* X = lds;
+  float2* Z = lds + nl * xs;
+  const int zs = d.Kx + 1;
+  const int p0 = blockIdx.x * PP;
+  for (int e = threadIdx.x; e < d.Hx * nl; e += blockDim.x) {
+    const int kx = e / nl, li = e - kx * nl;
+    const int l = 2 * p0 + li;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < d.nlines) {
+      const int yz = l / d.ny;
+      v = A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)];
+    }
+    X[li * xs + kx] = v;
+  }
+  __syncthreads();
+  const int half = d.Kx >> 1;
+  for (int e = threadIdx.x; e < PP * d.Kx; e += blockDim.x) {
+    const int p = e >> d.l2x, k = e & (d.Kx - 1);
+    const bool upper = k > half;
+    const int kk = upper ? d.Kx - k : k;
+    float2 xa = X[(2 * p) * xs + kk];
+    float2 xb = X[(2 * p + 1) * xs + kk];
+    if (kk == 0 || kk == half) { xa.y = 0.f; xb.y = 0.f; }
+    float2 zv;
+    if (!upper) zv = make_float2(xa.x - xb.y, xa.y + xb.x);
+    else zv = make_float2(xa.x + xb.y, xb.x - xa.y);
+    Z[p * zs + k] = zv;
+  }
+  __syncthreads();
+  lds_fft<E, true>(Z, d.l2x, PP, zs, tw);
+  const int nyz = d.ny * d.nz;
+  for (int e = threadIdx.x; e < nl * d.nx; e += blockDim.x) {
+    const int li = e / d.nx, x = e - li * d.nx;
+    const int l = 2 * p0 + li;
+    if (l >= d.nlines) continue;
+    const float2 zv = Z[(li >> 1) * zs + x];
+    float v = (li & 1) ? zv.y : zv.x;
+    const int cell = (l % nyz) * d.nx + x;
+    if (geom && !geom[cell]) v = 0.f;
+    h[(size_t)l * d.nx + x] = v;
+  }
+}
+
+// ------------------------------------------------ kernel spectrum (FP64)
+
+__global__ void k_newell_octant(double* __restrict__ out, int nx, int ny, int nz, double cx, double cy, double cz,
+                                int mgx, int mgy, int mgz, int px, int py, int pz) {
+  const int n = nx * ny * nz;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int x = idx % nx, y = (idx / nx) % ny, z = idx / (nx * ny);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = -px; i <= px; ++i)
+    for (int j = -py; j <= py; ++j)
+      for (int k = -pz; k <= pz; ++k) {
+        double v[6];
+        newell6(x + i * mgx, y + j * mgy, z + k * mgz, cx, cy, cz, v);
+        for (int c = 0; c < 6; ++c) acc[c] += v[c];
+      }
+  for (int c = 0; c < 6; ++c) out[(size_t)c * n + idx] = acc[c];
+}
+
+// out[(o*H + k)*inner + i] = sum_r in[(o*N + r)*inner + i] * w(r) * trig(2 pi k r / K)
+// with cosine and weights 1,2,2,... for components even along the axis and
+// sine with weight 2 for odd ones.  o runs over 6 components x `per` rows.
+__global__ void k_trig_axis(const double* __restrict__ in, double* __restrict__ out, int per, int N, int inner,
+                            int K, int H, int oddmask, const double* __restrict__ ct, const double* __restrict__ st) {
+  const long long total = (long long)6 * per * H * inner;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int i = (int)(t % inner);
+  const long long rest = t / inner;
+  const int k = (int)(rest % H);
+  const long long o = rest / H;
+  const int comp = (int)(o / per);
+  const bool odd = (oddmask >> comp) & 1;
+  const double* src = in + (size_t)o * N * inner + i;
+  double s = 0.0;
+  if (odd) {
+    for (int r = 1; r < N; ++r) s += 2.0 * src[(size_t)r * inner] * st[(int)(((long long)k * r) & (K - 1))];
+  } else {
+    s = src[0];
+    for (int r = 1; r < N; ++r) s += 2.0 * src[(size_t)r * inner] * ct[(int)(((long long)k * r) & (K - 1))];
+  }
+  out[t] = s;
+}
+
+__global__ void k_finalize(const double* __restrict__ in, float* __restrict__ T, int Hx, int Hy, int Hz, double pref) {
+  const long long total = (long long)6 * Hx * Hy * Hz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  // output index ((c*Hx + kx)*Hy + ky)*Hz + kz
+  const int kz = (int)(t % Hz);
+  const int ky = (int)((t / Hz) % Hy);
+  const int kx = (int)((t / ((long long)Hz * Hy)) % Hx);
+  const int c = (int)(t / ((long long)Hz * Hy * Hx));
+  const double sign = (c >= 3) ? -1.0 : 1.0;  // (-i)^2 of the two odd axes
+  const double v = in[(((size_t)c * Hz + kz) * Hy + ky) * Hx + kx];
+  T[t] = (float)(sign * pref * v);
+}
+
+__global__ void k_newell_grid(float* __restrict__ out, int Kx, int Ky, int Kz, int ox, int oy, int oz, double cx,
+                              double cy, double cz, int mgx, int mgy, int mgz, int px, int py, int pz) {
+  const int n = Kx * Ky * Kz;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int x = ox + idx % Kx, y = oy + (idx / Kx) % Ky, z = oz + idx / (Kx * Ky);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = -px; i <= px; ++i)
+    for (int j = -py; j <= py; ++j)
+      for (int k = -pz; k <= pz; ++k) {
+        double v[6];
+        newell6(x + i * mgx, y + j * mgy, z + k * mgz, cx, cy, cz, v);
+        for (int c = 0; c < 6; ++c) acc[c] += v[c];
+      }
+  for (int c = 0; c < 6; ++c) out[(size_t)c * n + idx] = (float)acc[c];
+}
+
+unsigned grid_for(long long n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
+
+void make_twiddles(DeviceArray<float2>& tw, int K) {
+  std::vector<float2> h(K);
+  for (int q = 0; q < K; ++q) {
+    const double a = -2.0 * M_PI * (double)q / (double)K;
+    h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  tw.allocate(K);
+  HIP_CHECK(hipMemcpy(tw.get(), h.data(), K * sizeof(float2), hipMemcpyHostToDevice));
+}
+
+void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
+  std::vector<double> c(K), s(K);
+  for (int q = 0; q < K; ++q) {
+    const double a = 2.0 * M_PI * (double)q / (double)K;
+    c[q] = std::cos(a);
+    s[q] = std::sin(a);
+  }
+  ct.allocate(K);
+  st.allocate(K);
+  HIP_CHECK(hipMemcpy(ct.get(), c.data(), K * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(st.get(), s.data(), K * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// elements one workgroup can transform at once: E complex values per thread
+constexpr int E_SMALL = 16;
+constexpr int E_LARGE = 32;
+constexpr int CAP_SMALL = E_SMALL * BLOCK;
+constexpr int CAP_LARGE = E_LARGE * BLOCK;
+
+template <template <int> class Launcher, typename... Args>
+void launch_e(bool large, Args&&... args) {
+  if (large) Launcher<E_LARGE>::run(args...);
+  else Launcher<E_SMALL>::run(args...);
+}
+
+}  // namespace
+
+int demag_fft_length(int n) {
+  if (n <= 1) return 1;
+  int k = 1;
+  while (k < 2 * n - 1) k <<= 1;
+  return k;
+}
+
+DemagPlan::DemagPlan(const DemagGeometry& g) : g_(g) {
+  for (int a = 0; a < 3; ++a) {
+    if (g.n[a] < 1) throw Error(MPA_ERR_INVALID, "demag: grid size must be positive");
+    K_[a] = demag_fft_length(g.n[a]);
+    log2K_[a] = ilog2(K_[a]);
+    if (K_[a] > CAP_SMALL) throw Error(MPA_ERR_INVALID, "demag: grid axis too large for the FFT passes (max 2048 cells)");
+  }
+  if (K_[2] > 1 && 3 * K_[2] > CAP_LARGE)
+    throw Error(MPA_ERR_INVALID, "demag: z axis too large for the fused z pass");
+  if (K_[2] == 1 && 3 * K_[1] > CAP_LARGE)
+    throw Error(MPA_ERR_INVALID, "demag: y axis too large for the fused 2-D pass");
+  N_ = g.n[0] * g.n[1] * g.n[2];
+  Hx_ = K_[0] / 2 + 1;
+  if (K_[0] == 1) Hx_ = 1;
+  Hy_ = K_[1] / 2 + 1;
+  Hz_ = K_[2] / 2 + 1;
+  A_.allocate((size_t)3 * g.n[2] * Hx_ * g.n[1]);
+  if (!is2d()) B_.allocate((size_t)3 * Hx_ * K_[1] * g.n[2]);
+  for (int a = 0; a < 3; ++a) make_twiddles(tw_[a], K_[a]);
+  build_kernel(nullptr);
+}
+
+void DemagPlan::build_kernel(hipStream_t s) {
+  const int nx = g_.n[0], ny = g_.n[1], nz = g_.n[2];
+  DeviceArray<double> oct((size_t)6 * N_);
+  k_newell_octant<<<grid_for(N_), BLOCK, 0, s>>>(oct.get(), nx, ny, nz, g_.cs[0], g_.cs[1], g_.cs[2],
+                                                 g_.mastergrid[0], g_.mastergrid[1], g_.mastergrid[2], g_.pbc[0],
+                                                 g_.pbc[1], g_.pbc[2]);
+  checkLaunch("k_newell_octant");
+  DeviceArray<double> ct[3], st[3];
+  for (int a = 0; a < 3; ++a) make_trig(ct[a], st[a], K_[a]);
+  // odd components per axis: x: xy(3), xz(4); y: xy(3), yz(5); z: xz(4), yz(5)
+  const int oddx = (1 << 3) | (1 << 4), oddy = (1 << 3) | (1 << 5), oddz = (1 << 4) | (1 << 5);
+  DeviceArray<double> t1((size_t)6 * nz * ny * Hx_);
+  k_trig_axis<<<grid_for((long long)6 * nz * ny * Hx_), BLOCK, 0, s>>>(oct.get(), t1.get(), nz * ny, nx, 1, K_[0],
+                                                                       Hx_, oddx, ct[0].get(), st[0].get());
+  checkLaunch("k_trig_axis x");
+  oct.release();
+  DeviceArray<double> t2((size_t)6 * nz * Hy_ * Hx_);
+  k_trig_axis<<<grid_for((long long)6 * nz * Hy_ * Hx_), BLOCK, 0, s>>>(t1.get(), t2.get(), nz, ny, Hx_, K_[1], Hy_,
+                                                                        oddy, ct[1].get(), st[1].get());
+  checkLaunch("k_trig_axis y");
+  t1.release();
+  DeviceArray<double> t3((size_t)6 * Hz_ * Hy_ * Hx_);
+  k_trig_axis<<<grid_for((long long)6 * Hz_ * Hy_ * Hx_), BLOCK, 0, s>>>(t2.get(), t3.get(), 1, nz, Hy_ * Hx_, K_[2],
+                                                                        Hz_, oddz, ct[2].get(), st[2].get());
+  checkLaunch("k_trig_axis z");
+  t2.release();
+  T_.allocate((size_t)6 * Hx_ * Hy_ * Hz_);
+  const double pref = -MU0 / ((double)K_[0] * K_[1] * K_[2]);
+  k_finalize<<<grid_for((long long)6 * Hx_ * Hy_ * Hz_), BLOCK, 0, s>>>(t3.get(), T_.get(), Hx_, Hy_, Hz_, pref);
+  checkLaunch("k_finalize");
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup used by every pass
+
+void check_lds(size_t bytes, const char* pass) {
+  if (bytes > LDS_LIMIT)
+    throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
+}
+
+size_t xinv_lds(int PP, int Hx, int Kx) {
+  return ((size_t)2 * PP * (Hx + 1) + (size_t)PP * (Kx + 1)) * sizeof(float2);
+}
+
+template <int E>
+struct LXfwd {
+  static void run(DD d, const float* m, PS ms, const uint8_t* geom, float2* A, const float2* tw, int PP, hipStream_t s) {
+    const int npairs = (d.nlines + 1) / 2;
+    const size_t shm = (size_t)PP * (d.Kx + 1) * sizeof(float2);
+    check_lds(shm, "k_xfwd");
+    k_xfwd<E><<<grid_for(npairs, PP), BLOCK, shm, s>>>(d, m, ms, geom, A, tw, PP);
+    checkLaunch("k_xfwd");
+  }
+};
+template <int E>
+struct LYfwd {
+  static void run(DD d, const float2* A, float2* B, const float2* tw, int PY, hipStream_t s) {
+    const size_t shm = (size_t)PY * (d.Ky + 1) * sizeof(float2);
+    check_lds(shm, "k_yfwd");
+    k_yfwd<E><<<grid_for(3LL * d.Hx * d.nz, PY), BLOCK, shm, s>>>(d, A, B, tw, PY);
+    checkLaunch("k_yfwd");
+  }
+};
+template <int E>
+struct LZpass {
+  static void run(DD d, float2* B, const float* T, const float2* tw, int PZ, hipStream_t s) {
+    const size_t shm = (size_t)3 * PZ * (d.Kz + 1) * sizeof(float2);
+    check_lds(shm, "k_zpass");
+    k_zpass<E><<<grid_for((long long)d.Hx * d.Ky, PZ), BLOCK, shm, s>>>(d, B, T, tw, PZ);
+    checkLaunch("k_zpass");
+  }
+};
+template <int E>
+struct LYinv {
+  static void run(DD d, const float2* B, float2* A, const float2* tw, int PY, hipStream_t s) {
+    const size_t shm = (size_t)PY * (d.Ky + 1) * sizeof(float2);
+    check_lds(shm, "k_yinv");
+    k_yinv<E><<<grid_for(3LL * d.Hx * d.nz, PY), BLOCK, shm, s>>>(d, B, A, tw, PY);
+    checkLaunch("k_yinv");
+  }
+};
+template <int E>
+struct LY2d {
+  static void run(DD d, float2* A, const float* T, const float2* tw, int PQ, hipStream_t s) {
+    const size_t shm = (size_t)3 * PQ * (d.Ky + 1) * sizeof(float2);
+    check_lds(shm, "k_ypass2d");
+    k_ypass2d<E><<<grid_for(d.Hx, PQ), BLOCK, shm, s>>>(d, A, T, tw, PQ);
+    checkLaunch("k_ypass2d");
+  }
+};
+template <int E>
+struct LXinv {
+  static void run(DD d, const float2* A, const uint8_t* geom, float* h, const float2* tw, int PP, hipStream_t s) {
+    const int npairs = (d.nlines + 1) / 2;
+    const size_t shm = ((size_t)2 * PP * (d.Hx + 1) + (size_t)PP * (d.Kx + 1)) * sizeof(float2);
+    check_lds(shm, "k_xinv");
+    k_xinv<E><<<grid_for(npairs, PP), BLOCK, shm, s>>>(d, A, geom, h, tw, PP);
+    checkLaunch("k_xinv");
+  }
+};
+
+// lines per workgroup so that lines*L <= cap, at most `maxl`
+int lines_for(int L, int cap, int maxl) {
+  int p = cap / L;
+  if (p > maxl) p = maxl;
+  return p < 1 ? 1 : p;
+}
+}  // namespace
+
+void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s) {
+  exec_timed(m, msat, geom, h, s, nullptr);
+}
+
+void DemagPlan::exec_timed(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, hipEvent_t* ev) {
+  int evi = 0;
+  auto mark = [&]() {
+    if (ev) HIP_CHECK(hipEventRecord(ev[evi++], s));
+  };
+  mark();
+  DD d;
+  d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
+  d.Kx = K_[0]; d.Ky = K_[1]; d.Kz = K_[2];
+  d.Hx = Hx_; d.Hy = Hy_; d.Hz = Hz_;
+  d.l2x = log2K_[0]; d.l2y = log2K_[1]; d.l2z = log2K_[2];
+  d.nlines = 3 * d.ny * d.nz;
+
+  // x passes: PP line pairs per workgroup
+  const bool xl = d.Kx > CAP_SMALL;
+  const int PPf = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 16);
+  launch_e<LXfwd>(xl, d, m, msat, geom, A_.get(), tw_[0].get(), PPf, s);
+  mark();
+  if (is2d()) {
+    const bool yl = 3 * d.Ky > CAP_SMALL;
+    const int PQ = lines_for(3 * d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
+    launch_e<LY2d>(yl, d, A_.get(), (const float*)T_.get(), tw_[1].get(), PQ, s);
+    mark();
+  } else {
+    const bool yl = d.Ky > CAP_SMALL;
+    const int PY = lines_for(d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
+    launch_e<LYfwd>(yl, d, (const float2*)A_.get(), B_.get(), tw_[1].get(), PY, s);
+    mark();
+    const bool zl = 3 * d.Kz > CAP_SMALL;
+    const int PZ = lines_for(3 * d.Kz, zl ? CAP_LARGE : CAP_SMALL, 16);
+    launch_e<LZpass>(zl, d, B_.get(), (const float*)T_.get(), tw_[2].get(), PZ, s);
+    mark();
+    launch_e<LYinv>(yl, d, (const float2*)B_.get(), A_.get(), tw_[1].get(), PY, s);
+    mark();
+  }
+  // the inverse x pass also stages the half spectrum: fit both in the LDS budget
+  int PPi = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 8);
+  while (PPi > 1 && xinv_lds(PPi, d.Hx, d.Kx) > LDS_LIMIT) --PPi;
+  check_lds(xinv_lds(PPi, d.Hx, d.Kx), "k_xinv");
+  launch_e<LXinv>(xl, d, (const float2*)A_.get(), geom, h, tw_[0].get(), PPi, s);
+  mark();
+}
+
+const char* DemagPlan::pass_name(int i) const {
+  static const char* n3[] = {"k_xfwd", "k_yfwd", "k_zpass", "k_yinv", "k_xinv"};
+  static const char* n2[] = {"k_xfwd", "k_ypass2d", "k_xinv"};
+  return is2d() ? n2[i] : n3[i];
+}
+
+double DemagPlan::pass_bytes(int i) const {
+  const double N = (double)g_.n[0] * g_.n[1] * g_.n[2];
+  const double ny = g_.n[1], nz = g_.n[2];
+  const double xa = 24.0 * Hx_ * ny * nz;   // one pass over A (complex 3-vectors)
+  const double yb = 24.0 * Hx_ * K_[1] * nz; // one pass over B
+  if (is2d()) {
+    const double t = 16.0 * Hx_ * Hy_;  // xx, yy, zz, xy (xz = yz = 0 in a single layer)
+    const double v[3] = {12 * N + xa, 2 * xa + t, xa + 12 * N};
+    return v[i];
+  }
+  const double t = 24.0 * Hx_ * Hy_ * Hz_;
+  const double v[5] = {12 * N + xa, xa + yb, 2 * yb + t, yb + xa, xa + 12 * N};
+  return v[i];
+}
+
+double DemagPlan::algorithmic_bytes() const {
+  const double nx = g_.n[0], ny = g_.n[1], nz = g_.n[2];
+  const double N = nx * ny * nz;
+  if (is2d()) return 12 * N + 96.0 * Hx_ * ny + 16.0 * Hx_ * Hy_ + 12 * N;
+  return 12 * N + 48.0 * Hx_ * ny * nz + 48.0 * Hx_ * K_[1] * nz + 24.0 * Hx_ * Hy_ * Hz_ + 48.0 * Hx_ * K_[1] * nz +
+         48.0 * Hx_ * ny * nz + 12 * N;
+}
+
+size_t DemagPlan::device_bytes() const { return A_.bytes() + B_.bytes() + T_.bytes(); }
+
+void DemagPlan::copy_spectrum(float* host) const {
+  HIP_CHECK(hipMemcpy(host, T_.get(), T_.bytes(), hipMemcpyDeviceToHost));
+}
+
+void reference_kernel_size(const DemagGeometry& g, int ksize[3]) {
+  for (int a = 0; a < 3; ++a) {
+    int s = 2 * g.n[a] - 1;
+    if (s > 5 && (s % 2) == 1) s += 1;
+    ksize[a] = s;
+  }
+}
+
+void reference_kernel(const DemagGeometry& g, int ksize[3], float* host_out) {
+  reference_kernel_size(g, ksize);
+  int origin[3];
+  for (int a = 0; a < 3; ++a) origin[a] = -(g.n[a] - 1) - (ksize[a] - (2 * g.n[a] - 1));
+  const long long n = (long long)ksize[0] * ksize[1] * ksize[2];
+  DeviceArray<float> out((size_t)6 * n);
+  k_newell_grid<<<grid_for(n), BLOCK>>>(out.get(), ksize[0], ksize[1], ksize[2], origin[0], origin[1], origin[2],
+                                        g.cs[0], g.cs[1], g.cs[2], g.mastergrid[0], g.mastergrid[1], g.mastergrid[2],
+                                        g.pbc[0], g.pbc[1], g.pbc[2]);
+  checkLaunch("k_newell_grid");
+  HIP_CHECK(hipMemcpy(host_out, out.get(), out.bytes(), hipMemcpyDeviceToHost));
+}
+
+}  // namespace mpa

@@ -1,0 +1,65 @@
+// Demag (self stray field) of one magnet by zero-padded FFT convolution.
+//
+// Replaces StrayFieldFFTExecutor (src/physics/strayfieldfft.cu:149-246) and
+// StrayFieldKernel (src/physics/strayfieldkernel.cu:11-99).
+#pragma once
+
+#include "common.hpp"
+
+namespace mpa {
+
+struct DemagGeometry {
+  int n[3];          // magnet grid size (cells)
+  double cs[3];      // cell size (m)
+  int mastergrid[3]; // periodic box (0 = open axis)
+  int pbc[3];        // image repetitions per axis
+};
+
+// FFT length used for an axis with n cells: 1 for n == 1, otherwise the
+// smallest power of two >= 2n-1 (any length >= 2n-1 gives the same linear
+// convolution; the reference uses 2n, strayfieldkernel.cu:84-96).
+int demag_fft_length(int n);
+
+class DemagPlan {
+ public:
+  explicit DemagPlan(const DemagGeometry& g);
+  // h = -mu0 N * (Ms m) on the magnet grid; zero outside the geometry.
+  // m, h: 3*N floats (SoA, x fastest).  geom: N bytes or nullptr.
+  void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s);
+  // Same, recording ev[0] before the first pass and ev[i+1] after pass i
+  // (npasses() events + 1).  Pass names and algorithmic bytes per pass:
+  void exec_timed(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, hipEvent_t* ev);
+  int npasses() const { return is2d() ? 3 : 5; }
+  const char* pass_name(int i) const;
+  double pass_bytes(int i) const;
+
+  const DemagGeometry& geometry() const { return g_; }
+  int K(int a) const { return K_[a]; }
+  bool is2d() const { return K_[2] == 1; }
+  // Algorithmic HBM bytes of one exec (see the byte model in the design notes).
+  double algorithmic_bytes() const;
+  size_t device_bytes() const;
+  // Real kernel spectrum table (6 x Hx x Hy x Hz floats, pre-scaled), for tests.
+  void copy_spectrum(float* host) const;
+  int spectrum_dims(int a) const { return a == 0 ? Hx_ : (a == 1 ? Hy_ : Hz_); }
+
+ private:
+  void build_kernel(hipStream_t s);
+  DemagGeometry g_;
+  int N_ = 0;
+  int K_[3];
+  int log2K_[3];
+  int Hx_, Hy_, Hz_;
+  DeviceArray<float2> A_, B_;
+  DeviceArray<float> T_;
+  DeviceArray<float2> tw_[3];
+};
+
+// The real-space kernel exactly as the reference lays it out for a magnet
+// (wrap_magnet.cpp:56-59 `_demag_kernel`): shape (6, Kz, Ky, Kx) on the
+// reference kernel grid (strayfieldkernel.cu:80-99), computed in FP64 on the
+// device and returned as float.
+void reference_kernel(const DemagGeometry& g, int ksize[3], float* host_out);
+void reference_kernel_size(const DemagGeometry& g, int ksize[3]);
+
+}  // namespace mpa

@@ -1,0 +1,224 @@
+// Fused local effective field + torque stencil.
+//
+// One thread per cell computes, in this order (the summation order of
+// evalEffectiveField, src/physics/effectivefield.cpp:15-28):
+//   exchange  (k_exchangeField FM variant, src/physics/exchange.cu:20-104)
+//   anisotropy (uniaxial, else cubic: src/physics/anisotropy.cu:25-140)
+//   external  (world bias + magnet bias, src/physics/zeeman.cu:53-91)
+//   DMI       (k_dmiFieldFM, src/physics/dmi.cu:87-178 with getGamma, dmi.hpp:53-69)
+//   demag     (added from the FFT passes)
+// and then the LLG torque (k_llgtorque, src/physics/torque.cu:21-45), the
+// damping-only torque (:58-77) or an energy density (src/physics/energy.cu:18-34,
+// anisotropy.cu:142-230).  Exchange and DMI share one pass over the six
+// neighbours; boundary handling (Neumann with the DMI boundary term, open
+// boundaries, msat = 0 neighbours, mastergrid wrap) follows the reference
+// branch by branch.
+#include "fields.hpp"
+
+namespace mpa {
+
+namespace {
+
+constexpr float GAMMALL = 1.7595e11f;  // src/physics/constants.hpp:7
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ float harmonic_mean(float a, float b) {
+  if (a + b == 0.0f) return 0.0f;
+  if (a == b) return a;
+  return 2 * a * b / (a + b);
+}
+
+__device__ __forceinline__ float3 load3(const float* __restrict__ v, int i, int n) {
+  return f3(v[i], v[n + i], v[2 * n + i]);
+}
+
+__device__ __forceinline__ float3 normalized3(float3 v) {
+  const float n = sqrtf(dot3(v, v));
+  return v / n;
+}
+
+// getGamma (dmi.hpp:53-69) with the absolute normal n along `axis`.
+__device__ __forceinline__ float3 dmi_gamma(const float D[D_COUNT], int axis, float3 m) {
+  const float nx = axis == 0, ny = axis == 1, nz = axis == 2;
+  return f3(-D[D_XXY] * nx * m.y - D[D_XXZ] * nx * m.z - D[D_YXZ] * ny * m.z - D[D_ZXY] * nz * m.y -
+                D[D_YXY] * ny * m.y - D[D_ZXZ] * nz * m.z,
+            D[D_XXY] * nx * m.x - D[D_ZYZ] * nz * m.z + D[D_YXY] * ny * m.x - D[D_XYZ] * nx * m.z +
+                D[D_ZXY] * nz * m.x - D[D_YYZ] * ny * m.z,
+            D[D_XXZ] * nx * m.x + D[D_YYZ] * ny * m.y + D[D_XYZ] * nx * m.y + D[D_YXZ] * ny * m.x +
+                D[D_ZXZ] * nz * m.x + D[D_ZYZ] * nz * m.y);
+}
+
+// Local index of the neighbour in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
+// after the mastergrid wrap, or -1 when it falls outside the magnet grid.
+__device__ __forceinline__ int neighbour(const FMDesc& d, int x, int y, int z, int dir) {
+  const int axis = dir >> 1;
+  const int step = (dir & 1) ? 1 : -1;
+  int c[3] = {x, y, z};
+  const int o[3] = {d.ox, d.oy, d.oz};
+  const int mg[3] = {d.mgx, d.mgy, d.mgz};
+  const int mo[3] = {d.mox, d.moy, d.moz};
+  const int n[3] = {d.nx, d.ny, d.nz};
+  int g = o[axis] + c[axis] + step;
+  if (mg[axis] > 0) g = mo[axis] + imod(g - mo[axis], mg[axis]);
+  const int l = g - o[axis];
+  if (l < 0 || l >= n[axis]) return -1;
+  c[axis] = l;
+  return (c[2] * d.ny + c[1]) * d.nx + c[0];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restrict__ m,
+                                                 const float* __restrict__ hdemag, float* __restrict__ out,
+                                                 int mode) {
+  const int nblk = gridDim.x;
+  const int bid = xcd_remap(blockIdx.x, nblk);
+  const int idx = bid * BLOCK + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  const bool ingeom = !d.geom || d.geom[idx];
+  if (!ingeom) {
+    if (mode == MODE_EDENS) out[idx] = 0.f;
+    else { out[idx] = 0.f; out[N + idx] = 0.f; out[2 * N + idx] = 0.f; }
+    return;
+  }
+  const int x = idx % d.nx;
+  const int y = (idx / d.nx) % d.ny;
+  const int z = idx / (d.nx * d.ny);
+  const float3 mc = load3(m, idx, N);
+  const float ms = d.msat.at(idx);
+  float edens = 0.f;
+
+  float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
+  const bool do_ex = (d.terms & TERM_EXCH) && ms != 0.f;
+  const bool do_dmi = (d.terms & TERM_DMI) && ms != 0.f;
+  if (do_ex || do_dmi) {
+    const float a = d.aex.at(idx);
+    float D[D_COUNT];
+#pragma unroll
+    for (int k = 0; k < D_COUNT; ++k) D[k] = d.dmi[k].at(idx);
+    const float w[3] = {1.f / (d.cx * d.cx), 1.f / (d.cy * d.cy), 1.f / (d.cz * d.cz)};
+    const float csz[3] = {d.cx, d.cy, d.cz};
+    const bool openbc = d.openbc != 0;
+    const bool open_dmi = openbc || a == 0.f;
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      const int axis = dir >> 1;
+      const float delta = (dir & 1) ? csz[axis] : -csz[axis];
+      const int nb = neighbour(d, x, y, z, dir);
+      const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+      float3 gamma = f3(0.f, 0.f, 0.f);
+      if (!nb_in) gamma = dmi_gamma(D, axis, mc);
+      if (do_ex) {
+        // exchange.cu:61-101
+        bool use = true;
+        float3 mn;
+        float an;
+        if (!nb_in) {
+          if (openbc || a == 0.f) use = false;
+          mn = mc + (delta / (2 * a)) * gamma;
+          an = a;
+        } else {
+          if (openbc && d.msat.at(nb) == 0.f) use = false;
+          mn = load3(m, nb, N);
+          an = d.aex.at(nb);
+        }
+        if (use) {
+          const float A = harmonic_mean(a, an);
+          hex = hex + (2 * A * w[axis]) * (mn - mc);
+        }
+      }
+      if (do_dmi) {
+        // dmi.cu:114-173
+        const int nbi = nb_in ? nb : idx;
+        bool skip = open_dmi && (!nb_in || d.msat.at(nbi) == 0.f);
+        if (!skip) {
+          const int kxz = axis == 0 ? D_XXZ : (axis == 1 ? D_YXZ : D_ZXZ);
+          const int kxy = axis == 0 ? D_XXY : (axis == 1 ? D_YXY : D_ZXY);
+          const int kyz = axis == 0 ? D_XYZ : (axis == 1 ? D_YYZ : D_ZYZ);
+          float Dxz = D[kxz], Dxy = D[kxy], Dyz = D[kyz];
+          if (nbi != idx) {
+            Dxz = harmonic_mean(Dxz, d.dmi[kxz].at(nbi));
+            Dxy = harmonic_mean(Dxy, d.dmi[kxy].at(nbi));
+            Dyz = harmonic_mean(Dyz, d.dmi[kyz].at(nbi));
+          }
+          float3 mn;
+          if (!nb_in && !open_dmi) mn = mc + (delta / (2 * a)) * gamma;
+          else mn = load3(m, nbi, N);
+          hdm.x += (Dxy * mn.y + Dxz * mn.z) / delta;
+          hdm.y += (-Dxy * mn.x + Dyz * mn.z) / delta;
+          hdm.z += (-Dxz * mn.x - Dyz * mn.y) / delta;
+        }
+      }
+    }
+    if (do_ex) hex = hex / ms;
+    if (do_dmi) hdm = hdm / ms;
+  }
+
+  float3 han = f3(0.f, 0.f, 0.f);
+  if (ms != 0.f) {
+    if (d.terms & TERM_UNI) {
+      const float3 u = normalized3(d.anisU.at(idx, N));
+      const float k1 = d.ku1.at(idx), k2 = d.ku2.at(idx);
+      const float mu = dot3(mc, u);
+      han = ((2 * k1 * mu + 4 * k2 * mu * mu * mu) / ms) * u;
+      if (mode == MODE_EDENS) edens += -k1 * mu * mu - k2 * mu * mu * mu * mu;
+    } else if (d.terms & TERM_CUBIC) {
+      const float3 c1 = normalized3(d.anisC1.at(idx, N));
+      const float3 c2 = normalized3(d.anisC2.at(idx, N));
+      const float3 c3 = cross3(c1, c2);
+      const float kc1 = d.kc1.at(idx), kc2 = d.kc2.at(idx), kc3 = d.kc3.at(idx);
+      const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
+      const float s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
+      han = (-2 * kc1 / ms) * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
+            (-2 * kc2 / ms) * ((s2 * s3) * a1 * c1 + (s1 * s3) * a2 * c2 + (s1 * s2) * a3 * c3) +
+            (-4 * kc3 / ms) * ((s2 * s2 + s3 * s3) * (s1 * a1) * c1 + (s1 * s1 + s3 * s3) * (s2 * a2) * c2 +
+                               (s1 * s1 + s2 * s2) * (s3 * a3) * c3);
+      if (mode == MODE_EDENS)
+        edens += kc1 * (s1 * s2 + s1 * s3 + s2 * s3) + kc2 * s1 * s2 * s3 +
+                 kc3 * (s1 * s1 * s2 * s2 + s1 * s1 * s3 * s3 + s2 * s2 * s3 * s3);
+    }
+  }
+  float3 hext = f3(0.f, 0.f, 0.f);
+  if (d.terms & TERM_EXT) hext = f3(d.wb[0], d.wb[1], d.wb[2]) + d.bias.at(idx, N);
+  float3 hdem = f3(0.f, 0.f, 0.f);
+  if (d.terms & TERM_DEMAG) hdem = load3(hdemag, idx, N);
+
+  if (mode == MODE_EDENS) {
+    edens += -0.5f * ms * dot3(mc, hex);
+    edens += -1.0f * ms * dot3(mc, hext);
+    edens += -0.5f * ms * dot3(mc, hdm);
+    edens += -0.5f * ms * dot3(mc, hdem);
+    out[idx] = edens;
+    return;
+  }
+  float3 h = hex;
+  h = h + han;
+  h = h + hext;
+  h = h + hdm;
+  h = h + hdem;
+  float3 r;
+  if (mode == MODE_FIELD) {
+    r = h;
+  } else if (d.frozen.at(idx) != 0.f) {
+    r = f3(0.f, 0.f, 0.f);
+  } else if (mode == MODE_LLG) {
+    const float al = d.alpha.at(idx);
+    const float3 mxh = cross3(mc, h);
+    const float3 mxmxh = cross3(mc, mxh);
+    r = (-GAMMALL / (1 + al * al)) * (mxh + al * mxmxh);
+  } else {
+    r = -GAMMALL * cross3(mc, cross3(mc, h));
+  }
+  out[idx] = r.x;
+  out[N + idx] = r.y;
+  out[2 * N + idx] = r.z;
+}
+
+}  // namespace
+
+void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {
+  const int nblk = (d.ncell + BLOCK - 1) / BLOCK;
+  k_field<<<nblk, BLOCK, 0, s>>>(d, m, hdemag, out, (int)mode);
+  checkLaunch("k_field");
+}
+
+}  // namespace mpa

@@ -1,0 +1,47 @@
+// Local effective-field terms, LLG torque and energy densities of a
+// ferromagnet, fused into one stencil kernel.
+#pragma once
+
+#include "common.hpp"
+
+namespace mpa {
+
+enum TermBits : unsigned {
+  TERM_EXCH = 1u << 0,
+  TERM_UNI = 1u << 1,
+  TERM_CUBIC = 1u << 2,
+  TERM_EXT = 1u << 3,
+  TERM_DMI = 1u << 4,
+  TERM_DEMAG = 1u << 5,
+};
+
+// DMI tensor entries in the member order of DmiTensor (src/physics/dmitensor.hpp:12-22).
+enum DmiIdx { D_XXY = 0, D_XYZ, D_XXZ, D_YXY, D_YYZ, D_YXZ, D_ZXY, D_ZYZ, D_ZXZ, D_COUNT };
+
+// Everything the stencil needs, passed to kernels by value.
+struct FMDesc {
+  int nx, ny, nz, ncell;
+  int ox, oy, oz;     // magnet grid origin (cells)
+  int mgx, mgy, mgz;  // mastergrid size (0 = not periodic)
+  int mox, moy, moz;  // mastergrid origin
+  float cx, cy, cz;
+  const uint8_t* geom;
+  PS msat, aex, alpha, ku1, ku2, kc1, kc2, kc3, frozen;
+  PV anisU, anisC1, anisC2, bias;
+  PS dmi[D_COUNT];
+  float wb[3];  // world bias field (T)
+  int openbc;
+  unsigned terms;
+};
+
+enum FieldMode {
+  MODE_FIELD = 0,    // out = sum of the selected terms (T)
+  MODE_LLG = 1,      // out = LLG torque of the selected terms (1/s)
+  MODE_DAMPING = 2,  // out = damping-only torque (relax)
+  MODE_EDENS = 3,    // out (1 comp) = energy density of the selected terms (J/m3)
+};
+
+// out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
+void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
+
+}  // namespace mpa

@@ -1,0 +1,161 @@
+// Runge-Kutta stage combination and reductions.
+//
+// k_combine replaces the per-stage chain of addTo + normalise launches of the
+// reference (RungeKuttaStageExecutor::setStageX / setFinalX,
+// src/core/rungekutta.cpp:108-134; k_addFields src/core/fieldops.cu:9-22;
+// k_normalize :232-247): one read of x0 and of each k_j, one write.
+// k_combo_maxnorm replaces getError (rungekutta.cpp:140-151) + k_maxVecNorm
+// (src/core/reduce.cu:53-99) without materialising the error field.
+// k_sums replaces k_average (reduce.cu:101-127) with a deterministic
+// two-level FP64 sum.
+#include "rk.hpp"
+
+namespace mpa {
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int SUM_BLOCKS = 512;
+
+__global__ void __launch_bounds__(BLOCK) k_combine(float* __restrict__ out, const float* __restrict__ x0, Combo co,
+                                                   int ncomp, int N, const uint8_t* __restrict__ geom, int normalize) {
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (idx >= N) return;
+  if (geom && !geom[idx]) {
+    for (int c = 0; c < ncomp; ++c) out[(size_t)c * N + idx] = 0.f;
+    return;
+  }
+  float v[3];
+  for (int c = 0; c < ncomp; ++c) {
+    float a = x0[(size_t)c * N + idx];
+    for (int j = 0; j < co.nk; ++j) a += co.c[j] * co.k[j][(size_t)c * N + idx];
+    v[c] = a;
+  }
+  if (normalize) {
+    float n2 = 0.f;
+    for (int c = 0; c < ncomp; ++c) n2 += v[c] * v[c];
+    const float inv = rsqrtf(n2);
+    for (int c = 0; c < ncomp; ++c) v[c] *= inv;
+  }
+  for (int c = 0; c < ncomp; ++c) out[(size_t)c * N + idx] = v[c];
+}
+
+__device__ __forceinline__ float block_max(float v) {
+  __shared__ float sh[BLOCK / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < BLOCK / 64; ++i) v = fmaxf(v, sh[i]);
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_combo_maxnorm(Combo co, int N, const uint8_t* __restrict__ geom,
+                                                         unsigned* __restrict__ result) {
+  float best = 0.f;
+  for (int idx = blockIdx.x * BLOCK + threadIdx.x; idx < N; idx += gridDim.x * BLOCK) {
+    if (geom && !geom[idx]) continue;
+    float e[3];
+    for (int c = 0; c < 3; ++c) {
+      float a = 0.f;
+      for (int j = 0; j < co.nk; ++j) a += co.c[j] * co.k[j][(size_t)c * N + idx];
+      e[c] = a;
+    }
+    best = fmaxf(best, sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]));
+  }
+  best = block_max(best);
+  if (threadIdx.x == 0) atomicMax(result, __float_as_uint(best));
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+__global__ void __launch_bounds__(BLOCK) k_sums(const float* __restrict__ v, int ncomp, int N,
+                                                const uint8_t* __restrict__ geom, double* __restrict__ partials) {
+  __shared__ double sh[BLOCK];
+  for (int c = 0; c < ncomp; ++c) {
+    double acc = 0.0;
+    for (int idx = blockIdx.x * BLOCK + threadIdx.x; idx < N; idx += gridDim.x * BLOCK) {
+      if (geom && !geom[idx]) continue;
+      acc += (double)v[(size_t)c * N + idx];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(size_t)c * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_sums_final(double* partials, int nparts, int ncomp) {
+  __shared__ double sh[BLOCK];
+  for (int c = 0; c < ncomp; ++c) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += BLOCK) acc += partials[(size_t)c * nparts + i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) partials[(size_t)ncomp * nparts + c] = sh[0];
+    __syncthreads();
+  }
+}
+
+int reduce_blocks(int N) {
+  int b = (N + BLOCK - 1) / BLOCK;
+  return b > 1024 ? 1024 : (b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int N, const uint8_t* geom,
+                    bool normalize, hipStream_t s) {
+  const int nblk = (N + BLOCK - 1) / BLOCK;
+  k_combine<<<nblk, BLOCK, 0, s>>>(out, x0, co, ncomp, N, geom, normalize ? 1 : 0);
+  checkLaunch("k_combine");
+}
+
+Reducer::Reducer() : maxbits_(1), partials_((size_t)3 * SUM_BLOCKS + 3) {}
+
+float Reducer::max_combo_norm(const Combo& co, int N, const uint8_t* geom, hipStream_t s) {
+  k_zero_u32<<<1, 1, 0, s>>>(maxbits_.get());
+  k_combo_maxnorm<<<reduce_blocks(N), BLOCK, 0, s>>>(co, N, geom, maxbits_.get());
+  checkLaunch("k_combo_maxnorm");
+  unsigned bits = 0;
+  HIP_CHECK(hipMemcpyAsync(&bits, maxbits_.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float r;
+  static_assert(sizeof(r) == sizeof(bits), "float size");
+  __builtin_memcpy(&r, &bits, sizeof(r));
+  return r;
+}
+
+float Reducer::max_vec_norm(const float* v, int N, const uint8_t* geom, hipStream_t s) {
+  Combo co{};
+  co.nk = 1;
+  co.c[0] = 1.f;
+  co.k[0] = v;
+  return max_combo_norm(co, N, geom, s);
+}
+
+void Reducer::component_sums(const float* v, int ncomp, int N, const uint8_t* geom, double* out, hipStream_t s) {
+  if (ncomp > 3) throw Error(MPA_ERR_INVALID, "component_sums: at most 3 components");
+  int nb = (N + BLOCK - 1) / BLOCK;
+  if (nb > SUM_BLOCKS) nb = SUM_BLOCKS;
+  if (nb < 1) nb = 1;
+  k_sums<<<nb, BLOCK, 0, s>>>(v, ncomp, N, geom, partials_.get());
+  checkLaunch("k_sums");
+  k_sums_final<<<1, BLOCK, 0, s>>>(partials_.get(), nb, ncomp);
+  checkLaunch("k_sums_final");
+  HIP_CHECK(hipMemcpyAsync(out, partials_.get() + (size_t)ncomp * nb, ncomp * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace mpa

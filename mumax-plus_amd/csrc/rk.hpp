@@ -1,0 +1,37 @@
+// Runge-Kutta stage combination, error norm and reductions.
+#pragma once
+
+#include "common.hpp"
+
+namespace mpa {
+
+constexpr int MAX_STAGES = 7;
+
+// x = x0 + sum_j c[j] * k[j] (sequential FP32 adds in j order), normalised when
+// `normalize` (rsqrt of |x|^2, as k_normalize in src/core/fieldops.cu:232-247);
+// zero outside the geometry.
+struct Combo {
+  const float* k[MAX_STAGES];
+  float c[MAX_STAGES];
+  int nk;
+};
+void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int N, const uint8_t* geom,
+                    bool normalize, hipStream_t s);
+
+// Reductions write into a small device workspace; results are read back by
+// the caller after a stream synchronisation.
+class Reducer {
+ public:
+  Reducer();
+  // max over geometry cells of |sum_j c[j] k[j]| (3 components), FP32 norms.
+  float max_combo_norm(const Combo& co, int N, const uint8_t* geom, hipStream_t s);
+  float max_vec_norm(const float* v, int N, const uint8_t* geom, hipStream_t s);
+  // sum over geometry cells of each component, accumulated in FP64.
+  void component_sums(const float* v, int ncomp, int N, const uint8_t* geom, double* out, hipStream_t s);
+
+ private:
+  DeviceArray<unsigned> maxbits_;
+  DeviceArray<double> partials_;
+};
+
+}  // namespace mpa

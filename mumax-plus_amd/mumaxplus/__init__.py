@@ -1,0 +1,22 @@
+"""mumaxplus on MI355X: micromagnetic time stepping with hand-written HIP kernels.
+
+The public classes mirror the reference package (mumaxplus/__init__.py of
+dirkpetersen/mumax-plus): World, Grid, Ferromagnet, TimeSolver and the
+quantity / parameter / variable wrappers.  The compute path is the HIP
+library libmumaxplus_amd.so, reached through ``_mumaxpluscpp``.
+"""
+
+from . import _mumaxpluscpp as _cpp
+from .grid import Grid
+from .world import World
+from .ferromagnet import Ferromagnet
+from .timesolver import TimeSolver, TimeSolverOutput
+from .fieldquantity import FieldQuantity
+from .scalarquantity import ScalarQuantity
+from .parameter import Parameter
+from .variable import Variable
+from .dmitensor import DmiTensor
+from . import util
+
+__all__ = ["_cpp", "Grid", "World", "Ferromagnet", "TimeSolver", "TimeSolverOutput", "FieldQuantity",
+           "ScalarQuantity", "Parameter", "Variable", "DmiTensor", "util"]

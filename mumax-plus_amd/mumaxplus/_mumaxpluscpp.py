@@ -1,0 +1,702 @@
+"""Binding layer over the C ABI of libmumaxplus_amd.so (include/mumaxplus_amd.h).
+
+This module plays the role of the reference's pybind11 extension
+``_mumaxpluscpp`` (src/bindings/main.cpp:3-27): it exposes the classes
+``Grid``, ``World``, ``Ferromagnet``, ``TimeSolver``, ``Parameter``,
+``VectorParameter``, ``Variable``, ``FieldQuantity``, ``ScalarQuantity``,
+``DmiTensor`` and ``System`` with the member names of src/bindings/wrap_*.cpp,
+and the module function ``_demag_kernel`` (wrap_magnet.cpp:56-59).
+
+All numerical work happens in the HIP library; this layer only converts
+arguments (copies, like the reference's numpy conversion) and turns error
+codes into Python exceptions (ValueError / RuntimeError / IndexError, the
+translations pybind11 applies to the reference's C++ exceptions).  Importing
+it fails loudly when the shared library is missing.
+"""
+
+import ctypes as _ct
+import os as _os
+
+import numpy as _np
+
+_LIBNAME = "libmumaxplus_amd.so"
+_LIBPATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), _LIBNAME)
+if not _os.path.exists(_LIBPATH):
+    raise ImportError(f"{_LIBNAME} is not built (expected at {_LIBPATH}); run the package build first")
+_lib = _ct.CDLL(_LIBPATH)
+
+_i3 = _ct.c_int * 3
+_d3 = _ct.c_double * 3
+_P = _ct.c_void_p
+_c_char_p = _ct.c_char_p
+_cint = _ct.c_int
+_cdouble = _ct.c_double
+
+_SIGS = {
+    "mpa_last_error": ([], _c_char_p),
+    "mpa_abi_version": ([], _cint),
+    "mpa_device_available": ([], _cint),
+    "mpa_world_create": ([_P, _P, _P, _P, _ct.POINTER(_P)], _cint),
+    "mpa_world_destroy": ([_P], _cint),
+    "mpa_world_get_cellsize": ([_P, _P], _cint),
+    "mpa_world_set_bias_magnetic_field": ([_P, _P], _cint),
+    "mpa_world_get_bias_magnetic_field": ([_P, _P], _cint),
+    "mpa_world_add_ferromagnet": ([_P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_world_num_ferromagnets": ([_P, _ct.POINTER(_cint)], _cint),
+    "mpa_world_ferromagnet_at": ([_P, _cint, _ct.POINTER(_P)], _cint),
+    "mpa_world_find_ferromagnet": ([_P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_world_set_pbc": ([_P, _P, _P, _P], _cint),
+    "mpa_world_set_pbc_auto": ([_P, _P], _cint),
+    "mpa_world_unset_pbc": ([_P], _cint),
+    "mpa_world_get_pbc": ([_P, _P, _P, _P], _cint),
+    "mpa_world_bounding_grid": ([_P, _P, _P], _cint),
+    "mpa_world_stream": ([_P, _ct.POINTER(_P)], _cint),
+    "mpa_fm_name": ([_P, _ct.c_char_p, _cint], _cint),
+    "mpa_fm_grid": ([_P, _P, _P], _cint),
+    "mpa_fm_geometry": ([_P, _P], _cint),
+    "mpa_fm_cellsingeo": ([_P, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_set_flag": ([_P, _c_char_p, _cint], _cint),
+    "mpa_fm_get_flag": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_param_info": ([_P, _c_char_p, _ct.POINTER(_cint), _ct.POINTER(_cint), _c_char_p, _cint,
+                           _c_char_p, _cint], _cint),
+    "mpa_fm_param_set_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_param_set_cells": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_param_get": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_param_get_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_set_magnetization": ([_P, _P, _ct.c_longlong], _cint),
+    "mpa_fm_get_magnetization": ([_P, _P], _cint),
+    "mpa_fm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_eval": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_average": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_scalar": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
+    "mpa_fm_demag_kernel": ([_P, _P, _P], _cint),
+    "mpa_ts_get": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
+    "mpa_ts_set": ([_P, _c_char_p, _cdouble], _cint),
+    "mpa_ts_sensible_timestep": ([_P, _ct.POINTER(_cdouble)], _cint),
+    "mpa_ts_set_method": ([_P, _c_char_p], _cint),
+    "mpa_ts_get_method": ([_P, _c_char_p, _cint], _cint),
+    "mpa_ts_step": ([_P], _cint),
+    "mpa_ts_steps": ([_P, _ct.c_longlong], _cint),
+    "mpa_ts_run": ([_P, _cdouble], _cint),
+    "mpa_demag_plan_create": ([_P, _P, _P, _P, _ct.POINTER(_P)], _cint),
+    "mpa_demag_plan_destroy": ([_P], _cint),
+    "mpa_demag_plan_exec": ([_P, _P, _P, _ct.c_float, _P, _P, _P], _cint),
+    "mpa_demag_plan_info": ([_P, _P, _ct.POINTER(_cdouble), _ct.POINTER(_cdouble)], _cint),
+    "mpa_world_profile_stage": ([_P, _cint, _cint, _P, _P, _P, _ct.POINTER(_cint)], _cint),
+}
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+EXPORTED = tuple(_SIGS)
+MPA_ERR_INVALID, MPA_ERR_RUNTIME, MPA_ERR_HIP, MPA_ERR_INDEX = 1, 2, 3, 4
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = _lib.mpa_last_error().decode(errors="replace")
+    if rc == MPA_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == MPA_ERR_INDEX:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def _enc(s):
+    if not isinstance(s, str):
+        raise TypeError(f"expected a string, got {type(s).__name__}")
+    return s.encode()
+
+
+def device_available():
+    return bool(_lib.mpa_device_available())
+
+
+def _as_i3(v, what):
+    v = tuple(v)
+    if len(v) != 3:
+        raise ValueError(f"'{what}' should have three dimensions.")
+    return _i3(*[int(x) for x in v])
+
+
+def _as_d3(v, what):
+    v = tuple(v)
+    if len(v) != 3:
+        raise ValueError(f"'{what}' should have three dimensions.")
+    return _d3(*[float(x) for x in v])
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+# ---------------------------------------------------------------- Grid
+class Grid:
+    """Grid(size, origin=(0,0,0)) (wrap_grid.cpp:4-17)."""
+
+    def __init__(self, size, origin=(0, 0, 0)):
+        self.size = tuple(int(x) for x in _as_i3(size, "size"))
+        self.origin = tuple(int(x) for x in _as_i3(origin, "origin"))
+
+    @property
+    def ncells(self):
+        return self.size[0] * self.size[1] * self.size[2]
+
+    def __eq__(self, other):
+        return isinstance(other, Grid) and self.size == other.size and self.origin == other.origin
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+    def __repr__(self):
+        return f"Grid(size={self.size}, origin={self.origin})"
+
+
+# -------------------------------------------------------------- System
+class System:
+    """Geometry information of a magnet (wrap_system.cpp:40-51)."""
+
+    def __init__(self, fm):
+        self._fm = fm
+
+    @property
+    def grid(self):
+        return self._fm.grid
+
+    @property
+    def cellsize(self):
+        return self._fm.world.cellsize
+
+    @property
+    def geometry(self):
+        nx, ny, nz = self.grid.size
+        out = _np.empty(nx * ny * nz, dtype=_np.uint8)
+        _check(_lib.mpa_fm_geometry(self._fm._h, _ptr(out)))
+        return out.reshape(nz, ny, nx).astype(bool)
+
+    @property
+    def regions(self):
+        nx, ny, nz = self.grid.size
+        return _np.zeros((nz, ny, nx), dtype=_np.uint32)
+
+    def cell_position(self, idx):
+        cs = self.cellsize
+        o = self.grid.origin
+        return tuple((o[a] + idx[a]) * cs[a] for a in range(3))
+
+    @property
+    def origin(self):
+        return self.cell_position((0, 0, 0))
+
+    @property
+    def center(self):
+        s = self.grid.size
+        cs = self.cellsize
+        o = self.origin
+        return tuple(o[a] + (s[a] - 1) * cs[a] / 2 for a in range(3))
+
+    @property
+    def cellsingeo(self):
+        v = _cint()
+        _check(_lib.mpa_fm_cellsingeo(self._fm._h, _ct.byref(v)))
+        return v.value
+
+
+# ------------------------------------------------------- FieldQuantity
+class FieldQuantity:
+    """A field quantity of a ferromagnet (wrap_fieldquantity.cpp:8-22)."""
+
+    _UNITS = {"magnetization": "", "exchange_field": "T", "anisotropy_field": "T", "external_field": "T",
+              "dmi_field": "T", "demag_field": "T", "effective_field": "T", "torque": "/s",
+              "llg_torque": "/s", "damping_torque": "/s"}
+
+    def __init__(self, fm, qname):
+        self._fm = fm
+        self._q = qname
+        nc = _cint()
+        _check(_lib.mpa_fm_quantity_ncomp(fm._h, _enc(qname), _ct.byref(nc)))
+        self.ncomp = nc.value
+
+    @property
+    def name(self):
+        return f"{self._fm.name}:{self._q}"
+
+    @property
+    def unit(self):
+        return self._UNITS.get(self._q, "J/m3" if self._q.endswith("density") else "")
+
+    @property
+    def grid(self):
+        return self._fm.grid
+
+    @property
+    def system(self):
+        return self._fm.system
+
+    def eval(self):
+        nx, ny, nz = self.grid.size
+        out = _np.empty((self.ncomp, nz, ny, nx), dtype=_np.float32)
+        _check(_lib.mpa_fm_eval(self._fm._h, _enc(self._q), _ptr(out)))
+        return out
+
+    def exec(self):
+        self.eval()
+
+    def average(self):
+        out = _np.zeros(self.ncomp, dtype=_np.float64)
+        _check(_lib.mpa_fm_average(self._fm._h, _enc(self._q), _ptr(out)))
+        return [float(v) for v in out]
+
+
+class ScalarQuantity:
+    """Scalar quantity of a ferromagnet (wrap_scalarquantity.cpp)."""
+
+    _UNITS = {"max_torque": "/s"}
+
+    def __init__(self, fm, qname):
+        self._fm = fm
+        self._q = qname
+
+    @property
+    def name(self):
+        return f"{self._fm.name}:{self._q}"
+
+    @property
+    def unit(self):
+        return self._UNITS.get(self._q, "J")
+
+    def eval(self):
+        v = _cdouble()
+        _check(_lib.mpa_fm_scalar(self._fm._h, _enc(self._q), _ct.byref(v)))
+        return v.value
+
+
+# ----------------------------------------------------------- Variable
+class Variable(FieldQuantity):
+    """The magnetisation of a ferromagnet (wrap_variable.cpp:8-16)."""
+
+    def __init__(self, fm):
+        super().__init__(fm, "magnetization")
+
+    def get(self):
+        nx, ny, nz = self.grid.size
+        out = _np.empty((3, nz, ny, nx), dtype=_np.float32)
+        _check(_lib.mpa_fm_get_magnetization(self._fm._h, _ptr(out)))
+        return out
+
+    def set(self, value):
+        a = _np.ascontiguousarray(_np.asarray(value, dtype=_np.float32))
+        nx, ny, nz = self.grid.size
+        if a.ndim == 1:
+            if a.shape[0] != 3:
+                raise ValueError(f"The number of components do not match, expected {a.shape[0]}, got 3.")
+        elif a.shape != (3, nz, ny, nx):
+            raise ValueError(f"The shape of the data does not match the shape of the field, expected "
+                             f"{(3, nz, ny, nx)}, got {a.shape}.")
+        _check(_lib.mpa_fm_set_magnetization(self._fm._h, _ptr(a), a.size))
+
+
+# ---------------------------------------------------------- Parameter
+class Parameter(FieldQuantity):
+    """Material parameter (wrap_parameter.cpp:13-107)."""
+
+    def __init__(self, fm, key):
+        self._fm = fm
+        self._q = key
+        self._key = _enc(key)
+        nc, uni = _cint(), _cint()
+        nm = _ct.create_string_buffer(256)
+        un = _ct.create_string_buffer(64)
+        _check(_lib.mpa_fm_param_info(fm._h, self._key, _ct.byref(nc), _ct.byref(uni), nm, 256, un, 64))
+        self.ncomp = nc.value
+        self._name = nm.value.decode()
+        self._unit = un.value.decode()
+
+    @property
+    def name(self):
+        return self._name
+
+    @property
+    def unit(self):
+        return self._unit
+
+    @property
+    def is_uniform(self):
+        nc, uni = _cint(), _cint()
+        _check(_lib.mpa_fm_param_info(self._fm._h, self._key, _ct.byref(nc), _ct.byref(uni), None, 0, None, 0))
+        return bool(uni.value)
+
+    @property
+    def is_dynamic(self):
+        return False
+
+    @property
+    def uniform_value(self):
+        out = _np.zeros(self.ncomp, dtype=_np.float64)
+        _check(_lib.mpa_fm_param_get_uniform(self._fm._h, self._key, _ptr(out)))
+        return float(out[0]) if self.ncomp == 1 else tuple(float(v) for v in out)
+
+    @uniform_value.setter
+    def uniform_value(self, value):
+        self.set(value)
+
+    def eval(self):
+        nx, ny, nz = self.grid.size
+        out = _np.empty((self.ncomp, nz, ny, nx), dtype=_np.float32)
+        _check(_lib.mpa_fm_param_get(self._fm._h, self._key, _ptr(out)))
+        return out
+
+    def average(self):
+        v = self.eval().astype(_np.float64)
+        geom = self._fm.system.geometry
+        n = max(int(geom.sum()), 1)
+        return [float(v[c][geom].sum() / n) for c in range(self.ncomp)]
+
+    def set(self, value):
+        if _np.isscalar(value) or (isinstance(value, (tuple, list)) and len(value) == self.ncomp
+                                   and all(_np.isscalar(v) for v in value)):
+            vals = _np.atleast_1d(_np.asarray(value, dtype=_np.float64))
+            if vals.size != self.ncomp:
+                raise ValueError(f"expected {self.ncomp} components")
+            _check(_lib.mpa_fm_param_set_uniform(self._fm._h, self._key, _ptr(_np.ascontiguousarray(vals))))
+            return
+        a = _np.asarray(value, dtype=_np.float32)
+        nx, ny, nz = self.grid.size
+        if a.shape == (nz, ny, nx) and self.ncomp == 1:
+            a = a[None]
+        if a.shape != (self.ncomp, nz, ny, nx):
+            raise ValueError(f"The shape of the data does not match the shape of the parameter, expected "
+                             f"{(self.ncomp, nz, ny, nx)}, got {a.shape}.")
+        a = _np.ascontiguousarray(a)
+        _check(_lib.mpa_fm_param_set_cells(self._fm._h, self._key, _ptr(a)))
+
+    def add_time_term(self, term, mask=None):
+        raise NotImplementedError("time-dependent parameter terms are not supported by this build")
+
+    def remove_time_terms(self):
+        pass
+
+
+class VectorParameter(Parameter):
+    pass
+
+
+class DmiTensor:
+    """The nine DMI parameters (wrap_dmitensor.cpp; dmitensor.hpp:12-22)."""
+
+    KEYS = ("xxy", "xyz", "xxz", "yxy", "yyz", "yxz", "zxy", "zyz", "zxz")
+
+    def __init__(self, fm):
+        for k in self.KEYS:
+            setattr(self, k, Parameter(fm, "dmi_" + k))
+
+
+# --------------------------------------------------------- Ferromagnet
+class Ferromagnet:
+    """Handle of a ferromagnet owned by a World (wrap_ferromagnet.cpp:28-73)."""
+
+    _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins")
+    _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field")
+
+    def __init__(self, handle, world):
+        object.__setattr__(self, "_h", handle)
+        object.__setattr__(self, "_world", world)
+        params = {k: Parameter(self, k) for k in self._PARAMS}
+        params.update({k: VectorParameter(self, k) for k in self._VPARAMS})
+        object.__setattr__(self, "_params", params)
+        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
+        object.__setattr__(self, "magnetization", Variable(self))
+
+    def __getattr__(self, key):
+        params = object.__getattribute__(self, "_params")
+        if key in params:
+            return params[key]
+        raise AttributeError(key)
+
+    @property
+    def world(self):
+        return self._world
+
+    @property
+    def name(self):
+        buf = _ct.create_string_buffer(256)
+        _check(_lib.mpa_fm_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    @property
+    def grid(self):
+        s, o = _i3(), _i3()
+        _check(_lib.mpa_fm_grid(self._h, s, o))
+        return Grid(tuple(s), tuple(o))
+
+    @property
+    def system(self):
+        return System(self)
+
+    def _flag(self, key):
+        v = _cint()
+        _check(_lib.mpa_fm_get_flag(self._h, _enc(key), _ct.byref(v)))
+        return bool(v.value)
+
+    @property
+    def enable_demag(self):
+        return self._flag("enable_demag")
+
+    @enable_demag.setter
+    def enable_demag(self, value):
+        _check(_lib.mpa_fm_set_flag(self._h, b"enable_demag", int(bool(value))))
+
+    @property
+    def enable_openbc(self):
+        return self._flag("enable_openbc")
+
+    @enable_openbc.setter
+    def enable_openbc(self, value):
+        _check(_lib.mpa_fm_set_flag(self._h, b"enable_openbc", int(bool(value))))
+
+    @property
+    def is_sublattice(self):
+        return False
+
+    def field_quantity(self, name):
+        return FieldQuantity(self, name)
+
+    def scalar_quantity(self, name):
+        return ScalarQuantity(self, name)
+
+
+def _demag_kernel(fm):
+    """The demag kernel on the reference kernel grid, shape (6, Kz, Ky, Kx)."""
+    ks = _i3()
+    _check(_lib.mpa_fm_demag_kernel(fm._h, ks, None))
+    kx, ky, kz = tuple(ks)
+    out = _np.empty((6, kz, ky, kx), dtype=_np.float32)
+    _check(_lib.mpa_fm_demag_kernel(fm._h, ks, _ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------- TimeSolver
+class TimeSolver:
+    """Time solver of a World (wrap_timesolver.cpp:15-44)."""
+
+    def __init__(self, world):
+        object.__setattr__(self, "_world", world)
+
+    _KEYS = ("time", "timestep", "max_error", "headroom", "lower_bound", "upper_bound", "sensible_factor",
+             "sensible_timestep_default")
+
+    def _get(self, key):
+        v = _cdouble()
+        _check(_lib.mpa_ts_get(self._world._h, _enc(key), _ct.byref(v)))
+        return v.value
+
+    def _set(self, key, value):
+        _check(_lib.mpa_ts_set(self._world._h, _enc(key), float(value)))
+
+    def __getattr__(self, key):
+        if key in TimeSolver._KEYS or key in ("nsteps", "nrejected", "nrhs"):
+            return self._get(key)
+        raise AttributeError(key)
+
+    def __setattr__(self, key, value):
+        if key in TimeSolver._KEYS:
+            self._set(key, value)
+        elif key == "adaptive_timestep":
+            self._set(key, 1.0 if value else 0.0)
+        else:
+            object.__setattr__(self, key, value)
+
+    @property
+    def adaptive_timestep(self):
+        return self._get("adaptive_timestep") != 0.0
+
+    @property
+    def sensible_timestep(self):
+        v = _cdouble()
+        _check(_lib.mpa_ts_sensible_timestep(self._world._h, _ct.byref(v)))
+        return v.value
+
+    @property
+    def method(self):
+        buf = _ct.create_string_buffer(64)
+        _check(_lib.mpa_ts_get_method(self._world._h, buf, 64))
+        return buf.value.decode()
+
+    def set_method(self, name):
+        _check(_lib.mpa_ts_set_method(self._world._h, _enc(name)))
+
+    def step(self):
+        _check(_lib.mpa_ts_step(self._world._h))
+
+    def steps(self, n):
+        _check(_lib.mpa_ts_steps(self._world._h, int(n)))
+
+    def run(self, duration):
+        _check(_lib.mpa_ts_run(self._world._h, float(duration)))
+
+    def run_while(self, condition):
+        while condition():
+            self.step()
+
+
+# --------------------------------------------------------------- World
+class World:
+    """World(cellsize[, mastergrid, pbc_repetitions]) (wrap_world.cpp:36-119)."""
+
+    def __init__(self, cellsize, mastergrid=None, pbc_repetitions=(0, 0, 0)):
+        if mastergrid is None:
+            mastergrid = Grid((0, 0, 0))
+        h = _P()
+        _check(_lib.mpa_world_create(_as_d3(cellsize, "cellsize"), _i3(*mastergrid.size), _i3(*mastergrid.origin),
+                                     _as_i3(pbc_repetitions, "pbc_repetitions"), _ct.byref(h)))
+        self._h = h
+        self._magnets = {}
+        self.timesolver = TimeSolver(self)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.mpa_world_destroy(h)
+            self._h = None
+
+    @property
+    def cellsize(self):
+        out = _d3()
+        _check(_lib.mpa_world_get_cellsize(self._h, out))
+        return tuple(out)
+
+    @property
+    def bias_magnetic_field(self):
+        out = _d3()
+        _check(_lib.mpa_world_get_bias_magnetic_field(self._h, out))
+        return tuple(out)
+
+    @bias_magnetic_field.setter
+    def bias_magnetic_field(self, b):
+        _check(_lib.mpa_world_set_bias_magnetic_field(self._h, _as_d3(b, "bias_magnetic_field")))
+
+    def add_ferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
+        if regions_array is not None and _np.any(_np.asarray(regions_array) != 0):
+            raise NotImplementedError("regions are not supported by this build")
+        geo = None
+        if geometry_array is not None:
+            nx, ny, nz = grid.size
+            g = _np.asarray(geometry_array, dtype=bool)
+            if g.shape != (nz, ny, nx):
+                raise ValueError("The dimensions of the geometry do not match the dimensions of the grid.")
+            geo = _np.ascontiguousarray(g.astype(_np.uint8))
+        h = _P()
+        _check(_lib.mpa_world_add_ferromagnet(self._h, _i3(*grid.size), _i3(*grid.origin),
+                                              None if geo is None else _ptr(geo), _enc(name or ""),
+                                              _ct.byref(h)))
+        fm = Ferromagnet(h, self)
+        self._magnets[fm.name] = fm
+        return fm
+
+    def get_ferromagnet(self, name):
+        return self._magnets.get(name)
+
+    @property
+    def ferromagnets(self):
+        return dict(self._magnets)
+
+    @property
+    def antiferromagnets(self):
+        return {}
+
+    def get_antiferromagnet(self, name):
+        return None
+
+    def _pbc(self):
+        s, o, r = _i3(), _i3(), _i3()
+        _check(_lib.mpa_world_get_pbc(self._h, s, o, r))
+        return Grid(tuple(s), tuple(o)), tuple(r)
+
+    @property
+    def mastergrid(self):
+        return self._pbc()[0]
+
+    @mastergrid.setter
+    def mastergrid(self, grid):
+        self.set_pbc(grid, self.pbc_repetitions)
+
+    @property
+    def pbc_repetitions(self):
+        return self._pbc()[1]
+
+    @pbc_repetitions.setter
+    def pbc_repetitions(self, reps):
+        self.set_pbc(self.mastergrid, reps)
+
+    def set_pbc(self, *args):
+        if len(args) == 1:
+            _check(_lib.mpa_world_set_pbc_auto(self._h, _as_i3(args[0], "pbc_repetitions")))
+        elif len(args) == 2:
+            grid, reps = args
+            _check(_lib.mpa_world_set_pbc(self._h, _i3(*grid.size), _i3(*grid.origin),
+                                          _as_i3(reps, "pbc_repetitions")))
+        else:
+            raise TypeError("set_pbc(pbc_repetitions) or set_pbc(mastergrid, pbc_repetitions)")
+
+    def unset_pbc(self):
+        _check(_lib.mpa_world_unset_pbc(self._h))
+
+    @property
+    def bounding_grid(self):
+        s, o = _i3(), _i3()
+        _check(_lib.mpa_world_bounding_grid(self._h, s, o))
+        return Grid(tuple(s), tuple(o))
+
+    def stream(self):
+        v = _P()
+        _check(_lib.mpa_world_stream(self._h, _ct.byref(v)))
+        return v.value
+
+    def profile_stage(self, reps=10):
+        cap = 16
+        names = _ct.create_string_buffer(32 * cap)
+        ms = _np.zeros(cap)
+        by = _np.zeros(cap)
+        cnt = _cint()
+        _check(_lib.mpa_world_profile_stage(self._h, int(reps), cap, names, _ptr(ms), _ptr(by), _ct.byref(cnt)))
+        raw = names.raw
+        out = []
+        for i in range(cnt.value):
+            nm = raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
+            out.append((nm, float(ms[i]), float(by[i])))
+        return out
+
+    def enable_multi_gpu(self, enable=True):
+        if enable:
+            raise RuntimeError("this build runs on one GPU")
+
+    def is_multi_gpu_enabled(self):
+        return False
+
+
+# ----------------------------------------------------------- DemagPlan
+class DemagPlan:
+    """Stand-alone demag operator on device pointers (mpa_demag_plan_*)."""
+
+    def __init__(self, size, cellsize, mastergrid=(0, 0, 0), pbc=(0, 0, 0)):
+        h = _P()
+        _check(_lib.mpa_demag_plan_create(_as_i3(size, "size"), _as_d3(cellsize, "cellsize"),
+                                          _as_i3(mastergrid, "mastergrid"), _as_i3(pbc, "pbc"), _ct.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.mpa_demag_plan_destroy(h)
+            self._h = None
+
+    def exec(self, m_ptr, h_ptr, msat_uniform=1.0, msat_ptr=None, geom_ptr=None, stream=None):
+        _check(_lib.mpa_demag_plan_exec(self._h, m_ptr, msat_ptr, float(msat_uniform), geom_ptr, h_ptr, stream))
+
+    def info(self):
+        ks = _i3()
+        b, d = _cdouble(), _cdouble()
+        _check(_lib.mpa_demag_plan_info(self._h, ks, _ct.byref(b), _ct.byref(d)))
+        return tuple(ks), b.value, d.value

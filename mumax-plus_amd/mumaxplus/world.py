@@ -1,0 +1,94 @@
+"""World (mumaxplus/world.py of the reference)."""
+
+from . import _mumaxpluscpp as _cpp
+from .grid import Grid
+from .timesolver import TimeSolver
+
+
+class World:
+    """Construct a world with a given cell size, optionally periodic.
+
+    pbc_repetitions : number of image repetitions of the mastergrid per axis
+    mastergrid      : Grid of the periodic box (0 = open axis)
+    """
+
+    def __init__(self, cellsize, pbc_repetitions=(0, 0, 0), mastergrid=Grid((0, 0, 0))):
+        if len(cellsize) != 3:
+            raise ValueError("'cellsize' should have three dimensions.")
+        if len(pbc_repetitions) != 3:
+            raise ValueError("'pbc_repetitions' should have three dimensions.")
+        if not all(isinstance(item, int) for item in pbc_repetitions):
+            raise ValueError("All elements of `pbc_repetitions` must be integers.")
+        self._impl = _cpp.World(cellsize, mastergrid._impl, pbc_repetitions)
+
+    def __repr__(self):
+        return f"World(cellsize={self.cellsize})"
+
+    @classmethod
+    def _from_impl(cls, impl):
+        world = cls.__new__(cls)
+        world._impl = impl
+        return world
+
+    @property
+    def timesolver(self):
+        return TimeSolver(self._impl.timesolver)
+
+    def get_ferromagnet(self, name):
+        from .ferromagnet import Ferromagnet
+        impl = self._impl.get_ferromagnet(name)
+        if impl is None:
+            raise KeyError(f"No magnet named {name}")
+        return Ferromagnet._from_impl(impl)
+
+    @property
+    def ferromagnets(self):
+        from .ferromagnet import Ferromagnet
+        return {k: Ferromagnet._from_impl(v) for k, v in self._impl.ferromagnets.items()}
+
+    @property
+    def antiferromagnets(self):
+        return {}
+
+    @property
+    def cellsize(self):
+        return self._impl.cellsize
+
+    @property
+    def bias_magnetic_field(self):
+        return self._impl.bias_magnetic_field
+
+    @bias_magnetic_field.setter
+    def bias_magnetic_field(self, value):
+        self._impl.bias_magnetic_field = value
+
+    @property
+    def mastergrid(self):
+        return Grid._from_impl(self._impl.mastergrid)
+
+    @mastergrid.setter
+    def mastergrid(self, grid):
+        self._impl.mastergrid = grid._impl
+
+    @property
+    def pbc_repetitions(self):
+        return self._impl.pbc_repetitions
+
+    @pbc_repetitions.setter
+    def pbc_repetitions(self, value):
+        self._impl.pbc_repetitions = value
+
+    @property
+    def bounding_grid(self):
+        return Grid._from_impl(self._impl.bounding_grid)
+
+    def set_pbc(self, pbc_repetitions, mastergrid=None):
+        """Set periodic boundary conditions; without a mastergrid the bounding
+        box of all magnets is used along the repeated axes."""
+        if mastergrid is None:
+            self._impl.set_pbc(pbc_repetitions)
+        else:
+            self._impl.set_pbc(mastergrid._impl, pbc_repetitions)
+
+    def unset_pbc(self):
+        self._impl.unset_pbc()
