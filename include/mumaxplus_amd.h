@@ -148,12 +148,15 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
 
 /* --------------------------------------------------------------- Timing -- */
 
-/* Time the kernels of one RK stage of world w (stage combine, demag passes, local field +
- * torque) with HIP events on the world stream, averaged over `reps` repetitions.  Fills up
- * to `cap` entries of names (each 32 bytes), avg_ms and algorithmic bytes per launch;
- * *count receives the number of kernels. */
-int mpa_world_profile_stage(mpa_world* w, int reps, int cap, char* names, double* avg_ms, double* bytes,
-                            int* count);
+/* Per-kernel timing of everything the world launches (no counterpart in the reference, which
+ * has no timers).  While enabled, each kernel launch is bracketed by HIP events on the world
+ * stream and its algorithmic HBM bytes are recorded.  The report gives, per kernel name (32
+ * bytes each), the summed milliseconds, launch count and summed algorithmic bytes; *count
+ * receives the number of kernel names (at most `cap` are written). */
+int mpa_world_set_profiling(mpa_world* w, int enable);
+int mpa_world_profile_reset(mpa_world* w);
+int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_ms, long long* launches,
+                             double* bytes, int* count);
 
 #ifdef __cplusplus
 }

@@ -509,69 +509,39 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
 }
 
 // ---------------------------------------------------------------- Timing
-int mpa_world_profile_stage(mpa_world* w, int reps, int cap, char* names, double* avg_ms, double* bytes, int* count) {
+int mpa_world_set_profiling(mpa_world* w, int enable) {
+  return guard([&] {
+    need(w, "world");
+    w->w.prof.flush_sync(w->w.stream);
+    w->w.prof.enabled = enable != 0;
+  });
+}
+
+int mpa_world_profile_reset(mpa_world* w) {
+  return guard([&] {
+    need(w, "world");
+    w->w.prof.flush_sync(w->w.stream);
+    w->w.prof.reset();
+  });
+}
+
+int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_ms, long long* launches,
+                             double* bytes, int* count) {
   return guard([&] {
     need(w, "world");
     need(count, "count");
-    World& W = w->w;
-    if (W.magnets.empty()) throw Error(MPA_ERR_RUNTIME, "no magnets to profile");
-    if (reps < 1) reps = 1;
-    Ferromagnet& f = *W.magnets[0];
-    hipStream_t s = W.stream;
-    f.ensure_stage_buffers(2);
-    HIP_CHECK(hipMemcpyAsync(f.x0.get(), f.m.get(), (size_t)3 * f.N * sizeof(float), hipMemcpyDeviceToDevice, s));
-    const unsigned terms = f.active_terms();
-    const bool dem = (terms & TERM_DEMAG) != 0;
-    const int npass = dem ? f.demag().npasses() : 0;
-    const int nk = npass + 2;  // combine, demag passes, local field + torque
-    std::vector<hipEvent_t> ev(nk + 1);
-    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-    std::vector<double> acc(nk, 0.0);
-    const double N = f.N;
-    for (int r = 0; r < reps; ++r) {
-      HIP_CHECK(hipEventRecord(ev[0], s));
-      Combo co{};
-      co.nk = 1;
-      co.k[0] = f.k[0].get();
-      co.c[0] = 0.f;
-      launch_combine(f.m.get(), f.x0.get(), co, 3, f.N, f.geom_ptr(), true, s);
-      if (dem) f.demag().exec_timed(f.m.get(), f.params.at("msat").ps(), f.geom_ptr(), f.hdemag.get(), s, &ev[1]);
-      else HIP_CHECK(hipEventRecord(ev[1], s));
-      launch_field(f.desc(terms), f.m.get(), f.hdemag.get(), f.k[1].get(), MODE_LLG, s);
-      HIP_CHECK(hipEventRecord(ev[nk], s));
-      HIP_CHECK(hipEventSynchronize(ev[nk]));
-      // ev[1] marks the end of the combine; ev[i+1] the end of kernel i
-      for (int i = 0; i < nk; ++i) {
-        float ms = 0.f;
-        const int a = (i == 0) ? 0 : i;
-        const int b = (i == 0) ? 1 : i + 1;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev[a], ev[b]));
-        acc[i] += ms;
+    w->w.prof.flush_sync(w->w.stream);
+    int i = 0;
+    for (const auto& kv : w->w.prof.stats) {
+      if (i < cap) {
+        if (names) copy_str(kv.first, names + 32 * i, 32);
+        if (total_ms) total_ms[i] = kv.second.total_ms;
+        if (launches) launches[i] = kv.second.count;
+        if (bytes) bytes[i] = kv.second.bytes;
       }
+      ++i;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    *count = nk;
-    for (int i = 0; i < nk && i < cap; ++i) {
-      std::string nm;
-      double b = 0.0;
-      if (i == 0) {
-        nm = "k_combine";
-        b = 12.0 * 3 * N;  // x0, one k, write m (the stage-1 combination)
-      } else if (i <= npass) {
-        nm = f.demag().pass_name(i - 1);
-        b = f.demag().pass_bytes(i - 1);
-      } else {
-        nm = "k_field";
-        b = 12.0 * N * (dem ? 3 : 2);  // m, h_demag, k
-      }
-      if (names) copy_str(nm, names + 32 * i, 32);
-      if (avg_ms) avg_ms[i] = acc[i] / reps;
-      if (bytes) bytes[i] = b;
-    }
-    // restore the state
-    HIP_CHECK(hipMemcpyAsync(f.m.get(), f.x0.get(), (size_t)3 * f.N * sizeof(float), hipMemcpyDeviceToDevice, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    W.touch();
+    *count = i;
   });
 }
 

@@ -556,16 +556,7 @@ int lines_for(int L, int cap, int maxl) {
 }
 }  // namespace
 
-void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s) {
-  exec_timed(m, msat, geom, h, s, nullptr);
-}
-
-void DemagPlan::exec_timed(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, hipEvent_t* ev) {
-  int evi = 0;
-  auto mark = [&]() {
-    if (ev) HIP_CHECK(hipEventRecord(ev[evi++], s));
-  };
-  mark();
+void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof) {
   DD d;
   d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
   d.Kx = K_[0]; d.Ky = K_[1]; d.Kz = K_[2];
@@ -576,31 +567,39 @@ void DemagPlan::exec_timed(const float* m, PS msat, const uint8_t* geom, float* 
   // x passes: PP line pairs per workgroup
   const bool xl = d.Kx > CAP_SMALL;
   const int PPf = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 16);
-  launch_e<LXfwd>(xl, d, m, msat, geom, A_.get(), tw_[0].get(), PPf, s);
-  mark();
+  {
+    Timed t(prof, "k_xfwd", pass_bytes(0), s);
+    launch_e<LXfwd>(xl, d, m, msat, geom, A_.get(), tw_[0].get(), PPf, s);
+  }
   if (is2d()) {
     const bool yl = 3 * d.Ky > CAP_SMALL;
     const int PQ = lines_for(3 * d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
+    Timed t(prof, "k_ypass2d", pass_bytes(1), s);
     launch_e<LY2d>(yl, d, A_.get(), (const float*)T_.get(), tw_[1].get(), PQ, s);
-    mark();
   } else {
     const bool yl = d.Ky > CAP_SMALL;
     const int PY = lines_for(d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
-    launch_e<LYfwd>(yl, d, (const float2*)A_.get(), B_.get(), tw_[1].get(), PY, s);
-    mark();
+    {
+      Timed t(prof, "k_yfwd", pass_bytes(1), s);
+      launch_e<LYfwd>(yl, d, (const float2*)A_.get(), B_.get(), tw_[1].get(), PY, s);
+    }
     const bool zl = 3 * d.Kz > CAP_SMALL;
     const int PZ = lines_for(3 * d.Kz, zl ? CAP_LARGE : CAP_SMALL, 16);
-    launch_e<LZpass>(zl, d, B_.get(), (const float*)T_.get(), tw_[2].get(), PZ, s);
-    mark();
-    launch_e<LYinv>(yl, d, (const float2*)B_.get(), A_.get(), tw_[1].get(), PY, s);
-    mark();
+    {
+      Timed t(prof, "k_zpass", pass_bytes(2), s);
+      launch_e<LZpass>(zl, d, B_.get(), (const float*)T_.get(), tw_[2].get(), PZ, s);
+    }
+    {
+      Timed t(prof, "k_yinv", pass_bytes(3), s);
+      launch_e<LYinv>(yl, d, (const float2*)B_.get(), A_.get(), tw_[1].get(), PY, s);
+    }
   }
   // the inverse x pass also stages the half spectrum: fit both in the LDS budget
   int PPi = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 8);
   while (PPi > 1 && xinv_lds(PPi, d.Hx, d.Kx) > LDS_LIMIT) --PPi;
   check_lds(xinv_lds(PPi, d.Hx, d.Kx), "k_xinv");
+  Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
   launch_e<LXinv>(xl, d, (const float2*)A_.get(), geom, h, tw_[0].get(), PPi, s);
-  mark();
 }
 
 const char* DemagPlan::pass_name(int i) const {

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "profiler.hpp"
 
 namespace mpa {
 
@@ -25,10 +26,9 @@ class DemagPlan {
   explicit DemagPlan(const DemagGeometry& g);
   // h = -mu0 N * (Ms m) on the magnet grid; zero outside the geometry.
   // m, h: 3*N floats (SoA, x fastest).  geom: N bytes or nullptr.
-  void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s);
-  // Same, recording ev[0] before the first pass and ev[i+1] after pass i
-  // (npasses() events + 1).  Pass names and algorithmic bytes per pass:
-  void exec_timed(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, hipEvent_t* ev);
+  // prof (optional) times each pass with HIP events.
+  void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof = nullptr);
+  // Pass names and algorithmic HBM bytes per pass.
   int npasses() const { return is2d() ? 3 : 5; }
   const char* pass_name(int i) const;
   double pass_bytes(int i) const;

@@ -236,7 +236,10 @@ void Ferromagnet::ensure_stage_buffers(int S) {
 
 void Ferromagnet::eval_rhs(float* out, hipStream_t s) {
   const unsigned terms = active_terms();
-  if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s);
+  Profiler* prof = &world->prof;
+  if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s, prof);
+  // algorithmic bytes: read m, read h_demag (when present), write the torque
+  Timed t(prof, "k_field", 12.0 * N * ((terms & TERM_DEMAG) ? 3 : 2), s);
   launch_field(desc(terms), m.get(), hdemag.get(), out, MODE_LLG, s);
 }
 
@@ -408,8 +411,10 @@ void TimeSolver::step() {
   hipStream_t s = world_->stream;
   const Tableau& tb = tableau_by_name(method);
   const int S = (int)tb.nodes.size();
+  Profiler* prof = &world_->prof;
   for (auto& mag : mags) {
     mag->ensure_stage_buffers(S);
+    Timed t(prof, "copy_x0", 24.0 * mag->N, s);
     HIP_CHECK(hipMemcpyAsync(mag->x0.get(), mag->m.get(), (size_t)3 * mag->N * sizeof(float),
                              hipMemcpyDeviceToDevice, s));
   }
@@ -427,6 +432,7 @@ void TimeSolver::step() {
             co.k[j] = mag->k[j].get();
             co.c[j] = (float)(h * tb.A[st][j]);
           }
+          Timed t(prof, "k_combine", 12.0 * mag->N * (st + 2), s);
           launch_combine(mag->m.get(), mag->x0.get(), co, 3, mag->N, mag->geom_ptr(), true, s);
         }
       }
@@ -445,6 +451,7 @@ void TimeSolver::step() {
           co.k[j] = mag->k[j].get();
           co.c[j] = (float)(h * tb.w1[j]);
         }
+        Timed t(prof, "k_combine", 12.0 * mag->N * (S + 2), s);
         launch_combine(mag->m.get(), mag->x0.get(), co, 3, mag->N, mag->geom_ptr(), true, s);
       }
     }
@@ -458,6 +465,7 @@ void TimeSolver::step() {
         co.k[j] = mag->k[j].get();
         co.c[j] = (float)(h * (tb.w1[j] - tb.w2[j]));
       }
+      Timed t(prof, "k_combo_maxnorm", 12.0 * mag->N * S, s);
       err = std::max(err, (double)world_->reducer.max_combo_norm(co, mag->N, mag->geom_ptr(), s));
     }
     const bool success = err < max_error;
@@ -485,6 +493,7 @@ void TimeSolver::step() {
 void TimeSolver::steps(long long n) {
   for (long long i = 0; i < n; ++i) step();
   HIP_CHECK(hipStreamSynchronize(world_->stream));
+  world_->prof.flush_sync(world_->stream);
 }
 
 // TimeSolver::run (src/core/timesolver.cpp:97-111)

@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "demag.hpp"
 #include "fields.hpp"
+#include "profiler.hpp"
 #include "rk.hpp"
 
 namespace mpa {
@@ -128,6 +129,7 @@ class World {
   TimeSolver solver;
   hipStream_t stream = nullptr;
   Reducer reducer;
+  Profiler prof;
   unsigned long long version = 1;  // bumped by every state change (invalidates FSAL reuse)
 
   Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry, const char* name);

@@ -82,7 +82,9 @@ _SIGS = {
     "mpa_demag_plan_destroy": ([_P], _cint),
     "mpa_demag_plan_exec": ([_P, _P, _P, _ct.c_float, _P, _P, _P], _cint),
     "mpa_demag_plan_info": ([_P, _P, _ct.POINTER(_cdouble), _ct.POINTER(_cdouble)], _cint),
-    "mpa_world_profile_stage": ([_P, _cint, _cint, _P, _P, _P, _ct.POINTER(_cint)], _cint),
+    "mpa_world_set_profiling": ([_P, _cint], _cint),
+    "mpa_world_profile_reset": ([_P], _cint),
+    "mpa_world_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -654,18 +656,26 @@ class World:
         _check(_lib.mpa_world_stream(self._h, _ct.byref(v)))
         return v.value
 
-    def profile_stage(self, reps=10):
-        cap = 16
+    def set_profiling(self, enable):
+        _check(_lib.mpa_world_set_profiling(self._h, int(bool(enable))))
+
+    def profile_reset(self):
+        _check(_lib.mpa_world_profile_reset(self._h))
+
+    def profile_report(self):
+        """{kernel name: (total ms, launches, algorithmic bytes)} since the last reset."""
+        cap = 64
         names = _ct.create_string_buffer(32 * cap)
         ms = _np.zeros(cap)
+        n = _np.zeros(cap, dtype=_np.int64)
         by = _np.zeros(cap)
         cnt = _cint()
-        _check(_lib.mpa_world_profile_stage(self._h, int(reps), cap, names, _ptr(ms), _ptr(by), _ct.byref(cnt)))
+        _check(_lib.mpa_world_profile_report(self._h, cap, names, _ptr(ms), _ptr(n), _ptr(by), _ct.byref(cnt)))
         raw = names.raw
-        out = []
-        for i in range(cnt.value):
+        out = {}
+        for i in range(min(cnt.value, cap)):
             nm = raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
-            out.append((nm, float(ms[i]), float(by[i])))
+            out[nm] = (float(ms[i]), int(n[i]), float(by[i]))
         return out
 
     def enable_multi_gpu(self, enable=True):

@@ -21,10 +21,18 @@ film limit H_z -> -mu0 Ms.
 """
 
 import numpy as np
+import scipy.fft as _sfft
 
 from . import newell
 
 MU0 = 1.25663706212e-6
+_WORKERS = 1
+
+
+def set_fft_workers(n):
+    """Threads used by the FFTs (scipy.fft workers)."""
+    global _WORKERS
+    _WORKERS = max(1, int(n))
 
 
 def padded_size(n):
@@ -42,19 +50,46 @@ def kernel_offsets(size, cs, mastergrid=(0, 0, 0), pbc=(0, 0, 0)):
                                  np.arange(-(nz - 1), nz), cs, mastergrid, pbc)
 
 
-def kernel_spectrum(size, cs, mastergrid=(0, 0, 0), pbc=(0, 0, 0), K=None):
-    """Real spectrum (6, Kz, Ky, Kx//2+1) of the circularly placed kernel."""
+# parity of each component (xx, yy, zz, xy, xz, yz) along x, y, z: True = odd
+ODD = ((False, False, False), (False, False, False), (False, False, False),
+       (True, True, False), (True, False, True), (False, True, True))
+
+
+def kernel_offsets_octant(size, cs, mastergrid=(0, 0, 0), pbc=(0, 0, 0)):
+    """Same array as kernel_offsets, built from the offsets 0..N-1 and the
+    even / odd symmetry of each component (for large grids)."""
+    nx, ny, nz = size
+    oct_ = newell.kernel_on_axes(np.arange(nx), np.arange(ny), np.arange(nz), cs, mastergrid, pbc)
+    out = np.empty((6, 2 * nz - 1, 2 * ny - 1, 2 * nx - 1))
+    ax = [np.abs(np.arange(-(n - 1), n)) for n in (nx, ny, nz)]
+    sg = [np.sign(np.arange(-(n - 1), n)).astype(float) for n in (nx, ny, nz)]
+    for c in range(6):
+        v = oct_[c][np.ix_(ax[2], ax[1], ax[0])]
+        for a, odd in enumerate(ODD[c]):
+            if odd:
+                shape = [1, 1, 1]
+                shape[2 - a] = -1
+                v = v * sg[a].reshape(shape)
+        out[c] = v
+    return out
+
+
+def kernel_spectrum(size, cs, mastergrid=(0, 0, 0), pbc=(0, 0, 0), K=None, octant=None):
+    """Spectrum (6, Kz, Ky, Kx//2+1) of the circularly placed kernel."""
     nx, ny, nz = size
     if K is None:
         K = (padded_size(nx), padded_size(ny), padded_size(nz))
     kx, ky, kz = K
-    koff = kernel_offsets(size, cs, mastergrid, pbc)
+    if octant is None:
+        octant = nx * ny * nz > 4096
+    koff = (kernel_offsets_octant if octant else kernel_offsets)(size, cs, mastergrid, pbc)
     full = np.zeros((6, kz, ky, kx))
     iz = np.arange(-(nz - 1), nz) % kz
     iy = np.arange(-(ny - 1), ny) % ky
     ix = np.arange(-(nx - 1), nx) % kx
     full[np.ix_(range(6), iz, iy, ix)] = koff
-    spec = np.fft.rfftn(full, axes=(1, 2, 3))
+    del koff
+    spec = _sfft.rfftn(full, axes=(1, 2, 3), workers=_WORKERS)
     return spec, K
 
 
@@ -74,13 +109,13 @@ class DemagPlan:
             M = np.where(geom[None], M, 0.0)
         Mp = np.zeros((3, kz, ky, kx))
         Mp[:, :nz, :ny, :nx] = M
-        Mf = np.fft.rfftn(Mp, axes=(1, 2, 3))
+        Mf = _sfft.rfftn(Mp, axes=(1, 2, 3), workers=_WORKERS)
         xx, yy, zz, xy, xz, yz = self.spec
         Hf = np.empty_like(Mf)
         Hf[0] = xx * Mf[0] + xy * Mf[1] + xz * Mf[2]
         Hf[1] = xy * Mf[0] + yy * Mf[1] + yz * Mf[2]
         Hf[2] = xz * Mf[0] + yz * Mf[1] + zz * Mf[2]
-        H = np.fft.irfftn(Hf, s=(kz, ky, kx), axes=(1, 2, 3))[:, :nz, :ny, :nx]
+        H = _sfft.irfftn(Hf, s=(kz, ky, kx), axes=(1, 2, 3), workers=_WORKERS)[:, :nz, :ny, :nx]
         H = -MU0 * H
         if geom is not None:
             H = np.where(geom[None], H, 0.0)

@@ -156,10 +156,11 @@ class World:
         self.magnets = []
         self.solver = _rk.TimeSolver(self)
 
-    def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None):
+    def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
         mag = Ferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
         self.magnets.append(mag)
-        self.solver.equations_changed()
+        if not defer_timestep:
+            self.solver.equations_changed()
         return mag
 
     def torque(self, mag, m):

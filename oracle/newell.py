@@ -131,6 +131,11 @@ def component(comp, ix, iy, iz, cs):
     F, (a0, a1, a2), c = _permute(comp, ix, iy, iz, cs)
     # stencil result is indexed [a2][a1][a0]; map back to [z][y][x]
     r = _stencil_sum(F, a0, a1, a2, c)
+    if F is f_xy:
+        # calcNewellNxy returns exactly 0 when the first or second (permuted)
+        # offset is 0 (newell.cu:99-100)
+        zero = (np.asarray(a1)[None, :, None] == 0) | (np.asarray(a0)[None, None, :] == 0)
+        r = np.where(zero, 0.0, r)
     order = {"xx": "zyx", "yy": "xzy", "zz": "yxz", "xy": "zyx", "xz": "yzx", "yz": "xzy"}[comp]
     # r's axes are named by `order` (outermost first); bring them to z, y, x
     perm = [order.index(a) for a in "zyx"]

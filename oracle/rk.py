@@ -94,11 +94,11 @@ class TimeSolver:
         if self.adaptive:
             self.dt = self.sensible_timestep()
 
-    def set_method(self, name):
+    def set_method(self, name, defer_timestep=False):
         if name not in TABLEAUS:
             raise ValueError(f"'{name}' is not a valid Runge Kutta method name")
         self.method = name
-        if self.adaptive:
+        if self.adaptive and not defer_timestep:
             self.dt = self.sensible_timestep()
 
     def sensible_timestep(self):
