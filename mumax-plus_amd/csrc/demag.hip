@@ -8,15 +8,17 @@
 //                                                   Ms*m and the zero pad fused in
 //   yfwd : A (y-lines)          -> B[c][kx][ky][z]  only Ny inputs per line are non-zero
 //   zpass: B (z-lines)          -> B                forward z, 3x3 real symmetric kernel
-//                                                   multiply, inverse z; keeps Nz outputs
+//                                                   multiply in registers, inverse z;
+//                                                   keeps Nz outputs
 //   yinv : B                    -> A                keeps Ny outputs
 //   xinv : A                    -> h (x-lines)      C2R of two lines at a time, crop and
 //                                                   geometry mask fused in
 //
-// For Kz == 1 the y pass does forward y, the kernel multiply and inverse y in
-// LDS (ypass2d) and B is not used.  Each pass reads or writes whole lines
-// contiguously on one side and 128-byte column segments on the other; the
-// transposition happens in LDS.
+// For Kz == 1 the y pass does forward y, the kernel multiply and inverse y
+// (ypass2d) and B is not used.  Every pass reads or writes whole lines
+// contiguously on one side; the transposing side goes through LDS.  The
+// line transforms are register-resident (fft.hpp): a 512-point line is 64
+// threads x 8 values with two LDS exchanges.
 //
 // Kernel spectrum: the Newell kernel on offsets -(N-1)..(N-1) placed
 // circularly is even (xx, yy, zz) or odd (xy: x,y; xz: x,z; yz: y,z) along
@@ -38,6 +40,8 @@ namespace {
 
 constexpr double MU0 = 1.25663706212e-6;  // src/physics/constants.hpp:22
 constexpr int BLOCK = 256;
+constexpr int MAXK = 2048;       // longest FFT line supported per axis
+constexpr int MAXTHREADS = 512;  // threads per block of the FFT passes
 
 int ilog2(int v) {
   int l = 0;
@@ -53,6 +57,9 @@ struct DD {
   int nlines;  // 3*ny*nz real x-lines
 };
 
+__host__ __device__ __forceinline__ int threads_per_line(int K) { return K >= 8 ? K >> 3 : 1; }
+__host__ __device__ __forceinline__ int elems_per_thread(int K) { return K >= 8 ? 8 : 1; }
+
 // ---------------------------------------------------------------- kernels
 
 __device__ __forceinline__ float load_M(const float* __restrict__ m, PS msat, const uint8_t* __restrict__ geom,
@@ -63,70 +70,81 @@ __device__ __forceinline__ float load_M(const float* __restrict__ m, PS msat, co
   return msat.at(cell) * m[(size_t)line * d.nx + x];
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_xfwd(DD d, const float* __restrict__ m, PS msat,
-                                               const uint8_t* __restrict__ geom, float2* __restrict__ A,
-                                               const float2* __restrict__ tw, int PP) {
+// x forward: PB line pairs per block, T = Kx/8 threads per pair.
+__global__ void __launch_bounds__(MAXTHREADS) k_xfwd(DD d, const float* __restrict__ m, PS msat,
+                                                     const uint8_t* __restrict__ geom, float2* __restrict__ A,
+                                                     const float2* __restrict__ tw, int PB) {
   extern __shared__ float2 lds[];
-  const int ls = d.Kx + 1;
-  const int p0 = blockIdx.x * PP;
-  for (int q = threadIdx.x; q < PP * d.Kx; q += blockDim.x) {
-    const int p = q >> d.l2x, x = q & (d.Kx - 1);
-    const int la = 2 * (p0 + p), lb = la + 1;
-    float2 v = make_float2(0.f, 0.f);
-    if (x < d.nx) {
-      if (la < d.nlines) v.x = load_M(m, msat, geom, d, la, x);
-      if (lb < d.nlines) v.y = load_M(m, msat, geom, d, lb, x);
+  const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
+  const int LP = padded_len(d.Kx);
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int la = 2 * (blockIdx.x * PB + g), lb = la + 1;
+  float2 v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = t + T * q;
+    float2 a = make_float2(0.f, 0.f);
+    if (q < E && x < d.nx) {
+      if (la < d.nlines) a.x = load_M(m, msat, geom, d, la, x);
+      if (lb < d.nlines) a.y = load_M(m, msat, geom, d, lb, x);
     }
-    lds[p * ls + x] = v;
+    v[0][q] = a;
   }
+  if (d.Kx >= 8) line_fft<false, 1>(v, t, d.l2x, lds + g * LP, 0, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
   __syncthreads();
-  lds_fft<E, false>(lds, d.l2x, PP, ls, tw);
-  const int nl = 2 * PP;
-  for (int q = threadIdx.x; q < d.Hx * nl; q += blockDim.x) {
-    const int kx = q / nl, li = q - kx * nl;
-    const int l = 2 * p0 + li;
+  const int nl = 2 * PB;
+  const int l0 = 2 * blockIdx.x * PB;
+  for (int e = threadIdx.x; e < d.Hx * nl; e += blockDim.x) {
+    const int kx = e / nl, li = e - kx * nl;
+    const int l = l0 + li;
     if (l >= d.nlines) continue;
     const int p = li >> 1;
-    const float2 z1 = lds[p * ls + kx];
-    const float2 z2 = lds[p * ls + ((d.Kx - kx) & (d.Kx - 1))];
+    const float2 z1 = lds[p * LP + lpad(kx)];
+    const float2 z2 = lds[p * LP + lpad((d.Kx - kx) & (d.Kx - 1))];
     float2 X;
-    if (!(li & 1)) {
-      X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
-    } else {
-      X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-    }
+    if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+    else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
     const int yz = l / d.ny;
     A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)] = X;
   }
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B,
-                                               const float2* __restrict__ tw, int PY) {
+// y forward (3-D): LB lines (c, kx, z), z fastest, per block.
+__global__ void __launch_bounds__(MAXTHREADS) k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B,
+                                                     const float2* __restrict__ tw, int LB) {
   extern __shared__ float2 lds[];
-  const int ls = d.Ky + 1;
+  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
+  const int LP = padded_len(d.Ky);
   const int nq = 3 * d.Hx * d.nz;
-  const int q0 = blockIdx.x * PY;
-  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
-    const int li = e >> d.l2y, y = e & (d.Ky - 1);
-    const int q = q0 + li;
-    float2 v = make_float2(0.f, 0.f);
-    if (q < nq && y < d.ny) {
-      const int ckx = q / d.nz, z = q - ckx * d.nz;
-      const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
-      v = A[((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny + y];
-    }
-    lds[li * ls + y] = v;
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int ql = blockIdx.x * LB + g;
+  float2 v[1][8];
+  const float2* src = nullptr;
+  if (ql < nq) {
+    const int ckx = ql / d.nz, z = ql - ckx * d.nz;
+    const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
+    src = A + ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
   }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int y = t + T * q;
+    v[0][q] = (src && q < E && y < d.ny) ? src[y] : make_float2(0.f, 0.f);
+  }
+  if (d.Ky >= 8) line_fft<false, 1>(v, t, d.l2y, lds + g * LP, 0, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
   __syncthreads();
-  lds_fft<E, false>(lds, d.l2y, PY, ls, tw);
-  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
-    const int ky = e / PY, li = e - ky * PY;
+  const int q0 = blockIdx.x * LB;
+  for (int e = threadIdx.x; e < d.Ky * LB; e += blockDim.x) {
+    const int ky = e / LB, li = e - ky * LB;
     const int q = q0 + li;
     if (q >= nq) continue;
     const int ckx = q / d.nz, z = q - ckx * d.nz;
-    B[((size_t)ckx * d.Ky + ky) * d.nz + z] = lds[li * ls + ky];
+    B[((size_t)ckx * d.Ky + ky) * d.nz + z] = lds[li * LP + lpad(ky)];
   }
 }
 
@@ -152,154 +170,173 @@ __device__ __forceinline__ void kernel_multiply(const float* __restrict__ T, con
   c = make_float2(nxz * ma.x + nyz * mb.x + nzz * mc.x, nxz * ma.y + nyz * mb.y + nzz * mc.y);
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ T,
-                                                const float2* __restrict__ tw, int PZ) {
+// z pass (3-D): PT spectral points (kx, ky) per block; each thread holds
+// the three components of its 8 z values, so the kernel multiply between
+// the forward and the inverse transform stays in registers.
+__global__ void __launch_bounds__(MAXTHREADS) k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk,
+                                                      const float2* __restrict__ tw, int PT) {
   extern __shared__ float2 lds[];
-  const int ls = d.Kz + 1;
+  const int T = threads_per_line(d.Kz);
+  const int LP = padded_len(d.Kz);
   const int npt = d.Hx * d.Ky;
-  const int pt0 = blockIdx.x * PZ;
-  const size_t cstride = (size_t)d.Hx * d.Ky;  // points per component
-  for (int e = threadIdx.x; e < 3 * PZ * d.Kz; e += blockDim.x) {
-    const int li = e >> d.l2z, z = e & (d.Kz - 1);
-    const int c = li / PZ, pp = li - c * PZ;
-    const int pt = pt0 + pp;
-    float2 v = make_float2(0.f, 0.f);
-    if (pt < npt && z < d.nz) v = B[((size_t)c * cstride + pt) * d.nz + z];
-    lds[li * ls + z] = v;
-  }
-  __syncthreads();
-  lds_fft<E, false>(lds, d.l2z, 3 * PZ, ls, tw);
-  for (int e = threadIdx.x; e < PZ * d.Kz; e += blockDim.x) {
-    const int pp = e >> d.l2z, kz = e & (d.Kz - 1);
-    const int pt = pt0 + pp;
-    if (pt >= npt) continue;
+  const size_t cs = (size_t)npt;  // points per component
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int pt = blockIdx.x * PT + g;
+  const bool valid = pt < npt;
+  float2 v[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int z = t + T * q;
+      v[c][q] = (valid && z < d.nz) ? B[((size_t)c * cs + pt) * d.nz + z] : make_float2(0.f, 0.f);
+    }
+  line_fft<false, 3>(v, t, d.l2z, lds + g * LP, PT * LP, tw);
+  if (valid) {
     const int kx = pt / d.Ky, ky = pt - kx * d.Ky;
-    float2& a = lds[(0 * PZ + pp) * ls + kz];
-    float2& b = lds[(1 * PZ + pp) * ls + kz];
-    float2& c = lds[(2 * PZ + pp) * ls + kz];
-    kernel_multiply(T, d, kx, ky, kz, a, b, c);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) kernel_multiply(Tk, d, kx, ky, t + T * q, v[0][q], v[1][q], v[2][q]);
   }
-  __syncthreads();
-  lds_fft<E, true>(lds, d.l2z, 3 * PZ, ls, tw);
-  for (int e = threadIdx.x; e < 3 * PZ * d.Kz; e += blockDim.x) {
-    const int li = e >> d.l2z, z = e & (d.Kz - 1);
-    const int c = li / PZ, pp = li - c * PZ;
-    const int pt = pt0 + pp;
-    if (pt < npt && z < d.nz) B[((size_t)c * cstride + pt) * d.nz + z] = lds[li * ls + z];
+  line_fft<true, 3>(v, t, d.l2z, lds + g * LP, PT * LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int z = t + T * q;
+        if (z < d.nz) B[((size_t)c * cs + pt) * d.nz + z] = v[c][q];
+      }
   }
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A,
-                                               const float2* __restrict__ tw, int PY) {
+// y inverse (3-D): B -> A, keeps y < Ny.
+__global__ void __launch_bounds__(MAXTHREADS) k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A,
+                                                     const float2* __restrict__ tw, int LB) {
   extern __shared__ float2 lds[];
-  const int ls = d.Ky + 1;
+  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
+  const int LP = padded_len(d.Ky);
   const int nq = 3 * d.Hx * d.nz;
-  const int q0 = blockIdx.x * PY;
-  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
-    const int ky = e / PY, li = e - ky * PY;
+  const int q0 = blockIdx.x * LB;
+  for (int e = threadIdx.x; e < d.Ky * LB; e += blockDim.x) {
+    const int ky = e / LB, li = e - ky * LB;
     const int q = q0 + li;
-    float2 v = make_float2(0.f, 0.f);
+    float2 val = make_float2(0.f, 0.f);
     if (q < nq) {
       const int ckx = q / d.nz, z = q - ckx * d.nz;
-      v = B[((size_t)ckx * d.Ky + ky) * d.nz + z];
+      val = B[((size_t)ckx * d.Ky + ky) * d.nz + z];
     }
-    lds[li * ls + ky] = v;
+    lds[li * LP + lpad(ky)] = val;
   }
   __syncthreads();
-  lds_fft<E, true>(lds, d.l2y, PY, ls, tw);
-  for (int e = threadIdx.x; e < PY * d.Ky; e += blockDim.x) {
-    const int li = e >> d.l2y, y = e & (d.Ky - 1);
-    const int q = q0 + li;
-    if (q >= nq || y >= d.ny) continue;
-    const int ckx = q / d.nz, z = q - ckx * d.nz;
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  float2 v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = q < E ? lds[g * LP + lpad(t + T * q)] : make_float2(0.f, 0.f);
+  __syncthreads();
+  if (d.Ky >= 8) line_fft<true, 1>(v, t, d.l2y, lds + g * LP, 0, tw);
+  const int ql = q0 + g;
+  if (ql < nq) {
+    const int ckx = ql / d.nz, z = ql - ckx * d.nz;
     const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
-    A[((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny + y] = lds[li * ls + y];
+    float2* dst = A + ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int y = t + T * q;
+      if (q < E && y < d.ny) dst[y] = v[0][q];
+    }
   }
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ T,
-                                                  const float2* __restrict__ tw, int PQ) {
+// single layer: forward y, kernel multiply, inverse y; PT points kx per block.
+__global__ void __launch_bounds__(MAXTHREADS) k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ Tk,
+                                                        const float2* __restrict__ tw, int PT) {
   extern __shared__ float2 lds[];
-  const int ls = d.Ky + 1;
-  const int kx0 = blockIdx.x * PQ;
-  for (int e = threadIdx.x; e < 3 * PQ * d.Ky; e += blockDim.x) {
-    const int li = e >> d.l2y, y = e & (d.Ky - 1);
-    const int c = li / PQ, pp = li - c * PQ;
-    const int kx = kx0 + pp;
-    float2 v = make_float2(0.f, 0.f);
-    if (kx < d.Hx && y < d.ny) v = A[((size_t)c * d.Hx + kx) * d.ny + y];
-    lds[li * ls + y] = v;
+  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
+  const int LP = padded_len(d.Ky);
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int kx = blockIdx.x * PT + g;
+  const bool valid = kx < d.Hx;
+  float2 v[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int y = t + T * q;
+      v[c][q] = (valid && q < E && y < d.ny) ? A[((size_t)c * d.Hx + kx) * d.ny + y] : make_float2(0.f, 0.f);
+    }
+  if (d.Ky >= 8) line_fft<false, 3>(v, t, d.l2y, lds + g * LP, PT * LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < E) kernel_multiply(Tk, d, kx, t + T * q, 0, v[0][q], v[1][q], v[2][q]);
   }
-  __syncthreads();
-  lds_fft<E, false>(lds, d.l2y, 3 * PQ, ls, tw);
-  for (int e = threadIdx.x; e < PQ * d.Ky; e += blockDim.x) {
-    const int pp = e >> d.l2y, ky = e & (d.Ky - 1);
-    const int kx = kx0 + pp;
-    if (kx >= d.Hx) continue;
-    float2& a = lds[(0 * PQ + pp) * ls + ky];
-    float2& b = lds[(1 * PQ + pp) * ls + ky];
-    float2& c = lds[(2 * PQ + pp) * ls + ky];
-    kernel_multiply(T, d, kx, ky, 0, a, b, c);
-  }
-  __syncthreads();
-  lds_fft<E, true>(lds, d.l2y, 3 * PQ, ls, tw);
-  for (int e = threadIdx.x; e < 3 * PQ * d.Ky; e += blockDim.x) {
-    const int li = e >> d.l2y, y = e & (d.Ky - 1);
-    const int c = li / PQ, pp = li - c * PQ;
-    const int kx = kx0 + pp;
-    if (kx < d.Hx && y < d.ny) A[((size_t)c * d.Hx + kx) * d.ny + y] = lds[li * ls + y];
+  if (d.Ky >= 8) line_fft<true, 3>(v, t, d.l2y, lds + g * LP, PT * LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int y = t + T * q;
+        if (q < E && y < d.ny) A[((size_t)c * d.Hx + kx) * d.ny + y] = v[c][q];
+      }
   }
 }
 
-template <int E>
-__global__ void __launch_bounds__(BLOCK) k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom,
-                                               float* __restrict__ h, const float2* __restrict__ tw, int PP) {
+// x inverse: PB line pairs per block; C2R of two lines at a time.
+__global__ void __launch_bounds__(MAXTHREADS) k_xinv(DD d, const float2* __restrict__ A,
+                                                     const uint8_t* __restrict__ geom, float* __restrict__ h,
+                                                     const float2* __restrict__ tw, int PB) {
   extern __shared__ float2 lds[];
-  const int xs = d.Hx + 1;
-  const int nl = 2 * PP;
+  const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
+  const int LP = padded_len(d.Kx);
+  const int XS = d.Hx + 1;
+  const int nl = 2 * PB;
   float2* X = lds;
-  float2* Z = lds + nl * xs;
-  const int zs = d.Kx + 1;
-  const int p0 = blockIdx.x * PP;
+  float2* Z = lds + nl * XS;
+  const int l0 = 2 * blockIdx.x * PB;
   for (int e = threadIdx.x; e < d.Hx * nl; e += blockDim.x) {
     const int kx = e / nl, li = e - kx * nl;
-    const int l = 2 * p0 + li;
-    float2 v = make_float2(0.f, 0.f);
+    const int l = l0 + li;
+    float2 val = make_float2(0.f, 0.f);
     if (l < d.nlines) {
       const int yz = l / d.ny;
-      v = A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)];
+      val = A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)];
     }
-    X[li * xs + kx] = v;
+    X[li * XS + kx] = val;
   }
   __syncthreads();
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
   const int half = d.Kx >> 1;
-  for (int e = threadIdx.x; e < PP * d.Kx; e += blockDim.x) {
-    const int p = e >> d.l2x, k = e & (d.Kx - 1);
-    const bool upper = k > half;
-    const int kk = upper ? d.Kx - k : k;
-    float2 xa = X[(2 * p) * xs + kk];
-    float2 xb = X[(2 * p + 1) * xs + kk];
-    if (kk == 0 || kk == half) { xa.y = 0.f; xb.y = 0.f; }
-    float2 zv;
-    if (!upper) zv = make_float2(xa.x - xb.y, xa.y + xb.x);
-    else zv = make_float2(xa.x + xb.y, xb.x - xa.y);
-    Z[p * zs + k] = zv;
+  float2 v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = t + T * q;
+    float2 zv = make_float2(0.f, 0.f);
+    if (q < E) {
+      const bool upper = k > half;
+      const int kk = upper ? d.Kx - k : k;
+      float2 xa = X[(2 * g) * XS + kk];
+      float2 xb = X[(2 * g + 1) * XS + kk];
+      if (kk == 0 || kk == half) { xa.y = 0.f; xb.y = 0.f; }
+      zv = upper ? make_float2(xa.x + xb.y, xb.x - xa.y) : make_float2(xa.x - xb.y, xa.y + xb.x);
+    }
+    v[0][q] = zv;
   }
-  __syncthreads();
-  lds_fft<E, true>(Z, d.l2x, PP, zs, tw);
+  if (d.Kx >= 8) line_fft<true, 1>(v, t, d.l2x, Z + g * LP, 0, tw);
+  const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
-  for (int e = threadIdx.x; e < nl * d.nx; e += blockDim.x) {
-    const int li = e / d.nx, x = e - li * d.nx;
-    const int l = 2 * p0 + li;
-    if (l >= d.nlines) continue;
-    const float2 zv = Z[(li >> 1) * zs + x];
-    float v = (li & 1) ? zv.y : zv.x;
-    const int cell = (l % nyz) * d.nx + x;
-    if (geom && !geom[cell]) v = 0.f;
-    h[(size_t)l * d.nx + x] = v;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = t + T * q;
+    if (q >= E || x >= d.nx) continue;
+    if (la < d.nlines) {
+      const int cell = (la % nyz) * d.nx + x;
+      h[(size_t)la * d.nx + x] = (geom && !geom[cell]) ? 0.f : v[0][q].x;
+    }
+    if (lb < d.nlines) {
+      const int cell = (lb % nyz) * d.nx + x;
+      h[(size_t)lb * d.nx + x] = (geom && !geom[cell]) ? 0.f : v[0][q].y;
+    }
   }
 }
 
@@ -403,23 +440,21 @@ void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
   HIP_CHECK(hipMemcpy(st.get(), s.data(), K * sizeof(double), hipMemcpyHostToDevice));
 }
 
-// elements one workgroup can transform at once: E complex values per thread
-constexpr int E_SMALL = 16;
-constexpr int E_LARGE = 32;
-constexpr int CAP_SMALL = E_SMALL * BLOCK;
-constexpr int CAP_LARGE = E_LARGE * BLOCK;
+constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup used by every pass
 
-template <template <int> class Launcher, typename... Args>
-void launch_e(bool large, Args&&... args) {
-  if (large) Launcher<E_LARGE>::run(args...);
-  else Launcher<E_SMALL>::run(args...);
+void check_launch_shape(size_t lds_bytes, int threads, const char* pass) {
+  if (lds_bytes > LDS_LIMIT) throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
+  if (threads < 1 || threads > MAXTHREADS)
+    throw Error(MPA_ERR_RUNTIME, std::string("demag: bad block size in ") + pass);
 }
+
+int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 }  // namespace
 
 int demag_fft_length(int n) {
   if (n <= 1) return 1;
-  int k = 1;
+  int k = 8;  // the line transforms need 1 or >= 8 points
   while (k < 2 * n - 1) k <<= 1;
   return k;
 }
@@ -429,15 +464,10 @@ DemagPlan::DemagPlan(const DemagGeometry& g) : g_(g) {
     if (g.n[a] < 1) throw Error(MPA_ERR_INVALID, "demag: grid size must be positive");
     K_[a] = demag_fft_length(g.n[a]);
     log2K_[a] = ilog2(K_[a]);
-    if (K_[a] > CAP_SMALL) throw Error(MPA_ERR_INVALID, "demag: grid axis too large for the FFT passes (max 2048 cells)");
+    if (K_[a] > MAXK) throw Error(MPA_ERR_INVALID, "demag: grid axis too large for the FFT passes (max 1024 cells)");
   }
-  if (K_[2] > 1 && 3 * K_[2] > CAP_LARGE)
-    throw Error(MPA_ERR_INVALID, "demag: z axis too large for the fused z pass");
-  if (K_[2] == 1 && 3 * K_[1] > CAP_LARGE)
-    throw Error(MPA_ERR_INVALID, "demag: y axis too large for the fused 2-D pass");
   N_ = g.n[0] * g.n[1] * g.n[2];
-  Hx_ = K_[0] / 2 + 1;
-  if (K_[0] == 1) Hx_ = 1;
+  Hx_ = K_[0] == 1 ? 1 : K_[0] / 2 + 1;
   Hy_ = K_[1] / 2 + 1;
   Hz_ = K_[2] / 2 + 1;
   A_.allocate((size_t)3 * g.n[2] * Hx_ * g.n[1]);
@@ -479,83 +509,6 @@ void DemagPlan::build_kernel(hipStream_t s) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
-namespace {
-constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup used by every pass
-
-void check_lds(size_t bytes, const char* pass) {
-  if (bytes > LDS_LIMIT)
-    throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
-}
-
-size_t xinv_lds(int PP, int Hx, int Kx) {
-  return ((size_t)2 * PP * (Hx + 1) + (size_t)PP * (Kx + 1)) * sizeof(float2);
-}
-
-template <int E>
-struct LXfwd {
-  static void run(DD d, const float* m, PS ms, const uint8_t* geom, float2* A, const float2* tw, int PP, hipStream_t s) {
-    const int npairs = (d.nlines + 1) / 2;
-    const size_t shm = (size_t)PP * (d.Kx + 1) * sizeof(float2);
-    check_lds(shm, "k_xfwd");
-    k_xfwd<E><<<grid_for(npairs, PP), BLOCK, shm, s>>>(d, m, ms, geom, A, tw, PP);
-    checkLaunch("k_xfwd");
-  }
-};
-template <int E>
-struct LYfwd {
-  static void run(DD d, const float2* A, float2* B, const float2* tw, int PY, hipStream_t s) {
-    const size_t shm = (size_t)PY * (d.Ky + 1) * sizeof(float2);
-    check_lds(shm, "k_yfwd");
-    k_yfwd<E><<<grid_for(3LL * d.Hx * d.nz, PY), BLOCK, shm, s>>>(d, A, B, tw, PY);
-    checkLaunch("k_yfwd");
-  }
-};
-template <int E>
-struct LZpass {
-  static void run(DD d, float2* B, const float* T, const float2* tw, int PZ, hipStream_t s) {
-    const size_t shm = (size_t)3 * PZ * (d.Kz + 1) * sizeof(float2);
-    check_lds(shm, "k_zpass");
-    k_zpass<E><<<grid_for((long long)d.Hx * d.Ky, PZ), BLOCK, shm, s>>>(d, B, T, tw, PZ);
-    checkLaunch("k_zpass");
-  }
-};
-template <int E>
-struct LYinv {
-  static void run(DD d, const float2* B, float2* A, const float2* tw, int PY, hipStream_t s) {
-    const size_t shm = (size_t)PY * (d.Ky + 1) * sizeof(float2);
-    check_lds(shm, "k_yinv");
-    k_yinv<E><<<grid_for(3LL * d.Hx * d.nz, PY), BLOCK, shm, s>>>(d, B, A, tw, PY);
-    checkLaunch("k_yinv");
-  }
-};
-template <int E>
-struct LY2d {
-  static void run(DD d, float2* A, const float* T, const float2* tw, int PQ, hipStream_t s) {
-    const size_t shm = (size_t)3 * PQ * (d.Ky + 1) * sizeof(float2);
-    check_lds(shm, "k_ypass2d");
-    k_ypass2d<E><<<grid_for(d.Hx, PQ), BLOCK, shm, s>>>(d, A, T, tw, PQ);
-    checkLaunch("k_ypass2d");
-  }
-};
-template <int E>
-struct LXinv {
-  static void run(DD d, const float2* A, const uint8_t* geom, float* h, const float2* tw, int PP, hipStream_t s) {
-    const int npairs = (d.nlines + 1) / 2;
-    const size_t shm = ((size_t)2 * PP * (d.Hx + 1) + (size_t)PP * (d.Kx + 1)) * sizeof(float2);
-    check_lds(shm, "k_xinv");
-    k_xinv<E><<<grid_for(npairs, PP), BLOCK, shm, s>>>(d, A, geom, h, tw, PP);
-    checkLaunch("k_xinv");
-  }
-};
-
-// lines per workgroup so that lines*L <= cap, at most `maxl`
-int lines_for(int L, int cap, int maxl) {
-  int p = cap / L;
-  if (p > maxl) p = maxl;
-  return p < 1 ? 1 : p;
-}
-}  // namespace
-
 void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof) {
   DD d;
   d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
@@ -563,43 +516,56 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
   d.Hx = Hx_; d.Hy = Hy_; d.Hz = Hz_;
   d.l2x = log2K_[0]; d.l2y = log2K_[1]; d.l2z = log2K_[2];
   d.nlines = 3 * d.ny * d.nz;
-
-  // x passes: PP line pairs per workgroup
-  const bool xl = d.Kx > CAP_SMALL;
-  const int PPf = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 16);
+  const int Tx = threads_per_line(d.Kx), Ty = threads_per_line(d.Ky), Tz = threads_per_line(d.Kz);
+  const int LPx = padded_len(d.Kx), LPy = padded_len(d.Ky), LPz = padded_len(d.Kz);
+  const int npairs = (d.nlines + 1) / 2;
   {
+    const int PB = clampi(MAXTHREADS / Tx, 1, 32);
+    const size_t shm = (size_t)PB * LPx * sizeof(float2);
+    check_launch_shape(shm, PB * Tx, "k_xfwd");
     Timed t(prof, "k_xfwd", pass_bytes(0), s);
-    launch_e<LXfwd>(xl, d, m, msat, geom, A_.get(), tw_[0].get(), PPf, s);
+    k_xfwd<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, m, msat, geom, A_.get(), tw_[0].get(), PB);
+    checkLaunch("k_xfwd");
   }
   if (is2d()) {
-    const bool yl = 3 * d.Ky > CAP_SMALL;
-    const int PQ = lines_for(3 * d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
+    const int PT = clampi(BLOCK / Ty, 1, BLOCK);
+    const size_t shm = (size_t)3 * PT * LPy * sizeof(float2);
+    check_launch_shape(shm, PT * Ty, "k_ypass2d");
     Timed t(prof, "k_ypass2d", pass_bytes(1), s);
-    launch_e<LY2d>(yl, d, A_.get(), (const float*)T_.get(), tw_[1].get(), PQ, s);
+    k_ypass2d<<<grid_for(d.Hx, PT), PT * Ty, shm, s>>>(d, A_.get(), T_.get(), tw_[1].get(), PT);
+    checkLaunch("k_ypass2d");
   } else {
-    const bool yl = d.Ky > CAP_SMALL;
-    const int PY = lines_for(d.Ky, yl ? CAP_LARGE : CAP_SMALL, 16);
+    const int LB = clampi(MAXTHREADS / Ty, 1, 32);
+    const size_t shmy = (size_t)LB * LPy * sizeof(float2);
+    check_launch_shape(shmy, LB * Ty, "k_yfwd");
+    const long long nq = 3LL * d.Hx * d.nz;
     {
       Timed t(prof, "k_yfwd", pass_bytes(1), s);
-      launch_e<LYfwd>(yl, d, (const float2*)A_.get(), B_.get(), tw_[1].get(), PY, s);
+      k_yfwd<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, A_.get(), B_.get(), tw_[1].get(), LB);
+      checkLaunch("k_yfwd");
     }
-    const bool zl = 3 * d.Kz > CAP_SMALL;
-    const int PZ = lines_for(3 * d.Kz, zl ? CAP_LARGE : CAP_SMALL, 16);
     {
+      const int PT = clampi(BLOCK / Tz, 1, BLOCK);
+      const size_t shm = (size_t)3 * PT * LPz * sizeof(float2);
+      check_launch_shape(shm, PT * Tz, "k_zpass");
       Timed t(prof, "k_zpass", pass_bytes(2), s);
-      launch_e<LZpass>(zl, d, B_.get(), (const float*)T_.get(), tw_[2].get(), PZ, s);
+      k_zpass<<<grid_for((long long)d.Hx * d.Ky, PT), PT * Tz, shm, s>>>(d, B_.get(), T_.get(), tw_[2].get(), PT);
+      checkLaunch("k_zpass");
     }
     {
       Timed t(prof, "k_yinv", pass_bytes(3), s);
-      launch_e<LYinv>(yl, d, (const float2*)B_.get(), A_.get(), tw_[1].get(), PY, s);
+      k_yinv<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, B_.get(), A_.get(), tw_[1].get(), LB);
+      checkLaunch("k_yinv");
     }
   }
-  // the inverse x pass also stages the half spectrum: fit both in the LDS budget
-  int PPi = lines_for(d.Kx, xl ? CAP_LARGE : CAP_SMALL, 8);
-  while (PPi > 1 && xinv_lds(PPi, d.Hx, d.Kx) > LDS_LIMIT) --PPi;
-  check_lds(xinv_lds(PPi, d.Hx, d.Kx), "k_xinv");
-  Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
-  launch_e<LXinv>(xl, d, (const float2*)A_.get(), geom, h, tw_[0].get(), PPi, s);
+  {
+    const int PB = clampi(BLOCK / Tx, 1, 32);
+    const size_t shm = ((size_t)2 * PB * (d.Hx + 1) + (size_t)PB * LPx) * sizeof(float2);
+    check_launch_shape(shm, PB * Tx, "k_xinv");
+    Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
+    k_xinv<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, A_.get(), geom, h, tw_[0].get(), PB);
+    checkLaunch("k_xinv");
+  }
 }
 
 const char* DemagPlan::pass_name(int i) const {
@@ -611,8 +577,8 @@ const char* DemagPlan::pass_name(int i) const {
 double DemagPlan::pass_bytes(int i) const {
   const double N = (double)g_.n[0] * g_.n[1] * g_.n[2];
   const double ny = g_.n[1], nz = g_.n[2];
-  const double xa = 24.0 * Hx_ * ny * nz;   // one pass over A (complex 3-vectors)
-  const double yb = 24.0 * Hx_ * K_[1] * nz; // one pass over B
+  const double xa = 24.0 * Hx_ * ny * nz;    // one pass over A (complex 3-vectors)
+  const double yb = 24.0 * Hx_ * K_[1] * nz;  // one pass over B
   if (is2d()) {
     const double t = 16.0 * Hx_ * Hy_;  // xx, yy, zz, xy (xz = yz = 0 in a single layer)
     const double v[3] = {12 * N + xa, 2 * xa + t, xa + 12 * N};
@@ -624,11 +590,9 @@ double DemagPlan::pass_bytes(int i) const {
 }
 
 double DemagPlan::algorithmic_bytes() const {
-  const double nx = g_.n[0], ny = g_.n[1], nz = g_.n[2];
-  const double N = nx * ny * nz;
-  if (is2d()) return 12 * N + 96.0 * Hx_ * ny + 16.0 * Hx_ * Hy_ + 12 * N;
-  return 12 * N + 48.0 * Hx_ * ny * nz + 48.0 * Hx_ * K_[1] * nz + 24.0 * Hx_ * Hy_ * Hz_ + 48.0 * Hx_ * K_[1] * nz +
-         48.0 * Hx_ * ny * nz + 12 * N;
+  double b = 0.0;
+  for (int i = 0; i < npasses(); ++i) b += pass_bytes(i);
+  return b;
 }
 
 size_t DemagPlan::device_bytes() const { return A_.bytes() + B_.bytes() + T_.bytes(); }

@@ -15,6 +15,8 @@
 // branch by branch.
 #include "fields.hpp"
 
+#include <cmath>
+
 namespace mpa {
 
 namespace {
@@ -213,9 +215,185 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   out[2 * N + idx] = r.z;
 }
 
+// ---------------------------------------------------------------------------
+// Uniform-parameter path.  When every material parameter is one value for
+// the whole magnet, the per-neighbour harmonic means reduce to that value
+// and all divisions fold into host-side coefficients:
+//   exchange   2A/(Ms c^2) (m_n - m);  Neumann boundary: s Gamma/(Ms c)
+//   DMI        s (D/(Ms c)) . m_n      with m_n = m + Gamma s c/(2A) at a
+//                                      Neumann boundary (s = +-1 the side)
+//   anisotropy (2K1/Ms mu + 4K2/Ms mu^3) u,  cubic with -2Kc1/Ms, ...
+// The branch structure (open boundaries, geometry holes, msat = 0 never
+// occurs since msat is uniform and non-zero here) is the same as above.
+struct FMUniform {
+  float kex[3];      // 2A/(Ms c^2) per axis
+  float inv_msc[3];  // 1/(Ms c)
+  float dxz[3], dxy[3], dyz[3];  // D_(axis,xz|xy|yz)/(Ms c)
+  float D[D_COUNT];  // raw tensor for the boundary term Gamma
+  float ncoef[3];    // c/(2A) (Neumann displacement), 0 when A == 0
+  float u[3], ku1, ku2;
+  float c1[3], c2[3], c3[3], kc1, kc2, kc3;
+  float hext[3];
+  float llg, alpha;  // -gamma/(1+alpha^2)
+  int has_ex, has_dmi, uni, cubic, demag, frozen, openbc_dmi, openbc;
+};
+
+__device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m) { return dmi_gamma(u.D, axis, m); }
+
+__global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
+                                                         const float* __restrict__ hdemag, float* __restrict__ out,
+                                                         int mode) {
+  // 2-D blocks: threadIdx.x along x, threadIdx.y along y; blockIdx.z = z
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= d.nx || y >= d.ny) return;
+  const int N = d.ncell;
+  const int idx = (z * d.ny + y) * d.nx + x;
+  if (d.geom && !d.geom[idx]) {
+    out[idx] = 0.f;
+    out[N + idx] = 0.f;
+    out[2 * N + idx] = 0.f;
+    return;
+  }
+  const float3 mc = load3(m, idx, N);
+  float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
+  if (u.has_ex || u.has_dmi) {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      const int axis = dir >> 1;
+      const float s = (dir & 1) ? 1.f : -1.f;
+      const int nb = neighbour(d, x, y, z, dir);
+      const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+      if (nb_in) {
+        const float3 mn = load3(m, nb, N);
+        if (u.has_ex) hex = hex + u.kex[axis] * (mn - mc);
+        if (u.has_dmi)
+          hdm = hdm + s * f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
+                             -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
+      } else {
+        const float3 gam = gamma_u(u, axis, mc);
+        if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
+        if (u.has_dmi && !u.openbc_dmi) {
+          const float3 mn = mc + (s * u.ncoef[axis]) * gam;
+          hdm = hdm + s * f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
+                             -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
+        }
+      }
+    }
+  }
+  float3 han = f3(0.f, 0.f, 0.f);
+  if (u.uni) {
+    const float3 ua = f3(u.u[0], u.u[1], u.u[2]);
+    const float mu = dot3(mc, ua);
+    han = (u.ku1 * mu + u.ku2 * mu * mu * mu) * ua;
+  } else if (u.cubic) {
+    const float3 c1 = f3(u.c1[0], u.c1[1], u.c1[2]), c2 = f3(u.c2[0], u.c2[1], u.c2[2]),
+                 c3 = f3(u.c3[0], u.c3[1], u.c3[2]);
+    const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
+    const float s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
+    han = u.kc1 * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
+          u.kc2 * ((s2 * s3) * a1 * c1 + (s1 * s3) * a2 * c2 + (s1 * s2) * a3 * c3) +
+          u.kc3 * ((s2 * s2 + s3 * s3) * (s1 * a1) * c1 + (s1 * s1 + s3 * s3) * (s2 * a2) * c2 +
+                   (s1 * s1 + s2 * s2) * (s3 * a3) * c3);
+  }
+  float3 h = hex;
+  h = h + han;
+  h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
+  h = h + hdm;
+  if (u.demag) h = h + load3(hdemag, idx, N);
+  float3 r;
+  if (mode == MODE_FIELD) {
+    r = h;
+  } else if (u.frozen) {
+    r = f3(0.f, 0.f, 0.f);
+  } else if (mode == MODE_LLG) {
+    const float3 mxh = cross3(mc, h);
+    r = u.llg * (mxh + u.alpha * cross3(mc, mxh));
+  } else {
+    r = -GAMMALL * cross3(mc, cross3(mc, h));
+  }
+  out[idx] = r.x;
+  out[N + idx] = r.y;
+  out[2 * N + idx] = r.z;
+}
+
+bool all_uniform(const FMDesc& d) {
+  bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
+           !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p;
+  for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p;
+  return u;
+}
+
+// Host-side coefficients of the uniform path, in double then rounded once.
+FMUniform make_uniform(const FMDesc& d) {
+  FMUniform u{};
+  const double ms = d.msat.v, A = d.aex.v;
+  const double c[3] = {d.cx, d.cy, d.cz};
+  const bool msok = ms != 0.0;
+  u.has_ex = (d.terms & TERM_EXCH) && msok;
+  u.has_dmi = (d.terms & TERM_DMI) && msok;
+  u.openbc = d.openbc != 0;
+  u.openbc_dmi = d.openbc != 0 || A == 0.0;
+  const int kxz[3] = {D_XXZ, D_YXZ, D_ZXZ}, kxy[3] = {D_XXY, D_YXY, D_ZXY}, kyz[3] = {D_XYZ, D_YYZ, D_ZYZ};
+  for (int k = 0; k < D_COUNT; ++k) u.D[k] = d.dmi[k].v;
+  for (int a = 0; a < 3; ++a) {
+    u.kex[a] = msok ? (float)(2.0 * A / (ms * c[a] * c[a])) : 0.f;
+    u.inv_msc[a] = msok ? (float)(1.0 / (ms * c[a])) : 0.f;
+    u.dxz[a] = msok ? (float)(d.dmi[kxz[a]].v / (ms * c[a])) : 0.f;
+    u.dxy[a] = msok ? (float)(d.dmi[kxy[a]].v / (ms * c[a])) : 0.f;
+    u.dyz[a] = msok ? (float)(d.dmi[kyz[a]].v / (ms * c[a])) : 0.f;
+    u.ncoef[a] = A != 0.0 ? (float)(c[a] / (2.0 * A)) : 0.f;
+  }
+  u.uni = (d.terms & TERM_UNI) && msok;
+  u.cubic = (d.terms & TERM_CUBIC) && msok;
+  if (u.uni) {
+    const double ux = d.anisU.x, uy = d.anisU.y, uz = d.anisU.z, n = std::sqrt(ux * ux + uy * uy + uz * uz);
+    u.u[0] = (float)(ux / n); u.u[1] = (float)(uy / n); u.u[2] = (float)(uz / n);
+    u.ku1 = (float)(2.0 * d.ku1.v / ms);
+    u.ku2 = (float)(4.0 * d.ku2.v / ms);
+  }
+  if (u.cubic) {
+    double a1[3] = {d.anisC1.x, d.anisC1.y, d.anisC1.z}, a2[3] = {d.anisC2.x, d.anisC2.y, d.anisC2.z};
+    const double n1 = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+    const double n2 = std::sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+    for (int i = 0; i < 3; ++i) { a1[i] /= n1; a2[i] /= n2; }
+    // c3 = c1 x c2 of the FP32-normalised axes, as in the per-cell kernel
+    float f1[3], f2[3];
+    for (int i = 0; i < 3; ++i) { f1[i] = (float)a1[i]; f2[i] = (float)a2[i]; }
+    const float3 c3 = cross3(f3(f1[0], f1[1], f1[2]), f3(f2[0], f2[1], f2[2]));
+    for (int i = 0; i < 3; ++i) { u.c1[i] = f1[i]; u.c2[i] = f2[i]; }
+    u.c3[0] = c3.x; u.c3[1] = c3.y; u.c3[2] = c3.z;
+    u.kc1 = (float)(-2.0 * d.kc1.v / ms);
+    u.kc2 = (float)(-2.0 * d.kc2.v / ms);
+    u.kc3 = (float)(-4.0 * d.kc3.v / ms);
+  }
+  if (d.terms & TERM_EXT) {
+    u.hext[0] = (float)((double)d.wb[0] + d.bias.x);
+    u.hext[1] = (float)((double)d.wb[1] + d.bias.y);
+    u.hext[2] = (float)((double)d.wb[2] + d.bias.z);
+  }
+  u.demag = (d.terms & TERM_DEMAG) != 0;
+  const double al = d.alpha.v;
+  u.alpha = (float)al;
+  u.llg = (float)(-1.7595e11 / (1.0 + al * al));
+  u.frozen = d.frozen.v != 0.f;
+  return u;
+}
+
 }  // namespace
 
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {
+  if (mode != MODE_EDENS && all_uniform(d) && d.nz <= 65535) {
+    const FMUniform u = make_uniform(d);
+    int bx = 1;
+    while (bx < d.nx && bx < BLOCK) bx <<= 1;
+    const int by = BLOCK / bx;
+    dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, d.nz);
+    k_field_uniform<<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode);
+    checkLaunch("k_field_uniform");
+    return;
+  }
   const int nblk = (d.ncell + BLOCK - 1) / BLOCK;
   k_field<<<nblk, BLOCK, 0, s>>>(d, m, hdemag, out, (int)mode);
   checkLaunch("k_field");

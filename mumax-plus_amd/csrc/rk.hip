@@ -40,6 +40,54 @@ __global__ void __launch_bounds__(BLOCK) k_combine(float* __restrict__ out, cons
   for (int c = 0; c < ncomp; ++c) out[(size_t)c * N + idx] = v[c];
 }
 
+// Same as k_combine for 3-vectors, four consecutive cells per thread with
+// 16-byte loads and stores (N divisible by 4).
+__global__ void __launch_bounds__(BLOCK) k_combine4(float* __restrict__ out, const float* __restrict__ x0, Combo co,
+                                                    int N, const uint8_t* __restrict__ geom, int normalize) {
+  const int i4 = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  const int N4 = N >> 2;
+  if (i4 >= N4) return;
+  float4 v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float4 a = reinterpret_cast<const float4*>(x0 + (size_t)c * N)[i4];
+    for (int j = 0; j < co.nk; ++j) {
+      const float4 kv = reinterpret_cast<const float4*>(co.k[j] + (size_t)c * N)[i4];
+      const float cj = co.c[j];
+      a.x += cj * kv.x;
+      a.y += cj * kv.y;
+      a.z += cj * kv.z;
+      a.w += cj * kv.w;
+    }
+    v[c] = a;
+  }
+  if (normalize) {
+    const float i0 = rsqrtf(v[0].x * v[0].x + v[1].x * v[1].x + v[2].x * v[2].x);
+    const float i1 = rsqrtf(v[0].y * v[0].y + v[1].y * v[1].y + v[2].y * v[2].y);
+    const float i2 = rsqrtf(v[0].z * v[0].z + v[1].z * v[1].z + v[2].z * v[2].z);
+    const float i3 = rsqrtf(v[0].w * v[0].w + v[1].w * v[1].w + v[2].w * v[2].w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c].x *= i0;
+      v[c].y *= i1;
+      v[c].z *= i2;
+      v[c].w *= i3;
+    }
+  }
+  if (geom) {
+    const uchar4 g = reinterpret_cast<const uchar4*>(geom)[i4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!g.x) v[c].x = 0.f;
+      if (!g.y) v[c].y = 0.f;
+      if (!g.z) v[c].z = 0.f;
+      if (!g.w) v[c].w = 0.f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) reinterpret_cast<float4*>(out + (size_t)c * N)[i4] = v[c];
+}
+
 __device__ __forceinline__ float block_max(float v) {
   __shared__ float sh[BLOCK / 64];
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
@@ -117,6 +165,12 @@ int reduce_blocks(int N) {
 
 void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int N, const uint8_t* geom,
                     bool normalize, hipStream_t s) {
+  if (ncomp == 3 && (N & 3) == 0) {
+    const int nblk = (N / 4 + BLOCK - 1) / BLOCK;
+    k_combine4<<<nblk, BLOCK, 0, s>>>(out, x0, co, N, geom, normalize ? 1 : 0);
+    checkLaunch("k_combine4");
+    return;
+  }
   const int nblk = (N + BLOCK - 1) / BLOCK;
   k_combine<<<nblk, BLOCK, 0, s>>>(out, x0, co, ncomp, N, geom, normalize ? 1 : 0);
   checkLaunch("k_combine");

@@ -64,6 +64,36 @@ def test_local_terms(term, dmi, openbc):
     assert rel_max_err(got, want) < TOL
 
 
+@pytest.mark.parametrize("geometry", [False, True])
+@pytest.mark.parametrize("openbc", [False, True])
+@pytest.mark.parametrize("dmi", ["interfacial", "bulk"])
+def test_uniform_parameter_path(dmi, openbc, geometry):
+    """All parameters uniform: the coefficient-folded stencil path."""
+    rng = np.random.default_rng(zlib.crc32(f"u-{dmi}-{openbc}-{geometry}".encode()))
+    world, mag, ow, om = _setup(rng, openbc=openbc, dmi=dmi, nonuniform=False, geometry=geometry)
+    heff = of.effective_field(om, om.m, ow.bias)
+    assert rel_max_err(mag.effective_field.eval(), heff) < TOL
+    assert rel_max_err(mag.torque.eval(), of.llg_torque(om, om.m, heff)) < TOL
+    for term in ("exchange_field", "dmi_field", "anisotropy_field"):
+        assert rel_max_err(getattr(mag, term).eval(), TERMS[term](om, om.m)) < TOL
+
+
+def test_uniform_cubic_and_periodic():
+    rng = np.random.default_rng(17)
+    size = (12, 10, 3)
+    world, mag, ow, om = _setup(rng, geometry=False, nonuniform=False, size=size, mg=(12, 0, 3), pbc=(1, 0, 2))
+    mag.ku1 = 0.0
+    mag.ku2 = 0.0
+    om.set("ku1", 0.0)
+    om.set("ku2", 0.0)
+    for k, v in {"kc1": 3e5, "kc2": -1e5, "kc3": 2e4, "anisC1": (1.0, 0.1, 0.0), "anisC2": (-0.1, 1.0, 0.2)}.items():
+        setattr(mag, k, v)
+        om.set(k, v)
+    heff = of.effective_field(om, om.m, ow.bias)
+    assert rel_max_err(mag.effective_field.eval(), heff) < TOL
+    assert rel_max_err(mag.torque.eval(), of.llg_torque(om, om.m, heff)) < TOL
+
+
 def test_periodic_exchange_dmi():
     rng = np.random.default_rng(5)
     size = (12, 10, 2)
