@@ -70,23 +70,38 @@ __device__ __forceinline__ float load_M(const float* __restrict__ m, PS msat, co
   return msat.at(cell) * m[(size_t)line * d.nx + x];
 }
 
-// x forward: PB line pairs per block, T = Kx/8 threads per pair.
+// Offset of x-line l (c, z, y) in A[c][z][kx][y] at kx = 0; -1 past the end.
+__device__ __forceinline__ long long a_line_base(const DD& d, int l) {
+  if (l >= d.nlines) return -1;
+  const int yz = l / d.ny;
+  return (long long)yz * d.Hx * d.ny + (l - yz * d.ny);
+}
+
+// x forward: PB line pairs per block (PB a power of two), T = Kx/8 threads
+// per pair.
 __global__ void __launch_bounds__(MAXTHREADS) k_xfwd(DD d, const float* __restrict__ m, PS msat,
                                                      const uint8_t* __restrict__ geom, float2* __restrict__ A,
-                                                     const float2* __restrict__ tw, int PB) {
+                                                     const float2* __restrict__ tw, int l2PB) {
   extern __shared__ float2 lds[];
+  __shared__ long long abase[64];
+  const int PB = 1 << l2PB;
   const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
   const int LP = padded_len(d.Kx);
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int la = 2 * (blockIdx.x * PB + g), lb = la + 1;
+  const int l0 = 2 * blockIdx.x * PB;
+  const int la = l0 + 2 * g, lb = la + 1;
+  // blocks can have fewer threads than lines (one thread per line when Kx < 8)
+  for (int i = threadIdx.x; i < 2 * PB; i += blockDim.x) abase[i] = a_line_base(d, l0 + i);
+  const int nyz = d.ny * d.nz;
+  const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
   float2 v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int x = t + T * q;
     float2 a = make_float2(0.f, 0.f);
     if (q < E && x < d.nx) {
-      if (la < d.nlines) a.x = load_M(m, msat, geom, d, la, x);
-      if (lb < d.nlines) a.y = load_M(m, msat, geom, d, lb, x);
+      if (la < d.nlines && (!geom || geom[cella + x])) a.x = msat.at(cella + x) * m[(size_t)la * d.nx + x];
+      if (lb < d.nlines && (!geom || geom[cellb + x])) a.y = msat.at(cellb + x) * m[(size_t)lb * d.nx + x];
     }
     v[0][q] = a;
   }
@@ -95,32 +110,41 @@ __global__ void __launch_bounds__(MAXTHREADS) k_xfwd(DD d, const float* __restri
   for (int q = 0; q < 8; ++q)
     if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
   __syncthreads();
-  const int nl = 2 * PB;
-  const int l0 = 2 * blockIdx.x * PB;
-  for (int e = threadIdx.x; e < d.Hx * nl; e += blockDim.x) {
-    const int kx = e / nl, li = e - kx * nl;
-    const int l = l0 + li;
-    if (l >= d.nlines) continue;
+  const int l2nl = l2PB + 1;
+  for (int e = threadIdx.x; e < (d.Hx << l2nl); e += blockDim.x) {
+    const int kx = e >> l2nl, li = e & ((1 << l2nl) - 1);
+    const long long base = abase[li];
+    if (base < 0) continue;
     const int p = li >> 1;
     const float2 z1 = lds[p * LP + lpad(kx)];
     const float2 z2 = lds[p * LP + lpad((d.Kx - kx) & (d.Kx - 1))];
     float2 X;
     if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
     else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-    const int yz = l / d.ny;
-    A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)] = X;
+    A[base + (long long)kx * d.ny] = X;
   }
 }
 
-// y forward (3-D): LB lines (c, kx, z), z fastest, per block.
+// Offset of y-line q = (c, kx, z) in B[c][kx][ky][z] at ky = 0; -1 past the end.
+__device__ __forceinline__ long long b_line_base(const DD& d, int q) {
+  if (q >= 3 * d.Hx * d.nz) return -1;
+  const int ckx = q / d.nz, z = q - ckx * d.nz;
+  return (long long)ckx * d.Ky * d.nz + z;
+}
+
+// y forward (3-D): LB lines (c, kx, z), z fastest, per block (LB a power of two).
 __global__ void __launch_bounds__(MAXTHREADS) k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B,
-                                                     const float2* __restrict__ tw, int LB) {
+                                                     const float2* __restrict__ tw, int l2LB) {
   extern __shared__ float2 lds[];
+  __shared__ long long bbase[64];
+  const int LB = 1 << l2LB;
   const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
   const int LP = padded_len(d.Ky);
   const int nq = 3 * d.Hx * d.nz;
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int ql = blockIdx.x * LB + g;
+  const int q0 = blockIdx.x * LB;
+  for (int i = threadIdx.x; i < LB; i += blockDim.x) bbase[i] = b_line_base(d, q0 + i);
+  const int ql = q0 + g;
   float2 v[1][8];
   const float2* src = nullptr;
   if (ql < nq) {
@@ -138,13 +162,11 @@ __global__ void __launch_bounds__(MAXTHREADS) k_yfwd(DD d, const float2* __restr
   for (int q = 0; q < 8; ++q)
     if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
   __syncthreads();
-  const int q0 = blockIdx.x * LB;
-  for (int e = threadIdx.x; e < d.Ky * LB; e += blockDim.x) {
-    const int ky = e / LB, li = e - ky * LB;
-    const int q = q0 + li;
-    if (q >= nq) continue;
-    const int ckx = q / d.nz, z = q - ckx * d.nz;
-    B[((size_t)ckx * d.Ky + ky) * d.nz + z] = lds[li * LP + lpad(ky)];
+  for (int e = threadIdx.x; e < (d.Ky << l2LB); e += blockDim.x) {
+    const int ky = e >> l2LB, li = e & (LB - 1);
+    const long long base = bbase[li];
+    if (base < 0) continue;
+    B[base + (long long)ky * d.nz] = lds[li * LP + lpad(ky)];
   }
 }
 
@@ -193,7 +215,7 @@ __global__ void __launch_bounds__(MAXTHREADS) k_zpass(DD d, float2* __restrict__
     }
   line_fft<false, 3>(v, t, d.l2z, lds + g * LP, PT * LP, tw);
   if (valid) {
-    const int kx = pt / d.Ky, ky = pt - kx * d.Ky;
+    const int kx = pt >> d.l2y, ky = pt & (d.Ky - 1);
 #pragma unroll
     for (int q = 0; q < 8; ++q) kernel_multiply(Tk, d, kx, ky, t + T * q, v[0][q], v[1][q], v[2][q]);
   }
@@ -211,21 +233,20 @@ __global__ void __launch_bounds__(MAXTHREADS) k_zpass(DD d, float2* __restrict__
 
 // y inverse (3-D): B -> A, keeps y < Ny.
 __global__ void __launch_bounds__(MAXTHREADS) k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A,
-                                                     const float2* __restrict__ tw, int LB) {
+                                                     const float2* __restrict__ tw, int l2LB) {
   extern __shared__ float2 lds[];
+  __shared__ long long bbase[64];
+  const int LB = 1 << l2LB;
   const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
   const int LP = padded_len(d.Ky);
   const int nq = 3 * d.Hx * d.nz;
   const int q0 = blockIdx.x * LB;
-  for (int e = threadIdx.x; e < d.Ky * LB; e += blockDim.x) {
-    const int ky = e / LB, li = e - ky * LB;
-    const int q = q0 + li;
-    float2 val = make_float2(0.f, 0.f);
-    if (q < nq) {
-      const int ckx = q / d.nz, z = q - ckx * d.nz;
-      val = B[((size_t)ckx * d.Ky + ky) * d.nz + z];
-    }
-    lds[li * LP + lpad(ky)] = val;
+  for (int i = threadIdx.x; i < LB; i += blockDim.x) bbase[i] = b_line_base(d, q0 + i);
+  __syncthreads();
+  for (int e = threadIdx.x; e < (d.Ky << l2LB); e += blockDim.x) {
+    const int ky = e >> l2LB, li = e & (LB - 1);
+    const long long base = bbase[li];
+    lds[li * LP + lpad(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
@@ -285,24 +306,24 @@ __global__ void __launch_bounds__(MAXTHREADS) k_ypass2d(DD d, float2* __restrict
 // x inverse: PB line pairs per block; C2R of two lines at a time.
 __global__ void __launch_bounds__(MAXTHREADS) k_xinv(DD d, const float2* __restrict__ A,
                                                      const uint8_t* __restrict__ geom, float* __restrict__ h,
-                                                     const float2* __restrict__ tw, int PB) {
+                                                     const float2* __restrict__ tw, int l2PB) {
   extern __shared__ float2 lds[];
+  __shared__ long long abase[64];
+  const int PB = 1 << l2PB;
   const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
   const int LP = padded_len(d.Kx);
   const int XS = d.Hx + 1;
+  const int l2nl = l2PB + 1;
   const int nl = 2 * PB;
   float2* X = lds;
   float2* Z = lds + nl * XS;
   const int l0 = 2 * blockIdx.x * PB;
-  for (int e = threadIdx.x; e < d.Hx * nl; e += blockDim.x) {
-    const int kx = e / nl, li = e - kx * nl;
-    const int l = l0 + li;
-    float2 val = make_float2(0.f, 0.f);
-    if (l < d.nlines) {
-      const int yz = l / d.ny;
-      val = A[((size_t)yz * d.Hx + kx) * d.ny + (l - yz * d.ny)];
-    }
-    X[li * XS + kx] = val;
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) abase[i] = a_line_base(d, l0 + i);
+  __syncthreads();
+  for (int e = threadIdx.x; e < (d.Hx << l2nl); e += blockDim.x) {
+    const int kx = e >> l2nl, li = e & (nl - 1);
+    const long long base = abase[li];
+    X[li * XS + kx] = base >= 0 ? A[base + (long long)kx * d.ny] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const int g = threadIdx.x / T, t = threadIdx.x - g * T;
@@ -325,18 +346,13 @@ __global__ void __launch_bounds__(MAXTHREADS) k_xinv(DD d, const float2* __restr
   if (d.Kx >= 8) line_fft<true, 1>(v, t, d.l2x, Z + g * LP, 0, tw);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
+  const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int x = t + T * q;
     if (q >= E || x >= d.nx) continue;
-    if (la < d.nlines) {
-      const int cell = (la % nyz) * d.nx + x;
-      h[(size_t)la * d.nx + x] = (geom && !geom[cell]) ? 0.f : v[0][q].x;
-    }
-    if (lb < d.nlines) {
-      const int cell = (lb % nyz) * d.nx + x;
-      h[(size_t)lb * d.nx + x] = (geom && !geom[cell]) ? 0.f : v[0][q].y;
-    }
+    if (la < d.nlines) h[(size_t)la * d.nx + x] = (geom && !geom[cella + x]) ? 0.f : v[0][q].x;
+    if (lb < d.nlines) h[(size_t)lb * d.nx + x] = (geom && !geom[cellb + x]) ? 0.f : v[0][q].y;
   }
 }
 
@@ -443,7 +459,8 @@ void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
 constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup used by every pass
 
 void check_launch_shape(size_t lds_bytes, int threads, const char* pass) {
-  if (lds_bytes > LDS_LIMIT) throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
+  // 1 KB is left for the static per-line offset tables of the passes
+  if (lds_bytes + 1024 > LDS_LIMIT) throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
   if (threads < 1 || threads > MAXTHREADS)
     throw Error(MPA_ERR_RUNTIME, std::string("demag: bad block size in ") + pass);
 }
@@ -524,7 +541,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
     const size_t shm = (size_t)PB * LPx * sizeof(float2);
     check_launch_shape(shm, PB * Tx, "k_xfwd");
     Timed t(prof, "k_xfwd", pass_bytes(0), s);
-    k_xfwd<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, m, msat, geom, A_.get(), tw_[0].get(), PB);
+    k_xfwd<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, m, msat, geom, A_.get(), tw_[0].get(), ilog2(PB));
     checkLaunch("k_xfwd");
   }
   if (is2d()) {
@@ -541,7 +558,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
     const long long nq = 3LL * d.Hx * d.nz;
     {
       Timed t(prof, "k_yfwd", pass_bytes(1), s);
-      k_yfwd<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, A_.get(), B_.get(), tw_[1].get(), LB);
+      k_yfwd<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, A_.get(), B_.get(), tw_[1].get(), ilog2(LB));
       checkLaunch("k_yfwd");
     }
     {
@@ -554,7 +571,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
     }
     {
       Timed t(prof, "k_yinv", pass_bytes(3), s);
-      k_yinv<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, B_.get(), A_.get(), tw_[1].get(), LB);
+      k_yinv<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, B_.get(), A_.get(), tw_[1].get(), ilog2(LB));
       checkLaunch("k_yinv");
     }
   }
@@ -563,7 +580,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
     const size_t shm = ((size_t)2 * PB * (d.Hx + 1) + (size_t)PB * LPx) * sizeof(float2);
     check_launch_shape(shm, PB * Tx, "k_xinv");
     Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
-    k_xinv<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, A_.get(), geom, h, tw_[0].get(), PB);
+    k_xinv<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, A_.get(), geom, h, tw_[0].get(), ilog2(PB));
     checkLaunch("k_xinv");
   }
 }
