@@ -18,7 +18,12 @@
 // (ypass2d) and B is not used.  Every pass reads or writes whole lines
 // contiguously on one side; the transposing side goes through LDS.  The
 // line transforms are register-resident (fft.hpp): a 512-point line is 64
-// threads x 8 values with two LDS exchanges.
+// threads x 8 values with two LDS exchanges.  Each pass is instantiated per
+// line length, so thread counts, pass schedules and LDS strides are
+// compile-time constants.  Since K >= 2N, the non-zero inputs of a forward
+// transform and the kept outputs of an inverse one are all in the first
+// K/2 points, i.e. in the first 4 of a thread's 8 values; the others are
+// compile-time zeros or never computed.
 //
 // Kernel spectrum: the Newell kernel on offsets -(N-1)..(N-1) placed
 // circularly is even (xx, yy, zz) or odd (xy: x,y; xz: x,z; yz: y,z) along
@@ -32,6 +37,7 @@
 #include "newell.hpp"
 
 #include <cmath>
+#include <utility>
 #include <vector>
 
 namespace mpa {
@@ -40,14 +46,17 @@ namespace {
 
 constexpr double MU0 = 1.25663706212e-6;  // src/physics/constants.hpp:22
 constexpr int BLOCK = 256;
-constexpr int MAXK = 2048;       // longest FFT line supported per axis
-constexpr int MAXTHREADS = 512;  // threads per block of the FFT passes
+constexpr int MAXL2 = 11;  // longest FFT line supported per axis: 2048 points
+constexpr size_t LDS_LIMIT = 65536;
 
 int ilog2(int v) {
   int l = 0;
   while ((1 << l) < v) ++l;
   return l;
 }
+
+constexpr int clampc(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
 struct DD {
   int nx, ny, nz;
@@ -57,18 +66,40 @@ struct DD {
   int nlines;  // 3*ny*nz real x-lines
 };
 
-__host__ __device__ __forceinline__ int threads_per_line(int K) { return K >= 8 ? K >> 3 : 1; }
-__host__ __device__ __forceinline__ int elems_per_thread(int K) { return K >= 8 ? 8 : 1; }
+// Thread layout of one line of 2^L2 points (L2 == 0: a single point).
+template <int L2>
+struct Line {
+  static_assert(L2 == 0 || (L2 >= 3 && L2 <= MAXL2), "unsupported line length");
+  static constexpr int L = 1 << L2;
+  static constexpr int L2T = L2 >= 3 ? L2 - 3 : 0;
+  static constexpr int T = 1 << L2T;          // threads per line
+  static constexpr int E = L2 >= 3 ? 8 : 1;   // values per thread
+  static constexpr int EH = L2 >= 3 ? 4 : 1;  // values covering the first L/2 points
+  static constexpr int H = L2 >= 3 ? L / 2 + 1 : 1;
+  // LDS stride of one line: room for lpad(L-1), rounded up to `target`
+  // modulo 32 float2 so that the lines a half-wave touches at once fall on
+  // different banks (target < 0: no rounding).
+  static constexpr int stride(int target) {
+    const int base = L2 >= 3 ? L + L / 8 - 1 : 1;
+    return target < 0 ? base : base + (((target - base) % 32) + 32) % 32;
+  }
+  // bank target for lines used only by the transform: T lines of T < 32
+  // threads share a half-wave
+  static constexpr int fft_target() { return (T >= 8 && T < 32) ? T : -1; }
+};
 
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- x forward
 
-__device__ __forceinline__ float load_M(const float* __restrict__ m, PS msat, const uint8_t* __restrict__ geom,
-                                        const DD& d, int line, int x) {
-  const int nyz = d.ny * d.nz;
-  const int cell = (line % nyz) * d.nx + x;
-  if (geom && !geom[cell]) return 0.0f;
-  return msat.at(cell) * m[(size_t)line * d.nx + x];
-}
+template <int L2>
+struct XfwdCfg {
+  using C = Line<L2>;
+  static constexpr int PB = clampc(512 / C::T, 1, 32);  // line pairs per block
+  static constexpr int L2PB = ilog2c(PB);
+  static constexpr int THREADS = PB * C::T;
+  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (PB < 16 ? PB : 16));
+  static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
+  static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xfwd LDS");
+};
 
 // Offset of x-line l (c, z, y) in A[c][z][kx][y] at kx = 0; -1 past the end.
 __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
@@ -77,53 +108,63 @@ __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
   return (long long)yz * d.Hx * d.ny + (l - yz * d.ny);
 }
 
-// x forward: PB line pairs per block (PB a power of two), T = Kx/8 threads
-// per pair.
-__global__ void __launch_bounds__(MAXTHREADS) k_xfwd(DD d, const float* __restrict__ m, PS msat,
-                                                     const uint8_t* __restrict__ geom, float2* __restrict__ A,
-                                                     const float2* __restrict__ tw, int l2PB) {
+template <int L2>
+__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
+    k_xfwd(DD d, const float* __restrict__ m, PS msat, const uint8_t* __restrict__ geom, float2* __restrict__ A,
+           const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = XfwdCfg<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long abase[64];
-  const int PB = 1 << l2PB;
-  const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
-  const int LP = padded_len(d.Kx);
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int l0 = 2 * blockIdx.x * PB;
+  __shared__ long long abase[2 * K::PB];
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  const int l0 = 2 * K::PB * blockIdx.x;
+  for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   const int la = l0 + 2 * g, lb = la + 1;
-  // blocks can have fewer threads than lines (one thread per line when Kx < 8)
-  for (int i = threadIdx.x; i < 2 * PB; i += blockDim.x) abase[i] = a_line_base(d, l0 + i);
   const int nyz = d.ny * d.nz;
   const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
   float2 v[1][8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int x = t + T * q;
-    float2 a = make_float2(0.f, 0.f);
-    if (q < E && x < d.nx) {
-      if (la < d.nlines && (!geom || geom[cella + x])) a.x = msat.at(cella + x) * m[(size_t)la * d.nx + x];
-      if (lb < d.nlines && (!geom || geom[cellb + x])) a.y = msat.at(cellb + x) * m[(size_t)lb * d.nx + x];
-    }
-    v[0][q] = a;
-  }
-  if (d.Kx >= 8) line_fft<false, 1>(v, t, d.l2x, lds + g * LP, 0, tw);
+  for (int q = 0; q < 8; ++q) v[0][q] = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-    if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
+  for (int q = 0; q < C::EH; ++q) {
+    const int x = t + C::T * q;
+    if (x < d.nx) {
+      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = msat.at(cella + x) * m[(size_t)la * d.nx + x];
+      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = msat.at(cellb + x) * m[(size_t)lb * d.nx + x];
+    }
+  }
+  float2* line = lds + g * K::LP;
+  line_fft<L2, false, 1>(v, t, line, 0, tw);
+#pragma unroll
+  for (int q = 0; q < C::E; ++q) line[lpad(t + C::T * q)] = v[0][q];
   __syncthreads();
-  const int l2nl = l2PB + 1;
-  for (int e = threadIdx.x; e < (d.Hx << l2nl); e += blockDim.x) {
-    const int kx = e >> l2nl, li = e & ((1 << l2nl) - 1);
+  constexpr int L2NL = K::L2PB + 1;
+  for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
+    const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
     const long long base = abase[li];
     if (base < 0) continue;
-    const int p = li >> 1;
-    const float2 z1 = lds[p * LP + lpad(kx)];
-    const float2 z2 = lds[p * LP + lpad((d.Kx - kx) & (d.Kx - 1))];
+    const float2* pl = lds + (li >> 1) * K::LP;
+    const float2 z1 = pl[lpad(kx)];
+    const float2 z2 = pl[lpad((C::L - kx) & (C::L - 1))];
     float2 X;
     if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
     else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
     A[base + (long long)kx * d.ny] = X;
   }
 }
+
+// ---------------------------------------------------------------- y forward / inverse (3-D)
+
+template <int L2>
+struct YCfg {
+  using C = Line<L2>;
+  static constexpr int LB = clampc(512 / C::T, 1, 32);  // lines (c, kx, z) per block, z fastest
+  static constexpr int L2LB = ilog2c(LB);
+  static constexpr int THREADS = LB * C::T;
+  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / LB);
+  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
+  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
+};
 
 // Offset of y-line q = (c, kx, z) in B[c][kx][ky][z] at ky = 0; -1 past the end.
 __device__ __forceinline__ long long b_line_base(const DD& d, int q) {
@@ -132,225 +173,277 @@ __device__ __forceinline__ long long b_line_base(const DD& d, int q) {
   return (long long)ckx * d.Ky * d.nz + z;
 }
 
-// y forward (3-D): LB lines (c, kx, z), z fastest, per block (LB a power of two).
-__global__ void __launch_bounds__(MAXTHREADS) k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B,
-                                                     const float2* __restrict__ tw, int l2LB) {
+// Start of y-line q = (c, kx, z) in A[c][z][kx][y].
+__device__ __forceinline__ size_t a_yline(const DD& d, int q) {
+  const int ckx = q / d.nz, z = q - ckx * d.nz;
+  const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
+  return ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
+}
+
+template <int L2>
+__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+    k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B, const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = YCfg<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long bbase[64];
-  const int LB = 1 << l2LB;
-  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
-  const int LP = padded_len(d.Ky);
+  __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int q0 = blockIdx.x * LB;
-  for (int i = threadIdx.x; i < LB; i += blockDim.x) bbase[i] = b_line_base(d, q0 + i);
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  const int q0 = blockIdx.x * K::LB;
+  for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   const int ql = q0 + g;
   float2 v[1][8];
-  const float2* src = nullptr;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = make_float2(0.f, 0.f);
   if (ql < nq) {
-    const int ckx = ql / d.nz, z = ql - ckx * d.nz;
-    const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
-    src = A + ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
-  }
+    const float2* src = A + a_yline(d, ql);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int y = t + T * q;
-    v[0][q] = (src && q < E && y < d.ny) ? src[y] : make_float2(0.f, 0.f);
+    for (int q = 0; q < C::EH; ++q) {
+      const int y = t + C::T * q;
+      if (y < d.ny) v[0][q] = src[y];
+    }
   }
-  if (d.Ky >= 8) line_fft<false, 1>(v, t, d.l2y, lds + g * LP, 0, tw);
+  float2* line = lds + g * K::LP;
+  line_fft<L2, false, 1>(v, t, line, 0, tw);
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-    if (q < E) lds[g * LP + lpad(t + T * q)] = v[0][q];
+  for (int q = 0; q < C::E; ++q) line[lpad(t + C::T * q)] = v[0][q];
   __syncthreads();
-  for (int e = threadIdx.x; e < (d.Ky << l2LB); e += blockDim.x) {
-    const int ky = e >> l2LB, li = e & (LB - 1);
+  for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
+    const int ky = e >> K::L2LB, li = e & (K::LB - 1);
     const long long base = bbase[li];
     if (base < 0) continue;
-    B[base + (long long)ky * d.nz] = lds[li * LP + lpad(ky)];
+    B[base + (long long)ky * d.nz] = lds[li * K::LP + lpad(ky)];
   }
 }
 
+template <int L2>
+__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+    k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A, const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = YCfg<L2>;
+  extern __shared__ float2 lds[];
+  __shared__ long long bbase[K::LB];
+  const int nq = 3 * d.Hx * d.nz;
+  const int q0 = blockIdx.x * K::LB;
+  for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
+  __syncthreads();
+  for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
+    const int ky = e >> K::L2LB, li = e & (K::LB - 1);
+    const long long base = bbase[li];
+    lds[li * K::LP + lpad(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  float2* line = lds + g * K::LP;
+  float2 v[1][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[0][q] = q < C::E ? line[lpad(t + C::T * q)] : make_float2(0.f, 0.f);
+  __syncthreads();
+  line_fft<L2, true, 1>(v, t, line, 0, tw);
+  const int ql = q0 + g;
+  if (ql < nq) {
+    float2* dst = A + a_yline(d, ql);
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int y = t + C::T * q;
+      if (y < d.ny) dst[y] = v[0][q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- kernel multiply
+
 // h = T . m at one spectral point; T components are xx, yy, zz, xy, xz, yz.
+template <bool FLAT>
 __device__ __forceinline__ void kernel_multiply(const float* __restrict__ T, const DD& d, int kx, int ky, int kz,
                                                 float2& a, float2& b, float2& c) {
   int kyp = ky, kzp = kz;
   float sy = 1.f, sz = 1.f;
   if (ky > d.Ky / 2) { kyp = d.Ky - ky; sy = -1.f; }
-  if (kz > d.Kz / 2) { kzp = d.Kz - kz; sz = -1.f; }
+  if (!FLAT && kz > d.Kz / 2) { kzp = d.Kz - kz; sz = -1.f; }
   const size_t stride = (size_t)d.Hx * d.Hy * d.Hz;
   const size_t base = ((size_t)kx * d.Hy + kyp) * d.Hz + kzp;
   const float nxx = T[base];
   const float nyy = T[stride + base];
   const float nzz = T[2 * stride + base];
   const float nxy = sy * T[3 * stride + base];
-  const bool flat = d.Kz == 1;  // single layer: xz and yz vanish
-  const float nxz = flat ? 0.f : sz * T[4 * stride + base];
-  const float nyz = flat ? 0.f : sy * sz * T[5 * stride + base];
+  // single layer: xz and yz vanish
+  const float nxz = FLAT ? 0.f : sz * T[4 * stride + base];
+  const float nyz = FLAT ? 0.f : sy * sz * T[5 * stride + base];
   const float2 ma = a, mb = b, mc = c;
   a = make_float2(nxx * ma.x + nxy * mb.x + nxz * mc.x, nxx * ma.y + nxy * mb.y + nxz * mc.y);
   b = make_float2(nxy * ma.x + nyy * mb.x + nyz * mc.x, nxy * ma.y + nyy * mb.y + nyz * mc.y);
   c = make_float2(nxz * ma.x + nyz * mb.x + nzz * mc.x, nxz * ma.y + nyz * mb.y + nzz * mc.y);
 }
 
-// z pass (3-D): PT spectral points (kx, ky) per block; each thread holds
-// the three components of its 8 z values, so the kernel multiply between
-// the forward and the inverse transform stays in registers.
-__global__ void __launch_bounds__(MAXTHREADS) k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk,
-                                                      const float2* __restrict__ tw, int PT) {
+// ---------------------------------------------------------------- z pass (3-D) and 2-D y pass
+
+template <int L2>
+struct PointCfg {
+  using C = Line<L2>;
+  static constexpr int PT = clampc(256 / C::T, 1, 256);  // spectral points per block
+  static constexpr int THREADS = PT * C::T;
+  static constexpr int LP = C::stride(C::fft_target());
+  static constexpr int NBUF = 2;  // exchange buffers per line for the three components
+  static constexpr size_t SHM = (size_t)NBUF * PT * LP * sizeof(float2);
+  static_assert(SHM <= LDS_LIMIT - 1024, "point pass LDS");
+};
+
+// PT spectral points (kx, ky) per block; each thread holds the three
+// components of its 8 z values, so the kernel multiply between the forward
+// and the inverse transform stays in registers.
+template <int L2>
+__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
+    k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
-  const int T = threads_per_line(d.Kz);
-  const int LP = padded_len(d.Kz);
   const int npt = d.Hx * d.Ky;
   const size_t cs = (size_t)npt;  // points per component
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int pt = blockIdx.x * PT + g;
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  const int pt = blockIdx.x * K::PT + g;
   const bool valid = pt < npt;
   float2 v[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int z = t + T * q;
-      v[c][q] = (valid && z < d.nz) ? B[((size_t)c * cs + pt) * d.nz + z] : make_float2(0.f, 0.f);
-    }
-  line_fft<false, 3>(v, t, d.l2z, lds + g * LP, PT * LP, tw);
-  if (valid) {
-    const int kx = pt >> d.l2y, ky = pt & (d.Ky - 1);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) kernel_multiply(Tk, d, kx, ky, t + T * q, v[0][q], v[1][q], v[2][q]);
-  }
-  line_fft<true, 3>(v, t, d.l2z, lds + g * LP, PT * LP, tw);
+    for (int q = 0; q < 8; ++q) v[c][q] = make_float2(0.f, 0.f);
   if (valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int z = t + T * q;
+      for (int q = 0; q < C::EH; ++q) {
+        const int z = t + C::T * q;
+        if (z < d.nz) v[c][q] = B[((size_t)c * cs + pt) * d.nz + z];
+      }
+  }
+  float2* line = lds + g * K::LP;
+  line_fft<L2, false, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
+  if (valid) {
+    const int kx = pt >> d.l2y, ky = pt & (d.Ky - 1);
+#pragma unroll
+    for (int q = 0; q < C::E; ++q) kernel_multiply<false>(Tk, d, kx, ky, t + C::T * q, v[0][q], v[1][q], v[2][q]);
+  }
+  line_fft<L2, true, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < C::EH; ++q) {
+        const int z = t + C::T * q;
         if (z < d.nz) B[((size_t)c * cs + pt) * d.nz + z] = v[c][q];
       }
   }
 }
 
-// y inverse (3-D): B -> A, keeps y < Ny.
-__global__ void __launch_bounds__(MAXTHREADS) k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A,
-                                                     const float2* __restrict__ tw, int l2LB) {
+// Single layer: forward y, kernel multiply, inverse y; PT points kx per block.
+template <int L2>
+__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
+    k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long bbase[64];
-  const int LB = 1 << l2LB;
-  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
-  const int LP = padded_len(d.Ky);
-  const int nq = 3 * d.Hx * d.nz;
-  const int q0 = blockIdx.x * LB;
-  for (int i = threadIdx.x; i < LB; i += blockDim.x) bbase[i] = b_line_base(d, q0 + i);
-  __syncthreads();
-  for (int e = threadIdx.x; e < (d.Ky << l2LB); e += blockDim.x) {
-    const int ky = e >> l2LB, li = e & (LB - 1);
-    const long long base = bbase[li];
-    lds[li * LP + lpad(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  float2 v[1][8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) v[0][q] = q < E ? lds[g * LP + lpad(t + T * q)] : make_float2(0.f, 0.f);
-  __syncthreads();
-  if (d.Ky >= 8) line_fft<true, 1>(v, t, d.l2y, lds + g * LP, 0, tw);
-  const int ql = q0 + g;
-  if (ql < nq) {
-    const int ckx = ql / d.nz, z = ql - ckx * d.nz;
-    const int c = ckx / d.Hx, kx = ckx - c * d.Hx;
-    float2* dst = A + ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int y = t + T * q;
-      if (q < E && y < d.ny) dst[y] = v[0][q];
-    }
-  }
-}
-
-// single layer: forward y, kernel multiply, inverse y; PT points kx per block.
-__global__ void __launch_bounds__(MAXTHREADS) k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ Tk,
-                                                        const float2* __restrict__ tw, int PT) {
-  extern __shared__ float2 lds[];
-  const int T = threads_per_line(d.Ky), E = elems_per_thread(d.Ky);
-  const int LP = padded_len(d.Ky);
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int kx = blockIdx.x * PT + g;
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  const int kx = blockIdx.x * K::PT + g;
   const bool valid = kx < d.Hx;
   float2 v[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int y = t + T * q;
-      v[c][q] = (valid && q < E && y < d.ny) ? A[((size_t)c * d.Hx + kx) * d.ny + y] : make_float2(0.f, 0.f);
-    }
-  if (d.Ky >= 8) line_fft<false, 3>(v, t, d.l2y, lds + g * LP, PT * LP, tw);
-  if (valid) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (q < E) kernel_multiply(Tk, d, kx, t + T * q, 0, v[0][q], v[1][q], v[2][q]);
-  }
-  if (d.Ky >= 8) line_fft<true, 3>(v, t, d.l2y, lds + g * LP, PT * LP, tw);
+    for (int q = 0; q < 8; ++q) v[c][q] = make_float2(0.f, 0.f);
   if (valid) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int y = t + T * q;
-        if (q < E && y < d.ny) A[((size_t)c * d.Hx + kx) * d.ny + y] = v[c][q];
+      for (int q = 0; q < C::EH; ++q) {
+        const int y = t + C::T * q;
+        if (y < d.ny) v[c][q] = A[((size_t)c * d.Hx + kx) * d.ny + y];
+      }
+  }
+  float2* line = lds + g * K::LP;
+  line_fft<L2, false, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < C::E; ++q) kernel_multiply<true>(Tk, d, kx, t + C::T * q, 0, v[0][q], v[1][q], v[2][q]);
+  }
+  line_fft<L2, true, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < C::EH; ++q) {
+        const int y = t + C::T * q;
+        if (y < d.ny) A[((size_t)c * d.Hx + kx) * d.ny + y] = v[c][q];
       }
   }
 }
 
-// x inverse: PB line pairs per block; C2R of two lines at a time.
-__global__ void __launch_bounds__(MAXTHREADS) k_xinv(DD d, const float2* __restrict__ A,
-                                                     const uint8_t* __restrict__ geom, float* __restrict__ h,
-                                                     const float2* __restrict__ tw, int l2PB) {
+// ---------------------------------------------------------------- x inverse
+
+template <int L2>
+struct XinvCfg {
+  using C = Line<L2>;
+  static constexpr int PB = clampc(256 / C::T, 1, 32);  // line pairs per block
+  static constexpr int L2PB = ilog2c(PB);
+  static constexpr int THREADS = PB * C::T;
+  static constexpr int XS = C::H + 1;  // stride of a staged half spectrum
+  static constexpr int LP = C::stride(C::fft_target());
+  // the staged half spectra and the transform buffers share the LDS
+  static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
+  static constexpr size_t SHM = (XB > ZB ? XB : ZB) * sizeof(float2);
+  static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xinv LDS");
+};
+
+// PB line pairs per block; C2R of two lines at a time.
+template <int L2>
+__global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
+    k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom, float* __restrict__ h,
+           const float2* __restrict__ tw) {
+  using C = Line<L2>;
+  using K = XinvCfg<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long abase[64];
-  const int PB = 1 << l2PB;
-  const int T = threads_per_line(d.Kx), E = elems_per_thread(d.Kx);
-  const int LP = padded_len(d.Kx);
-  const int XS = d.Hx + 1;
-  const int l2nl = l2PB + 1;
-  const int nl = 2 * PB;
+  __shared__ long long abase[2 * K::PB];
+  constexpr int L2NL = K::L2PB + 1;
+  constexpr int NL = 2 * K::PB;
   float2* X = lds;
-  float2* Z = lds + nl * XS;
-  const int l0 = 2 * blockIdx.x * PB;
-  for (int i = threadIdx.x; i < nl; i += blockDim.x) abase[i] = a_line_base(d, l0 + i);
+  float2* Z = lds;  // reused once the half spectra are in registers
+  const int l0 = 2 * blockIdx.x * K::PB;
+  for (int i = threadIdx.x; i < NL; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   __syncthreads();
-  for (int e = threadIdx.x; e < (d.Hx << l2nl); e += blockDim.x) {
-    const int kx = e >> l2nl, li = e & (nl - 1);
+  for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
+    const int kx = e >> L2NL, li = e & (NL - 1);
     const long long base = abase[li];
-    X[li * XS + kx] = base >= 0 ? A[base + (long long)kx * d.ny] : make_float2(0.f, 0.f);
+    X[li * K::XS + kx] = base >= 0 ? A[base + (long long)kx * d.ny] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int half = d.Kx >> 1;
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  constexpr int HALF = C::L >> 1;
+  const float2* xa_line = X + (2 * g) * K::XS;
+  const float2* xb_line = xa_line + K::XS;
   float2 v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int k = t + T * q;
     float2 zv = make_float2(0.f, 0.f);
-    if (q < E) {
-      const bool upper = k > half;
-      const int kk = upper ? d.Kx - k : k;
-      float2 xa = X[(2 * g) * XS + kk];
-      float2 xb = X[(2 * g + 1) * XS + kk];
-      if (kk == 0 || kk == half) { xa.y = 0.f; xb.y = 0.f; }
+    if (q < C::E) {
+      const int k = t + C::T * q;
+      const bool upper = k > HALF;
+      const int kk = upper ? C::L - k : k;
+      float2 xa = xa_line[kk];
+      float2 xb = xb_line[kk];
+      if (kk == 0 || kk == HALF) { xa.y = 0.f; xb.y = 0.f; }
       zv = upper ? make_float2(xa.x + xb.y, xb.x - xa.y) : make_float2(xa.x - xb.y, xa.y + xb.x);
     }
     v[0][q] = zv;
   }
-  if (d.Kx >= 8) line_fft<true, 1>(v, t, d.l2x, Z + g * LP, 0, tw);
+  __syncthreads();
+  line_fft<L2, true, 1>(v, t, Z + g * K::LP, 0, tw);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
   const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int x = t + T * q;
-    if (q >= E || x >= d.nx) continue;
+  for (int q = 0; q < C::EH; ++q) {
+    const int x = t + C::T * q;
+    if (x >= d.nx) continue;
     if (la < d.nlines) h[(size_t)la * d.nx + x] = (geom && !geom[cella + x]) ? 0.f : v[0][q].x;
     if (lb < d.nlines) h[(size_t)lb * d.nx + x] = (geom && !geom[cellb + x]) ? 0.f : v[0][q].y;
   }
@@ -433,14 +526,10 @@ __global__ void k_newell_grid(float* __restrict__ out, int Kx, int Ky, int Kz, i
 
 unsigned grid_for(long long n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
 
-void make_twiddles(DeviceArray<float2>& tw, int K) {
-  std::vector<float2> h(K);
-  for (int q = 0; q < K; ++q) {
-    const double a = -2.0 * M_PI * (double)q / (double)K;
-    h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
-  }
-  tw.allocate(K);
-  HIP_CHECK(hipMemcpy(tw.get(), h.data(), K * sizeof(float2), hipMemcpyHostToDevice));
+void make_twiddles(DeviceArray<float2>& tw, int l2) {
+  const std::vector<float2> h = stage_twiddles(l2);
+  tw.allocate(h.size());
+  HIP_CHECK(hipMemcpy(tw.get(), h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
 }
 
 void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
@@ -456,16 +545,83 @@ void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
   HIP_CHECK(hipMemcpy(st.get(), s.data(), K * sizeof(double), hipMemcpyHostToDevice));
 }
 
-constexpr size_t LDS_LIMIT = 65536;  // dynamic LDS per workgroup used by every pass
+// ------------------------------------------------ launchers per line length
 
-void check_launch_shape(size_t lds_bytes, int threads, const char* pass) {
-  // 1 KB is left for the static per-line offset tables of the passes
-  if (lds_bytes + 1024 > LDS_LIMIT) throw Error(MPA_ERR_RUNTIME, std::string("demag: LDS budget exceeded in ") + pass);
-  if (threads < 1 || threads > MAXTHREADS)
-    throw Error(MPA_ERR_RUNTIME, std::string("demag: bad block size in ") + pass);
+template <int L2>
+struct RunXfwd {
+  static void run(const DD& d, const float* m, PS msat, const uint8_t* geom, float2* A, const float2* tw,
+                  hipStream_t s) {
+    using K = XfwdCfg<L2>;
+    const int npairs = (d.nlines + 1) / 2;
+    k_xfwd<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, m, msat, geom, A, tw);
+    checkLaunch("k_xfwd");
+  }
+};
+
+template <int L2>
+struct RunYfwd {
+  static void run(const DD& d, const float2* A, float2* B, const float2* tw, hipStream_t s) {
+    using K = YCfg<L2>;
+    k_yfwd<L2><<<grid_for(3LL * d.Hx * d.nz, K::LB), K::THREADS, K::SHM, s>>>(d, A, B, tw);
+    checkLaunch("k_yfwd");
+  }
+};
+
+template <int L2>
+struct RunYinv {
+  static void run(const DD& d, const float2* B, float2* A, const float2* tw, hipStream_t s) {
+    using K = YCfg<L2>;
+    k_yinv<L2><<<grid_for(3LL * d.Hx * d.nz, K::LB), K::THREADS, K::SHM, s>>>(d, B, A, tw);
+    checkLaunch("k_yinv");
+  }
+};
+
+template <int L2>
+struct RunZpass {
+  static void run(const DD& d, float2* B, const float* Tk, const float2* tw, hipStream_t s) {
+    using K = PointCfg<L2>;
+    k_zpass<L2><<<grid_for((long long)d.Hx * d.Ky, K::PT), K::THREADS, K::SHM, s>>>(d, B, Tk, tw);
+    checkLaunch("k_zpass");
+  }
+};
+
+template <int L2>
+struct RunYpass2d {
+  static void run(const DD& d, float2* A, const float* Tk, const float2* tw, hipStream_t s) {
+    using K = PointCfg<L2>;
+    k_ypass2d<L2><<<grid_for(d.Hx, K::PT), K::THREADS, K::SHM, s>>>(d, A, Tk, tw);
+    checkLaunch("k_ypass2d");
+  }
+};
+
+template <int L2>
+struct RunXinv {
+  static void run(const DD& d, const float2* A, const uint8_t* geom, float* h, const float2* tw, hipStream_t s) {
+    using K = XinvCfg<L2>;
+    const int npairs = (d.nlines + 1) / 2;
+    k_xinv<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, A, geom, h, tw);
+    checkLaunch("k_xinv");
+  }
+};
+
+// Calls R<l2>::run(args...) for a supported line length 2^l2.
+template <template <int> class R, typename... Args>
+void with_line(int l2, Args&&... args) {
+  switch (l2) {
+    case 0: R<0>::run(std::forward<Args>(args)...); return;
+    case 3: R<3>::run(std::forward<Args>(args)...); return;
+    case 4: R<4>::run(std::forward<Args>(args)...); return;
+    case 5: R<5>::run(std::forward<Args>(args)...); return;
+    case 6: R<6>::run(std::forward<Args>(args)...); return;
+    case 7: R<7>::run(std::forward<Args>(args)...); return;
+    case 8: R<8>::run(std::forward<Args>(args)...); return;
+    case 9: R<9>::run(std::forward<Args>(args)...); return;
+    case 10: R<10>::run(std::forward<Args>(args)...); return;
+    case 11: R<11>::run(std::forward<Args>(args)...); return;
+    default: break;
+  }
+  throw Error(MPA_ERR_RUNTIME, "demag: unsupported FFT line length 2^" + std::to_string(l2));
 }
-
-int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 }  // namespace
 
@@ -481,7 +637,7 @@ DemagPlan::DemagPlan(const DemagGeometry& g) : g_(g) {
     if (g.n[a] < 1) throw Error(MPA_ERR_INVALID, "demag: grid size must be positive");
     K_[a] = demag_fft_length(g.n[a]);
     log2K_[a] = ilog2(K_[a]);
-    if (K_[a] > MAXK) throw Error(MPA_ERR_INVALID, "demag: grid axis too large for the FFT passes (max 1024 cells)");
+    if (log2K_[a] > MAXL2) throw Error(MPA_ERR_INVALID, "demag: grid axis too large for the FFT passes (max 1024 cells)");
   }
   N_ = g.n[0] * g.n[1] * g.n[2];
   Hx_ = K_[0] == 1 ? 1 : K_[0] / 2 + 1;
@@ -489,7 +645,7 @@ DemagPlan::DemagPlan(const DemagGeometry& g) : g_(g) {
   Hz_ = K_[2] / 2 + 1;
   A_.allocate((size_t)3 * g.n[2] * Hx_ * g.n[1]);
   if (!is2d()) B_.allocate((size_t)3 * Hx_ * K_[1] * g.n[2]);
-  for (int a = 0; a < 3; ++a) make_twiddles(tw_[a], K_[a]);
+  for (int a = 0; a < 3; ++a) make_twiddles(tw_[a], log2K_[a]);
   build_kernel(nullptr);
 }
 
@@ -533,55 +689,30 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
   d.Hx = Hx_; d.Hy = Hy_; d.Hz = Hz_;
   d.l2x = log2K_[0]; d.l2y = log2K_[1]; d.l2z = log2K_[2];
   d.nlines = 3 * d.ny * d.nz;
-  const int Tx = threads_per_line(d.Kx), Ty = threads_per_line(d.Ky), Tz = threads_per_line(d.Kz);
-  const int LPx = padded_len(d.Kx), LPy = padded_len(d.Ky), LPz = padded_len(d.Kz);
-  const int npairs = (d.nlines + 1) / 2;
   {
-    const int PB = clampi(MAXTHREADS / Tx, 1, 32);
-    const size_t shm = (size_t)PB * LPx * sizeof(float2);
-    check_launch_shape(shm, PB * Tx, "k_xfwd");
     Timed t(prof, "k_xfwd", pass_bytes(0), s);
-    k_xfwd<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, m, msat, geom, A_.get(), tw_[0].get(), ilog2(PB));
-    checkLaunch("k_xfwd");
+    with_line<RunXfwd>(d.l2x, d, m, msat, geom, A_.get(), tw_[0].get(), s);
   }
   if (is2d()) {
-    const int PT = clampi(BLOCK / Ty, 1, BLOCK);
-    const size_t shm = (size_t)3 * PT * LPy * sizeof(float2);
-    check_launch_shape(shm, PT * Ty, "k_ypass2d");
     Timed t(prof, "k_ypass2d", pass_bytes(1), s);
-    k_ypass2d<<<grid_for(d.Hx, PT), PT * Ty, shm, s>>>(d, A_.get(), T_.get(), tw_[1].get(), PT);
-    checkLaunch("k_ypass2d");
+    with_line<RunYpass2d>(d.l2y, d, A_.get(), T_.get(), tw_[1].get(), s);
   } else {
-    const int LB = clampi(MAXTHREADS / Ty, 1, 32);
-    const size_t shmy = (size_t)LB * LPy * sizeof(float2);
-    check_launch_shape(shmy, LB * Ty, "k_yfwd");
-    const long long nq = 3LL * d.Hx * d.nz;
     {
       Timed t(prof, "k_yfwd", pass_bytes(1), s);
-      k_yfwd<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, A_.get(), B_.get(), tw_[1].get(), ilog2(LB));
-      checkLaunch("k_yfwd");
+      with_line<RunYfwd>(d.l2y, d, A_.get(), B_.get(), tw_[1].get(), s);
     }
     {
-      const int PT = clampi(BLOCK / Tz, 1, BLOCK);
-      const size_t shm = (size_t)3 * PT * LPz * sizeof(float2);
-      check_launch_shape(shm, PT * Tz, "k_zpass");
       Timed t(prof, "k_zpass", pass_bytes(2), s);
-      k_zpass<<<grid_for((long long)d.Hx * d.Ky, PT), PT * Tz, shm, s>>>(d, B_.get(), T_.get(), tw_[2].get(), PT);
-      checkLaunch("k_zpass");
+      with_line<RunZpass>(d.l2z, d, B_.get(), T_.get(), tw_[2].get(), s);
     }
     {
       Timed t(prof, "k_yinv", pass_bytes(3), s);
-      k_yinv<<<grid_for(nq, LB), LB * Ty, shmy, s>>>(d, B_.get(), A_.get(), tw_[1].get(), ilog2(LB));
-      checkLaunch("k_yinv");
+      with_line<RunYinv>(d.l2y, d, B_.get(), A_.get(), tw_[1].get(), s);
     }
   }
   {
-    const int PB = clampi(BLOCK / Tx, 1, 32);
-    const size_t shm = ((size_t)2 * PB * (d.Hx + 1) + (size_t)PB * LPx) * sizeof(float2);
-    check_launch_shape(shm, PB * Tx, "k_xinv");
     Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
-    k_xinv<<<grid_for(npairs, PB), PB * Tx, shm, s>>>(d, A_.get(), geom, h, tw_[0].get(), ilog2(PB));
-    checkLaunch("k_xinv");
+    with_line<RunXinv>(d.l2x, d, A_.get(), geom, h, tw_[0].get(), s);
   }
 }
 

@@ -1,4 +1,4 @@
-// Register-resident complex FFT of one line, cooperatively computed by
+// Register-resident complex FFT of one line of L = 2^L2 points, computed by
 // T = L/8 threads, used by the demag convolution passes.
 //
 // Self-sorting Stockham formulation.  A pass of radix R with current
@@ -10,9 +10,18 @@
 // (q = 0..7).  The first pass therefore reads straight from global memory,
 // the outputs of the last pass are again the elements t + T*q, and only the
 // passes in between exchange data through LDS.  Passes: one radix-2 or
-// radix-4 pass when log2(L) is not a multiple of 3, then radix-8 passes.
-// The transform is unnormalised.  L is 1 (no work) or a power of two >= 8.
+// radix-4 pass when L2 is not a multiple of 3, then radix-8 passes.
+//
+// L2 is a template parameter, so every pass's radix, Ns and twiddle-table
+// offset are compile-time constants.  LDS uses one pad slot every 8
+// elements, which keeps all Stockham read and write offsets affine in the
+// per-thread base (immediate offsets in the LDS instructions).  Twiddles come
+// from a per-pass table (stage_twiddles) holding, for each k < Ns, the R-1
+// factors w^(r k) consecutively.  The transform is unnormalised; L is 1 (no
+// work) or a power of two >= 8.
 #pragma once
+
+#include <vector>
 
 #include "common.hpp"
 
@@ -30,18 +39,48 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
   return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
-// LDS index with one pad slot every 16 elements (spreads the strided
-// Stockham writes over the banks).
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
-__host__ __device__ __forceinline__ int padded_len(int L) { return L + (L >> 4) + 1; }
+// LDS index with one pad slot every 8 elements.
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 3); }
+
+// Pass schedule of an L = 2^L2 transform.
+__host__ __device__ constexpr int fft_npass(int L2) { return L2 < 3 ? 0 : L2 / 3 + (L2 % 3 ? 1 : 0); }
+__host__ __device__ constexpr int fft_radix(int L2, int p) {
+  return p == 0 ? (L2 % 3 == 1 ? 2 : (L2 % 3 == 2 ? 4 : 8)) : 8;
+}
+__host__ __device__ constexpr int log2i(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
+__host__ __device__ constexpr int fft_l2ns(int L2, int p) {
+  int s = 0;
+  for (int q = 0; q < p; ++q) s += log2i(fft_radix(L2, q));
+  return s;
+}
+__host__ __device__ constexpr int fft_twoff(int L2, int p) {
+  int o = 0;
+  for (int q = 0; q < p; ++q) o += (1 << fft_l2ns(L2, q)) * (fft_radix(L2, q) - 1);
+  return o;
+}
+__host__ __device__ constexpr int fft_twsize(int L2) { return fft_twoff(L2, fft_npass(L2)); }
+
+// Host: per-pass twiddle table of an L = 2^L2 transform (FP64 -> FP32).
+inline std::vector<float2> stage_twiddles(int L2) {
+  std::vector<float2> t((size_t)(fft_twsize(L2) > 0 ? fft_twsize(L2) : 1), make_float2(1.f, 0.f));
+  for (int p = 0; p < fft_npass(L2); ++p) {
+    const int R = fft_radix(L2, p), Ns = 1 << fft_l2ns(L2, p), off = fft_twoff(L2, p);
+    for (int k = 0; k < Ns; ++k)
+      for (int r = 1; r < R; ++r) {
+        const double a = -2.0 * M_PI * (double)(r * k) / (double)(Ns * R);
+        t[(size_t)off + (size_t)k * (R - 1) + (r - 1)] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+  }
+  return t;
+}
 
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* a) {
-  if (R == 2) {
+  if constexpr (R == 2) {
     const float2 t = a[1];
     a[1] = csub(a[0], t);
     a[0] = cadd(a[0], t);
-  } else if (R == 4) {
+  } else if constexpr (R == 4) {
     const float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
     const float2 t2 = cadd(a[1], a[3]), t3 = mul_mi<INV>(csub(a[1], a[3]));
     a[0] = cadd(t0, t2);
@@ -75,80 +114,115 @@ __device__ __forceinline__ void dft(float2* a) {
   }
 }
 
-// One Stockham pass for NC lines held by the same thread group.  Line c of
-// the group lives at lds + c*cstride (padded).  `last` passes keep their
-// outputs in registers; the others exchange through LDS (two barriers, so
-// every thread of the block must call this with the same arguments).
-template <int R, bool INV, int NC>
-__device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, int T, int log2L, int log2Ns, bool last,
-                                         float2* lds, int cstride, const float2* __restrict__ tw) {
-  constexpr int M = 8 / R;
-  constexpr int LOG2R = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  const int Ns = 1 << log2Ns;
-  const int twsh = log2L - log2Ns - LOG2R;
+// Radix-R butterfly m of component c in pass P: twiddle, DFT, and either keep
+// the outputs (last pass) or store them into the line buffer `buf`.
+template <int L2, int P, bool INV, int NC>
+__device__ __forceinline__ void fft_butterfly(float2 (&v)[NC][8], int c, int m, const float2* w, int base,
+                                              float2* buf) {
+  constexpr int R = fft_radix(L2, P), M = 8 / R;
+  constexpr int NS = 1 << fft_l2ns(L2, P);
+  constexpr bool LAST = P == fft_npass(L2) - 1;
+  float2 a[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] = v[c][m + r * M];
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+  }
+  dft<R, INV>(a);
+  if constexpr (LAST) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[c][m + r * M] = a[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[base + r * NS + ((r * NS) >> 3)] = a[r];
+  }
+}
+
+// One Stockham pass for NC lines held by the same thread group, exchanged
+// through NBUF line buffers at lds + b*bstride.  NBUF >= NC: all lines are
+// stored, then read back (two barriers).  NBUF < NC: lines go one at a time,
+// line i = P*NC + c through buffer i % NBUF; with NBUF >= 2 one barrier per
+// line suffices, because a buffer is rewritten only after the barrier that
+// follows the next line's stores.  The last pass keeps its outputs in
+// registers.  Every thread of the block must call this.
+template <int L2, int P, bool INV, int NC, int NBUF>
+__device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds, int bstride,
+                                         const float2* __restrict__ tw) {
+  constexpr int L = 1 << L2, T = L / 8;
+  constexpr int R = fft_radix(L2, P), M = 8 / R, LOG2R = log2i(R);
+  constexpr int L2NS = fft_l2ns(L2, P), NS = 1 << L2NS;
+  constexpr int TWOFF = fft_twoff(L2, P);
+  constexpr bool LAST = P == fft_npass(L2) - 1;
+  float2 w[M][R];
+  int base[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = t + m * T;
-    const int k = j & (Ns - 1);
-    float2 w[R];
+    const int k = j & (NS - 1);
+    if constexpr (NS > 1) {
+      const float2* tp = tw + TWOFF + k * (R - 1);
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      w[r] = make_float2(1.f, 0.f);
-      if (log2Ns > 0 && k != 0) {
-        w[r] = tw[(r * k) << twsh];
-        if (INV) w[r].y = -w[r].y;
+      for (int r = 1; r < R; ++r) {
+        w[m][r] = tp[r - 1];
+        if (INV) w[m][r].y = -w[m][r].y;
       }
     }
-    const int base = ((j >> log2Ns) << (log2Ns + LOG2R)) + k;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      float2 a[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) a[r] = v[c][m + r * M];
-#pragma unroll
-      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
-      dft<R, INV>(a);
-      if (last) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[c][m + r * M] = a[r];
-      } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[c * cstride + lpad(base + r * Ns)] = a[r];
-      }
-    }
+    base[m] = lpad(((j >> L2NS) << (L2NS + LOG2R)) + k);
   }
-  if (!last) {
+  const int rb = lpad(t);
+  if constexpr (LAST) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], nullptr);
+  } else if constexpr (NBUF >= NC) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], lds + c * bstride);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[c][q] = lds[c * cstride + lpad(t + T * q)];
+      for (int q = 0; q < 8; ++q) v[c][q] = lds[c * bstride + rb + T * q + ((T * q) >> 3)];
     __syncthreads();
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      float2* buf = lds + ((P * NC + c) % NBUF) * bstride;
+#pragma unroll
+      for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], buf);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[c][q] = buf[rb + T * q + ((T * q) >> 3)];
+      if constexpr (NBUF == 1) __syncthreads();
+    }
+  }
+}
+
+template <int L2, int P, bool INV, int NC, int NBUF>
+__device__ __forceinline__ void fft_passes(float2 (&v)[NC][8], int t, float2* lds, int bstride,
+                                           const float2* __restrict__ tw) {
+  if constexpr (P < fft_npass(L2)) {
+    fft_pass<L2, P, INV, NC, NBUF>(v, t, lds, bstride, tw);
+    fft_passes<L2, P + 1, INV, NC, NBUF>(v, t, lds, bstride, tw);
   }
 }
 
 // Full transform: on entry v[c][q] holds element t + T*q of line c, on exit
-// the transform's element t + T*q.  log2L >= 3.
-template <bool INV, int NC>
-__device__ __forceinline__ void line_fft(float2 (&v)[NC][8], int t, int log2L, float2* lds, int cstride,
+// the transform's element t + T*q.  No work for L2 == 0.  Lines exchange
+// through NBUF buffers of bstride float2 each (NBUF = NC: one per line).  On
+// return every LDS access of the transform has completed block-wide, so the
+// caller may reuse the buffers.
+template <int L2, bool INV, int NC, int NBUF = NC>
+__device__ __forceinline__ void line_fft(float2 (&v)[NC][8], int t, float2* lds, int bstride,
                                          const float2* __restrict__ tw) {
-  const int T = 1 << (log2L - 3);
-  const int rem = log2L % 3;
-  const int npass = log2L / 3 + (rem ? 1 : 0);
-  int p = 0, log2Ns = 0;
-  if (rem == 1) {
-    fft_pass<2, INV, NC>(v, t, T, log2L, 0, npass == 1, lds, cstride, tw);
-    log2Ns = 1;
-    p = 1;
-  } else if (rem == 2) {
-    fft_pass<4, INV, NC>(v, t, T, log2L, 0, npass == 1, lds, cstride, tw);
-    log2Ns = 2;
-    p = 1;
-  }
-  for (; p < npass; ++p) {
-    fft_pass<8, INV, NC>(v, t, T, log2L, log2Ns, p == npass - 1, lds, cstride, tw);
-    log2Ns += 3;
-  }
+  static_assert(L2 == 0 || L2 >= 3, "line length must be 1 or >= 8");
+  static_assert(NBUF >= 1, "at least one exchange buffer");
+  fft_passes<L2, 0, INV, NC, NBUF>(v, t, lds, bstride, tw);
+  // sequential exchanges through two or more buffers end without a barrier
+  if constexpr (NBUF >= 2 && NBUF < NC && fft_npass(L2) > 1) __syncthreads();
 }
 
 }  // namespace mpa
