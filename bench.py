@@ -110,6 +110,21 @@ def cpu_baseline(seed, budget_s=30.0):
             "seconds": round(t, 3)}
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE counters (scaled by a copy of known size), from the committed
+    summary profiles/pmc_traffic_256x256x64.txt; None if it has no entry."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_256x256x64.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            fields = line.split()
+            if len(fields) >= 4 and not fields[0].startswith("#") and fields[0] == kernel:
+                return float(fields[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,12 +171,7 @@ def main():
     t_step = wall / args.steps
     avg = mag.magnetization.average()
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_256x256x64.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tr = json.load(f)
-        traffic = tr.get(dom)
+    traffic = pmc_traffic(dom)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -195,3 +195,9 @@ def _canting(kind, axis):
 def test_edge_canting_golden(kind, axis):
     got = _canting(kind, axis)
     assert abs(got - CANT[kind]) / CANT[kind] < 2e-4
+    if kind != "neumann":
+        # test/test_boundaries.py:52-55,97-140: open and periodic are compared
+        # with the analytic 1-D canting shifted to the cell centre
+        theta0 = np.arcsin(DMI / 2)
+        want = 2 * np.arctan(np.exp(-0.05 / 2) * np.tan(theta0 / 2))
+        assert abs(got - want) / want < 2e-4
