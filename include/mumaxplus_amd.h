@@ -81,8 +81,11 @@ int mpa_fm_name(const mpa_ferromagnet* m, char* buf, int buflen);
 int mpa_fm_grid(const mpa_ferromagnet* m, int size[3], int origin[3]);
 int mpa_fm_geometry(const mpa_ferromagnet* m, uint8_t* out); /* ncells bytes */
 int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsingeo system.cpp:80-83 */
-/* Ferromagnet.enable_demag / enable_openbc (wrap_ferromagnet.cpp:33-38). flag: "enable_demag",
- * "enable_openbc". */
+/* Ferromagnet.enable_demag / enable_openbc (wrap_ferromagnet.cpp:33-38) and
+ * Magnet.enable_as_stray_field_source / _destination (wrap_magnet.cpp:17-54). flag: "enable_demag",
+ * "enable_openbc", "enable_as_stray_field_source", "enable_as_stray_field_destination".  Stray
+ * fields between magnets are not built: evaluating a field they would enter fails with
+ * MPA_ERR_RUNTIME while both flags are set and the other magnet's msat is not zero. */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
 /* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,

@@ -102,7 +102,7 @@ int mpa_world_get_cellsize(const mpa_world* w, double out[3]) {
 int mpa_world_set_bias_magnetic_field(mpa_world* w, const double b[3]) {
   return guard([&] {
     need(w, "world");
-    for (int a = 0; a < 3; ++a) w->w.bias[a] = b[a];
+    for (int a = 0; a < 3; ++a) w->w.bias[a] = (double)(float)b[a];  // FP32 real3 in the reference
     w->w.touch();
   });
 }
@@ -229,6 +229,8 @@ int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value) {
     const std::string k = flag;
     if (k == "enable_demag") f->enable_demag = value != 0;
     else if (k == "enable_openbc") f->openbc = value != 0;
+    else if (k == "enable_as_stray_field_source") f->stray_source = value != 0;
+    else if (k == "enable_as_stray_field_destination") f->stray_destination = value != 0;
     else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
     f->world->touch();
   });
@@ -241,6 +243,8 @@ int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
     const std::string k = flag;
     if (k == "enable_demag") *out = f->enable_demag;
     else if (k == "enable_openbc") *out = f->openbc;
+    else if (k == "enable_as_stray_field_source") *out = f->stray_source;
+    else if (k == "enable_as_stray_field_destination") *out = f->stray_destination;
     else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
   });
 }
@@ -474,7 +478,7 @@ int mpa_demag_plan_create(const int size[3], const double cellsize[3], const int
     DemagGeometry g;
     for (int a = 0; a < 3; ++a) {
       g.n[a] = size[a];
-      g.cs[a] = cellsize[a];
+      g.cs[a] = (double)(float)cellsize[a];  // the reference's FP32 cell size
       g.mastergrid[a] = mastergrid_size ? mastergrid_size[a] : 0;
       g.pbc[a] = pbc_repetitions ? pbc_repetitions[a] : 0;
       if ((g.mastergrid[a] == 0) != (g.pbc[a] == 0))

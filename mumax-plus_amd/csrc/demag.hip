@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   extern __shared__ float2 lds[];
   __shared__ long long abase[2 * K::PB];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int l0 = 2 * K::PB * blockIdx.x;
+  const int l0 = 2 * K::PB * xcd_remap(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int q0 = blockIdx.x * K::LB;
+  const int q0 = xcd_remap(blockIdx.x, gridDim.x) * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   const int ql = q0 + g;
   float2 v[1][8];
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
   extern __shared__ float2 lds[];
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
-  const int q0 = blockIdx.x * K::LB;
+  const int q0 = xcd_remap(blockIdx.x, gridDim.x) * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   __syncthreads();
   for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
@@ -253,23 +253,24 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
 
 // ---------------------------------------------------------------- kernel multiply
 
-// h = T . m at one spectral point; T components are xx, yy, zz, xy, xz, yz.
+// h = T . m at one spectral point.  `e` points at the octant entry (six
+// adjacent floats xx, yy, zz, xy, xz, yz of the spectrum table
+// T[Hx][Hy][Hz][6]); sy and sz are the signs of the components odd in y and
+// z at the point (-1 where ky > Ky/2 or kz > Kz/2; at Ky/2 and Kz/2 those
+// components are zero, so either sign is right there).
 template <bool FLAT>
-__device__ __forceinline__ void kernel_multiply(const float* __restrict__ T, const DD& d, int kx, int ky, int kz,
-                                                float2& a, float2& b, float2& c) {
-  int kyp = ky, kzp = kz;
-  float sy = 1.f, sz = 1.f;
-  if (ky > d.Ky / 2) { kyp = d.Ky - ky; sy = -1.f; }
-  if (!FLAT && kz > d.Kz / 2) { kzp = d.Kz - kz; sz = -1.f; }
-  const size_t stride = (size_t)d.Hx * d.Hy * d.Hz;
-  const size_t base = ((size_t)kx * d.Hy + kyp) * d.Hz + kzp;
-  const float nxx = T[base];
-  const float nyy = T[stride + base];
-  const float nzz = T[2 * stride + base];
-  const float nxy = sy * T[3 * stride + base];
-  // single layer: xz and yz vanish
-  const float nxz = FLAT ? 0.f : sz * T[4 * stride + base];
-  const float nyz = FLAT ? 0.f : sy * sz * T[5 * stride + base];
+__device__ __forceinline__ void kernel_multiply(const float* __restrict__ e, float sy, float sz, float2& a,
+                                                float2& b, float2& c) {
+  const float2* e2 = reinterpret_cast<const float2*>(e);  // entries start on 8-byte boundaries
+  const float2 t0 = e2[0], t1 = e2[1];
+  const float nxx = t0.x, nyy = t0.y, nzz = t1.x;
+  const float nxy = sy * t1.y;
+  float nxz = 0.f, nyz = 0.f;  // single layer: xz and yz vanish
+  if constexpr (!FLAT) {
+    const float2 t2 = e2[2];
+    nxz = sz * t2.x;
+    nyz = sy * sz * t2.y;
+  }
   const float2 ma = a, mb = b, mc = c;
   a = make_float2(nxx * ma.x + nxy * mb.x + nxz * mc.x, nxx * ma.y + nxy * mb.y + nxz * mc.y);
   b = make_float2(nxy * ma.x + nyy * mb.x + nyz * mc.x, nxy * ma.y + nyy * mb.y + nyz * mc.y);
@@ -291,7 +292,9 @@ struct PointCfg {
 
 // PT spectral points (kx, ky) per block; each thread holds the three
 // components of its 8 z values, so the kernel multiply between the forward
-// and the inverse transform stays in registers.
+// and the inverse transform stays in registers.  Adjacent thread groups take
+// the mirror points ky and Ky - ky (0 pairs with Ky/2), which read the same
+// row of the octant spectrum, so each row comes from HBM once.
 template <int L2>
 __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
     k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk, const float2* __restrict__ tw) {
@@ -301,8 +304,11 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   const int npt = d.Hx * d.Ky;
   const size_t cs = (size_t)npt;  // points per component
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int pt = blockIdx.x * K::PT + g;
+  const int pt = blockIdx.x * K::PT + g;  // pair-ordered point index
   const bool valid = pt < npt;
+  const int kx = pt >> d.l2y, r = pt & (d.Ky - 1), kh = r >> 1;
+  const int ky = (r & 1) ? (kh == 0 ? d.Ky >> 1 : d.Ky - kh) : kh;
+  const size_t line0 = (size_t)kx * d.Ky + ky;  // z-line of (kx, ky) in component 0
   float2 v[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -314,15 +320,22 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
 #pragma unroll
       for (int q = 0; q < C::EH; ++q) {
         const int z = t + C::T * q;
-        if (z < d.nz) v[c][q] = B[((size_t)c * cs + pt) * d.nz + z];
+        if (z < d.nz) v[c][q] = B[((size_t)c * cs + line0) * d.nz + z];
       }
   }
   float2* line = lds + g * K::LP;
   line_fft<L2, false, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
   if (valid) {
-    const int kx = pt >> d.l2y, ky = pt & (d.Ky - 1);
+    // values q < EH have kz < Kz/2; the others mirror to Kz - kz with the odd-z sign flipped
+    const bool yflip = ky > (d.Ky >> 1);
+    const float sy = yflip ? -1.f : 1.f;
+    const float* row = Tk + ((size_t)kx * d.Hy + (yflip ? d.Ky - ky : ky)) * d.Hz * 6;
 #pragma unroll
-    for (int q = 0; q < C::E; ++q) kernel_multiply<false>(Tk, d, kx, ky, t + C::T * q, v[0][q], v[1][q], v[2][q]);
+    for (int q = 0; q < C::E; ++q) {
+      const int kz = t + C::T * q;
+      const bool upper = q >= C::EH;
+      kernel_multiply<false>(row + 6 * (upper ? C::L - kz : kz), sy, upper ? -1.f : 1.f, v[0][q], v[1][q], v[2][q]);
+    }
   }
   line_fft<L2, true, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
   if (valid) {
@@ -331,7 +344,7 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
 #pragma unroll
       for (int q = 0; q < C::EH; ++q) {
         const int z = t + C::T * q;
-        if (z < d.nz) B[((size_t)c * cs + pt) * d.nz + z] = v[c][q];
+        if (z < d.nz) B[((size_t)c * cs + line0) * d.nz + z] = v[c][q];
       }
   }
 }
@@ -363,8 +376,13 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   float2* line = lds + g * K::LP;
   line_fft<L2, false, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
   if (valid) {
+    const float* row = Tk + (size_t)kx * d.Hy * 6;  // Hz == 1
 #pragma unroll
-    for (int q = 0; q < C::E; ++q) kernel_multiply<true>(Tk, d, kx, t + C::T * q, 0, v[0][q], v[1][q], v[2][q]);
+    for (int q = 0; q < C::E; ++q) {
+      const int ky = t + C::T * q;
+      const bool upper = q >= C::EH;  // ky >= Ky/2: mirror with the odd-y sign flipped
+      kernel_multiply<true>(row + 6 * (upper ? C::L - ky : ky), upper ? -1.f : 1.f, 1.f, v[0][q], v[1][q], v[2][q]);
+    }
   }
   line_fft<L2, true, 3, K::NBUF>(v, t, line, K::PT * K::LP, tw);
   if (valid) {
@@ -383,7 +401,8 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
 template <int L2>
 struct XinvCfg {
   using C = Line<L2>;
-  static constexpr int PB = clampc(256 / C::T, 1, 32);  // line pairs per block
+  // line pairs per block: 16 lines give 128-byte runs on the strided A side
+  static constexpr int PB = clampc(512 / C::T, 1, 32);
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
   static constexpr int XS = C::H + 1;  // stride of a staged half spectrum
@@ -407,7 +426,7 @@ __global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
   constexpr int NL = 2 * K::PB;
   float2* X = lds;
   float2* Z = lds;  // reused once the half spectra are in registers
-  const int l0 = 2 * blockIdx.x * K::PB;
+  const int l0 = 2 * xcd_remap(blockIdx.x, gridDim.x) * K::PB;
   for (int i = threadIdx.x; i < NL; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   __syncthreads();
   for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
@@ -497,11 +516,13 @@ __global__ void k_finalize(const double* __restrict__ in, float* __restrict__ T,
   const long long total = (long long)6 * Hx * Hy * Hz;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
-  // output index ((c*Hx + kx)*Hy + ky)*Hz + kz
-  const int kz = (int)(t % Hz);
-  const int ky = (int)((t / Hz) % Hy);
-  const int kx = (int)((t / ((long long)Hz * Hy)) % Hx);
-  const int c = (int)(t / ((long long)Hz * Hy * Hx));
+  // output index ((kx*Hy + ky)*Hz + kz)*6 + c: the six components of an
+  // octant entry are adjacent
+  const int c = (int)(t % 6);
+  const long long e = t / 6;
+  const int kz = (int)(e % Hz);
+  const int ky = (int)((e / Hz) % Hy);
+  const int kx = (int)(e / ((long long)Hz * Hy));
   const double sign = (c >= 3) ? -1.0 : 1.0;  // (-i)^2 of the two odd axes
   const double v = in[(((size_t)c * Hz + kz) * Hy + ky) * Hx + kx];
   T[t] = (float)(sign * pref * v);

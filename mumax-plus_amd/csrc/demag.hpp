@@ -39,7 +39,8 @@ class DemagPlan {
   // Algorithmic HBM bytes of one exec (see the byte model in the design notes).
   double algorithmic_bytes() const;
   size_t device_bytes() const;
-  // Real kernel spectrum table (6 x Hx x Hy x Hz floats, pre-scaled), for tests.
+  // Real kernel spectrum table (Hx x Hy x Hz entries of 6 floats xx, yy, zz,
+  // xy, xz, yz; pre-scaled), for tests.
   void copy_spectrum(float* host) const;
   int spectrum_dims(int a) const { return a == 0 ? Hx_ : (a == 1 ? Hy_ : Hz_); }
 

@@ -236,6 +236,9 @@ struct FMUniform {
   float hext[3];
   float llg, alpha;  // -gamma/(1+alpha^2)
   int has_ex, has_dmi, uni, cubic, demag, frozen, openbc_dmi, openbc;
+  // one layer (Nz == 1) and no DMI entries along z: the exchange and DMI
+  // terms of the two missing (or self-wrapped) z neighbours cancel
+  int zflat;
 };
 
 __device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m) { return dmi_gamma(u.D, axis, m); }
@@ -243,10 +246,17 @@ __device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m
 __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                          const float* __restrict__ hdemag, float* __restrict__ out,
                                                          int mode) {
-  // 2-D blocks: threadIdx.x along x, threadIdx.y along y; blockIdx.z = z
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
+  // 2-D blocks: threadIdx.x along x, threadIdx.y along y, one z plane per
+  // block row.  Workgroups are dealt round-robin to the 8 XCDs in dispatch
+  // order; the remap gives each XCD a contiguous run of tiles, so the y and z
+  // neighbour rows a tile reads are mostly in that XCD's L2 already.
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const int tile = xcd_remap(lin, nbx * nby * gridDim.z);
+  const int bxi = tile % nbx, rest = tile / nbx;
+  const int x = bxi * blockDim.x + threadIdx.x;
+  const int y = (rest % nby) * blockDim.y + threadIdx.y;
+  const int z = rest / nby;
   if (x >= d.nx || y >= d.ny) return;
   const int N = d.ncell;
   const int idx = (z * d.ny + y) * d.nx + x;
@@ -258,29 +268,52 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   }
   const float3 mc = load3(m, idx, N);
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
-  if (u.has_ex || u.has_dmi) {
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-      const int axis = dir >> 1;
-      const float s = (dir & 1) ? 1.f : -1.f;
-      const int nb = neighbour(d, x, y, z, dir);
-      const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
-      if (nb_in) {
-        const float3 mn = load3(m, nb, N);
-        if (u.has_ex) hex = hex + u.kex[axis] * (mn - mc);
-        if (u.has_dmi)
-          hdm = hdm + s * f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
-                             -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
-      } else {
-        const float3 gam = gamma_u(u, axis, mc);
-        if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
-        if (u.has_dmi && !u.openbc_dmi) {
-          const float3 mn = mc + (s * u.ncoef[axis]) * gam;
-          hdm = hdm + s * f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
-                             -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
-        }
-      }
+  // DMI field of neighbour value mn along `axis`, before the direction sign
+  auto dmi_of = [&](int axis, float3 mn) {
+    return f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
+              -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
+  };
+  // One neighbour direction with the full boundary rules (wrap, geometry,
+  // Neumann with the DMI correction, open boundary).
+  auto general = [&](int dir) {
+    const int axis = dir >> 1;
+    const float s = (dir & 1) ? 1.f : -1.f;
+    const int nb = neighbour(d, x, y, z, dir);
+    const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+    if (nb_in) {
+      const float3 mn = load3(m, nb, N);
+      if (u.has_ex) hex = hex + u.kex[axis] * (mn - mc);
+      if (u.has_dmi) hdm = hdm + s * dmi_of(axis, mn);
+    } else {
+      const float3 gam = gamma_u(u, axis, mc);
+      if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
+      if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
     }
+  };
+  // Both neighbours along `axis` present and plain: the two exchange terms
+  // combine to kex (m+ + m- - 2m), the two DMI terms to D (m+ - m-).
+  auto pair = [&](int axis, int stride) {
+    const float3 mm = load3(m, idx - stride, N), mp = load3(m, idx + stride, N);
+    if (u.has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
+    if (u.has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
+  };
+  if (u.has_ex || u.has_dmi) {
+    // x keeps the general rules in every lane: the two edge lanes of a row
+    // would otherwise make their waves run both paths for all axes
+    general(0);
+    general(1);
+    if (!d.geom && y > 0 && y < d.ny - 1) {
+      pair(1, d.nx);
+    } else {
+      general(2);
+      general(3);
+    }
+    if (!d.geom && z > 0 && z < d.nz - 1) {
+      pair(2, d.nx * d.ny);
+    } else if (!u.zflat) {
+      general(4);
+      general(5);
+    }  // single layer without z-axis DMI entries: the two z terms cancel
   }
   float3 han = f3(0.f, 0.f, 0.f);
   if (u.uni) {
@@ -374,6 +407,7 @@ FMUniform make_uniform(const FMDesc& d) {
     u.hext[2] = (float)((double)d.wb[2] + d.bias.z);
   }
   u.demag = (d.terms & TERM_DEMAG) != 0;
+  u.zflat = d.nz == 1 && d.dmi[D_ZXY].v == 0.f && d.dmi[D_ZYZ].v == 0.f && d.dmi[D_ZXZ].v == 0.f;
   const double al = d.alpha.v;
   u.alpha = (float)al;
   u.llg = (float)(-1.7595e11 / (1.0 + al * al));

@@ -234,7 +234,23 @@ void Ferromagnet::ensure_stage_buffers(int S) {
   while ((int)k.size() < S) k.emplace_back((size_t)3 * N);
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s) {
+// evalExternalField (zeeman.cu:70-88) adds the stray field of every other
+// magnet that is a source (unless its msat is assured zero,
+// strayfield.cpp:88-95) to a destination magnet.  That coupling is not part
+// of this build, so an evaluation it would change is refused rather than
+// computed without it.
+void Ferromagnet::check_stray_coupling() const {
+  if (!stray_destination) return;
+  for (const auto& o : world->magnets) {
+    if (o.get() == this || !o->stray_source || o->params.at("msat").assured_zero()) continue;
+    throw Error(MPA_ERR_RUNTIME, "the stray field of magnet '" + o->name + "' on magnet '" + name +
+                                     "' is not implemented; set enable_as_stray_field_source or "
+                                     "enable_as_stray_field_destination to False");
+  }
+}
+
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray) {
+  if (check_stray) check_stray_coupling();
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
   if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s, prof);
@@ -282,6 +298,7 @@ const float* Ferromagnet::eval_device(const std::string& q, hipStream_t s, int* 
   const QInfo& qi = qinfo(q);
   *ncomp = qi.ncomp;
   if (qi.mode < 0) return m.get();
+  if (qi.terms & TERM_EXT) check_stray_coupling();
   const unsigned terms = active_terms() & qi.terms;
   if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s);
   launch_field(desc(terms), m.get(), hdemag.get(), scratch.get(), (FieldMode)qi.mode, s);
@@ -374,7 +391,10 @@ double TimeSolver::sensible_timestep() {
   if (world_->magnets.empty()) return 0.0;
   double mx = 0.0;
   for (auto& mag : world_->magnets) {
-    mag->eval_rhs(mag->scratch.get(), world_->stream);
+    // The reference re-estimates dt whenever a magnet is added
+    // (timesolver.cpp:49-52), before the caller can switch stray coupling
+    // off; the estimate runs without the refusal, the steps keep it.
+    mag->eval_rhs(mag->scratch.get(), world_->stream, false);
     ++nrhs;
     mx = std::max(mx, (double)world_->reducer.max_vec_norm(mag->scratch.get(), mag->N, mag->geom_ptr(), world_->stream));
   }
@@ -513,7 +533,10 @@ void TimeSolver::run(double duration) {
 World::World(const double cs[3], const int mg_size[3], const int mg_origin[3], const int reps[3]) : solver(this) {
   for (int a = 0; a < 3; ++a) {
     if (!(cs[a] > 0)) throw Error(MPA_ERR_INVALID, "cellsize should be positive");
-    cellsize[a] = cs[a];
+    // the reference keeps the cell size as FP32 `real3` (world.hpp:73,
+    // datatypes.hpp:15): geometry callables, stencils and the Newell kernel
+    // all see the rounded value
+    cellsize[a] = (double)(float)cs[a];
     mastergrid_size[a] = mg_size[a];
     mastergrid_origin[a] = mg_origin[a];
     pbc[a] = reps[a];

@@ -58,6 +58,8 @@ class Ferromagnet {
   std::map<std::string, Param> params;
   bool enable_demag = true;
   bool openbc = false;
+  bool stray_source = true;       // Magnet::enableAsStrayFieldSource (magnet.cpp:26-34)
+  bool stray_destination = true;  // Magnet::enableAsStrayFieldDestination
 
   DeviceArray<float> m;       // magnetisation, 3N
   DeviceArray<float> x0;      // RK start state
@@ -68,13 +70,16 @@ class Ferromagnet {
   const uint8_t* geom_ptr() const { return geom.get(); }
   Param& param(const std::string& key);
   unsigned active_terms() const;  // the reference's "assured zero" rules
+  // Throws when another magnet's stray field would enter this magnet's
+  // external field (not built); called by every evaluation that includes it.
+  void check_stray_coupling() const;
   FMDesc desc(unsigned terms) const;
   DemagPlan& demag();
   void invalidate_demag() { demag_.reset(); }
   void ensure_stage_buffers(int S);
 
   // out (device, 3N) = torque of the current magnetisation
-  void eval_rhs(float* out, hipStream_t s);
+  void eval_rhs(float* out, hipStream_t s, bool check_stray = true);
   // Evaluate a named quantity into host memory (ncomp*N floats); returns ncomp.
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;

@@ -459,6 +459,22 @@ class Ferromagnet:
         _check(_lib.mpa_fm_set_flag(self._h, b"enable_openbc", int(bool(value))))
 
     @property
+    def enable_as_stray_field_source(self):
+        return self._flag("enable_as_stray_field_source")
+
+    @enable_as_stray_field_source.setter
+    def enable_as_stray_field_source(self, value):
+        _check(_lib.mpa_fm_set_flag(self._h, b"enable_as_stray_field_source", int(bool(value))))
+
+    @property
+    def enable_as_stray_field_destination(self):
+        return self._flag("enable_as_stray_field_destination")
+
+    @enable_as_stray_field_destination.setter
+    def enable_as_stray_field_destination(self, value):
+        _check(_lib.mpa_fm_set_flag(self._h, b"enable_as_stray_field_destination", int(bool(value))))
+
+    @property
     def is_sublattice(self):
         return False
 

@@ -125,6 +125,26 @@ class Ferromagnet:
         self._impl.enable_openbc = value
 
     @property
+    def enable_as_stray_field_source(self):
+        """Whether this magnet's stray field acts on other magnets
+        (magnet.py of the reference).  Coupling between magnets is not built:
+        evaluating a field it would enter raises RuntimeError."""
+        return self._impl.enable_as_stray_field_source
+
+    @enable_as_stray_field_source.setter
+    def enable_as_stray_field_source(self, value):
+        self._impl.enable_as_stray_field_source = value
+
+    @property
+    def enable_as_stray_field_destination(self):
+        """Whether stray fields of other magnets act on this magnet."""
+        return self._impl.enable_as_stray_field_destination
+
+    @enable_as_stray_field_destination.setter
+    def enable_as_stray_field_destination(self, value):
+        self._impl.enable_as_stray_field_destination = value
+
+    @property
     def dmi_tensor(self):
         return DmiTensor(self._impl.dmi_tensor)
 

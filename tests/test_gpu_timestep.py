@@ -50,6 +50,9 @@ def test_two_uncoupled_spins(method):
     for i in (0, 1):
         m = mp.Ferromagnet(world, grid=mp.Grid((1, 1, 1), origin=(i, 0, 0)))
         m.enable_demag = False
+        # the reference couples the two through their (negligible, Ms = 1 A/m)
+        # stray fields; coupling between magnets is not built here
+        m.enable_as_stray_field_source = False
         m.magnetization = (1, 0, 0)
         m.alpha = 0.2
         mags.append(m)
@@ -140,3 +143,52 @@ def test_rejection_path_and_errors():
         ts.set_method("asdf")
     with pytest.raises((TypeError, ValueError)):
         ts.set_method(None)
+
+
+def test_film128x32_fixed_step_matches_fp64_fixture():
+    """128x32x1 film, exchange + demag + 25 mT bias, Dormand-Prince at a fixed
+    1e-13 s for 10000 steps (1 ns): <m> every 1000 steps within 1e-4 of the
+    FP64 oracle trajectory stored in tests/golden (make_film128x32.py)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "film128x32_dp_fixed.npz"))
+    world = mp.World(cellsize=(500e-9 / 128, 125e-9 / 32, 3e-9))
+    mag = mp.Ferromagnet(world, mp.Grid((128, 32, 1)))
+    mag.msat = 8e5
+    mag.aex = 1.3e-11
+    mag.alpha = 0.02
+    world.bias_magnetic_field = tuple(float(v) for v in g["bias"])
+    mag.magnetization = g["m0"]  # FP32 unit vectors; renormalising moves them by at most an ulp
+    ts = world.timesolver
+    ts.set_method("DormandPrince")
+    ts.adaptive_timestep = False
+    ts.timestep = float(g["dt"])
+    errs = []
+    for want in g["averages"]:
+        ts.steps(int(g["check_every"]))
+        errs.append(float(np.abs(np.array(mag.magnetization.average()) - want).max()))
+    assert max(errs) < 1e-4, errs
+    assert abs(ts.time - float(g["time"])) <= 1e-9 * float(g["time"])
+
+
+def test_stray_field_coupling_is_refused():
+    """Two magnets with non-zero msat would couple through their stray fields
+    (zeeman.cu:70-88); that is not built, so evaluations that include the
+    external field raise instead of silently leaving it out."""
+    world = mp.World(cellsize=(2e-9, 2e-9, 2e-9))
+    a = mp.Ferromagnet(world, mp.Grid((4, 4, 1)))
+    b = mp.Ferromagnet(world, mp.Grid((4, 4, 1), origin=(6, 0, 0)))
+    for mag in (a, b):
+        mag.msat = 8e5
+        mag.aex = 1e-11
+    with pytest.raises(RuntimeError):
+        a.effective_field.eval()
+    with pytest.raises(RuntimeError):
+        world.timesolver.steps(1)
+    a.exchange_field.eval()  # does not involve the external field
+    b.enable_as_stray_field_source = False
+    a.effective_field.eval()
+    with pytest.raises(RuntimeError):
+        b.effective_field.eval()  # a still acts on b
+    b.enable_as_stray_field_destination = False
+    world.timesolver.steps(2)
+    assert a.enable_as_stray_field_source and not b.enable_as_stray_field_destination
