@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   extern __shared__ float2 lds[];
   __shared__ long long abase[2 * K::PB];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int l0 = 2 * K::PB * xcd_remap(blockIdx.x, gridDim.x);
+  const int l0 = 2 * K::PB * blockIdx.x;
   for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
