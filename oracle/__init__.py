@@ -20,10 +20,17 @@ Modules and the reference files each one restates:
 * ``rk``      -- Butcher tableaus (src/core/butchertableau.cpp:89-224), the
   Runge-Kutta stepper (src/core/rungekutta.cpp:25-151) and the time solver
   (src/core/timesolver.cpp:37-111, mumaxplus/timesolver.py:61-152).
-* ``model``   -- a minimal world / ferromagnet container that assembles the
-  above the way src/physics/mumaxworld.cpp:159-205 does.
+* ``afm``     -- antiferromagnet sublattice terms: sister-aware exchange
+  (src/physics/exchange.cu:106-202), AFM DMI (dmi.cu:112-240), homogeneous and
+  inhomogeneous AFM exchange (afmexchange.cu:28-149).
+* ``elastic`` -- strain, stress, elastic force, magnetoelastic field and force,
+  elastic damping (straintensor.cu, stresstensor.cu, elasticforce.cu,
+  magnetoelasticfield.cu, magnetoelasticforce.cu, elasticdamping.cu).
+* ``model``   -- a minimal world / ferromagnet / antiferromagnet container
+  that assembles the above the way src/physics/mumaxworld.cpp:159-205 and
+  elastodynamics.cu:23-96 do.
 
 Pinning status of every function is recorded in each module's docstring.
 """
 
-from . import newell, demag, fields, rk, model  # noqa: F401
+from . import newell, demag, fields, rk, afm, elastic, model  # noqa: F401

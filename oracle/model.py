@@ -15,19 +15,64 @@ Parameters are either Python scalars (uniform) or arrays of shape
 
 import numpy as np
 
+from . import afm as _afm
 from . import demag as _demag
+from . import elastic as _elastic
 from . import fields as _fields
 from . import rk as _rk
 
 DMI_KEYS = ("xxy", "xyz", "xxz", "yxy", "yyz", "yxz", "zxy", "zyz", "zxz")
 
 SCALAR_DEFAULTS = {"msat": 1.0, "aex": 0.0, "alpha": 0.0, "ku1": 0.0, "ku2": 0.0,
-                   "kc1": 0.0, "kc2": 0.0, "kc3": 0.0, "frozen_spins": 0.0}
+                   "kc1": 0.0, "kc2": 0.0, "kc3": 0.0, "frozen_spins": 0.0,
+                   # magnetoelastic constants (ferromagnet.cpp:24-61)
+                   "B1": 0.0, "B2": 0.0,
+                   # elastic parameters of an independent magnet (magnet.cpp:26-34)
+                   "C11": 0.0, "C12": 0.0, "C44": 0.0, "eta": 0.0, "rho": 1.0}
 VECTOR_DEFAULTS = {"anisU": (0.0, 0.0, 0.0), "anisC1": (0.0, 0.0, 0.0),
-                   "anisC2": (0.0, 0.0, 0.0), "bias": (0.0, 0.0, 0.0)}
+                   "anisC2": (0.0, 0.0, 0.0), "bias": (0.0, 0.0, 0.0),
+                   "external_body_force": (0.0, 0.0, 0.0), "rigid_norm_strain": (0.0, 0.0, 0.0),
+                   "rigid_shear_strain": (0.0, 0.0, 0.0)}
 
 
-class Ferromagnet:
+class _Params:
+    """Uniform (scalar or 3-vector) or per-cell parameters plus a DMI tensor."""
+
+    def set(self, name, value):
+        if name in DMI_KEYS:
+            self.dmi[name] = value
+        else:
+            self.params[name] = value
+
+    def val(self, name):
+        v = self.dmi[name] if name in DMI_KEYS else self.params[name]
+        return np.broadcast_to(np.asarray(v, dtype=float), self.shape)
+
+    def vec(self, name):
+        v = np.asarray(self.params[name], dtype=float)
+        if v.shape == (3,):
+            return np.broadcast_to(v[:, None, None, None], (3,) + self.shape)
+        return np.broadcast_to(v, (3,) + self.shape)
+
+    def assured_zero(self, name):
+        v = self.dmi[name] if name in DMI_KEYS else self.params[name]
+        a = np.asarray(v, dtype=float)
+        return a.ndim == 0 and a == 0 or (a.shape == (3,) and not a.any())
+
+    def dmi_arrays(self):
+        return {k: self.val(k) for k in DMI_KEYS}
+
+    # elasticity of the magnet that owns the displacement (magnet.cpp:26-34)
+    def elasticity_assured_zero(self):
+        """elastodynamics.cu:47-51."""
+        return (not self.enable_elastodynamics
+                or (self.assured_zero("C11") and self.assured_zero("C12") and self.assured_zero("C44")))
+
+    def rigid_strain_applied(self):
+        return not (self.assured_zero("rigid_norm_strain") and self.assured_zero("rigid_shear_strain"))
+
+
+class Ferromagnet(_Params):
     def __init__(self, size, cellsize, origin=(0, 0, 0), geometry=None,
                  mastergrid=(0, 0, 0), mastergrid_origin=(0, 0, 0), pbc=(0, 0, 0)):
         self.size = tuple(int(v) for v in size)
@@ -48,33 +93,19 @@ class Ferromagnet:
         self.m[0] = 1.0
         self._plan = None
         self._nb_cache = {}
+        # sublattice of an antiferromagnet: the host holds the AFM and elastic state
+        self.host = None
+        # elastodynamics of an independent magnet (Magnet::setEnableElastodynamics,
+        # magnet.cpp:177-213): displacement u and velocity v, zero when enabled
+        self.enable_elastodynamics = False
+        self.u = np.zeros((3,) + self.shape)
+        self.v = np.zeros((3,) + self.shape)
 
     # ----------------------------------------------------------- parameters
     def set(self, name, value):
-        if name in DMI_KEYS:
-            self.dmi[name] = value
-        else:
-            self.params[name] = value
+        super().set(name, value)
         if name == "msat":
             self._plan = None
-
-    def val(self, name):
-        v = self.dmi[name] if name in DMI_KEYS else self.params[name]
-        return np.broadcast_to(np.asarray(v, dtype=float), self.shape)
-
-    def vec(self, name):
-        v = np.asarray(self.params[name], dtype=float)
-        if v.shape == (3,):
-            return np.broadcast_to(v[:, None, None, None], (3,) + self.shape)
-        return np.broadcast_to(v, (3,) + self.shape)
-
-    def assured_zero(self, name):
-        v = self.dmi[name] if name in DMI_KEYS else self.params[name]
-        a = np.asarray(v, dtype=float)
-        return a.ndim == 0 and a == 0 or (a.shape == (3,) and not a.any())
-
-    def dmi_arrays(self):
-        return {k: self.val(k) for k in DMI_KEYS}
 
     def exchange_assured_zero(self):
         return self.assured_zero("aex") or self.assured_zero("msat")
@@ -145,15 +176,65 @@ def normalize(m, geom):
     return np.where(geom[None], out, 0.0)
 
 
+AFM_DEFAULTS = {"afmex_cell": 0.0, "afmex_nn": 0.0, "latcon": 0.35e-9,
+                "C11": 0.0, "C12": 0.0, "C44": 0.0, "eta": 0.0, "rho": 1.0,
+                "external_body_force": (0.0, 0.0, 0.0), "rigid_norm_strain": (0.0, 0.0, 0.0),
+                "rigid_shear_strain": (0.0, 0.0, 0.0)}
+
+
+class Antiferromagnet(_Params):
+    """Two ferromagnetic sublattices on one grid (antiferromagnet.cpp:15-28):
+    host parameters afmex_cell, afmex_nn, latcon (default 0.35 nm), an
+    inter-sublattice DMI tensor, and the elastic state of the magnet."""
+
+    def __init__(self, size, cellsize, origin=(0, 0, 0), geometry=None,
+                 mastergrid=(0, 0, 0), mastergrid_origin=(0, 0, 0), pbc=(0, 0, 0)):
+        self.sub1 = Ferromagnet(size, cellsize, origin, geometry, mastergrid, mastergrid_origin, pbc)
+        self.sub2 = Ferromagnet(size, cellsize, origin, geometry, mastergrid, mastergrid_origin, pbc)
+        for sub in (self.sub1, self.sub2):
+            sub.host = self
+        self.shape = self.sub1.shape
+        self.cellsize = self.sub1.cellsize
+        self.params = dict(AFM_DEFAULTS)
+        self.dmi = {k: 0.0 for k in DMI_KEYS}
+        self.enable_elastodynamics = False
+        self.u = np.zeros((3,) + self.shape)
+        self.v = np.zeros((3,) + self.shape)
+        self._plan = None
+
+    @property
+    def sublattices(self):
+        return (self.sub1, self.sub2)
+
+    def neighbor(self, d):
+        return self.sub1.neighbor(d)
+
+    def geom_mask(self):
+        return self.sub1.geom_mask()
+
+    def demag_plan(self):
+        if self._plan is None:
+            s = self.sub1
+            self._plan = _demag.DemagPlan(s.size, s.cellsize, s.mastergrid, s.pbc)
+        return self._plan
+
+
 class World:
-    """World with ferromagnets and a time solver (FP64)."""
+    """World with ferromagnets, antiferromagnets and a time solver (FP64).
+
+    Equations, in the order of MumaxWorld::resetTimeSolverEquations
+    (mumaxworld.cpp:159-205): (m, LLG torque) per ferromagnet, per
+    antiferromagnet sublattice, then (u, v) and (v, a) per magnet with
+    elastodynamics.  Only the magnetisations are normalised when a stage
+    state is set (NormalizedVariable, variable.cpp:70-87)."""
 
     def __init__(self, cellsize, mastergrid=(0, 0, 0), pbc=(0, 0, 0)):
         self.cellsize = tuple(float(c) for c in cellsize)
         self.mastergrid = tuple(mastergrid)
         self.pbc = tuple(pbc)
         self.bias = (0.0, 0.0, 0.0)
-        self.magnets = []
+        self.magnets = []  # independent ferromagnets
+        self.afms = []
         self.solver = _rk.TimeSolver(self)
 
     def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
@@ -163,20 +244,144 @@ class World:
             self.solver.equations_changed()
         return mag
 
-    def torque(self, mag, m):
-        h = _fields.effective_field(mag, m, self.bias)
-        return _fields.llg_torque(mag, m, h)
+    def add_antiferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
+        afm = Antiferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
+        self.afms.append(afm)
+        if not defer_timestep:
+            self.solver.equations_changed()
+        return afm
 
-    # interface used by rk.TimeSolver
+    # -------------------------------------------------------------- physics
+    def effective_field(self, mag, m, m2=None, u=None):
+        """evalEffectiveField (effectivefield.cpp:15-28) of a ferromagnet or a
+        sublattice; m2 is the sister sublattice's magnetisation, u the
+        displacement of the magnet that owns the elastic state."""
+        host = mag.host
+        if host is not None and not mag.openbc:
+            h = _afm.exchange_field_sublattice(mag, m, m2)
+        else:
+            h = _fields.exchange_field(mag, m)
+        h = h + _fields.anisotropy_field(mag, m)
+        if not _fields.external_assured_zero(mag, self.bias):
+            h = h + _fields.external_field(mag, m, self.bias)
+        if host is None:
+            h = h + _fields.dmi_field(mag, m)
+        else:
+            h = h + _afm.dmi_field_afm(mag, m, m2)
+        if mag.enable_demag and not mag.assured_zero("msat"):
+            if host is None:
+                h = h + _fields.demag_field(mag, m)
+            else:
+                s1, s2 = host.sub1, host.sub2
+                m1_, m2_ = (m, m2) if mag is s1 else (m2, m)
+                M = s1.val("msat") * m1_ + s2.val("msat") * m2_
+                h = h + host.demag_plan().field(M, np.ones(host.shape), host.geom_mask())
+        h = h + self.magnetoelastic_field(mag, m, u)
+        if host is not None:
+            if not _afm.inhomo_assured_zero(mag):
+                h = h + _afm.afm_nn_field(mag, m, m2)
+            if not _afm.homo_assured_zero(mag):
+                h = h + _afm.afm_site_field(mag, m2)
+        return h
+
+    @staticmethod
+    def _owner(mag):
+        return mag.host if mag.host is not None else mag
+
+    def magnetoelastic_field(self, mag, m, u):
+        """evalMagnetoelasticField (magnetoelasticfield.cu:10-42,105-150)."""
+        own = self._owner(mag)
+        no_b = mag.assured_zero("msat") or (mag.assured_zero("B1") and mag.assured_zero("B2"))
+        if not no_b and own.rigid_strain_applied():
+            return _elastic.magnetoelastic_field_rigid(mag, m, own.vec("rigid_norm_strain"),
+                                                       own.vec("rigid_shear_strain"))
+        if no_b or not own.enable_elastodynamics:
+            return np.zeros_like(m)
+        return _elastic.magnetoelastic_field_dynamic(mag, m, _elastic.strain_tensor(own, u))
+
+    @staticmethod
+    def magnetoelastic_assured_zero(fm):
+        own = fm.host if fm.host is not None else fm
+        no_b = fm.assured_zero("msat") or (fm.assured_zero("B1") and fm.assured_zero("B2"))
+        return no_b or not (own.enable_elastodynamics or own.rigid_strain_applied())
+
+    def body_force(self, own, u, ms):
+        """evalEffectiveBodyForce (elastodynamics.cu:23-45); ms: magnetisation
+        of each ferromagnet of the magnet (one, or the two sublattices)."""
+        f = np.zeros((3,) + own.shape)
+        if not own.elasticity_assured_zero():
+            f = f + _elastic.elastic_force(own, _elastic.stress_tensor(own, _elastic.strain_tensor(own, u)))
+        if not own.assured_zero("external_body_force"):
+            f = f + own.vec("external_body_force")
+        fms = own.sublattices if isinstance(own, Antiferromagnet) else (own,)
+        for fm, m in zip(fms, ms):
+            if not self.magnetoelastic_assured_zero(fm):
+                f = f + _elastic.magnetoelastic_force(fm, m)
+        return f
+
+    def acceleration(self, own, u, v, ms):
+        """evalElasticAcceleration (elastodynamics.cu:54-84)."""
+        a = self.body_force(own, u, ms)
+        if own.enable_elastodynamics and not own.assured_zero("eta"):
+            a = a + _elastic.elastic_damping(own, v)
+        geom = own.geom_mask()
+        if not own.assured_zero("rho"):
+            rho = own.val("rho")
+            with np.errstate(divide="ignore", invalid="ignore"):
+                a = np.where(rho != 0, a / rho, 0.0)
+        return np.where(geom[None], a, 0.0)
+
+    # ----------------------------------------------------- equation plumbing
+    def _equations(self):
+        """[(kind, object)]: 'm' for a ferromagnet or sublattice, 'u'/'v' for
+        the elastic state of a magnet."""
+        eqs = [("m", mag) for mag in self.magnets]
+        for afm in self.afms:
+            eqs += [("m", afm.sub1), ("m", afm.sub2)]
+        for own in self.magnets + self.afms:
+            if own.enable_elastodynamics:
+                eqs += [("u", own), ("v", own)]
+        return eqs
+
     def get_states(self):
-        return [mag.m for mag in self.magnets]
+        return [getattr(obj, kind) for kind, obj in self._equations()]
 
     def set_states(self, states):
-        for mag, x in zip(self.magnets, states):
-            mag.m = normalize(x, mag.geom_mask())
+        for (kind, obj), x in zip(self._equations(), states):
+            if kind == "m":
+                obj.m = normalize(x, obj.geom_mask())
+            else:
+                setattr(obj, kind, np.array(x))
 
     def rhs(self, states):
-        return [self.torque(mag, x) for mag, x in zip(self.magnets, states)]
+        eqs = self._equations()
+        cur = {(id(obj), kind): x for (kind, obj), x in zip(eqs, states)}
+
+        def m_of(fm):
+            return cur.get((id(fm), "m"), fm.m)
+
+        def uv_of(own, kind):
+            return cur.get((id(own), kind), getattr(own, kind))
+
+        out = []
+        for kind, obj in eqs:
+            if kind == "m":
+                own = self._owner(obj)
+                m2 = None
+                if obj.host is not None:
+                    m2 = m_of(obj.host.sub2 if obj is obj.host.sub1 else obj.host.sub1)
+                h = self.effective_field(obj, m_of(obj), m2, uv_of(own, "u"))
+                out.append(_fields.llg_torque(obj, m_of(obj), h))
+            elif kind == "u":
+                out.append(uv_of(obj, "v"))
+            else:
+                fms = obj.sublattices if isinstance(obj, Antiferromagnet) else (obj,)
+                out.append(self.acceleration(obj, uv_of(obj, "u"), uv_of(obj, "v"), [m_of(f) for f in fms]))
+        return out
 
     def geom_masks(self):
-        return [mag.geom_mask() for mag in self.magnets]
+        return [obj.geom_mask() for kind, obj in self._equations()]
+
+    def torque(self, mag, m):
+        """LLG torque of an independent ferromagnet at magnetisation m."""
+        return _fields.llg_torque(mag, m, self.effective_field(mag, m, None, mag.u))
