@@ -32,6 +32,7 @@ extern "C" {
 
 typedef struct mpa_world mpa_world;
 typedef struct mpa_ferromagnet mpa_ferromagnet;
+typedef struct mpa_antiferromagnet mpa_antiferromagnet;
 typedef struct mpa_demag_plan mpa_demag_plan;
 
 /* Message of the last error on this thread ("" if none). */
@@ -62,6 +63,15 @@ int mpa_world_num_ferromagnets(const mpa_world* w, int* out);
 int mpa_world_ferromagnet_at(mpa_world* w, int index, mpa_ferromagnet** out);
 /* World.get_ferromagnet(name) (wrap_world.cpp:79-81); *out = NULL when absent. */
 int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet** out);
+/* World.add_antiferromagnet(grid, geometry_array, regions_array, name): wrap_world.cpp:66-76,
+ * MumaxWorld::addAntiferromagnet mumaxworld.cpp:82-103.  Its two sublattices are ferromagnet
+ * handles (mpa_afm_sublattice) named "<name>:sublattice_1" / "<name>:sublattice_2". */
+int mpa_world_add_antiferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
+                                  const char* name, mpa_antiferromagnet** out);
+/* World.antiferromagnets / get_antiferromagnet(name) (wrap_world.cpp:83-91); *out = NULL when absent. */
+int mpa_world_num_antiferromagnets(const mpa_world* w, int* out);
+int mpa_world_antiferromagnet_at(mpa_world* w, int index, mpa_antiferromagnet** out);
+int mpa_world_find_antiferromagnet(mpa_world* w, const char* name, mpa_antiferromagnet** out);
 /* World.set_pbc(mastergrid, reps) / set_pbc(reps) / unset_pbc / mastergrid / pbc_repetitions
  * (wrap_world.cpp:100-113, mumaxworld.cpp:288-342). */
 int mpa_world_set_pbc(mpa_world* w, const int mastergrid_size[3], const int mastergrid_origin[3],
@@ -82,15 +92,21 @@ int mpa_fm_grid(const mpa_ferromagnet* m, int size[3], int origin[3]);
 int mpa_fm_geometry(const mpa_ferromagnet* m, uint8_t* out); /* ncells bytes */
 int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsingeo system.cpp:80-83 */
 /* Ferromagnet.enable_demag / enable_openbc (wrap_ferromagnet.cpp:33-38) and
- * Magnet.enable_as_stray_field_source / _destination (wrap_magnet.cpp:17-54). flag: "enable_demag",
- * "enable_openbc", "enable_as_stray_field_source", "enable_as_stray_field_destination".  Stray
- * fields between magnets are not built: evaluating a field they would enter fails with
- * MPA_ERR_RUNTIME while both flags are set and the other magnet's msat is not zero. */
+ * Magnet.enable_as_stray_field_source / _destination / enable_elastodynamics (wrap_magnet.cpp:26-33,
+ * Magnet::setEnableElastodynamics magnet.cpp:177-213). flag: "enable_demag", "enable_openbc",
+ * "enable_as_stray_field_source", "enable_as_stray_field_destination", "enable_elastodynamics"
+ * (not for a sublattice, nor while a rigid strain is set).  Stray fields between magnets are not
+ * built: evaluating a field they would enter fails with MPA_ERR_RUNTIME while the source's and
+ * the destination's flags are set (a sublattice uses its antiferromagnet's) and the source's
+ * msat is not zero. */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
 /* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,
  * alpha, ku1, ku2, kc1, kc2, kc3, frozen_spins, anisU, anisC1, anisC2, bias_magnetic_field,
- * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz. */
+ * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz, B1, B2
+ * (wrap_ferromagnet.cpp:41-70); an independent ferromagnet also has the elastic parameters C11,
+ * C12, C44, eta, rho, external_body_force, rigid_norm_strain, rigid_shear_strain
+ * (wrap_magnet.cpp:35-43). */
 int mpa_fm_param_info(const mpa_ferromagnet* m, const char* key, int* ncomp, int* is_uniform, char* name,
                       int namelen, char* unit, int unitlen);
 int mpa_fm_param_set_uniform(mpa_ferromagnet* m, const char* key, const double* value /* ncomp */);
@@ -102,19 +118,71 @@ int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* 
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count);
 int mpa_fm_get_magnetization(mpa_ferromagnet* m, float* out);
 /* FieldQuantity.eval / average of the quantity factories (wrap_ferromagnet.cpp:75-128,
- * wrap_fieldquantity.cpp:8-22).  q: magnetization, exchange_field, anisotropy_field,
- * external_field, dmi_field, demag_field, effective_field, torque, llg_torque, damping_torque,
- * exchange_energy_density, anisotropy_energy_density, zeeman_energy_density,
- * dmi_energy_density, demag_energy_density, total_energy_density. */
+ * wrap_magnet.cpp:61-75, wrap_fieldquantity.cpp:8-22).  q: magnetization, full_magnetization,
+ * exchange_field, anisotropy_field, external_field, dmi_field, demag_field, magnetoelastic_field,
+ * homogeneous_exchange_field, inhomogeneous_exchange_field, effective_field, torque, llg_torque,
+ * damping_torque, magnetoelastic_force, exchange_energy_density, anisotropy_energy_density,
+ * zeeman_energy_density, dmi_energy_density, demag_energy_density,
+ * magnetoelastic_energy_density, homogeneous_exchange_energy_density,
+ * inhomogeneous_exchange_energy_density, total_energy_density; for an independent ferromagnet
+ * also the magnet quantities strain_tensor, stress_tensor (6 components xx yy zz xy xz yz),
+ * elastic_force, effective_body_force, elastic_damping, elastic_acceleration, elastic_velocity,
+ * elastic_displacement, kinetic_energy_density, elastic_energy_density, poynting_vector. */
 int mpa_fm_quantity_ncomp(const mpa_ferromagnet* m, const char* q, int* ncomp);
 int mpa_fm_eval(mpa_ferromagnet* m, const char* q, float* out /* ncomp*ncells */);
 int mpa_fm_average(mpa_ferromagnet* m, const char* q, double* out /* ncomp */);
 /* ScalarQuantity.eval: max_torque, exchange_energy, anisotropy_energy, zeeman_energy,
- * dmi_energy, demag_energy, total_energy (wrap_scalarquantity.cpp). */
+ * dmi_energy, demag_energy, magnetoelastic_energy, homogeneous_exchange_energy,
+ * inhomogeneous_exchange_energy, total_energy, and for an independent ferromagnet
+ * kinetic_energy, elastic_energy (wrap_scalarquantity.cpp, wrap_magnet.cpp:70-73). */
 int mpa_fm_scalar(mpa_ferromagnet* m, const char* q, double* out);
+/* Ferromagnet.is_sublattice / hostMagnet (wrap_ferromagnet.cpp:32, ferromagnet.hpp hostMagnet):
+ * *out = the antiferromagnet of a sublattice, NULL for an independent ferromagnet. */
+int mpa_fm_host(mpa_ferromagnet* m, mpa_antiferromagnet** out);
+/* Magnet.elastic_displacement / elastic_velocity Variable.set / get (wrap_magnet.cpp:22-25,
+ * wrap_variable.cpp:8-16): which = "elastic_displacement" or "elastic_velocity"; count 3
+ * (uniform) or 3*ncells; zero outside the geometry.  Needs enable_elastodynamics; not for a
+ * sublattice. */
+int mpa_fm_set_variable(mpa_ferromagnet* m, const char* which, const float* values, long long count);
+int mpa_fm_get_variable(mpa_ferromagnet* m, const char* which, float* out /* 3*ncells */);
 /* _demag_kernel(magnet) (wrap_magnet.cpp:56-59): reference kernel-grid layout (6, Kz, Ky, Kx).
  * Call with out = NULL to get ksize only. */
 int mpa_fm_demag_kernel(mpa_ferromagnet* m, int ksize[3], float* out);
+
+/* ------------------------------------------------------- Antiferromagnet -- */
+
+/* Magnet.name / system.grid / geometry of an antiferromagnet (wrap_magnet.cpp:17-21). */
+int mpa_afm_name(const mpa_antiferromagnet* a, char* buf, int buflen);
+int mpa_afm_grid(const mpa_antiferromagnet* a, int size[3], int origin[3]);
+int mpa_afm_geometry(const mpa_antiferromagnet* a, uint8_t* out);
+int mpa_afm_cellsingeo(const mpa_antiferromagnet* a, int* out);
+/* Antiferromagnet.sub1 / sub2 (wrap_antiferromagnet.cpp:19-20, antiferromagnet.cpp:33-39):
+ * index 0 or 1.  The sublattice handles take every mpa_fm_* call. */
+int mpa_afm_sublattice(mpa_antiferromagnet* a, int index, mpa_ferromagnet** out);
+/* flag: "enable_as_stray_field_source", "enable_as_stray_field_destination",
+ * "enable_elastodynamics" (wrap_magnet.cpp:26-33). */
+int mpa_afm_set_flag(mpa_antiferromagnet* a, const char* flag, int value);
+int mpa_afm_get_flag(const mpa_antiferromagnet* a, const char* flag, int* out);
+/* Antiferromagnet parameters (wrap_antiferromagnet.cpp:24-30, antiferromagnet.cpp:15-25): afmex_cell,
+ * afmex_nn, latcon, and the inter-sublattice DMI tensor dmi_xxy ... dmi_zxz; the elastic
+ * parameters of the magnet (wrap_magnet.cpp:35-43): C11, C12, C44, eta, rho,
+ * external_body_force, rigid_norm_strain, rigid_shear_strain. */
+int mpa_afm_param_info(const mpa_antiferromagnet* a, const char* key, int* ncomp, int* is_uniform, char* name,
+                       int namelen, char* unit, int unitlen);
+int mpa_afm_param_set_uniform(mpa_antiferromagnet* a, const char* key, const double* value);
+int mpa_afm_param_set_cells(mpa_antiferromagnet* a, const char* key, const float* values);
+int mpa_afm_param_get(const mpa_antiferromagnet* a, const char* key, float* out);
+int mpa_afm_param_get_uniform(const mpa_antiferromagnet* a, const char* key, double* out);
+/* Elastic displacement / velocity of the antiferromagnet, as mpa_fm_set_variable. */
+int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count);
+int mpa_afm_get_variable(mpa_antiferromagnet* a, const char* which, float* out);
+/* Quantities of the whole antiferromagnet: neel_vector, full_magnetization (neel.cu, fullmag.cu:13-18),
+ * total_energy_density (energy.cu:70-76) and the magnet quantities listed at mpa_fm_eval. */
+int mpa_afm_quantity_ncomp(const mpa_antiferromagnet* a, const char* q, int* ncomp);
+int mpa_afm_eval(mpa_antiferromagnet* a, const char* q, float* out);
+int mpa_afm_average(mpa_antiferromagnet* a, const char* q, double* out);
+/* total_energy (energy.cu:78-92), kinetic_energy, elastic_energy (elasticenergies.cu:40-95). */
+int mpa_afm_scalar(mpa_antiferromagnet* a, const char* q, double* out);
 
 /* ------------------------------------------------------------ TimeSolver -- */
 

@@ -13,7 +13,8 @@ struct mpa_world {
   World w;
   mpa_world(const double cs[3], const int mgs[3], const int mgo[3], const int reps[3]) : w(cs, mgs, mgo, reps) {}
 };
-struct mpa_ferromagnet {};  // opaque: the pointer is the Ferromagnet itself
+struct mpa_ferromagnet {};      // opaque: the pointer is the Ferromagnet itself
+struct mpa_antiferromagnet {};  // opaque: the pointer is the Antiferromagnet itself
 struct mpa_demag_plan {
   DemagPlan plan;
   explicit mpa_demag_plan(const DemagGeometry& g) : plan(g) {}
@@ -57,11 +58,113 @@ const Ferromagnet* FM(const mpa_ferromagnet* m) {
 }
 mpa_ferromagnet* H(Ferromagnet* f) { return reinterpret_cast<mpa_ferromagnet*>(f); }
 
+Antiferromagnet* AFM(mpa_antiferromagnet* a) {
+  need(a, "antiferromagnet");
+  return reinterpret_cast<Antiferromagnet*>(a);
+}
+const Antiferromagnet* AFM(const mpa_antiferromagnet* a) {
+  need(a, "antiferromagnet");
+  return reinterpret_cast<const Antiferromagnet*>(a);
+}
+mpa_antiferromagnet* HA(Antiferromagnet* a) { return reinterpret_cast<mpa_antiferromagnet*>(a); }
+
+// Parameter access shared by ferromagnets and antiferromagnets.
+void param_info(const MagnetCore* m, const char* key, int* ncomp, int* is_uniform, char* name, int namelen,
+                char* unit, int unitlen);
+void param_set_uniform(MagnetCore* m, const char* key, const double* value);
+void param_set_cells(MagnetCore* m, const char* key, const float* values);
+void param_get(const MagnetCore* m, const char* key, float* out);
+void param_get_uniform(const MagnetCore* m, const char* key, double* out);
+void magnet_grid(const MagnetCore* m, int size[3], int origin[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (size) size[a] = m->n[a];
+    if (origin) origin[a] = m->origin[a];
+  }
+}
+void magnet_geometry(const MagnetCore* m, uint8_t* out) {
+  need(out, "out");
+  if (m->geom_host.empty()) std::memset(out, 1, m->N);
+  else std::memcpy(out, m->geom_host.data(), m->N);
+}
+
 void copy_str(const std::string& s, char* buf, int buflen) {
   if (!buf || buflen <= 0) return;
   const size_t n = std::min<size_t>(s.size(), (size_t)buflen - 1);
   std::memcpy(buf, s.data(), n);
   buf[n] = 0;
+}
+void param_info(const MagnetCore* m, const char* key, int* ncomp, int* is_uniform, char* name, int namelen,
+                char* unit, int unitlen) {
+  need(key, "key");
+  const Param& p = m->param(key);
+  if (ncomp) *ncomp = p.ncomp;
+  if (is_uniform) *is_uniform = p.is_uniform();
+  copy_str(p.name, name, namelen);
+  copy_str(p.unit, unit, unitlen);
+}
+
+void param_set_uniform(MagnetCore* m, const char* key, const double* value) {
+  need(key, "key");
+  need(value, "value");
+  Param& p = m->param(key);
+  p.cells.release();
+  for (int c = 0; c < 3; ++c) p.uniform[c] = c < p.ncomp ? (float)value[c] : 0.f;
+  m->world->touch();
+}
+
+void param_set_cells(MagnetCore* m, const char* key, const float* values) {
+  need(key, "key");
+  need(values, "values");
+  Param& p = m->param(key);
+  const size_t n = (size_t)p.ncomp * m->N;
+  if (p.cells.n != n) p.cells.allocate(n);
+  HIP_CHECK(hipMemcpy(p.cells.get(), values, n * sizeof(float), hipMemcpyHostToDevice));
+  for (int c = 0; c < 3; ++c) p.uniform[c] = 0.f;
+  m->world->touch();
+}
+
+void param_get(const MagnetCore* m, const char* key, float* out) {
+  need(key, "key");
+  need(out, "out");
+  const Param& p = m->param(key);
+  const size_t n = (size_t)p.ncomp * m->N;
+  if (p.is_uniform()) {
+    for (int c = 0; c < p.ncomp; ++c)
+      for (int i = 0; i < m->N; ++i) out[(size_t)c * m->N + i] = p.uniform[c];
+  } else {
+    HIP_CHECK(hipMemcpy(out, p.cells.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+  }
+}
+
+void param_get_uniform(const MagnetCore* m, const char* key, double* out) {
+  need(key, "key");
+  need(out, "out");
+  const Param& p = m->param(key);
+  if (!p.is_uniform()) throw Error(MPA_ERR_RUNTIME, "parameter '" + std::string(key) + "' is not uniform");
+  for (int c = 0; c < p.ncomp; ++c) out[c] = p.uniform[c];
+}
+
+// Flags every magnet has: stray-field coupling and elastodynamics.
+bool magnet_set_flag(MagnetCore* m, const std::string& k, int value) {
+  if (k == "enable_as_stray_field_source") m->stray_source = value != 0;
+  else if (k == "enable_as_stray_field_destination") m->stray_destination = value != 0;
+  else if (k == "enable_elastodynamics") {
+    // Magnet::setEnableElastodynamics (magnet.cpp:177-213)
+    if (value && m->rigid_strain_applied())
+      throw Error(MPA_ERR_INVALID, "Cannot enable elastodynamics when rigid strain is set.");
+    m->set_elastodynamics(value != 0);
+  } else
+    return false;
+  m->world->touch();
+  return true;
+}
+
+bool magnet_get_flag(const MagnetCore* m, const std::string& k, int* out) {
+  if (k == "enable_as_stray_field_source") *out = m->stray_source;
+  else if (k == "enable_as_stray_field_destination") *out = m->stray_destination;
+  else if (k == "enable_elastodynamics") *out = m->el.enabled;
+  else return false;
+  return true;
 }
 }  // namespace
 
@@ -148,6 +251,44 @@ int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet**
   });
 }
 
+int mpa_world_add_antiferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
+                                  const char* name, mpa_antiferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    need(size, "size");
+    const int zero[3] = {0, 0, 0};
+    Antiferromagnet* a = w->w.add_antiferromagnet(size, origin ? origin : zero, geometry, name);
+    if (out) *out = HA(a);
+  });
+}
+
+int mpa_world_num_antiferromagnets(const mpa_world* w, int* out) {
+  return guard([&] {
+    need(w, "world");
+    need(out, "out");
+    *out = (int)w->w.afms.size();
+  });
+}
+
+int mpa_world_antiferromagnet_at(mpa_world* w, int index, mpa_antiferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    need(out, "out");
+    if (index < 0 || index >= (int)w->w.afms.size())
+      throw Error(MPA_ERR_INDEX, "antiferromagnet index out of range");
+    *out = HA(w->w.afms[index].get());
+  });
+}
+
+int mpa_world_find_antiferromagnet(mpa_world* w, const char* name, mpa_antiferromagnet** out) {
+  return guard([&] {
+    need(w, "world");
+    need(name, "name");
+    need(out, "out");
+    *out = HA(w->w.find_afm(name));
+  });
+}
+
 int mpa_world_set_pbc(mpa_world* w, const int mastergrid_size[3], const int mastergrid_origin[3], const int reps[3]) {
   return guard([&] {
     need(w, "world");
@@ -201,21 +342,11 @@ int mpa_fm_name(const mpa_ferromagnet* m, char* buf, int buflen) {
 }
 
 int mpa_fm_grid(const mpa_ferromagnet* m, int size[3], int origin[3]) {
-  return guard([&] {
-    const Ferromagnet* f = FM(m);
-    for (int a = 0; a < 3; ++a) {
-      if (size) size[a] = f->n[a];
-      if (origin) origin[a] = f->origin[a];
-    }
-  });
+  return guard([&] { magnet_grid(FM(m), size, origin); });
 }
 
 int mpa_fm_geometry(const mpa_ferromagnet* m, uint8_t* out) {
-  return guard([&] {
-    const Ferromagnet* f = FM(m);
-    if (f->geom_host.empty()) std::memset(out, 1, f->N);
-    else std::memcpy(out, f->geom_host.data(), f->N);
-  });
+  return guard([&] { magnet_geometry(FM(m), out); });
 }
 
 int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out) {
@@ -229,9 +360,9 @@ int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value) {
     const std::string k = flag;
     if (k == "enable_demag") f->enable_demag = value != 0;
     else if (k == "enable_openbc") f->openbc = value != 0;
-    else if (k == "enable_as_stray_field_source") f->stray_source = value != 0;
-    else if (k == "enable_as_stray_field_destination") f->stray_destination = value != 0;
-    else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
+    else if (k == "enable_elastodynamics" && f->host)
+      throw Error(MPA_ERR_INVALID, "Cannot enable/disable elastodynamics for a sublattice.");
+    else if (!magnet_set_flag(f, k, value)) throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
     f->world->touch();
   });
 }
@@ -243,75 +374,29 @@ int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
     const std::string k = flag;
     if (k == "enable_demag") *out = f->enable_demag;
     else if (k == "enable_openbc") *out = f->openbc;
-    else if (k == "enable_as_stray_field_source") *out = f->stray_source;
-    else if (k == "enable_as_stray_field_destination") *out = f->stray_destination;
-    else throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
+    else if (!magnet_get_flag(f, k, out)) throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
   });
 }
 
 int mpa_fm_param_info(const mpa_ferromagnet* m, const char* key, int* ncomp, int* is_uniform, char* name,
                       int namelen, char* unit, int unitlen) {
-  return guard([&] {
-    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
-    need(key, "key");
-    const Param& p = f->param(key);
-    if (ncomp) *ncomp = p.ncomp;
-    if (is_uniform) *is_uniform = p.is_uniform();
-    copy_str(p.name, name, namelen);
-    copy_str(p.unit, unit, unitlen);
-  });
+  return guard([&] { param_info(FM(m), key, ncomp, is_uniform, name, namelen, unit, unitlen); });
 }
 
 int mpa_fm_param_set_uniform(mpa_ferromagnet* m, const char* key, const double* value) {
-  return guard([&] {
-    Ferromagnet* f = FM(m);
-    need(key, "key");
-    need(value, "value");
-    Param& p = f->param(key);
-    p.cells.release();
-    for (int c = 0; c < 3; ++c) p.uniform[c] = c < p.ncomp ? (float)value[c] : 0.f;
-    f->world->touch();
-  });
+  return guard([&] { param_set_uniform(FM(m), key, value); });
 }
 
 int mpa_fm_param_set_cells(mpa_ferromagnet* m, const char* key, const float* values) {
-  return guard([&] {
-    Ferromagnet* f = FM(m);
-    need(key, "key");
-    need(values, "values");
-    Param& p = f->param(key);
-    const size_t n = (size_t)p.ncomp * f->N;
-    if (p.cells.n != n) p.cells.allocate(n);
-    HIP_CHECK(hipMemcpy(p.cells.get(), values, n * sizeof(float), hipMemcpyHostToDevice));
-    for (int c = 0; c < 3; ++c) p.uniform[c] = 0.f;
-    f->world->touch();
-  });
+  return guard([&] { param_set_cells(FM(m), key, values); });
 }
 
 int mpa_fm_param_get(const mpa_ferromagnet* m, const char* key, float* out) {
-  return guard([&] {
-    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
-    need(key, "key");
-    need(out, "out");
-    const Param& p = f->param(key);
-    const size_t n = (size_t)p.ncomp * f->N;
-    if (p.is_uniform()) {
-      for (int c = 0; c < p.ncomp; ++c)
-        for (int i = 0; i < f->N; ++i) out[(size_t)c * f->N + i] = p.uniform[c];
-    } else {
-      HIP_CHECK(hipMemcpy(out, p.cells.get(), n * sizeof(float), hipMemcpyDeviceToHost));
-    }
-  });
+  return guard([&] { param_get(FM(m), key, out); });
 }
 
 int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* out) {
-  return guard([&] {
-    Ferromagnet* f = const_cast<Ferromagnet*>(FM(m));
-    need(key, "key");
-    const Param& p = f->param(key);
-    if (!p.is_uniform()) throw Error(MPA_ERR_RUNTIME, "parameter '" + std::string(key) + "' is not uniform");
-    for (int c = 0; c < p.ncomp; ++c) out[c] = p.uniform[c];
-  });
+  return guard([&] { param_get_uniform(FM(m), key, out); });
 }
 
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count) {
@@ -376,6 +461,151 @@ int mpa_fm_demag_kernel(mpa_ferromagnet* m, int ksize[3], float* out) {
     }
     if (!out) reference_kernel_size(g, ksize);
     else reference_kernel(g, ksize, out);
+  });
+}
+
+int mpa_fm_host(mpa_ferromagnet* m, mpa_antiferromagnet** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = HA(FM(m)->host);
+  });
+}
+
+int mpa_fm_set_variable(mpa_ferromagnet* m, const char* which, const float* values, long long count) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(which, "variable");
+    need(values, "values");
+    if (f->host) throw Error(MPA_ERR_INVALID, "a sublattice has no elastic variables; use its antiferromagnet");
+    f->set_variable(which, values, count, f->world->stream);
+  });
+}
+
+int mpa_fm_get_variable(mpa_ferromagnet* m, const char* which, float* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(which, "variable");
+    need(out, "out");
+    if (f->host) throw Error(MPA_ERR_INVALID, "a sublattice has no elastic variables; use its antiferromagnet");
+    f->get_variable(which, out, f->world->stream);
+  });
+}
+
+// ------------------------------------------------------- Antiferromagnet
+int mpa_afm_name(const mpa_antiferromagnet* a, char* buf, int buflen) {
+  return guard([&] { copy_str(AFM(a)->name, buf, buflen); });
+}
+
+int mpa_afm_grid(const mpa_antiferromagnet* a, int size[3], int origin[3]) {
+  return guard([&] { magnet_grid(AFM(a), size, origin); });
+}
+
+int mpa_afm_geometry(const mpa_antiferromagnet* a, uint8_t* out) {
+  return guard([&] { magnet_geometry(AFM(a), out); });
+}
+
+int mpa_afm_cellsingeo(const mpa_antiferromagnet* a, int* out) {
+  return guard([&] {
+    need(out, "out");
+    *out = AFM(a)->cellsingeo;
+  });
+}
+
+int mpa_afm_sublattice(mpa_antiferromagnet* a, int index, mpa_ferromagnet** out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(out, "out");
+    if (index < 0 || index > 1) throw Error(MPA_ERR_INDEX, "sublattice index should be 0 or 1");
+    *out = H(af->sub[index].get());
+  });
+}
+
+int mpa_afm_set_flag(mpa_antiferromagnet* a, const char* flag, int value) {
+  return guard([&] {
+    need(flag, "flag");
+    if (!magnet_set_flag(AFM(a), flag, value)) throw Error(MPA_ERR_INVALID, "unknown flag '" + std::string(flag) + "'");
+  });
+}
+
+int mpa_afm_get_flag(const mpa_antiferromagnet* a, const char* flag, int* out) {
+  return guard([&] {
+    need(flag, "flag");
+    need(out, "out");
+    if (!magnet_get_flag(AFM(a), flag, out)) throw Error(MPA_ERR_INVALID, "unknown flag '" + std::string(flag) + "'");
+  });
+}
+
+int mpa_afm_param_info(const mpa_antiferromagnet* a, const char* key, int* ncomp, int* is_uniform, char* name,
+                       int namelen, char* unit, int unitlen) {
+  return guard([&] { param_info(AFM(a), key, ncomp, is_uniform, name, namelen, unit, unitlen); });
+}
+
+int mpa_afm_param_set_uniform(mpa_antiferromagnet* a, const char* key, const double* value) {
+  return guard([&] { param_set_uniform(AFM(a), key, value); });
+}
+
+int mpa_afm_param_set_cells(mpa_antiferromagnet* a, const char* key, const float* values) {
+  return guard([&] { param_set_cells(AFM(a), key, values); });
+}
+
+int mpa_afm_param_get(const mpa_antiferromagnet* a, const char* key, float* out) {
+  return guard([&] { param_get(AFM(a), key, out); });
+}
+
+int mpa_afm_param_get_uniform(const mpa_antiferromagnet* a, const char* key, double* out) {
+  return guard([&] { param_get_uniform(AFM(a), key, out); });
+}
+
+int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(which, "variable");
+    need(values, "values");
+    af->set_variable(which, values, count, af->world->stream);
+  });
+}
+
+int mpa_afm_get_variable(mpa_antiferromagnet* a, const char* which, float* out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(which, "variable");
+    need(out, "out");
+    af->get_variable(which, out, af->world->stream);
+  });
+}
+
+int mpa_afm_quantity_ncomp(const mpa_antiferromagnet* a, const char* q, int* ncomp) {
+  return guard([&] {
+    need(q, "quantity");
+    need(ncomp, "ncomp");
+    *ncomp = AFM(a)->quantity_ncomp(q);
+  });
+}
+
+int mpa_afm_eval(mpa_antiferromagnet* a, const char* q, float* out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(q, "quantity");
+    need(out, "out");
+    af->eval_quantity(q, out, af->world->stream);
+  });
+}
+
+int mpa_afm_average(mpa_antiferromagnet* a, const char* q, double* out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(q, "quantity");
+    need(out, "out");
+    af->average_quantity(q, out, af->world->stream);
+  });
+}
+
+int mpa_afm_scalar(mpa_antiferromagnet* a, const char* q, double* out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(q, "quantity");
+    need(out, "out");
+    *out = af->scalar_quantity(q, af->world->stream);
   });
 }
 

@@ -110,8 +110,8 @@ __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
 
 template <int L2>
 __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
-    k_xfwd(DD d, const float* __restrict__ m, PS msat, const uint8_t* __restrict__ geom, float2* __restrict__ A,
-           const float2* __restrict__ tw) {
+    k_xfwd(DD d, const float* __restrict__ m, PS msat, const float* __restrict__ m2, PS msat2,
+           const uint8_t* __restrict__ geom, float2* __restrict__ A, const float2* __restrict__ tw) {
   using C = Line<L2>;
   using K = XfwdCfg<L2>;
   extern __shared__ float2 lds[];
@@ -122,6 +122,13 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
   const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
+  // M = Ms m, or Ms1 m1 + Ms2 m2 for the two sublattices of an antiferromagnet
+  // (k_pad, strayfieldfft.cu:51-73)
+  auto M = [&](int line, int cell, int x) {
+    float v = msat.at(cell) * m[(size_t)line * d.nx + x];
+    if (m2) v += msat2.at(cell) * m2[(size_t)line * d.nx + x];
+    return v;
+  };
   float2 v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[0][q] = make_float2(0.f, 0.f);
@@ -129,8 +136,8 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   for (int q = 0; q < C::EH; ++q) {
     const int x = t + C::T * q;
     if (x < d.nx) {
-      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = msat.at(cella + x) * m[(size_t)la * d.nx + x];
-      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = msat.at(cellb + x) * m[(size_t)lb * d.nx + x];
+      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = M(la, cella + x, x);
+      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = M(lb, cellb + x, x);
     }
   }
   float2* line = lds + g * K::LP;
@@ -570,11 +577,11 @@ void make_trig(DeviceArray<double>& ct, DeviceArray<double>& st, int K) {
 
 template <int L2>
 struct RunXfwd {
-  static void run(const DD& d, const float* m, PS msat, const uint8_t* geom, float2* A, const float2* tw,
-                  hipStream_t s) {
+  static void run(const DD& d, const float* m, PS msat, const float* m2, PS msat2, const uint8_t* geom, float2* A,
+                  const float2* tw, hipStream_t s) {
     using K = XfwdCfg<L2>;
     const int npairs = (d.nlines + 1) / 2;
-    k_xfwd<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, m, msat, geom, A, tw);
+    k_xfwd<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
     checkLaunch("k_xfwd");
   }
 };
@@ -703,7 +710,8 @@ void DemagPlan::build_kernel(hipStream_t s) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof) {
+void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof,
+                     const float* m2, PS msat2) {
   DD d;
   d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
   d.Kx = K_[0]; d.Ky = K_[1]; d.Kz = K_[2];
@@ -712,7 +720,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
   d.nlines = 3 * d.ny * d.nz;
   {
     Timed t(prof, "k_xfwd", pass_bytes(0), s);
-    with_line<RunXfwd>(d.l2x, d, m, msat, geom, A_.get(), tw_[0].get(), s);
+    with_line<RunXfwd>(d.l2x, d, m, msat, m2, msat2, geom, A_.get(), tw_[0].get(), s);
   }
   if (is2d()) {
     Timed t(prof, "k_ypass2d", pass_bytes(1), s);

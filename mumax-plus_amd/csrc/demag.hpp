@@ -27,7 +27,10 @@ class DemagPlan {
   // h = -mu0 N * (Ms m) on the magnet grid; zero outside the geometry.
   // m, h: 3*N floats (SoA, x fastest).  geom: N bytes or nullptr.
   // prof (optional) times each pass with HIP events.
-  void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof = nullptr);
+  // With m2 != nullptr the source is Ms m + Ms2 m2 (the two sublattices of an
+  // antiferromagnet, strayfieldfft.cu:206-213).
+  void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof = nullptr,
+            const float* m2 = nullptr, PS msat2 = PS{nullptr, 0.f});
   // Pass names and algorithmic HBM bytes per pass.
   int npasses() const { return is2d() ? 3 : 5; }
   const char* pass_name(int i) const;

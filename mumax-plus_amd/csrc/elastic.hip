@@ -1,0 +1,213 @@
+// Elastodynamics kernels: strain and stress of the displacement, and the
+// elastic, magnetoelastic, damping and total body forces (see elastic.hpp).
+#include "elastic.hpp"
+
+namespace mpa {
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ float3 ld3(const float* p, int i, int N) {
+  return f3(p[i], p[(size_t)N + i], p[(size_t)2 * N + i]);
+}
+
+// Local index of the cell `k` steps along `axis`, after the mastergrid wrap
+// (grid.hpp:75-83); -1 outside the magnet grid.
+__device__ __forceinline__ int shifted(const ElDesc& d, int x, int y, int z, int axis, int k) {
+  int c[3] = {x, y, z};
+  const int o[3] = {d.ox, d.oy, d.oz};
+  const int mg[3] = {d.mgx, d.mgy, d.mgz};
+  const int mo[3] = {d.mox, d.moy, d.moz};
+  const int n[3] = {d.nx, d.ny, d.nz};
+  int g = o[axis] + c[axis] + k;
+  if (mg[axis] > 0) g = mo[axis] + imod(g - mo[axis], mg[axis]);
+  const int l = g - o[axis];
+  if (l < 0 || l >= n[axis]) return -1;
+  c[axis] = l;
+  return (c[2] * d.ny + c[1]) * d.nx + c[0];
+}
+
+__device__ __forceinline__ bool in_geo(const ElDesc& d, int i) { return i >= 0 && (!d.geom || d.geom[i]); }
+
+// d f / d axis of a 3-vector field with the stencil chain of
+// straintensor.cu:45-79 (shared by magnetoelasticforce.cu:42-76).  Every
+// branch reads only cells it has checked to be in the geometry.
+__device__ float3 deriv(const ElDesc& d, const float* f, int x, int y, int z, int axis, float3 f0) {
+  const int N = d.ncell;
+  const int im2 = shifted(d, x, y, z, axis, -2), im1 = shifted(d, x, y, z, axis, -1);
+  const int ip1 = shifted(d, x, y, z, axis, 1), ip2 = shifted(d, x, y, z, axis, 2);
+  const bool gm2 = in_geo(d, im2), gm1 = in_geo(d, im1), gp1 = in_geo(d, ip1), gp2 = in_geo(d, ip2);
+  float3 r;
+  if (!gm1 && !gp1) {
+    r = f3(0.f, 0.f, 0.f);                                                   // --1--
+  } else if ((!gm2 || !gp2) && gm1 && gp1) {
+    r = 0.5f * (ld3(f, ip1, N) - ld3(f, im1, N));                            // -111-, 1111-, -1111
+  } else if (!gm2 && !gp1) {
+    r = f0 - ld3(f, im1, N);                                                 // -11--
+  } else if (!gm1 && !gp2) {
+    r = ld3(f, ip1, N) - f0;                                                 // --11-
+  } else if (gm2 && !gp1) {
+    r = 0.5f * ld3(f, im2, N) - 2.0f * ld3(f, im1, N) + 1.5f * f0;           // 111--
+  } else if (!gm1 && gp1) {
+    r = -0.5f * ld3(f, ip2, N) + 2.0f * ld3(f, ip1, N) - 1.5f * f0;          // --111
+  } else {
+    r = (2.0f / 3.0f) * (ld3(f, ip1, N) - ld3(f, im1, N)) +
+        (1.0f / 12.0f) * (ld3(f, im2, N) - ld3(f, ip2, N));                 // 11111
+  }
+  const float c[3] = {d.cx, d.cy, d.cz};
+  return (1.0f / c[axis]) * r;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_strain(ElDesc d, const float* __restrict__ u, float* __restrict__ strain,
+                                                  float* __restrict__ stress) {
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  if (!in_geo(d, idx)) {
+    for (int c = 0; c < 6; ++c) {
+      strain[(size_t)c * N + idx] = 0.f;
+      if (stress) stress[(size_t)c * N + idx] = 0.f;
+    }
+    return;
+  }
+  const int x = idx % d.nx, y = (idx / d.nx) % d.ny, z = idx / (d.nx * d.ny);
+  const float3 u0 = ld3(u, idx, N);
+  const float3 dx = deriv(d, u, x, y, z, 0, u0);  // d_x u
+  const float3 dy = deriv(d, u, x, y, z, 1, u0);
+  const float3 dz = deriv(d, u, x, y, z, 2, u0);
+  // straintensor.cu:81-97: e_ii = d_i u_i, e_ij = (d_i u_j + d_j u_i)/2
+  const float e[6] = {dx.x, dy.y, dz.z, 0.5f * (dx.y + dy.x), 0.5f * (dx.z + dz.x), 0.5f * (dy.z + dz.y)};
+  for (int c = 0; c < 6; ++c) strain[(size_t)c * N + idx] = e[c];
+  if (stress) {
+    // stresstensor.cu:27-38 (2 C44 on the real shear strain)
+    const float c11 = d.C11.at(idx), c12 = d.C12.at(idx), c44 = d.C44.at(idx);
+    const float s[6] = {c11 * e[0] + c12 * e[1] + c12 * e[2], c11 * e[1] + c12 * e[2] + c12 * e[0],
+                        c11 * e[2] + c12 * e[0] + c12 * e[1], 2.f * c44 * e[3],
+                        2.f * c44 * e[4], 2.f * c44 * e[5]};
+    for (int c = 0; c < 6; ++c) stress[(size_t)c * N + idx] = s[c];
+  }
+}
+
+// Row i of the stress tensor (sigma_i0, sigma_i1, sigma_i2) at cell j.
+__device__ __forceinline__ float3 stress_row(const float* s, int j, int N, int i) {
+  const int c0 = i == 0 ? 0 : (i == 1 ? 3 : 4);
+  const int c1 = i == 0 ? 3 : (i == 1 ? 1 : 5);
+  const int c2 = i == 0 ? 4 : (i == 1 ? 5 : 2);
+  return f3(s[(size_t)c0 * N + j], s[(size_t)c1 * N + j], s[(size_t)c2 * N + j]);
+}
+
+// f_j = d_i sigma_ij with traction-free boundaries, elasticforce.cu:47-98.
+__device__ float3 stress_divergence(const ElDesc& d, const float* s, int idx, int x, int y, int z) {
+  const int N = d.ncell;
+  const float c[3] = {d.cx, d.cy, d.cz};
+  float3 f = f3(0.f, 0.f, 0.f);
+  for (int i = 0; i < 3; ++i) {
+    const float w = 1.0f / c[i];
+    const int im2 = shifted(d, x, y, z, i, -2), im1 = shifted(d, x, y, z, i, -1);
+    const int ip1 = shifted(d, x, y, z, i, 1), ip2 = shifted(d, x, y, z, i, 2);
+    const bool gm2 = in_geo(d, im2), gm1 = in_geo(d, im1), gp1 = in_geo(d, ip1), gp2 = in_geo(d, ip2);
+    if (!gm1 && !gp1) continue;
+    if (!gm1) {
+      f = f + w * (stress_row(s, idx, N, i) + (1.f / 3.f) * stress_row(s, ip1, N, i));
+    } else if (!gp1) {
+      f = f + w * ((-1.f / 3.f) * stress_row(s, im1, N, i) - stress_row(s, idx, N, i));
+    } else if (!gm2 || !gp2) {
+      f = f + (0.5f * w) * (stress_row(s, ip1, N, i) - stress_row(s, im1, N, i));
+    } else {
+      f = f + w * ((4.f / 6.f) * (stress_row(s, ip1, N, i) - stress_row(s, im1, N, i)) +
+                   (1.f / 12.f) * (stress_row(s, im2, N, i) - stress_row(s, ip2, N, i)));
+    }
+  }
+  return f;
+}
+
+// magnetoelasticforce.cu:82-91 with der[k] = d_k m.
+__device__ float3 me_force(const ElDesc& d, const MEForce& a, int idx, int x, int y, int z) {
+  const int N = d.ncell;
+  const float3 m = ld3(a.m, idx, N);
+  const float3 d0 = deriv(d, a.m, x, y, z, 0, m), d1 = deriv(d, a.m, x, y, z, 1, m), d2 = deriv(d, a.m, x, y, z, 2, m);
+  const float b1 = a.B1.at(idx), b2 = a.B2.at(idx);
+  return f3(2.f * b1 * m.x * d0.x + b2 * m.x * (d1.y + d2.z) + b2 * (m.y * d1.x + m.z * d2.x),
+            2.f * b1 * m.y * d1.y + b2 * m.y * (d2.z + d0.x) + b2 * (m.z * d2.y + m.x * d0.y),
+            2.f * b1 * m.z * d2.z + b2 * m.z * (d0.x + d1.y) + b2 * (m.x * d0.z + m.y * d1.z));
+}
+
+__global__ void __launch_bounds__(BLOCK) k_elastic(ElDesc d, const float* __restrict__ stress,
+                                                   const float* __restrict__ v, MEForce a, MEForce b,
+                                                   float* __restrict__ out, int mode) {
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  float3 f = f3(0.f, 0.f, 0.f);
+  if (in_geo(d, idx)) {
+    const int x = idx % d.nx, y = (idx / d.nx) % d.ny, z = idx / (d.nx * d.ny);
+    const bool body = mode == EL_BODY || mode == EL_ACCEL;
+    if (mode == EL_FORCE || (body && d.elastic)) f = f + stress_divergence(d, stress, idx, x, y, z);
+    if (body && d.ext) f = f + d.fext.at(idx, N);
+    if ((mode == EL_MEFORCE || body) && a.m) f = f + me_force(d, a, idx, x, y, z);
+    if (body && b.m) f = f + me_force(d, b, idx, x, y, z);
+    if (mode == EL_DAMPING || (mode == EL_ACCEL && d.damping)) f = f + (-d.eta.at(idx)) * ld3(v, idx, N);
+    if (mode == EL_ACCEL && d.divrho) {
+      const float p = d.rho.at(idx);
+      f = p != 0.f ? (1.f / p) * f : f3(0.f, 0.f, 0.f);
+    }
+  }
+  out[idx] = f.x;
+  out[(size_t)N + idx] = f.y;
+  out[(size_t)2 * N + idx] = f.z;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_elastic_local(ElDesc d, const float* __restrict__ stress,
+                                                         const float* __restrict__ strain,
+                                                         const float* __restrict__ v, float* __restrict__ out,
+                                                         int mode, int accumulate) {
+  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  const bool in = in_geo(d, idx);
+  if (mode == EL_KINETIC_EDENS) {
+    const float3 w = ld3(v, idx, N);
+    const float e = in ? 0.5f * dot3(w, w) * d.rho.at(idx) : 0.f;
+    out[idx] = accumulate ? out[idx] + e : e;
+  } else if (mode == EL_ELASTIC_EDENS) {
+    float e = 0.f;
+    if (in)
+      for (int i = 0; i < 3; ++i) {
+        e += 0.5f * stress[(size_t)i * N + idx] * strain[(size_t)i * N + idx];
+        e += stress[(size_t)(i + 3) * N + idx] * strain[(size_t)(i + 3) * N + idx];
+      }
+    out[idx] = accumulate ? out[idx] + e : e;
+  } else {
+    const float3 w = ld3(v, idx, N);
+    for (int i = 0; i < 3; ++i) {
+      const float3 row = stress_row(stress, idx, N, i);
+      out[(size_t)i * N + idx] = in ? -dot3(row, w) : 0.f;
+    }
+  }
+}
+
+unsigned blocks(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+void launch_strain(const ElDesc& d, const float* u, float* strain, float* stress, hipStream_t s) {
+  k_strain<<<blocks(d.ncell), BLOCK, 0, s>>>(d, u, strain, stress);
+  checkLaunch("k_strain");
+}
+
+void launch_elastic(const ElDesc& d, const float* stress, const float* v, const MEForce& a, const MEForce& b,
+                    float* out, ElMode mode, hipStream_t s) {
+  if ((mode == EL_FORCE || ((mode == EL_BODY || mode == EL_ACCEL) && d.elastic)) && !stress)
+    throw Error(MPA_ERR_RUNTIME, "elastic force needs the stress tensor");
+  k_elastic<<<blocks(d.ncell), BLOCK, 0, s>>>(d, stress, v, a, b, out, (int)mode);
+  checkLaunch("k_elastic");
+}
+
+void launch_elastic_local(const ElDesc& d, const float* stress, const float* strain, const float* v, float* out,
+                          ElLocal mode, hipStream_t s, bool accumulate) {
+  k_elastic_local<<<blocks(d.ncell), BLOCK, 0, s>>>(d, stress, strain, v, out, (int)mode, accumulate ? 1 : 0);
+  checkLaunch("k_elastic_local");
+}
+
+}  // namespace mpa

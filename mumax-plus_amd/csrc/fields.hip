@@ -92,13 +92,16 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
   const bool do_ex = (d.terms & TERM_EXCH) && ms != 0.f;
   const bool do_dmi = (d.terms & TERM_DMI) && ms != 0.f;
+  const bool sub = d.sublattice != 0;
+  const float3 m2c = sub ? load3(d.m2, idx, N) : f3(0.f, 0.f, 0.f);
+  const float csz[3] = {d.cx, d.cy, d.cz};
   if (do_ex || do_dmi) {
     const float a = d.aex.at(idx);
+    const float an = sub ? d.afmex_nn.at(idx) : 0.f;
     float D[D_COUNT];
 #pragma unroll
     for (int k = 0; k < D_COUNT; ++k) D[k] = d.dmi[k].at(idx);
     const float w[3] = {1.f / (d.cx * d.cx), 1.f / (d.cy * d.cy), 1.f / (d.cz * d.cz)};
-    const float csz[3] = {d.cx, d.cy, d.cz};
     const bool openbc = d.openbc != 0;
     const bool open_dmi = openbc || a == 0.f;
 #pragma unroll
@@ -108,51 +111,155 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
       const int nb = neighbour(d, x, y, z, dir);
       const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
       float3 gamma = f3(0.f, 0.f, 0.f);
-      if (!nb_in) gamma = dmi_gamma(D, axis, mc);
+      // sister's normal derivative taken on the bulk side of a boundary
+      // (exchange.cu:176-183, dmi.cu:200-207)
+      float3 dm2 = f3(0.f, 0.f, 0.f);
+      if (!nb_in) {
+        gamma = dmi_gamma(D, axis, mc);
+        if (sub) {
+          const int ob = neighbour(d, x, y, z, dir ^ 1);
+          if (ob >= 0 && (!d.geom || d.geom[ob])) dm2 = (1.f / delta) * (m2c - load3(d.m2, ob, N));
+        }
+      }
       if (do_ex) {
-        // exchange.cu:61-101
         bool use = true;
         float3 mn;
-        float an;
-        if (!nb_in) {
+        float an_;
+        if (sub && !openbc) {
+          // sublattice variant, exchange.cu:106-202 (no open boundaries)
+          if (!nb_in) {
+            if (a == 0.f) use = false;
+            mn = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
+            an_ = a;
+          } else {
+            if (d.msat.at(nb) == 0.f) use = false;
+            mn = load3(m, nb, N);
+            an_ = d.aex.at(nb);
+          }
+        } else if (!nb_in) {
+          // exchange.cu:61-101
           if (openbc || a == 0.f) use = false;
           mn = mc + (delta / (2 * a)) * gamma;
-          an = a;
+          an_ = a;
         } else {
           if (openbc && d.msat.at(nb) == 0.f) use = false;
           mn = load3(m, nb, N);
-          an = d.aex.at(nb);
+          an_ = d.aex.at(nb);
         }
         if (use) {
-          const float A = harmonic_mean(a, an);
+          const float A = harmonic_mean(a, an_);
           hex = hex + (2 * A * w[axis]) * (mn - mc);
         }
       }
       if (do_dmi) {
-        // dmi.cu:114-173
         const int nbi = nb_in ? nb : idx;
-        bool skip = open_dmi && (!nb_in || d.msat.at(nbi) == 0.f);
-        if (!skip) {
-          const int kxz = axis == 0 ? D_XXZ : (axis == 1 ? D_YXZ : D_ZXZ);
-          const int kxy = axis == 0 ? D_XXY : (axis == 1 ? D_YXY : D_ZXY);
-          const int kyz = axis == 0 ? D_XYZ : (axis == 1 ? D_YYZ : D_ZYZ);
-          float Dxz = D[kxz], Dxy = D[kxy], Dyz = D[kyz];
-          if (nbi != idx) {
-            Dxz = harmonic_mean(Dxz, d.dmi[kxz].at(nbi));
-            Dxy = harmonic_mean(Dxy, d.dmi[kxy].at(nbi));
-            Dyz = harmonic_mean(Dyz, d.dmi[kyz].at(nbi));
+        const int kxz = axis == 0 ? D_XXZ : (axis == 1 ? D_YXZ : D_ZXZ);
+        const int kxy = axis == 0 ? D_XXY : (axis == 1 ? D_YXY : D_ZXY);
+        const int kyz = axis == 0 ? D_XYZ : (axis == 1 ? D_YYZ : D_ZYZ);
+        float Dxz = D[kxz], Dxy = D[kxy], Dyz = D[kyz];
+        if (nbi != idx) {
+          Dxz = harmonic_mean(Dxz, d.dmi[kxz].at(nbi));
+          Dxy = harmonic_mean(Dxy, d.dmi[kxy].at(nbi));
+          Dyz = harmonic_mean(Dyz, d.dmi[kyz].at(nbi));
+        }
+        if (!sub) {
+          // dmi.cu:114-173
+          const bool skip = open_dmi && (!nb_in || d.msat.at(nbi) == 0.f);
+          if (!skip) {
+            float3 mn;
+            if (!nb_in && !open_dmi) mn = mc + (delta / (2 * a)) * gamma;
+            else mn = load3(m, nbi, N);
+            hdm.x += (Dxy * mn.y + Dxz * mn.z) / delta;
+            hdm.y += (-Dxy * mn.x + Dyz * mn.z) / delta;
+            hdm.z += (-Dxz * mn.x - Dyz * mn.y) / delta;
           }
-          float3 mn;
-          if (!nb_in && !open_dmi) mn = mc + (delta / (2 * a)) * gamma;
-          else mn = load3(m, nbi, N);
-          hdm.x += (Dxy * mn.y + Dxz * mn.z) / delta;
-          hdm.y += (-Dxy * mn.x + Dyz * mn.z) / delta;
-          hdm.z += (-Dxz * mn.x - Dyz * mn.y) / delta;
+        } else {
+          // sublattice with the inter-sublattice tensor, dmi.cu:112-240
+          const bool skip = open_dmi && (!nb_in || (d.msat.at(nbi) == 0.f && d.msat2.at(nbi) == 0.f));
+          if (!skip) {
+            const float Dixz = harmonic_mean(d.hdmi[kxz].at(idx), d.hdmi[kxz].at(nbi));
+            const float Dixy = harmonic_mean(d.hdmi[kxy].at(idx), d.hdmi[kxy].at(nbi));
+            const float Diyz = harmonic_mean(d.hdmi[kyz].at(idx), d.hdmi[kyz].at(nbi));
+            float3 m1n, m2n;
+            if (!nb_in) {
+              // Neumann boundary; both sublattices use this sublattice's own tensor
+              const float3 g2 = dmi_gamma(D, axis, m2c);
+              m1n = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
+              m2n = m2c + (delta / (2 * a)) * (an * cross3(cross3((1.f / delta) * (m1n - mc), m2c), m2c) + g2);
+            } else {
+              m1n = load3(m, nbi, N);
+              m2n = load3(d.m2, nbi, N);
+            }
+            const float s1 = d.msat.at(nbi) != 0.f ? 1.f : 0.f;
+            const float s2 = d.msat2.at(nbi) != 0.f ? 1.f : 0.f;
+            hdm.x += (s1 * (Dxy * m1n.y + Dxz * m1n.z) - s2 * (Dixy * m2n.y - Dixz * m2n.z)) / delta;
+            hdm.y += (s1 * (-Dxy * m1n.x + Dyz * m1n.z) - s2 * (-Dixy * m2n.x - Diyz * m2n.z)) / delta;
+            hdm.z += (s1 * (-Dxz * m1n.x - Dyz * m1n.y) - s2 * (-Dixz * m2n.x + Diyz * m2n.y)) / delta;
+          }
         }
       }
     }
     if (do_ex) hex = hex / ms;
     if (do_dmi) hdm = hdm / ms;
+  }
+
+  // magnetoelastic field, magnetoelasticfield.cu:44-103 (no msat check, as there)
+  float3 hme = f3(0.f, 0.f, 0.f);
+  if (d.terms & TERM_ME) {
+    float e[6];
+    if (d.me_rigid) {
+      const float3 n = d.rnorm.at(idx, N), s = d.rshear.at(idx, N);
+      e[0] = n.x; e[1] = n.y; e[2] = n.z; e[3] = s.x; e[4] = s.y; e[5] = s.z;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) e[c] = d.strain[(size_t)c * N + idx];
+    }
+    const float b1 = d.B1.at(idx), b2 = d.B2.at(idx), f = -2.f / ms;
+    hme = f3(f * (b1 * e[0] * mc.x + b2 * (e[3] * mc.y + e[4] * mc.z)),
+             f * (b1 * e[1] * mc.y + b2 * (e[5] * mc.z + e[3] * mc.x)),
+             f * (b1 * e[2] * mc.z + b2 * (e[4] * mc.x + e[5] * mc.y)));
+  }
+
+  // antiferromagnetic exchange with the sister sublattice, divided by the
+  // sister's msat (afmexchange.cu:28-149)
+  float3 hafm_nn = f3(0.f, 0.f, 0.f), hafm_site = f3(0.f, 0.f, 0.f);
+  if (sub && (d.terms & (TERM_AFM_NN | TERM_AFM_SITE))) {
+    const float ms2 = d.msat2.at(idx);
+    if (ms2 != 0.f && (d.terms & TERM_AFM_SITE)) {
+      const float l = d.latcon.at(idx);
+      hafm_site = (4.f * d.afmex_cell.at(idx) / (l * l) / ms2) * m2c;
+    }
+    if (ms2 != 0.f && (d.terms & TERM_AFM_NN)) {
+      const float a = d.aex.at(idx);
+      const float ann = d.afmex_nn.at(idx);
+      const bool open_nn = d.openbc != 0 || a == 0.f;
+      float D[D_COUNT];
+#pragma unroll
+      for (int k = 0; k < D_COUNT; ++k) D[k] = d.dmi[k].at(idx);
+      for (int dir = 0; dir < 6; ++dir) {
+        const int axis = dir >> 1;
+        const float delta = (dir & 1) ? csz[axis] : -csz[axis];
+        const int nb = neighbour(d, x, y, z, dir);
+        const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+        if (!nb_in && open_nn) continue;
+        if (open_nn && d.msat2.at(nb) == 0.f) continue;  // nb_in here
+        float3 m2n;
+        float ann_;
+        if (nb_in) {
+          m2n = load3(d.m2, nb, N);
+          ann_ = d.afmex_nn.at(nb);
+        } else {
+          const float3 g2 = dmi_gamma(D, axis, m2c);
+          float3 dm1 = f3(0.f, 0.f, 0.f);
+          const int ob = neighbour(d, x, y, z, dir ^ 1);
+          if (ob >= 0 && (!d.geom || d.geom[ob])) dm1 = (1.f / delta) * (mc - load3(m, ob, N));
+          m2n = m2c + (delta / (2 * a)) * (ann * cross3(cross3(dm1, m2c), m2c) + g2);
+          ann_ = ann;
+        }
+        hafm_nn = hafm_nn + (harmonic_mean(ann, ann_) / (delta * delta)) * (m2n - m2c);
+      }
+      hafm_nn = hafm_nn / ms2;
+    }
   }
 
   float3 han = f3(0.f, 0.f, 0.f);
@@ -189,14 +296,19 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
     edens += -1.0f * ms * dot3(mc, hext);
     edens += -0.5f * ms * dot3(mc, hdm);
     edens += -0.5f * ms * dot3(mc, hdem);
+    edens += -0.5f * ms * dot3(mc, hme + hafm_nn + hafm_site);
     out[idx] = edens;
     return;
   }
+  // effectivefield.cpp:15-28 order
   float3 h = hex;
   h = h + han;
   h = h + hext;
   h = h + hdm;
   h = h + hdem;
+  h = h + hme;
+  h = h + hafm_nn;
+  h = h + hafm_site;
   float3 r;
   if (mode == MODE_FIELD) {
     r = h;
@@ -355,7 +467,8 @@ bool all_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
            !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p;
   for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p;
-  return u;
+  // sublattices and magnetoelastic coupling use the per-cell kernel only
+  return u && !d.sublattice && !(d.terms & (TERM_ME | TERM_AFM_NN | TERM_AFM_SITE));
 }
 
 // Host-side coefficients of the uniform path, in double then rounded once.

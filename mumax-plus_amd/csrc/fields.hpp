@@ -13,6 +13,9 @@ enum TermBits : unsigned {
   TERM_EXT = 1u << 3,
   TERM_DMI = 1u << 4,
   TERM_DEMAG = 1u << 5,
+  TERM_ME = 1u << 6,         // magnetoelastic field
+  TERM_AFM_SITE = 1u << 7,   // homogeneous (same-cell) antiferromagnetic exchange
+  TERM_AFM_NN = 1u << 8,     // inhomogeneous (neighbour) antiferromagnetic exchange
 };
 
 // DMI tensor entries in the member order of DmiTensor (src/physics/dmitensor.hpp:12-22).
@@ -32,6 +35,19 @@ struct FMDesc {
   float wb[3];  // world bias field (T)
   int openbc;
   unsigned terms;
+  // magnetoelastic coupling (magnetoelasticfield.cu:44-103): dynamic strain
+  // (6N: xx yy zz xy xz yz) of the magnet owning the displacement, or a rigid
+  // applied strain (normal xx yy zz, shear xy xz yz) when me_rigid is set
+  PS B1, B2;
+  const float* strain;
+  PV rnorm, rshear;
+  int me_rigid;
+  // antiferromagnet sublattice: sister magnetisation and msat, host
+  // parameters and the inter-sublattice DMI tensor
+  int sublattice;
+  const float* m2;
+  PS msat2, afmex_cell, afmex_nn, latcon;
+  PS hdmi[D_COUNT];
 };
 
 enum FieldMode {

@@ -69,7 +69,7 @@ const Tableau& tableau_by_name(const std::string& name) {
   return it->second;
 }
 
-// ------------------------------------------------------------ Ferromagnet
+// ------------------------------------------------------------ parameters
 
 namespace {
 struct ParamSpec {
@@ -79,7 +79,7 @@ struct ParamSpec {
   const char* unit;
 };
 // Defaults of the reference Ferromagnet constructor (ferromagnet.cpp:24-61).
-const ParamSpec kParams[] = {
+const ParamSpec kFmParams[] = {
     {"msat", 1, {1.f, 0, 0}, "A/m"},          {"aex", 1, {0, 0, 0}, "J/m"},
     {"alpha", 1, {0, 0, 0}, ""},              {"ku1", 1, {0, 0, 0}, "J/m3"},
     {"ku2", 1, {0, 0, 0}, "J/m3"},            {"kc1", 1, {0, 0, 0}, "J/m3"},
@@ -91,6 +91,24 @@ const ParamSpec kParams[] = {
     {"dmi_yxy", 1, {0, 0, 0}, "J/m2"},        {"dmi_yyz", 1, {0, 0, 0}, "J/m2"},
     {"dmi_yxz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxy", 1, {0, 0, 0}, "J/m2"},
     {"dmi_zyz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxz", 1, {0, 0, 0}, "J/m2"},
+    {"B1", 1, {0, 0, 0}, "J/m3"},             {"B2", 1, {0, 0, 0}, "J/m3"},
+};
+// Elastic parameters of every magnet (magnet.cpp:26-34).
+const ParamSpec kElasticParams[] = {
+    {"C11", 1, {0, 0, 0}, "N/m2"},  {"C12", 1, {0, 0, 0}, "N/m2"},
+    {"C44", 1, {0, 0, 0}, "N/m2"},  {"eta", 1, {0, 0, 0}, "kg/m3s"},
+    {"rho", 1, {1.f, 0, 0}, "kg/m3"}, {"external_body_force", 3, {0, 0, 0}, "N/m3"},
+    {"rigid_norm_strain", 3, {0, 0, 0}, ""}, {"rigid_shear_strain", 3, {0, 0, 0}, ""},
+};
+// Host parameters of an antiferromagnet (antiferromagnet.cpp:15-25) and its
+// inter-sublattice DMI tensor.
+const ParamSpec kAfmParams[] = {
+    {"afmex_cell", 1, {0, 0, 0}, "J/m"}, {"afmex_nn", 1, {0, 0, 0}, "J/m"},
+    {"latcon", 1, {0.35e-9f, 0, 0}, "m"}, {"dmi_xxy", 1, {0, 0, 0}, "J/m2"},
+    {"dmi_xyz", 1, {0, 0, 0}, "J/m2"},   {"dmi_xxz", 1, {0, 0, 0}, "J/m2"},
+    {"dmi_yxy", 1, {0, 0, 0}, "J/m2"},   {"dmi_yyz", 1, {0, 0, 0}, "J/m2"},
+    {"dmi_yxz", 1, {0, 0, 0}, "J/m2"},   {"dmi_zxy", 1, {0, 0, 0}, "J/m2"},
+    {"dmi_zyz", 1, {0, 0, 0}, "J/m2"},   {"dmi_zxz", 1, {0, 0, 0}, "J/m2"},
 };
 const char* kDmiKeys[D_COUNT] = {"dmi_xxy", "dmi_xyz", "dmi_xxz", "dmi_yxy", "dmi_yyz",
                                  "dmi_yxz", "dmi_zxy", "dmi_zyz", "dmi_zxz"};
@@ -118,9 +136,12 @@ void random_unit_vectors(std::vector<float>& out, int N, unsigned long long seed
     for (int c = 0; c < 3; ++c) out[(size_t)c * N + i] = (float)(v[c] / nrm);
   }
 }
+
 }  // namespace
 
-Ferromagnet::Ferromagnet(World* w, const int size[3], const int org[3], const uint8_t* geometry, std::string nm)
+// ------------------------------------------------------------ MagnetCore
+
+MagnetCore::MagnetCore(World* w, const int size[3], const int org[3], const uint8_t* geometry, std::string nm)
     : world(w), name(std::move(nm)) {
   for (int a = 0; a < 3; ++a) {
     if (size[a] < 1) throw Error(MPA_ERR_INVALID, "grid size should be positive");
@@ -139,32 +160,298 @@ Ferromagnet::Ferromagnet(World* w, const int size[3], const int org[3], const ui
     geom.allocate(N);
     HIP_CHECK(hipMemcpy(geom.get(), geom_host.data(), N, hipMemcpyHostToDevice));
   }
-  for (const auto& sp : kParams) {
+}
+
+void MagnetCore::add_params(const void* specs, int count) {
+  const ParamSpec* sp = static_cast<const ParamSpec*>(specs);
+  for (int i = 0; i < count; ++i) {
     Param p;
-    p.name = name + ":" + sp.key;
-    p.unit = sp.unit;
-    p.ncomp = sp.ncomp;
-    for (int c = 0; c < 3; ++c) p.uniform[c] = sp.def[c];
-    params.emplace(sp.key, std::move(p));
+    p.name = name + ":" + sp[i].key;
+    p.unit = sp[i].unit;
+    p.ncomp = sp[i].ncomp;
+    for (int c = 0; c < 3; ++c) p.uniform[c] = sp[i].def[c];
+    params.emplace(sp[i].key, std::move(p));
   }
+}
+
+Param& MagnetCore::param(const std::string& key) {
+  auto it = params.find(key);
+  if (it == params.end()) throw Error(MPA_ERR_INVALID, "unknown parameter '" + key + "'");
+  return it->second;
+}
+
+const Param& MagnetCore::param(const std::string& key) const {
+  auto it = params.find(key);
+  if (it == params.end()) throw Error(MPA_ERR_INVALID, "unknown parameter '" + key + "'");
+  return it->second;
+}
+
+// Magnet::setEnableElastodynamics (magnet.cpp:177-213): u and v start at zero.
+void MagnetCore::set_elastodynamics(bool on) {
+  if (on == el.enabled) return;
+  el.enabled = on;
+  if (on) {
+    el.u.allocate((size_t)3 * N);
+    el.v.allocate((size_t)3 * N);
+    el.strain.allocate((size_t)6 * N);
+    el.stress.allocate((size_t)6 * N);
+    HIP_CHECK(hipMemset(el.u.get(), 0, el.u.bytes()));
+    HIP_CHECK(hipMemset(el.v.get(), 0, el.v.bytes()));
+  } else {
+    el = ElasticState{};
+  }
+  world->touch();
+  world->solver.equations_changed();
+}
+
+bool MagnetCore::elasticity_assured_zero() const {
+  return !el.enabled ||
+         (param("C11").assured_zero() && param("C12").assured_zero() && param("C44").assured_zero());
+}
+
+bool MagnetCore::rigid_strain_applied() const {
+  return !(param("rigid_norm_strain").assured_zero() && param("rigid_shear_strain").assured_zero());
+}
+
+ElDesc MagnetCore::eldesc() const {
+  ElDesc d{};
+  d.nx = n[0]; d.ny = n[1]; d.nz = n[2]; d.ncell = N;
+  d.ox = origin[0]; d.oy = origin[1]; d.oz = origin[2];
+  d.mgx = world->mastergrid_size[0]; d.mgy = world->mastergrid_size[1]; d.mgz = world->mastergrid_size[2];
+  d.mox = world->mastergrid_origin[0]; d.moy = world->mastergrid_origin[1]; d.moz = world->mastergrid_origin[2];
+  d.cx = (float)world->cellsize[0]; d.cy = (float)world->cellsize[1]; d.cz = (float)world->cellsize[2];
+  d.geom = geom.get();
+  d.C11 = param("C11").ps();
+  d.C12 = param("C12").ps();
+  d.C44 = param("C44").ps();
+  d.eta = param("eta").ps();
+  d.rho = param("rho").ps();
+  d.fext = param("external_body_force").pv();
+  d.elastic = elasticity_assured_zero() ? 0 : 1;
+  d.ext = param("external_body_force").assured_zero() ? 0 : 1;
+  d.damping = (el.enabled && !param("eta").assured_zero()) ? 1 : 0;
+  d.divrho = param("rho").assured_zero() ? 0 : 1;
+  return d;
+}
+
+MEForce MagnetCore::me_force_of(int i) {
+  MEForce f{};
+  Ferromagnet* fm = fm_at(i);
+  if (fm->magnetoelastic_assured_zero()) return f;
+  f.m = fm->m.get();
+  f.B1 = fm->param("B1").ps();
+  f.B2 = fm->param("B2").ps();
+  return f;
+}
+
+void MagnetCore::compute_strain(hipStream_t s) {
+  if (!el.enabled) throw Error(MPA_ERR_RUNTIME, "elastodynamics is not enabled for '" + name + "'");
+  Timed t(&world->prof, "k_strain", 12.0 * N + 48.0 * N, s);
+  launch_strain(eldesc(), el.u.get(), el.strain.get(), el.stress.get(), s);
+}
+
+void MagnetCore::ensure_elastic_stage_buffers(int S) {
+  if (el.u0.n == 0) el.u0.allocate((size_t)3 * N);
+  if (el.v0.n == 0) el.v0.allocate((size_t)3 * N);
+  while ((int)el.ku.size() < S) el.ku.emplace_back((size_t)3 * N);
+  while ((int)el.kv.size() < S) el.kv.emplace_back((size_t)3 * N);
+}
+
+// Variable::set / get of elastic_displacement and elastic_velocity
+// (wrap_variable.cpp:8-16): 3 values (uniform) or 3*ncells; zero outside the
+// geometry.
+void MagnetCore::set_variable(const std::string& which, const float* host, long long count, hipStream_t s) {
+  if (!el.enabled)
+    throw Error(MPA_ERR_RUNTIME, which + " Variable does not exist yet. Enable elastodynamics first.");
+  DeviceArray<float>* dst = which == "elastic_displacement" ? &el.u : (which == "elastic_velocity" ? &el.v : nullptr);
+  if (!dst) throw Error(MPA_ERR_INVALID, "unknown variable '" + which + "'");
+  std::vector<float> buf((size_t)3 * N);
+  if (count == 3) {
+    for (int c = 0; c < 3; ++c)
+      for (int i = 0; i < N; ++i) buf[(size_t)c * N + i] = host[c];
+  } else if (count == 3LL * N) {
+    std::memcpy(buf.data(), host, buf.size() * sizeof(float));
+  } else {
+    throw Error(MPA_ERR_INVALID, which + ": expected 3 or 3*ncells values");
+  }
+  if (!geom_host.empty())
+    for (int i = 0; i < N; ++i)
+      if (!geom_host[i]) buf[i] = buf[(size_t)N + i] = buf[(size_t)2 * N + i] = 0.f;
+  HIP_CHECK(hipMemcpyAsync(dst->get(), buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  world->touch();
+}
+
+void MagnetCore::get_variable(const std::string& which, float* host, hipStream_t s) {
+  if (!el.enabled)
+    throw Error(MPA_ERR_RUNTIME, which + " Variable does not exist yet. Enable elastodynamics first.");
+  const DeviceArray<float>* src =
+      which == "elastic_displacement" ? &el.u : (which == "elastic_velocity" ? &el.v : nullptr);
+  if (!src) throw Error(MPA_ERR_INVALID, "unknown variable '" + which + "'");
+  HIP_CHECK(hipMemcpyAsync(host, src->get(), (size_t)3 * N * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+enum MagnetQMode { MQ_STRAIN = -1, MQ_STRESS = -2, MQ_VELOCITY = -3, MQ_DISPLACEMENT = -4, MQ_KINETIC = -5,
+                   MQ_ELASTIC_EDENS = -6, MQ_POYNTING = -7 };
+struct MagnetQ {
+  const char* name;
+  int ncomp;
+  int mode;  // ElMode, or a MagnetQMode
+};
+const MagnetQ kMagnetQuantities[] = {
+    {"strain_tensor", 6, MQ_STRAIN},
+    {"stress_tensor", 6, MQ_STRESS},
+    {"elastic_force", 3, EL_FORCE},
+    {"effective_body_force", 3, EL_BODY},
+    {"elastic_damping", 3, EL_DAMPING},
+    {"elastic_acceleration", 3, EL_ACCEL},
+    {"elastic_velocity", 3, MQ_VELOCITY},
+    {"elastic_displacement", 3, MQ_DISPLACEMENT},
+    {"kinetic_energy_density", 1, MQ_KINETIC},
+    {"elastic_energy_density", 1, MQ_ELASTIC_EDENS},
+    {"poynting_vector", 3, MQ_POYNTING},
+};
+}  // namespace
+
+int MagnetCore::magnet_quantity_ncomp(const std::string& q) const {
+  for (const auto& mq : kMagnetQuantities)
+    if (q == mq.name) return mq.ncomp;
+  return 0;
+}
+
+// Magnet-level quantities (straintensor.cu:99-116, stresstensor.cu:42-61,
+// elasticforce.cu:104-124, elastodynamics.cu:23-96, elasticdamping.cu:30-49,
+// elasticenergies.cu:25-101, poyntingvector.cu:32-46) with their assured-zero
+// rules: strain needs elastodynamics; stress, elastic force, elastic energy
+// and energy flux need some non-zero stiffness too; damping needs eta;
+// kinetic energy needs rho.
+const float* MagnetCore::magnet_eval_device(const std::string& q, hipStream_t s, int* ncomp) {
+  const MagnetQ* mq = nullptr;
+  for (const auto& e : kMagnetQuantities)
+    if (q == e.name) mq = &e;
+  if (!mq) throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
+  *ncomp = mq->ncomp;
+  const int mode = mq->mode;
+  if (!el.enabled && (mode == MQ_VELOCITY || mode == MQ_DISPLACEMENT))
+    throw Error(MPA_ERR_RUNTIME, q + " Variable does not exist yet. Enable elastodynamics first.");
+  if (mode == MQ_VELOCITY) return el.v.get();
+  if (mode == MQ_DISPLACEMENT) return el.u.get();
+  if (qbuf.n == 0) qbuf.allocate((size_t)6 * N);
+  float* out = qbuf.get();
+  const ElDesc d = eldesc();
+  bool zero = false;
+  switch (mode) {
+    case MQ_STRAIN: zero = !el.enabled; break;
+    case MQ_STRESS: case EL_FORCE: case MQ_ELASTIC_EDENS: case MQ_POYNTING: zero = elasticity_assured_zero(); break;
+    case EL_DAMPING: zero = !d.damping; break;
+    case MQ_KINETIC: zero = !el.enabled || param("rho").assured_zero(); break;
+    default: break;
+  }
+  if (zero) {
+    HIP_CHECK(hipMemsetAsync(out, 0, (size_t)mq->ncomp * N * sizeof(float), s));
+    return out;
+  }
+  const bool needs_strain = mode == MQ_STRAIN || mode == MQ_STRESS || mode == MQ_ELASTIC_EDENS ||
+                            mode == MQ_POYNTING || ((mode == EL_FORCE || mode == EL_BODY || mode == EL_ACCEL) && d.elastic);
+  if (needs_strain) compute_strain(s);
+  if (mode == MQ_STRAIN) return el.strain.get();
+  if (mode == MQ_STRESS) return el.stress.get();
+  if (mode == MQ_KINETIC) {
+    launch_elastic_local(d, nullptr, nullptr, el.v.get(), out, EL_KINETIC_EDENS, s);
+  } else if (mode == MQ_ELASTIC_EDENS) {
+    launch_elastic_local(d, el.stress.get(), el.strain.get(), nullptr, out, EL_ELASTIC_EDENS, s);
+  } else if (mode == MQ_POYNTING) {
+    launch_elastic_local(d, el.stress.get(), nullptr, el.v.get(), out, EL_POYNTING, s);
+  } else {
+    launch_elastic(d, el.enabled ? el.stress.get() : nullptr, el.enabled ? el.v.get() : nullptr, me_force_of(0),
+                   num_fm() > 1 ? me_force_of(1) : MEForce{}, out, (ElMode)mode, s);
+  }
+  return out;
+}
+
+// kinetic_energy / elastic_energy: ncells * <e> * V (elasticenergies.cu:40-47, :87-95)
+double MagnetCore::magnet_scalar(const std::string& q, hipStream_t s) {
+  const char* dens = q == "kinetic_energy" ? "kinetic_energy_density"
+                                           : (q == "elastic_energy" ? "elastic_energy_density" : nullptr);
+  if (!dens) throw Error(MPA_ERR_INVALID, "unknown scalar quantity '" + q + "'");
+  int nc = 0;
+  const float* dev = magnet_eval_device(dens, s, &nc);
+  double sum = 0.0;
+  world->reducer.component_sums(dev, 1, N, geom.get(), &sum, s);
+  const double avg = sum / (cellsingeo > 0 ? cellsingeo : 1);
+  const double V = world->cellsize[0] * world->cellsize[1] * world->cellsize[2];
+  return (double)N * avg * V;
+}
+
+namespace {
+// out = (a1 Ms1 m1 + a2 Ms2 m2) * scale, with scale = 1 or 1/(Ms1 + Ms2);
+// m2 may be nullptr (one magnet).
+__global__ void k_afm_combine(const float* __restrict__ m1, PS ms1, const float* __restrict__ m2, PS ms2, float a1,
+                              float a2, int normalize, int N, const uint8_t* __restrict__ geom,
+                              float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const bool in = !geom || geom[i];
+  const float s1 = ms1.at(i), s2 = ms2.at(i);
+  float sc = 1.f;
+  if (normalize) sc = (s1 + s2) != 0.f ? 1.f / (s1 + s2) : 0.f;
+  for (int c = 0; c < 3; ++c)
+    out[(size_t)c * N + i] =
+        in ? sc * (a1 * s1 * m1[(size_t)c * N + i] + (m2 ? a2 * s2 * m2[(size_t)c * N + i] : 0.f)) : 0.f;
+}
+
+__global__ void k_add2(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+}  // namespace
+
+// Kinetic and elastic energy densities added to out (energy.cu:64-66).
+void MagnetCore::add_elastic_energies(float* out, hipStream_t s) {
+  if (!el.enabled) return;
+  const ElDesc d = eldesc();
+  if (!param("rho").assured_zero()) launch_elastic_local(d, nullptr, nullptr, el.v.get(), out, EL_KINETIC_EDENS, s, true);
+  if (!elasticity_assured_zero()) {
+    compute_strain(s);
+    launch_elastic_local(d, el.stress.get(), el.strain.get(), nullptr, out, EL_ELASTIC_EDENS, s, true);
+  }
+}
+
+// ------------------------------------------------------------ Ferromagnet
+
+Ferromagnet::Ferromagnet(World* w, const int size[3], const int org[3], const uint8_t* geometry, std::string nm,
+                         Antiferromagnet* h)
+    : MagnetCore(w, size, org, geometry, std::move(nm)), host(h) {
+  add_params(kFmParams, (int)(sizeof(kFmParams) / sizeof(kFmParams[0])));
+  if (!host) add_params(kElasticParams, (int)(sizeof(kElasticParams) / sizeof(kElasticParams[0])));
   m.allocate((size_t)3 * N);
-  hdemag.allocate((size_t)3 * N);
+  if (!host) hdemag.allocate((size_t)3 * N);
   scratch.allocate((size_t)3 * N);
   std::vector<float> init;
-  random_unit_vectors(init, N, 0x5eedULL + (unsigned long long)w->magnets.size());
+  random_unit_vectors(init, N, 0x5eedULL + (unsigned long long)w->magnets.size() + 7ULL * w->afms.size() +
+                                   (host ? (unsigned long long)(name.back() - '0') * 1000ULL : 0ULL));
   if (!geom_host.empty())
     for (int i = 0; i < N; ++i)
       if (!geom_host[i]) init[i] = init[(size_t)N + i] = init[(size_t)2 * N + i] = 0.f;
   HIP_CHECK(hipMemcpy(m.get(), init.data(), init.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
-Param& Ferromagnet::param(const std::string& key) {
-  auto it = params.find(key);
-  if (it == params.end()) throw Error(MPA_ERR_INVALID, "unknown parameter '" + key + "'");
-  return it->second;
+Ferromagnet* Ferromagnet::sister() const {
+  if (!host) return nullptr;
+  return host->sub[0].get() == this ? host->sub[1].get() : host->sub[0].get();
 }
 
-unsigned Ferromagnet::active_terms() const {
+MagnetCore* Ferromagnet::owner() { return host ? static_cast<MagnetCore*>(host) : static_cast<MagnetCore*>(this); }
+
+bool Ferromagnet::magnetoelastic_assured_zero() {
+  if (param("msat").assured_zero() || (param("B1").assured_zero() && param("B2").assured_zero())) return true;
+  MagnetCore* own = owner();
+  return !own->el.enabled && !own->rigid_strain_applied();
+}
+
+unsigned Ferromagnet::active_terms() {
   auto az = [&](const char* k) { return params.at(k).assured_zero(); };
   const bool msat0 = az("msat");
   unsigned t = 0;
@@ -175,19 +462,28 @@ unsigned Ferromagnet::active_terms() const {
   const bool cub0 = (az("kc1") && az("kc2") && az("kc3")) || (az("anisC1") && az("anisC2")) || msat0;
   if (!uni0) t |= TERM_UNI;
   else if (!cub0) t |= TERM_CUBIC;
-  // zeeman.cu:38-51 (stray fields of other magnets are not coupled here)
+  // zeeman.cu:38-51 (stray fields of other magnets are refused, not coupled)
   const bool wb0 = world->bias[0] == 0 && world->bias[1] == 0 && world->bias[2] == 0;
   if (!(wb0 && az("bias_magnetic_field"))) t |= TERM_EXT;
-  // dmi.cu:80-85
+  // dmi.cu:12-17: a sublattice also counts the inter-sublattice tensor
   bool dmi0 = true;
   for (auto k : kDmiKeys) dmi0 = dmi0 && az(k);
+  if (host)
+    for (auto k : kDmiKeys) dmi0 = dmi0 && host->param(k).assured_zero();
   if (!(msat0 || dmi0)) t |= TERM_DMI;
   // demag.cpp:9-11
   if (enable_demag && !msat0) t |= TERM_DEMAG;
+  if (!magnetoelastic_assured_zero()) t |= TERM_ME;
+  // afmexchange.cu:13-25
+  if (host) {
+    const bool ms2_0 = sister()->param("msat").assured_zero();
+    if (!(host->param("afmex_nn").assured_zero() || ms2_0)) t |= TERM_AFM_NN;
+    if (!(host->param("afmex_cell").assured_zero() || ms2_0)) t |= TERM_AFM_SITE;
+  }
   return t;
 }
 
-FMDesc Ferromagnet::desc(unsigned terms) const {
+FMDesc Ferromagnet::desc(unsigned terms) {
   FMDesc d{};
   d.nx = n[0]; d.ny = n[1]; d.nz = n[2]; d.ncell = N;
   d.ox = origin[0]; d.oy = origin[1]; d.oz = origin[2];
@@ -212,10 +508,30 @@ FMDesc Ferromagnet::desc(unsigned terms) const {
   for (int c = 0; c < 3; ++c) d.wb[c] = (float)world->bias[c];
   d.openbc = openbc ? 1 : 0;
   d.terms = terms;
+  d.B1 = params.at("B1").ps();
+  d.B2 = params.at("B2").ps();
+  if (terms & TERM_ME) {
+    MagnetCore* own = owner();
+    d.me_rigid = own->rigid_strain_applied() ? 1 : 0;
+    d.rnorm = own->param("rigid_norm_strain").pv();
+    d.rshear = own->param("rigid_shear_strain").pv();
+    d.strain = own->el.strain.get();
+  }
+  if (host) {
+    Ferromagnet* s2 = sister();
+    d.sublattice = 1;
+    d.m2 = s2->m.get();
+    d.msat2 = s2->param("msat").ps();
+    d.afmex_cell = host->param("afmex_cell").ps();
+    d.afmex_nn = host->param("afmex_nn").ps();
+    d.latcon = host->param("latcon").ps();
+    for (int k = 0; k < D_COUNT; ++k) d.hdmi[k] = host->param(kDmiKeys[k]).ps();
+  }
   return d;
 }
 
 DemagPlan& Ferromagnet::demag() {
+  if (host) return host->demag();
   if (!demag_) {
     DemagGeometry g;
     for (int a = 0; a < 3; ++a) {
@@ -229,44 +545,63 @@ DemagPlan& Ferromagnet::demag() {
   return *demag_;
 }
 
+const float* Ferromagnet::demag_field() const { return host ? host->hdemag.get() : hdemag.get(); }
+
 void Ferromagnet::ensure_stage_buffers(int S) {
   if (x0.n == 0) x0.allocate((size_t)3 * N);
   while ((int)k.size() < S) k.emplace_back((size_t)3 * N);
+}
+
+// Demag field (the host's field of both sublattices for a sublattice) and the
+// strain of the magnet holding the displacement, as far as `terms` use them.
+void Ferromagnet::prepare(unsigned terms, hipStream_t s, Profiler* prof) {
+  if (terms & TERM_DEMAG) {
+    if (host) host->compute_demag(s, prof);
+    else demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s, prof);
+  }
+  if ((terms & TERM_ME) && !owner()->rigid_strain_applied()) owner()->compute_strain(s);
 }
 
 // evalExternalField (zeeman.cu:70-88) adds the stray field of every other
 // magnet that is a source (unless its msat is assured zero,
 // strayfield.cpp:88-95) to a destination magnet.  That coupling is not part
 // of this build, so an evaluation it would change is refused rather than
-// computed without it.
+// computed without it.  Sublattices receive the stray fields of their host's
+// other magnets.
 void Ferromagnet::check_stray_coupling() const {
-  if (!stray_destination) return;
-  for (const auto& o : world->magnets) {
-    if (o.get() == this || !o->stray_source || o->params.at("msat").assured_zero()) continue;
-    throw Error(MPA_ERR_RUNTIME, "the stray field of magnet '" + o->name + "' on magnet '" + name +
+  const MagnetCore* self = host ? static_cast<const MagnetCore*>(host) : static_cast<const MagnetCore*>(this);
+  if (!self->stray_destination) return;
+  auto refuse = [&](const std::string& src) {
+    throw Error(MPA_ERR_RUNTIME, "the stray field of magnet '" + src + "' on magnet '" + self->name +
                                      "' is not implemented; set enable_as_stray_field_source or "
                                      "enable_as_stray_field_destination to False");
+  };
+  for (const auto& o : world->magnets)
+    if (o.get() != self && o->stray_source && !o->param("msat").assured_zero()) refuse(o->name);
+  for (const auto& a : world->afms) {
+    const bool zero = a->sub[0]->param("msat").assured_zero() && a->sub[1]->param("msat").assured_zero();
+    if (a.get() != self && a->stray_source && !zero) refuse(a->name);
   }
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray) {
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray, bool prepared) {
   if (check_stray) check_stray_coupling();
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
-  if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s, prof);
+  if (!prepared) prepare(terms, s, prof);
   // algorithmic bytes: read m, read h_demag (when present), write the torque
   Timed t(prof, "k_field", 12.0 * N * ((terms & TERM_DEMAG) ? 3 : 2), s);
-  launch_field(desc(terms), m.get(), hdemag.get(), out, MODE_LLG, s);
+  launch_field(desc(terms), m.get(), demag_field(), out, MODE_LLG, s);
 }
 
 namespace {
 struct QInfo {
   const char* name;
   int ncomp;
-  unsigned terms;  // 0xffffffff: all active terms
-  int mode;        // FieldMode, or -1 for the magnetisation itself
+  unsigned terms;
+  int mode;  // FieldMode, or -1 for the magnetisation itself
 };
-constexpr unsigned ALL = 0xffffffffu;
+constexpr unsigned ALL = ~0u;
 const QInfo kQuantities[] = {
     {"magnetization", 3, 0, -1},
     {"exchange_field", 3, TERM_EXCH, MODE_FIELD},
@@ -274,6 +609,9 @@ const QInfo kQuantities[] = {
     {"external_field", 3, TERM_EXT, MODE_FIELD},
     {"dmi_field", 3, TERM_DMI, MODE_FIELD},
     {"demag_field", 3, TERM_DEMAG, MODE_FIELD},
+    {"magnetoelastic_field", 3, TERM_ME, MODE_FIELD},
+    {"homogeneous_exchange_field", 3, TERM_AFM_SITE, MODE_FIELD},
+    {"inhomogeneous_exchange_field", 3, TERM_AFM_NN, MODE_FIELD},
     {"effective_field", 3, ALL, MODE_FIELD},
     {"torque", 3, ALL, MODE_LLG},
     {"llg_torque", 3, ALL, MODE_LLG},
@@ -283,32 +621,64 @@ const QInfo kQuantities[] = {
     {"zeeman_energy_density", 1, TERM_EXT, MODE_EDENS},
     {"dmi_energy_density", 1, TERM_DMI, MODE_EDENS},
     {"demag_energy_density", 1, TERM_DEMAG, MODE_EDENS},
+    {"magnetoelastic_energy_density", 1, TERM_ME, MODE_EDENS},
+    {"homogeneous_exchange_energy_density", 1, TERM_AFM_SITE, MODE_EDENS},
+    {"inhomogeneous_exchange_energy_density", 1, TERM_AFM_NN, MODE_EDENS},
     {"total_energy_density", 1, ALL, MODE_EDENS},
 };
-const QInfo& qinfo(const std::string& q) {
+const QInfo* find_qinfo(const std::string& q) {
   for (const auto& qi : kQuantities)
-    if (q == qi.name) return qi;
-  throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
+    if (q == qi.name) return &qi;
+  return nullptr;
 }
 }  // namespace
 
-int Ferromagnet::quantity_ncomp(const std::string& q) const { return qinfo(q).ncomp; }
+int Ferromagnet::quantity_ncomp(const std::string& q) const {
+  if (const QInfo* qi = find_qinfo(q)) return qi->ncomp;
+  if (q == "magnetoelastic_force" || q == "full_magnetization") return 3;
+  if (!host) {
+    const int nc = magnet_quantity_ncomp(q);
+    if (nc) return nc;
+  }
+  throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
+}
 
 const float* Ferromagnet::eval_device(const std::string& q, hipStream_t s, int* ncomp) {
-  const QInfo& qi = qinfo(q);
-  *ncomp = qi.ncomp;
-  if (qi.mode < 0) return m.get();
-  if (qi.terms & TERM_EXT) check_stray_coupling();
-  const unsigned terms = active_terms() & qi.terms;
-  if (terms & TERM_DEMAG) demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s);
-  launch_field(desc(terms), m.get(), hdemag.get(), scratch.get(), (FieldMode)qi.mode, s);
+  const QInfo* qi = find_qinfo(q);
+  if (!qi) {
+    if (q == "magnetoelastic_force") {
+      // magnetoelasticforce.cu:97-111 (geometry and cell size of the owner)
+      *ncomp = 3;
+      MEForce f{};
+      if (!magnetoelastic_assured_zero()) f = MEForce{m.get(), param("B1").ps(), param("B2").ps()};
+      launch_elastic(owner()->eldesc(), nullptr, nullptr, f, MEForce{}, scratch.get(), EL_MEFORCE, s);
+      return scratch.get();
+    }
+    if (q == "full_magnetization") {
+      // Ms m (fullmag.cu:9-11)
+      *ncomp = 3;
+      k_afm_combine<<<(N + 255) / 256, 256, 0, s>>>(m.get(), param("msat").ps(), nullptr, PS{nullptr, 0.f}, 1.f, 0.f,
+                                                    0, N, geom.get(), scratch.get());
+      checkLaunch("k_afm_combine");
+      return scratch.get();
+    }
+    if (!host && magnet_quantity_ncomp(q)) return magnet_eval_device(q, s, ncomp);
+    throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
+  }
+  *ncomp = qi->ncomp;
+  if (qi->mode < 0) return m.get();
+  if (qi->terms & TERM_EXT) check_stray_coupling();
+  const unsigned terms = active_terms() & qi->terms;
+  prepare(terms, s, nullptr);
+  launch_field(desc(terms), m.get(), demag_field(), scratch.get(), (FieldMode)qi->mode, s);
+  if (q == "total_energy_density" && !host) add_elastic_energies(scratch.get(), s);
   return scratch.get();
 }
 
-int Ferromagnet::eval_quantity(const std::string& q, float* host, hipStream_t s) {
+int Ferromagnet::eval_quantity(const std::string& q, float* host_out, hipStream_t s) {
   int nc = 0;
   const float* dev = eval_device(q, s, &nc);
-  HIP_CHECK(hipMemcpyAsync(host, dev, (size_t)nc * N * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(host_out, dev, (size_t)nc * N * sizeof(float), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   return nc;
 }
@@ -316,7 +686,10 @@ int Ferromagnet::eval_quantity(const std::string& q, float* host, hipStream_t s)
 void Ferromagnet::average_quantity(const std::string& q, double* out, hipStream_t s) {
   int nc = 0;
   const float* dev = eval_device(q, s, &nc);
-  world->reducer.component_sums(dev, nc, N, geom.get(), out, s);
+  for (int c0 = 0; c0 < nc; c0 += 3) {
+    const int cnt = std::min(3, nc - c0);
+    world->reducer.component_sums(dev + (size_t)c0 * N, cnt, N, geom.get(), out + c0, s);
+  }
   for (int c = 0; c < nc; ++c) out[c] /= (cellsingeo > 0 ? cellsingeo : 1);
 }
 
@@ -327,24 +700,33 @@ double Ferromagnet::scalar_quantity(const std::string& q, hipStream_t s) {
   }
   // energies: E = ncells * <e> * V (exchange.cu:247-256, energy.cu:78-92)
   static const std::map<std::string, std::string> dens = {
-      {"exchange_energy", "exchange_energy_density"}, {"anisotropy_energy", "anisotropy_energy_density"},
-      {"zeeman_energy", "zeeman_energy_density"},     {"dmi_energy", "dmi_energy_density"},
-      {"demag_energy", "demag_energy_density"},       {"total_energy", "total_energy_density"}};
+      {"exchange_energy", "exchange_energy_density"},
+      {"anisotropy_energy", "anisotropy_energy_density"},
+      {"zeeman_energy", "zeeman_energy_density"},
+      {"dmi_energy", "dmi_energy_density"},
+      {"demag_energy", "demag_energy_density"},
+      {"magnetoelastic_energy", "magnetoelastic_energy_density"},
+      {"homogeneous_exchange_energy", "homogeneous_exchange_energy_density"},
+      {"inhomogeneous_exchange_energy", "inhomogeneous_exchange_energy_density"},
+      {"total_energy", "total_energy_density"}};
   auto it = dens.find(q);
-  if (it == dens.end()) throw Error(MPA_ERR_INVALID, "unknown scalar quantity '" + q + "'");
+  if (it == dens.end()) {
+    if (!host && (q == "kinetic_energy" || q == "elastic_energy")) return magnet_scalar(q, s);
+    throw Error(MPA_ERR_INVALID, "unknown scalar quantity '" + q + "'");
+  }
   double avg = 0.0;
   average_quantity(it->second, &avg, s);
   const double V = world->cellsize[0] * world->cellsize[1] * world->cellsize[2];
   return (double)N * avg * V;
 }
 
-void Ferromagnet::set_magnetization(const float* host, long long count, hipStream_t s) {
+void Ferromagnet::set_magnetization(const float* host_in, long long count, hipStream_t s) {
   std::vector<float> buf((size_t)3 * N);
   if (count == 3) {
     for (int c = 0; c < 3; ++c)
-      for (int i = 0; i < N; ++i) buf[(size_t)c * N + i] = host[c];
+      for (int i = 0; i < N; ++i) buf[(size_t)c * N + i] = host_in[c];
   } else if (count == 3LL * N) {
-    std::memcpy(buf.data(), host, buf.size() * sizeof(float));
+    std::memcpy(buf.data(), host_in, buf.size() * sizeof(float));
   } else {
     throw Error(MPA_ERR_INVALID, "magnetization: expected 3 or 3*ncells values");
   }
@@ -365,9 +747,166 @@ void Ferromagnet::set_magnetization(const float* host, long long count, hipStrea
   world->touch();
 }
 
-void Ferromagnet::get_magnetization(float* host, hipStream_t s) {
-  HIP_CHECK(hipMemcpyAsync(host, m.get(), (size_t)3 * N * sizeof(float), hipMemcpyDeviceToHost, s));
+void Ferromagnet::get_magnetization(float* host_out, hipStream_t s) {
+  HIP_CHECK(hipMemcpyAsync(host_out, m.get(), (size_t)3 * N * sizeof(float), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------ Antiferromagnet
+
+Antiferromagnet::Antiferromagnet(World* w, const int size[3], const int org[3], const uint8_t* geometry,
+                                 std::string nm)
+    : MagnetCore(w, size, org, geometry, std::move(nm)) {
+  add_params(kAfmParams, (int)(sizeof(kAfmParams) / sizeof(kAfmParams[0])));
+  add_params(kElasticParams, (int)(sizeof(kElasticParams) / sizeof(kElasticParams[0])));
+  const uint8_t* g = geom_host.empty() ? nullptr : geom_host.data();
+  sub[0] = std::make_unique<Ferromagnet>(w, size, org, g, name + ":sublattice_1", this);
+  sub[1] = std::make_unique<Ferromagnet>(w, size, org, g, name + ":sublattice_2", this);
+  hdemag.allocate((size_t)3 * N);
+  scratch.allocate((size_t)3 * N);
+}
+
+DemagPlan& Antiferromagnet::demag() {
+  if (!demag_) {
+    DemagGeometry g;
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = n[a];
+      g.cs[a] = world->cellsize[a];
+      g.mastergrid[a] = world->mastergrid_size[a];
+      g.pbc[a] = world->pbc[a];
+    }
+    demag_ = std::make_unique<DemagPlan>(g);
+  }
+  return *demag_;
+}
+
+// StrayFieldFFTExecutor::exec for an antiferromagnet source: the field of
+// Ms1 m1 + Ms2 m2 (strayfieldfft.cu:206-213), shared by both sublattices.
+void Antiferromagnet::compute_demag(hipStream_t s, Profiler* prof) {
+  demag().exec(sub[0]->m.get(), sub[0]->param("msat").ps(), geom.get(), hdemag.get(), s, prof, sub[1]->m.get(),
+               sub[1]->param("msat").ps());
+}
+
+int Antiferromagnet::quantity_ncomp(const std::string& q) const {
+  if (q == "neel_vector" || q == "full_magnetization") return 3;
+  if (q == "total_energy_density") return 1;
+  const int nc = magnet_quantity_ncomp(q);
+  if (nc) return nc;
+  throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
+}
+
+
+// neel_vector (Ms1 m1 - Ms2 m2)/(Ms1 + Ms2) and full_magnetization Ms1 m1 + Ms2 m2
+// (antiferromagnet.py:341-356 of the reference package).
+const float* Antiferromagnet::eval_device(const std::string& q, hipStream_t s, int* ncomp) {
+  if (q == "neel_vector" || q == "full_magnetization") {
+    *ncomp = 3;
+    const bool neel = q == "neel_vector";
+    k_afm_combine<<<(N + 255) / 256, 256, 0, s>>>(sub[0]->m.get(), sub[0]->param("msat").ps(), sub[1]->m.get(),
+                                                  sub[1]->param("msat").ps(), 1.f, neel ? -1.f : 1.f, neel ? 1 : 0,
+                                                  N, geom.get(), scratch.get());
+    checkLaunch("k_afm_combine");
+    return scratch.get();
+  }
+  if (q == "total_energy_density") {
+    // energy.cu:70-76: both sublattices, then the magnet's kinetic and elastic
+    *ncomp = 1;
+    int nc = 0;
+    const float* e1 = sub[0]->eval_device(q, s, &nc);
+    const float* e2 = sub[1]->eval_device(q, s, &nc);
+    k_add2<<<(N + 255) / 256, 256, 0, s>>>(e1, e2, scratch.get(), N);
+    checkLaunch("k_add2");
+    add_elastic_energies(scratch.get(), s);
+    return scratch.get();
+  }
+  return magnet_eval_device(q, s, ncomp);
+}
+
+double Antiferromagnet::scalar_quantity(const std::string& q, hipStream_t s) {
+  if (q == "kinetic_energy" || q == "elastic_energy") return magnet_scalar(q, s);
+  if (q != "total_energy") throw Error(MPA_ERR_INVALID, "unknown scalar quantity '" + q + "'");
+  double avg = 0.0;
+  average_quantity("total_energy_density", &avg, s);
+  const double V = world->cellsize[0] * world->cellsize[1] * world->cellsize[2];
+  return (double)N * avg * V;
+}
+
+int Antiferromagnet::eval_quantity(const std::string& q, float* host_out, hipStream_t s) {
+  int nc = 0;
+  const float* dev = eval_device(q, s, &nc);
+  HIP_CHECK(hipMemcpyAsync(host_out, dev, (size_t)nc * N * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return nc;
+}
+
+void Antiferromagnet::average_quantity(const std::string& q, double* out, hipStream_t s) {
+  int nc = 0;
+  const float* dev = eval_device(q, s, &nc);
+  for (int c0 = 0; c0 < nc; c0 += 3) {
+    const int cnt = std::min(3, nc - c0);
+    world->reducer.component_sums(dev + (size_t)c0 * N, cnt, N, geom.get(), out + c0, s);
+  }
+  for (int c = 0; c < nc; ++c) out[c] /= (cellsingeo > 0 ? cellsingeo : 1);
+}
+
+// ------------------------------------------------------------ equations
+
+std::vector<StateVar> World::state_vars() {
+  std::vector<StateVar> v;
+  auto add_mag = [&](Ferromagnet* fm) {
+    v.push_back(StateVar{StateVar::MAG, fm, nullptr, fm->m.get(), &fm->x0, &fm->k, fm->N, fm->geom_ptr(), true});
+  };
+  for (auto& fm : magnets) add_mag(fm.get());
+  for (auto& a : afms) {
+    add_mag(a->sub[0].get());
+    add_mag(a->sub[1].get());
+  }
+  auto add_elastic = [&](MagnetCore* own) {
+    if (!own->el.enabled) return;
+    v.push_back(StateVar{StateVar::DISP, nullptr, own, own->el.u.get(), &own->el.u0, &own->el.ku, own->N,
+                         own->geom_ptr(), false});
+    v.push_back(StateVar{StateVar::VEL, nullptr, own, own->el.v.get(), &own->el.v0, &own->el.kv, own->N,
+                         own->geom_ptr(), false});
+  };
+  for (auto& fm : magnets) add_elastic(fm.get());
+  for (auto& a : afms) add_elastic(a.get());
+  return v;
+}
+
+// All right-hand sides at the current (stage) state: strain of every magnet
+// with elastodynamics and the stray field of every antiferromagnet once,
+// then the torques, velocities and accelerations.
+void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray) {
+  for (auto& fm : magnets)
+    if (fm->el.enabled) fm->compute_strain(s);
+  for (auto& a : afms) {
+    if (a->el.enabled) a->compute_strain(s);
+    if ((a->sub[0]->active_terms() | a->sub[1]->active_terms()) & TERM_DEMAG) a->compute_demag(s, &prof);
+  }
+  for (const auto& sv : vars) {
+    float* out = (*sv.k)[stage].get();
+    if (sv.kind == StateVar::MAG) {
+      Ferromagnet* fm = sv.fm;
+      if (fm->host) {
+        fm->eval_rhs(out, s, check_stray, true);
+      } else {
+        const unsigned t = fm->active_terms();
+        if (t & TERM_DEMAG)
+          fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
+        fm->eval_rhs(out, s, check_stray, true);
+      }
+    } else if (sv.kind == StateVar::DISP) {
+      // du/dt = v (elastodynamics.cu:93-96)
+      Timed t(&prof, "copy_v", 24.0 * sv.N, s);
+      HIP_CHECK(hipMemcpyAsync(out, sv.owner->el.v.get(), (size_t)3 * sv.N * sizeof(float),
+                               hipMemcpyDeviceToDevice, s));
+    } else {
+      MagnetCore* own = sv.owner;
+      Timed t(&prof, "k_elastic", 12.0 * sv.N * 4, s);
+      launch_elastic(own->eldesc(), own->el.stress.get(), own->el.v.get(), own->me_force_of(0),
+                     own->num_fm() > 1 ? own->me_force_of(1) : MEForce{}, out, EL_ACCEL, s);
+    }
+  }
 }
 
 // ------------------------------------------------------------ TimeSolver
@@ -386,23 +925,25 @@ void TimeSolver::equations_changed() {
   if (adaptive) dt = sensible_timestep();
 }
 
-// TimeSolver::sensibleTimeStep (src/core/timesolver.cpp:37-47)
+// TimeSolver::sensibleTimeStep (src/core/timesolver.cpp:37-47).  The
+// reference re-estimates dt whenever a magnet is added (timesolver.cpp:49-52),
+// before the caller can switch stray coupling off; the estimate runs without
+// the refusal, the steps keep it.
 double TimeSolver::sensible_timestep() {
-  if (world_->magnets.empty()) return 0.0;
+  std::vector<StateVar> vars = world_->state_vars();
+  if (vars.empty()) return 0.0;
+  for (auto& sv : vars)
+    while ((int)sv.k->size() < 1) sv.k->emplace_back((size_t)3 * sv.N);
+  world_->eval_rhs_all(vars, 0, world_->stream, false);
+  nrhs += (long long)vars.size();
   double mx = 0.0;
-  for (auto& mag : world_->magnets) {
-    // The reference re-estimates dt whenever a magnet is added
-    // (timesolver.cpp:49-52), before the caller can switch stray coupling
-    // off; the estimate runs without the refusal, the steps keep it.
-    mag->eval_rhs(mag->scratch.get(), world_->stream, false);
-    ++nrhs;
-    mx = std::max(mx, (double)world_->reducer.max_vec_norm(mag->scratch.get(), mag->N, mag->geom_ptr(), world_->stream));
-  }
+  for (auto& sv : vars)
+    mx = std::max(mx, (double)world_->reducer.max_vec_norm((*sv.k)[0].get(), sv.N, sv.geom, world_->stream));
+  fsal_valid_ = false;  // k[0] now holds the rhs at the current state, not the FSAL value
   if (mx == 0.0) return sensible_default;
   return sensible_factor / mx;
 }
 
-// TimeSolver::adaptTimeStep (src/core/timesolver.cpp:60-75)
 void TimeSolver::adapt(double corr) {
   if (!adaptive) return;
   if (std::isnan(corr)) corr = 1.0;
@@ -413,18 +954,20 @@ void TimeSolver::adapt(double corr) {
   dt *= corr;
 }
 
-// RungeKuttaStepper::step (src/core/rungekutta.cpp:25-84).  The stage states
-// are x0 + dt * sum_j a_sj k_j, normalised; the final state uses weights1;
-// the error is the max-norm of dt * sum_j (w1_j - w2_j) k_j.  For tableaus
-// whose last stage is the final state (DormandPrince, BogackiShampine) the
-// last stage's k is the next step's first k (first-same-as-last), and the
-// final combination is the last stage state itself.
+// RungeKuttaStepper::step (src/core/rungekutta.cpp:25-84).  Every stage first
+// sets the stage state of all variables (x0 + dt * sum_j a_sj k_j,
+// normalised for magnetisations), then evaluates all right-hand sides; the
+// final state uses weights1; the error is the largest max-norm of
+// dt * sum_j (w1_j - w2_j) k_j over the variables.  For tableaus whose last
+// stage is the final state (DormandPrince, BogackiShampine) the last stage's
+// k is the next step's first k (first-same-as-last), and the final
+// combination is the last stage state itself.
 void TimeSolver::step() {
   if (dt <= 0)
     throw Error(MPA_ERR_RUNTIME,
                 "Timesolver can not make a step because the timestep is smaller than or equal to zero.");
-  auto& mags = world_->magnets;
-  if (mags.empty()) {
+  std::vector<StateVar> vars = world_->state_vars();
+  if (vars.empty()) {
     time += dt;
     return;
   }
@@ -432,61 +975,49 @@ void TimeSolver::step() {
   const Tableau& tb = tableau_by_name(method);
   const int S = (int)tb.nodes.size();
   Profiler* prof = &world_->prof;
-  for (auto& mag : mags) {
-    mag->ensure_stage_buffers(S);
-    Timed t(prof, "copy_x0", 24.0 * mag->N, s);
-    HIP_CHECK(hipMemcpyAsync(mag->x0.get(), mag->m.get(), (size_t)3 * mag->N * sizeof(float),
-                             hipMemcpyDeviceToDevice, s));
+  for (auto& sv : vars) {
+    if (sv.x0->n == 0) sv.x0->allocate((size_t)3 * sv.N);
+    while ((int)sv.k->size() < S) sv.k->emplace_back((size_t)3 * sv.N);
+    Timed t(prof, "copy_x0", 24.0 * sv.N, s);
+    HIP_CHECK(hipMemcpyAsync(sv.x0->get(), sv.x, (size_t)3 * sv.N * sizeof(float), hipMemcpyDeviceToDevice, s));
   }
   const double t0 = time;
   bool k0_valid = fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
+  auto combine = [&](const std::vector<double>& w, int nk, double h) {
+    for (auto& sv : vars) {
+      Combo co{};
+      co.nk = nk;
+      for (int j = 0; j < nk; ++j) {
+        co.k[j] = (*sv.k)[j].get();
+        co.c[j] = (float)(h * w[j]);
+      }
+      Timed t(prof, "k_combine", 12.0 * sv.N * (nk + 2), s);
+      launch_combine(sv.x, sv.x0->get(), co, 3, sv.N, sv.geom, sv.normalize, s);
+    }
+  };
   while (true) {
     const double h = dt;
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
-      if (st > 0) {
-        for (auto& mag : mags) {
-          Combo co{};
-          co.nk = st;
-          for (int j = 0; j < st; ++j) {
-            co.k[j] = mag->k[j].get();
-            co.c[j] = (float)(h * tb.A[st][j]);
-          }
-          Timed t(prof, "k_combine", 12.0 * mag->N * (st + 2), s);
-          launch_combine(mag->m.get(), mag->x0.get(), co, 3, mag->N, mag->geom_ptr(), true, s);
-        }
-      }
+      if (st > 0) combine(tb.A[st], st, h);
       if (st == 0 && k0_valid) continue;
-      for (auto& mag : mags) {
-        mag->eval_rhs(mag->k[st].get(), s);
-        ++nrhs;
-      }
+      world_->eval_rhs_all(vars, st, s, true);
+      nrhs += (long long)vars.size();
     }
     time = t0 + h;
-    if (!tb.fsal) {
-      for (auto& mag : mags) {
-        Combo co{};
-        co.nk = S;
-        for (int j = 0; j < S; ++j) {
-          co.k[j] = mag->k[j].get();
-          co.c[j] = (float)(h * tb.w1[j]);
-        }
-        Timed t(prof, "k_combine", 12.0 * mag->N * (S + 2), s);
-        launch_combine(mag->m.get(), mag->x0.get(), co, 3, mag->N, mag->geom_ptr(), true, s);
-      }
-    }
+    if (!tb.fsal) combine(tb.w1, S, h);
     ++nsteps;
     if (!adaptive) break;
     double err = 0.0;
-    for (auto& mag : mags) {
+    for (auto& sv : vars) {
       Combo co{};
       co.nk = S;
       for (int j = 0; j < S; ++j) {
-        co.k[j] = mag->k[j].get();
+        co.k[j] = (*sv.k)[j].get();
         co.c[j] = (float)(h * (tb.w1[j] - tb.w2[j]));
       }
-      Timed t(prof, "k_combo_maxnorm", 12.0 * mag->N * S, s);
-      err = std::max(err, (double)world_->reducer.max_combo_norm(co, mag->N, mag->geom_ptr(), s));
+      Timed t(prof, "k_combo_maxnorm", 12.0 * sv.N * S, s);
+      err = std::max(err, (double)world_->reducer.max_combo_norm(co, sv.N, sv.geom, s));
     }
     const bool success = err < max_error;
     const double ratio = max_error / err;  // err == 0 gives +inf, clamped by adapt()
@@ -494,14 +1025,13 @@ void TimeSolver::step() {
     adapt(corr);
     if (success) break;
     ++nrejected;
-    for (auto& mag : mags)
-      HIP_CHECK(hipMemcpyAsync(mag->m.get(), mag->x0.get(), (size_t)3 * mag->N * sizeof(float),
-                               hipMemcpyDeviceToDevice, s));
+    for (auto& sv : vars)
+      HIP_CHECK(hipMemcpyAsync(sv.x, sv.x0->get(), (size_t)3 * sv.N * sizeof(float), hipMemcpyDeviceToDevice, s));
     time = t0;
     k0_valid = true;  // k[0] = rhs(x0, t0) was computed in this attempt
   }
   if (tb.fsal) {
-    for (auto& mag : mags) std::swap(mag->k[0], mag->k[S - 1]);
+    for (auto& sv : vars) std::swap((*sv.k)[0], (*sv.k)[S - 1]);
     fsal_valid_ = true;
     fsal_time_ = time;
     fsal_version_ = world_->version;
@@ -510,8 +1040,8 @@ void TimeSolver::step() {
   }
 }
 
-void TimeSolver::steps(long long n) {
-  for (long long i = 0; i < n; ++i) step();
+void TimeSolver::steps(long long nstep) {
+  for (long long i = 0; i < nstep; ++i) step();
   HIP_CHECK(hipStreamSynchronize(world_->stream));
   world_->prof.flush_sync(world_->stream);
 }
@@ -547,6 +1077,7 @@ World::World(const double cs[3], const int mg_size[3], const int mg_origin[3], c
 
 World::~World() {
   magnets.clear();
+  afms.clear();
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -568,16 +1099,33 @@ void World::check_in_mastergrid(const int size[3], const int org[3], const int m
   }
 }
 
+void World::check_name(const std::string& name) {
+  if (find(name) || find_afm(name)) throw Error(MPA_ERR_INVALID, "A magnet with the name '" + name + "' already exists");
+}
+
 Ferromagnet* World::add_ferromagnet(const int size[3], const int org[3], const uint8_t* geometry, const char* nm) {
   static int unnamed = 1;  // ferromagnet_<n> naming of mumaxworld.cpp:61-64
   std::string name = nm ? nm : "";
   if (name.empty()) name = "ferromagnet_" + std::to_string(unnamed++);
-  if (find(name)) throw Error(MPA_ERR_INVALID, "A magnet with the name '" + name + "' already exists");
+  check_name(name);
   check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
   magnets.push_back(std::make_unique<Ferromagnet>(this, size, org, geometry, name));
   touch();
   solver.equations_changed();
   return magnets.back().get();
+}
+
+Antiferromagnet* World::add_antiferromagnet(const int size[3], const int org[3], const uint8_t* geometry,
+                                            const char* nm) {
+  static int unnamed = 1;  // antiferromagnet_<n> naming of mumaxworld.cpp:86-89
+  std::string name = nm ? nm : "";
+  if (name.empty()) name = "antiferromagnet_" + std::to_string(unnamed++);
+  check_name(name);
+  check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
+  afms.push_back(std::make_unique<Antiferromagnet>(this, size, org, geometry, name));
+  touch();
+  solver.equations_changed();
+  return afms.back().get();
 }
 
 Ferromagnet* World::find(const std::string& name) {
@@ -586,17 +1134,36 @@ Ferromagnet* World::find(const std::string& name) {
   return nullptr;
 }
 
+Antiferromagnet* World::find_afm(const std::string& name) {
+  for (auto& a : afms)
+    if (a->name == name) return a.get();
+  return nullptr;
+}
+
+std::vector<Ferromagnet*> World::all_ferromagnets() {
+  std::vector<Ferromagnet*> v;
+  for (auto& m : magnets) v.push_back(m.get());
+  for (auto& a : afms) {
+    v.push_back(a->sub[0].get());
+    v.push_back(a->sub[1].get());
+  }
+  return v;
+}
+
 void World::bounding_grid(int size[3], int org[3]) const {
-  if (magnets.empty()) {
+  std::vector<const MagnetCore*> all;
+  for (auto& m : magnets) all.push_back(m.get());
+  for (auto& a : afms) all.push_back(a.get());
+  if (all.empty()) {
     for (int a = 0; a < 3; ++a) size[a] = org[a] = 0;
     return;
   }
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
-    lo[a] = magnets[0]->origin[a];
-    hi[a] = lo[a] + magnets[0]->n[a];
+    lo[a] = all[0]->origin[a];
+    hi[a] = lo[a] + all[0]->n[a];
   }
-  for (auto& m : magnets)
+  for (auto* m : all)
     for (int a = 0; a < 3; ++a) {
       lo[a] = std::min(lo[a], m->origin[a]);
       hi[a] = std::max(hi[a], m->origin[a] + m->n[a]);
@@ -609,6 +1176,7 @@ void World::bounding_grid(int size[3], int org[3]) const {
 
 void World::pbc_changed() {
   for (auto& m : magnets) m->invalidate_demag();
+  for (auto& a : afms) a->invalidate_demag();
   touch();
 }
 
@@ -616,6 +1184,7 @@ void World::pbc_changed() {
 void World::set_pbc(const int mg_size[3], const int mg_origin[3], const int reps[3]) {
   check_pbc(mg_size, reps);
   for (auto& m : magnets) check_in_mastergrid(m->n, m->origin, mg_size, mg_origin);
+  for (auto& a : afms) check_in_mastergrid(a->n, a->origin, mg_size, mg_origin);
   for (int a = 0; a < 3; ++a) {
     mastergrid_size[a] = mg_size[a];
     mastergrid_origin[a] = mg_origin[a];

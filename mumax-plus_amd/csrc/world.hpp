@@ -1,7 +1,9 @@
-// Host object model: World, Ferromagnet, TimeSolver.
+// Host object model: World, magnets (ferromagnets, antiferromagnets and their
+// sublattices) and the Runge-Kutta TimeSolver.
 //
 // Mirrors the reference's object graph (MumaxWorld src/physics/mumaxworld.cpp,
-// Ferromagnet src/physics/ferromagnet.cpp:18-82, TimeSolver
+// Magnet src/physics/magnet.cpp, Ferromagnet src/physics/ferromagnet.cpp:18-82,
+// Antiferromagnet src/physics/antiferromagnet.cpp, TimeSolver
 // src/core/timesolver.cpp, RungeKuttaStepper src/core/rungekutta.cpp) with
 // device memory owned per magnet and all work on one HIP stream per world.
 #pragma once
@@ -13,6 +15,7 @@
 
 #include "common.hpp"
 #include "demag.hpp"
+#include "elastic.hpp"
 #include "fields.hpp"
 #include "profiler.hpp"
 #include "rk.hpp"
@@ -20,6 +23,8 @@
 namespace mpa {
 
 class World;
+class Ferromagnet;
+class Antiferromagnet;
 
 // A scalar (ncomp 1) or vector (ncomp 3) material parameter: a uniform value
 // or one value per cell (Parameter / VectorParameter, src/core/parameter.hpp).
@@ -45,9 +50,21 @@ struct Tableau {
 };
 const Tableau& tableau_by_name(const std::string& name);  // throws MPA_ERR_INVALID
 
-class Ferromagnet {
+// Elastic displacement and velocity of a magnet (magnet.cpp:177-213), with
+// their Runge-Kutta buffers and the strain / stress of the current u.
+struct ElasticState {
+  bool enabled = false;
+  DeviceArray<float> u, v;    // 3N each
+  DeviceArray<float> u0, v0;  // step start
+  std::vector<DeviceArray<float>> ku, kv;
+  DeviceArray<float> strain, stress;  // 6N each
+};
+
+// Grid, geometry, parameters and elastic state common to every magnet.
+class MagnetCore {
  public:
-  Ferromagnet(World* world, const int size[3], const int origin[3], const uint8_t* geometry, std::string name);
+  MagnetCore(World* w, const int size[3], const int origin[3], const uint8_t* geometry, std::string name);
+  virtual ~MagnetCore() = default;
 
   World* world;
   std::string name;
@@ -56,44 +73,133 @@ class Ferromagnet {
   DeviceArray<uint8_t> geom;
   int cellsingeo;
   std::map<std::string, Param> params;
-  bool enable_demag = true;
-  bool openbc = false;
-  bool stray_source = true;       // Magnet::enableAsStrayFieldSource (magnet.cpp:26-34)
-  bool stray_destination = true;  // Magnet::enableAsStrayFieldDestination
-
-  DeviceArray<float> m;       // magnetisation, 3N
-  DeviceArray<float> x0;      // RK start state
-  DeviceArray<float> hdemag;  // demag field scratch
-  DeviceArray<float> scratch; // quantity evaluation
-  std::vector<DeviceArray<float>> k;
+  ElasticState el;
+  // Magnet::enableAsStrayFieldSource / Destination (magnet.cpp:23-24); a
+  // sublattice follows its host's flags (zeeman.cu:13-22).
+  bool stray_source = true;
+  bool stray_destination = true;
+  DeviceArray<float> qbuf;  // 6N, magnet-level quantity results
 
   const uint8_t* geom_ptr() const { return geom.get(); }
   Param& param(const std::string& key);
-  unsigned active_terms() const;  // the reference's "assured zero" rules
+  const Param& param(const std::string& key) const;
+  bool has_param(const std::string& key) const { return params.count(key) != 0; }
+
+  // Elastodynamics of a magnet that is not a sublattice.
+  void set_elastodynamics(bool on);
+  bool elasticity_assured_zero() const;  // elastodynamics.cu:47-51
+  bool rigid_strain_applied() const;     // magnetoelasticfield.cu:29-42
+  ElDesc eldesc() const;
+  // Ferromagnets whose magnetoelastic force acts on this magnet.
+  virtual int num_fm() const = 0;
+  virtual Ferromagnet* fm_at(int i) = 0;
+  MEForce me_force_of(int i);
+  void compute_strain(hipStream_t s);  // el.strain and el.stress from el.u
+  void ensure_elastic_stage_buffers(int S);
+  void set_variable(const std::string& which, const float* host, long long count, hipStream_t s);
+  void get_variable(const std::string& which, float* host, hipStream_t s);
+  // Magnet-level quantities (strain_tensor, stress_tensor, elastic forces,
+  // acceleration, velocity); ncomp, or 0 if q is not one of them.
+  int magnet_quantity_ncomp(const std::string& q) const;
+  const float* magnet_eval_device(const std::string& q, hipStream_t s, int* ncomp);
+  // kinetic_energy, elastic_energy
+  double magnet_scalar(const std::string& q, hipStream_t s);
+  void add_elastic_energies(float* out, hipStream_t s);
+
+ protected:
+  void add_params(const void* specs, int count);
+};
+
+class Ferromagnet : public MagnetCore {
+ public:
+  Ferromagnet(World* world, const int size[3], const int origin[3], const uint8_t* geometry, std::string name,
+              Antiferromagnet* host = nullptr);
+
+  Antiferromagnet* host;  // non-null for a sublattice
+  bool enable_demag = true;
+  bool openbc = false;
+
+  DeviceArray<float> m;       // magnetisation, 3N
+  DeviceArray<float> x0;      // RK start state
+  DeviceArray<float> hdemag;  // demag field scratch (independent magnets)
+  DeviceArray<float> scratch; // quantity evaluation
+  std::vector<DeviceArray<float>> k;
+
+  Ferromagnet* sister() const;
+  MagnetCore* owner();  // the magnet holding the displacement: the host, or this
+  unsigned active_terms();  // the reference's "assured zero" rules
+  bool magnetoelastic_assured_zero();  // magnetoelasticfield.cu:10-42
   // Throws when another magnet's stray field would enter this magnet's
   // external field (not built); called by every evaluation that includes it.
   void check_stray_coupling() const;
-  FMDesc desc(unsigned terms) const;
+  FMDesc desc(unsigned terms);
   DemagPlan& demag();
   void invalidate_demag() { demag_.reset(); }
   void ensure_stage_buffers(int S);
+  // Fills the demag field and the strain this magnet's terms need.
+  void prepare(unsigned terms, hipStream_t s, Profiler* prof);
 
-  // out (device, 3N) = torque of the current magnetisation
-  void eval_rhs(float* out, hipStream_t s, bool check_stray = true);
-  // Evaluate a named quantity into host memory (ncomp*N floats); returns ncomp.
+  // out (device, 3N) = torque of the current magnetisation.  With prepared,
+  // the demag field and strain are taken as already computed.
+  void eval_rhs(float* out, hipStream_t s, bool check_stray = true, bool prepared = false);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;
-  // Average of each component over the geometry cells (FP64).
   void average_quantity(const std::string& q, double* out, hipStream_t s);
   double scalar_quantity(const std::string& q, hipStream_t s);
 
   void set_magnetization(const float* host, long long count, hipStream_t s);
   void get_magnetization(float* host, hipStream_t s);
 
- private:
-  // field / density of the given term set into scratch (device)
+  int num_fm() const override { return 1; }
+  Ferromagnet* fm_at(int) override { return this; }
+
   const float* eval_device(const std::string& q, hipStream_t s, int* ncomp);
+
+ private:
+  const float* demag_field() const;
   std::unique_ptr<DemagPlan> demag_;
+};
+
+// Two ferromagnetic sublattices on one grid (antiferromagnet.cpp:15-28).
+class Antiferromagnet : public MagnetCore {
+ public:
+  Antiferromagnet(World* world, const int size[3], const int origin[3], const uint8_t* geometry, std::string name);
+
+  std::unique_ptr<Ferromagnet> sub[2];
+  DeviceArray<float> hdemag;  // stray field of both sublattices
+  DeviceArray<float> scratch;
+
+  DemagPlan& demag();
+  void invalidate_demag() { demag_.reset(); }
+  void compute_demag(hipStream_t s, Profiler* prof);
+
+  int num_fm() const override { return 2; }
+  Ferromagnet* fm_at(int i) override { return sub[i].get(); }
+
+  // Quantities of the whole antiferromagnet (neel_vector, full_magnetization
+  // and the magnet-level elastic ones).
+  int quantity_ncomp(const std::string& q) const;
+  const float* eval_device(const std::string& q, hipStream_t s, int* ncomp);
+  int eval_quantity(const std::string& q, float* host, hipStream_t s);
+  void average_quantity(const std::string& q, double* out, hipStream_t s);
+  double scalar_quantity(const std::string& q, hipStream_t s);  // total_energy, kinetic_energy, elastic_energy
+
+ private:
+  std::unique_ptr<DemagPlan> demag_;
+};
+
+// One state variable of the time integration (DynamicEquation,
+// src/core/dynamicequation.cpp:10-37).
+struct StateVar {
+  enum Kind { MAG, DISP, VEL } kind;
+  Ferromagnet* fm;    // MAG
+  MagnetCore* owner;  // DISP / VEL
+  float* x;
+  DeviceArray<float>* x0;
+  std::vector<DeviceArray<float>>* k;
+  int N;
+  const uint8_t* geom;
+  bool normalize;
 };
 
 class TimeSolver {
@@ -130,7 +236,8 @@ class World {
   double cellsize[3];
   int mastergrid_size[3], mastergrid_origin[3], pbc[3];
   double bias[3] = {0, 0, 0};
-  std::vector<std::unique_ptr<Ferromagnet>> magnets;
+  std::vector<std::unique_ptr<Ferromagnet>> magnets;  // independent ferromagnets
+  std::vector<std::unique_ptr<Antiferromagnet>> afms;
   TimeSolver solver;
   hipStream_t stream = nullptr;
   Reducer reducer;
@@ -138,16 +245,28 @@ class World {
   unsigned long long version = 1;  // bumped by every state change (invalidates FSAL reuse)
 
   Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry, const char* name);
+  Antiferromagnet* add_antiferromagnet(const int size[3], const int origin[3], const uint8_t* geometry,
+                                       const char* name);
   Ferromagnet* find(const std::string& name);
+  Antiferromagnet* find_afm(const std::string& name);
+  // every ferromagnet with a magnetisation: independent ones and sublattices
+  std::vector<Ferromagnet*> all_ferromagnets();
   void bounding_grid(int size[3], int origin[3]) const;
   void set_pbc(const int mg_size[3], const int mg_origin[3], const int reps[3]);
   void set_pbc_auto(const int reps[3]);
   void unset_pbc();
   void touch() { ++version; }
 
+  // Equations in the order of MumaxWorld::resetTimeSolverEquations
+  // (mumaxworld.cpp:159-205).
+  std::vector<StateVar> state_vars();
+  // k[stage] of every variable at the current state (all stage states set).
+  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray);
+
  private:
   void check_pbc(const int mg_size[3], const int reps[3]) const;
   void check_in_mastergrid(const int size[3], const int origin[3], const int mg_size[3], const int mg_origin[3]) const;
+  void check_name(const std::string& name);
   void pbc_changed();
 };
 

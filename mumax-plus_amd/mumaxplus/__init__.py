@@ -1,7 +1,7 @@
 """mumaxplus on MI355X: micromagnetic time stepping with hand-written HIP kernels.
 
 The public classes mirror the reference package (mumaxplus/__init__.py of
-dirkpetersen/mumax-plus): World, Grid, Ferromagnet, TimeSolver and the
+dirkpetersen/mumax-plus): World, Grid, Ferromagnet, Antiferromagnet, TimeSolver and the
 quantity / parameter / variable wrappers.  The compute path is the HIP
 library libmumaxplus_amd.so, reached through ``_mumaxpluscpp``.
 """
@@ -9,14 +9,17 @@ library libmumaxplus_amd.so, reached through ``_mumaxpluscpp``.
 from . import _mumaxpluscpp as _cpp
 from .grid import Grid
 from .world import World
+from .magnet import Magnet
 from .ferromagnet import Ferromagnet
+from .antiferromagnet import Antiferromagnet
 from .timesolver import TimeSolver, TimeSolverOutput
 from .fieldquantity import FieldQuantity
 from .scalarquantity import ScalarQuantity
 from .parameter import Parameter
 from .variable import Variable
-from .dmitensor import DmiTensor
+from .dmitensor import DmiTensor, DmiTensorGroup
 from . import util
 
-__all__ = ["_cpp", "Grid", "World", "Ferromagnet", "TimeSolver", "TimeSolverOutput", "FieldQuantity",
-           "ScalarQuantity", "Parameter", "Variable", "DmiTensor", "util"]
+__all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
+           "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
+           "DmiTensorGroup", "util"]

@@ -2,7 +2,7 @@
 
 This module plays the role of the reference's pybind11 extension
 ``_mumaxpluscpp`` (src/bindings/main.cpp:3-27): it exposes the classes
-``Grid``, ``World``, ``Ferromagnet``, ``TimeSolver``, ``Parameter``,
+``Grid``, ``World``, ``Ferromagnet``, ``Antiferromagnet``, ``TimeSolver``, ``Parameter``,
 ``VectorParameter``, ``Variable``, ``FieldQuantity``, ``ScalarQuantity``,
 ``DmiTensor`` and ``System`` with the member names of src/bindings/wrap_*.cpp,
 and the module function ``_demag_kernel`` (wrap_magnet.cpp:56-59).
@@ -70,6 +70,32 @@ _SIGS = {
     "mpa_fm_average": ([_P, _c_char_p, _P], _cint),
     "mpa_fm_scalar": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
     "mpa_fm_demag_kernel": ([_P, _P, _P], _cint),
+    "mpa_fm_host": ([_P, _ct.POINTER(_P)], _cint),
+    "mpa_fm_set_variable": ([_P, _c_char_p, _P, _ct.c_longlong], _cint),
+    "mpa_fm_get_variable": ([_P, _c_char_p, _P], _cint),
+    "mpa_world_add_antiferromagnet": ([_P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_world_num_antiferromagnets": ([_P, _ct.POINTER(_cint)], _cint),
+    "mpa_world_antiferromagnet_at": ([_P, _cint, _ct.POINTER(_P)], _cint),
+    "mpa_world_find_antiferromagnet": ([_P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_afm_name": ([_P, _ct.c_char_p, _cint], _cint),
+    "mpa_afm_grid": ([_P, _P, _P], _cint),
+    "mpa_afm_geometry": ([_P, _P], _cint),
+    "mpa_afm_cellsingeo": ([_P, _ct.POINTER(_cint)], _cint),
+    "mpa_afm_sublattice": ([_P, _cint, _ct.POINTER(_P)], _cint),
+    "mpa_afm_set_flag": ([_P, _c_char_p, _cint], _cint),
+    "mpa_afm_get_flag": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_afm_param_info": ([_P, _c_char_p, _ct.POINTER(_cint), _ct.POINTER(_cint), _c_char_p, _cint,
+                            _c_char_p, _cint], _cint),
+    "mpa_afm_param_set_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_param_set_cells": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_param_get": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_param_get_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_set_variable": ([_P, _c_char_p, _P, _ct.c_longlong], _cint),
+    "mpa_afm_get_variable": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_afm_eval": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_average": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_scalar": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
     "mpa_ts_get": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
     "mpa_ts_set": ([_P, _c_char_p, _cdouble], _cint),
     "mpa_ts_sensible_timestep": ([_P, _ct.POINTER(_cdouble)], _cint),
@@ -134,6 +160,11 @@ def _ptr(a):
     return a.ctypes.data_as(_P)
 
 
+def _call(owner, op, *args):
+    """mpa_<fm|afm>_<op>(handle, *args) of a magnet handle, with error check."""
+    _check(getattr(_lib, f"mpa_{owner._pfx}_{op}")(owner._h, *args))
+
+
 # ---------------------------------------------------------------- Grid
 class Grid:
     """Grid(size, origin=(0,0,0)) (wrap_grid.cpp:4-17)."""
@@ -160,8 +191,8 @@ class Grid:
 class System:
     """Geometry information of a magnet (wrap_system.cpp:40-51)."""
 
-    def __init__(self, fm):
-        self._fm = fm
+    def __init__(self, magnet):
+        self._fm = magnet
 
     @property
     def grid(self):
@@ -175,7 +206,7 @@ class System:
     def geometry(self):
         nx, ny, nz = self.grid.size
         out = _np.empty(nx * ny * nz, dtype=_np.uint8)
-        _check(_lib.mpa_fm_geometry(self._fm._h, _ptr(out)))
+        _call(self._fm, "geometry", _ptr(out))
         return out.reshape(nz, ny, nx).astype(bool)
 
     @property
@@ -202,23 +233,28 @@ class System:
     @property
     def cellsingeo(self):
         v = _cint()
-        _check(_lib.mpa_fm_cellsingeo(self._fm._h, _ct.byref(v)))
+        _call(self._fm, "cellsingeo", _ct.byref(v))
         return v.value
 
 
 # ------------------------------------------------------- FieldQuantity
 class FieldQuantity:
-    """A field quantity of a ferromagnet (wrap_fieldquantity.cpp:8-22)."""
+    """A field quantity of a magnet (wrap_fieldquantity.cpp:8-22)."""
 
-    _UNITS = {"magnetization": "", "exchange_field": "T", "anisotropy_field": "T", "external_field": "T",
-              "dmi_field": "T", "demag_field": "T", "effective_field": "T", "torque": "/s",
-              "llg_torque": "/s", "damping_torque": "/s"}
+    _UNITS = {"magnetization": "", "full_magnetization": "A/m", "neel_vector": "", "exchange_field": "T",
+              "anisotropy_field": "T", "external_field": "T", "dmi_field": "T", "demag_field": "T",
+              "magnetoelastic_field": "T", "homogeneous_exchange_field": "T",
+              "inhomogeneous_exchange_field": "T", "effective_field": "T", "torque": "/s", "llg_torque": "/s",
+              "damping_torque": "/s", "magnetoelastic_force": "N/m3", "strain_tensor": "",
+              "stress_tensor": "N/m2", "elastic_force": "N/m3", "effective_body_force": "N/m3",
+              "elastic_damping": "N/m3", "elastic_acceleration": "m/s2", "elastic_velocity": "m/s",
+              "elastic_displacement": "m", "poynting_vector": "W/m2"}
 
     def __init__(self, fm, qname):
         self._fm = fm
         self._q = qname
         nc = _cint()
-        _check(_lib.mpa_fm_quantity_ncomp(fm._h, _enc(qname), _ct.byref(nc)))
+        _call(fm, "quantity_ncomp", _enc(qname), _ct.byref(nc))
         self.ncomp = nc.value
 
     @property
@@ -240,7 +276,7 @@ class FieldQuantity:
     def eval(self):
         nx, ny, nz = self.grid.size
         out = _np.empty((self.ncomp, nz, ny, nx), dtype=_np.float32)
-        _check(_lib.mpa_fm_eval(self._fm._h, _enc(self._q), _ptr(out)))
+        _call(self._fm, "eval", _enc(self._q), _ptr(out))
         return out
 
     def exec(self):
@@ -248,12 +284,12 @@ class FieldQuantity:
 
     def average(self):
         out = _np.zeros(self.ncomp, dtype=_np.float64)
-        _check(_lib.mpa_fm_average(self._fm._h, _enc(self._q), _ptr(out)))
+        _call(self._fm, "average", _enc(self._q), _ptr(out))
         return [float(v) for v in out]
 
 
 class ScalarQuantity:
-    """Scalar quantity of a ferromagnet (wrap_scalarquantity.cpp)."""
+    """Scalar quantity of a magnet (wrap_scalarquantity.cpp)."""
 
     _UNITS = {"max_torque": "/s"}
 
@@ -271,22 +307,31 @@ class ScalarQuantity:
 
     def eval(self):
         v = _cdouble()
-        _check(_lib.mpa_fm_scalar(self._fm._h, _enc(self._q), _ct.byref(v)))
+        _call(self._fm, "scalar", _enc(self._q), _ct.byref(v))
         return v.value
 
 
 # ----------------------------------------------------------- Variable
 class Variable(FieldQuantity):
-    """The magnetisation of a ferromagnet (wrap_variable.cpp:8-16)."""
+    """The magnetisation of a ferromagnet, or the elastic displacement or
+    velocity of a magnet (wrap_variable.cpp:8-16)."""
 
-    def __init__(self, fm):
-        super().__init__(fm, "magnetization")
+    def __init__(self, fm, which="magnetization"):
+        self._fm = fm
+        self._q = which
+        self.ncomp = 3
 
     def get(self):
         nx, ny, nz = self.grid.size
         out = _np.empty((3, nz, ny, nx), dtype=_np.float32)
-        _check(_lib.mpa_fm_get_magnetization(self._fm._h, _ptr(out)))
+        if self._q == "magnetization":
+            _call(self._fm, "get_magnetization", _ptr(out))
+        else:
+            _call(self._fm, "get_variable", _enc(self._q), _ptr(out))
         return out
+
+    def eval(self):
+        return self.get()
 
     def set(self, value):
         a = _np.ascontiguousarray(_np.asarray(value, dtype=_np.float32))
@@ -297,7 +342,10 @@ class Variable(FieldQuantity):
         elif a.shape != (3, nz, ny, nx):
             raise ValueError(f"The shape of the data does not match the shape of the field, expected "
                              f"{(3, nz, ny, nx)}, got {a.shape}.")
-        _check(_lib.mpa_fm_set_magnetization(self._fm._h, _ptr(a), a.size))
+        if self._q == "magnetization":
+            _call(self._fm, "set_magnetization", _ptr(a), a.size)
+        else:
+            _call(self._fm, "set_variable", _enc(self._q), _ptr(a), a.size)
 
 
 # ---------------------------------------------------------- Parameter
@@ -311,7 +359,7 @@ class Parameter(FieldQuantity):
         nc, uni = _cint(), _cint()
         nm = _ct.create_string_buffer(256)
         un = _ct.create_string_buffer(64)
-        _check(_lib.mpa_fm_param_info(fm._h, self._key, _ct.byref(nc), _ct.byref(uni), nm, 256, un, 64))
+        _call(fm, "param_info", self._key, _ct.byref(nc), _ct.byref(uni), nm, 256, un, 64)
         self.ncomp = nc.value
         self._name = nm.value.decode()
         self._unit = un.value.decode()
@@ -327,7 +375,7 @@ class Parameter(FieldQuantity):
     @property
     def is_uniform(self):
         nc, uni = _cint(), _cint()
-        _check(_lib.mpa_fm_param_info(self._fm._h, self._key, _ct.byref(nc), _ct.byref(uni), None, 0, None, 0))
+        _call(self._fm, "param_info", self._key, _ct.byref(nc), _ct.byref(uni), None, 0, None, 0)
         return bool(uni.value)
 
     @property
@@ -337,7 +385,7 @@ class Parameter(FieldQuantity):
     @property
     def uniform_value(self):
         out = _np.zeros(self.ncomp, dtype=_np.float64)
-        _check(_lib.mpa_fm_param_get_uniform(self._fm._h, self._key, _ptr(out)))
+        _call(self._fm, "param_get_uniform", self._key, _ptr(out))
         return float(out[0]) if self.ncomp == 1 else tuple(float(v) for v in out)
 
     @uniform_value.setter
@@ -347,7 +395,7 @@ class Parameter(FieldQuantity):
     def eval(self):
         nx, ny, nz = self.grid.size
         out = _np.empty((self.ncomp, nz, ny, nx), dtype=_np.float32)
-        _check(_lib.mpa_fm_param_get(self._fm._h, self._key, _ptr(out)))
+        _call(self._fm, "param_get", self._key, _ptr(out))
         return out
 
     def average(self):
@@ -362,7 +410,7 @@ class Parameter(FieldQuantity):
             vals = _np.atleast_1d(_np.asarray(value, dtype=_np.float64))
             if vals.size != self.ncomp:
                 raise ValueError(f"expected {self.ncomp} components")
-            _check(_lib.mpa_fm_param_set_uniform(self._fm._h, self._key, _ptr(_np.ascontiguousarray(vals))))
+            _call(self._fm, "param_set_uniform", self._key, _ptr(_np.ascontiguousarray(vals)))
             return
         a = _np.asarray(value, dtype=_np.float32)
         nx, ny, nz = self.grid.size
@@ -372,7 +420,7 @@ class Parameter(FieldQuantity):
             raise ValueError(f"The shape of the data does not match the shape of the parameter, expected "
                              f"{(self.ncomp, nz, ny, nx)}, got {a.shape}.")
         a = _np.ascontiguousarray(a)
-        _check(_lib.mpa_fm_param_set_cells(self._fm._h, self._key, _ptr(a)))
+        _call(self._fm, "param_set_cells", self._key, _ptr(a))
 
     def add_time_term(self, term, mask=None):
         raise NotImplementedError("time-dependent parameter terms are not supported by this build")
@@ -395,27 +443,40 @@ class DmiTensor:
             setattr(self, k, Parameter(fm, "dmi_" + k))
 
 
-# --------------------------------------------------------- Ferromagnet
-class Ferromagnet:
-    """Handle of a ferromagnet owned by a World (wrap_ferromagnet.cpp:28-73)."""
+# -------------------------------------------------------------- Magnets
+class _Magnet:
+    """Members every magnet has (wrap_magnet.cpp:17-54)."""
 
-    _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins")
-    _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field")
+    _pfx = None
+    _PARAMS = ()
+    _VPARAMS = ()
+    _ELASTIC_PARAMS = ("C11", "C12", "C44", "eta", "rho")
+    _ELASTIC_VPARAMS = ("external_body_force", "rigid_norm_strain", "rigid_shear_strain")
 
     def __init__(self, handle, world):
         object.__setattr__(self, "_h", handle)
         object.__setattr__(self, "_world", world)
         params = {k: Parameter(self, k) for k in self._PARAMS}
         params.update({k: VectorParameter(self, k) for k in self._VPARAMS})
+        if self._has_elastic_params():
+            params.update({k: Parameter(self, k) for k in self._ELASTIC_PARAMS})
+            params.update({k: VectorParameter(self, k) for k in self._ELASTIC_VPARAMS})
         object.__setattr__(self, "_params", params)
-        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
-        object.__setattr__(self, "magnetization", Variable(self))
+
+    def _has_elastic_params(self):
+        return True
 
     def __getattr__(self, key):
         params = object.__getattribute__(self, "_params")
         if key in params:
             return params[key]
         raise AttributeError(key)
+
+    def __eq__(self, other):
+        return isinstance(other, _Magnet) and self._pfx == other._pfx and self._h.value == other._h.value
+
+    def __hash__(self):
+        return hash((self._pfx, self._h.value))
 
     @property
     def world(self):
@@ -424,13 +485,13 @@ class Ferromagnet:
     @property
     def name(self):
         buf = _ct.create_string_buffer(256)
-        _check(_lib.mpa_fm_name(self._h, buf, 256))
+        _call(self, "name", buf, 256)
         return buf.value.decode()
 
     @property
     def grid(self):
         s, o = _i3(), _i3()
-        _check(_lib.mpa_fm_grid(self._h, s, o))
+        _call(self, "grid", s, o)
         return Grid(tuple(s), tuple(o))
 
     @property
@@ -439,24 +500,11 @@ class Ferromagnet:
 
     def _flag(self, key):
         v = _cint()
-        _check(_lib.mpa_fm_get_flag(self._h, _enc(key), _ct.byref(v)))
+        _call(self, "get_flag", _enc(key), _ct.byref(v))
         return bool(v.value)
 
-    @property
-    def enable_demag(self):
-        return self._flag("enable_demag")
-
-    @enable_demag.setter
-    def enable_demag(self, value):
-        _check(_lib.mpa_fm_set_flag(self._h, b"enable_demag", int(bool(value))))
-
-    @property
-    def enable_openbc(self):
-        return self._flag("enable_openbc")
-
-    @enable_openbc.setter
-    def enable_openbc(self, value):
-        _check(_lib.mpa_fm_set_flag(self._h, b"enable_openbc", int(bool(value))))
+    def _set_flag(self, key, value):
+        _call(self, "set_flag", _enc(key), int(bool(value)))
 
     @property
     def enable_as_stray_field_source(self):
@@ -464,7 +512,7 @@ class Ferromagnet:
 
     @enable_as_stray_field_source.setter
     def enable_as_stray_field_source(self, value):
-        _check(_lib.mpa_fm_set_flag(self._h, b"enable_as_stray_field_source", int(bool(value))))
+        self._set_flag("enable_as_stray_field_source", value)
 
     @property
     def enable_as_stray_field_destination(self):
@@ -472,17 +520,107 @@ class Ferromagnet:
 
     @enable_as_stray_field_destination.setter
     def enable_as_stray_field_destination(self, value):
-        _check(_lib.mpa_fm_set_flag(self._h, b"enable_as_stray_field_destination", int(bool(value))))
+        self._set_flag("enable_as_stray_field_destination", value)
 
     @property
-    def is_sublattice(self):
-        return False
+    def enable_elastodynamics(self):
+        return self._flag("enable_elastodynamics")
+
+    @enable_elastodynamics.setter
+    def enable_elastodynamics(self, value):
+        self._set_flag("enable_elastodynamics", value)
+
+    @property
+    def elastic_displacement(self):
+        if not self.enable_elastodynamics:
+            raise ValueError("elasticDisplacement Variable does not exist yet. Enable elastodynamics first.")
+        return Variable(self, "elastic_displacement")
+
+    @property
+    def elastic_velocity(self):
+        if not self.enable_elastodynamics:
+            raise ValueError("elasticVelocity Variable does not exist yet. Enable elastodynamics first.")
+        return Variable(self, "elastic_velocity")
 
     def field_quantity(self, name):
         return FieldQuantity(self, name)
 
     def scalar_quantity(self, name):
         return ScalarQuantity(self, name)
+
+
+class Ferromagnet(_Magnet):
+    """Handle of a ferromagnet or sublattice owned by a World (wrap_ferromagnet.cpp:28-73)."""
+
+    _pfx = "fm"
+    _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "B1", "B2")
+    _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field")
+
+    def __init__(self, handle, world):
+        super().__init__(handle, world)
+        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
+        object.__setattr__(self, "magnetization", Variable(self))
+
+    def _has_elastic_params(self):
+        return self._host_handle() is None
+
+    def _host_handle(self):
+        h = _P()
+        _check(_lib.mpa_fm_host(self._h, _ct.byref(h)))
+        return h if h.value else None
+
+    @property
+    def enable_demag(self):
+        return self._flag("enable_demag")
+
+    @enable_demag.setter
+    def enable_demag(self, value):
+        self._set_flag("enable_demag", value)
+
+    @property
+    def enable_openbc(self):
+        return self._flag("enable_openbc")
+
+    @enable_openbc.setter
+    def enable_openbc(self, value):
+        self._set_flag("enable_openbc", value)
+
+    @property
+    def is_sublattice(self):
+        return self._host_handle() is not None
+
+
+class Antiferromagnet(_Magnet):
+    """Handle of an antiferromagnet owned by a World (wrap_antiferromagnet.cpp:17-32)."""
+
+    _pfx = "afm"
+    _PARAMS = ("afmex_cell", "afmex_nn", "latcon")
+
+    def __init__(self, handle, world):
+        super().__init__(handle, world)
+        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
+        subs = []
+        for i in range(2):
+            h = _P()
+            _check(_lib.mpa_afm_sublattice(self._h, i, _ct.byref(h)))
+            subs.append(Ferromagnet(h, world))
+        object.__setattr__(self, "_subs", tuple(subs))
+
+    def sub1(self):
+        return self._subs[0]
+
+    def sub2(self):
+        return self._subs[1]
+
+    def sublattices(self):
+        return list(self._subs)
+
+    def other_sublattice(self, sub):
+        if sub == self._subs[0]:
+            return self._subs[1]
+        if sub == self._subs[1]:
+            return self._subs[0]
+        raise IndexError("Sublattice not found in Antiferromagnet.")
 
 
 def _demag_kernel(fm):
@@ -571,6 +709,7 @@ class World:
                                      _as_i3(pbc_repetitions, "pbc_repetitions"), _ct.byref(h)))
         self._h = h
         self._magnets = {}
+        self._afms = {}
         self.timesolver = TimeSolver(self)
 
     def __del__(self):
@@ -595,16 +734,20 @@ class World:
     def bias_magnetic_field(self, b):
         _check(_lib.mpa_world_set_bias_magnetic_field(self._h, _as_d3(b, "bias_magnetic_field")))
 
-    def add_ferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
+    @staticmethod
+    def _geometry_bytes(grid, geometry_array, regions_array):
         if regions_array is not None and _np.any(_np.asarray(regions_array) != 0):
             raise NotImplementedError("regions are not supported by this build")
-        geo = None
-        if geometry_array is not None:
-            nx, ny, nz = grid.size
-            g = _np.asarray(geometry_array, dtype=bool)
-            if g.shape != (nz, ny, nx):
-                raise ValueError("The dimensions of the geometry do not match the dimensions of the grid.")
-            geo = _np.ascontiguousarray(g.astype(_np.uint8))
+        if geometry_array is None:
+            return None
+        nx, ny, nz = grid.size
+        g = _np.asarray(geometry_array, dtype=bool)
+        if g.shape != (nz, ny, nx):
+            raise ValueError("The dimensions of the geometry do not match the dimensions of the grid.")
+        return _np.ascontiguousarray(g.astype(_np.uint8))
+
+    def add_ferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
+        geo = self._geometry_bytes(grid, geometry_array, regions_array)
         h = _P()
         _check(_lib.mpa_world_add_ferromagnet(self._h, _i3(*grid.size), _i3(*grid.origin),
                                               None if geo is None else _ptr(geo), _enc(name or ""),
@@ -612,6 +755,16 @@ class World:
         fm = Ferromagnet(h, self)
         self._magnets[fm.name] = fm
         return fm
+
+    def add_antiferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
+        geo = self._geometry_bytes(grid, geometry_array, regions_array)
+        h = _P()
+        _check(_lib.mpa_world_add_antiferromagnet(self._h, _i3(*grid.size), _i3(*grid.origin),
+                                                  None if geo is None else _ptr(geo), _enc(name or ""),
+                                                  _ct.byref(h)))
+        afm = Antiferromagnet(h, self)
+        self._afms[afm.name] = afm
+        return afm
 
     def get_ferromagnet(self, name):
         return self._magnets.get(name)
@@ -622,10 +775,10 @@ class World:
 
     @property
     def antiferromagnets(self):
-        return {}
+        return dict(self._afms)
 
     def get_antiferromagnet(self, name):
-        return None
+        return self._afms.get(name)
 
     def _pbc(self):
         s, o, r = _i3(), _i3(), _i3()

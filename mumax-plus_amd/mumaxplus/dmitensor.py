@@ -38,3 +38,26 @@ class DmiTensor:
         self.xyz.set(value)
         self.yxz.set(-value)
         self.zxy.set(value)
+
+
+class DmiTensorGroup:
+    """Several DMI tensors set together (dmitensor.py:5-29 of the reference):
+    element assignment and the set_* methods apply to every tensor."""
+
+    def __init__(self, tensors):
+        object.__setattr__(self, "tensors", list(tensors))
+
+    def __getattr__(self, name):
+        if name in ("make_zero", "set_interfacial_dmi", "set_bulk_dmi"):
+            def method(*args, **kwargs):
+                for tensor in self.tensors:
+                    getattr(tensor, name)(*args, **kwargs)
+            return method
+        raise AttributeError(f"'DmiTensor' object has no attribute '{name}'")
+
+    def __setattr__(self, name, value):
+        if name in DmiTensor.KEYS:
+            for tensor in self.tensors:
+                setattr(tensor, name, value)
+        else:
+            raise AttributeError(f"'DmiTensor' object has no attribute '{name}'")

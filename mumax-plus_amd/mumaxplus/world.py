@@ -46,9 +46,17 @@ class World:
         from .ferromagnet import Ferromagnet
         return {k: Ferromagnet._from_impl(v) for k, v in self._impl.ferromagnets.items()}
 
+    def get_antiferromagnet(self, name):
+        from .antiferromagnet import Antiferromagnet
+        impl = self._impl.get_antiferromagnet(name)
+        if impl is None:
+            raise KeyError(f"No magnet named {name}")
+        return Antiferromagnet._from_impl(impl)
+
     @property
     def antiferromagnets(self):
-        return {}
+        from .antiferromagnet import Antiferromagnet
+        return {k: Antiferromagnet._from_impl(v) for k, v in self._impl.antiferromagnets.items()}
 
     @property
     def cellsize(self):

@@ -317,7 +317,9 @@ class World:
         for fm, m in zip(fms, ms):
             if not self.magnetoelastic_assured_zero(fm):
                 f = f + _elastic.magnetoelastic_force(fm, m)
-        return f
+        # Field += skips cells outside the geometry (k_addFields,
+        # src/core/fieldops.cu:1-14), so they keep the initial zero
+        return np.where(own.geom_mask()[None], f, 0.0)
 
     def acceleration(self, own, u, v, ms):
         """evalElasticAcceleration (elastodynamics.cu:54-84)."""
