@@ -201,6 +201,30 @@ int mpa_ts_step(mpa_world* w);
 int mpa_ts_steps(mpa_world* w, long long n);
 int mpa_ts_run(mpa_world* w, double duration);
 
+/* ------------------------------------------------------ Relax / minimize -- */
+
+/* World.relax(tol) / Ferromagnet.relax / Antiferromagnet.relax (wrap_world.cpp:95-96,
+ * wrap_ferromagnet.cpp:72-73, wrap_antiferromagnet.cpp:31-32; Relaxer::exec relaxer.cpp:87-174):
+ * damping-only Fehlberg dynamics while the total energy decreases, then with the error tolerance
+ * lowered to tol while the torque decreases (or until the max torque is below the
+ * RelaxTorqueThreshold where one is set).  The solver's method, tolerance, step mode, time and
+ * step are restored afterwards. */
+int mpa_world_relax(mpa_world* w, double tol);
+int mpa_fm_relax(mpa_ferromagnet* m, double tol);
+int mpa_afm_relax(mpa_antiferromagnet* a, double tol);
+/* World.minimize / Ferromagnet.minimize / Antiferromagnet.minimize(tol, nsamples) (Minimizer,
+ * minimizer.cu:8-160): steepest descent with Barzilai-Borwein steps until nsamples consecutive
+ * maximum magnetisation changes are <= tol. */
+int mpa_world_minimize(mpa_world* w, double tol, int nsamples);
+int mpa_fm_minimize(mpa_ferromagnet* m, double tol, int nsamples);
+int mpa_afm_minimize(mpa_antiferromagnet* a, double tol, int nsamples);
+/* World.RelaxTorqueThreshold / Ferromagnet.RelaxTorqueThreshold (default -1;
+ * wrap_world.cpp, wrap_ferromagnet.cpp:67). */
+int mpa_world_set_relax_torque_threshold(mpa_world* w, double value);
+int mpa_world_get_relax_torque_threshold(const mpa_world* w, double* out);
+int mpa_fm_set_relax_torque_threshold(mpa_ferromagnet* m, double value);
+int mpa_fm_get_relax_torque_threshold(const mpa_ferromagnet* m, double* out);
+
 /* ------------------------------------------------------- Demag operator -- */
 
 /* StrayFieldFFTExecutor (strayfieldfft.cu:149-246) + StrayFieldKernel (strayfieldkernel.cu:11-99)

@@ -698,6 +698,75 @@ int mpa_ts_run(mpa_world* w, double duration) {
   });
 }
 
+// --------------------------------------------------------- relax / minimize
+int mpa_world_relax(mpa_world* w, double tol) {
+  return guard([&] {
+    need(w, "world");
+    w->w.relax(nullptr, tol);
+  });
+}
+
+int mpa_world_minimize(mpa_world* w, double tol, int nsamples) {
+  return guard([&] {
+    need(w, "world");
+    w->w.minimize(nullptr, tol, nsamples);
+  });
+}
+
+int mpa_world_set_relax_torque_threshold(mpa_world* w, double value) {
+  return guard([&] {
+    need(w, "world");
+    w->w.relax_threshold = value;
+  });
+}
+
+int mpa_world_get_relax_torque_threshold(const mpa_world* w, double* out) {
+  return guard([&] {
+    need(w, "world");
+    need(out, "out");
+    *out = w->w.relax_threshold;
+  });
+}
+
+int mpa_fm_relax(mpa_ferromagnet* m, double tol) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->world->relax(f, tol);
+  });
+}
+
+int mpa_fm_minimize(mpa_ferromagnet* m, double tol, int nsamples) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->world->minimize(f, tol, nsamples);
+  });
+}
+
+int mpa_fm_set_relax_torque_threshold(mpa_ferromagnet* m, double value) {
+  return guard([&] { FM(m)->relax_threshold = value; });
+}
+
+int mpa_fm_get_relax_torque_threshold(const mpa_ferromagnet* m, double* out) {
+  return guard([&] {
+    need(out, "out");
+    *out = FM(m)->relax_threshold;
+  });
+}
+
+int mpa_afm_relax(mpa_antiferromagnet* a, double tol) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    af->world->relax(af, tol);
+  });
+}
+
+int mpa_afm_minimize(mpa_antiferromagnet* a, double tol, int nsamples) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    af->world->minimize(af, tol, nsamples);
+  });
+}
+
 // -------------------------------------------------------- Demag operator
 int mpa_demag_plan_create(const int size[3], const double cellsize[3], const int mastergrid_size[3],
                           const int pbc_repetitions[3], mpa_demag_plan** out) {

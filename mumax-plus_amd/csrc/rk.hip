@@ -139,6 +139,79 @@ __global__ void __launch_bounds__(BLOCK) k_sums(const float* __restrict__ v, int
   }
 }
 
+// Per-block FP64 sums of |v|^2 (one component slot).
+__global__ void __launch_bounds__(BLOCK) k_sumsq(const float* __restrict__ v, int N, const uint8_t* __restrict__ geom,
+                                                 double* __restrict__ partials) {
+  __shared__ double sh[BLOCK];
+  double acc = 0.0;
+  for (int idx = blockIdx.x * BLOCK + threadIdx.x; idx < N; idx += gridDim.x * BLOCK) {
+    if (geom && !geom[idx]) continue;
+    for (int c = 0; c < 3; ++c) {
+      const double x = v[(size_t)c * N + idx];
+      acc += x * x;
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// The three Barzilai-Borwein products of dm = m1 - m0 and dt = t0 - t1
+// (minimizer.cu:76-90, 118-120) in FP64, and max |dm| (minimizer.cu:124).
+__global__ void __launch_bounds__(BLOCK) k_bb_sums(const float* __restrict__ m1, const float* __restrict__ m0,
+                                                   const float* __restrict__ t1, const float* __restrict__ t0, int N,
+                                                   const uint8_t* __restrict__ geom, double* __restrict__ partials,
+                                                   unsigned* __restrict__ maxbits) {
+  __shared__ double sh[BLOCK];
+  double acc[3] = {0.0, 0.0, 0.0};
+  float best = 0.f;
+  for (int idx = blockIdx.x * BLOCK + threadIdx.x; idx < N; idx += gridDim.x * BLOCK) {
+    if (geom && !geom[idx]) continue;
+    float dm2 = 0.f;
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = (size_t)c * N + idx;
+      const float dm = m1[i] - m0[i];
+      const float dt = t0[i] - t1[i];
+      acc[0] += (double)dm * dm;
+      acc[1] += (double)dm * dt;
+      acc[2] += (double)dt * dt;
+      dm2 += dm * dm;
+    }
+    best = fmaxf(best, sqrtf(dm2));
+  }
+  for (int c = 0; c < 3; ++c) {
+    sh[threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(size_t)c * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+  best = block_max(best);
+  if (threadIdx.x == 0) atomicMax(maxbits, __float_as_uint(best));
+}
+
+// One descent step of every cell (k_step, minimizer.cu:59-74):
+// m = ((4 - h^2 |t|^2) m0 + 4 h t) / (4 + h^2 |t|^2).
+__global__ void __launch_bounds__(BLOCK) k_descent(float* __restrict__ m, const float* __restrict__ m0,
+                                                   const float* __restrict__ t, float h, int N) {
+  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (idx >= N) return;
+  float a[3], b[3];
+  for (int c = 0; c < 3; ++c) {
+    a[c] = m0[(size_t)c * N + idx];
+    b[c] = t[(size_t)c * N + idx];
+  }
+  const float t2 = h * h * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  for (int c = 0; c < 3; ++c) m[(size_t)c * N + idx] = ((4.f - t2) * a[c] + 4.f * h * b[c]) / (4.f + t2);
+}
+
 __global__ void k_sums_final(double* partials, int nparts, int ncomp) {
   __shared__ double sh[BLOCK];
   for (int c = 0; c < ncomp; ++c) {
@@ -197,6 +270,42 @@ float Reducer::max_vec_norm(const float* v, int N, const uint8_t* geom, hipStrea
   co.c[0] = 1.f;
   co.k[0] = v;
   return max_combo_norm(co, N, geom, s);
+}
+
+double Reducer::dot_sum(const float* v, int N, const uint8_t* geom, hipStream_t s) {
+  int nb = (N + BLOCK - 1) / BLOCK;
+  if (nb > SUM_BLOCKS) nb = SUM_BLOCKS;
+  if (nb < 1) nb = 1;
+  k_sumsq<<<nb, BLOCK, 0, s>>>(v, N, geom, partials_.get());
+  checkLaunch("k_sumsq");
+  k_sums_final<<<1, BLOCK, 0, s>>>(partials_.get(), nb, 1);
+  checkLaunch("k_sums_final");
+  double out = 0.0;
+  HIP_CHECK(hipMemcpyAsync(&out, partials_.get() + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return out;
+}
+
+void Reducer::bb_sums(const float* m1, const float* m0, const float* t1, const float* t0, int N,
+                      const uint8_t* geom, double out[3], float* max_dm, hipStream_t s) {
+  int nb = (N + BLOCK - 1) / BLOCK;
+  if (nb > SUM_BLOCKS) nb = SUM_BLOCKS;
+  if (nb < 1) nb = 1;
+  k_zero_u32<<<1, 1, 0, s>>>(maxbits_.get());
+  k_bb_sums<<<nb, BLOCK, 0, s>>>(m1, m0, t1, t0, N, geom, partials_.get(), maxbits_.get());
+  checkLaunch("k_bb_sums");
+  k_sums_final<<<1, BLOCK, 0, s>>>(partials_.get(), nb, 3);
+  checkLaunch("k_sums_final");
+  unsigned bits = 0;
+  HIP_CHECK(hipMemcpyAsync(out, partials_.get() + (size_t)3 * nb, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(&bits, maxbits_.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  __builtin_memcpy(max_dm, &bits, sizeof(float));
+}
+
+void launch_descent(float* m, const float* m0, const float* t, float h, int N, hipStream_t s) {
+  k_descent<<<(N + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(m, m0, t, h, N);
+  checkLaunch("k_descent");
 }
 
 void Reducer::component_sums(const float* v, int ncomp, int N, const uint8_t* geom, double* out, hipStream_t s) {

@@ -18,6 +18,10 @@ struct Combo {
 void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int N, const uint8_t* geom,
                     bool normalize, hipStream_t s);
 
+// m = ((4 - h^2 |t|^2) m0 + 4 h t) / (4 + h^2 |t|^2) per cell, the descent step
+// of the energy minimiser (src/physics/minimizer.cu:59-74).
+void launch_descent(float* m, const float* m0, const float* t, float h, int N, hipStream_t s);
+
 // Reductions write into a small device workspace; results are read back by
 // the caller after a stream synchronisation.
 class Reducer {
@@ -28,6 +32,11 @@ class Reducer {
   float max_vec_norm(const float* v, int N, const uint8_t* geom, hipStream_t s);
   // sum over geometry cells of each component, accumulated in FP64.
   void component_sums(const float* v, int ncomp, int N, const uint8_t* geom, double* out, hipStream_t s);
+  // sum over geometry cells of |v|^2 (3 components), FP64 (dotSum, src/core/reduce.cu:153-195).
+  double dot_sum(const float* v, int N, const uint8_t* geom, hipStream_t s);
+  // dm = m1 - m0, dt = t0 - t1: out = {dm.dm, dm.dt, dt.dt} summed in FP64, *max_dm = max |dm|.
+  void bb_sums(const float* m1, const float* m0, const float* t1, const float* t0, int N, const uint8_t* geom,
+               double out[3], float* max_dm, hipStream_t s);
 
  private:
   DeviceArray<unsigned> maxbits_;

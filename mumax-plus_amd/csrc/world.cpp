@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 
 namespace mpa {
 
@@ -584,14 +585,18 @@ void Ferromagnet::check_stray_coupling() const {
   }
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray, bool prepared) {
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray, bool prepared, FieldMode mode) {
   if (check_stray) check_stray_coupling();
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
   if (!prepared) prepare(terms, s, prof);
-  // algorithmic bytes: read m, read h_demag (when present), write the torque
-  Timed t(prof, "k_field", 12.0 * N * ((terms & TERM_DEMAG) ? 3 : 2), s);
-  launch_field(desc(terms), m.get(), demag_field(), out, MODE_LLG, s);
+  // algorithmic bytes: read m, h_demag (when present), the sister's m (for
+  // a sublattice) and the strain (dynamic magnetoelastic term); write the torque
+  double bytes = 12.0 * N * ((terms & TERM_DEMAG) ? 3 : 2);
+  if (host) bytes += 12.0 * N;
+  if ((terms & TERM_ME) && !owner()->rigid_strain_applied()) bytes += 24.0 * N;
+  Timed t(prof, "k_field", bytes, s);
+  launch_field(desc(terms), m.get(), demag_field(), out, mode, s);
 }
 
 namespace {
@@ -856,11 +861,16 @@ std::vector<StateVar> World::state_vars() {
   auto add_mag = [&](Ferromagnet* fm) {
     v.push_back(StateVar{StateVar::MAG, fm, nullptr, fm->m.get(), &fm->x0, &fm->k, fm->N, fm->geom_ptr(), true});
   };
-  for (auto& fm : magnets) add_mag(fm.get());
-  for (auto& a : afms) {
-    add_mag(a->sub[0].get());
-    add_mag(a->sub[1].get());
+  if (relaxing_) {
+    for (Ferromagnet* fm : relax_fms_) add_mag(fm);
+  } else {
+    for (auto& fm : magnets) add_mag(fm.get());
+    for (auto& a : afms) {
+      add_mag(a->sub[0].get());
+      add_mag(a->sub[1].get());
+    }
   }
+  if (relaxing_ && !relax_elastic_) return v;
   auto add_elastic = [&](MagnetCore* own) {
     if (!own->el.enabled) return;
     v.push_back(StateVar{StateVar::DISP, nullptr, own, own->el.u.get(), &own->el.u0, &own->el.ku, own->N,
@@ -887,13 +897,14 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
     float* out = (*sv.k)[stage].get();
     if (sv.kind == StateVar::MAG) {
       Ferromagnet* fm = sv.fm;
+      const FieldMode mode = relaxing_ ? MODE_DAMPING : MODE_LLG;
       if (fm->host) {
-        fm->eval_rhs(out, s, check_stray, true);
+        fm->eval_rhs(out, s, check_stray, true, mode);
       } else {
         const unsigned t = fm->active_terms();
         if (t & TERM_DEMAG)
           fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
-        fm->eval_rhs(out, s, check_stray, true);
+        fm->eval_rhs(out, s, check_stray, true, mode);
       }
     } else if (sv.kind == StateVar::DISP) {
       // du/dt = v (elastodynamics.cu:93-96)
@@ -902,11 +913,218 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
                                hipMemcpyDeviceToDevice, s));
     } else {
       MagnetCore* own = sv.owner;
-      Timed t(&prof, "k_elastic", 12.0 * sv.N * 4, s);
+      // read stress (24 B), v (12 B) and each coupled magnetisation (12 B); write a (12 B)
+      Timed t(&prof, "k_elastic", (48.0 + 12.0 * own->num_fm()) * sv.N, s);
       launch_elastic(own->eldesc(), own->el.stress.get(), own->el.v.get(), own->me_force_of(0),
                      own->num_fm() > 1 ? own->me_force_of(1) : MEForce{}, out, EL_ACCEL, s);
     }
   }
+}
+
+// ------------------------------------------------------ relax / minimize
+
+double World::total_energy(MagnetCore* m) {
+  if (auto* fm = dynamic_cast<Ferromagnet*>(m)) return fm->scalar_quantity("total_energy", stream);
+  return static_cast<Antiferromagnet*>(m)->scalar_quantity("total_energy", stream);
+}
+
+// Relaxer::exec (relaxer.cpp:87-174): damping-only dynamics with Fehlberg
+// and an adaptive step, first while the total energy decreases (checked
+// every 3 steps), then with the error tolerance lowered by sqrt(2) at a time
+// down to `tol`, each level run while the summed squared torque decreases (or,
+// with positive thresholds, until every max torque is below its threshold).
+// The solver's method, error, step mode, time and step are restored; with an
+// adaptive step the step is the sensible one of the restored equations.
+void World::relax(MagnetCore* target, double tol) {
+  std::vector<MagnetCore*> targets;
+  if (target) {
+    targets.push_back(target);
+  } else {
+    for (auto& m : magnets) targets.push_back(m.get());
+    for (auto& a : afms) targets.push_back(a.get());
+  }
+  std::vector<Ferromagnet*> fms;
+  std::vector<double> thr;
+  for (MagnetCore* t : targets) {
+    if (auto* fm = dynamic_cast<Ferromagnet*>(t)) {
+      fms.push_back(fm);
+      thr.push_back(target ? fm->relax_threshold : relax_threshold);
+    } else {
+      auto* a = static_cast<Antiferromagnet*>(t);
+      double t1 = a->sub[0]->relax_threshold, t2 = a->sub[1]->relax_threshold;
+      if (!target) t1 = t2 = relax_threshold;
+      // antiferromagnet.cpp:57-68: one user-set threshold serves both sublattices
+      if (t1 > 0 && t2 <= 0) t2 = t1;
+      else if (t1 <= 0 && t2 > 0) t1 = t2;
+      fms.push_back(a->sub[0].get());
+      fms.push_back(a->sub[1].get());
+      thr.push_back(t1);
+      thr.push_back(t2);
+    }
+  }
+  TimeSolver& ts = solver;
+  const double time0 = ts.time, dt0 = ts.dt, maxerr0 = ts.max_error;
+  const bool adaptive0 = ts.adaptive;
+  const std::string method0 = ts.method;
+  auto restore = [&]() {
+    relaxing_ = false;
+    relax_elastic_ = false;
+    relax_fms_.clear();
+    ts.method = method0;
+    ts.max_error = maxerr0;
+    ts.adaptive = adaptive0;
+    ts.time = time0;
+    ts.dt = dt0;
+    ts.equations_changed();  // FSAL state dropped; sensible step when adaptive
+  };
+  try {
+    ts.adaptive = true;
+    ts.method = "Fehlberg";
+    relaxing_ = true;
+    relax_fms_ = fms;
+    relax_elastic_ = target == nullptr;  // a world relax keeps the elastic equations
+    ts.equations_changed();
+    constexpr int NS = 3;  // energy every 3 steps
+    auto energy = [&]() {
+      double e = 0.0;
+      for (MagnetCore* t : targets) e += total_energy(t);
+      return e;
+    };
+    auto torque_of = [&](Ferromagnet* fm) {
+      int nc = 0;
+      return fm->eval_device("damping_torque", stream, &nc);
+    };
+    auto torque_sum = [&]() {
+      double t = 0.0;
+      for (Ferromagnet* fm : fms) t += reducer.dot_sum(torque_of(fm), fm->N, fm->geom_ptr(), stream);
+      return t;
+    };
+    double E0 = energy();
+    ts.steps(NS);
+    double E1 = energy();
+    while (E1 < E0) {
+      ts.steps(NS);
+      E0 = E1;
+      E1 = energy();
+    }
+    const bool all_negative = std::all_of(thr.begin(), thr.end(), [](double t) { return t < 0; });
+    if (all_negative) {
+      double t1 = torque_sum(), t0 = 0.0;
+      double err = ts.max_error;
+      while (err > tol) {
+        err /= std::sqrt(2.0);
+        ts.max_error = err;
+        ts.steps(NS);
+        t0 = t1;
+        t1 = torque_sum();
+        while (t1 < t0) {
+          ts.steps(NS);
+          t0 = t1;
+          t1 = torque_sum();
+        }
+      }
+    } else if (std::find(thr.begin(), thr.end(), 0.0) != thr.end()) {
+      throw Error(MPA_ERR_INVALID, "The relax threshold should not be zero.");
+    } else {
+      double err = ts.max_error;
+      while (err > tol) {
+        bool converged = true;
+        for (size_t i = 0; i < fms.size() && converged; ++i)
+          if (reducer.max_vec_norm(torque_of(fms[i]), fms[i]->N, fms[i]->geom_ptr(), stream) > thr[i])
+            converged = false;
+        if (converged) {
+          err /= std::sqrt(2.0);
+          ts.max_error = err;
+        }
+        ts.steps(NS);
+      }
+    }
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+}
+
+// Minimizer (minimizer.cu:68-160): steepest descent along the damping-only
+// torque with Barzilai-Borwein step sizes (starting at 1e-14), all magnets
+// stepped, then normalised, then their torques re-evaluated; converged when
+// the last nsamples (times the number of ferromagnets for a world) maximum
+// magnetisation changes are all <= tol.
+void World::minimize(MagnetCore* target, double tol, int nsamples) {
+  if (nsamples < 1) throw Error(MPA_ERR_INVALID, "nsamples should be positive");
+  std::vector<Ferromagnet*> fms;
+  auto add = [&](MagnetCore* t) {
+    if (auto* fm = dynamic_cast<Ferromagnet*>(t)) {
+      fms.push_back(fm);
+    } else {
+      auto* a = static_cast<Antiferromagnet*>(t);
+      fms.push_back(a->sub[0].get());
+      fms.push_back(a->sub[1].get());
+    }
+  };
+  size_t nsamp = (size_t)nsamples;
+  if (target) {
+    add(target);
+  } else {
+    for (auto& m : magnets) add(m.get());
+    for (auto& a : afms) add(a.get());
+    nsamp *= fms.size();
+  }
+  const size_t F = fms.size();
+  std::vector<DeviceArray<float>> m0(F), m1(F), t0(F), t1(F);
+  for (size_t i = 0; i < F; ++i) {
+    const size_t n3 = (size_t)3 * fms[i]->N;
+    m0[i].allocate(n3);
+    m1[i].allocate(n3);
+    t0[i].allocate(n3);
+    t1[i].allocate(n3);
+  }
+  std::vector<double> h(F, 1e-14);
+  std::deque<float> last;
+  auto torque_into = [&](size_t i, DeviceArray<float>& dst) {
+    int nc = 0;
+    const float* t = fms[i]->eval_device("damping_torque", stream, &nc);
+    HIP_CHECK(hipMemcpyAsync(dst.get(), t, dst.bytes(), hipMemcpyDeviceToDevice, stream));
+  };
+  long long nsteps = 0;
+  auto converged = [&]() {
+    if (last.size() < nsamp) return false;
+    for (float d : last)
+      if (d > tol) return false;
+    return true;
+  };
+  while (!converged()) {
+    for (size_t i = 0; i < F; ++i) {
+      Ferromagnet* fm = fms[i];
+      HIP_CHECK(hipMemcpyAsync(m0[i].get(), fm->m.get(), m0[i].bytes(), hipMemcpyDeviceToDevice, stream));
+      if (nsteps == 0) torque_into(i, t0[i]);
+      else std::swap(t0[i], t1[i]);
+      launch_descent(m1[i].get(), m0[i].get(), t0[i].get(), (float)h[i], fm->N, stream);
+    }
+    for (size_t i = 0; i < F; ++i) {
+      // NormalizedVariable::set: normalised inside the geometry, zero outside
+      Combo co{};
+      co.nk = 0;
+      launch_combine(fms[i]->m.get(), m1[i].get(), co, 3, fms[i]->N, fms[i]->geom_ptr(), true, stream);
+    }
+    touch();
+    for (size_t i = 0; i < F; ++i) torque_into(i, t1[i]);
+    for (size_t i = 0; i < F; ++i) {
+      double d[3];
+      float maxdm = 0.f;
+      reducer.bb_sums(m1[i].get(), m0[i].get(), t1[i].get(), t0[i].get(), fms[i]->N, fms[i]->geom_ptr(), d, &maxdm,
+                      stream);
+      // BarzilianBorweinStepSize (minimizer.cu:76-90): alternating forms
+      const double nom = nsteps % 2 == 0 ? d[0] : d[1];
+      const double div = nsteps % 2 == 0 ? d[1] : d[2];
+      h[i] = div == 0.0 ? 1e-14 : (double)(float)(nom / div);
+      last.push_back(maxdm);
+      if (last.size() > nsamp) last.pop_front();
+    }
+    ++nsteps;
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 // ------------------------------------------------------------ TimeSolver

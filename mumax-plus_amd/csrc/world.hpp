@@ -118,6 +118,7 @@ class Ferromagnet : public MagnetCore {
   Antiferromagnet* host;  // non-null for a sublattice
   bool enable_demag = true;
   bool openbc = false;
+  double relax_threshold = -1.0;  // Ferromagnet.RelaxTorqueThreshold (wrap_ferromagnet.cpp:67)
 
   DeviceArray<float> m;       // magnetisation, 3N
   DeviceArray<float> x0;      // RK start state
@@ -139,9 +140,11 @@ class Ferromagnet : public MagnetCore {
   // Fills the demag field and the strain this magnet's terms need.
   void prepare(unsigned terms, hipStream_t s, Profiler* prof);
 
-  // out (device, 3N) = torque of the current magnetisation.  With prepared,
-  // the demag field and strain are taken as already computed.
-  void eval_rhs(float* out, hipStream_t s, bool check_stray = true, bool prepared = false);
+  // out (device, 3N) = torque of the current magnetisation (the LLG torque,
+  // or the damping-only torque of a relaxation).  With prepared, the demag
+  // field and strain are taken as already computed.
+  void eval_rhs(float* out, hipStream_t s, bool check_stray = true, bool prepared = false,
+                FieldMode mode = MODE_LLG);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;
   void average_quantity(const std::string& q, double* out, hipStream_t s);
@@ -258,16 +261,30 @@ class World {
   void touch() { ++version; }
 
   // Equations in the order of MumaxWorld::resetTimeSolverEquations
-  // (mumaxworld.cpp:159-205).
+  // (mumaxworld.cpp:159-205), or the relaxation's equations while it runs.
   std::vector<StateVar> state_vars();
   // k[stage] of every variable at the current state (all stage states set).
   void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray);
+
+  double relax_threshold = -1.0;  // World.RelaxTorqueThreshold (mumaxworld.cpp:26)
+  // Relaxer::exec (relaxer.cpp:87-174) of one magnet, or of the whole world
+  // when target is nullptr.
+  void relax(MagnetCore* target, double tol);
+  // Minimizer::exec (minimizer.cu:68-160) of one magnet, or of the whole
+  // world when target is nullptr.
+  void minimize(MagnetCore* target, double tol, int nsamples);
 
  private:
   void check_pbc(const int mg_size[3], const int reps[3]) const;
   void check_in_mastergrid(const int size[3], const int origin[3], const int mg_size[3], const int mg_origin[3]) const;
   void check_name(const std::string& name);
   void pbc_changed();
+  double total_energy(MagnetCore* m);
+  // Active while a relaxation runs: damping-only torques of these
+  // ferromagnets are the equations (plus the elastic ones for a world relax).
+  bool relaxing_ = false;
+  bool relax_elastic_ = false;
+  std::vector<Ferromagnet*> relax_fms_;
 };
 
 }  // namespace mpa

@@ -96,6 +96,16 @@ _SIGS = {
     "mpa_afm_eval": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_average": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_scalar": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
+    "mpa_world_relax": ([_P, _cdouble], _cint),
+    "mpa_fm_relax": ([_P, _cdouble], _cint),
+    "mpa_afm_relax": ([_P, _cdouble], _cint),
+    "mpa_world_minimize": ([_P, _cdouble, _cint], _cint),
+    "mpa_fm_minimize": ([_P, _cdouble, _cint], _cint),
+    "mpa_afm_minimize": ([_P, _cdouble, _cint], _cint),
+    "mpa_world_set_relax_torque_threshold": ([_P, _cdouble], _cint),
+    "mpa_world_get_relax_torque_threshold": ([_P, _ct.POINTER(_cdouble)], _cint),
+    "mpa_fm_set_relax_torque_threshold": ([_P, _cdouble], _cint),
+    "mpa_fm_get_relax_torque_threshold": ([_P, _ct.POINTER(_cdouble)], _cint),
     "mpa_ts_get": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
     "mpa_ts_set": ([_P, _c_char_p, _cdouble], _cint),
     "mpa_ts_sensible_timestep": ([_P, _ct.POINTER(_cdouble)], _cint),
@@ -548,6 +558,12 @@ class _Magnet:
     def scalar_quantity(self, name):
         return ScalarQuantity(self, name)
 
+    def relax(self, tol):
+        _call(self, "relax", float(tol))
+
+    def minimize(self, tol, nsamples):
+        _call(self, "minimize", float(tol), int(nsamples))
+
 
 class Ferromagnet(_Magnet):
     """Handle of a ferromagnet or sublattice owned by a World (wrap_ferromagnet.cpp:28-73)."""
@@ -588,6 +604,16 @@ class Ferromagnet(_Magnet):
     @property
     def is_sublattice(self):
         return self._host_handle() is not None
+
+    @property
+    def RelaxTorqueThreshold(self):
+        v = _cdouble()
+        _check(_lib.mpa_fm_get_relax_torque_threshold(self._h, _ct.byref(v)))
+        return v.value
+
+    @RelaxTorqueThreshold.setter
+    def RelaxTorqueThreshold(self, value):
+        _check(_lib.mpa_fm_set_relax_torque_threshold(self._h, float(value)))
 
 
 class Antiferromagnet(_Magnet):
@@ -846,6 +872,22 @@ class World:
             nm = raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(n[i]), float(by[i]))
         return out
+
+    def relax(self, tol):
+        _check(_lib.mpa_world_relax(self._h, float(tol)))
+
+    def minimize(self, tol, nsamples):
+        _check(_lib.mpa_world_minimize(self._h, float(tol), int(nsamples)))
+
+    @property
+    def RelaxTorqueThreshold(self):
+        v = _cdouble()
+        _check(_lib.mpa_world_get_relax_torque_threshold(self._h, _ct.byref(v)))
+        return v.value
+
+    @RelaxTorqueThreshold.setter
+    def RelaxTorqueThreshold(self, value):
+        _check(_lib.mpa_world_set_relax_torque_threshold(self._h, float(value)))
 
     def enable_multi_gpu(self, enable=True):
         if enable:

@@ -21,6 +21,7 @@ class Antiferromagnet(Magnet):
     """
 
     _PARAMS = ()
+    _MINIMIZE_SAMPLES = 20
     _FIELD_QUANTITIES = ("neel_vector", "full_magnetization", "total_energy_density")
     _SCALAR_QUANTITIES = ("total_energy",)
 

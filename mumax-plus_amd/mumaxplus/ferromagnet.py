@@ -76,6 +76,17 @@ class Ferromagnet(Magnet):
     def dmi_tensor(self):
         return DmiTensor(self._impl.dmi_tensor)
 
+    @property
+    def RelaxTorqueThreshold(self):
+        """Torque threshold of `relax` (default -1: relax until the torque
+        stops decreasing; positive: until max torque <= threshold)."""
+        return self._impl.RelaxTorqueThreshold
+
+    @RelaxTorqueThreshold.setter
+    def RelaxTorqueThreshold(self, value):
+        assert value != 0, "The relax threshold should not be zero."
+        self._impl.RelaxTorqueThreshold = value
+
     # --------------------------------------------------- magnetoelasticity
     @property
     def B1(self):

@@ -2,6 +2,8 @@
 of the reference): geometry, stray-field switches, elastodynamics with its
 variables, parameters and quantities."""
 
+import warnings
+
 import numpy as _np
 
 from .fieldquantity import FieldQuantity
@@ -174,6 +176,26 @@ class Magnet:
         if self.enable_elastodynamics:
             raise Exception("Can not use shear strain with elastodynamics enabled.")
         self.rigid_shear_strain.set(value)
+
+    # ------------------------------------------------ relax and minimize
+    _MINIMIZE_SAMPLES = 10
+
+    def minimize(self, tol=1e-6, nsamples=None):
+        """Minimise the total energy by steepest descent along the damping
+        torque with Barzilai-Borwein steps, until `nsamples` consecutive
+        maximum magnetisation changes are <= tol (10 by default, 20 for an
+        antiferromagnet)."""
+        self._impl.minimize(tol, self._MINIMIZE_SAMPLES if nsamples is None else nsamples)
+
+    def relax(self, tol=1e-9):
+        """Relax to an energy minimum by damping-only dynamics: first while
+        the energy decreases, then while the torque decreases as the solver
+        tolerance is lowered to `tol`."""
+        if tol >= 1e-5:
+            warnings.warn("The set tolerance is greater than or equal to the default value used for the "
+                          "timesolver (1e-5). Using this value results in no torque minimization, only energy "
+                          "minimization.", UserWarning)
+        self._impl.relax(tol)
 
     # ----------------------------------------- parameters and quantities
     def _lookup(self, key):

@@ -1,5 +1,7 @@
 """World (mumaxplus/world.py of the reference)."""
 
+import warnings
+
 from . import _mumaxpluscpp as _cpp
 from .grid import Grid
 from .timesolver import TimeSolver
@@ -100,3 +102,25 @@ class World:
 
     def unset_pbc(self):
         self._impl.unset_pbc()
+
+    def minimize(self, tol=1e-6, nsamples=10):
+        """Minimise the total energy of every magnet of the world together."""
+        self._impl.minimize(tol, nsamples)
+
+    def relax(self, tol=1e-9):
+        """Relax every magnet of the world (damping-only dynamics)."""
+        if tol >= 1e-5:
+            warnings.warn("The set tolerance is greater than or equal to the default value used for the "
+                          "timesolver (1e-5). Using this value results in no torque minimization, only energy "
+                          "minimization.", UserWarning)
+        self._impl.relax(tol)
+
+    @property
+    def RelaxTorqueThreshold(self):
+        """Torque threshold of `relax` (default -1)."""
+        return self._impl.RelaxTorqueThreshold
+
+    @RelaxTorqueThreshold.setter
+    def RelaxTorqueThreshold(self, value):
+        assert value != 0, "The relax threshold should not be zero."
+        self._impl.RelaxTorqueThreshold = value
