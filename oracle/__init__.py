@@ -29,8 +29,10 @@ Modules and the reference files each one restates:
 * ``model``   -- a minimal world / ferromagnet / antiferromagnet container
   that assembles the above the way src/physics/mumaxworld.cpp:159-205 and
   elastodynamics.cu:23-96 do.
+* ``relax``   -- total energies, the relaxer (src/physics/relaxer.cpp:87-174)
+  and the energy minimiser (src/physics/minimizer.cu:8-160).
 
 Pinning status of every function is recorded in each module's docstring.
 """
 
-from . import newell, demag, fields, rk, afm, elastic, model  # noqa: F401
+from . import newell, demag, fields, rk, afm, elastic, model, relax  # noqa: F401

@@ -242,12 +242,23 @@ def energy_density(mag, m, h, prefactor):
 
 
 def anisotropy_energy_density(mag, m):
-    if mag.uniaxial_assured_zero():
+    """k_unianisotropyEnergyDensity, else k_cubanisotropyEnergyDensity
+    (anisotropy.cu:142-221, chosen as in :223-253)."""
+    if not mag.uniaxial_assured_zero():
+        u = _normalized(mag.vec("anisU"))
+        k1, k2 = mag.val("ku1"), mag.val("ku2")
+        mu = dot(m, u)
+        e = -k1 * mu * mu - k2 * mu ** 4
+    elif not mag.cubic_assured_zero():
+        c1 = _normalized(mag.vec("anisC1"))
+        c2 = _normalized(mag.vec("anisC2"))
+        c3 = cross(c1, c2)
+        kc1, kc2, kc3 = mag.val("kc1"), mag.val("kc2"), mag.val("kc3")
+        a1, a2, a3 = dot(c1, m) ** 2, dot(c2, m) ** 2, dot(c3, m) ** 2
+        e = (kc1 * (a1 * a2 + a1 * a3 + a2 * a3) + kc2 * a1 * a2 * a3
+             + kc3 * (a1 * a1 * a2 * a2 + a1 * a1 * a3 * a3 + a2 * a2 * a3 * a3))
+    else:
         return np.zeros(mag.shape)
-    u = _normalized(mag.vec("anisU"))
-    k1, k2 = mag.val("ku1"), mag.val("ku2")
-    mu = dot(m, u)
-    e = -k1 * mu * mu - k2 * mu ** 4
     return np.where(_active(mag), e, 0.0)
 
 

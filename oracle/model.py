@@ -236,6 +236,10 @@ class World:
         self.magnets = []  # independent ferromagnets
         self.afms = []
         self.solver = _rk.TimeSolver(self)
+        # set by oracle.relax while a relaxation runs: the ferromagnets whose
+        # damping-only torque are the equations, and whether (u, v) stay in
+        self.relax_fms = None
+        self.relax_elastic = False
 
     def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
         mag = Ferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
@@ -337,9 +341,14 @@ class World:
     def _equations(self):
         """[(kind, object)]: 'm' for a ferromagnet or sublattice, 'u'/'v' for
         the elastic state of a magnet."""
-        eqs = [("m", mag) for mag in self.magnets]
-        for afm in self.afms:
-            eqs += [("m", afm.sub1), ("m", afm.sub2)]
+        if self.relax_fms is not None:
+            eqs = [("m", mag) for mag in self.relax_fms]
+            if not self.relax_elastic:
+                return eqs
+        else:
+            eqs = [("m", mag) for mag in self.magnets]
+            for afm in self.afms:
+                eqs += [("m", afm.sub1), ("m", afm.sub2)]
         for own in self.magnets + self.afms:
             if own.enable_elastodynamics:
                 eqs += [("u", own), ("v", own)]
@@ -373,7 +382,8 @@ class World:
                 if obj.host is not None:
                     m2 = m_of(obj.host.sub2 if obj is obj.host.sub1 else obj.host.sub1)
                 h = self.effective_field(obj, m_of(obj), m2, uv_of(own, "u"))
-                out.append(_fields.llg_torque(obj, m_of(obj), h))
+                torque = _fields.damping_torque if self.relax_fms is not None else _fields.llg_torque
+                out.append(torque(obj, m_of(obj), h))
             elif kind == "u":
                 out.append(uv_of(obj, "v"))
             else:
