@@ -78,13 +78,14 @@ struct ParamSpec {
   int ncomp;
   float def[3];
   const char* unit;
+  const char* label;  // name suffix when it is not ":" + key
 };
 // Defaults of the reference Ferromagnet constructor (ferromagnet.cpp:24-61).
 const ParamSpec kFmParams[] = {
     {"msat", 1, {1.f, 0, 0}, "A/m"},          {"aex", 1, {0, 0, 0}, "J/m"},
     {"alpha", 1, {0, 0, 0}, ""},              {"ku1", 1, {0, 0, 0}, "J/m3"},
-    {"ku2", 1, {0, 0, 0}, "J/m3"},            {"kc1", 1, {0, 0, 0}, "J/m3"},
-    {"kc2", 1, {0, 0, 0}, "J/m3"},            {"kc3", 1, {0, 0, 0}, "J/m3"},
+    {"ku2", 1, {0, 0, 0}, "J/m3", "ku2"},     {"kc1", 1, {0, 0, 0}, "J/m3"},
+    {"kc2", 1, {0, 0, 0}, "J/m3"},            {"kc3", 1, {0, 0, 0}, "J/m3", "kc3"},
     {"frozen_spins", 1, {0, 0, 0}, ""},       {"anisU", 3, {0, 0, 0}, ""},
     {"anisC1", 3, {0, 0, 0}, ""},             {"anisC2", 3, {0, 0, 0}, ""},
     {"bias_magnetic_field", 3, {0, 0, 0}, "T"}, {"dmi_xxy", 1, {0, 0, 0}, "J/m2"},
@@ -92,8 +93,10 @@ const ParamSpec kFmParams[] = {
     {"dmi_yxy", 1, {0, 0, 0}, "J/m2"},        {"dmi_yyz", 1, {0, 0, 0}, "J/m2"},
     {"dmi_yxz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxy", 1, {0, 0, 0}, "J/m2"},
     {"dmi_zyz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxz", 1, {0, 0, 0}, "J/m2"},
-    {"B1", 1, {0, 0, 0}, "J/m3"},             {"B2", 1, {0, 0, 0}, "J/m3"},
+    {"B1", 1, {0, 0, 0}, "J/m3"},             {"B2", 1, {0, 0, 0}, "J/m3", ":B1"},
 };
+// The labels of ku2, kc3 (no colon) and B2 (":B1") are the reference's own
+// parameter names (ferromagnet.cpp:29,32,61), exposed as Parameter.name.
 // Elastic parameters of every magnet (magnet.cpp:26-34).
 const ParamSpec kElasticParams[] = {
     {"C11", 1, {0, 0, 0}, "N/m2"},  {"C12", 1, {0, 0, 0}, "N/m2"},
@@ -167,7 +170,7 @@ void MagnetCore::add_params(const void* specs, int count) {
   const ParamSpec* sp = static_cast<const ParamSpec*>(specs);
   for (int i = 0; i < count; ++i) {
     Param p;
-    p.name = name + ":" + sp[i].key;
+    p.name = sp[i].label ? name + sp[i].label : name + ":" + sp[i].key;
     p.unit = sp[i].unit;
     p.ncomp = sp[i].ncomp;
     for (int c = 0; c < 3; ++c) p.uniform[c] = sp[i].def[c];
