@@ -42,6 +42,23 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
 // LDS index with one pad slot every 8 elements.
 __host__ __device__ constexpr int lpad(int i) { return i + (i >> 3); }
 
+// Synchronisation of an exchange between the T threads of one line.  Lines
+// are aligned to wavefronts (thread t of line g is threadIdx.x = g*T + t), so
+// with T <= 64 a line lives in one wavefront: its LDS accesses execute in
+// issue order, and ordering them at compile time (a wavefront-scope fence
+// around the wave barrier) is enough.  Longer lines span several wavefronts
+// and need the block barrier.
+template <int T>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (T <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Pass schedule of an L = 2^L2 transform.
 __host__ __device__ constexpr int fft_npass(int L2) { return L2 < 3 ? 0 : L2 / 3 + (L2 % 3 ? 1 : 0); }
 __host__ __device__ constexpr int fft_radix(int L2, int p) {
@@ -145,7 +162,8 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[NC][8], int c, int m, 
 // line i = P*NC + c through buffer i % NBUF; with NBUF >= 2 one barrier per
 // line suffices, because a buffer is rewritten only after the barrier that
 // follows the next line's stores.  The last pass keeps its outputs in
-// registers.  Every thread of the block must call this.
+// registers.  Every thread of the block must call this; the barriers are
+// line_sync (wavefront-level for lines of at most 64 threads).
 template <int L2, int P, bool INV, int NC, int NBUF>
 __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds, int bstride,
                                          const float2* __restrict__ tw) {
@@ -181,22 +199,22 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds,
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], lds + c * bstride);
-    __syncthreads();
+    line_sync<T>();
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int q = 0; q < 8; ++q) v[c][q] = lds[c * bstride + rb + T * q + ((T * q) >> 3)];
-    __syncthreads();
+    line_sync<T>();
   } else {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       float2* buf = lds + ((P * NC + c) % NBUF) * bstride;
 #pragma unroll
       for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], buf);
-      __syncthreads();
+      line_sync<T>();
 #pragma unroll
       for (int q = 0; q < 8; ++q) v[c][q] = buf[rb + T * q + ((T * q) >> 3)];
-      if constexpr (NBUF == 1) __syncthreads();
+      if constexpr (NBUF == 1) line_sync<T>();
     }
   }
 }
@@ -222,7 +240,7 @@ __device__ __forceinline__ void line_fft(float2 (&v)[NC][8], int t, float2* lds,
   static_assert(NBUF >= 1, "at least one exchange buffer");
   fft_passes<L2, 0, INV, NC, NBUF>(v, t, lds, bstride, tw);
   // sequential exchanges through two or more buffers end without a barrier
-  if constexpr (NBUF >= 2 && NBUF < NC && fft_npass(L2) > 1) __syncthreads();
+  if constexpr (NBUF >= 2 && NBUF < NC && fft_npass(L2) > 1) line_sync<(1 << L2) / 8>();
 }
 
 }  // namespace mpa
