@@ -385,47 +385,60 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
     return f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
               -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
   };
-  // One neighbour direction with the full boundary rules (wrap, geometry,
-  // Neumann with the DMI correction, open boundary).
-  auto general = [&](int dir) {
-    const int axis = dir >> 1;
-    const float s = (dir & 1) ? 1.f : -1.f;
-    const int nb = neighbour(d, x, y, z, dir);
-    const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
-    if (nb_in) {
-      const float3 mn = load3(m, nb, N);
-      if (u.has_ex) hex = hex + u.kex[axis] * (mn - mc);
-      if (u.has_dmi) hdm = hdm + s * dmi_of(axis, mn);
-    } else {
-      const float3 gam = gamma_u(u, axis, mc);
-      if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
-      if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
-    }
-  };
-  // Both neighbours along `axis` present and plain: the two exchange terms
-  // combine to kex (m+ + m- - 2m), the two DMI terms to D (m+ - m-).
-  auto pair = [&](int axis, int stride) {
-    const float3 mm = load3(m, idx - stride, N), mp = load3(m, idx + stride, N);
-    if (u.has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
-    if (u.has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
+  // Missing or outside-geometry neighbour: Neumann with the DMI correction,
+  // or the open boundary.
+  auto boundary = [&](int axis, float s) {
+    const float3 gam = gamma_u(u, axis, mc);
+    if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
+    if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
   };
   if (u.has_ex || u.has_dmi) {
-    // x keeps the general rules in every lane: the two edge lanes of a row
-    // would otherwise make their waves run both paths for all axes
-    general(0);
-    general(1);
-    if (!d.geom && y > 0 && y < d.ny - 1) {
-      pair(1, d.nx);
-    } else {
-      general(2);
-      general(3);
+    // Every neighbour value is loaded before any is used (a missing
+    // neighbour reads the cell itself and is not used), so the six loads are
+    // in flight together; the wrap, geometry and boundary rules then act on
+    // registers.  A single layer without z-axis DMI entries skips z: its two
+    // terms cancel.
+    const int ndir = u.zflat ? 4 : 6;
+    int nbi[6];
+    bool ok[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (dir < ndir) {
+        const int nb = neighbour(d, x, y, z, dir);
+        ok[dir] = nb >= 0 && (!d.geom || d.geom[nb]);
+        nbi[dir] = nb >= 0 ? nb : idx;
+      } else {
+        ok[dir] = false;
+        nbi[dir] = idx;
+      }
     }
-    if (!d.geom && z > 0 && z < d.nz - 1) {
-      pair(2, d.nx * d.ny);
-    } else if (!u.zflat) {
-      general(4);
-      general(5);
-    }  // single layer without z-axis DMI entries: the two z terms cancel
+    float3 mn[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir)
+      if (dir < ndir) mn[dir] = load3(m, nbi[dir], N);
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+      if (2 * axis >= ndir) break;
+      const float3 mm = mn[2 * axis], mp = mn[2 * axis + 1];
+      if (ok[2 * axis] && ok[2 * axis + 1]) {
+        // both neighbours plain: kex (m+ + m- - 2m) and D (m+ - m-)
+        if (u.has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
+        if (u.has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
+      } else {
+        if (ok[2 * axis]) {
+          if (u.has_ex) hex = hex + u.kex[axis] * (mm - mc);
+          if (u.has_dmi) hdm = hdm - dmi_of(axis, mm);
+        } else {
+          boundary(axis, -1.f);
+        }
+        if (ok[2 * axis + 1]) {
+          if (u.has_ex) hex = hex + u.kex[axis] * (mp - mc);
+          if (u.has_dmi) hdm = hdm + dmi_of(axis, mp);
+        } else {
+          boundary(axis, 1.f);
+        }
+      }
+    }
   }
   float3 han = f3(0.f, 0.f, 0.f);
   if (u.uni) {
