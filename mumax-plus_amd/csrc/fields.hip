@@ -68,6 +68,9 @@ __device__ __forceinline__ int neighbour(const FMDesc& d, int x, int y, int z, i
   return (c[2] * d.ny + c[1]) * d.nx + c[0];
 }
 
+// UNI: every parameter is uniform, so each lookup is the scalar and the
+// harmonic mean of a cell and its neighbour is the value itself.
+template <bool UNI>
 __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restrict__ m,
                                                  const float* __restrict__ hdemag, float* __restrict__ out,
                                                  int mode) {
@@ -76,6 +79,9 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   const int idx = bid * BLOCK + threadIdx.x;
   const int N = d.ncell;
   if (idx >= N) return;
+  auto S = [&](const PS& p, int i) { return UNI ? p.v : p.at(i); };
+  auto V = [&](const PV& p, int i) { return UNI ? make_float3(p.x, p.y, p.z) : p.at(i, N); };
+  auto HM = [&](float u, float v) { return UNI ? u : harmonic_mean(u, v); };
   const bool ingeom = !d.geom || d.geom[idx];
   if (!ingeom) {
     if (mode == MODE_EDENS) out[idx] = 0.f;
@@ -86,7 +92,7 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   const int y = (idx / d.nx) % d.ny;
   const int z = idx / (d.nx * d.ny);
   const float3 mc = load3(m, idx, N);
-  const float ms = d.msat.at(idx);
+  const float ms = S(d.msat, idx);
   float edens = 0.f;
 
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
@@ -95,12 +101,34 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   const bool sub = d.sublattice != 0;
   const float3 m2c = sub ? load3(d.m2, idx, N) : f3(0.f, 0.f, 0.f);
   const float csz[3] = {d.cx, d.cy, d.cz};
+  const float icsz[3] = {1.f / d.cx, 1.f / d.cy, 1.f / d.cz};
+  // Neighbour table, and every neighbour value loaded before any is used: the
+  // index after the wrap (the cell itself when missing), whether it is an
+  // in-geometry neighbour, and m (and the sister's m) there.  The opposite
+  // neighbour of direction dir is entry dir ^ 1.
+  int nbtab[6];
+  bool nin[6];
+  float3 mn_[6], m2n_[6];
+  const bool stencil = do_ex || do_dmi || (sub && (d.terms & TERM_AFM_NN));
+  if (stencil) {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      const int nb = neighbour(d, x, y, z, dir);
+      nin[dir] = nb >= 0 && (!d.geom || d.geom[nb]);
+      nbtab[dir] = nin[dir] ? nb : idx;
+    }
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      mn_[dir] = load3(m, nbtab[dir], N);
+      m2n_[dir] = sub ? load3(d.m2, nbtab[dir], N) : f3(0.f, 0.f, 0.f);
+    }
+  }
   if (do_ex || do_dmi) {
-    const float a = d.aex.at(idx);
-    const float an = sub ? d.afmex_nn.at(idx) : 0.f;
+    const float a = S(d.aex, idx);
+    const float an = sub ? S(d.afmex_nn, idx) : 0.f;
     float D[D_COUNT];
 #pragma unroll
-    for (int k = 0; k < D_COUNT; ++k) D[k] = d.dmi[k].at(idx);
+    for (int k = 0; k < D_COUNT; ++k) D[k] = S(d.dmi[k], idx);
     const float w[3] = {1.f / (d.cx * d.cx), 1.f / (d.cy * d.cy), 1.f / (d.cz * d.cz)};
     const bool openbc = d.openbc != 0;
     const bool open_dmi = openbc || a == 0.f;
@@ -108,18 +136,16 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
     for (int dir = 0; dir < 6; ++dir) {
       const int axis = dir >> 1;
       const float delta = (dir & 1) ? csz[axis] : -csz[axis];
-      const int nb = neighbour(d, x, y, z, dir);
-      const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+      const float idelta = (dir & 1) ? icsz[axis] : -icsz[axis];
+      const int nb = nbtab[dir];
+      const bool nb_in = nin[dir];
       float3 gamma = f3(0.f, 0.f, 0.f);
       // sister's normal derivative taken on the bulk side of a boundary
       // (exchange.cu:176-183, dmi.cu:200-207)
       float3 dm2 = f3(0.f, 0.f, 0.f);
       if (!nb_in) {
         gamma = dmi_gamma(D, axis, mc);
-        if (sub) {
-          const int ob = neighbour(d, x, y, z, dir ^ 1);
-          if (ob >= 0 && (!d.geom || d.geom[ob])) dm2 = (1.f / delta) * (m2c - load3(d.m2, ob, N));
-        }
+        if (sub && nin[dir ^ 1]) dm2 = idelta * (m2c - m2n_[dir ^ 1]);
       }
       if (do_ex) {
         bool use = true;
@@ -132,9 +158,9 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
             mn = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
             an_ = a;
           } else {
-            if (d.msat.at(nb) == 0.f) use = false;
-            mn = load3(m, nb, N);
-            an_ = d.aex.at(nb);
+            if (S(d.msat, nb) == 0.f) use = false;
+            mn = mn_[dir];
+            an_ = S(d.aex, nb);
           }
         } else if (!nb_in) {
           // exchange.cu:61-101
@@ -142,65 +168,66 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
           mn = mc + (delta / (2 * a)) * gamma;
           an_ = a;
         } else {
-          if (openbc && d.msat.at(nb) == 0.f) use = false;
-          mn = load3(m, nb, N);
-          an_ = d.aex.at(nb);
+          if (openbc && S(d.msat, nb) == 0.f) use = false;
+          mn = mn_[dir];
+          an_ = S(d.aex, nb);
         }
         if (use) {
-          const float A = harmonic_mean(a, an_);
+          const float A = HM(a, an_);
           hex = hex + (2 * A * w[axis]) * (mn - mc);
         }
       }
       if (do_dmi) {
-        const int nbi = nb_in ? nb : idx;
+        const int nbi = nb;  // the cell itself when the neighbour is missing
         const int kxz = axis == 0 ? D_XXZ : (axis == 1 ? D_YXZ : D_ZXZ);
         const int kxy = axis == 0 ? D_XXY : (axis == 1 ? D_YXY : D_ZXY);
         const int kyz = axis == 0 ? D_XYZ : (axis == 1 ? D_YYZ : D_ZYZ);
         float Dxz = D[kxz], Dxy = D[kxy], Dyz = D[kyz];
         if (nbi != idx) {
-          Dxz = harmonic_mean(Dxz, d.dmi[kxz].at(nbi));
-          Dxy = harmonic_mean(Dxy, d.dmi[kxy].at(nbi));
-          Dyz = harmonic_mean(Dyz, d.dmi[kyz].at(nbi));
+          Dxz = HM(Dxz, S(d.dmi[kxz], nbi));
+          Dxy = HM(Dxy, S(d.dmi[kxy], nbi));
+          Dyz = HM(Dyz, S(d.dmi[kyz], nbi));
         }
         if (!sub) {
           // dmi.cu:114-173
-          const bool skip = open_dmi && (!nb_in || d.msat.at(nbi) == 0.f);
+          const bool skip = open_dmi && (!nb_in || S(d.msat, nbi) == 0.f);
           if (!skip) {
             float3 mn;
             if (!nb_in && !open_dmi) mn = mc + (delta / (2 * a)) * gamma;
-            else mn = load3(m, nbi, N);
-            hdm.x += (Dxy * mn.y + Dxz * mn.z) / delta;
-            hdm.y += (-Dxy * mn.x + Dyz * mn.z) / delta;
-            hdm.z += (-Dxz * mn.x - Dyz * mn.y) / delta;
+            else mn = mn_[dir];
+            hdm.x += (Dxy * mn.y + Dxz * mn.z) * idelta;
+            hdm.y += (-Dxy * mn.x + Dyz * mn.z) * idelta;
+            hdm.z += (-Dxz * mn.x - Dyz * mn.y) * idelta;
           }
         } else {
           // sublattice with the inter-sublattice tensor, dmi.cu:112-240
-          const bool skip = open_dmi && (!nb_in || (d.msat.at(nbi) == 0.f && d.msat2.at(nbi) == 0.f));
+          const bool skip = open_dmi && (!nb_in || (S(d.msat, nbi) == 0.f && S(d.msat2, nbi) == 0.f));
           if (!skip) {
-            const float Dixz = harmonic_mean(d.hdmi[kxz].at(idx), d.hdmi[kxz].at(nbi));
-            const float Dixy = harmonic_mean(d.hdmi[kxy].at(idx), d.hdmi[kxy].at(nbi));
-            const float Diyz = harmonic_mean(d.hdmi[kyz].at(idx), d.hdmi[kyz].at(nbi));
+            const float Dixz = HM(S(d.hdmi[kxz], idx), S(d.hdmi[kxz], nbi));
+            const float Dixy = HM(S(d.hdmi[kxy], idx), S(d.hdmi[kxy], nbi));
+            const float Diyz = HM(S(d.hdmi[kyz], idx), S(d.hdmi[kyz], nbi));
             float3 m1n, m2n;
             if (!nb_in) {
               // Neumann boundary; both sublattices use this sublattice's own tensor
               const float3 g2 = dmi_gamma(D, axis, m2c);
               m1n = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
-              m2n = m2c + (delta / (2 * a)) * (an * cross3(cross3((1.f / delta) * (m1n - mc), m2c), m2c) + g2);
+              m2n = m2c + (delta / (2 * a)) * (an * cross3(cross3(idelta * (m1n - mc), m2c), m2c) + g2);
             } else {
-              m1n = load3(m, nbi, N);
-              m2n = load3(d.m2, nbi, N);
+              m1n = mn_[dir];
+              m2n = m2n_[dir];
             }
-            const float s1 = d.msat.at(nbi) != 0.f ? 1.f : 0.f;
-            const float s2 = d.msat2.at(nbi) != 0.f ? 1.f : 0.f;
-            hdm.x += (s1 * (Dxy * m1n.y + Dxz * m1n.z) - s2 * (Dixy * m2n.y - Dixz * m2n.z)) / delta;
-            hdm.y += (s1 * (-Dxy * m1n.x + Dyz * m1n.z) - s2 * (-Dixy * m2n.x - Diyz * m2n.z)) / delta;
-            hdm.z += (s1 * (-Dxz * m1n.x - Dyz * m1n.y) - s2 * (-Dixz * m2n.x + Diyz * m2n.y)) / delta;
+            const float s1 = S(d.msat, nbi) != 0.f ? 1.f : 0.f;
+            const float s2 = S(d.msat2, nbi) != 0.f ? 1.f : 0.f;
+            hdm.x += (s1 * (Dxy * m1n.y + Dxz * m1n.z) - s2 * (Dixy * m2n.y - Dixz * m2n.z)) * idelta;
+            hdm.y += (s1 * (-Dxy * m1n.x + Dyz * m1n.z) - s2 * (-Dixy * m2n.x - Diyz * m2n.z)) * idelta;
+            hdm.z += (s1 * (-Dxz * m1n.x - Dyz * m1n.y) - s2 * (-Dixz * m2n.x + Diyz * m2n.y)) * idelta;
           }
         }
       }
     }
-    if (do_ex) hex = hex / ms;
-    if (do_dmi) hdm = hdm / ms;
+    const float ims = 1.f / ms;
+    if (do_ex) hex = ims * hex;
+    if (do_dmi) hdm = ims * hdm;
   }
 
   // magnetoelastic field, magnetoelasticfield.cu:44-103 (no msat check, as there)
@@ -208,13 +235,13 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   if (d.terms & TERM_ME) {
     float e[6];
     if (d.me_rigid) {
-      const float3 n = d.rnorm.at(idx, N), s = d.rshear.at(idx, N);
+      const float3 n = V(d.rnorm, idx), s = V(d.rshear, idx);
       e[0] = n.x; e[1] = n.y; e[2] = n.z; e[3] = s.x; e[4] = s.y; e[5] = s.z;
     } else {
 #pragma unroll
       for (int c = 0; c < 6; ++c) e[c] = d.strain[(size_t)c * N + idx];
     }
-    const float b1 = d.B1.at(idx), b2 = d.B2.at(idx), f = -2.f / ms;
+    const float b1 = S(d.B1, idx), b2 = S(d.B2, idx), f = -2.f / ms;
     hme = f3(f * (b1 * e[0] * mc.x + b2 * (e[3] * mc.y + e[4] * mc.z)),
              f * (b1 * e[1] * mc.y + b2 * (e[5] * mc.z + e[3] * mc.x)),
              f * (b1 * e[2] * mc.z + b2 * (e[4] * mc.x + e[5] * mc.y)));
@@ -224,39 +251,40 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   // sister's msat (afmexchange.cu:28-149)
   float3 hafm_nn = f3(0.f, 0.f, 0.f), hafm_site = f3(0.f, 0.f, 0.f);
   if (sub && (d.terms & (TERM_AFM_NN | TERM_AFM_SITE))) {
-    const float ms2 = d.msat2.at(idx);
+    const float ms2 = S(d.msat2, idx);
     if (ms2 != 0.f && (d.terms & TERM_AFM_SITE)) {
-      const float l = d.latcon.at(idx);
-      hafm_site = (4.f * d.afmex_cell.at(idx) / (l * l) / ms2) * m2c;
+      const float l = S(d.latcon, idx);
+      hafm_site = (4.f * S(d.afmex_cell, idx) / (l * l) / ms2) * m2c;
     }
     if (ms2 != 0.f && (d.terms & TERM_AFM_NN)) {
-      const float a = d.aex.at(idx);
-      const float ann = d.afmex_nn.at(idx);
+      const float a = S(d.aex, idx);
+      const float ann = S(d.afmex_nn, idx);
       const bool open_nn = d.openbc != 0 || a == 0.f;
       float D[D_COUNT];
 #pragma unroll
-      for (int k = 0; k < D_COUNT; ++k) D[k] = d.dmi[k].at(idx);
+      for (int k = 0; k < D_COUNT; ++k) D[k] = S(d.dmi[k], idx);
+#pragma unroll
       for (int dir = 0; dir < 6; ++dir) {
         const int axis = dir >> 1;
         const float delta = (dir & 1) ? csz[axis] : -csz[axis];
-        const int nb = neighbour(d, x, y, z, dir);
-        const bool nb_in = nb >= 0 && (!d.geom || d.geom[nb]);
+        const float idelta = (dir & 1) ? icsz[axis] : -icsz[axis];
+        const int nb = nbtab[dir];
+        const bool nb_in = nin[dir];
         if (!nb_in && open_nn) continue;
-        if (open_nn && d.msat2.at(nb) == 0.f) continue;  // nb_in here
+        if (open_nn && S(d.msat2, nb) == 0.f) continue;  // nb_in here
         float3 m2n;
         float ann_;
         if (nb_in) {
-          m2n = load3(d.m2, nb, N);
-          ann_ = d.afmex_nn.at(nb);
+          m2n = m2n_[dir];
+          ann_ = S(d.afmex_nn, nb);
         } else {
           const float3 g2 = dmi_gamma(D, axis, m2c);
           float3 dm1 = f3(0.f, 0.f, 0.f);
-          const int ob = neighbour(d, x, y, z, dir ^ 1);
-          if (ob >= 0 && (!d.geom || d.geom[ob])) dm1 = (1.f / delta) * (mc - load3(m, ob, N));
+          if (nin[dir ^ 1]) dm1 = idelta * (mc - mn_[dir ^ 1]);
           m2n = m2c + (delta / (2 * a)) * (ann * cross3(cross3(dm1, m2c), m2c) + g2);
           ann_ = ann;
         }
-        hafm_nn = hafm_nn + (harmonic_mean(ann, ann_) / (delta * delta)) * (m2n - m2c);
+        hafm_nn = hafm_nn + (HM(ann, ann_) * (idelta * idelta)) * (m2n - m2c);
       }
       hafm_nn = hafm_nn / ms2;
     }
@@ -265,16 +293,16 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   float3 han = f3(0.f, 0.f, 0.f);
   if (ms != 0.f) {
     if (d.terms & TERM_UNI) {
-      const float3 u = normalized3(d.anisU.at(idx, N));
-      const float k1 = d.ku1.at(idx), k2 = d.ku2.at(idx);
+      const float3 u = normalized3(V(d.anisU, idx));
+      const float k1 = S(d.ku1, idx), k2 = S(d.ku2, idx);
       const float mu = dot3(mc, u);
       han = ((2 * k1 * mu + 4 * k2 * mu * mu * mu) / ms) * u;
       if (mode == MODE_EDENS) edens += -k1 * mu * mu - k2 * mu * mu * mu * mu;
     } else if (d.terms & TERM_CUBIC) {
-      const float3 c1 = normalized3(d.anisC1.at(idx, N));
-      const float3 c2 = normalized3(d.anisC2.at(idx, N));
+      const float3 c1 = normalized3(V(d.anisC1, idx));
+      const float3 c2 = normalized3(V(d.anisC2, idx));
       const float3 c3 = cross3(c1, c2);
-      const float kc1 = d.kc1.at(idx), kc2 = d.kc2.at(idx), kc3 = d.kc3.at(idx);
+      const float kc1 = S(d.kc1, idx), kc2 = S(d.kc2, idx), kc3 = S(d.kc3, idx);
       const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
       const float s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
       han = (-2 * kc1 / ms) * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
@@ -287,7 +315,7 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
     }
   }
   float3 hext = f3(0.f, 0.f, 0.f);
-  if (d.terms & TERM_EXT) hext = f3(d.wb[0], d.wb[1], d.wb[2]) + d.bias.at(idx, N);
+  if (d.terms & TERM_EXT) hext = f3(d.wb[0], d.wb[1], d.wb[2]) + V(d.bias, idx);
   float3 hdem = f3(0.f, 0.f, 0.f);
   if (d.terms & TERM_DEMAG) hdem = load3(hdemag, idx, N);
 
@@ -312,10 +340,10 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   float3 r;
   if (mode == MODE_FIELD) {
     r = h;
-  } else if (d.frozen.at(idx) != 0.f) {
+  } else if (S(d.frozen, idx) != 0.f) {
     r = f3(0.f, 0.f, 0.f);
   } else if (mode == MODE_LLG) {
-    const float al = d.alpha.at(idx);
+    const float al = S(d.alpha, idx);
     const float3 mxh = cross3(mc, h);
     const float3 mxmxh = cross3(mc, mxh);
     r = (-GAMMALL / (1 + al * al)) * (mxh + al * mxmxh);
@@ -476,6 +504,15 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   out[2 * N + idx] = r.z;
 }
 
+// Every parameter the per-cell kernel reads is uniform.
+bool params_uniform(const FMDesc& d) {
+  bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
+           !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p && !d.B1.p && !d.B2.p &&
+           !d.rnorm.p && !d.rshear.p && !d.msat2.p && !d.afmex_cell.p && !d.afmex_nn.p && !d.latcon.p;
+  for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p && !d.hdmi[k].p;
+  return u;
+}
+
 bool all_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
            !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p;
@@ -555,7 +592,8 @@ void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* o
     return;
   }
   const int nblk = (d.ncell + BLOCK - 1) / BLOCK;
-  k_field<<<nblk, BLOCK, 0, s>>>(d, m, hdemag, out, (int)mode);
+  if (params_uniform(d)) k_field<true><<<nblk, BLOCK, 0, s>>>(d, m, hdemag, out, (int)mode);
+  else k_field<false><<<nblk, BLOCK, 0, s>>>(d, m, hdemag, out, (int)mode);
   checkLaunch("k_field");
 }
 
