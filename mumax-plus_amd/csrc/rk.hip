@@ -88,6 +88,83 @@ __global__ void __launch_bounds__(BLOCK) k_combine4(float* __restrict__ out, con
   for (int c = 0; c < 3; ++c) reinterpret_cast<float4*>(out + (size_t)c * N)[i4] = v[c];
 }
 
+struct CombineBatch {
+  CombineJob job[MAX_JOBS];
+};
+
+// One job per blockIdx.y; four consecutive cells per thread when N is a
+// multiple of 4, one otherwise.  Same arithmetic as k_combine / k_combine4.
+__global__ void __launch_bounds__(BLOCK) k_combine_batch(CombineBatch b) {
+  const CombineJob& jb = b.job[blockIdx.y];
+  const int N = jb.N;
+  const Combo& co = jb.co;
+  if ((N & 3) == 0) {
+    const int i4 = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (i4 >= (N >> 2)) return;
+    float4 v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float4 a = reinterpret_cast<const float4*>(jb.x0 + (size_t)c * N)[i4];
+      for (int j = 0; j < co.nk; ++j) {
+        const float4 kv = reinterpret_cast<const float4*>(co.k[j] + (size_t)c * N)[i4];
+        const float cj = co.c[j];
+        a.x += cj * kv.x;
+        a.y += cj * kv.y;
+        a.z += cj * kv.z;
+        a.w += cj * kv.w;
+      }
+      v[c] = a;
+    }
+    if (jb.normalize) {
+      const float i0 = rsqrtf(v[0].x * v[0].x + v[1].x * v[1].x + v[2].x * v[2].x);
+      const float i1 = rsqrtf(v[0].y * v[0].y + v[1].y * v[1].y + v[2].y * v[2].y);
+      const float i2 = rsqrtf(v[0].z * v[0].z + v[1].z * v[1].z + v[2].z * v[2].z);
+      const float i3 = rsqrtf(v[0].w * v[0].w + v[1].w * v[1].w + v[2].w * v[2].w);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        v[c].x *= i0;
+        v[c].y *= i1;
+        v[c].z *= i2;
+        v[c].w *= i3;
+      }
+    }
+    if (jb.geom) {
+      const uchar4 g = reinterpret_cast<const uchar4*>(jb.geom)[i4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (!g.x) v[c].x = 0.f;
+        if (!g.y) v[c].y = 0.f;
+        if (!g.z) v[c].z = 0.f;
+        if (!g.w) v[c].w = 0.f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      reinterpret_cast<float4*>(jb.out + (size_t)c * N)[i4] = v[c];
+      if (jb.out2) reinterpret_cast<float4*>(jb.out2 + (size_t)c * N)[i4] = v[c];
+    }
+    return;
+  }
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (idx >= N) return;
+  float v[3];
+  const bool in = !jb.geom || jb.geom[idx];
+  for (int c = 0; c < 3; ++c) {
+    float a = jb.x0[(size_t)c * N + idx];
+    for (int j = 0; j < co.nk; ++j) a += co.c[j] * co.k[j][(size_t)c * N + idx];
+    v[c] = a;
+  }
+  if (jb.normalize) {
+    const float inv = rsqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int c = 0; c < 3; ++c) v[c] *= inv;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const float r = in ? v[c] : 0.f;
+    jb.out[(size_t)c * N + idx] = r;
+    if (jb.out2) jb.out2[(size_t)c * N + idx] = r;
+  }
+}
+
 __device__ __forceinline__ float block_max(float v) {
   __shared__ float sh[BLOCK / 64];
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
@@ -247,6 +324,21 @@ void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int
   const int nblk = (N + BLOCK - 1) / BLOCK;
   k_combine<<<nblk, BLOCK, 0, s>>>(out, x0, co, ncomp, N, geom, normalize ? 1 : 0);
   checkLaunch("k_combine");
+}
+
+void launch_combine_batch(const CombineJob* jobs, int njobs, hipStream_t s) {
+  if (njobs < 1) return;
+  if (njobs > MAX_JOBS) throw Error(MPA_ERR_INVALID, "launch_combine_batch: too many variables");
+  CombineBatch b{};
+  int nmax = 0;
+  for (int i = 0; i < njobs; ++i) {
+    b.job[i] = jobs[i];
+    const int per = (jobs[i].N & 3) == 0 ? jobs[i].N / 4 : jobs[i].N;
+    nmax = per > nmax ? per : nmax;
+  }
+  const dim3 grid((unsigned)((nmax + BLOCK - 1) / BLOCK), (unsigned)njobs);
+  k_combine_batch<<<grid, BLOCK, 0, s>>>(b);
+  checkLaunch("k_combine_batch");
 }
 
 Reducer::Reducer() : maxbits_(1), partials_((size_t)3 * SUM_BLOCKS + 3) {}

@@ -18,6 +18,20 @@ struct Combo {
 void launch_combine(float* out, const float* x0, const Combo& co, int ncomp, int N, const uint8_t* geom,
                     bool normalize, hipStream_t s);
 
+// Several 3-vector combinations in one launch (the state variables of one
+// Runge-Kutta stage).  out2, when set, receives a second copy of the result.
+constexpr int MAX_JOBS = 8;
+struct CombineJob {
+  float* out;
+  float* out2;
+  const float* x0;
+  const uint8_t* geom;
+  Combo co;
+  int N;
+  int normalize;
+};
+void launch_combine_batch(const CombineJob* jobs, int njobs, hipStream_t s);
+
 // m = ((4 - h^2 |t|^2) m0 + 4 h t) / (4 + h^2 |t|^2) per cell, the descent step
 // of the energy minimiser (src/physics/minimizer.cu:59-74).
 void launch_descent(float* m, const float* m0, const float* t, float h, int N, hipStream_t s);

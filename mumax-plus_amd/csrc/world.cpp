@@ -862,7 +862,8 @@ void Antiferromagnet::average_quantity(const std::string& q, double* out, hipStr
 std::vector<StateVar> World::state_vars() {
   std::vector<StateVar> v;
   auto add_mag = [&](Ferromagnet* fm) {
-    v.push_back(StateVar{StateVar::MAG, fm, nullptr, fm->m.get(), &fm->x0, &fm->k, fm->N, fm->geom_ptr(), true});
+    v.push_back(
+        StateVar{StateVar::MAG, fm, nullptr, fm->m.get(), &fm->m, &fm->x0, &fm->k, fm->N, fm->geom_ptr(), true});
   };
   if (relaxing_) {
     for (Ferromagnet* fm : relax_fms_) add_mag(fm);
@@ -876,9 +877,9 @@ std::vector<StateVar> World::state_vars() {
   if (relaxing_ && !relax_elastic_) return v;
   auto add_elastic = [&](MagnetCore* own) {
     if (!own->el.enabled) return;
-    v.push_back(StateVar{StateVar::DISP, nullptr, own, own->el.u.get(), &own->el.u0, &own->el.ku, own->N,
+    v.push_back(StateVar{StateVar::DISP, nullptr, own, own->el.u.get(), &own->el.u, &own->el.u0, &own->el.ku, own->N,
                          own->geom_ptr(), false});
-    v.push_back(StateVar{StateVar::VEL, nullptr, own, own->el.v.get(), &own->el.v0, &own->el.kv, own->N,
+    v.push_back(StateVar{StateVar::VEL, nullptr, own, own->el.v.get(), &own->el.v, &own->el.v0, &own->el.kv, own->N,
                          own->geom_ptr(), false});
   };
   for (auto& fm : magnets) add_elastic(fm.get());
@@ -889,7 +890,8 @@ std::vector<StateVar> World::state_vars() {
 // All right-hand sides at the current (stage) state: strain of every magnet
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
-void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray) {
+void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray,
+                         bool disp_ready) {
   for (auto& fm : magnets)
     if (fm->el.enabled) fm->compute_strain(s);
   for (auto& a : afms) {
@@ -910,7 +912,9 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
         fm->eval_rhs(out, s, check_stray, true, mode);
       }
     } else if (sv.kind == StateVar::DISP) {
-      // du/dt = v (elastodynamics.cu:93-96)
+      // du/dt = v (elastodynamics.cu:93-96); a stage's combination may have
+      // written it already
+      if (disp_ready) continue;
       Timed t(&prof, "copy_v", 24.0 * sv.N, s);
       HIP_CHECK(hipMemcpyAsync(out, sv.owner->el.v.get(), (size_t)3 * sv.N * sizeof(float),
                                hipMemcpyDeviceToDevice, s));
@@ -1196,37 +1200,72 @@ void TimeSolver::step() {
   const Tableau& tb = tableau_by_name(method);
   const int S = (int)tb.nodes.size();
   Profiler* prof = &world_->prof;
+  const double t0 = time;
+  bool k0_valid = fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
+  // x0 = the state at the step start.  When the first stage's derivative is
+  // reused, nothing reads the state before the second stage's combination
+  // overwrites it, so the state array and x0 trade buffers instead of copying.
   for (auto& sv : vars) {
     if (sv.x0->n == 0) sv.x0->allocate((size_t)3 * sv.N);
     while ((int)sv.k->size() < S) sv.k->emplace_back((size_t)3 * sv.N);
-    Timed t(prof, "copy_x0", 24.0 * sv.N, s);
-    HIP_CHECK(hipMemcpyAsync(sv.x0->get(), sv.x, (size_t)3 * sv.N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (k0_valid && S > 1) {
+      std::swap(*sv.xa, *sv.x0);
+      sv.x = sv.xa->get();
+    } else {
+      Timed t(prof, "copy_x0", 24.0 * sv.N, s);
+      HIP_CHECK(hipMemcpyAsync(sv.x0->get(), sv.x, (size_t)3 * sv.N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
   }
-  const double t0 = time;
-  bool k0_valid = fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
-  auto combine = [&](const std::vector<double>& w, int nk, double h) {
-    for (auto& sv : vars) {
-      Combo co{};
-      co.nk = nk;
+  // The displacement's derivative at a stage is that stage's velocity: the
+  // velocity's combination writes it into the displacement's k as well
+  // (du/dt = v, elastodynamics.cu:93-96), so no copy is needed.
+  std::vector<int> disp_of(vars.size(), -1);
+  for (size_t i = 0; i < vars.size(); ++i)
+    if (vars[i].kind == StateVar::VEL)
+      for (size_t j = 0; j < vars.size(); ++j)
+        if (vars[j].kind == StateVar::DISP && vars[j].owner == vars[i].owner) disp_of[i] = (int)j;
+  // All variables' combinations in one launch per batch of MAX_JOBS;
+  // `stage` >= 0 names the k slot the velocity also fills.
+  auto combine = [&](const std::vector<double>& w, int nk, double h, int stage) {
+    double bytes = 0.0;
+    std::vector<CombineJob> jobs;
+    for (size_t i = 0; i < vars.size(); ++i) {
+      const StateVar& sv = vars[i];
+      CombineJob jb{};
+      jb.out = sv.x;
+      jb.x0 = sv.x0->get();
+      jb.geom = sv.geom;
+      jb.N = sv.N;
+      jb.normalize = sv.normalize ? 1 : 0;
+      jb.co.nk = nk;
       for (int j = 0; j < nk; ++j) {
-        co.k[j] = (*sv.k)[j].get();
-        co.c[j] = (float)(h * w[j]);
+        jb.co.k[j] = (*sv.k)[j].get();
+        jb.co.c[j] = (float)(h * w[j]);
       }
-      Timed t(prof, "k_combine", 12.0 * sv.N * (nk + 2), s);
-      launch_combine(sv.x, sv.x0->get(), co, 3, sv.N, sv.geom, sv.normalize, s);
+      bytes += 12.0 * sv.N * (nk + 2);
+      if (stage >= 0 && disp_of[i] >= 0) {
+        jb.out2 = (*vars[disp_of[i]].k)[stage].get();
+        bytes += 12.0 * sv.N;
+      }
+      jobs.push_back(jb);
+    }
+    for (size_t j0 = 0; j0 < jobs.size(); j0 += MAX_JOBS) {
+      const int n = (int)std::min<size_t>(MAX_JOBS, jobs.size() - j0);
+      Timed t(prof, "k_combine", bytes * n / jobs.size(), s);
+      launch_combine_batch(jobs.data() + j0, n, s);
     }
   };
   while (true) {
     const double h = dt;
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
-      if (st > 0) combine(tb.A[st], st, h);
+      if (st > 0) combine(tb.A[st], st, h, st);
       if (st == 0 && k0_valid) continue;
-      world_->eval_rhs_all(vars, st, s, true);
+      world_->eval_rhs_all(vars, st, s, true, st > 0);
       nrhs += (long long)vars.size();
     }
     time = t0 + h;
-    if (!tb.fsal) combine(tb.w1, S, h);
+    if (!tb.fsal) combine(tb.w1, S, h, -1);
     ++nsteps;
     if (!adaptive) break;
     double err = 0.0;

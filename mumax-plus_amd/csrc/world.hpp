@@ -198,6 +198,7 @@ struct StateVar {
   Ferromagnet* fm;    // MAG
   MagnetCore* owner;  // DISP / VEL
   float* x;
+  DeviceArray<float>* xa;  // the array holding x
   DeviceArray<float>* x0;
   std::vector<DeviceArray<float>>* k;
   int N;
@@ -264,7 +265,9 @@ class World {
   // (mumaxworld.cpp:159-205), or the relaxation's equations while it runs.
   std::vector<StateVar> state_vars();
   // k[stage] of every variable at the current state (all stage states set).
-  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray);
+  // disp_ready: the displacements' k[stage] (the stage velocity) is written.
+  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray,
+                    bool disp_ready = false);
 
   double relax_threshold = -1.0;  // World.RelaxTorqueThreshold (mumaxworld.cpp:26)
   // Relaxer::exec (relaxer.cpp:87-174) of one magnet, or of the whole world
