@@ -107,11 +107,12 @@ def afm_step_bytes(N, demag_bytes):
     antiferromagnet with elastodynamics: per evaluation the two-sublattice
     demag (demag_bytes, both magnetisations read), strain and stress (read u,
     write 12 floats: 60 B), two field kernels (read m, sister m, h, strain;
-    write the torque: 72 B each), the acceleration (read stress, v, m1, m2;
-    write a: 72 B) and du/dt = v (24 B); per step the stage combinations of
-    the four 3-vector variables m1, m2, u, v (384 B each)."""
-    per_eval = demag_bytes + (60.0 + 2 * 72.0 + 72.0 + 24.0) * N
-    return 6 * per_eval + 4 * 384.0 * N
+    write the torque: 72 B each) and the acceleration (read stress, v, m1,
+    m2; write a: 72 B); per step the stage combinations of the four 3-vector
+    variables m1, m2, u, v (384 B each) and, per stage, the stage velocity
+    written once more as du/dt (12 B)."""
+    per_eval = demag_bytes + (60.0 + 2 * 72.0 + 72.0) * N
+    return 6 * per_eval + 4 * 384.0 * N + 6 * 12.0 * N
 
 
 def run_afm(args):

@@ -79,6 +79,14 @@ struct PV {
   }
 };
 
+// i mod d for i within a few periods of [0, d) (stencil offsets): repeated
+// add / subtract instead of an integer division.
+__host__ __device__ __forceinline__ int wrap_near(int i, int d) {
+  while (i < 0) i += d;
+  while (i >= d) i -= d;
+  return i;
+}
+
 __host__ __device__ __forceinline__ int imod(int i, int d) {
   i %= d;
   return i >= 0 ? i : i + d;

@@ -21,7 +21,7 @@ __device__ __forceinline__ int shifted(const ElDesc& d, int x, int y, int z, int
   const int mo[3] = {d.mox, d.moy, d.moz};
   const int n[3] = {d.nx, d.ny, d.nz};
   int g = o[axis] + c[axis] + k;
-  if (mg[axis] > 0) g = mo[axis] + imod(g - mo[axis], mg[axis]);
+  if (mg[axis] > 0) g = mo[axis] + wrap_near(g - mo[axis], mg[axis]);
   const int l = g - o[axis];
   if (l < 0 || l >= n[axis]) return -1;
   c[axis] = l;
