@@ -95,10 +95,9 @@ int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsi
  * Magnet.enable_as_stray_field_source / _destination / enable_elastodynamics (wrap_magnet.cpp:26-33,
  * Magnet::setEnableElastodynamics magnet.cpp:177-213). flag: "enable_demag", "enable_openbc",
  * "enable_as_stray_field_source", "enable_as_stray_field_destination", "enable_elastodynamics"
- * (not for a sublattice, nor while a rigid strain is set).  Stray fields between magnets are not
- * built: evaluating a field they would enter fails with MPA_ERR_RUNTIME while the source's and
- * the destination's flags are set (a sublattice uses its antiferromagnet's) and the source's
- * msat is not zero. */
+ * (not for a sublattice, nor while a rigid strain is set).  The stray fields of the other magnets
+ * enter a destination's external field (zeeman.cu:70-88; a sublattice uses its antiferromagnet's
+ * destination switch). */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
 /* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,
@@ -136,6 +135,10 @@ int mpa_fm_average(mpa_ferromagnet* m, const char* q, double* out /* ncomp */);
  * inhomogeneous_exchange_energy, total_energy, and for an independent ferromagnet
  * kinetic_energy, elastic_energy (wrap_scalarquantity.cpp, wrap_magnet.cpp:70-73). */
 int mpa_fm_scalar(mpa_ferromagnet* m, const char* q, double* out);
+/* StrayField(magnet, grid).eval with set_method("fft" | "brute") (wrap_strayfield.cpp:10-25,
+ * strayfield.cpp:72-77): the field (tesla) of an independent ferromagnet on the grid (size,
+ * origin), method 0 = FFT, 1 = direct sum; out holds 3 * ncells of the grid. */
+int mpa_fm_stray_field(mpa_ferromagnet* m, const int size[3], const int origin[3], int method, float* out);
 /* Ferromagnet.is_sublattice / hostMagnet (wrap_ferromagnet.cpp:32, ferromagnet.hpp hostMagnet):
  * *out = the antiferromagnet of a sublattice, NULL for an independent ferromagnet. */
 int mpa_fm_host(mpa_ferromagnet* m, mpa_antiferromagnet** out);
@@ -183,6 +186,8 @@ int mpa_afm_eval(mpa_antiferromagnet* a, const char* q, float* out);
 int mpa_afm_average(mpa_antiferromagnet* a, const char* q, double* out);
 /* total_energy (energy.cu:78-92), kinetic_energy, elastic_energy (elasticenergies.cu:40-95). */
 int mpa_afm_scalar(mpa_antiferromagnet* a, const char* q, double* out);
+/* Stray field of the antiferromagnet (Ms1 m1 + Ms2 m2) on a grid, as mpa_fm_stray_field. */
+int mpa_afm_stray_field(mpa_antiferromagnet* a, const int size[3], const int origin[3], int method, float* out);
 
 /* ------------------------------------------------------------ TimeSolver -- */
 

@@ -491,6 +491,16 @@ int mpa_fm_get_variable(mpa_ferromagnet* m, const char* which, float* out) {
   });
 }
 
+int mpa_fm_stray_field(mpa_ferromagnet* m, const int size[3], const int origin[3], int method, float* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    need(size, "size");
+    need(out, "out");
+    const int zero[3] = {0, 0, 0};
+    f->world->stray_field_on_grid(f, size, origin ? origin : zero, method, out);
+  });
+}
+
 // ------------------------------------------------------- Antiferromagnet
 int mpa_afm_name(const mpa_antiferromagnet* a, char* buf, int buflen) {
   return guard([&] { copy_str(AFM(a)->name, buf, buflen); });
@@ -571,6 +581,16 @@ int mpa_afm_get_variable(mpa_antiferromagnet* a, const char* which, float* out) 
     need(which, "variable");
     need(out, "out");
     af->get_variable(which, out, af->world->stream);
+  });
+}
+
+int mpa_afm_stray_field(mpa_antiferromagnet* a, const int size[3], const int origin[3], int method, float* out) {
+  return guard([&] {
+    Antiferromagnet* af = AFM(a);
+    need(size, "size");
+    need(out, "out");
+    const int zero[3] = {0, 0, 0};
+    af->world->stray_field_on_grid(af, size, origin ? origin : zero, method, out);
   });
 }
 

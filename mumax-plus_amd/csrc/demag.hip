@@ -787,6 +787,14 @@ void reference_kernel_size(const DemagGeometry& g, int ksize[3]) {
   }
 }
 
+void newell_grid_device(float* out, const int ksize[3], const int origin[3], const DemagGeometry& g, hipStream_t s) {
+  const long long n = (long long)ksize[0] * ksize[1] * ksize[2];
+  k_newell_grid<<<grid_for(n), BLOCK, 0, s>>>(out, ksize[0], ksize[1], ksize[2], origin[0], origin[1], origin[2],
+                                              g.cs[0], g.cs[1], g.cs[2], g.mastergrid[0], g.mastergrid[1],
+                                              g.mastergrid[2], g.pbc[0], g.pbc[1], g.pbc[2]);
+  checkLaunch("k_newell_grid");
+}
+
 void reference_kernel(const DemagGeometry& g, int ksize[3], float* host_out) {
   reference_kernel_size(g, ksize);
   int origin[3];

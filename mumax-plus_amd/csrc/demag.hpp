@@ -65,5 +65,9 @@ class DemagPlan {
 // device and returned as float.
 void reference_kernel(const DemagGeometry& g, int ksize[3], float* host_out);
 void reference_kernel_size(const DemagGeometry& g, int ksize[3]);
+// The six kernel components N(origin + j) for j on a ksize grid, FP64
+// (with the images of g's periodic axes), stored as float (6, Kz, Ky, Kx) in
+// device memory; g.n is not used.
+void newell_grid_device(float* out, const int ksize[3], const int origin[3], const DemagGeometry& g, hipStream_t s);
 
 }  // namespace mpa

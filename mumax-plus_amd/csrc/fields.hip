@@ -315,7 +315,10 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
     }
   }
   float3 hext = f3(0.f, 0.f, 0.f);
-  if (d.terms & TERM_EXT) hext = f3(d.wb[0], d.wb[1], d.wb[2]) + V(d.bias, idx);
+  if (d.terms & TERM_EXT) {
+    hext = f3(d.wb[0], d.wb[1], d.wb[2]) + V(d.bias, idx);
+    if (d.hstray) hext = hext + load3(d.hstray, idx, N);
+  }
   float3 hdem = f3(0.f, 0.f, 0.f);
   if (d.terms & TERM_DEMAG) hdem = load3(hdemag, idx, N);
 
@@ -486,6 +489,7 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   float3 h = hex;
   h = h + han;
   h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
+  if (d.hstray) h = h + load3(d.hstray, idx, N);
   h = h + hdm;
   if (u.demag) h = h + load3(hdemag, idx, N);
   float3 r;

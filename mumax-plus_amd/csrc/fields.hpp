@@ -48,6 +48,9 @@ struct FMDesc {
   const float* m2;
   PS msat2, afmex_cell, afmex_nn, latcon;
   PS hdmi[D_COUNT];
+  // stray field of the other magnets (3N, tesla), part of the external field
+  // (zeeman.cu:70-88); nullptr when none
+  const float* hstray;
 };
 
 enum FieldMode {

@@ -466,9 +466,9 @@ unsigned Ferromagnet::active_terms() {
   const bool cub0 = (az("kc1") && az("kc2") && az("kc3")) || (az("anisC1") && az("anisC2")) || msat0;
   if (!uni0) t |= TERM_UNI;
   else if (!cub0) t |= TERM_CUBIC;
-  // zeeman.cu:38-51 (stray fields of other magnets are refused, not coupled)
+  // zeeman.cu:46-51: world bias, magnet bias and the other magnets' stray fields
   const bool wb0 = world->bias[0] == 0 && world->bias[1] == 0 && world->bias[2] == 0;
-  if (!(wb0 && az("bias_magnetic_field"))) t |= TERM_EXT;
+  if (!(wb0 && az("bias_magnetic_field")) || owner()->stray_active()) t |= TERM_EXT;
   // dmi.cu:12-17: a sublattice also counts the inter-sublattice tensor
   bool dmi0 = true;
   for (auto k : kDmiKeys) dmi0 = dmi0 && az(k);
@@ -512,6 +512,12 @@ FMDesc Ferromagnet::desc(unsigned terms) {
   for (int c = 0; c < 3; ++c) d.wb[c] = (float)world->bias[c];
   d.openbc = openbc ? 1 : 0;
   d.terms = terms;
+  d.hstray = nullptr;
+  if ((terms & TERM_EXT) && owner()->stray_active()) {
+    MagnetCore* own = owner();
+    if (own->hstray.n == 0) own->hstray.allocate((size_t)3 * own->N);
+    d.hstray = own->hstray.get();
+  }
   d.B1 = params.at("B1").ps();
   d.B2 = params.at("B2").ps();
   if (terms & TERM_ME) {
@@ -564,32 +570,113 @@ void Ferromagnet::prepare(unsigned terms, hipStream_t s, Profiler* prof) {
     else demag().exec(m.get(), params.at("msat").ps(), geom.get(), hdemag.get(), s, prof);
   }
   if ((terms & TERM_ME) && !owner()->rigid_strain_applied()) owner()->compute_strain(s);
+  if (terms & TERM_EXT) owner()->compute_stray(s, prof);
 }
 
-// evalExternalField (zeeman.cu:70-88) adds the stray field of every other
-// magnet that is a source (unless its msat is assured zero,
-// strayfield.cpp:88-95) to a destination magnet.  That coupling is not part
-// of this build, so an evaluation it would change is refused rather than
-// computed without it.  Sublattices receive the stray fields of their host's
-// other magnets.
-void Ferromagnet::check_stray_coupling() const {
-  const MagnetCore* self = host ? static_cast<const MagnetCore*>(host) : static_cast<const MagnetCore*>(this);
-  if (!self->stray_destination) return;
-  auto refuse = [&](const std::string& src) {
-    throw Error(MPA_ERR_RUNTIME, "the stray field of magnet '" + src + "' on magnet '" + self->name +
-                                     "' is not implemented; set enable_as_stray_field_source or "
-                                     "enable_as_stray_field_destination to False");
-  };
-  for (const auto& o : world->magnets)
-    if (o.get() != self && o->stray_source && !o->param("msat").assured_zero()) refuse(o->name);
-  for (const auto& a : world->afms) {
-    const bool zero = a->sub[0]->param("msat").assured_zero() && a->sub[1]->param("msat").assured_zero();
-    if (a.get() != self && a->stray_source && !zero) refuse(a->name);
+// ------------------------------------------------------------ stray fields
+
+bool Ferromagnet::stray_source_of(StraySource& out) {
+  if (host || !stray_source || param("msat").assured_zero()) return false;
+  out = StraySource{m.get(), param("msat").ps(), nullptr, PS{nullptr, 0.f}, geom.get(),
+                    {origin[0], origin[1], origin[2]}, {n[0], n[1], n[2]}};
+  return true;
+}
+
+bool Antiferromagnet::stray_source_of(StraySource& out) {
+  if (!stray_source) return false;
+  if (sub[0]->param("msat").assured_zero() && sub[1]->param("msat").assured_zero()) return false;
+  out = StraySource{sub[0]->m.get(), sub[0]->param("msat").ps(), sub[1]->m.get(), sub[1]->param("msat").ps(),
+                    geom.get(), {origin[0], origin[1], origin[2]}, {n[0], n[1], n[2]}};
+  return true;
+}
+
+// StrayField(magnet, grid).eval (strayfield.cpp:72-77, wrap_strayfield.cpp:12-24):
+// the field of `src` on an arbitrary grid, by the FFT box convolution
+// (method 0) or the direct sum (method 1); zero when the source's msat is
+// assured zero.  The source switches do not apply here.
+void World::stray_field_on_grid(MagnetCore* src, const int size[3], const int org[3], int method, float* host_out) {
+  for (int a = 0; a < 3; ++a)
+    if (size[a] < 1) throw Error(MPA_ERR_INVALID, "grid size should be positive");
+  const size_t nd = (size_t)size[0] * size[1] * size[2];
+  StraySource S{};
+  bool zero;
+  if (auto* fm = dynamic_cast<Ferromagnet*>(src)) {
+    if (fm->host) throw Error(MPA_ERR_INVALID, "the stray field source must be a magnet, not a sublattice");
+    zero = fm->param("msat").assured_zero();
+    S = StraySource{fm->m.get(), fm->param("msat").ps(), nullptr, PS{nullptr, 0.f}, fm->geom.get(),
+                    {fm->origin[0], fm->origin[1], fm->origin[2]}, {fm->n[0], fm->n[1], fm->n[2]}};
+  } else {
+    auto* a = static_cast<Antiferromagnet*>(src);
+    zero = a->sub[0]->param("msat").assured_zero() && a->sub[1]->param("msat").assured_zero();
+    S = StraySource{a->sub[0]->m.get(), a->sub[0]->param("msat").ps(), a->sub[1]->m.get(),
+                    a->sub[1]->param("msat").ps(), a->geom.get(), {a->origin[0], a->origin[1], a->origin[2]},
+                    {a->n[0], a->n[1], a->n[2]}};
+  }
+  if (zero) {
+    std::fill(host_out, host_out + 3 * nd, 0.f);
+    return;
+  }
+  DeviceArray<float> out(3 * nd);
+  StrayTarget t{{org[0], org[1], org[2]}, {size[0], size[1], size[2]}, nullptr, out.get()};
+  if (method == 0) {
+    int bo[3], bn[3];
+    stray_box(&S, 1, t, bo, bn);
+    StrayPlan plan(bo, bn, cellsize, mastergrid_size, pbc);
+    plan.exec(&S, 1, t, false, stream, nullptr);
+  } else if (method == 1) {
+    stray_brute(S, t, cellsize, mastergrid_size, pbc, stream);
+  } else {
+    throw Error(MPA_ERR_INVALID, "Method should be \"fft\" or \"brute\"");
+  }
+  HIP_CHECK(hipMemcpyAsync(host_out, out.get(), 3 * nd * sizeof(float), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// evalExternalField (zeeman.cu:70-88): a destination-enabled magnet (a
+// sublattice uses its antiferromagnet) receives the stray field of every
+// other magnet that is enabled as a source and whose msat is not assured zero
+// (strayFieldsAssuredZero, zeeman.cu:13-34).
+std::vector<std::pair<MagnetCore*, StraySource>> MagnetCore::stray_sources() {
+  std::vector<std::pair<MagnetCore*, StraySource>> v;
+  if (!stray_destination) return v;
+  StraySource src;
+  for (auto& o : world->magnets)
+    if (o.get() != this && o->stray_source_of(src)) v.emplace_back(o.get(), src);
+  for (auto& a : world->afms)
+    if (a.get() != this && a->stray_source_of(src)) v.emplace_back(a.get(), src);
+  return v;
+}
+
+// One convolution per source, as the reference registers one StrayField per
+// (source, destination) pair (mumaxworld.cpp:106-116): the demag convolution
+// of the source's magnetisation on the box that holds the source and this
+// magnet, read back on this magnet's cells and summed.
+void MagnetCore::compute_stray(hipStream_t s, Profiler* prof) {
+  const std::vector<std::pair<MagnetCore*, StraySource>> src = stray_sources();
+  if (src.empty()) return;
+  if (hstray.n == 0) hstray.allocate((size_t)3 * N);
+  StrayTarget t{{origin[0], origin[1], origin[2]}, {n[0], n[1], n[2]}, geom.get(), hstray.get()};
+  for (size_t k = 0; k < src.size(); ++k) {
+    int bo[3], bn[3];
+    stray_box(&src[k].second, 1, t, bo, bn);
+    StrayPlan* plan = nullptr;
+    for (auto& e : stray_plans)
+      if (e.first == src[k].first && e.second->matches(bo, bn)) plan = e.second.get();
+    if (!plan) {
+      for (auto it = stray_plans.begin(); it != stray_plans.end(); ++it)
+        if (it->first == src[k].first) {
+          stray_plans.erase(it);
+          break;
+        }
+      stray_plans.emplace_back(src[k].first,
+                               std::make_unique<StrayPlan>(bo, bn, world->cellsize, world->mastergrid_size, world->pbc));
+      plan = stray_plans.back().second.get();
+    }
+    plan->exec(&src[k].second, 1, t, k > 0, s, prof);
   }
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool check_stray, bool prepared, FieldMode mode) {
-  if (check_stray) check_stray_coupling();
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode) {
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
   if (!prepared) prepare(terms, s, prof);
@@ -675,7 +762,6 @@ const float* Ferromagnet::eval_device(const std::string& q, hipStream_t s, int* 
   }
   *ncomp = qi->ncomp;
   if (qi->mode < 0) return m.get();
-  if (qi->terms & TERM_EXT) check_stray_coupling();
   const unsigned terms = active_terms() & qi->terms;
   prepare(terms, s, nullptr);
   launch_field(desc(terms), m.get(), demag_field(), scratch.get(), (FieldMode)qi->mode, s);
@@ -890,13 +976,15 @@ std::vector<StateVar> World::state_vars() {
 // All right-hand sides at the current (stage) state: strain of every magnet
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
-void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray,
-                         bool disp_ready) {
-  for (auto& fm : magnets)
+void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready) {
+  for (auto& fm : magnets) {
     if (fm->el.enabled) fm->compute_strain(s);
+    fm->compute_stray(s, &prof);
+  }
   for (auto& a : afms) {
     if (a->el.enabled) a->compute_strain(s);
     if ((a->sub[0]->active_terms() | a->sub[1]->active_terms()) & TERM_DEMAG) a->compute_demag(s, &prof);
+    a->compute_stray(s, &prof);
   }
   for (const auto& sv : vars) {
     float* out = (*sv.k)[stage].get();
@@ -904,12 +992,12 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
       Ferromagnet* fm = sv.fm;
       const FieldMode mode = relaxing_ ? MODE_DAMPING : MODE_LLG;
       if (fm->host) {
-        fm->eval_rhs(out, s, check_stray, true, mode);
+        fm->eval_rhs(out, s, true, mode);
       } else {
         const unsigned t = fm->active_terms();
         if (t & TERM_DEMAG)
           fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
-        fm->eval_rhs(out, s, check_stray, true, mode);
+        fm->eval_rhs(out, s, true, mode);
       }
     } else if (sv.kind == StateVar::DISP) {
       // du/dt = v (elastodynamics.cu:93-96); a stage's combination may have
@@ -1159,7 +1247,7 @@ double TimeSolver::sensible_timestep() {
   if (vars.empty()) return 0.0;
   for (auto& sv : vars)
     while ((int)sv.k->size() < 1) sv.k->emplace_back((size_t)3 * sv.N);
-  world_->eval_rhs_all(vars, 0, world_->stream, false);
+  world_->eval_rhs_all(vars, 0, world_->stream);
   nrhs += (long long)vars.size();
   double mx = 0.0;
   for (auto& sv : vars)
@@ -1261,7 +1349,7 @@ void TimeSolver::step() {
       time = t0 + h * tb.nodes[st];
       if (st > 0) combine(tb.A[st], st, h, st);
       if (st == 0 && k0_valid) continue;
-      world_->eval_rhs_all(vars, st, s, true, st > 0);
+      world_->eval_rhs_all(vars, st, s, st > 0);
       nrhs += (long long)vars.size();
     }
     time = t0 + h;
@@ -1435,8 +1523,14 @@ void World::bounding_grid(int size[3], int org[3]) const {
 }
 
 void World::pbc_changed() {
-  for (auto& m : magnets) m->invalidate_demag();
-  for (auto& a : afms) a->invalidate_demag();
+  for (auto& m : magnets) {
+    m->invalidate_demag();
+    m->stray_plans.clear();
+  }
+  for (auto& a : afms) {
+    a->invalidate_demag();
+    a->stray_plans.clear();
+  }
   touch();
 }
 

@@ -19,6 +19,7 @@
 #include "fields.hpp"
 #include "profiler.hpp"
 #include "rk.hpp"
+#include "stray.hpp"
 
 namespace mpa {
 
@@ -79,6 +80,19 @@ class MagnetCore {
   bool stray_source = true;
   bool stray_destination = true;
   DeviceArray<float> qbuf;  // 6N, magnet-level quantity results
+  // Stray field of the other magnets on this one (3N), and one plan per
+  // source magnet.
+  DeviceArray<float> hstray;
+  std::vector<std::pair<const MagnetCore*, std::unique_ptr<StrayPlan>>> stray_plans;
+  // This magnet as a stray-field source (Ms m, or Ms1 m1 + Ms2 m2); false
+  // when it is disabled as a source or its msat is assured zero
+  // (strayfield.cpp:88-95).
+  virtual bool stray_source_of(StraySource& out) = 0;
+  // The other magnets' stray fields enter this magnet's external field
+  // (zeeman.cu:13-34, 70-88).
+  std::vector<std::pair<MagnetCore*, StraySource>> stray_sources();
+  bool stray_active() { return !stray_sources().empty(); }
+  void compute_stray(hipStream_t s, Profiler* prof);
 
   const uint8_t* geom_ptr() const { return geom.get(); }
   Param& param(const std::string& key);
@@ -130,9 +144,6 @@ class Ferromagnet : public MagnetCore {
   MagnetCore* owner();  // the magnet holding the displacement: the host, or this
   unsigned active_terms();  // the reference's "assured zero" rules
   bool magnetoelastic_assured_zero();  // magnetoelasticfield.cu:10-42
-  // Throws when another magnet's stray field would enter this magnet's
-  // external field (not built); called by every evaluation that includes it.
-  void check_stray_coupling() const;
   FMDesc desc(unsigned terms);
   DemagPlan& demag();
   void invalidate_demag() { demag_.reset(); }
@@ -143,8 +154,7 @@ class Ferromagnet : public MagnetCore {
   // out (device, 3N) = torque of the current magnetisation (the LLG torque,
   // or the damping-only torque of a relaxation).  With prepared, the demag
   // field and strain are taken as already computed.
-  void eval_rhs(float* out, hipStream_t s, bool check_stray = true, bool prepared = false,
-                FieldMode mode = MODE_LLG);
+  void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;
   void average_quantity(const std::string& q, double* out, hipStream_t s);
@@ -155,6 +165,7 @@ class Ferromagnet : public MagnetCore {
 
   int num_fm() const override { return 1; }
   Ferromagnet* fm_at(int) override { return this; }
+  bool stray_source_of(StraySource& out) override;
 
   const float* eval_device(const std::string& q, hipStream_t s, int* ncomp);
 
@@ -178,6 +189,7 @@ class Antiferromagnet : public MagnetCore {
 
   int num_fm() const override { return 2; }
   Ferromagnet* fm_at(int i) override { return sub[i].get(); }
+  bool stray_source_of(StraySource& out) override;
 
   // Quantities of the whole antiferromagnet (neel_vector, full_magnetization
   // and the magnet-level elastic ones).
@@ -266,8 +278,10 @@ class World {
   std::vector<StateVar> state_vars();
   // k[stage] of every variable at the current state (all stage states set).
   // disp_ready: the displacements' k[stage] (the stage velocity) is written.
-  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool check_stray,
-                    bool disp_ready = false);
+  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false);
+
+  // Field of `src` on the grid (size, origin); method 0 = FFT, 1 = direct sum.
+  void stray_field_on_grid(MagnetCore* src, const int size[3], const int origin[3], int method, float* host_out);
 
   double relax_threshold = -1.0;  // World.RelaxTorqueThreshold (mumaxworld.cpp:26)
   // Relaxer::exec (relaxer.cpp:87-174) of one magnet, or of the whole world

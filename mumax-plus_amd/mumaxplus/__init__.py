@@ -18,8 +18,9 @@ from .scalarquantity import ScalarQuantity
 from .parameter import Parameter
 from .variable import Variable
 from .dmitensor import DmiTensor, DmiTensorGroup
+from .strayfield import StrayField
 from . import util
 
 __all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
            "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
-           "DmiTensorGroup", "util"]
+           "DmiTensorGroup", "StrayField", "util"]

@@ -96,6 +96,8 @@ _SIGS = {
     "mpa_afm_eval": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_average": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_scalar": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
+    "mpa_fm_stray_field": ([_P, _P, _P, _cint, _P], _cint),
+    "mpa_afm_stray_field": ([_P, _P, _P, _cint, _P], _cint),
     "mpa_world_relax": ([_P, _cdouble], _cint),
     "mpa_fm_relax": ([_P, _cdouble], _cint),
     "mpa_afm_relax": ([_P, _cdouble], _cint),
@@ -647,6 +649,61 @@ class Antiferromagnet(_Magnet):
         if sub == self._subs[1]:
             return self._subs[0]
         raise IndexError("Sublattice not found in Antiferromagnet.")
+
+
+class StrayField:
+    """StrayField(magnet, grid) (wrap_strayfield.cpp:10-25): the field of a
+    magnet on a grid; set_method("fft" | "brute")."""
+
+    _METHODS = {"fft": 0, "brute": 1}
+
+    def __init__(self, magnet, grid, mask=None):
+        if isinstance(magnet, Ferromagnet) and magnet.is_sublattice:
+            raise ValueError("the stray field source must be a magnet, not a sublattice")
+        self._src = magnet
+        self._grid = Grid(grid.size, grid.origin)
+        self._method = 0
+        self._mask = mask  # destination geometry (bool (nz, ny, nx)) or None
+        self.ncomp = 3
+
+    def set_method(self, method):
+        if method not in self._METHODS:
+            raise ValueError('Method should be "fft" or "brute"')
+        self._method = self._METHODS[method]
+
+    @property
+    def name(self):
+        return f"{self._src.name}:stray_field"
+
+    @property
+    def unit(self):
+        return "T"
+
+    @property
+    def grid(self):
+        return self._grid
+
+    @property
+    def system(self):
+        return None
+
+    def eval(self):
+        nx, ny, nz = self._grid.size
+        out = _np.empty((3, nz, ny, nx), dtype=_np.float32)
+        _call(self._src, "stray_field", _i3(*self._grid.size), _i3(*self._grid.origin), self._method, _ptr(out))
+        if self._mask is not None:
+            out = _np.where(self._mask[None], out, _np.float32(0))
+        return out
+
+    def exec(self):
+        self.eval()
+
+    def average(self):
+        v = self.eval().astype(_np.float64)
+        if self._mask is not None:
+            n = max(int(self._mask.sum()), 1)
+            return [float(v[c][self._mask].sum() / n) for c in range(3)]
+        return [float(v[c].mean()) for c in range(3)]
 
 
 def _demag_kernel(fm):

@@ -124,6 +124,15 @@ class Magnet:
     def enable_as_stray_field_destination(self, value):
         self._impl.enable_as_stray_field_destination = value
 
+    def stray_field_from_magnet(self, source_magnet):
+        """The stray field of `source_magnet` on this magnet (zero outside
+        this magnet's geometry)."""
+        from .strayfield import StrayField
+        from . import _mumaxpluscpp as _cpp
+        if source_magnet == self:
+            raise RuntimeError("Can not compute the magnetic field of the magnet")
+        return StrayField._from_impl(_cpp.StrayField(source_magnet._impl, self._impl.grid, mask=self.geometry))
+
     # ---------------------------------------------------- elastodynamics
     @property
     def enable_elastodynamics(self):

@@ -31,8 +31,10 @@ Modules and the reference files each one restates:
   elastodynamics.cu:23-96 do.
 * ``relax``   -- total energies, the relaxer (src/physics/relaxer.cpp:87-174)
   and the energy minimiser (src/physics/minimizer.cu:8-160).
+* ``stray``   -- stray fields between grids by the direct sum
+  (src/physics/strayfieldbrute.cu:15-88).
 
 Pinning status of every function is recorded in each module's docstring.
 """
 
-from . import newell, demag, fields, rk, afm, elastic, model, relax  # noqa: F401
+from . import newell, demag, fields, rk, afm, elastic, model, relax, stray  # noqa: F401

@@ -100,6 +100,9 @@ class Ferromagnet(_Params):
         self.enable_elastodynamics = False
         self.u = np.zeros((3,) + self.shape)
         self.v = np.zeros((3,) + self.shape)
+        # Magnet::enableAsStrayFieldSource / Destination (magnet.cpp:23-24)
+        self.stray_source = True
+        self.stray_destination = True
 
     # ----------------------------------------------------------- parameters
     def set(self, name, value):
@@ -201,6 +204,10 @@ class Antiferromagnet(_Params):
         self.u = np.zeros((3,) + self.shape)
         self.v = np.zeros((3,) + self.shape)
         self._plan = None
+        self.origin = self.sub1.origin
+        self.size = self.sub1.size
+        self.stray_source = True
+        self.stray_destination = True
 
     @property
     def sublattices(self):
@@ -255,8 +262,42 @@ class World:
             self.solver.equations_changed()
         return afm
 
+    # --------------------------------------------------------- stray fields
+    def _source_M(self, src, m_of):
+        """Ms m of a source magnet (both sublattices for an antiferromagnet),
+        zero outside its geometry; None when its msat is assured zero
+        (strayfield.cpp:88-95)."""
+        if isinstance(src, Antiferromagnet):
+            if src.sub1.assured_zero("msat") and src.sub2.assured_zero("msat"):
+                return None
+            M = src.sub1.val("msat") * m_of(src.sub1) + src.sub2.val("msat") * m_of(src.sub2)
+        else:
+            if src.assured_zero("msat"):
+                return None
+            M = src.val("msat") * m_of(src)
+        return np.where(src.geom_mask()[None], M, 0.0)
+
+    def stray_field_on(self, dst, m_of=None):
+        """Sum of the other magnets' stray fields on magnet `dst` (an
+        independent ferromagnet or an antiferromagnet), or None when no
+        source acts (zeeman.cu:13-34, 70-88)."""
+        from . import stray as _stray
+        m_of = m_of or (lambda f: f.m)
+        if not dst.stray_destination:
+            return None
+        H = None
+        for src in self.magnets + self.afms:
+            if src is dst or not src.stray_source:
+                continue
+            M = self._source_M(src, m_of)
+            if M is None:
+                continue
+            h = _stray.stray_field(M, src.origin, dst.size, dst.origin, self.cellsize, self.mastergrid, self.pbc)
+            H = h if H is None else H + h
+        return H
+
     # -------------------------------------------------------------- physics
-    def effective_field(self, mag, m, m2=None, u=None):
+    def effective_field(self, mag, m, m2=None, u=None, m_of=None):
         """evalEffectiveField (effectivefield.cpp:15-28) of a ferromagnet or a
         sublattice; m2 is the sister sublattice's magnetisation, u the
         displacement of the magnet that owns the elastic state."""
@@ -266,8 +307,11 @@ class World:
         else:
             h = _fields.exchange_field(mag, m)
         h = h + _fields.anisotropy_field(mag, m)
-        if not _fields.external_assured_zero(mag, self.bias):
+        hs = self.stray_field_on(self._owner(mag), m_of)
+        if not _fields.external_assured_zero(mag, self.bias) or hs is not None:
             h = h + _fields.external_field(mag, m, self.bias)
+            if hs is not None:
+                h = h + np.where(mag.geom_mask()[None], hs, 0.0)
         if host is None:
             h = h + _fields.dmi_field(mag, m)
         else:
@@ -381,7 +425,7 @@ class World:
                 m2 = None
                 if obj.host is not None:
                     m2 = m_of(obj.host.sub2 if obj is obj.host.sub1 else obj.host.sub1)
-                h = self.effective_field(obj, m_of(obj), m2, uv_of(own, "u"))
+                h = self.effective_field(obj, m_of(obj), m2, uv_of(own, "u"), m_of)
                 torque = _fields.damping_torque if self.relax_fms is not None else _fields.llg_torque
                 out.append(torque(obj, m_of(obj), h))
             elif kind == "u":

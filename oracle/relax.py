@@ -51,8 +51,12 @@ def total_energy_density(world, fm):
     else:
         add(_fields.exchange_field(fm, m), 0.5)
     e = e + _fields.anisotropy_energy_density(fm, m)
-    if not _fields.external_assured_zero(fm, world.bias):
-        add(_fields.external_field(fm, m, world.bias), 1.0)
+    hs = world.stray_field_on(own)
+    if not _fields.external_assured_zero(fm, world.bias) or hs is not None:
+        h_ext = _fields.external_field(fm, m, world.bias)
+        if hs is not None:
+            h_ext = h_ext + np.where(fm.geom_mask()[None], hs, 0.0)
+        add(h_ext, 1.0)
     add(_fields.dmi_field(fm, m) if fm.host is None else _afm.dmi_field_afm(fm, m, m2), 0.5)
     if fm.enable_demag and not fm.assured_zero("msat"):
         if fm.host is None:

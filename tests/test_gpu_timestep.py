@@ -168,27 +168,3 @@ def test_film128x32_fixed_step_matches_fp64_fixture():
         errs.append(float(np.abs(np.array(mag.magnetization.average()) - want).max()))
     assert max(errs) < 1e-4, errs
     assert abs(ts.time - float(g["time"])) <= 1e-9 * float(g["time"])
-
-
-def test_stray_field_coupling_is_refused():
-    """Two magnets with non-zero msat would couple through their stray fields
-    (zeeman.cu:70-88); that is not built, so evaluations that include the
-    external field raise instead of silently leaving it out."""
-    world = mp.World(cellsize=(2e-9, 2e-9, 2e-9))
-    a = mp.Ferromagnet(world, mp.Grid((4, 4, 1)))
-    b = mp.Ferromagnet(world, mp.Grid((4, 4, 1), origin=(6, 0, 0)))
-    for mag in (a, b):
-        mag.msat = 8e5
-        mag.aex = 1e-11
-    with pytest.raises(RuntimeError):
-        a.effective_field.eval()
-    with pytest.raises(RuntimeError):
-        world.timesolver.steps(1)
-    a.exchange_field.eval()  # does not involve the external field
-    b.enable_as_stray_field_source = False
-    a.effective_field.eval()
-    with pytest.raises(RuntimeError):
-        b.effective_field.eval()  # a still acts on b
-    b.enable_as_stray_field_destination = False
-    world.timesolver.steps(2)
-    assert a.enable_as_stray_field_source and not b.enable_as_stray_field_destination
