@@ -67,3 +67,24 @@ def test_far_field_dipole():
     mv = M[:, 0, 0, 0]
     want = MU0 * V / (4 * np.pi * rn ** 3) * (3 * np.dot(mv, rh) * rh - mv)
     assert np.abs(h - want).max() <= 1e-3 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("pbc", [None, (1, 1, 0)])
+def test_mutual_energy_is_reciprocal(pbc):
+    """Two magnets of different sizes and origins: sum M1 . B21 over magnet 1
+    equals sum M2 . B12 over magnet 2 (the mutual energy of two magnets does
+    not depend on which one is the source); a misplaced destination origin or
+    crop breaks the equality."""
+    rng = np.random.default_rng(3)
+    n1, o1 = (5, 3, 2), (0, 0, 0)
+    n2, o2 = (3, 4, 1), (7, 1, 3)
+    M1 = _M(rng, (n1[2], n1[1], n1[0]))
+    M2 = _M(rng, (n2[2], n2[1], n2[0]))
+    mg, reps = (0, 0, 0), (0, 0, 0)
+    if pbc is not None:
+        mg, reps = (12, 8, 0), pbc
+    b21 = ost.stray_field(M2, o2, n1, o1, CS, mg, reps)
+    b12 = ost.stray_field(M1, o1, n2, o2, CS, mg, reps)
+    e1, e2 = np.sum(M1 * b21), np.sum(M2 * b12)
+    assert abs(e1 - e2) <= 1e-10 * max(np.sum(np.abs(M1 * b21)), np.sum(np.abs(M2 * b12)))
+    assert abs(e1) > 1e-3 * np.sum(np.abs(M1 * b21))
