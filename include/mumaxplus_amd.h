@@ -258,6 +258,41 @@ int mpa_world_profile_reset(mpa_world* w);
 int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);
 
+/* ---------------------------------------------------------------- Batch -- */
+
+/* Many independent small ferromagnets stepped together (no counterpart in the reference, where
+ * each World is stepped on its own by TimeSolver::steps / run, timesolver.cpp:37-111; for callers
+ * that run many small solves: parameter sweeps, ensembles, a request server).  A member is a
+ * ferromagnet alone in its world, with uniform parameters and no elastodynamics; it keeps its own
+ * step size, time, step controller and stopping rule.  Its state lives in the batch's packed
+ * device arrays from mpa_batch_add until mpa_batch_remove, which copies it back into the magnet
+ * and its world's time solver.  A member's world must outlive its membership.
+ * capacity: packed cells (members occupy their cell count rounded up to 256); method: a tableau
+ * whose last stage is the new state ("DormandPrince" or "BogackiShampine"). */
+typedef struct mpa_batch mpa_batch;
+int mpa_batch_create(long long capacity, const char* method, mpa_batch** out);
+int mpa_batch_destroy(mpa_batch* b);
+/* mode 0: `arg` accepted steps; 1: run for `arg` seconds (TimeSolver::run); 2: damping-only
+ * dynamics until the max damping torque <= `arg` (1/s).  The member starts with its world's
+ * solver settings (timestep, adaptive_timestep, max_error, headroom, bounds, sensible factor).
+ * *id = -1 when its cells do not fit in the free capacity now. */
+int mpa_batch_add(mpa_batch* b, mpa_ferromagnet* m, int mode, double arg, int* id);
+int mpa_batch_remove(mpa_batch* b, int id);
+/* n batch steps: one step attempt of every active member each. */
+int mpa_batch_advance(mpa_batch* b, long long n);
+/* Members whose stopping rule has been met (at most `cap` ids written; *count = all). */
+int mpa_batch_finished(mpa_batch* b, int cap, int* ids, int* count);
+int mpa_batch_member(mpa_batch* b, int id, double* time, double* dt, long long* steps, long long* rejected,
+                     int* done, float* last_error, float* last_torque);
+/* capacity, free cells, members, algorithmic HBM bytes and kernel launches of one batch step. */
+int mpa_batch_info(mpa_batch* b, long long* capacity, long long* free_cells, int* members, double* step_bytes,
+                   int* launches_per_step);
+/* "graph": replay a captured HIP graph per step (default 1); "profiling": per-kernel HIP-event
+ * timing (steps then run without the graph). */
+int mpa_batch_set_option(mpa_batch* b, const char* key, int value);
+int mpa_batch_profile_report(mpa_batch* b, int cap, char* names, double* total_ms, long long* launches,
+                             double* bytes, int* count);
+
 #ifdef __cplusplus
 }
 #endif

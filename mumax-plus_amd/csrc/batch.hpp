@@ -1,0 +1,115 @@
+// Many small independent ferromagnets stepped together.
+//
+// The reference steps one World at a time (TimeSolver::steps / run,
+// src/core/timesolver.cpp:37-111, one kernel launch per term per magnet).
+// Here the members' magnetisations are packed into one SoA state of
+// capacity `cs` cells (component c of cell i of a member at c * cs + off + i,
+// every member starting on a multiple of 256 cells), so a Runge-Kutta stage
+// of every member is a handful of launches: one stage combination, the
+// batched demag passes (one launch per pass and FFT line length) and one
+// batched field kernel.  Each member keeps its own step size, time,
+// controller and stopping rule in device memory; the step controller, the
+// accept / reject decision and the stopping rules run on the device, so the
+// host only launches steps (a captured HIP graph once the membership is
+// fixed) and polls for finished members now and then.
+//
+// A member's arithmetic does not depend on the other members: its field
+// comes from its own cells and demag plan, its error norm and controller are
+// its own.  Its trajectory is therefore the same alone and in any mix.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "demag.hpp"
+#include "fields.hpp"
+#include "profiler.hpp"
+
+namespace mpa {
+
+class Ferromagnet;
+
+enum BatchMode {
+  BATCH_STEPS = 0,  // a fixed number of accepted steps
+  BATCH_RUN = 1,    // TimeSolver::run(duration): land on time + duration
+  BATCH_RELAX = 2,  // damping-only dynamics until the max torque <= threshold
+};
+
+// Per-member state in device memory.
+struct BatchMember {
+  double time, dt, stop, dt_saved;
+  double max_error, headroom, lower, upper, sfactor, sdefault;
+  long long steps, target, rejected;
+  float threshold;
+  float last_error, last_torque;
+  int mode, adaptive, active, accept, last, done;
+};
+
+class Batch {
+ public:
+  // capacity: packed cells (rounded up to 256); method: a tableau whose last
+  // stage is the new state (DormandPrince or BogackiShampine).
+  Batch(long long capacity, const std::string& method);
+  ~Batch();
+
+  // Admits a ferromagnet (its magnetisation, parameters, geometry and its
+  // world's solver settings: dt, adaptive, max_error, headroom, bounds).
+  // Returns the member id, or -1 when its cells do not fit now.
+  // arg: steps (BATCH_STEPS), duration (BATCH_RUN), torque threshold
+  // (BATCH_RELAX).  The member's world must hold only this magnet.
+  int add(Ferromagnet* fm, int mode, double arg);
+  // Copies the member's state back into its magnet and its world's solver
+  // (time, dt, step counts) and frees its cells.
+  void remove(int id);
+  // n batch steps (one attempt per active member each).
+  void advance(long long n);
+  // Active members' ids, and those finished since they were admitted.
+  std::vector<int> finished();
+  int nmembers() const;
+  long long free_cells() const;
+  long long capacity() const { return cs_; }
+  const BatchMember& member_state(int id);  // synchronises
+  // Algorithmic HBM bytes of one batch step (all active members).
+  double step_bytes();
+  int launches_per_step();
+
+  bool use_graph = true;
+  hipStream_t stream = nullptr;
+  Profiler prof;
+
+ private:
+  struct Slot {
+    Ferromagnet* fm = nullptr;
+    long long off = 0, ncell = 0, span = 0;
+    int mode = 0;
+    bool demag = false;
+    bool used = false;
+  };
+  void rebuild();
+  void prime(const std::vector<int>& fresh);
+  void record_step();
+  void eval_rhs(float* out);
+  void drop_graph();
+  long long alloc_range(long long span);
+  void free_range(long long off, long long span);
+
+  std::string method_;
+  long long cs_;
+  std::vector<Slot> slots_;
+  std::vector<std::pair<long long, long long>> free_;  // (off, span), sorted
+  DeviceArray<float> X_, X0_, HD_;
+  std::vector<DeviceArray<float>> K_;
+  DeviceArray<uint8_t> G_;
+  DeviceArray<int> c2m_;  // 256-cell chunk -> member id, -1 when free
+  DeviceArray<BatchMember> st_;
+  DeviceArray<unsigned> errbits_, tqbits_;
+  std::vector<int> c2m_host_;
+  std::unique_ptr<DemagBatch> demag_;
+  std::unique_ptr<FieldBatch> field_;
+  bool dirty_ = true;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+};
+
+}  // namespace mpa

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "world.hpp"
 
 using namespace mpa;
@@ -15,6 +16,10 @@ struct mpa_world {
 };
 struct mpa_ferromagnet {};      // opaque: the pointer is the Ferromagnet itself
 struct mpa_antiferromagnet {};  // opaque: the pointer is the Antiferromagnet itself
+struct mpa_batch {
+  Batch b;
+  mpa_batch(long long cap, const std::string& method) : b(cap, method) {}
+};
 struct mpa_demag_plan {
   DemagPlan plan;
   explicit mpa_demag_plan(const DemagGeometry& g) : plan(g) {}
@@ -856,6 +861,116 @@ int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_m
     w->w.prof.flush_sync(w->w.stream);
     int i = 0;
     for (const auto& kv : w->w.prof.stats) {
+      if (i < cap) {
+        if (names) copy_str(kv.first, names + 32 * i, 32);
+        if (total_ms) total_ms[i] = kv.second.total_ms;
+        if (launches) launches[i] = kv.second.count;
+        if (bytes) bytes[i] = kv.second.bytes;
+      }
+      ++i;
+    }
+    *count = i;
+  });
+}
+
+int mpa_batch_create(long long capacity, const char* method, mpa_batch** out) {
+  return guard([&] {
+    need(method, "method");
+    need(out, "out");
+    *out = nullptr;
+    *out = new mpa_batch(capacity, method);
+  });
+}
+
+int mpa_batch_destroy(mpa_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int mpa_batch_add(mpa_batch* b, mpa_ferromagnet* m, int mode, double arg, int* id) {
+  return guard([&] {
+    need(b, "batch");
+    need(id, "id");
+    *id = b->b.add(FM(m), mode, arg);
+  });
+}
+
+int mpa_batch_remove(mpa_batch* b, int id) {
+  return guard([&] {
+    need(b, "batch");
+    b->b.remove(id);
+  });
+}
+
+int mpa_batch_advance(mpa_batch* b, long long n) {
+  return guard([&] {
+    need(b, "batch");
+    b->b.advance(n);
+  });
+}
+
+int mpa_batch_finished(mpa_batch* b, int cap, int* ids, int* count) {
+  return guard([&] {
+    need(b, "batch");
+    need(count, "count");
+    const std::vector<int> f = b->b.finished();
+    for (int i = 0; i < (int)f.size() && i < cap; ++i)
+      if (ids) ids[i] = f[(size_t)i];
+    *count = (int)f.size();
+  });
+}
+
+int mpa_batch_member(mpa_batch* b, int id, double* time, double* dt, long long* steps, long long* rejected,
+                     int* done, float* last_error, float* last_torque) {
+  return guard([&] {
+    need(b, "batch");
+    const BatchMember& S = b->b.member_state(id);
+    if (time) *time = S.time;
+    if (dt) *dt = S.dt;
+    if (steps) *steps = S.steps;
+    if (rejected) *rejected = S.rejected;
+    if (done) *done = S.done;
+    if (last_error) *last_error = S.last_error;
+    if (last_torque) *last_torque = S.last_torque;
+  });
+}
+
+int mpa_batch_info(mpa_batch* b, long long* capacity, long long* free_cells, int* members, double* step_bytes,
+                   int* launches_per_step) {
+  return guard([&] {
+    need(b, "batch");
+    if (capacity) *capacity = b->b.capacity();
+    if (free_cells) *free_cells = b->b.free_cells();
+    if (members) *members = b->b.nmembers();
+    if (step_bytes) *step_bytes = b->b.nmembers() ? b->b.step_bytes() : 0.0;
+    if (launches_per_step) *launches_per_step = b->b.nmembers() ? b->b.launches_per_step() : 0;
+  });
+}
+
+int mpa_batch_set_option(mpa_batch* b, const char* key, int value) {
+  return guard([&] {
+    need(b, "batch");
+    need(key, "key");
+    const std::string k = key;
+    if (k == "graph") {
+      b->b.use_graph = value != 0;
+    } else if (k == "profiling") {
+      b->b.prof.flush_sync(b->b.stream);
+      b->b.prof.enabled = value != 0;
+      if (value) b->b.prof.reset();
+    } else {
+      throw Error(MPA_ERR_INVALID, "unknown batch option '" + k + "'");
+    }
+  });
+}
+
+int mpa_batch_profile_report(mpa_batch* b, int cap, char* names, double* total_ms, long long* launches,
+                             double* bytes, int* count) {
+  return guard([&] {
+    need(b, "batch");
+    need(count, "count");
+    b->b.prof.flush_sync(b->b.stream);
+    int i = 0;
+    for (const auto& kv : b->b.prof.stats) {
       if (i < cap) {
         if (names) copy_str(kv.first, names + 32 * i, 32);
         if (total_ms) total_ms[i] = kv.second.total_ms;

@@ -63,7 +63,23 @@ struct DD {
   int Kx, Ky, Kz;
   int Hx, Hy, Hz;
   int l2x, l2y, l2z;
-  int nlines;  // 3*ny*nz real x-lines
+  int nlines;    // 3*ny*nz real x-lines
+  long long cs;  // component stride of m and h (N, or the packed length of a batch)
+};
+
+// One magnet of a batched launch: its sizes and buffers, where its cells
+// start in the packed state (component stride d.cs), and the first block of
+// the launch that works on it.
+struct BSeg {
+  DD d;
+  long long off;
+  PS msat;
+  const uint8_t* geom;  // the magnet's own mask (cell 0 at its first cell) or nullptr
+  float2* A;
+  float2* B;
+  const float* T;
+  const float2* tw[3];
+  int first;
 };
 
 // Thread layout of one line of 2^L2 points (L2 == 0: a single point).
@@ -109,24 +125,25 @@ __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
 }
 
 template <int L2>
-__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
-    k_xfwd(DD d, const float* __restrict__ m, PS msat, const float* __restrict__ m2, PS msat2,
-           const uint8_t* __restrict__ geom, float2* __restrict__ A, const float2* __restrict__ tw) {
+__device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__ m, PS msat,
+                                          const float* __restrict__ m2, PS msat2, const uint8_t* __restrict__ geom,
+                                          float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = XfwdCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long abase[2 * K::PB];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int l0 = 2 * K::PB * blockIdx.x;
+  const int l0 = 2 * K::PB * blk;
   for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
   const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
   // M = Ms m, or Ms1 m1 + Ms2 m2 for the two sublattices of an antiferromagnet
   // (k_pad, strayfieldfft.cu:51-73)
-  auto M = [&](int line, int cell, int x) {
-    float v = msat.at(cell) * m[(size_t)line * d.nx + x];
-    if (m2) v += msat2.at(cell) * m2[(size_t)line * d.nx + x];
+  auto M = [&](int line, int cell) {
+    const size_t i = (size_t)(line / nyz) * d.cs + cell;
+    float v = msat.at(cell) * m[i];
+    if (m2) v += msat2.at(cell) * m2[i];
     return v;
   };
   float2 v[1][8];
@@ -136,8 +153,8 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   for (int q = 0; q < C::EH; ++q) {
     const int x = t + C::T * q;
     if (x < d.nx) {
-      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = M(la, cella + x, x);
-      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = M(lb, cellb + x, x);
+      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = M(la, cella + x);
+      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = M(lb, cellb + x);
     }
   }
   float2* line = lds + g * K::LP;
@@ -158,6 +175,23 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
     else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
     A[base + (long long)kx * d.ny] = X;
   }
+}
+
+template <int L2>
+__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
+    k_xfwd(DD d, const float* __restrict__ m, PS msat, const float* __restrict__ m2, PS msat2,
+           const uint8_t* __restrict__ geom, float2* __restrict__ A, const float2* __restrict__ tw) {
+  xfwd_body<L2>(d, m, msat, m2, msat2, geom, A, tw, blockIdx.x);
+}
+
+// Batched launches: block -> magnet through b2s, then the magnet's own
+// block index.
+template <int L2>
+__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
+    k_xfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s, const float* __restrict__ X) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  xfwd_body<L2>(s.d, X + s.off, s.msat, nullptr, PS{nullptr, 0.f}, s.geom, s.A, s.tw[0], b - s.first);
 }
 
 // ---------------------------------------------------------------- y forward / inverse (3-D)
@@ -188,15 +222,15 @@ __device__ __forceinline__ size_t a_yline(const DD& d, int q) {
 }
 
 template <int L2>
-__global__ void __launch_bounds__(YCfg<L2>::THREADS)
-    k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B, const float2* __restrict__ tw) {
+__device__ __forceinline__ void yfwd_body(const DD& d, const float2* __restrict__ A, float2* __restrict__ B,
+                                          const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = YCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int q0 = xcd_remap(blockIdx.x, gridDim.x) * K::LB;
+  const int q0 = blk * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   const int ql = q0 + g;
   float2 v[1][8];
@@ -225,13 +259,27 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
 
 template <int L2>
 __global__ void __launch_bounds__(YCfg<L2>::THREADS)
-    k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A, const float2* __restrict__ tw) {
+    k_yfwd(DD d, const float2* __restrict__ A, float2* __restrict__ B, const float2* __restrict__ tw) {
+  yfwd_body<L2>(d, A, B, tw, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int L2>
+__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+    k_yfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  yfwd_body<L2>(s.d, s.A, s.B, s.tw[1], b - s.first);
+}
+
+template <int L2>
+__device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict__ B, float2* __restrict__ A,
+                                          const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = YCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
-  const int q0 = xcd_remap(blockIdx.x, gridDim.x) * K::LB;
+  const int q0 = blk * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   __syncthreads();
   for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
@@ -256,6 +304,20 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
       if (y < d.ny) dst[y] = v[0][q];
     }
   }
+}
+
+template <int L2>
+__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+    k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A, const float2* __restrict__ tw) {
+  yinv_body<L2>(d, B, A, tw, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int L2>
+__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+    k_yinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  yinv_body<L2>(s.d, s.B, s.A, s.tw[1], b - s.first);
 }
 
 // ---------------------------------------------------------------- kernel multiply
@@ -303,15 +365,15 @@ struct PointCfg {
 // the mirror points ky and Ky - ky (0 pairs with Ky/2), which read the same
 // row of the octant spectrum, so each row comes from HBM once.
 template <int L2>
-__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
-    k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+__device__ __forceinline__ void zpass_body(const DD& d, float2* __restrict__ B, const float* __restrict__ Tk,
+                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
   const int npt = d.Hx * d.Ky;
   const size_t cs = (size_t)npt;  // points per component
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int pt = blockIdx.x * K::PT + g;  // pair-ordered point index
+  const int pt = blk * K::PT + g;  // pair-ordered point index
   const bool valid = pt < npt;
   const int kx = pt >> d.l2y, r = pt & (d.Ky - 1), kh = r >> 1;
   const int ky = (r & 1) ? (kh == 0 ? d.Ky >> 1 : d.Ky - kh) : kh;
@@ -356,15 +418,29 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   }
 }
 
-// Single layer: forward y, kernel multiply, inverse y; PT points kx per block.
 template <int L2>
 __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
-    k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+    k_zpass(DD d, float2* __restrict__ B, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+  zpass_body<L2>(d, B, Tk, tw, blockIdx.x);
+}
+
+template <int L2>
+__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
+    k_zpass_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  zpass_body<L2>(s.d, s.B, s.T, s.tw[2], b - s.first);
+}
+
+// Single layer: forward y, kernel multiply, inverse y; PT points kx per block.
+template <int L2>
+__device__ __forceinline__ void ypass2d_body(const DD& d, float2* __restrict__ A, const float* __restrict__ Tk,
+                                             const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
-  const int kx = blockIdx.x * K::PT + g;
+  const int kx = blk * K::PT + g;
   const bool valid = kx < d.Hx;
   float2 v[3][8];
 #pragma unroll
@@ -403,6 +479,20 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   }
 }
 
+template <int L2>
+__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
+    k_ypass2d(DD d, float2* __restrict__ A, const float* __restrict__ Tk, const float2* __restrict__ tw) {
+  ypass2d_body<L2>(d, A, Tk, tw, blockIdx.x);
+}
+
+template <int L2>
+__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
+    k_ypass2d_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  ypass2d_body<L2>(s.d, s.A, s.T, s.tw[1], b - s.first);
+}
+
 // ---------------------------------------------------------------- x inverse
 
 template <int L2>
@@ -422,9 +512,8 @@ struct XinvCfg {
 
 // PB line pairs per block; C2R of two lines at a time.
 template <int L2>
-__global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
-    k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom, float* __restrict__ h,
-           const float2* __restrict__ tw) {
+__device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict__ A, const uint8_t* __restrict__ geom,
+                                          float* __restrict__ h, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   using K = XinvCfg<L2>;
   extern __shared__ float2 lds[];
@@ -433,7 +522,7 @@ __global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
   constexpr int NL = 2 * K::PB;
   float2* X = lds;
   float2* Z = lds;  // reused once the half spectra are in registers
-  const int l0 = 2 * xcd_remap(blockIdx.x, gridDim.x) * K::PB;
+  const int l0 = 2 * blk * K::PB;
   for (int i = threadIdx.x; i < NL; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   __syncthreads();
   for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
@@ -470,9 +559,24 @@ __global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
   for (int q = 0; q < C::EH; ++q) {
     const int x = t + C::T * q;
     if (x >= d.nx) continue;
-    if (la < d.nlines) h[(size_t)la * d.nx + x] = (geom && !geom[cella + x]) ? 0.f : v[0][q].x;
-    if (lb < d.nlines) h[(size_t)lb * d.nx + x] = (geom && !geom[cellb + x]) ? 0.f : v[0][q].y;
+    if (la < d.nlines) h[(size_t)(la / nyz) * d.cs + cella + x] = (geom && !geom[cella + x]) ? 0.f : v[0][q].x;
+    if (lb < d.nlines) h[(size_t)(lb / nyz) * d.cs + cellb + x] = (geom && !geom[cellb + x]) ? 0.f : v[0][q].y;
   }
+}
+
+template <int L2>
+__global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
+    k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom, float* __restrict__ h,
+           const float2* __restrict__ tw) {
+  xinv_body<L2>(d, A, geom, h, tw, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int L2>
+__global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
+    k_xinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s, float* __restrict__ H) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  xinv_body<L2>(s.d, s.A, s.geom, H + s.off, s.tw[0], b - s.first);
 }
 
 // ------------------------------------------------ kernel spectrum (FP64)
@@ -632,6 +736,36 @@ struct RunXinv {
   }
 };
 
+// Batched pass launches and their block counts per magnet.
+template <int L2>
+struct RunBatch {
+  static void run(int pass, unsigned nb, const BSeg* segs, const int* b2s, const float* X, float* H, hipStream_t s) {
+    switch (pass) {
+      case 0: k_xfwd_b<L2><<<nb, XfwdCfg<L2>::THREADS, XfwdCfg<L2>::SHM, s>>>(segs, b2s, X); break;
+      case 1: k_yfwd_b<L2><<<nb, YCfg<L2>::THREADS, YCfg<L2>::SHM, s>>>(segs, b2s); break;
+      case 2: k_zpass_b<L2><<<nb, PointCfg<L2>::THREADS, PointCfg<L2>::SHM, s>>>(segs, b2s); break;
+      case 3: k_yinv_b<L2><<<nb, YCfg<L2>::THREADS, YCfg<L2>::SHM, s>>>(segs, b2s); break;
+      case 4: k_ypass2d_b<L2><<<nb, PointCfg<L2>::THREADS, PointCfg<L2>::SHM, s>>>(segs, b2s); break;
+      default: k_xinv_b<L2><<<nb, XinvCfg<L2>::THREADS, XinvCfg<L2>::SHM, s>>>(segs, b2s, H); break;
+    }
+    checkLaunch("batched demag pass");
+  }
+};
+
+template <int L2>
+struct BlocksOf {
+  static void run(int pass, const DD& d, unsigned* out) {
+    switch (pass) {
+      case 0: *out = grid_for((d.nlines + 1) / 2, XfwdCfg<L2>::PB); break;
+      case 1:
+      case 3: *out = grid_for(3LL * d.Hx * d.nz, YCfg<L2>::LB); break;
+      case 2: *out = grid_for((long long)d.Hx * d.Ky, PointCfg<L2>::PT); break;
+      case 4: *out = grid_for(d.Hx, PointCfg<L2>::PT); break;
+      default: *out = grid_for((d.nlines + 1) / 2, XinvCfg<L2>::PB); break;
+    }
+  }
+};
+
 // Calls R<l2>::run(args...) for a supported line length 2^l2.
 template <template <int> class R, typename... Args>
 void with_line(int l2, Args&&... args) {
@@ -718,6 +852,7 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
   d.Hx = Hx_; d.Hy = Hy_; d.Hz = Hz_;
   d.l2x = log2K_[0]; d.l2y = log2K_[1]; d.l2z = log2K_[2];
   d.nlines = 3 * d.ny * d.nz;
+  d.cs = (long long)d.nx * d.ny * d.nz;
   {
     // the second sublattice's magnetisation is one more read of 12 B per cell
     Timed t(prof, "k_xfwd", pass_bytes(0) + (m2 ? 12.0 * d.nx * d.ny * d.nz : 0.0), s);
@@ -744,6 +879,85 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
     Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
     with_line<RunXinv>(d.l2x, d, A_.get(), geom, h, tw_[0].get(), s);
   }
+}
+
+DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
+  const int M = (int)members.size();
+  std::vector<DD> dd(M);
+  for (int j = 0; j < M; ++j) {
+    const DemagPlan& p = *members[j].plan;
+    DD& d = dd[j];
+    d.nx = p.g_.n[0]; d.ny = p.g_.n[1]; d.nz = p.g_.n[2];
+    d.Kx = p.K_[0]; d.Ky = p.K_[1]; d.Kz = p.K_[2];
+    d.Hx = p.Hx_; d.Hy = p.Hy_; d.Hz = p.Hz_;
+    d.l2x = p.log2K_[0]; d.l2y = p.log2K_[1]; d.l2z = p.log2K_[2];
+    d.nlines = 3 * d.ny * d.nz;
+    d.cs = cs;
+  }
+  // pass -> (applies to a member, line length of its axis, plan pass index)
+  auto applies = [&](int pass, int j) {
+    const bool flat = members[j].plan->is2d();
+    return pass == 0 || pass == 5 || (pass == 4 ? flat : !flat);
+  };
+  auto l2_of = [&](int pass, int j) {
+    const DD& d = dd[j];
+    return (pass == 0 || pass == 5) ? d.l2x : (pass == 2 ? d.l2z : d.l2y);
+  };
+  auto plan_pass = [&](int pass, int j) {
+    if (members[j].plan->is2d()) return pass == 0 ? 0 : (pass == 4 ? 1 : 2);
+    return pass == 5 ? 4 : pass;
+  };
+  for (int pass = 0; pass < 6; ++pass) {
+    for (int l2 = 0; l2 <= MAXL2; ++l2) {
+      std::vector<BSeg> segs;
+      std::vector<int> b2s;
+      double bytes = 0.0;
+      for (int j = 0; j < M; ++j) {
+        if (!applies(pass, j) || l2_of(pass, j) != l2) continue;
+        unsigned nb = 0;
+        with_line<BlocksOf>(l2, pass, dd[j], &nb);
+        const DemagPlan& p = *members[j].plan;
+        BSeg sg{};
+        sg.d = dd[j];
+        sg.off = members[j].off;
+        sg.msat = members[j].msat;
+        sg.geom = members[j].geom;
+        sg.A = p.A_.get();
+        sg.B = p.B_.get();
+        sg.T = p.T_.get();
+        for (int a = 0; a < 3; ++a) sg.tw[a] = p.tw_[a].get();
+        sg.first = (int)b2s.size();
+        b2s.insert(b2s.end(), nb, (int)segs.size());
+        segs.push_back(sg);
+        bytes += p.pass_bytes(plan_pass(pass, j));
+      }
+      if (segs.empty()) continue;
+      Launch L;
+      L.pass = pass;
+      L.l2 = l2;
+      L.nblocks = (unsigned)b2s.size();
+      L.bytes = bytes;
+      L.segs.allocate(segs.size() * sizeof(BSeg));
+      HIP_CHECK(hipMemcpy(L.segs.get(), segs.data(), segs.size() * sizeof(BSeg), hipMemcpyHostToDevice));
+      L.b2s.allocate(b2s.size());
+      HIP_CHECK(hipMemcpy(L.b2s.get(), b2s.data(), b2s.size() * sizeof(int), hipMemcpyHostToDevice));
+      launches_.push_back(std::move(L));
+    }
+  }
+}
+
+void DemagBatch::exec(const float* X, float* H, hipStream_t s, Profiler* prof) const {
+  static const char* names[] = {"k_xfwd_b", "k_yfwd_b", "k_zpass_b", "k_yinv_b", "k_ypass2d_b", "k_xinv_b"};
+  for (const Launch& L : launches_) {
+    Timed t(prof, names[L.pass], L.bytes, s);
+    with_line<RunBatch>(L.l2, L.pass, L.nblocks, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, s);
+  }
+}
+
+double DemagBatch::algorithmic_bytes() const {
+  double b = 0.0;
+  for (const Launch& L : launches_) b += L.bytes;
+  return b;
 }
 
 const char* DemagPlan::pass_name(int i) const {

@@ -4,6 +4,8 @@
 // StrayFieldKernel (src/physics/strayfieldkernel.cu:11-99).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "profiler.hpp"
 
@@ -48,6 +50,7 @@ class DemagPlan {
   int spectrum_dims(int a) const { return a == 0 ? Hx_ : (a == 1 ? Hy_ : Hz_); }
 
  private:
+  friend class DemagBatch;
   void build_kernel(hipStream_t s);
   DemagGeometry g_;
   int N_ = 0;
@@ -57,6 +60,39 @@ class DemagPlan {
   DeviceArray<float2> A_, B_;
   DeviceArray<float> T_;
   DeviceArray<float2> tw_[3];
+};
+
+// Demag of many independent magnets at once, each with its own plan.  The
+// magnetisations and fields are packed SoA arrays: component c of cell i of
+// member j is at c * cs + off_j + i.  Every pass is one launch per FFT line
+// length over all members with that length (a block finds its member through
+// a block table), so a batch of small magnets fills the GPU in a handful of
+// launches instead of a few per magnet.
+class DemagBatch {
+ public:
+  struct Member {
+    DemagPlan* plan;
+    long long off;
+    PS msat;
+    const uint8_t* geom;  // the member's own mask or nullptr
+  };
+  DemagBatch(const std::vector<Member>& members, long long cs);
+  // H = demag field of X for every member (the plans' A, B buffers are the
+  // scratch space).
+  void exec(const float* X, float* H, hipStream_t s, Profiler* prof) const;
+  double algorithmic_bytes() const;
+  int nlaunches() const { return (int)launches_.size(); }
+
+ private:
+  struct Launch {
+    int pass;  // 0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d, 5 xinv
+    int l2;
+    unsigned nblocks;
+    DeviceArray<char> segs;
+    DeviceArray<int> b2s;
+    double bytes;
+  };
+  std::vector<Launch> launches_;
 };
 
 // The real-space kernel exactly as the reference lays it out for a magnet

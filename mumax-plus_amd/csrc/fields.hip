@@ -386,22 +386,11 @@ struct FMUniform {
 
 __device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m) { return dmi_gamma(u.D, axis, m); }
 
-__global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
-                                                         const float* __restrict__ hdemag, float* __restrict__ out,
-                                                         int mode) {
-  // 2-D blocks: threadIdx.x along x, threadIdx.y along y, one z plane per
-  // block row.  Workgroups are dealt round-robin to the 8 XCDs in dispatch
-  // order; the remap gives each XCD a contiguous run of tiles, so the y and z
-  // neighbour rows a tile reads are mostly in that XCD's L2 already.
-  const int nbx = gridDim.x, nby = gridDim.y;
-  const int lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
-  const int tile = xcd_remap(lin, nbx * nby * gridDim.z);
-  const int bxi = tile % nbx, rest = tile / nbx;
-  const int x = bxi * blockDim.x + threadIdx.x;
-  const int y = (rest % nby) * blockDim.y + threadIdx.y;
-  const int z = rest / nby;
-  if (x >= d.nx || y >= d.ny) return;
-  const int N = d.ncell;
+// One cell (x, y, z) of the uniform path; m, hdemag and out have component
+// stride N (the magnet's cell count, or the packed length of a batch).
+__device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUniform& u, const float* __restrict__ m,
+                                                   const float* __restrict__ hdemag, float* __restrict__ out,
+                                                   int mode, int x, int y, int z, int N) {
   const int idx = (z * d.ny + y) * d.nx + x;
   if (d.geom && !d.geom[idx]) {
     out[idx] = 0.f;
@@ -489,7 +478,7 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   float3 h = hex;
   h = h + han;
   h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
-  if (d.hstray) h = h + load3(d.hstray, idx, N);
+  if (d.hstray) h = h + load3(d.hstray, idx, d.ncell);
   h = h + hdm;
   if (u.demag) h = h + load3(hdemag, idx, N);
   float3 r;
@@ -506,6 +495,48 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   out[idx] = r.x;
   out[N + idx] = r.y;
   out[2 * N + idx] = r.z;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
+                                                         const float* __restrict__ hdemag, float* __restrict__ out,
+                                                         int mode) {
+  // 2-D blocks: threadIdx.x along x, threadIdx.y along y, one z plane per
+  // block row.  Workgroups are dealt round-robin to the 8 XCDs in dispatch
+  // order; the remap gives each XCD a contiguous run of tiles, so the y and z
+  // neighbour rows a tile reads are mostly in that XCD's L2 already.
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const int tile = xcd_remap(lin, nbx * nby * gridDim.z);
+  const int bxi = tile % nbx, rest = tile / nbx;
+  const int x = bxi * blockDim.x + threadIdx.x;
+  const int y = (rest % nby) * blockDim.y + threadIdx.y;
+  const int z = rest / nby;
+  if (x >= d.nx || y >= d.ny) return;
+  field_uniform_cell(d, u, m, hdemag, out, mode, x, y, z, d.ncell);
+}
+
+// A member of a batched launch: its description, coefficients, first cell
+// in the packed arrays and first block of the launch.
+struct FBInst {
+  FMDesc d;
+  FMUniform u;
+  int off;
+  int first;
+  int mode;
+};
+
+// Batched uniform path: 256 consecutive cells of one magnet per block (every
+// magnet starts on a multiple of 256 cells of the packed arrays).
+__global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
+                                                           const float* __restrict__ X, const float* __restrict__ HD,
+                                                           float* __restrict__ OUT, int cs) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const FBInst& I = tab[b2i[b]];
+  const int j = (b - I.first) * BLOCK + threadIdx.x;
+  const int nx = I.d.nx, ny = I.d.ny;
+  if (j >= I.d.ncell) return;
+  const int x = j % nx, yz = j / nx, y = yz % ny, z = yz / ny;
+  field_uniform_cell(I.d, I.u, X + I.off, HD + I.off, OUT + I.off, I.mode, x, y, z, cs);
 }
 
 // Every parameter the per-cell kernel reads is uniform.
@@ -583,6 +614,41 @@ FMUniform make_uniform(const FMDesc& d) {
 }
 
 }  // namespace
+
+bool field_batchable(const FMDesc& d) { return all_uniform(d) && !d.hstray; }
+
+FieldBatch::FieldBatch(const std::vector<FieldBatchMember>& members, int cs) : cs_(cs) {
+  std::vector<FBInst> tab;
+  std::vector<int> b2i;
+  for (const FieldBatchMember& mb : members) {
+    if (!field_batchable(mb.d)) throw Error(MPA_ERR_INVALID, "batched field: every parameter must be uniform");
+    if (mb.off % BLOCK != 0) throw Error(MPA_ERR_RUNTIME, "batched field: magnets must start on 256-cell boundaries");
+    if (mb.mode != MODE_FIELD && mb.mode != MODE_LLG && mb.mode != MODE_DAMPING)
+      throw Error(MPA_ERR_INVALID, "batched field: fields and torques only");
+    FBInst I{};
+    I.d = mb.d;
+    I.u = make_uniform(mb.d);
+    I.off = mb.off;
+    I.first = (int)b2i.size();
+    I.mode = mb.mode;
+    const int nb = (mb.d.ncell + BLOCK - 1) / BLOCK;
+    b2i.insert(b2i.end(), nb, (int)tab.size());
+    tab.push_back(I);
+  }
+  nblocks_ = (unsigned)b2i.size();
+  if (tab.empty()) return;
+  tab_.allocate(tab.size() * sizeof(FBInst));
+  HIP_CHECK(hipMemcpy(tab_.get(), tab.data(), tab.size() * sizeof(FBInst), hipMemcpyHostToDevice));
+  b2i_.allocate(b2i.size());
+  HIP_CHECK(hipMemcpy(b2i_.get(), b2i.data(), b2i.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t s) const {
+  if (nblocks_ == 0) return;
+  k_field_uniform_b<<<nblocks_, BLOCK, 0, s>>>(reinterpret_cast<const FBInst*>(tab_.get()), b2i_.get(), X, HD, OUT,
+                                                cs_);
+  checkLaunch("k_field_uniform_b");
+}
 
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {
   if (mode != MODE_EDENS && all_uniform(d) && d.nz <= 65535) {

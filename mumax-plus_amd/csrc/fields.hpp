@@ -2,6 +2,8 @@
 // ferromagnet, fused into one stencil kernel.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace mpa {
@@ -62,5 +64,30 @@ enum FieldMode {
 
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
+
+// The field kernels of many ferromagnets with uniform parameters and no
+// stray field in one launch.  Their cells are packed: component c of cell i
+// of a member is at c * cs + off + i, and every member starts on a multiple
+// of 256 cells (one block never spans two members).
+bool field_batchable(const FMDesc& d);
+struct FieldBatchMember {
+  FMDesc d;  // d.geom: the member's own mask or nullptr
+  int off;
+  int mode;  // MODE_FIELD, MODE_LLG or MODE_DAMPING
+};
+class FieldBatch {
+ public:
+  FieldBatch(const std::vector<FieldBatchMember>& members, int cs);
+  // OUT = field or torque (each member's mode) of X, with HD where the
+  // member has TERM_DEMAG, for every member.
+  void launch(const float* X, const float* HD, float* OUT, hipStream_t s) const;
+  unsigned nblocks() const { return nblocks_; }
+
+ private:
+  int cs_;
+  unsigned nblocks_ = 0;
+  DeviceArray<char> tab_;
+  DeviceArray<int> b2i_;
+};
 
 }  // namespace mpa

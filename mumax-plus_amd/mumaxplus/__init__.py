@@ -19,8 +19,9 @@ from .parameter import Parameter
 from .variable import Variable
 from .dmitensor import DmiTensor, DmiTensorGroup
 from .strayfield import StrayField
+from .batch import Batch
 from . import util
 
 __all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
            "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
-           "DmiTensorGroup", "StrayField", "util"]
+           "DmiTensorGroup", "StrayField", "Batch", "util"]

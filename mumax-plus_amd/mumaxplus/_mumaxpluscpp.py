@@ -123,6 +123,19 @@ _SIGS = {
     "mpa_world_set_profiling": ([_P, _cint], _cint),
     "mpa_world_profile_reset": ([_P], _cint),
     "mpa_world_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
+    "mpa_batch_create": ([_ct.c_longlong, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_batch_destroy": ([_P], _cint),
+    "mpa_batch_add": ([_P, _P, _cint, _cdouble, _ct.POINTER(_cint)], _cint),
+    "mpa_batch_remove": ([_P, _cint], _cint),
+    "mpa_batch_advance": ([_P, _ct.c_longlong], _cint),
+    "mpa_batch_finished": ([_P, _cint, _P, _ct.POINTER(_cint)], _cint),
+    "mpa_batch_member": ([_P, _cint, _ct.POINTER(_cdouble), _ct.POINTER(_cdouble), _ct.POINTER(_ct.c_longlong),
+                          _ct.POINTER(_ct.c_longlong), _ct.POINTER(_cint), _ct.POINTER(_ct.c_float),
+                          _ct.POINTER(_ct.c_float)], _cint),
+    "mpa_batch_info": ([_P, _ct.POINTER(_ct.c_longlong), _ct.POINTER(_ct.c_longlong), _ct.POINTER(_cint),
+                        _ct.POINTER(_cdouble), _ct.POINTER(_cint)], _cint),
+    "mpa_batch_set_option": ([_P, _c_char_p, _cint], _cint),
+    "mpa_batch_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -955,6 +968,88 @@ class World:
 
 
 # ----------------------------------------------------------- DemagPlan
+def _profile_report(fn, h):
+    cap = 64
+    names = _ct.create_string_buffer(32 * cap)
+    ms = _np.zeros(cap)
+    n = _np.zeros(cap, dtype=_np.int64)
+    by = _np.zeros(cap)
+    cnt = _cint()
+    _check(fn(h, cap, names, _ptr(ms), _ptr(n), _ptr(by), _ct.byref(cnt)))
+    raw = names.raw
+    out = {}
+    for i in range(min(cnt.value, cap)):
+        nm = raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(n[i]), float(by[i]))
+    return out
+
+
+class Batch:
+    """Independent ferromagnets stepped together (mpa_batch_*; no counterpart
+    in the reference)."""
+
+    MODES = {"steps": 0, "run": 1, "relax": 2}
+
+    def __init__(self, capacity, method="DormandPrince"):
+        h = _P()
+        _check(_lib.mpa_batch_create(int(capacity), _enc(method), _ct.byref(h)))
+        self._h = h
+        self._members = {}  # id -> magnet binding (keeps its world alive)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.mpa_batch_destroy(h)
+            self._h = None
+
+    def add(self, magnet, mode, arg):
+        out = _cint()
+        _check(_lib.mpa_batch_add(self._h, magnet._h, self.MODES[mode], float(arg), _ct.byref(out)))
+        if out.value < 0:
+            return None
+        self._members[out.value] = magnet
+        return out.value
+
+    def remove(self, mid):
+        _check(_lib.mpa_batch_remove(self._h, int(mid)))
+        return self._members.pop(mid)
+
+    def advance(self, n):
+        _check(_lib.mpa_batch_advance(self._h, int(n)))
+
+    def finished(self):
+        cnt = _cint()
+        _check(_lib.mpa_batch_finished(self._h, 0, None, _ct.byref(cnt)))
+        ids = _np.zeros(max(cnt.value, 1), dtype=_np.int32)
+        _check(_lib.mpa_batch_finished(self._h, len(ids), _ptr(ids), _ct.byref(cnt)))
+        return [int(i) for i in ids[:cnt.value]]
+
+    def member(self, mid):
+        t, dt = _cdouble(), _cdouble()
+        st, rj = _ct.c_longlong(), _ct.c_longlong()
+        dn = _cint()
+        er, tq = _ct.c_float(), _ct.c_float()
+        _check(_lib.mpa_batch_member(self._h, int(mid), _ct.byref(t), _ct.byref(dt), _ct.byref(st), _ct.byref(rj),
+                                     _ct.byref(dn), _ct.byref(er), _ct.byref(tq)))
+        return {"time": t.value, "timestep": dt.value, "steps": st.value, "rejected": rj.value,
+                "done": bool(dn.value), "error": er.value, "max_torque": tq.value}
+
+    def info(self):
+        cap, free = _ct.c_longlong(), _ct.c_longlong()
+        mem, lps = _cint(), _cint()
+        by = _cdouble()
+        _check(_lib.mpa_batch_info(self._h, _ct.byref(cap), _ct.byref(free), _ct.byref(mem), _ct.byref(by),
+                                   _ct.byref(lps)))
+        return {"capacity": cap.value, "free_cells": free.value, "members": mem.value, "step_bytes": by.value,
+                "launches_per_step": lps.value}
+
+    def set_option(self, key, value):
+        _check(_lib.mpa_batch_set_option(self._h, _enc(key), int(value)))
+
+    def profile_report(self):
+        return _profile_report(_lib.mpa_batch_profile_report, self._h)
+
+
 class DemagPlan:
     """Stand-alone demag operator on device pointers (mpa_demag_plan_*)."""
 

@@ -1,0 +1,96 @@
+"""Many small independent ferromagnets stepped together on one GPU.
+
+The reference steps one World at a time; a sweep or an ensemble of small
+magnets then costs a few kernel launches per term, per magnet, per stage.
+``Batch`` packs the magnetisations of its members into one device state and
+advances every member with one set of launches per Runge-Kutta stage (a
+captured HIP graph per step).  Each member is a Ferromagnet alone in its
+World, with uniform material parameters, and keeps its own time step, time,
+step controller and stopping rule: a fixed number of steps, ``run`` for a
+duration (the rules of TimeSolver.run), or damping-only relaxation until its
+max torque is below a threshold.  A member's trajectory does not depend on
+the other members.
+
+    batch = Batch(capacity_cells)
+    ids = [batch.add(magnet, steps=100) for magnet in magnets]
+    batch.run_until_done()          # results are back in each magnet
+"""
+
+from . import _mumaxpluscpp as _cpp
+
+
+class Batch:
+    """Batch(capacity, method="DormandPrince"): room for `capacity` cells
+    (every member takes its cell count rounded up to 256)."""
+
+    def __init__(self, capacity, method="DormandPrince"):
+        self._impl = _cpp.Batch(capacity, method)
+        self._magnets = {}
+
+    def add(self, magnet, steps=None, duration=None, relax_torque=None):
+        """Admit `magnet` with exactly one stopping rule; returns its member
+        id, or None when its cells do not fit in the free capacity now."""
+        rules = [(k, v) for k, v in (("steps", steps), ("run", duration), ("relax", relax_torque)) if v is not None]
+        if len(rules) != 1:
+            raise ValueError("give exactly one of steps, duration, relax_torque")
+        mode, arg = rules[0]
+        mid = self._impl.add(magnet._impl, mode, arg)
+        if mid is not None:
+            self._magnets[mid] = magnet
+        return mid
+
+    def remove(self, member):
+        """Copy the member's state back into its magnet and world, free its
+        cells and return the magnet."""
+        self._impl.remove(member)
+        return self._magnets.pop(member)
+
+    def advance(self, nsteps=1):
+        """`nsteps` batch steps: one step attempt of every active member each."""
+        self._impl.advance(nsteps)
+
+    def finished(self):
+        """Ids of the members whose stopping rule is met."""
+        return self._impl.finished()
+
+    def member(self, member):
+        """time, timestep, steps, rejected, done, error and max_torque of a member."""
+        return self._impl.member(member)
+
+    @property
+    def members(self):
+        return dict(self._magnets)
+
+    def run_until_done(self, poll=32, max_steps=10**9):
+        """Advance until every member is finished (checking every `poll`
+        steps), then return all members' states to their magnets."""
+        taken = 0
+        while self._magnets:
+            done = set(self.finished())
+            if len(done) == len(self._magnets):
+                break
+            if taken >= max_steps:
+                raise RuntimeError("batch members did not finish within max_steps")
+            self.advance(poll)
+            taken += poll
+        for mid in list(self._magnets):
+            self.remove(mid)
+
+    def info(self):
+        return self._impl.info()
+
+    @property
+    def use_graph(self):
+        return getattr(self, "_graph", True)
+
+    @use_graph.setter
+    def use_graph(self, value):
+        self._impl.set_option("graph", bool(value))
+        self._graph = bool(value)
+
+    def set_profiling(self, enable):
+        self._impl.set_option("profiling", bool(enable))
+
+    def profile_report(self):
+        """{kernel name: (total ms, launches, algorithmic bytes)} since profiling was enabled."""
+        return self._impl.profile_report()
