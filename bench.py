@@ -22,6 +22,13 @@ afmex_cell -4e-12 J/m, bulk DMI 1e-3 J/m^2 on the inter- and intra-sublattice
 tensors, alpha 0.004, B1 = B2 = -8.8e6 J/m^3, C11/C12/C44 = 283/166/58 GPa,
 rho 8900 kg/m^3, eta 1e10, u = v = 0, m1 random and m2 = -m1 + 1% noise,
 Dormand-Prince at a fixed dt of 1e-14 s) and prints one JSON line for it.
+
+`--workload batch` measures batched solves/s: 256 independent ferromagnets
+(log2 of nx, ny uniform in [4, 7], log2 nz in [0, 2], 2 nm cells, Ms, A, Ku
+along z and alpha drawn uniformly from [4e5, 1.4e6], [5e-12, 2e-11], [0, 1e6],
+[0.005, 0.1]; exchange + demag + anisotropy), one solve being 100 fixed
+Dormand-Prince steps at 1e-13 s, stepped together by mumaxplus.Batch, next to
+the same instances stepped one World at a time.
 """
 
 import argparse
@@ -177,6 +184,178 @@ def run_afm(args):
     print(json.dumps(out))
 
 
+BATCH_N, BATCH_STEPS, BATCH_DT = 256, 100, 1e-13
+
+
+def batch_specs(seed):
+    """The batched workload: BATCH_N ferromagnets with log2(nx), log2(ny)
+    uniform in [4, 7] and log2(nz) uniform in [0, 2] (rounded), 2 nm cells,
+    Ms in [4e5, 1.4e6] A/m, A in [5e-12, 2e-11] J/m, Ku in [0, 1e6] J/m^3
+    along z, alpha in [0.005, 0.1], random unit m0; exchange + demag +
+    anisotropy."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(BATCH_N):
+        nx, ny = (int(2 ** round(v)) for v in rng.uniform(4, 7, 2))
+        nz = int(2 ** round(rng.uniform(0, 2)))
+        p = {"msat": rng.uniform(4e5, 1.4e6), "aex": rng.uniform(5e-12, 2e-11), "ku1": rng.uniform(0, 1e6),
+             "alpha": rng.uniform(0.005, 0.1)}
+        m = rng.normal(size=(3, nz, ny, nx))
+        m /= np.linalg.norm(m, axis=0)
+        specs.append(((nx, ny, nz), p, m.astype(np.float32)))
+    return specs
+
+
+def build_batch_member(spec):
+    import mumaxplus as mp
+    size, p, m0 = spec
+    world = mp.World((CELL, CELL, CELL))
+    mag = mp.Ferromagnet(world, mp.Grid(size))
+    mag.msat = p["msat"]
+    mag.aex = p["aex"]
+    mag.ku1 = p["ku1"]
+    mag.anisU = (0, 0, 1)
+    mag.alpha = p["alpha"]
+    mag.magnetization = m0
+    ts = world.timesolver
+    ts.set_method("DormandPrince")
+    ts.adaptive_timestep = False
+    ts.timestep = BATCH_DT
+    mag.demag_field.eval()  # builds the demag plan (kernel construction is not timed)
+    return world, mag
+
+
+def batch_cpu_baseline(specs, budget_s=20.0):
+    """FP64 oracle: one Dormand-Prince step of every 32nd instance, time per
+    cell-step extrapolated to BATCH_STEPS steps of all instances."""
+    from oracle import demag as odemag
+    from oracle import model as omodel
+
+    cores = os.cpu_count() or 1
+    odemag.set_fft_workers(cores)
+    t_tot, cells = 0.0, 0
+    picked = 0
+    for size, p, m0 in specs[::32]:
+        ow = omodel.World((CELL, CELL, CELL))
+        om = ow.add_ferromagnet(size, defer_timestep=True)
+        for k, v in p.items():
+            om.set(k, v)
+        om.set("anisU", (0.0, 0.0, 1.0))
+        om.set_m(m0.astype(np.float64))
+        om.demag_plan()
+        ow.solver.set_method("DormandPrince", defer_timestep=True)
+        ow.solver.adaptive = False
+        ow.solver.dt = BATCH_DT
+        t0 = time.perf_counter()
+        ow.solver.step()
+        t_tot += time.perf_counter() - t0
+        cells += size[0] * size[1] * size[2]
+        picked += 1
+        if t_tot > budget_s:
+            break
+    all_cells = sum(sz[0] * sz[1] * sz[2] for sz, _, _ in specs)
+    t_all = t_tot / cells * all_cells * BATCH_STEPS
+    return {"value": len(specs) / t_all, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"one FP64 Dormand-Prince step of {picked} of the {len(specs)} instances (every 32nd), "
+                      f"time per cell-step scaled to {BATCH_STEPS} steps of all instances",
+            "seconds": round(t_tot, 3)}
+
+
+def run_batch(args):
+    """Batched solves: every instance advanced BATCH_STEPS fixed
+    Dormand-Prince steps (10 ps); solves/s = instances / wall seconds from
+    the first admission to the last result copied back, against the same
+    instances stepped one World at a time."""
+    import mumaxplus as mp
+    if not mp._cpp.device_available():
+        raise RuntimeError("no HIP device visible")
+    specs = batch_specs(args.seed)
+    cells = sum(sz[0] * sz[1] * sz[2] for sz, _, _ in specs)
+    span = sum(-(-sz[0] * sz[1] * sz[2] // 256) * 256 for sz, _, _ in specs)
+    steps = args.steps if args.steps_given else BATCH_STEPS
+    # warm-up: every kernel once, on a few instances
+    warm = [build_batch_member(sp) for sp in specs[:8]]
+    wb = mp.Batch(span)
+    for _, mg in warm:
+        wb.add(mg, steps=2)
+    wb.run_until_done(poll=2)
+    for _, mg in warm:
+        _, _ = mg, mg.magnetization.average()
+    # batched path
+    members = [build_batch_member(sp) for sp in specs]
+    batch = mp.Batch(span)
+    t0 = time.perf_counter()
+    for _, mg in members:
+        if batch.add(mg, steps=steps) is None:
+            raise RuntimeError("batch capacity too small")
+    batch.run_until_done(poll=steps)
+    wall = time.perf_counter() - t0
+    # plain sequential path: the same instances one World at a time
+    plain = [build_batch_member(sp) for sp in specs]
+    plain[0][0].timesolver.steps(1)  # warm-up of the single-magnet kernels
+    plain[0] = build_batch_member(specs[0])
+    t0 = time.perf_counter()
+    for w, _ in plain:
+        w.timesolver.steps(steps)
+    wall_seq = time.perf_counter() - t0
+    dm = max(float(np.abs(a.magnetization.get() - b.magnetization.get()).max())
+             for (_, a), (_, b) in zip(members, plain))
+    # steady-state step time and per-kernel roofline on the full membership
+    prof = [build_batch_member(sp) for sp in specs]
+    pb = mp.Batch(span)
+    for _, mg in prof:
+        pb.add(mg, steps=10 ** 9)
+    pb.advance(5)
+    t0 = time.perf_counter()
+    pb.advance(args.profile_steps)
+    t_step = (time.perf_counter() - t0) / args.profile_steps
+    info = pb.info()
+    pb.set_profiling(True)
+    pb.advance(min(args.profile_steps, 20))
+    rep = pb.profile_report()
+    pb.set_profiling(False)
+    tot_ms = sum(v[0] for v in rep.values())
+    with_bytes = {k: v for k, v in rep.items() if v[2] > 0}
+    dom = max(with_bytes, key=lambda k: with_bytes[k][0])
+    d_ms, d_n, d_b = rep[dom]
+    achieved = (d_b / d_n) / (d_ms / d_n * 1e-3)
+    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
+                   "GBps": round((v[2] / v[1]) / (v[0] / v[1] * 1e-3) / 1e9, 1) if v[2] > 0 else None,
+                   "share": round(v[0] / tot_ms, 4)} for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}
+    b_step = info["step_bytes"]
+    cpu = None if args.no_cpu_baseline else batch_cpu_baseline(specs)
+    out = {
+        "metric": "batched solves/s (256 independent ferromagnets, RK45)",
+        "value": len(specs) / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 2,
+        "ms_per_step": 1e3 * t_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(specs)} ferromagnets, log2(nx), log2(ny) in [4, 7], log2(nz) in [0, 2], "
+                               "2 nm cells, exchange + demag + anisotropy, one solve = "
+                               f"{steps} Dormand-Prince steps at a fixed dt of 1e-13 s",
+                   "instances": len(specs), "cells": cells, "packed_cells": span, "dt": BATCH_DT,
+                   "method": "DormandPrince", "seed": args.seed},
+        "sequential": {"value": len(specs) / wall_seq, "unit": "solves/s", "wall_s": wall_seq,
+                       "speedup": wall_seq / wall, "max_abs_dm_vs_batched": dm},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_launch": d_b / d_n,
+                     "step": {"algorithmic_bytes": b_step, "achieved_GBps": b_step / t_step / 1e9,
+                              "frac": b_step / t_step / HBM_PEAK,
+                              "launches": info["launches_per_step"]}},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def step_bytes(N, Hx, Ky, Hy, Hz, ny, nz):
     """Algorithmic HBM bytes of one DP step with FSAL (6 RHS evaluations)."""
     b_dm = 12 * N + 48.0 * Hx * ny * nz + 48.0 * Hx * Ky * nz + 24.0 * Hx * Hy * Hz + 48.0 * Hx * Ky * nz \
@@ -250,7 +429,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("ferromagnet", "afm"), default="ferromagnet")
+    ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch"), default="ferromagnet")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -258,6 +437,9 @@ def main():
         sys.exit(2)
     if args.workload == "afm":
         run_afm(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
         return
 
     import mumaxplus as mp
