@@ -48,44 +48,69 @@ struct CtlArgs {
   int prime;
 };
 
-// One chunk (one member) per block.
-__global__ void __launch_bounds__(CH) k_bstage(StageArgs sa, float* __restrict__ X, float* __restrict__ X0,
-                                               float* __restrict__ K0, const float* __restrict__ KL,
-                                               const uint8_t* __restrict__ G, const int* __restrict__ c2m,
-                                               const BatchMember* __restrict__ st, int cs) {
-  const int chunk = blockIdx.x;
+__device__ __forceinline__ void axpy4(float4& a, float c, const float4& k) {
+  a.x += c * k.x;
+  a.y += c * k.y;
+  a.z += c * k.z;
+  a.w += c * k.w;
+}
+
+// One wavefront per chunk (64 threads x 4 consecutive cells), four chunks
+// per block; a chunk belongs to one member, so every branch on the member is
+// uniform over the wavefront.
+__global__ void __launch_bounds__(256) k_bstage(StageArgs sa, float* __restrict__ X, float* __restrict__ X0,
+                                                float* __restrict__ K0, const float* __restrict__ KL,
+                                                const uint8_t* __restrict__ G, const int* __restrict__ c2m,
+                                                const BatchMember* __restrict__ st, int cs, int nch) {
+  const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (chunk >= nch) return;
   const int m = c2m[chunk];
   if (m < 0) return;
   const BatchMember& S = st[m];
   if (!S.active) return;
-  const int g = chunk * CH + threadIdx.x;
+  const int q = chunk * (CH / 4) + (threadIdx.x & 63);  // float4 index within a component
+  const int cs4 = cs >> 2;
   const bool take = sa.commit && S.accept;
   float c[MAX_STAGES];
 #pragma unroll
   for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < sa.nk ? (float)(S.dt * sa.a[j]) : 0.f;
-  float v[3];
+  float4* X4 = reinterpret_cast<float4*>(X);
+  float4* X04 = reinterpret_cast<float4*>(X0);
+  float4* K04 = reinterpret_cast<float4*>(K0);
+  const float4* KL4 = reinterpret_cast<const float4*>(KL);
+  float4 v[3];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
-    const size_t i = (size_t)comp * cs + g;
-    float x0, k0;
+    const size_t i = (size_t)comp * cs4 + q;
+    float4 x0, k0;
     if (take) {
-      x0 = X[i];
-      k0 = KL[i];
-      X0[i] = x0;
-      K0[i] = k0;
+      x0 = X4[i];
+      k0 = KL4[i];
+      X04[i] = x0;
+      K04[i] = k0;
     } else {
-      x0 = X0[i];
-      k0 = K0[i];
+      x0 = X04[i];
+      k0 = K04[i];
     }
-    float a = x0;
-    a += c[0] * k0;
-    for (int j = 1; j < sa.nk; ++j) a += c[j] * sa.k[j][i];
+    float4 a = x0;
+    axpy4(a, c[0], k0);
+    for (int j = 1; j < sa.nk; ++j) axpy4(a, c[j], reinterpret_cast<const float4*>(sa.k[j])[i]);
     v[comp] = a;
   }
-  const float inv = rsqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-  const bool in = G[g] != 0;
+  const float i0 = rsqrtf(v[0].x * v[0].x + v[1].x * v[1].x + v[2].x * v[2].x);
+  const float i1 = rsqrtf(v[0].y * v[0].y + v[1].y * v[1].y + v[2].y * v[2].y);
+  const float i2 = rsqrtf(v[0].z * v[0].z + v[1].z * v[1].z + v[2].z * v[2].z);
+  const float i3 = rsqrtf(v[0].w * v[0].w + v[1].w * v[1].w + v[2].w * v[2].w);
+  const uchar4 g = reinterpret_cast<const uchar4*>(G)[q];
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp) X[(size_t)comp * cs + g] = in ? v[comp] * inv : 0.f;
+  for (int comp = 0; comp < 3; ++comp) {
+    float4 o;
+    o.x = g.x ? v[comp].x * i0 : 0.f;
+    o.y = g.y ? v[comp].y * i1 : 0.f;
+    o.z = g.z ? v[comp].z * i2 : 0.f;
+    o.w = g.w ? v[comp].w * i3 : 0.f;
+    X4[(size_t)comp * cs4 + q] = o;
+  }
 }
 
 __device__ __forceinline__ float block_max(float v, float* red) {
@@ -471,8 +496,8 @@ void Batch::record_step() {
     }
     {
       Timed t(&prof, "k_bstage", 12.0 * used * (2 + s) + (s == 1 ? 24.0 * used : 0.0), stream);
-      k_bstage<<<nch, CH, 0, stream>>>(sa, X_.get(), X0_.get(), K_[0].get(), K_[S - 1].get(), G_.get(),
-                                       c2m_.get(), st_.get(), (int)cs_);
+      k_bstage<<<(nch + 3) / 4, 256, 0, stream>>>(sa, X_.get(), X0_.get(), K_[0].get(), K_[S - 1].get(), G_.get(),
+                                                  c2m_.get(), st_.get(), (int)cs_, (int)nch);
       checkLaunch("k_bstage");
     }
     eval_rhs(K_[s].get());

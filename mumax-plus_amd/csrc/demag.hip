@@ -117,6 +117,19 @@ struct XfwdCfg {
   static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xfwd LDS");
 };
 
+// Batched launches run every line length with 256 threads per block, so
+// one launch covers magnets of all sizes.
+template <int L2>
+struct XfwdCfgB {
+  using C = Line<L2>;
+  static constexpr int PB = 256 / C::T;
+  static constexpr int L2PB = ilog2c(PB);
+  static constexpr int THREADS = PB * C::T;
+  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (PB < 16 ? PB : 16));
+  static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
+  static_assert(THREADS == 256 && SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "batched k_xfwd");
+};
+
 // Offset of x-line l (c, z, y) in A[c][z][kx][y] at kx = 0; -1 past the end.
 __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
   if (l >= d.nlines) return -1;
@@ -124,12 +137,11 @@ __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
   return (long long)yz * d.Hx * d.ny + (l - yz * d.ny);
 }
 
-template <int L2>
+template <int L2, class K = XfwdCfg<L2>>
 __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__ m, PS msat,
                                           const float* __restrict__ m2, PS msat2, const uint8_t* __restrict__ geom,
                                           float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = XfwdCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long abase[2 * K::PB];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
@@ -184,16 +196,6 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   xfwd_body<L2>(d, m, msat, m2, msat2, geom, A, tw, blockIdx.x);
 }
 
-// Batched launches: block -> magnet through b2s, then the magnet's own
-// block index.
-template <int L2>
-__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
-    k_xfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s, const float* __restrict__ X) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  xfwd_body<L2>(s.d, X + s.off, s.msat, nullptr, PS{nullptr, 0.f}, s.geom, s.A, s.tw[0], b - s.first);
-}
-
 // ---------------------------------------------------------------- y forward / inverse (3-D)
 
 template <int L2>
@@ -205,6 +207,17 @@ struct YCfg {
   static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / LB);
   static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
   static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
+};
+
+template <int L2>
+struct YCfgB {
+  using C = Line<L2>;
+  static constexpr int LB = 256 / C::T;
+  static constexpr int L2LB = ilog2c(LB);
+  static constexpr int THREADS = LB * C::T;
+  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (LB < 32 ? LB : 32));
+  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
+  static_assert(THREADS == 256 && SHM + LB * sizeof(long long) <= LDS_LIMIT, "batched k_yfwd");
 };
 
 // Offset of y-line q = (c, kx, z) in B[c][kx][ky][z] at ky = 0; -1 past the end.
@@ -221,11 +234,10 @@ __device__ __forceinline__ size_t a_yline(const DD& d, int q) {
   return ((size_t)(c * d.nz + z) * d.Hx + kx) * d.ny;
 }
 
-template <int L2>
+template <int L2, class K = YCfg<L2>>
 __device__ __forceinline__ void yfwd_body(const DD& d, const float2* __restrict__ A, float2* __restrict__ B,
                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = YCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
@@ -263,19 +275,10 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
   yfwd_body<L2>(d, A, B, tw, xcd_remap(blockIdx.x, gridDim.x));
 }
 
-template <int L2>
-__global__ void __launch_bounds__(YCfg<L2>::THREADS)
-    k_yfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  yfwd_body<L2>(s.d, s.A, s.B, s.tw[1], b - s.first);
-}
-
-template <int L2>
+template <int L2, class K = YCfg<L2>>
 __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict__ B, float2* __restrict__ A,
                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = YCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long bbase[K::LB];
   const int nq = 3 * d.Hx * d.nz;
@@ -310,14 +313,6 @@ template <int L2>
 __global__ void __launch_bounds__(YCfg<L2>::THREADS)
     k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A, const float2* __restrict__ tw) {
   yinv_body<L2>(d, B, A, tw, xcd_remap(blockIdx.x, gridDim.x));
-}
-
-template <int L2>
-__global__ void __launch_bounds__(YCfg<L2>::THREADS)
-    k_yinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  yinv_body<L2>(s.d, s.B, s.A, s.tw[1], b - s.first);
 }
 
 // ---------------------------------------------------------------- kernel multiply
@@ -424,14 +419,6 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   zpass_body<L2>(d, B, Tk, tw, blockIdx.x);
 }
 
-template <int L2>
-__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
-    k_zpass_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  zpass_body<L2>(s.d, s.B, s.T, s.tw[2], b - s.first);
-}
-
 // Single layer: forward y, kernel multiply, inverse y; PT points kx per block.
 template <int L2>
 __device__ __forceinline__ void ypass2d_body(const DD& d, float2* __restrict__ A, const float* __restrict__ Tk,
@@ -485,14 +472,6 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
   ypass2d_body<L2>(d, A, Tk, tw, blockIdx.x);
 }
 
-template <int L2>
-__global__ void __launch_bounds__(PointCfg<L2>::THREADS)
-    k_ypass2d_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  ypass2d_body<L2>(s.d, s.A, s.T, s.tw[1], b - s.first);
-}
-
 // ---------------------------------------------------------------- x inverse
 
 template <int L2>
@@ -510,12 +489,24 @@ struct XinvCfg {
   static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xinv LDS");
 };
 
-// PB line pairs per block; C2R of two lines at a time.
 template <int L2>
+struct XinvCfgB {
+  using C = Line<L2>;
+  static constexpr int PB = 256 / C::T;
+  static constexpr int L2PB = ilog2c(PB);
+  static constexpr int THREADS = PB * C::T;
+  static constexpr int XS = C::H + 1;
+  static constexpr int LP = C::stride(C::fft_target());
+  static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
+  static constexpr size_t SHM = (XB > ZB ? XB : ZB) * sizeof(float2);
+  static_assert(THREADS == 256 && SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "batched k_xinv");
+};
+
+// PB line pairs per block; C2R of two lines at a time.
+template <int L2, class K = XinvCfg<L2>>
 __device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict__ A, const uint8_t* __restrict__ geom,
                                           float* __restrict__ h, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = XinvCfg<L2>;
   extern __shared__ float2 lds[];
   __shared__ long long abase[2 * K::PB];
   constexpr int L2NL = K::L2PB + 1;
@@ -569,14 +560,6 @@ __global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
     k_xinv(DD d, const float2* __restrict__ A, const uint8_t* __restrict__ geom, float* __restrict__ h,
            const float2* __restrict__ tw) {
   xinv_body<L2>(d, A, geom, h, tw, xcd_remap(blockIdx.x, gridDim.x));
-}
-
-template <int L2>
-__global__ void __launch_bounds__(XinvCfg<L2>::THREADS)
-    k_xinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s, float* __restrict__ H) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  xinv_body<L2>(s.d, s.A, s.geom, H + s.off, s.tw[0], b - s.first);
 }
 
 // ------------------------------------------------ kernel spectrum (FP64)
@@ -736,35 +719,123 @@ struct RunXinv {
   }
 };
 
-// Batched pass launches and their block counts per magnet.
+// Batched passes: one launch of 256-thread blocks per pass over magnets of
+// every size; a block finds its magnet through b2s and runs that magnet's
+// line length.
+#define MPA_L2_CASES(l2, CALL)                                       \
+  switch (l2) {                                                       \
+    case 0: CALL(0); break;                                           \
+    case 3: CALL(3); break;                                           \
+    case 4: CALL(4); break;                                           \
+    case 5: CALL(5); break;                                           \
+    case 6: CALL(6); break;                                           \
+    case 7: CALL(7); break;                                           \
+    case 8: CALL(8); break;                                           \
+    case 9: CALL(9); break;                                           \
+    case 10: CALL(10); break;                                         \
+    case 11: CALL(11); break;                                         \
+    default: break;                                                   \
+  }
+
+__global__ void __launch_bounds__(256) k_xfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s,
+                                                const float* __restrict__ X) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) xfwd_body<L, XfwdCfgB<L>>(s.d, X + s.off, s.msat, nullptr, PS{nullptr, 0.f}, s.geom, s.A, s.tw[0], lb)
+  MPA_L2_CASES(s.d.l2x, MPA_CALL)
+#undef MPA_CALL
+}
+
+__global__ void __launch_bounds__(256) k_yfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) yfwd_body<L, YCfgB<L>>(s.d, s.A, s.B, s.tw[1], lb)
+  MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+}
+
+__global__ void __launch_bounds__(256) k_zpass_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) zpass_body<L>(s.d, s.B, s.T, s.tw[2], lb)
+  MPA_L2_CASES(s.d.l2z, MPA_CALL)
+#undef MPA_CALL
+}
+
+__global__ void __launch_bounds__(256) k_yinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) yinv_body<L, YCfgB<L>>(s.d, s.B, s.A, s.tw[1], lb)
+  MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+}
+
+__global__ void __launch_bounds__(256) k_ypass2d_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) ypass2d_body<L>(s.d, s.A, s.T, s.tw[1], lb)
+  MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+}
+
+__global__ void __launch_bounds__(256) k_xinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s,
+                                                float* __restrict__ H) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_CALL(L) xinv_body<L, XinvCfgB<L>>(s.d, s.A, s.geom, H + s.off, s.tw[0], lb)
+  MPA_L2_CASES(s.d.l2x, MPA_CALL)
+#undef MPA_CALL
+}
+
+// Blocks a magnet needs in a batched pass, and the pass's LDS bytes at its
+// line length.
 template <int L2>
-struct RunBatch {
-  static void run(int pass, unsigned nb, const BSeg* segs, const int* b2s, const float* X, float* H, hipStream_t s) {
+struct BatchShape {
+  static void run(int pass, const DD& d, unsigned* blocks, size_t* shm) {
     switch (pass) {
-      case 0: k_xfwd_b<L2><<<nb, XfwdCfg<L2>::THREADS, XfwdCfg<L2>::SHM, s>>>(segs, b2s, X); break;
-      case 1: k_yfwd_b<L2><<<nb, YCfg<L2>::THREADS, YCfg<L2>::SHM, s>>>(segs, b2s); break;
-      case 2: k_zpass_b<L2><<<nb, PointCfg<L2>::THREADS, PointCfg<L2>::SHM, s>>>(segs, b2s); break;
-      case 3: k_yinv_b<L2><<<nb, YCfg<L2>::THREADS, YCfg<L2>::SHM, s>>>(segs, b2s); break;
-      case 4: k_ypass2d_b<L2><<<nb, PointCfg<L2>::THREADS, PointCfg<L2>::SHM, s>>>(segs, b2s); break;
-      default: k_xinv_b<L2><<<nb, XinvCfg<L2>::THREADS, XinvCfg<L2>::SHM, s>>>(segs, b2s, H); break;
+      case 0:
+        *blocks = grid_for((d.nlines + 1) / 2, XfwdCfgB<L2>::PB);
+        *shm = XfwdCfgB<L2>::SHM;
+        break;
+      case 1:
+      case 3:
+        *blocks = grid_for(3LL * d.Hx * d.nz, YCfgB<L2>::LB);
+        *shm = YCfgB<L2>::SHM;
+        break;
+      case 2:
+        *blocks = grid_for((long long)d.Hx * d.Ky, PointCfg<L2>::PT);
+        *shm = PointCfg<L2>::SHM;
+        break;
+      case 4:
+        *blocks = grid_for(d.Hx, PointCfg<L2>::PT);
+        *shm = PointCfg<L2>::SHM;
+        break;
+      default:
+        *blocks = grid_for((d.nlines + 1) / 2, XinvCfgB<L2>::PB);
+        *shm = XinvCfgB<L2>::SHM;
+        break;
     }
-    checkLaunch("batched demag pass");
   }
 };
 
-template <int L2>
-struct BlocksOf {
-  static void run(int pass, const DD& d, unsigned* out) {
-    switch (pass) {
-      case 0: *out = grid_for((d.nlines + 1) / 2, XfwdCfg<L2>::PB); break;
-      case 1:
-      case 3: *out = grid_for(3LL * d.Hx * d.nz, YCfg<L2>::LB); break;
-      case 2: *out = grid_for((long long)d.Hx * d.Ky, PointCfg<L2>::PT); break;
-      case 4: *out = grid_for(d.Hx, PointCfg<L2>::PT); break;
-      default: *out = grid_for((d.nlines + 1) / 2, XinvCfg<L2>::PB); break;
-    }
+void run_batch_pass(int pass, unsigned nb, size_t shm, const BSeg* segs, const int* b2s, const float* X, float* H,
+                    hipStream_t s) {
+  switch (pass) {
+    case 0: k_xfwd_b<<<nb, 256, shm, s>>>(segs, b2s, X); break;
+    case 1: k_yfwd_b<<<nb, 256, shm, s>>>(segs, b2s); break;
+    case 2: k_zpass_b<<<nb, 256, shm, s>>>(segs, b2s); break;
+    case 3: k_yinv_b<<<nb, 256, shm, s>>>(segs, b2s); break;
+    case 4: k_ypass2d_b<<<nb, 256, shm, s>>>(segs, b2s); break;
+    default: k_xinv_b<<<nb, 256, shm, s>>>(segs, b2s, H); break;
   }
-};
+  checkLaunch("batched demag pass");
+}
 
 // Calls R<l2>::run(args...) for a supported line length 2^l2.
 template <template <int> class R, typename... Args>
@@ -908,14 +979,17 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
     return pass == 5 ? 4 : pass;
   };
   for (int pass = 0; pass < 6; ++pass) {
-    for (int l2 = 0; l2 <= MAXL2; ++l2) {
+    {
       std::vector<BSeg> segs;
       std::vector<int> b2s;
       double bytes = 0.0;
+      size_t shm_max = 0;
       for (int j = 0; j < M; ++j) {
-        if (!applies(pass, j) || l2_of(pass, j) != l2) continue;
+        if (!applies(pass, j)) continue;
         unsigned nb = 0;
-        with_line<BlocksOf>(l2, pass, dd[j], &nb);
+        size_t shm = 0;
+        with_line<BatchShape>(l2_of(pass, j), pass, dd[j], &nb, &shm);
+        shm_max = shm > shm_max ? shm : shm_max;
         const DemagPlan& p = *members[j].plan;
         BSeg sg{};
         sg.d = dd[j];
@@ -934,7 +1008,7 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
       if (segs.empty()) continue;
       Launch L;
       L.pass = pass;
-      L.l2 = l2;
+      L.shm = shm_max;
       L.nblocks = (unsigned)b2s.size();
       L.bytes = bytes;
       L.segs.allocate(segs.size() * sizeof(BSeg));
@@ -950,7 +1024,7 @@ void DemagBatch::exec(const float* X, float* H, hipStream_t s, Profiler* prof) c
   static const char* names[] = {"k_xfwd_b", "k_yfwd_b", "k_zpass_b", "k_yinv_b", "k_ypass2d_b", "k_xinv_b"};
   for (const Launch& L : launches_) {
     Timed t(prof, names[L.pass], L.bytes, s);
-    with_line<RunBatch>(L.l2, L.pass, L.nblocks, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, s);
+    run_batch_pass(L.pass, L.nblocks, L.shm, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, s);
   }
 }
 

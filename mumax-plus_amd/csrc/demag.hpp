@@ -64,8 +64,8 @@ class DemagPlan {
 
 // Demag of many independent magnets at once, each with its own plan.  The
 // magnetisations and fields are packed SoA arrays: component c of cell i of
-// member j is at c * cs + off_j + i.  Every pass is one launch per FFT line
-// length over all members with that length (a block finds its member through
+// member j is at c * cs + off_j + i.  Every pass is one launch over all
+// members, whatever their FFT line lengths (a block finds its member through
 // a block table), so a batch of small magnets fills the GPU in a handful of
 // launches instead of a few per magnet.
 class DemagBatch {
@@ -86,7 +86,7 @@ class DemagBatch {
  private:
   struct Launch {
     int pass;  // 0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d, 5 xinv
-    int l2;
+    size_t shm;
     unsigned nblocks;
     DeviceArray<char> segs;
     DeviceArray<int> b2s;
