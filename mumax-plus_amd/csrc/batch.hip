@@ -230,7 +230,7 @@ __global__ void k_bcontrol(BatchMember* __restrict__ st, unsigned* __restrict__ 
   } else if (S.mode == BATCH_STEPS) {
     done = S.steps >= S.target;
   } else if (S.mode == BATCH_RELAX) {
-    done = tq <= S.threshold;
+    done = tq <= S.threshold || S.steps >= S.target;  // the target caps a relaxation that stalls
   } else if (!(S.time < S.stop - S.dt)) {
     S.dt_saved = S.dt;
     S.dt = S.stop - S.time;
@@ -396,7 +396,10 @@ int Batch::add(Ferromagnet* fm, int mode, double arg) {
   S.mode = mode;
   if (mode == BATCH_STEPS) S.target = (long long)arg;
   else if (mode == BATCH_RUN) S.stop = ts.time + arg;
-  else S.threshold = (float)arg;
+  else {
+    S.threshold = (float)arg;
+    S.target = BATCH_RELAX_MAX_STEPS;
+  }
   S.active = 1;
   S.accept = 2;  // waiting for its first right-hand side
   HIP_CHECK(hipMemcpyAsync(st_.get() + id, &S, sizeof(S), hipMemcpyHostToDevice, stream));

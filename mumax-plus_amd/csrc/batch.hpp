@@ -35,6 +35,9 @@ enum BatchMode {
   BATCH_RUN = 1,    // TimeSolver::run(duration): land on time + duration
   BATCH_RELAX = 2,  // damping-only dynamics until the max torque <= threshold
 };
+// A relaxation whose torque stalls above its threshold (the adaptive step's
+// error floor) stops after this many accepted steps, not converged.
+constexpr long long BATCH_RELAX_MAX_STEPS = 200000;
 
 // Per-member state in device memory.
 struct BatchMember {

@@ -20,8 +20,9 @@ from .variable import Variable
 from .dmitensor import DmiTensor, DmiTensorGroup
 from .strayfield import StrayField
 from .batch import Batch
+from . import serving
 from . import util
 
 __all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
            "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
-           "DmiTensorGroup", "StrayField", "Batch", "util"]
+           "DmiTensorGroup", "StrayField", "Batch", "serving", "util"]
