@@ -29,6 +29,12 @@ along z and alpha drawn uniformly from [4e5, 1.4e6], [5e-12, 2e-11], [0, 1e6],
 [0.005, 0.1]; exchange + demag + anisotropy), one solve being 100 fixed
 Dormand-Prince steps at 1e-13 s, stepped together by mumaxplus.Batch, next to
 the same instances stepped one World at a time.
+
+`--workload serve` measures served solves/s and p50 / p99 latency of a
+Poisson stream of 128 relaxation and switching requests on elliptical disks
+(mumaxplus.serving, continuous batching), next to the first 12 requests
+solved one at a time through the plain API with the same arrival times (their
+mean service time gives the plain path's saturated solves/s).
 """
 
 import argparse
@@ -356,6 +362,145 @@ def run_batch(args):
     print(json.dumps(out))
 
 
+SERVE_N, SERVE_RATE, SERVE_CAP, SERVE_PLAIN = 128, 100.0, 262144, 24
+
+
+def _pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def serve_cpu_baseline(req, steps, budget_s=20.0):
+    """FP64 oracle: a few Dormand-Prince steps of one switching request at
+    the step size the GPU ended with, scaled to the steps the GPU needed."""
+    from oracle import demag as odemag
+    from oracle import model as omodel
+
+    cores = os.cpu_count() or 1
+    odemag.set_fft_workers(cores)
+    ow = omodel.World(req.cellsize)
+    om = ow.add_ferromagnet(req.size, geometry=req.geometry, defer_timestep=True)
+    for k, v in req.params.items():
+        om.set(k, v)
+    ow.bias = req.bias
+    om.set_m(req.m0.astype(np.float64))
+    om.demag_plan()
+    ow.solver.set_method("DormandPrince", defer_timestep=True)
+    ow.solver.adaptive = False
+    ow.solver.dt = req.duration / max(steps, 1)
+    n, t = 0, 0.0
+    while n < 20 and t < budget_s:
+        t0 = time.perf_counter()
+        ow.solver.step()
+        t += time.perf_counter() - t0
+        n += 1
+    per_solve = t / n * steps
+    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"{n} FP64 Dormand-Prince steps of request {req.rid} ({req.size[0]}x{req.size[1]}x{req.size[2]}, "
+                      f"{req.duration * 1e9:.2f} ns switching run), scaled to the {steps} steps the GPU took",
+            "seconds": round(t, 3)}
+
+
+def run_serve(args):
+    """Served solves: Poisson arrivals of relaxation and switching requests on
+    elliptical disks, continuous batching; solves/s over the whole stream and
+    p50 / p99 latency from arrival to result, next to the same requests
+    solved one at a time through the plain API (the first SERVE_PLAIN
+    requests, with their arrival times)."""
+    import mumaxplus as mp
+    from mumaxplus import serving
+    if not mp._cpp.device_available():
+        raise RuntimeError("no HIP device visible")
+    reqs = serving.disk_requests(args.seed, SERVE_N, SERVE_RATE)
+    # warm-up of every kernel
+    warm = serving.disk_requests(args.seed + 1, 4, 1e6)
+    for r in warm:
+        r.duration = 2e-12 if r.kind == "switch" else None
+        r.threshold = 1e11 if r.kind == "relax" else None
+    serving.Server(SERVE_CAP).serve(warm, realtime=False)
+    for r in warm[:1]:
+        r.solve_plain()
+    # served stream
+    server = serving.Server(SERVE_CAP, poll_steps=16)
+    t0 = time.perf_counter()
+    res = server.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    print(f"served {len(res)} requests in {wall:.2f} s", file=sys.stderr, flush=True)
+    lat = [r.latency for r in res]
+    byid = {r.rid: r for r in res}
+    # the first requests one at a time through the plain API, same arrivals
+    sub = reqs[:SERVE_PLAIN]
+    plain_lat, dms, service = [], [], []
+    t0 = time.perf_counter()
+    for r in sub:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        ts = time.perf_counter()
+        w, m = r.solve_plain()
+        service.append(time.perf_counter() - ts)
+        plain_lat.append(time.perf_counter() - t0 - r.arrival)
+        print(f"plain request {r.rid} ({r.kind}) done at {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+              flush=True)
+        if r.kind == "switch":
+            dms.append(float(np.abs(np.array(m.magnetization.average())
+                                    - np.array(byid[r.rid].magnet.magnetization.average())).max()))
+    wall_plain = time.perf_counter() - t0
+    # the same first requests served, for a like-for-like latency
+    res_sub = serving.Server(SERVE_CAP, poll_steps=16).serve(sub, realtime=True)
+    # per-kernel roofline of a profiled pass over the first requests
+    pserver = serving.Server(SERVE_CAP, poll_steps=16)
+    pserver.batch.set_profiling(True)
+    pserver.serve(sub, realtime=False)
+    rep = pserver.batch.profile_report()
+    tot_ms = sum(v[0] for v in rep.values())
+    with_bytes = {k: v for k, v in rep.items() if v[2] > 0}
+    dom = max(with_bytes, key=lambda k: with_bytes[k][0])
+    d_ms, d_n, d_b = rep[dom]
+    achieved = (d_b / d_n) / (d_ms / d_n * 1e-3)
+    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
+                   "GBps": round((v[2] / v[1]) / (v[0] / v[1] * 1e-3) / 1e9, 1) if v[2] > 0 else None,
+                   "share": round(v[0] / tot_ms, 4)} for k, v in sorted(rep.items(), key=lambda kv: -kv[1][0])}
+    steps_total = sum(r.steps for r in res)
+    first_switch = next(r for r in reqs if r.kind == "switch")
+    cpu = None if args.no_cpu_baseline else serve_cpu_baseline(first_switch, byid[first_switch.rid].steps)
+    out = {
+        "metric": "served solves/s + p50/p99 latency (Poisson stream of disk relaxation / switching requests)",
+        "value": len(res) / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": steps_total,
+        "warmup": len(warm),
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{SERVE_N} requests, Poisson arrivals at {SERVE_RATE}/s, elliptical disks "
+                               "(major axis 20-100 nm, aspect 1-3, 1-3 layers, 2 nm cells), half relaxations "
+                               "(damping-only until max torque <= 1e9 /s), half switching runs (10-200 mT "
+                               "pulses against the easy axis for 0.1-2 ns), adaptive Dormand-Prince",
+                   "requests": SERVE_N, "rate_per_s": SERVE_RATE, "capacity_cells": SERVE_CAP,
+                   "poll_steps": 16, "seed": args.seed},
+        "latency_s": {"p50": _pct(lat, 50), "p99": _pct(lat, 99), "max": max(lat)},
+        "wall_s": wall,
+        "plain": {"requests": len(sub), "value": len(sub) / wall_plain, "unit": "solves/s",
+                  "mean_service_s": float(np.mean(service)),
+                  "saturated_value": 1.0 / float(np.mean(service)),
+                  "served_over_saturated_plain": (len(res) / wall) * float(np.mean(service)),
+                  "latency_s": {"p50": _pct(plain_lat, 50), "p99": _pct(plain_lat, 99)},
+                  "served_same_requests": {"value": len(sub) / max(r.finished for r in res_sub),
+                                           "latency_s": {"p50": _pct([r.latency for r in res_sub], 50),
+                                                         "p99": _pct([r.latency for r in res_sub], 99)}},
+                  "max_abs_dm_switch_vs_served": max(dms) if dms else None},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": d_b / d_n},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def step_bytes(N, Hx, Ky, Hy, Hz, ny, nz):
     """Algorithmic HBM bytes of one DP step with FSAL (6 RHS evaluations)."""
     b_dm = 12 * N + 48.0 * Hx * ny * nz + 48.0 * Hx * Ky * nz + 24.0 * Hx * Hy * Hz + 48.0 * Hx * Ky * nz \
@@ -429,7 +574,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch"), default="ferromagnet")
+    ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve"), default="ferromagnet")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -440,6 +585,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
 
     import mumaxplus as mp
