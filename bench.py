@@ -297,15 +297,17 @@ def run_batch(args):
     batch.run_until_done(poll=steps)
     wall = time.perf_counter() - t0
     # plain sequential path: the same instances one World at a time
-    plain = [build_batch_member(sp) for sp in specs]
-    plain[0][0].timesolver.steps(1)  # warm-up of the single-magnet kernels
-    plain[0] = build_batch_member(specs[0])
-    t0 = time.perf_counter()
-    for w, _ in plain:
-        w.timesolver.steps(steps)
-    wall_seq = time.perf_counter() - t0
-    dm = max(float(np.abs(a.magnetization.get() - b.magnetization.get()).max())
-             for (_, a), (_, b) in zip(members, plain))
+    wall_seq, dm = None, None
+    if not args.no_plain:
+        plain = [build_batch_member(sp) for sp in specs]
+        plain[0][0].timesolver.steps(1)  # warm-up of the single-magnet kernels
+        plain[0] = build_batch_member(specs[0])
+        t0 = time.perf_counter()
+        for w, _ in plain:
+            w.timesolver.steps(steps)
+        wall_seq = time.perf_counter() - t0
+        dm = max(float(np.abs(a.magnetization.get() - b.magnetization.get()).max())
+                 for (_, a), (_, b) in zip(members, plain))
     # steady-state step time and per-kernel roofline on the full membership
     prof = [build_batch_member(sp) for sp in specs]
     pb = mp.Batch(span)
@@ -348,8 +350,9 @@ def run_batch(args):
                                f"{steps} Dormand-Prince steps at a fixed dt of 1e-13 s",
                    "instances": len(specs), "cells": cells, "packed_cells": span, "dt": BATCH_DT,
                    "method": "DormandPrince", "seed": args.seed},
-        "sequential": {"value": len(specs) / wall_seq, "unit": "solves/s", "wall_s": wall_seq,
-                       "speedup": wall_seq / wall, "max_abs_dm_vs_batched": dm},
+        "sequential": None if wall_seq is None else {
+            "value": len(specs) / wall_seq, "unit": "solves/s", "wall_s": wall_seq, "speedup": wall_seq / wall,
+            "max_abs_dm_vs_batched": dm},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_launch": d_b / d_n,
@@ -428,7 +431,7 @@ def run_serve(args):
     lat = [r.latency for r in res]
     byid = {r.rid: r for r in res}
     # the first requests one at a time through the plain API, same arrivals
-    sub = reqs[:SERVE_PLAIN]
+    sub = reqs[:SERVE_PLAIN] if not args.no_plain else []
     plain_lat, dms, service = [], [], []
     t0 = time.perf_counter()
     for r in sub:
@@ -446,11 +449,12 @@ def run_serve(args):
                                     - np.array(byid[r.rid].magnet.magnetization.average())).max()))
     wall_plain = time.perf_counter() - t0
     # the same first requests served, for a like-for-like latency
-    res_sub = serving.Server(SERVE_CAP, poll_steps=16).serve(sub, realtime=True)
+    res_sub = serving.Server(SERVE_CAP, poll_steps=16).serve(sub, realtime=True) if sub else []
+    sub_prof = reqs[:SERVE_PLAIN]
     # per-kernel roofline of a profiled pass over the first requests
     pserver = serving.Server(SERVE_CAP, poll_steps=16)
     pserver.batch.set_profiling(True)
-    pserver.serve(sub, realtime=False)
+    pserver.serve(sub_prof, realtime=False)
     rep = pserver.batch.profile_report()
     tot_ms = sum(v[0] for v in rep.values())
     with_bytes = {k: v for k, v in rep.items() if v[2] > 0}
@@ -484,7 +488,8 @@ def run_serve(args):
                    "poll_steps": 16, "seed": args.seed},
         "latency_s": {"p50": _pct(lat, 50), "p99": _pct(lat, 99), "max": max(lat)},
         "wall_s": wall,
-        "plain": {"requests": len(sub), "value": len(sub) / wall_plain, "unit": "solves/s",
+        "server": dict(server.stats, ms_per_batch_step=1e3 * server.stats["advance_s"] / max(server.stats["batch_steps"], 1)),
+        "plain": None if not sub else {"requests": len(sub), "value": len(sub) / wall_plain, "unit": "solves/s",
                   "mean_service_s": float(np.mean(service)),
                   "saturated_value": 1.0 / float(np.mean(service)),
                   "served_over_saturated_plain": (len(res) / wall) * float(np.mean(service)),
@@ -574,6 +579,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batch / serve: skip the one-at-a-time comparison (for profiler runs)")
     ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve"), default="ferromagnet")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])

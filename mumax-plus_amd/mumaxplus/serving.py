@@ -104,6 +104,9 @@ class Server:
     def __init__(self, capacity, poll_steps=16, method="DormandPrince"):
         self.batch = Batch(capacity, method)
         self.poll_steps = int(poll_steps)
+        # wall seconds spent admitting (building and packing requests),
+        # stepping and retiring, and the batch steps taken
+        self.stats = {"admit_s": 0.0, "advance_s": 0.0, "retire_s": 0.0, "batch_steps": 0, "polls": 0}
 
     def serve(self, requests, realtime=True):
         """Serve `requests` (sorted by arrival, seconds from the start).  With
@@ -125,6 +128,7 @@ class Server:
         while head < len(queue) or running:
             now = clock()
             # admit in arrival order while the head fits
+            ta = time.perf_counter()
             while head < len(queue) and queue[head].arrival <= now:
                 req = queue[head]
                 world, mag = req.build()
@@ -136,17 +140,25 @@ class Server:
                     break
                 running[mid] = (req, world, mag, time.perf_counter() - t0)
                 head += 1
+            self.stats["admit_s"] += time.perf_counter() - ta
             if not running:
                 wait = queue[head].arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
                 continue
+            ta = time.perf_counter()
             batch.advance(self.poll_steps)
-            for mid in batch.finished():
+            done = batch.finished()
+            tb = time.perf_counter()
+            self.stats["advance_s"] += tb - ta
+            self.stats["batch_steps"] += self.poll_steps
+            self.stats["polls"] += 1
+            for mid in done:
                 req, world, mag, admitted = running.pop(mid)
                 state = batch.member(mid)
                 batch.remove(mid)
                 results.append(Result(req, admitted, time.perf_counter() - t0, world, mag, state))
+            self.stats["retire_s"] += time.perf_counter() - tb
         return results
 
 
