@@ -97,12 +97,16 @@ int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsi
  * "enable_as_stray_field_source", "enable_as_stray_field_destination", "enable_elastodynamics"
  * (not for a sublattice, nor while a rigid strain is set).  The stray fields of the other magnets
  * enter a destination's external field (zeeman.cu:70-88; a sublattice uses its antiferromagnet's
- * destination switch). */
+ * destination switch).  Also "enable_zhang_li_torque", "enable_slonczewski_torque",
+ * "fixed_layer_on_top" (wrap_ferromagnet.cpp:35-39), and "noise_seed" (the 32-bit key of the
+ * magnet's thermal field; the reference seeds cuRAND from the clock, ferromagnet.cpp:79-81, so it
+ * has no such setting) with the read-only "noise_draws" (thermal draws taken by time steps). */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
 /* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,
  * alpha, ku1, ku2, kc1, kc2, kc3, frozen_spins, anisU, anisC1, anisC2, bias_magnetic_field,
- * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz, B1, B2
+ * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz, B1, B2,
+ * temperature, xi, Lambda, free_layer_thickness, epsilon_prime, fixed_layer, pol, jcur
  * (wrap_ferromagnet.cpp:41-70); an independent ferromagnet also has the elastic parameters C11,
  * C12, C44, eta, rho, external_body_force, rigid_norm_strain, rigid_shear_strain
  * (wrap_magnet.cpp:35-43). */
@@ -120,7 +124,8 @@ int mpa_fm_get_magnetization(mpa_ferromagnet* m, float* out);
  * wrap_magnet.cpp:61-75, wrap_fieldquantity.cpp:8-22).  q: magnetization, full_magnetization,
  * exchange_field, anisotropy_field, external_field, dmi_field, demag_field, magnetoelastic_field,
  * homogeneous_exchange_field, inhomogeneous_exchange_field, effective_field, torque, llg_torque,
- * damping_torque, magnetoelastic_force, exchange_energy_density, anisotropy_energy_density,
+ * damping_torque, spin_transfer_torque, thermal_noise (a new draw each evaluation),
+ * magnetoelastic_force, exchange_energy_density, anisotropy_energy_density,
  * zeeman_energy_density, dmi_energy_density, demag_energy_density,
  * magnetoelastic_energy_density, homogeneous_exchange_energy_density,
  * inhomogeneous_exchange_energy_density, total_energy_density; for an independent ferromagnet

@@ -365,6 +365,10 @@ int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value) {
     const std::string k = flag;
     if (k == "enable_demag") f->enable_demag = value != 0;
     else if (k == "enable_openbc") f->openbc = value != 0;
+    else if (k == "enable_zhang_li_torque") f->enable_zhangli = value != 0;
+    else if (k == "enable_slonczewski_torque") f->enable_slonc = value != 0;
+    else if (k == "fixed_layer_on_top") f->fixed_on_top = value != 0;
+    else if (k == "noise_seed") f->noise_seed = (unsigned)value;
     else if (k == "enable_elastodynamics" && f->host)
       throw Error(MPA_ERR_INVALID, "Cannot enable/disable elastodynamics for a sublattice.");
     else if (!magnet_set_flag(f, k, value)) throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
@@ -379,6 +383,11 @@ int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
     const std::string k = flag;
     if (k == "enable_demag") *out = f->enable_demag;
     else if (k == "enable_openbc") *out = f->openbc;
+    else if (k == "enable_zhang_li_torque") *out = f->enable_zhangli;
+    else if (k == "enable_slonczewski_torque") *out = f->enable_slonc;
+    else if (k == "fixed_layer_on_top") *out = f->fixed_on_top;
+    else if (k == "noise_seed") *out = (int)f->noise_seed;
+    else if (k == "noise_draws") *out = (int)f->noise_draws[0];
     else if (!magnet_get_flag(f, k, out)) throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
   });
 }

@@ -17,11 +17,16 @@
 
 #include <cmath>
 
+#include "philox.hpp"
+
 namespace mpa {
 
 namespace {
 
 constexpr float GAMMALL = 1.7595e11f;  // src/physics/constants.hpp:7
+constexpr float QE = 1.60217646e-19f;     // constants.hpp:13
+constexpr float MUB = 9.274009152e-24f;   // constants.hpp:15
+constexpr float HBAR = 1.054571817e-34f;  // constants.hpp:17
 constexpr int BLOCK = 256;
 
 __device__ __forceinline__ float harmonic_mean(float a, float b) {
@@ -340,9 +345,68 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   h = h + hme;
   h = h + hafm_nn;
   h = h + hafm_site;
+
+  // spin-transfer torque (evalSpinTransferTorque, stt.cu:152-195): Slonczewski
+  // when it is not assured zero, else Zhang-Li; zero for frozen spins
+  float3 tstt = f3(0.f, 0.f, 0.f);
+  if ((d.terms & TERM_STT) && S(d.frozen, idx) == 0.f && (mode == MODE_LLG || mode == MODE_STT)) {
+    const float3 j = V(d.jcur, idx);
+    const float al = S(d.alpha, idx), pol = S(d.pol, idx);
+    if (d.terms & TERM_SLONC) {
+      // k_Slonczewski, stt.cu:104-150
+      const float jz = j.z;
+      const float3 p = V(d.fixed, idx);
+      const float lam = S(d.lambda, idx), epsp = S(d.epsp, idx);
+      float th = S(d.fthick, idx);
+      if (!d.fixed_on_top) th = -th;
+      const bool zero = ms == 0.f || jz == 0.f || th == 0.f || (p.x == 0.f && p.y == 0.f && p.z == 0.f) ||
+                        (epsp == 0.f && (lam == 0.f || pol == 0.f));
+      if (!zero) {
+        const float B = (HBAR / QE) * jz / (ms * th);
+        const float l2 = lam * lam;
+        const float eps = pol * l2 / ((l2 + 1) + (l2 - 1) * dot3(mc, p));
+        const float3 pxm = cross3(p, mc);
+        const float3 mxpxm = cross3(mc, pxm);
+        tstt = ((B / (1 + al * al)) * GAMMALL) * ((eps + epsp * al) * mxpxm + (epsp - eps * al) * pxm);
+      }
+    } else {
+      // k_ZhangLi, stt.cu:27-102: central differences over in-grid neighbours
+      // with non-zero msat (after the mastergrid wrap)
+      const float xi = S(d.xi, idx);
+      if (!(ms == 0.f || pol == 0.f || (j.x == 0.f && j.y == 0.f && j.z == 0.f))) {
+        const float3 u = (MUB * pol / (QE * ms * (1 + xi * xi))) * j;
+        const float uc[3] = {u.x, u.y, u.z};
+        float3 hs = f3(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int dir = 0; dir < 6; ++dir) {
+          const int axis = dir >> 1;
+          const float sign = (dir & 1) ? 1.f : -1.f;
+          const int nb = neighbour(d, x, y, z, dir);
+          if (nb >= 0 && S(d.msat, nb) != 0.f) hs = hs + (sign * uc[axis] / (2 * csz[axis])) * load3(m, nb, N);
+        }
+        const float3 mxh = cross3(mc, hs);
+        const float3 mxmxh = cross3(mc, mxh);
+        tstt = (-1 / (1 + al * al)) * ((1 + xi * al) * mxmxh + (xi - al) * mxh);
+      }
+    }
+  }
+  // thermal field of this cell (k_thermalNoise, thermalnoise.cu:21-44)
+  float3 tn = f3(0.f, 0.f, 0.f);
+  if ((d.terms & TERM_THERMAL) && (d.noise || mode == MODE_NOISE) && ms != 0.f) {
+    float zn[3];
+    normal3(d.noise_seed, (unsigned)idx, d.noise_purpose, d.noise_draw, zn);
+    const float a = S(d.alpha, idx), T = S(d.temperature, idx);
+    const float amp = sqrtf(d.noise_pref * a * T / ((1 + a * a) * ms));
+    tn = f3(amp * zn[0], amp * zn[1], amp * zn[2]);
+  }
+
   float3 r;
   if (mode == MODE_FIELD) {
     r = h;
+  } else if (mode == MODE_STT) {
+    r = tstt;
+  } else if (mode == MODE_NOISE) {
+    r = tn;
   } else if (S(d.frozen, idx) != 0.f) {
     r = f3(0.f, 0.f, 0.f);
   } else if (mode == MODE_LLG) {
@@ -350,9 +414,12 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
     const float3 mxh = cross3(mc, h);
     const float3 mxmxh = cross3(mc, mxh);
     r = (-GAMMALL / (1 + al * al)) * (mxh + al * mxmxh);
+    r = r + tstt;  // evalTorque, torque.cu:14-19
   } else {
     r = -GAMMALL * cross3(mc, cross3(mc, h));
   }
+  // the stepper's noise term, added to every stage derivative (also of frozen spins)
+  if (d.noise && (mode == MODE_LLG || mode == MODE_DAMPING)) r = r + d.noise_scale * tn;
   out[idx] = r.x;
   out[N + idx] = r.y;
   out[2 * N + idx] = r.z;
@@ -543,7 +610,9 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restr
 bool params_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
            !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p && !d.B1.p && !d.B2.p &&
-           !d.rnorm.p && !d.rshear.p && !d.msat2.p && !d.afmex_cell.p && !d.afmex_nn.p && !d.latcon.p;
+           !d.rnorm.p && !d.rshear.p && !d.msat2.p && !d.afmex_cell.p && !d.afmex_nn.p && !d.latcon.p &&
+           !d.pol.p && !d.xi.p && !d.lambda.p && !d.epsp.p && !d.fthick.p && !d.jcur.p && !d.fixed.p &&
+           !d.temperature.p;
   for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p && !d.hdmi[k].p;
   return u;
 }
@@ -552,8 +621,9 @@ bool all_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
            !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p;
   for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p;
-  // sublattices and magnetoelastic coupling use the per-cell kernel only
-  return u && !d.sublattice && !(d.terms & (TERM_ME | TERM_AFM_NN | TERM_AFM_SITE));
+  // sublattices, magnetoelastic coupling, spin-transfer torques and the
+  // thermal field use the per-cell kernel only
+  return u && !d.sublattice && !(d.terms & (TERM_ME | TERM_AFM_NN | TERM_AFM_SITE | TERM_STT | TERM_THERMAL));
 }
 
 // Host-side coefficients of the uniform path, in double then rounded once.

@@ -18,7 +18,11 @@ enum TermBits : unsigned {
   TERM_ME = 1u << 6,         // magnetoelastic field
   TERM_AFM_SITE = 1u << 7,   // homogeneous (same-cell) antiferromagnetic exchange
   TERM_AFM_NN = 1u << 8,     // inhomogeneous (neighbour) antiferromagnetic exchange
+  TERM_ZHANGLI = 1u << 9,    // Zhang-Li spin-transfer torque
+  TERM_SLONC = 1u << 10,     // Slonczewski spin-transfer torque
+  TERM_THERMAL = 1u << 11,   // thermal field (temperature not zero)
 };
+constexpr unsigned TERM_STT = TERM_ZHANGLI | TERM_SLONC;
 
 // DMI tensor entries in the member order of DmiTensor (src/physics/dmitensor.hpp:12-22).
 enum DmiIdx { D_XXY = 0, D_XYZ, D_XXZ, D_YXY, D_YYZ, D_YXZ, D_ZXY, D_ZYZ, D_ZXZ, D_COUNT };
@@ -53,6 +57,19 @@ struct FMDesc {
   // stray field of the other magnets (3N, tesla), part of the external field
   // (zeeman.cu:70-88); nullptr when none
   const float* hstray;
+  // spin-transfer torques (stt.cu:27-195)
+  PS pol, xi, lambda, epsp, fthick;
+  PV jcur, fixed;
+  int fixed_on_top;
+  // thermal field (thermalnoise.cu:21-70): per cell sqrt(pref alpha T /
+  // ((1 + alpha^2) Ms)) times three normal deviates of (seed, cell, purpose,
+  // draw); added to the torque times noise_scale = 1/sqrt(dt) when `noise`
+  // (rungekutta.cpp:95-106)
+  PS temperature;
+  int noise;
+  float noise_pref, noise_scale;
+  unsigned noise_seed, noise_purpose;
+  unsigned long long noise_draw;
 };
 
 enum FieldMode {
@@ -60,6 +77,8 @@ enum FieldMode {
   MODE_LLG = 1,      // out = LLG torque of the selected terms (1/s)
   MODE_DAMPING = 2,  // out = damping-only torque (relax)
   MODE_EDENS = 3,    // out (1 comp) = energy density of the selected terms (J/m3)
+  MODE_STT = 4,      // out = spin-transfer torque (1/s)
+  MODE_NOISE = 5,    // out = thermal field quantity (thermalnoise.cu:46-69)
 };
 
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.

@@ -94,6 +94,12 @@ const ParamSpec kFmParams[] = {
     {"dmi_yxz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxy", 1, {0, 0, 0}, "J/m2"},
     {"dmi_zyz", 1, {0, 0, 0}, "J/m2"},        {"dmi_zxz", 1, {0, 0, 0}, "J/m2"},
     {"B1", 1, {0, 0, 0}, "J/m3"},             {"B2", 1, {0, 0, 0}, "J/m3", ":B1"},
+    // spin-transfer torque and temperature (ferromagnet.cpp:34-56); the
+    // free layer thickness defaults to the grid's thickness (set per magnet)
+    {"temperature", 1, {0, 0, 0}, "K"},       {"xi", 1, {0, 0, 0}, ""},
+    {"Lambda", 1, {0, 0, 0}, "", ":lambda"},  {"free_layer_thickness", 1, {0, 0, 0}, "m"},
+    {"epsilon_prime", 1, {0, 0, 0}, ""},      {"fixed_layer", 3, {0, 0, 0}, ""},
+    {"pol", 1, {0, 0, 0}, ""},                {"jcur", 3, {0, 0, 0}, "A/m2"},
 };
 // The labels of ku2, kc3 (no colon) and B2 (":B1") are the reference's own
 // parameter names (ferromagnet.cpp:29,32,61), exposed as Parameter.name.
@@ -429,6 +435,9 @@ Ferromagnet::Ferromagnet(World* w, const int size[3], const int org[3], const ui
                          Antiferromagnet* h)
     : MagnetCore(w, size, org, geometry, std::move(nm)), host(h) {
   add_params(kFmParams, (int)(sizeof(kFmParams) / sizeof(kFmParams[0])));
+  param("free_layer_thickness").uniform[0] = (float)n[2] * (float)w->cellsize[2];
+  noise_seed = 0x9e3779b9u * (unsigned)(1 + w->magnets.size() + 7 * w->afms.size()) +
+               (host ? (unsigned)(name.back() - '0') * 0x85ebca6bu : 0u);
   if (!host) add_params(kElasticParams, (int)(sizeof(kElasticParams) / sizeof(kElasticParams[0])));
   m.allocate((size_t)3 * N);
   if (!host) hdemag.allocate((size_t)3 * N);
@@ -478,6 +487,14 @@ unsigned Ferromagnet::active_terms() {
   // demag.cpp:9-11
   if (enable_demag && !msat0) t |= TERM_DEMAG;
   if (!magnetoelastic_assured_zero()) t |= TERM_ME;
+  // stt.cu:9-26: Slonczewski when it is not assured zero, else Zhang-Li
+  const bool zl0 = !enable_zhangli || msat0 || az("jcur") || az("pol");
+  const bool sl0 = !enable_slonc || msat0 || az("jcur") || az("free_layer_thickness") || az("fixed_layer") ||
+                   (az("epsilon_prime") && (az("Lambda") || az("pol")));
+  if (!sl0) t |= TERM_SLONC;
+  else if (!zl0) t |= TERM_ZHANGLI;
+  // thermalnoise.cu:17-19
+  if (!az("temperature")) t |= TERM_THERMAL;
   // afmexchange.cu:13-25
   if (host) {
     const bool ms2_0 = sister()->param("msat").assured_zero();
@@ -520,6 +537,21 @@ FMDesc Ferromagnet::desc(unsigned terms) {
   }
   d.B1 = params.at("B1").ps();
   d.B2 = params.at("B2").ps();
+  d.pol = params.at("pol").ps();
+  d.xi = params.at("xi").ps();
+  d.lambda = params.at("Lambda").ps();
+  d.epsp = params.at("epsilon_prime").ps();
+  d.fthick = params.at("free_layer_thickness").ps();
+  d.jcur = params.at("jcur").pv();
+  d.fixed = params.at("fixed_layer").pv();
+  d.fixed_on_top = fixed_on_top ? 1 : 0;
+  d.temperature = params.at("temperature").ps();
+  {
+    // 2 kB gamma / V in FP32, as the reference forms it (thermalnoise.cu:66-67)
+    const float V = d.cx * d.cy * d.cz;
+    d.noise_pref = 2.f * 1.38064852e-23f * 1.7595e11f / V;
+  }
+  d.noise_seed = noise_seed;
   if (terms & TERM_ME) {
     MagnetCore* own = owner();
     d.me_rigid = own->rigid_strain_applied() ? 1 : 0;
@@ -676,7 +708,7 @@ void MagnetCore::compute_stray(hipStream_t s, Profiler* prof) {
   }
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode) {
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode, double noise_dt) {
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
   if (!prepared) prepare(terms, s, prof);
@@ -686,7 +718,15 @@ void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode m
   if (host) bytes += 12.0 * N;
   if ((terms & TERM_ME) && !owner()->rigid_strain_applied()) bytes += 24.0 * N;
   Timed t(prof, "k_field", bytes, s);
-  launch_field(desc(terms), m.get(), demag_field(), out, mode, s);
+  FMDesc d = desc(terms);
+  if (noise_dt > 0 && (terms & TERM_THERMAL)) {
+    // k += noise / sqrt(dt) with the step's draw (rungekutta.cpp:95-106, in FP32)
+    d.noise = 1;
+    d.noise_scale = 1.f / sqrtf((float)noise_dt);
+    d.noise_purpose = 0;
+    d.noise_draw = step_draw;
+  }
+  launch_field(d, m.get(), demag_field(), out, mode, s);
 }
 
 namespace {
@@ -709,7 +749,9 @@ const QInfo kQuantities[] = {
     {"inhomogeneous_exchange_field", 3, TERM_AFM_NN, MODE_FIELD},
     {"effective_field", 3, ALL, MODE_FIELD},
     {"torque", 3, ALL, MODE_LLG},
-    {"llg_torque", 3, ALL, MODE_LLG},
+    {"llg_torque", 3, ALL & ~TERM_STT, MODE_LLG},
+    {"spin_transfer_torque", 3, TERM_STT, MODE_STT},
+    {"thermal_noise", 3, TERM_THERMAL, MODE_NOISE},
     {"damping_torque", 3, ALL, MODE_DAMPING},
     {"exchange_energy_density", 1, TERM_EXCH, MODE_EDENS},
     {"anisotropy_energy_density", 1, TERM_UNI | TERM_CUBIC, MODE_EDENS},
@@ -764,7 +806,13 @@ const float* Ferromagnet::eval_device(const std::string& q, hipStream_t s, int* 
   if (qi->mode < 0) return m.get();
   const unsigned terms = active_terms() & qi->terms;
   prepare(terms, s, nullptr);
-  launch_field(desc(terms), m.get(), demag_field(), scratch.get(), (FieldMode)qi->mode, s);
+  FMDesc d = desc(terms);
+  if (qi->mode == MODE_NOISE) {
+    // a fresh thermal field each evaluation (evalThermalNoise, thermalnoise.cu:46-69)
+    d.noise_purpose = 1;
+    d.noise_draw = noise_draws[1]++;
+  }
+  launch_field(d, m.get(), demag_field(), scratch.get(), (FieldMode)qi->mode, s);
   if (q == "total_energy_density" && !host) add_elastic_energies(scratch.get(), s);
   return scratch.get();
 }
@@ -976,7 +1024,8 @@ std::vector<StateVar> World::state_vars() {
 // All right-hand sides at the current (stage) state: strain of every magnet
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
-void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready) {
+void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready,
+                         double noise_dt) {
   for (auto& fm : magnets) {
     if (fm->el.enabled) fm->compute_strain(s);
     fm->compute_stray(s, &prof);
@@ -992,12 +1041,12 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
       Ferromagnet* fm = sv.fm;
       const FieldMode mode = relaxing_ ? MODE_DAMPING : MODE_LLG;
       if (fm->host) {
-        fm->eval_rhs(out, s, true, mode);
+        fm->eval_rhs(out, s, true, mode, noise_dt);
       } else {
         const unsigned t = fm->active_terms();
         if (t & TERM_DEMAG)
           fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
-        fm->eval_rhs(out, s, true, mode);
+        fm->eval_rhs(out, s, true, mode, noise_dt);
       }
     } else if (sv.kind == StateVar::DISP) {
       // du/dt = v (elastodynamics.cu:93-96); a stage's combination may have
@@ -1289,7 +1338,17 @@ void TimeSolver::step() {
   const int S = (int)tb.nodes.size();
   Profiler* prof = &world_->prof;
   const double t0 = time;
-  bool k0_valid = fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
+  // thermal noise: one draw per step for every magnetisation with a
+  // temperature, kept through rejected attempts and all stages (the stage
+  // executor draws it once, rungekutta.cpp:85-93); its stage derivatives
+  // hold this step's noise, so the last one is not the next step's first
+  bool noisy = false;
+  for (auto& sv : vars)
+    if (sv.kind == StateVar::MAG && (sv.fm->active_terms() & TERM_THERMAL)) {
+      sv.fm->step_draw = sv.fm->noise_draws[0]++;
+      noisy = true;
+    }
+  bool k0_valid = !noisy && fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
   // x0 = the state at the step start.  When the first stage's derivative is
   // reused, nothing reads the state before the second stage's combination
   // overwrites it, so the state array and x0 trade buffers instead of copying.
@@ -1349,7 +1408,7 @@ void TimeSolver::step() {
       time = t0 + h * tb.nodes[st];
       if (st > 0) combine(tb.A[st], st, h, st);
       if (st == 0 && k0_valid) continue;
-      world_->eval_rhs_all(vars, st, s, st > 0);
+      world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0);
       nrhs += (long long)vars.size();
     }
     time = t0 + h;
@@ -1376,9 +1435,11 @@ void TimeSolver::step() {
     for (auto& sv : vars)
       HIP_CHECK(hipMemcpyAsync(sv.x, sv.x0->get(), (size_t)3 * sv.N * sizeof(float), hipMemcpyDeviceToDevice, s));
     time = t0;
-    k0_valid = true;  // k[0] = rhs(x0, t0) was computed in this attempt
+    // k[0] = rhs(x0, t0) was computed in this attempt; with noise it holds
+    // noise / sqrt(dt) of the rejected dt and is computed again
+    k0_valid = !noisy;
   }
-  if (tb.fsal) {
+  if (tb.fsal && !noisy) {
     for (auto& sv : vars) std::swap((*sv.k)[0], (*sv.k)[S - 1]);
     fsal_valid_ = true;
     fsal_time_ = time;

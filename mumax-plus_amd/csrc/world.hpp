@@ -132,6 +132,13 @@ class Ferromagnet : public MagnetCore {
   Antiferromagnet* host;  // non-null for a sublattice
   bool enable_demag = true;
   bool openbc = false;
+  // spin-transfer torque switches (ferromagnet.cpp:40-44, 51-52)
+  bool enable_zhangli = true, enable_slonc = true, fixed_on_top = true;
+  // thermal field random streams: seed, and the draws used by time steps
+  // (purpose 0) and by evaluations of the thermal_noise quantity (purpose 1)
+  unsigned noise_seed = 0;
+  unsigned long long noise_draws[2] = {0, 0};
+  unsigned long long step_draw = 0;  // the draw of the step in progress
   double relax_threshold = -1.0;  // Ferromagnet.RelaxTorqueThreshold (wrap_ferromagnet.cpp:67)
 
   DeviceArray<float> m;       // magnetisation, 3N
@@ -154,7 +161,9 @@ class Ferromagnet : public MagnetCore {
   // out (device, 3N) = torque of the current magnetisation (the LLG torque,
   // or the damping-only torque of a relaxation).  With prepared, the demag
   // field and strain are taken as already computed.
-  void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG);
+  // noise_dt > 0: add the thermal field of the step in progress divided by
+  // sqrt(noise_dt) (the stepper's noise term).
+  void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG, double noise_dt = 0.0);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;
   void average_quantity(const std::string& q, double* out, hipStream_t s);
@@ -278,7 +287,10 @@ class World {
   std::vector<StateVar> state_vars();
   // k[stage] of every variable at the current state (all stage states set).
   // disp_ready: the displacements' k[stage] (the stage velocity) is written.
-  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false);
+  // noise_dt > 0: magnetisation right-hand sides include the thermal noise
+  // term of the step in progress (rungekutta.cpp:95-106).
+  void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false,
+                    double noise_dt = 0.0);
 
   // Field of `src` on the grid (size, origin); method 0 = FFT, 1 = direct sum.
   void stray_field_on_grid(MagnetCore* src, const int size[3], const int origin[3], int method, float* host_out);

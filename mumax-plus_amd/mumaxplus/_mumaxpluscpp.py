@@ -531,6 +531,14 @@ class _Magnet:
     def _set_flag(self, key, value):
         _call(self, "set_flag", _enc(key), int(bool(value)))
 
+    def _int_flag(self, key):
+        v = _cint()
+        _call(self, "get_flag", _enc(key), _ct.byref(v))
+        return int(v.value)
+
+    def _set_int_flag(self, key, value):
+        _call(self, "set_flag", _enc(key), int(value))
+
     @property
     def enable_as_stray_field_source(self):
         return self._flag("enable_as_stray_field_source")
@@ -584,8 +592,9 @@ class Ferromagnet(_Magnet):
     """Handle of a ferromagnet or sublattice owned by a World (wrap_ferromagnet.cpp:28-73)."""
 
     _pfx = "fm"
-    _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "B1", "B2")
-    _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field")
+    _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "B1", "B2",
+               "temperature", "xi", "Lambda", "free_layer_thickness", "epsilon_prime", "pol")
+    _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field", "fixed_layer", "jcur")
 
     def __init__(self, handle, world):
         super().__init__(handle, world)
@@ -615,6 +624,43 @@ class Ferromagnet(_Magnet):
     @enable_openbc.setter
     def enable_openbc(self, value):
         self._set_flag("enable_openbc", value)
+
+    @property
+    def enable_zhang_li_torque(self):
+        return self._flag("enable_zhang_li_torque")
+
+    @enable_zhang_li_torque.setter
+    def enable_zhang_li_torque(self, value):
+        self._set_flag("enable_zhang_li_torque", value)
+
+    @property
+    def enable_slonczewski_torque(self):
+        return self._flag("enable_slonczewski_torque")
+
+    @enable_slonczewski_torque.setter
+    def enable_slonczewski_torque(self, value):
+        self._set_flag("enable_slonczewski_torque", value)
+
+    @property
+    def fixed_layer_on_top(self):
+        return self._flag("fixed_layer_on_top")
+
+    @fixed_layer_on_top.setter
+    def fixed_layer_on_top(self, value):
+        self._set_flag("fixed_layer_on_top", value)
+
+    @property
+    def noise_seed(self):
+        return self._int_flag("noise_seed") & 0xFFFFFFFF
+
+    @noise_seed.setter
+    def noise_seed(self, value):
+        v = int(value) & 0xFFFFFFFF
+        self._set_int_flag("noise_seed", v - (1 << 32) if v >= (1 << 31) else v)
+
+    @property
+    def noise_draws(self):
+        return self._int_flag("noise_draws")
 
     @property
     def is_sublattice(self):

@@ -8,11 +8,13 @@ from .dmitensor import DmiTensor
 from .magnet import Magnet
 from .variable import Variable
 
-_SCALAR_PARAMS = ("msat", "alpha", "aex", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins")
-_VECTOR_PARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field")
+_SCALAR_PARAMS = ("msat", "alpha", "aex", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "xi", "Lambda",
+                  "free_layer_thickness", "epsilon_prime", "pol")
+_VECTOR_PARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field", "fixed_layer", "jcur")
 _FIELD_QUANTITIES = ("exchange_field", "anisotropy_field", "external_field", "dmi_field", "demag_field",
                      "magnetoelastic_field", "homogeneous_exchange_field", "inhomogeneous_exchange_field",
-                     "effective_field", "torque", "llg_torque", "damping_torque", "magnetoelastic_force",
+                     "effective_field", "torque", "llg_torque", "damping_torque", "spin_transfer_torque",
+                     "thermal_noise", "magnetoelastic_force",
                      "full_magnetization", "exchange_energy_density", "anisotropy_energy_density",
                      "zeeman_energy_density", "dmi_energy_density", "demag_energy_density",
                      "magnetoelastic_energy_density", "homogeneous_exchange_energy_density",
@@ -75,6 +77,67 @@ class Ferromagnet(Magnet):
     @property
     def dmi_tensor(self):
         return DmiTensor(self._impl.dmi_tensor)
+
+    # ------------------------------------------ spin-transfer torque, noise
+    @property
+    def enable_zhang_li_torque(self):
+        """Enable the Zhang-Li spin-transfer torque (default True)."""
+        return self._impl.enable_zhang_li_torque
+
+    @enable_zhang_li_torque.setter
+    def enable_zhang_li_torque(self, value):
+        self._impl.enable_zhang_li_torque = value
+
+    @property
+    def enable_slonczewski_torque(self):
+        """Enable the Slonczewski spin-transfer torque (default True)."""
+        return self._impl.enable_slonczewski_torque
+
+    @enable_slonczewski_torque.setter
+    def enable_slonczewski_torque(self, value):
+        self._impl.enable_slonczewski_torque = value
+
+    @property
+    def fixed_layer_on_top(self):
+        """Whether the Slonczewski fixed layer is above the free layer (default True)."""
+        return self._impl.fixed_layer_on_top
+
+    @fixed_layer_on_top.setter
+    def fixed_layer_on_top(self, value):
+        if not isinstance(value, bool):
+            raise TypeError("fixed_layer_on_top should be a boolean")
+        self._impl.fixed_layer_on_top = value
+
+    @property
+    def temperature(self):
+        """Temperature (K); a non-zero value adds the thermal field to the time steps."""
+        from .parameter import Parameter
+        return Parameter(getattr(self._impl, "temperature"))
+
+    @temperature.setter
+    def temperature(self, value):
+        # the reference's check for its random generator (ferromagnet.py:536-542),
+        # kept so that the same scripts fail the same way
+        if self.grid.ncells % 2:
+            raise ValueError("The CUDA random number generator used to generate"
+                             + " a random noise field only works for an even"
+                             + " number of grid cells.\n"
+                             + "The used number of grid cells is {}.".format(self.grid.ncells))
+        self.temperature.set(value)
+
+    @property
+    def noise_seed(self):
+        """32-bit key of this magnet's thermal field (reproducible random streams)."""
+        return self._impl.noise_seed
+
+    @noise_seed.setter
+    def noise_seed(self, value):
+        self._impl.noise_seed = value
+
+    @property
+    def noise_draws(self):
+        """Thermal draws taken by time steps so far (one per step)."""
+        return self._impl.noise_draws
 
     @property
     def RelaxTorqueThreshold(self):

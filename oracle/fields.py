@@ -18,6 +18,14 @@ Restated functions (reference file:line):
 * ``damping_torque``    k_dampingtorque, src/physics/torque.cu:58-77
 * energy densities      k_energyDensity (src/physics/energy.cu:18-50) and
                         k_unianisotropyEnergyDensity (anisotropy.cu:142-180)
+* ``zhang_li_torque``, ``slonczewski_torque``, ``spin_transfer_torque``
+                        k_ZhangLi, k_Slonczewski, evalSpinTransferTorque,
+                        src/physics/stt.cu:9-188 (pinned by tests/test_oracle_stt.py:
+                        the central-difference closed form on a spiral, zero torque
+                        for uniform m, torque perpendicular to m, the empty-layer
+                        test of test/test_free_layer_thickness.py, the sign flip of
+                        the fixed-layer position; the GPU test of the domain-wall
+                        drift -u t checks the product's signs physically)
 
 Constants: GAMMALL = 1.7595e11 rad/(T s) (src/physics/constants.hpp:20).
 
@@ -280,3 +288,102 @@ def integrate(mag, density):
     avg = density[geom].sum() / max(int(geom.sum()), 1)
     V = mag.cellsize[0] * mag.cellsize[1] * mag.cellsize[2]
     return ncells * avg * V
+
+
+# ------------------------------------------------------------ spin transfer
+QE = 1.60217646e-19  # src/physics/constants.hpp:13
+MUB = 9.2740091523e-24
+HBAR = 1.054571817e-34
+
+
+def zhang_li_assured_zero(mag):
+    """ZhangLiSTTAssuredZero (src/physics/stt.cu:13-16)."""
+    return (not mag.enable_zhangli or mag.assured_zero("msat") or mag.assured_zero("jcur")
+            or mag.assured_zero("pol"))
+
+
+def slonczewski_assured_zero(mag):
+    """SlonczewskiSTTAssuredZero (src/physics/stt.cu:18-26)."""
+    return (not mag.enable_slonc or mag.assured_zero("msat") or mag.assured_zero("jcur")
+            or mag.assured_zero("free_layer_thickness") or mag.assured_zero("fixed_layer")
+            or (mag.assured_zero("epsilon_prime") and (mag.assured_zero("Lambda") or mag.assured_zero("pol"))))
+
+
+def zhang_li_torque(mag, m):
+    """k_ZhangLi (src/physics/stt.cu:27-102): u = muB P j / (e Ms (1 + xi^2)),
+    h = sum over axes of u_a times the central difference of m over in-grid
+    neighbours with non-zero msat (after the mastergrid wrap), then
+    t = -1/(1+alpha^2) ((1 + xi alpha) m x (m x h) + (xi - alpha) m x h)."""
+    ms, pol, xi, al = mag.val("msat"), mag.val("pol"), mag.val("xi"), mag.val("alpha")
+    j = mag.vec("jcur")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = MUB * pol / (QE * ms * (1 + xi * xi)) * j
+    u = np.where((ms != 0)[None], u, 0.0)
+    h = np.zeros((3,) + mag.shape)
+    for d in DIRS:
+        axis = 0 if d[0] else (1 if d[1] else 2)
+        sign = d[0] + d[1] + d[2]
+        in_grid, idx = _in_grid_neighbor(mag, d)
+        ok = in_grid & (mag.val("msat")[idx] != 0)
+        mn = m[(slice(None),) + idx]
+        h += np.where(ok[None], sign * u[axis][None] * mn / (2 * mag.cellsize[axis]), 0.0)
+    mxh = cross(m, h)
+    mxmxh = cross(m, mxh)
+    t = (-1 / (1 + al * al)) * ((1 + xi * al) * mxmxh + (xi - al) * mxh)
+    zero = (ms == 0) | (pol == 0) | np.all(j == 0, axis=0)
+    ok = mag.geom_mask() & (mag.val("frozen_spins") == 0) & ~zero
+    return np.where(ok[None], t, 0.0)
+
+
+def slonczewski_torque(mag, m):
+    """k_Slonczewski (src/physics/stt.cu:104-150): B = (hbar/e) jz / (Ms d),
+    eps = P L^2 / ((L^2 + 1) + (L^2 - 1) m.p),
+    t = gamma B / (1+alpha^2) ((eps + eps' alpha) m x (p x m) + (eps' - eps alpha) p x m);
+    d changes sign when the fixed layer is at the bottom."""
+    ms, pol, al = mag.val("msat"), mag.val("pol"), mag.val("alpha")
+    lam, epsp = mag.val("Lambda"), mag.val("epsilon_prime")
+    th = mag.val("free_layer_thickness") * (1.0 if mag.fixed_on_top else -1.0)
+    jz = mag.vec("jcur")[2]
+    p = mag.vec("fixed_layer")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        B = (HBAR / QE) * jz / (ms * th)
+        l2 = lam * lam
+        eps = pol * l2 / ((l2 + 1) + (l2 - 1) * dot(m, p))
+    pxm = cross(p, m)
+    mxpxm = cross(m, pxm)
+    t = (B / (1 + al * al) * GAMMALL) * ((eps + epsp * al) * mxpxm + (epsp - eps * al) * pxm)
+    zero = ((ms == 0) | (jz == 0) | (th == 0) | np.all(p == 0, axis=0)
+            | ((epsp == 0) & ((lam == 0) | (pol == 0))))
+    ok = mag.geom_mask() & (mag.val("frozen_spins") == 0) & ~zero
+    return np.where(ok[None], np.nan_to_num(t), 0.0)
+
+
+def spin_transfer_torque(mag, m):
+    """evalSpinTransferTorque (src/physics/stt.cu:152-188): Slonczewski when it
+    is not assured zero, else Zhang-Li, else zero."""
+    if not slonczewski_assured_zero(mag):
+        return slonczewski_torque(mag, m)
+    if not zhang_li_assured_zero(mag):
+        return zhang_li_torque(mag, m)
+    return np.zeros((3,) + mag.shape)
+
+
+def _in_grid_neighbor(mag, d):
+    """(in-grid mask after the wrap, clipped index) of the neighbour in direction d."""
+    nz, ny, nx = mag.shape
+    Z, Y, X = np.indices(mag.shape)
+    loc = [X, Y, Z]
+    n = [nx, ny, nz]
+    glob = []
+    for a in range(3):
+        g = loc[a] + mag.origin[a] + d[a]
+        if mag.mastergrid[a] > 0:
+            mo = mag.mastergrid_origin[a]
+            g = mo + np.mod(g - mo, mag.mastergrid[a])
+        glob.append(g - mag.origin[a])
+    in_grid = np.ones(mag.shape, dtype=bool)
+    clipped = []
+    for a in range(3):
+        in_grid &= (glob[a] >= 0) & (glob[a] < n[a])
+        clipped.append(np.clip(glob[a], 0, n[a] - 1))
+    return in_grid, (clipped[2], clipped[1], clipped[0])

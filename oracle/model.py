@@ -20,6 +20,7 @@ from . import demag as _demag
 from . import elastic as _elastic
 from . import fields as _fields
 from . import rk as _rk
+from . import thermal as _thermal
 
 DMI_KEYS = ("xxy", "xyz", "xxz", "yxy", "yyz", "yxz", "zxy", "zyz", "zxz")
 
@@ -28,11 +29,14 @@ SCALAR_DEFAULTS = {"msat": 1.0, "aex": 0.0, "alpha": 0.0, "ku1": 0.0, "ku2": 0.0
                    # magnetoelastic constants (ferromagnet.cpp:24-61)
                    "B1": 0.0, "B2": 0.0,
                    # elastic parameters of an independent magnet (magnet.cpp:26-34)
-                   "C11": 0.0, "C12": 0.0, "C44": 0.0, "eta": 0.0, "rho": 1.0}
+                   "C11": 0.0, "C12": 0.0, "C44": 0.0, "eta": 0.0, "rho": 1.0,
+                   # temperature and spin-transfer torque (ferromagnet.cpp:34-56)
+                   "temperature": 0.0, "xi": 0.0, "Lambda": 0.0, "epsilon_prime": 0.0, "pol": 0.0}
 VECTOR_DEFAULTS = {"anisU": (0.0, 0.0, 0.0), "anisC1": (0.0, 0.0, 0.0),
                    "anisC2": (0.0, 0.0, 0.0), "bias": (0.0, 0.0, 0.0),
                    "external_body_force": (0.0, 0.0, 0.0), "rigid_norm_strain": (0.0, 0.0, 0.0),
-                   "rigid_shear_strain": (0.0, 0.0, 0.0)}
+                   "rigid_shear_strain": (0.0, 0.0, 0.0), "fixed_layer": (0.0, 0.0, 0.0),
+                   "jcur": (0.0, 0.0, 0.0)}
 
 
 class _Params:
@@ -89,6 +93,17 @@ class Ferromagnet(_Params):
         self.dmi = {k: 0.0 for k in DMI_KEYS}
         self.enable_demag = True
         self.openbc = False
+        # free layer thickness defaults to the grid's thickness, an FP32 product
+        # as in the reference (ferromagnet.cpp:38-39)
+        self.params["free_layer_thickness"] = float(np.float32(nz) * np.float32(self.cellsize[2]))
+        self.enable_zhangli = True
+        self.enable_slonc = True
+        self.fixed_on_top = True
+        # thermal field streams (see oracle.thermal): seed, draws of the time
+        # steps (purpose 0) and of the thermal_noise quantity (purpose 1)
+        self.noise_seed = 0
+        self.noise_draws = [0, 0]
+        self.step_draw = 0
         self.m = np.zeros((3,) + self.shape)
         self.m[0] = 1.0
         self._plan = None
@@ -247,6 +262,9 @@ class World:
         # damping-only torque are the equations, and whether (u, v) stay in
         self.relax_fms = None
         self.relax_elastic = False
+        # set by the time solver while a step with thermal noise runs: the dt
+        # the noise term is divided by the square root of
+        self.noise_dt = None
 
     def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
         mag = Ferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
@@ -426,8 +444,15 @@ class World:
                 if obj.host is not None:
                     m2 = m_of(obj.host.sub2 if obj is obj.host.sub1 else obj.host.sub1)
                 h = self.effective_field(obj, m_of(obj), m2, uv_of(own, "u"), m_of)
-                torque = _fields.damping_torque if self.relax_fms is not None else _fields.llg_torque
-                out.append(torque(obj, m_of(obj), h))
+                if self.relax_fms is not None:
+                    t = _fields.damping_torque(obj, m_of(obj), h)
+                else:
+                    # evalTorque (torque.cu:14-19): LLG plus spin-transfer torque
+                    t = _fields.llg_torque(obj, m_of(obj), h) + _fields.spin_transfer_torque(obj, m_of(obj))
+                if self.noise_dt is not None and not obj.assured_zero("temperature"):
+                    # the stepper's noise term (rungekutta.cpp:95-106)
+                    t = t + _thermal.thermal_field(obj, 0, obj.step_draw) / np.sqrt(self.noise_dt)
+                out.append(t)
             elif kind == "u":
                 out.append(uv_of(obj, "v"))
             else:

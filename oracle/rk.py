@@ -136,8 +136,18 @@ class TimeSolver:
             return
         geoms = self.system.geom_masks()
         t0 = self.time
+        # one thermal draw per step for every magnetisation with a temperature,
+        # kept through the stages and rejected attempts (rungekutta.cpp:85-93)
+        noisy = False
+        for kind, obj in getattr(self.system, "_equations", lambda: [])():
+            if kind == "m" and not obj.assured_zero("temperature"):
+                obj.step_draw = obj.noise_draws[0]
+                obj.noise_draws[0] += 1
+                noisy = True
         while True:
             dt = self.dt
+            if noisy:
+                self.system.noise_dt = dt
             k = []
             for s in range(S):
                 self.time = t0 + dt * nodes[s]
@@ -180,6 +190,8 @@ class TimeSolver:
             self.nrejected += 1
             self.system.set_states(x0)
             self.time = t0
+        if noisy:
+            self.system.noise_dt = None
 
     # ------------------------------------------ Python-wrapper level calls
     def assure_sensible_timestep(self):
