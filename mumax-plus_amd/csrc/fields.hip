@@ -17,6 +17,8 @@
 
 #include <cmath>
 
+#include "field_cell.hpp"
+
 #include "philox.hpp"
 
 namespace mpa {
@@ -35,42 +37,9 @@ __device__ __forceinline__ float harmonic_mean(float a, float b) {
   return 2 * a * b / (a + b);
 }
 
-__device__ __forceinline__ float3 load3(const float* __restrict__ v, int i, int n) {
-  return f3(v[i], v[n + i], v[2 * n + i]);
-}
-
 __device__ __forceinline__ float3 normalized3(float3 v) {
   const float n = sqrtf(dot3(v, v));
   return v / n;
-}
-
-// getGamma (dmi.hpp:53-69) with the absolute normal n along `axis`.
-__device__ __forceinline__ float3 dmi_gamma(const float D[D_COUNT], int axis, float3 m) {
-  const float nx = axis == 0, ny = axis == 1, nz = axis == 2;
-  return f3(-D[D_XXY] * nx * m.y - D[D_XXZ] * nx * m.z - D[D_YXZ] * ny * m.z - D[D_ZXY] * nz * m.y -
-                D[D_YXY] * ny * m.y - D[D_ZXZ] * nz * m.z,
-            D[D_XXY] * nx * m.x - D[D_ZYZ] * nz * m.z + D[D_YXY] * ny * m.x - D[D_XYZ] * nx * m.z +
-                D[D_ZXY] * nz * m.x - D[D_YYZ] * ny * m.z,
-            D[D_XXZ] * nx * m.x + D[D_YYZ] * ny * m.y + D[D_XYZ] * nx * m.y + D[D_YXZ] * ny * m.x +
-                D[D_ZXZ] * nz * m.x + D[D_ZYZ] * nz * m.y);
-}
-
-// Local index of the neighbour in direction dir (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
-// after the mastergrid wrap, or -1 when it falls outside the magnet grid.
-__device__ __forceinline__ int neighbour(const FMDesc& d, int x, int y, int z, int dir) {
-  const int axis = dir >> 1;
-  const int step = (dir & 1) ? 1 : -1;
-  int c[3] = {x, y, z};
-  const int o[3] = {d.ox, d.oy, d.oz};
-  const int mg[3] = {d.mgx, d.mgy, d.mgz};
-  const int mo[3] = {d.mox, d.moy, d.moz};
-  const int n[3] = {d.nx, d.ny, d.nz};
-  int g = o[axis] + c[axis] + step;
-  if (mg[axis] > 0) g = mo[axis] + wrap_near(g - mo[axis], mg[axis]);
-  const int l = g - o[axis];
-  if (l < 0 || l >= n[axis]) return -1;
-  c[axis] = l;
-  return (c[2] * d.ny + c[1]) * d.nx + c[0];
 }
 
 // UNI: every parameter is uniform, so each lookup is the scalar and the
@@ -425,145 +394,6 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   out[2 * N + idx] = r.z;
 }
 
-// ---------------------------------------------------------------------------
-// Uniform-parameter path.  When every material parameter is one value for
-// the whole magnet, the per-neighbour harmonic means reduce to that value
-// and all divisions fold into host-side coefficients:
-//   exchange   2A/(Ms c^2) (m_n - m);  Neumann boundary: s Gamma/(Ms c)
-//   DMI        s (D/(Ms c)) . m_n      with m_n = m + Gamma s c/(2A) at a
-//                                      Neumann boundary (s = +-1 the side)
-//   anisotropy (2K1/Ms mu + 4K2/Ms mu^3) u,  cubic with -2Kc1/Ms, ...
-// The branch structure (open boundaries, geometry holes, msat = 0 never
-// occurs since msat is uniform and non-zero here) is the same as above.
-struct FMUniform {
-  float kex[3];      // 2A/(Ms c^2) per axis
-  float inv_msc[3];  // 1/(Ms c)
-  float dxz[3], dxy[3], dyz[3];  // D_(axis,xz|xy|yz)/(Ms c)
-  float D[D_COUNT];  // raw tensor for the boundary term Gamma
-  float ncoef[3];    // c/(2A) (Neumann displacement), 0 when A == 0
-  float u[3], ku1, ku2;
-  float c1[3], c2[3], c3[3], kc1, kc2, kc3;
-  float hext[3];
-  float llg, alpha;  // -gamma/(1+alpha^2)
-  int has_ex, has_dmi, uni, cubic, demag, frozen, openbc_dmi, openbc;
-  // one layer (Nz == 1) and no DMI entries along z: the exchange and DMI
-  // terms of the two missing (or self-wrapped) z neighbours cancel
-  int zflat;
-};
-
-__device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m) { return dmi_gamma(u.D, axis, m); }
-
-// One cell (x, y, z) of the uniform path; m, hdemag and out have component
-// stride N (the magnet's cell count, or the packed length of a batch).
-__device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUniform& u, const float* __restrict__ m,
-                                                   const float* __restrict__ hdemag, float* __restrict__ out,
-                                                   int mode, int x, int y, int z, int N) {
-  const int idx = (z * d.ny + y) * d.nx + x;
-  if (d.geom && !d.geom[idx]) {
-    out[idx] = 0.f;
-    out[N + idx] = 0.f;
-    out[2 * N + idx] = 0.f;
-    return;
-  }
-  const float3 mc = load3(m, idx, N);
-  float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
-  // DMI field of neighbour value mn along `axis`, before the direction sign
-  auto dmi_of = [&](int axis, float3 mn) {
-    return f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
-              -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
-  };
-  // Missing or outside-geometry neighbour: Neumann with the DMI correction,
-  // or the open boundary.
-  auto boundary = [&](int axis, float s) {
-    const float3 gam = gamma_u(u, axis, mc);
-    if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
-    if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
-  };
-  if (u.has_ex || u.has_dmi) {
-    // Every neighbour value is loaded before any is used (a missing
-    // neighbour reads the cell itself and is not used), so the six loads are
-    // in flight together; the wrap, geometry and boundary rules then act on
-    // registers.  A single layer without z-axis DMI entries skips z: its two
-    // terms cancel.
-    const int ndir = u.zflat ? 4 : 6;
-    int nbi[6];
-    bool ok[6];
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-      if (dir < ndir) {
-        const int nb = neighbour(d, x, y, z, dir);
-        ok[dir] = nb >= 0 && (!d.geom || d.geom[nb]);
-        nbi[dir] = nb >= 0 ? nb : idx;
-      } else {
-        ok[dir] = false;
-        nbi[dir] = idx;
-      }
-    }
-    float3 mn[6];
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir)
-      if (dir < ndir) mn[dir] = load3(m, nbi[dir], N);
-#pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      if (2 * axis >= ndir) break;
-      const float3 mm = mn[2 * axis], mp = mn[2 * axis + 1];
-      if (ok[2 * axis] && ok[2 * axis + 1]) {
-        // both neighbours plain: kex (m+ + m- - 2m) and D (m+ - m-)
-        if (u.has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
-        if (u.has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
-      } else {
-        if (ok[2 * axis]) {
-          if (u.has_ex) hex = hex + u.kex[axis] * (mm - mc);
-          if (u.has_dmi) hdm = hdm - dmi_of(axis, mm);
-        } else {
-          boundary(axis, -1.f);
-        }
-        if (ok[2 * axis + 1]) {
-          if (u.has_ex) hex = hex + u.kex[axis] * (mp - mc);
-          if (u.has_dmi) hdm = hdm + dmi_of(axis, mp);
-        } else {
-          boundary(axis, 1.f);
-        }
-      }
-    }
-  }
-  float3 han = f3(0.f, 0.f, 0.f);
-  if (u.uni) {
-    const float3 ua = f3(u.u[0], u.u[1], u.u[2]);
-    const float mu = dot3(mc, ua);
-    han = (u.ku1 * mu + u.ku2 * mu * mu * mu) * ua;
-  } else if (u.cubic) {
-    const float3 c1 = f3(u.c1[0], u.c1[1], u.c1[2]), c2 = f3(u.c2[0], u.c2[1], u.c2[2]),
-                 c3 = f3(u.c3[0], u.c3[1], u.c3[2]);
-    const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
-    const float s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
-    han = u.kc1 * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
-          u.kc2 * ((s2 * s3) * a1 * c1 + (s1 * s3) * a2 * c2 + (s1 * s2) * a3 * c3) +
-          u.kc3 * ((s2 * s2 + s3 * s3) * (s1 * a1) * c1 + (s1 * s1 + s3 * s3) * (s2 * a2) * c2 +
-                   (s1 * s1 + s2 * s2) * (s3 * a3) * c3);
-  }
-  float3 h = hex;
-  h = h + han;
-  h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
-  if (d.hstray) h = h + load3(d.hstray, idx, d.ncell);
-  h = h + hdm;
-  if (u.demag) h = h + load3(hdemag, idx, N);
-  float3 r;
-  if (mode == MODE_FIELD) {
-    r = h;
-  } else if (u.frozen) {
-    r = f3(0.f, 0.f, 0.f);
-  } else if (mode == MODE_LLG) {
-    const float3 mxh = cross3(mc, h);
-    r = u.llg * (mxh + u.alpha * cross3(mc, mxh));
-  } else {
-    r = -GAMMALL * cross3(mc, cross3(mc, h));
-  }
-  out[idx] = r.x;
-  out[N + idx] = r.y;
-  out[2 * N + idx] = r.z;
-}
-
 __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                          const float* __restrict__ hdemag, float* __restrict__ out,
                                                          int mode) {
@@ -579,7 +409,8 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   const int y = (rest % nby) * blockDim.y + threadIdx.y;
   const int z = rest / nby;
   if (x >= d.nx || y >= d.ny) return;
-  field_uniform_cell(d, u, m, hdemag, out, mode, x, y, z, d.ncell);
+  const int N = d.ncell;
+  field_uniform_cell(d, u, m, [&](int i) { return load3(hdemag, i, N); }, out, mode, x, y, z, N);
 }
 
 // A member of a batched launch: its description, coefficients, first cell
@@ -603,7 +434,8 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restr
   const int nx = I.d.nx, ny = I.d.ny;
   if (j >= I.d.ncell) return;
   const int x = j % nx, yz = j / nx, y = yz % ny, z = yz / ny;
-  field_uniform_cell(I.d, I.u, X + I.off, HD + I.off, OUT + I.off, I.mode, x, y, z, cs);
+  const float* hd = HD + I.off;
+  field_uniform_cell(I.d, I.u, X + I.off, [&](int i) { return load3(hd, i, cs); }, OUT + I.off, I.mode, x, y, z, cs);
 }
 
 // Every parameter the per-cell kernel reads is uniform.
@@ -616,6 +448,9 @@ bool params_uniform(const FMDesc& d) {
   for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p && !d.hdmi[k].p;
   return u;
 }
+
+
+}  // namespace
 
 bool all_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
@@ -683,7 +518,6 @@ FMUniform make_uniform(const FMDesc& d) {
   return u;
 }
 
-}  // namespace
 
 bool field_batchable(const FMDesc& d) { return all_uniform(d) && !d.hstray; }
 
