@@ -47,7 +47,20 @@ namespace {
 constexpr double MU0 = 1.25663706212e-6;  // src/physics/constants.hpp:22
 constexpr int BLOCK = 256;
 constexpr int MAXL2 = 11;  // longest FFT line supported per axis: 2048 points
-constexpr size_t LDS_LIMIT = 65536;
+constexpr size_t LDS_LIMIT = 163840;  // LDS a gfx950 workgroup may allocate
+// Block sizes of the line passes (threads), chosen by timing on MI355X.
+#ifndef MPA_POINT_THREADS
+#define MPA_POINT_THREADS 64
+#endif
+#ifndef MPA_XFWD_THREADS
+#define MPA_XFWD_THREADS 512
+#endif
+#ifndef MPA_XINV_THREADS
+#define MPA_XINV_THREADS 512
+#endif
+#ifndef MPA_YINV_THREADS
+#define MPA_YINV_THREADS 512
+#endif
 
 int ilog2(int v) {
   int l = 0;
@@ -109,7 +122,7 @@ struct Line {
 template <int L2>
 struct XfwdCfg {
   using C = Line<L2>;
-  static constexpr int PB = clampc(512 / C::T, 1, 32);  // line pairs per block
+  static constexpr int PB = clampc(MPA_XFWD_THREADS / C::T, 1, 32);  // line pairs per block
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
   static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (PB < 16 ? PB : 16));
@@ -201,12 +214,26 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
 template <int L2>
 struct YCfg {
   using C = Line<L2>;
-  static constexpr int LB = clampc(512 / C::T, 1, 32);  // lines (c, kx, z) per block, z fastest
+  // lines (c, kx, z) per block, z fastest: at least 16 (128-byte runs of z on
+  // the strided B side) while a block stays within 1024 threads
+  static constexpr int LB = C::T >= 64 ? clampc(1024 / C::T, 1, 16) : clampc(512 / C::T, 1, 32);
+  static constexpr int L2LB = ilog2c(LB);
+  static constexpr int THREADS = LB * C::T;
+  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (LB < 32 ? LB : 32));
+  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
+  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
+};
+
+// The y inverse reads B with LB consecutive z per (c, kx) and writes A lines.
+template <int L2>
+struct YinvCfg {
+  using C = Line<L2>;
+  static constexpr int LB = clampc(MPA_YINV_THREADS / C::T, 1, 32);
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
   static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / LB);
   static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
-  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
+  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yinv LDS");
 };
 
 template <int L2>
@@ -275,7 +302,7 @@ __global__ void __launch_bounds__(YCfg<L2>::THREADS)
   yfwd_body<L2>(d, A, B, tw, xcd_remap(blockIdx.x, gridDim.x));
 }
 
-template <int L2, class K = YCfg<L2>>
+template <int L2, class K = YinvCfg<L2>>
 __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict__ B, float2* __restrict__ A,
                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
@@ -310,7 +337,7 @@ __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict_
 }
 
 template <int L2>
-__global__ void __launch_bounds__(YCfg<L2>::THREADS)
+__global__ void __launch_bounds__(YinvCfg<L2>::THREADS)
     k_yinv(DD d, const float2* __restrict__ B, float2* __restrict__ A, const float2* __restrict__ tw) {
   yinv_body<L2>(d, B, A, tw, xcd_remap(blockIdx.x, gridDim.x));
 }
@@ -346,7 +373,7 @@ __device__ __forceinline__ void kernel_multiply(const float* __restrict__ e, flo
 template <int L2>
 struct PointCfg {
   using C = Line<L2>;
-  static constexpr int PT = clampc(256 / C::T, 1, 256);  // spectral points per block
+  static constexpr int PT = clampc(MPA_POINT_THREADS / C::T, 1, 256);  // spectral points per block
   static constexpr int THREADS = PT * C::T;
   static constexpr int LP = C::stride(C::fft_target());
   static constexpr int NBUF = 2;  // exchange buffers per line for the three components
@@ -354,16 +381,27 @@ struct PointCfg {
   static_assert(SHM <= LDS_LIMIT - 1024, "point pass LDS");
 };
 
+// Batched launches: 256 threads for every line length.
+template <int L2>
+struct PointCfgB {
+  using C = Line<L2>;
+  static constexpr int PT = clampc(256 / C::T, 1, 256);  // spectral points per block
+  static constexpr int THREADS = PT * C::T;
+  static constexpr int LP = C::stride(C::fft_target());
+  static constexpr int NBUF = 2;  // exchange buffers per line for the three components
+  static constexpr size_t SHM = (size_t)NBUF * PT * LP * sizeof(float2);
+  static_assert(SHM <= LDS_LIMIT - 1024, "batched point pass LDS");
+};
+
 // PT spectral points (kx, ky) per block; each thread holds the three
 // components of its 8 z values, so the kernel multiply between the forward
 // and the inverse transform stays in registers.  Adjacent thread groups take
 // the mirror points ky and Ky - ky (0 pairs with Ky/2), which read the same
 // row of the octant spectrum, so each row comes from HBM once.
-template <int L2>
+template <int L2, class K = PointCfg<L2>>
 __device__ __forceinline__ void zpass_body(const DD& d, float2* __restrict__ B, const float* __restrict__ Tk,
                                            const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
   const int npt = d.Hx * d.Ky;
   const size_t cs = (size_t)npt;  // points per component
@@ -420,11 +458,10 @@ __global__ void __launch_bounds__(PointCfg<L2>::THREADS)
 }
 
 // Single layer: forward y, kernel multiply, inverse y; PT points kx per block.
-template <int L2>
+template <int L2, class K = PointCfg<L2>>
 __device__ __forceinline__ void ypass2d_body(const DD& d, float2* __restrict__ A, const float* __restrict__ Tk,
                                              const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
-  using K = PointCfg<L2>;
   extern __shared__ float2 lds[];
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   const int kx = blk * K::PT + g;
@@ -478,7 +515,7 @@ template <int L2>
 struct XinvCfg {
   using C = Line<L2>;
   // line pairs per block: 16 lines give 128-byte runs on the strided A side
-  static constexpr int PB = clampc(512 / C::T, 1, 32);
+  static constexpr int PB = clampc(MPA_XINV_THREADS / C::T, 1, 32);
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
   static constexpr int XS = C::H + 1;  // stride of a staged half spectrum
@@ -685,7 +722,7 @@ struct RunYfwd {
 template <int L2>
 struct RunYinv {
   static void run(const DD& d, const float2* B, float2* A, const float2* tw, hipStream_t s) {
-    using K = YCfg<L2>;
+    using K = YinvCfg<L2>;
     k_yinv<L2><<<grid_for(3LL * d.Hx * d.nz, K::LB), K::THREADS, K::SHM, s>>>(d, B, A, tw);
     checkLaunch("k_yinv");
   }
@@ -760,7 +797,7 @@ __global__ void __launch_bounds__(256) k_zpass_b(const BSeg* __restrict__ segs, 
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const BSeg s = segs[b2s[b]];
   const int lb = b - s.first;
-#define MPA_CALL(L) zpass_body<L>(s.d, s.B, s.T, s.tw[2], lb)
+#define MPA_CALL(L) zpass_body<L, PointCfgB<L>>(s.d, s.B, s.T, s.tw[2], lb)
   MPA_L2_CASES(s.d.l2z, MPA_CALL)
 #undef MPA_CALL
 }
@@ -778,7 +815,7 @@ __global__ void __launch_bounds__(256) k_ypass2d_b(const BSeg* __restrict__ segs
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const BSeg s = segs[b2s[b]];
   const int lb = b - s.first;
-#define MPA_CALL(L) ypass2d_body<L>(s.d, s.A, s.T, s.tw[1], lb)
+#define MPA_CALL(L) ypass2d_body<L, PointCfgB<L>>(s.d, s.A, s.T, s.tw[1], lb)
   MPA_L2_CASES(s.d.l2y, MPA_CALL)
 #undef MPA_CALL
 }
@@ -809,12 +846,12 @@ struct BatchShape {
         *shm = YCfgB<L2>::SHM;
         break;
       case 2:
-        *blocks = grid_for((long long)d.Hx * d.Ky, PointCfg<L2>::PT);
-        *shm = PointCfg<L2>::SHM;
+        *blocks = grid_for((long long)d.Hx * d.Ky, PointCfgB<L2>::PT);
+        *shm = PointCfgB<L2>::SHM;
         break;
       case 4:
-        *blocks = grid_for(d.Hx, PointCfg<L2>::PT);
-        *shm = PointCfg<L2>::SHM;
+        *blocks = grid_for(d.Hx, PointCfgB<L2>::PT);
+        *shm = PointCfgB<L2>::SHM;
         break;
       default:
         *blocks = grid_for((d.nlines + 1) / 2, XinvCfgB<L2>::PB);

@@ -30,6 +30,11 @@ constexpr float QE = 1.60217646e-19f;     // constants.hpp:13
 constexpr float MUB = 9.274009152e-24f;   // constants.hpp:15
 constexpr float HBAR = 1.054571817e-34f;  // constants.hpp:17
 constexpr int BLOCK = 256;
+// Widest x extent of a uniform-path tile (the rest of the 256 threads go
+// along y), chosen by timing on MI355X.
+#ifndef MPA_FIELD_TILE_X
+#define MPA_FIELD_TILE_X 256
+#endif
 
 __device__ __forceinline__ float harmonic_mean(float a, float b) {
   if (a + b == 0.0f) return 0.0f;
@@ -558,7 +563,7 @@ void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* o
   if (mode != MODE_EDENS && all_uniform(d) && d.nz <= 65535) {
     const FMUniform u = make_uniform(d);
     int bx = 1;
-    while (bx < d.nx && bx < BLOCK) bx <<= 1;
+    while (bx < d.nx && bx < MPA_FIELD_TILE_X) bx <<= 1;
     const int by = BLOCK / bx;
     dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, d.nz);
     k_field_uniform<<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode);
