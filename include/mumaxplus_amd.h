@@ -279,7 +279,8 @@ int mpa_batch_create(long long capacity, const char* method, mpa_batch** out);
 int mpa_batch_destroy(mpa_batch* b);
 /* mode 0: `arg` accepted steps; 1: run for `arg` seconds (TimeSolver::run); 2: damping-only
  * dynamics until the max damping torque <= `arg` (1/s), or 200000 accepted steps when the torque
- * stalls above it (last_torque of mpa_batch_member then exceeds `arg`).  The member starts with its world's
+ * stalls above it (last_torque of mpa_batch_member then exceeds `arg`).  Adding 0x100 to the mode
+ * keeps the world's current step instead of applying the sensible-step rule (a resumed member).  The member starts with its world's
  * solver settings (timestep, adaptive_timestep, max_error, headroom, bounds, sensible factor).
  * *id = -1 when its cells do not fit in the free capacity now. */
 int mpa_batch_add(mpa_batch* b, mpa_ferromagnet* m, int mode, double arg, int* id);

@@ -180,7 +180,7 @@ __global__ void k_bcontrol(BatchMember* __restrict__ st, unsigned* __restrict__ 
     if (S.accept != 2) return;
     // the initial step of TimeSolver.run / steps (timesolver.py:42-47,
     // timesolver.cpp:37-47): the sensible step when it is smaller
-    if (S.adaptive) {
+    if (S.adaptive && !S.keep_step) {
       const double sens = tq == 0.f ? S.sdefault : S.sfactor / (double)tq;
       if (S.dt == 0.0 || S.dt > sens) S.dt = sens;
     }
@@ -319,8 +319,11 @@ int Batch::nmembers() const {
   return n;
 }
 
-int Batch::add(Ferromagnet* fm, int mode, double arg) {
+int Batch::add(Ferromagnet* fm, int mode_flags, double arg) {
   if (!fm) throw Error(MPA_ERR_INVALID, "batch: magnet must not be NULL");
+  const int mode = mode_flags & 0xff;
+  const bool keep_step = (mode_flags & BATCH_KEEP_STEP) != 0;
+  if ((mode_flags & ~(0xff | BATCH_KEEP_STEP)) != 0) throw Error(MPA_ERR_INVALID, "batch: unknown mode flags");
   if (fm->host) throw Error(MPA_ERR_INVALID, "batch: sublattices are not batched");
   World* w = fm->world;
   if (w->magnets.size() != 1 || !w->afms.empty())
@@ -393,6 +396,8 @@ int Batch::add(Ferromagnet* fm, int mode, double arg) {
   S.sfactor = ts.sensible_factor;
   S.sdefault = ts.sensible_default;
   S.adaptive = ts.adaptive ? 1 : 0;
+  S.keep_step = keep_step ? 1 : 0;
+  if (keep_step && S.adaptive && !(S.dt > 0)) throw Error(MPA_ERR_INVALID, "batch: no step to keep");
   S.mode = mode;
   if (mode == BATCH_STEPS) S.target = (long long)arg;
   else if (mode == BATCH_RUN) S.stop = ts.time + arg;

@@ -38,6 +38,9 @@ enum BatchMode {
 // A relaxation whose torque stalls above its threshold (the adaptive step's
 // error floor) stops after this many accepted steps, not converged.
 constexpr long long BATCH_RELAX_MAX_STEPS = 200000;
+// Mode flag: start with the world's step as it is (a resumed member) instead
+// of the sensible-step rule of TimeSolver.run / steps.
+constexpr int BATCH_KEEP_STEP = 0x100;
 
 // Per-member state in device memory.
 struct BatchMember {
@@ -46,7 +49,7 @@ struct BatchMember {
   long long steps, target, rejected;
   float threshold;
   float last_error, last_torque;
-  int mode, adaptive, active, accept, last, done;
+  int mode, adaptive, active, accept, last, done, keep_step;
 };
 
 class Batch {
@@ -60,7 +63,8 @@ class Batch {
   // world's solver settings: dt, adaptive, max_error, headroom, bounds).
   // Returns the member id, or -1 when its cells do not fit now.
   // arg: steps (BATCH_STEPS), duration (BATCH_RUN), torque threshold
-  // (BATCH_RELAX).  The member's world must hold only this magnet.
+  // (BATCH_RELAX); mode may carry BATCH_KEEP_STEP.  The member's world must
+  // hold only this magnet.
   int add(Ferromagnet* fm, int mode, double arg);
   // Copies the member's state back into its magnet and its world's solver
   // (time, dt, step counts) and frees its cells.

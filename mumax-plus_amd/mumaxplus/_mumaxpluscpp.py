@@ -1048,9 +1048,10 @@ class Batch:
             _lib.mpa_batch_destroy(h)
             self._h = None
 
-    def add(self, magnet, mode, arg):
+    def add(self, magnet, mode, arg, keep_step=False):
         out = _cint()
-        _check(_lib.mpa_batch_add(self._h, magnet._h, self.MODES[mode], float(arg), _ct.byref(out)))
+        flags = self.MODES[mode] | (0x100 if keep_step else 0)
+        _check(_lib.mpa_batch_add(self._h, magnet._h, flags, float(arg), _ct.byref(out)))
         if out.value < 0:
             return None
         self._members[out.value] = magnet

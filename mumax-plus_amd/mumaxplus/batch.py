@@ -26,24 +26,54 @@ class Batch:
     def __init__(self, capacity, method="DormandPrince"):
         self._impl = _cpp.Batch(capacity, method)
         self._magnets = {}
+        self._rules = {}
 
-    def add(self, magnet, steps=None, duration=None, relax_torque=None):
+    def add(self, magnet, steps=None, duration=None, relax_torque=None, keep_step=False):
         """Admit `magnet` with exactly one stopping rule; returns its member
-        id, or None when its cells do not fit in the free capacity now."""
+        id, or None when its cells do not fit in the free capacity now.
+        `keep_step` starts from the world's current step instead of the
+        sensible-step rule (as a resumed member does)."""
         rules = [(k, v) for k, v in (("steps", steps), ("run", duration), ("relax", relax_torque)) if v is not None]
         if len(rules) != 1:
             raise ValueError("give exactly one of steps, duration, relax_torque")
         mode, arg = rules[0]
-        mid = self._impl.add(magnet._impl, mode, arg)
+        mid = self._impl.add(magnet._impl, mode, arg, keep_step)
         if mid is not None:
             self._magnets[mid] = magnet
+            # a run ends at the world time of admission plus its duration
+            stop = magnet._impl._world.timesolver.time + arg if mode == "run" else None
+            self._rules[mid] = (mode, arg, stop)
         return mid
 
     def remove(self, member):
         """Copy the member's state back into its magnet and world, free its
         cells and return the magnet."""
         self._impl.remove(member)
+        self._rules.pop(member, None)
         return self._magnets.pop(member)
+
+    def preempt(self, member):
+        """Take a running member off the GPU: its state goes back into its
+        magnet and world (as with :meth:`remove`) and the returned token
+        holds the work left, for :meth:`resume` here or in another batch."""
+        st = self.member(member)
+        mode, arg, stop = self._rules[member]
+        magnet = self.remove(member)
+        if mode == "steps":
+            left = ("steps", int(arg) - int(st["steps"]))
+        elif mode == "run":
+            left = ("duration", stop - st["time"])
+        else:
+            left = ("relax_torque", arg)
+        return magnet, left
+
+    def resume(self, token):
+        """Admit a preempted member again with the work it had left; returns
+        its new member id (None when it does not fit now)."""
+        magnet, (key, value) = token
+        if key == "steps" and value <= 0 or key == "duration" and value <= 0:
+            return None
+        return self.add(magnet, keep_step=True, **{key: value})
 
     def advance(self, nsteps=1):
         """`nsteps` batch steps: one step attempt of every active member each."""

@@ -103,6 +103,28 @@ class World:
     def unset_pbc(self):
         self._impl.unset_pbc()
 
+    # ------------------------------------------------- snapshots, checkpoints
+    def snapshot(self):
+        """The dynamic state (magnetisations, elastic state, time solver); see
+        :mod:`mumaxplus.checkpoint`."""
+        from . import checkpoint
+        return checkpoint.snapshot(self)
+
+    def restore(self, snapshot):
+        """Put a snapshot taken with :meth:`snapshot` back."""
+        from . import checkpoint
+        checkpoint.restore(self, snapshot)
+
+    def save_checkpoint(self, path):
+        """Write the dynamic state to an .npz file."""
+        from . import checkpoint
+        checkpoint.save(self, path)
+
+    def load_checkpoint(self, path):
+        """Restore the dynamic state from a file written by :meth:`save_checkpoint`."""
+        from . import checkpoint
+        checkpoint.load(self, path)
+
     def minimize(self, tol=1e-6, nsamples=10):
         """Minimise the total energy of every magnet of the world together."""
         self._impl.minimize(tol, nsamples)
