@@ -255,6 +255,9 @@ Batch::Batch(long long capacity, const std::string& method) : method_(method) {
   cs_ = round_chunk(capacity);
   if (3 * cs_ >= (1LL << 31)) throw Error(MPA_ERR_INVALID, "batch: capacity above 715 million cells");
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
   const size_t n3 = (size_t)3 * cs_;
   X_.allocate(n3);
   X0_.allocate(n3);
@@ -274,6 +277,9 @@ Batch::Batch(long long capacity, const std::string& method) : method_(method) {
 
 Batch::~Batch() {
   drop_graph();
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -459,7 +465,7 @@ void Batch::rebuild() {
 }
 
 void Batch::eval_rhs(float* out) {
-  if (demag_) demag_->exec(X_.get(), HD_.get(), stream, &prof);
+  if (demag_) demag_->exec(X_.get(), HD_.get(), stream, &prof, stream2, fork_, join_);
   double bytes = 0.0;
   for (const Slot& sl : slots_)
     if (sl.used) bytes += 12.0 * sl.ncell * (sl.demag ? 3 : 2);

@@ -83,6 +83,8 @@ class Batch {
 
   bool use_graph = true;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // the single layers' FFT chain
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
   Profiler prof;
 
  private:

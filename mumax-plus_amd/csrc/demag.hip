@@ -1015,6 +1015,9 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
     if (members[j].plan->is2d()) return pass == 0 ? 0 : (pass == 4 ? 1 : 2);
     return pass == 5 ? 4 : pass;
   };
+  // two independent chains: the members with a z transform (group 0) and the
+  // single layers (group 1)
+  for (int group = 0; group < 2; ++group)
   for (int pass = 0; pass < 6; ++pass) {
     {
       std::vector<BSeg> segs;
@@ -1022,7 +1025,7 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
       double bytes = 0.0;
       size_t shm_max = 0;
       for (int j = 0; j < M; ++j) {
-        if (!applies(pass, j)) continue;
+        if ((members[j].plan->is2d() ? 1 : 0) != group || !applies(pass, j)) continue;
         unsigned nb = 0;
         size_t shm = 0;
         with_line<BatchShape>(l2_of(pass, j), pass, dd[j], &nb, &shm);
@@ -1045,6 +1048,7 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
       if (segs.empty()) continue;
       Launch L;
       L.pass = pass;
+      L.group = group;
       L.shm = shm_max;
       L.nblocks = (unsigned)b2s.size();
       L.bytes = bytes;
@@ -1057,11 +1061,25 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
   }
 }
 
-void DemagBatch::exec(const float* X, float* H, hipStream_t s, Profiler* prof) const {
+void DemagBatch::exec(const float* X, float* H, hipStream_t s, Profiler* prof, hipStream_t s2, hipEvent_t fork,
+                      hipEvent_t join) const {
   static const char* names[] = {"k_xfwd_b", "k_yfwd_b", "k_zpass_b", "k_yinv_b", "k_ypass2d_b", "k_xinv_b"};
+  bool has[2] = {false, false};
+  for (const Launch& L : launches_) has[L.group] = true;
+  const bool split = s2 && fork && join && has[0] && has[1];
+  if (split) {
+    // the single layers' chain on the second stream, concurrently
+    HIP_CHECK(hipEventRecord(fork, s));
+    HIP_CHECK(hipStreamWaitEvent(s2, fork, 0));
+  }
   for (const Launch& L : launches_) {
-    Timed t(prof, names[L.pass], L.bytes, s);
-    run_batch_pass(L.pass, L.nblocks, L.shm, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, s);
+    hipStream_t st = split && L.group == 1 ? s2 : s;
+    Timed t(prof, names[L.pass], L.bytes, st);
+    run_batch_pass(L.pass, L.nblocks, L.shm, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, st);
+  }
+  if (split) {
+    HIP_CHECK(hipEventRecord(join, s2));
+    HIP_CHECK(hipStreamWaitEvent(s, join, 0));
   }
 }
 

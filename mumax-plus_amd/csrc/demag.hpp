@@ -78,14 +78,18 @@ class DemagBatch {
   };
   DemagBatch(const std::vector<Member>& members, long long cs);
   // H = demag field of X for every member (the plans' A, B buffers are the
-  // scratch space).
-  void exec(const float* X, float* H, hipStream_t s, Profiler* prof) const;
+  // scratch space).  With a second stream and two events, the single-layer
+  // members' passes run on s2 concurrently with the others' (fork and join
+  // through the events; works under graph capture).
+  void exec(const float* X, float* H, hipStream_t s, Profiler* prof, hipStream_t s2 = nullptr,
+            hipEvent_t fork = nullptr, hipEvent_t join = nullptr) const;
   double algorithmic_bytes() const;
   int nlaunches() const { return (int)launches_.size(); }
 
  private:
   struct Launch {
-    int pass;  // 0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d, 5 xinv
+    int pass;   // 0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d, 5 xinv
+    int group;  // 0: members with a z transform, 1: single layers
     size_t shm;
     unsigned nblocks;
     DeviceArray<char> segs;
