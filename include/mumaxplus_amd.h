@@ -116,6 +116,18 @@ int mpa_fm_param_set_uniform(mpa_ferromagnet* m, const char* key, const double* 
 int mpa_fm_param_set_cells(mpa_ferromagnet* m, const char* key, const float* values /* ncomp*ncells */);
 int mpa_fm_param_get(const mpa_ferromagnet* m, const char* key, float* out /* ncomp*ncells */);
 int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* out /* ncomp */);
+/* Time-dependent terms (Parameter.add_time_term / remove_time_terms / is_dynamic,
+ * wrap_parameter.cpp:16-30 and 44-98; DynamicParameter, dynamic_parameter.hpp:10-68): the value at
+ * time t is the static value + sum_k fn_k(t) (* mask_k per cell and component when given,
+ * ncomp*ncells floats, copied).  fn writes ncomp values and returns 0, or non-zero to fail the
+ * evaluation (the calling entry point then returns MPA_ERR_RUNTIME).  t is the time solver's time:
+ * the stage time inside a Runge-Kutta step, World time otherwise.  mpa_fm_param_get returns the
+ * value at that time; a masked term makes the parameter non-uniform (is_uniform 0). */
+typedef int (*mpa_time_fn)(double t, void* ctx, double* out);
+int mpa_fm_param_add_time_term(mpa_ferromagnet* m, const char* key, mpa_time_fn fn, void* ctx,
+                               const float* mask /* ncomp*ncells or NULL */);
+int mpa_fm_param_remove_time_terms(mpa_ferromagnet* m, const char* key);
+int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out);
 /* Variable.set / get of the magnetisation (wrap_variable.cpp:8-16, wrap_field.cpp:19-68,
  * NormalizedVariable::set variable.cpp:76-79): count 3 (uniform) or 3*ncells. */
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count);
@@ -181,6 +193,11 @@ int mpa_afm_param_set_uniform(mpa_antiferromagnet* a, const char* key, const dou
 int mpa_afm_param_set_cells(mpa_antiferromagnet* a, const char* key, const float* values);
 int mpa_afm_param_get(const mpa_antiferromagnet* a, const char* key, float* out);
 int mpa_afm_param_get_uniform(const mpa_antiferromagnet* a, const char* key, double* out);
+/* Time-dependent terms of an antiferromagnet parameter, as mpa_fm_param_add_time_term. */
+int mpa_afm_param_add_time_term(mpa_antiferromagnet* a, const char* key, mpa_time_fn fn, void* ctx,
+                                const float* mask);
+int mpa_afm_param_remove_time_terms(mpa_antiferromagnet* a, const char* key);
+int mpa_afm_param_is_dynamic(const mpa_antiferromagnet* a, const char* key, int* out);
 /* Elastic displacement / velocity of the antiferromagnet, as mpa_fm_set_variable. */
 int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count);
 int mpa_afm_get_variable(mpa_antiferromagnet* a, const char* which, float* out);

@@ -338,6 +338,10 @@ int Batch::add(Ferromagnet* fm, int mode_flags, double arg) {
   if (mode < BATCH_STEPS || mode > BATCH_RELAX) throw Error(MPA_ERR_INVALID, "batch: unknown mode");
   for (auto& s : slots_)
     if (s.used && s.fm == fm) throw Error(MPA_ERR_INVALID, "batch: magnet is already a member");
+  // a member's parameters are read once, on admission
+  for (const auto& kv : fm->params)
+    if (kv.second.is_dynamic())
+      throw Error(MPA_ERR_INVALID, "batch: parameter '" + kv.first + "' has time-dependent terms");
   const unsigned terms = fm->active_terms();
   const FMDesc d = fm->desc(terms);
   if (!field_batchable(d)) throw Error(MPA_ERR_INVALID, "batch: every material parameter must be uniform");

@@ -80,6 +80,9 @@ void param_set_uniform(MagnetCore* m, const char* key, const double* value);
 void param_set_cells(MagnetCore* m, const char* key, const float* values);
 void param_get(const MagnetCore* m, const char* key, float* out);
 void param_get_uniform(const MagnetCore* m, const char* key, double* out);
+void param_add_time_term(MagnetCore* m, const char* key, TimeFn fn, void* ctx, const float* mask);
+void param_remove_time_terms(MagnetCore* m, const char* key);
+void param_is_dynamic(const MagnetCore* m, const char* key, int* out);
 void magnet_grid(const MagnetCore* m, int size[3], int origin[3]) {
   for (int a = 0; a < 3; ++a) {
     if (size) size[a] = m->n[a];
@@ -114,6 +117,7 @@ void param_set_uniform(MagnetCore* m, const char* key, const double* value) {
   Param& p = m->param(key);
   p.cells.release();
   for (int c = 0; c < 3; ++c) p.uniform[c] = c < p.ncomp ? (float)value[c] : 0.f;
+  p.changed();
   m->world->touch();
 }
 
@@ -125,6 +129,7 @@ void param_set_cells(MagnetCore* m, const char* key, const float* values) {
   if (p.cells.n != n) p.cells.allocate(n);
   HIP_CHECK(hipMemcpy(p.cells.get(), values, n * sizeof(float), hipMemcpyHostToDevice));
   for (int c = 0; c < 3; ++c) p.uniform[c] = 0.f;
+  p.changed();
   m->world->touch();
 }
 
@@ -133,12 +138,48 @@ void param_get(const MagnetCore* m, const char* key, float* out) {
   need(out, "out");
   const Param& p = m->param(key);
   const size_t n = (size_t)p.ncomp * m->N;
-  if (p.is_uniform()) {
+  // the value at the solver's time, time-dependent terms included (parameter.cu:65-91)
+  const PV v = p.pv();
+  if (!v.p) {
+    const float u[3] = {v.x, v.y, v.z};
     for (int c = 0; c < p.ncomp; ++c)
-      for (int i = 0; i < m->N; ++i) out[(size_t)c * m->N + i] = p.uniform[c];
+      for (int i = 0; i < m->N; ++i) out[(size_t)c * m->N + i] = u[c];
   } else {
-    HIP_CHECK(hipMemcpy(out, p.cells.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+    hipStream_t s = m->world->stream;
+    HIP_CHECK(hipMemcpyAsync(out, v.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
   }
+}
+
+// DynamicParameter::addTimeDependentTerm (dynamic_parameter.hpp:20-48)
+void param_add_time_term(MagnetCore* m, const char* key, TimeFn fn, void* ctx, const float* mask) {
+  need(key, "key");
+  if (!fn) throw Error(MPA_ERR_INVALID, "the time-dependent term is a null function");
+  Param& p = m->param(key);
+  TimeTerm t{fn, ctx, DeviceArray<float>()};
+  if (mask) {
+    const size_t n = (size_t)p.ncomp * m->N;
+    t.mask.allocate(n);
+    HIP_CHECK(hipMemcpy(t.mask.get(), mask, n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  p.terms.push_back(std::move(t));
+  p.changed();
+  m->world->touch();
+}
+
+void param_remove_time_terms(MagnetCore* m, const char* key) {
+  need(key, "key");
+  Param& p = m->param(key);
+  HIP_CHECK(hipStreamSynchronize(m->world->stream));  // masks may still be read
+  p.terms.clear();
+  p.changed();
+  m->world->touch();
+}
+
+void param_is_dynamic(const MagnetCore* m, const char* key, int* out) {
+  need(key, "key");
+  need(out, "out");
+  *out = m->param(key).is_dynamic() ? 1 : 0;
 }
 
 void param_get_uniform(const MagnetCore* m, const char* key, double* out) {
@@ -413,6 +454,18 @@ int mpa_fm_param_get_uniform(const mpa_ferromagnet* m, const char* key, double* 
   return guard([&] { param_get_uniform(FM(m), key, out); });
 }
 
+int mpa_fm_param_add_time_term(mpa_ferromagnet* m, const char* key, mpa_time_fn fn, void* ctx, const float* mask) {
+  return guard([&] { param_add_time_term(FM(m), key, fn, ctx, mask); });
+}
+
+int mpa_fm_param_remove_time_terms(mpa_ferromagnet* m, const char* key) {
+  return guard([&] { param_remove_time_terms(FM(m), key); });
+}
+
+int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out) {
+  return guard([&] { param_is_dynamic(FM(m), key, out); });
+}
+
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count) {
   return guard([&] {
     Ferromagnet* f = FM(m);
@@ -578,6 +631,19 @@ int mpa_afm_param_get(const mpa_antiferromagnet* a, const char* key, float* out)
 
 int mpa_afm_param_get_uniform(const mpa_antiferromagnet* a, const char* key, double* out) {
   return guard([&] { param_get_uniform(AFM(a), key, out); });
+}
+
+int mpa_afm_param_add_time_term(mpa_antiferromagnet* a, const char* key, mpa_time_fn fn, void* ctx,
+                                const float* mask) {
+  return guard([&] { param_add_time_term(AFM(a), key, fn, ctx, mask); });
+}
+
+int mpa_afm_param_remove_time_terms(mpa_antiferromagnet* a, const char* key) {
+  return guard([&] { param_remove_time_terms(AFM(a), key); });
+}
+
+int mpa_afm_param_is_dynamic(const mpa_antiferromagnet* a, const char* key, int* out) {
+  return guard([&] { param_is_dynamic(AFM(a), key, out); });
 }
 
 int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count) {

@@ -559,6 +559,26 @@ void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t
   checkLaunch("k_field_uniform_b");
 }
 
+__global__ void k_param_fill(float* out, const float* base, const ParamFill f, int ncomp,
+                             int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int c = 0; c < ncomp; ++c) {
+    const size_t j = (size_t)c * N + i;
+    float v = (base ? base[j] : 0.f) + f.add[c];
+    for (int k = 0; k < f.nmask; ++k) v += f.coef[k][c] * f.mask[k][j];
+    out[j] = v;
+  }
+}
+
+void launch_param_fill(float* out, const float* base, const ParamFill& f, int ncomp, int N, hipStream_t s) {
+  if (N <= 0) return;
+  if (f.nmask < 0 || f.nmask > ParamFill::MAX || ncomp < 1 || ncomp > 3)
+    throw Error(MPA_ERR_RUNTIME, "launch_param_fill: bad arguments");
+  k_param_fill<<<(N + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(out, base, f, ncomp, N);
+  checkLaunch("k_param_fill");
+}
+
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {
   if (mode != MODE_EDENS && all_uniform(d) && d.nz <= 65535) {
     const FMUniform u = make_uniform(d);

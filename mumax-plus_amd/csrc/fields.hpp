@@ -84,6 +84,18 @@ enum FieldMode {
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
 
+// Per-cell values of a parameter with time-dependent terms:
+//   out[c N + i] = (base ? base[c N + i] : 0) + add[c] + sum_k coef[k][c] mask[k][c N + i]
+// for c < ncomp (dynamic_parameter.cpp:8-29 plus the static field).
+struct ParamFill {
+  static constexpr int MAX = 8;
+  const float* mask[MAX];
+  float coef[MAX][3];
+  float add[3];
+  int nmask;
+};
+void launch_param_fill(float* out, const float* base, const ParamFill& f, int ncomp, int N, hipStream_t s);
+
 // The field kernels of many ferromagnets with uniform parameters and no
 // stray field in one launch.  Their cells are packed: component c of cell i
 // of a member is at c * cs + off + i, and every member starts on a multiple

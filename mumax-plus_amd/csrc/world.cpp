@@ -2,6 +2,7 @@
 #include "world.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -180,8 +181,53 @@ void MagnetCore::add_params(const void* specs, int count) {
     p.unit = sp[i].unit;
     p.ncomp = sp[i].ncomp;
     for (int c = 0; c < 3; ++c) p.uniform[c] = sp[i].def[c];
+    p.N = N;
+    p.world = world;
     params.emplace(sp[i].key, std::move(p));
   }
+}
+
+// DynamicParameter::evalTimeDependentTerms (dynamic_parameter.cpp:8-29) at
+// the solver's time, added to the static value (parameter.cu:65-110).
+void Param::refresh() const {
+  const double t = world->solver.time;
+  if (fresh_ && at_ == t) return;
+  double u[3] = {uniform[0], uniform[1], uniform[2]};
+  std::vector<std::pair<const float*, std::array<float, 3>>> masked;
+  for (const auto& term : terms) {
+    double v[3] = {0.0, 0.0, 0.0};
+    if (term.fn(t, term.ctx, v) != 0)
+      throw Error(MPA_ERR_RUNTIME, "the time-dependent term of parameter '" + name + "' failed");
+    if (term.mask.n) {
+      masked.push_back({term.mask.get(), {(float)v[0], (float)v[1], (float)v[2]}});
+    } else {
+      for (int c = 0; c < ncomp; ++c) u[c] += v[c];
+    }
+  }
+  for (int c = 0; c < 3; ++c) eff_[c] = c < ncomp ? (float)u[c] : 0.f;
+  dyn_cells_ = cells.n != 0 || !masked.empty();
+  if (dyn_cells_) {
+    const size_t n = (size_t)ncomp * N;
+    if (dyn_.n != n) dyn_.allocate(n);
+    // the first launch starts from the static cells, later ones (more than
+    // ParamFill::MAX masks) accumulate onto the first's output
+    size_t k = 0;
+    bool first = true;
+    do {
+      ParamFill f{};
+      f.nmask = (int)std::min<size_t>(ParamFill::MAX, masked.size() - k);
+      for (int j = 0; j < f.nmask; ++j) {
+        f.mask[j] = masked[k + j].first;
+        for (int c = 0; c < 3; ++c) f.coef[j][c] = masked[k + j].second[c];
+      }
+      for (int c = 0; c < 3; ++c) f.add[c] = first ? eff_[c] : 0.f;
+      launch_param_fill(dyn_.get(), first ? cells.get() : dyn_.get(), f, ncomp, N, world->stream);
+      k += f.nmask;
+      first = false;
+    } while (k < masked.size());
+  }
+  at_ = t;
+  fresh_ = true;
 }
 
 Param& MagnetCore::param(const std::string& key) {

@@ -29,17 +29,65 @@ class Antiferromagnet;
 
 // A scalar (ncomp 1) or vector (ncomp 3) material parameter: a uniform value
 // or one value per cell (Parameter / VectorParameter, src/core/parameter.hpp).
+//
+// Time-dependent terms (DynamicParameter, src/core/dynamic_parameter.hpp:10-68):
+// the value at time t is static + sum_k f_k(t) (* mask_k, per cell and
+// component, when the term has a mask), with t the time solver's (stage)
+// time.  ps() / pv() evaluate the terms on the world's stream when the time
+// or the terms changed since the last evaluation: a uniform sum when no term
+// has a mask and the static value is uniform, otherwise one fill kernel into
+// a per-cell array.
+typedef int (*TimeFn)(double t, void* ctx, double* out /* ncomp */);
+struct TimeTerm {
+  TimeFn fn;
+  void* ctx;
+  DeviceArray<float> mask;  // ncomp * N, or empty
+};
+
 struct Param {
   std::string name, unit;
   int ncomp = 1;
+  int N = 0;
+  World* world = nullptr;
   float uniform[3] = {0.f, 0.f, 0.f};
   DeviceArray<float> cells;  // ncomp * N when non-uniform
-  bool is_uniform() const { return cells.n == 0; }
-  bool assured_zero() const {
-    return is_uniform() && uniform[0] == 0.f && uniform[1] == 0.f && uniform[2] == 0.f;
+  std::vector<TimeTerm> terms;
+  bool is_dynamic() const { return !terms.empty(); }
+  bool has_masks() const {
+    for (const auto& t : terms)
+      if (t.mask.n) return true;
+    return false;
   }
-  PS ps() const { return PS{cells.get(), uniform[0]}; }
-  PV pv() const { return PV{cells.get(), uniform[0], uniform[1], uniform[2]}; }
+  // Parameter::isUniform (parameter.cu:47-49): no per-cell values and no masked term
+  bool is_uniform() const { return cells.n == 0 && !has_masks(); }
+  // Parameter::assuredZero (parameter.cu:51-53)
+  bool assured_zero() const {
+    return !is_dynamic() && cells.n == 0 && uniform[0] == 0.f && uniform[1] == 0.f && uniform[2] == 0.f;
+  }
+  PS ps() const {
+    if (is_dynamic()) {
+      refresh();
+      return PS{dyn_cells_ ? dyn_.get() : nullptr, eff_[0]};
+    }
+    return PS{cells.get(), uniform[0]};
+  }
+  PV pv() const {
+    if (is_dynamic()) {
+      refresh();
+      return PV{dyn_cells_ ? dyn_.get() : nullptr, eff_[0], eff_[1], eff_[2]};
+    }
+    return PV{cells.get(), uniform[0], uniform[1], uniform[2]};
+  }
+  // the static value or the terms changed: the next ps() / pv() evaluates again
+  void changed() { fresh_ = false; }
+
+ private:
+  void refresh() const;
+  mutable float eff_[3] = {0.f, 0.f, 0.f};  // uniform value, or the per-cell fill's uniform addend
+  mutable DeviceArray<float> dyn_;          // ncomp * N values at at_ when dyn_cells_
+  mutable bool dyn_cells_ = false;
+  mutable bool fresh_ = false;
+  mutable double at_ = 0.0;
 };
 
 struct Tableau {

@@ -32,6 +32,9 @@ _c_char_p = _ct.c_char_p
 _cint = _ct.c_int
 _cdouble = _ct.c_double
 
+# int fn(double t, void* ctx, double* out): a parameter's time-dependent term
+_TIME_FN = _ct.CFUNCTYPE(_cint, _cdouble, _P, _ct.POINTER(_cdouble))
+
 _SIGS = {
     "mpa_last_error": ([], _c_char_p),
     "mpa_abi_version": ([], _cint),
@@ -63,6 +66,9 @@ _SIGS = {
     "mpa_fm_param_set_cells": ([_P, _c_char_p, _P], _cint),
     "mpa_fm_param_get": ([_P, _c_char_p, _P], _cint),
     "mpa_fm_param_get_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_fm_param_add_time_term": ([_P, _c_char_p, _TIME_FN, _P, _P], _cint),
+    "mpa_fm_param_remove_time_terms": ([_P, _c_char_p], _cint),
+    "mpa_fm_param_is_dynamic": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
     "mpa_fm_set_magnetization": ([_P, _P, _ct.c_longlong], _cint),
     "mpa_fm_get_magnetization": ([_P, _P], _cint),
     "mpa_fm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
@@ -90,6 +96,9 @@ _SIGS = {
     "mpa_afm_param_set_cells": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_param_get": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_param_get_uniform": ([_P, _c_char_p, _P], _cint),
+    "mpa_afm_param_add_time_term": ([_P, _c_char_p, _TIME_FN, _P, _P], _cint),
+    "mpa_afm_param_remove_time_terms": ([_P, _c_char_p], _cint),
+    "mpa_afm_param_is_dynamic": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
     "mpa_afm_set_variable": ([_P, _c_char_p, _P, _ct.c_longlong], _cint),
     "mpa_afm_get_variable": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
@@ -146,10 +155,21 @@ EXPORTED = tuple(_SIGS)
 MPA_ERR_INVALID, MPA_ERR_RUNTIME, MPA_ERR_HIP, MPA_ERR_INDEX = 1, 2, 3, 4
 
 
+# The exception a time-dependent term raised inside the library; the failing
+# entry point re-raises it.
+_term_error = None
+# (magnet kind, handle, parameter key) -> the callbacks the library holds
+_TIME_FNS = {}
+
+
 def _check(rc):
+    global _term_error
     if rc == 0:
         return
     msg = _lib.mpa_last_error().decode(errors="replace")
+    if _term_error is not None:
+        err, _term_error = _term_error, None
+        raise err
     if rc == MPA_ERR_INVALID:
         raise ValueError(msg)
     if rc == MPA_ERR_INDEX:
@@ -405,7 +425,9 @@ class Parameter(FieldQuantity):
 
     @property
     def is_dynamic(self):
-        return False
+        out = _cint()
+        _call(self._fm, "param_is_dynamic", self._key, _ct.byref(out))
+        return bool(out.value)
 
     @property
     def uniform_value(self):
@@ -448,10 +470,44 @@ class Parameter(FieldQuantity):
         _call(self._fm, "param_set_cells", self._key, _ptr(a))
 
     def add_time_term(self, term, mask=None):
-        raise NotImplementedError("time-dependent parameter terms are not supported by this build")
+        """term(t) -> ncomp values; mask None or an array of shape
+        (ncomp, nz, ny, nx) (dynamic_parameter.hpp:20-48)."""
+        nc = self.ncomp
+
+        def call(t, ctx, out):
+            global _term_error
+            try:
+                v = _np.asarray(term(t), dtype=_np.float64).reshape(-1)
+                if v.size != nc:
+                    raise ValueError(f"the time-dependent term returned {v.size} values, expected {nc}")
+                for c in range(nc):
+                    out[c] = float(v[c])
+                return 0
+            except BaseException as e:  # handed to the entry point that evaluated the term
+                _term_error = e
+                return 1
+
+        fn = _TIME_FN(call)
+        mptr = None
+        if mask is not None:
+            nx, ny, nz = self.grid.size
+            a = _np.asarray(mask, dtype=_np.float32)
+            if a.shape != (nc, nz, ny, nx):
+                raise ValueError(f"The shape of the mask does not match the shape of the parameter, expected "
+                                 f"{(nc, nz, ny, nx)}, got {a.shape}.")
+            a = _np.ascontiguousarray(a)
+            mptr = _ptr(a)
+        _call(self._fm, "param_add_time_term", self._key, fn, None, mptr)
+        # the library calls fn until the terms are removed; magnet wrappers
+        # come and go, so the callbacks are kept by handle
+        _TIME_FNS.setdefault(self._fn_key(), []).append(fn)
 
     def remove_time_terms(self):
-        pass
+        _call(self._fm, "param_remove_time_terms", self._key)
+        _TIME_FNS.pop(self._fn_key(), None)
+
+    def _fn_key(self):
+        return (self._fm._pfx, self._fm._h.value, self._q)
 
 
 class VectorParameter(Parameter):

@@ -28,14 +28,47 @@ class Parameter(FieldQuantity):
         self._impl.set(value)
 
     def add_time_term(self, term, mask=None):
+        """Add a time-dependent term (parameter.py:61-113 of the reference).
+
+        The value becomes static + term(t), or static + term(t) * mask per
+        cell when a mask is given; terms add up.  term: (t) -> float, or
+        (t) -> 3 floats for a vector parameter.  mask: an array of shape
+        (nz, ny, nx) or (ncomp, nz, ny, nx), or a function of the cell
+        position (x, y, z)."""
+        if mask is not None:
+            if callable(mask):
+                mask = self._eval_mask(mask)
+            mask = self._check_mask_shape(_np.asarray(mask, dtype=float))
         self._impl.add_time_term(term, mask)
+
+    def _eval_mask(self, mask):
+        try:
+            ncomp = len(mask(0, 0, 0))
+        except TypeError:  # a scalar mask
+            ncomp = 1
+        X, Y, Z = self.meshgrid
+        return _np.array(_np.vectorize(mask, otypes=[float] * ncomp)(X, Y, Z))
+
+    def _check_mask_shape(self, mask):
+        """(nz, ny, nx), (1, nz, ny, nx) or (ncomp, nz, ny, nx) -> (ncomp, nz, ny, nx)."""
+        if mask.ndim == 4 and mask.shape[0] == self.ncomp:
+            return mask
+        core = mask if mask.ndim != 4 else mask[0]
+        return _np.broadcast_to(core, (self.ncomp, *core.shape)).copy()
 
     def remove_time_terms(self):
         self._impl.remove_time_terms()
 
+    def _reset_fields_default(self):
+        self._impl.set(0.0 if self.ncomp == 1 else (0.0, 0.0, 0.0))
+        self.remove_time_terms()
+
     def set(self, value):
-        """Set a uniform value (scalar / 3-tuple), a per-cell array or a
-        function of the cell position (x, y, z)."""
+        """Set a uniform value (scalar / 3-tuple), a per-cell array, a
+        function of the cell position (x, y, z), a time-dependent term
+        (t) -> value, or a (term, mask) pair (parameter.py:146-214 of the
+        reference).  Earlier values and time terms are dropped first."""
+        self._reset_fields_default()
         if callable(value):
             try:
                 value(0)

@@ -48,17 +48,50 @@ class _Params:
         else:
             self.params[name] = value
 
+    # Time-dependent terms (DynamicParameter::evalTimeDependentTerms,
+    # dynamic_parameter.cpp:8-29; Parameter::eval, parameter.cu:65-91): the
+    # value is static + sum_k fn_k(t) (* mask_k), t = clock(), which the world
+    # binds to its time solver's (stage) time.
+    def add_time_term(self, name, fn, mask=None):
+        terms = self.__dict__.setdefault("time_terms", {})
+        terms.setdefault(name, []).append((fn, None if mask is None else np.asarray(mask, dtype=float)))
+
+    def remove_time_terms(self, name):
+        self.__dict__.get("time_terms", {}).pop(name, None)
+
+    def _dynamic(self, name, ncomp):
+        terms = self.__dict__.get("time_terms", {}).get(name)
+        if not terms:
+            return None
+        t = self.__dict__.get("clock", lambda: 0.0)()
+        total = 0.0
+        for fn, mask in terms:
+            v = np.asarray(fn(t), dtype=float).reshape(-1)
+            if ncomp == 1:
+                total = total + (v[0] if mask is None else v[0] * mask.reshape(self.shape))
+            else:
+                vv = v[:, None, None, None]
+                total = total + (vv if mask is None else vv * mask.reshape((3,) + self.shape))
+        return total
+
     def val(self, name):
         v = self.dmi[name] if name in DMI_KEYS else self.params[name]
-        return np.broadcast_to(np.asarray(v, dtype=float), self.shape)
+        base = np.broadcast_to(np.asarray(v, dtype=float), self.shape)
+        d = self._dynamic(name, 1)
+        return base if d is None else base + d
 
     def vec(self, name):
         v = np.asarray(self.params[name], dtype=float)
         if v.shape == (3,):
-            return np.broadcast_to(v[:, None, None, None], (3,) + self.shape)
-        return np.broadcast_to(v, (3,) + self.shape)
+            base = np.broadcast_to(v[:, None, None, None], (3,) + self.shape)
+        else:
+            base = np.broadcast_to(v, (3,) + self.shape)
+        d = self._dynamic(name, 3)
+        return base if d is None else base + d
 
     def assured_zero(self, name):
+        if self.__dict__.get("time_terms", {}).get(name):
+            return False
         v = self.dmi[name] if name in DMI_KEYS else self.params[name]
         a = np.asarray(v, dtype=float)
         return a.ndim == 0 and a == 0 or (a.shape == (3,) and not a.any())
@@ -266,8 +299,12 @@ class World:
         # the noise term is divided by the square root of
         self.noise_dt = None
 
+    def _clock(self):
+        return self.solver.time
+
     def add_ferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
         mag = Ferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
+        mag.clock = self._clock
         self.magnets.append(mag)
         if not defer_timestep:
             self.solver.equations_changed()
@@ -275,6 +312,8 @@ class World:
 
     def add_antiferromagnet(self, size, origin=(0, 0, 0), geometry=None, defer_timestep=False):
         afm = Antiferromagnet(size, self.cellsize, origin, geometry, self.mastergrid, (0, 0, 0), self.pbc)
+        for obj in (afm, afm.sub1, afm.sub2):
+            obj.clock = self._clock
         self.afms.append(afm)
         if not defer_timestep:
             self.solver.equations_changed()
