@@ -55,9 +55,11 @@ int mpa_world_get_cellsize(const mpa_world* w, double out[3]);
 int mpa_world_set_bias_magnetic_field(mpa_world* w, const double b[3]);
 int mpa_world_get_bias_magnetic_field(const mpa_world* w, double out[3]);
 /* World.add_ferromagnet(grid, geometry_array, regions_array, name): wrap_world.cpp:52-64,
- * MumaxWorld::addFerromagnet mumaxworld.cpp:57-80.  geometry: ncells bytes or NULL. */
+ * MumaxWorld::addFerromagnet mumaxworld.cpp:57-80.  geometry: ncells bytes or NULL; regions:
+ * ncells region indices (System, system.cpp:13-36) or NULL (every cell in region 0, no region
+ * defined). */
 int mpa_world_add_ferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
-                              const char* name, mpa_ferromagnet** out);
+                              const unsigned int* regions, const char* name, mpa_ferromagnet** out);
 /* World.ferromagnets (wrap_world.cpp:87-88) */
 int mpa_world_num_ferromagnets(const mpa_world* w, int* out);
 int mpa_world_ferromagnet_at(mpa_world* w, int index, mpa_ferromagnet** out);
@@ -67,7 +69,7 @@ int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet**
  * MumaxWorld::addAntiferromagnet mumaxworld.cpp:82-103.  Its two sublattices are ferromagnet
  * handles (mpa_afm_sublattice) named "<name>:sublattice_1" / "<name>:sublattice_2". */
 int mpa_world_add_antiferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
-                                  const char* name, mpa_antiferromagnet** out);
+                                  const unsigned int* regions, const char* name, mpa_antiferromagnet** out);
 /* World.antiferromagnets / get_antiferromagnet(name) (wrap_world.cpp:83-91); *out = NULL when absent. */
 int mpa_world_num_antiferromagnets(const mpa_world* w, int* out);
 int mpa_world_antiferromagnet_at(mpa_world* w, int index, mpa_antiferromagnet** out);
@@ -128,6 +130,27 @@ int mpa_fm_param_add_time_term(mpa_ferromagnet* m, const char* key, mpa_time_fn 
                                const float* mask /* ncomp*ncells or NULL */);
 int mpa_fm_param_remove_time_terms(mpa_ferromagnet* m, const char* key);
 int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out);
+/* Parameter.set_in_region(idx, value) (wrap_parameter.cpp:37-38, 105-106; parameter.cu:41-47,
+ * 144-150): the static value (ncomp doubles) in every cell of region idx; MPA_ERR_INVALID when idx
+ * is not one of the magnet's regions (system.cpp:73-77). */
+int mpa_fm_param_set_in_region(mpa_ferromagnet* m, const char* key, unsigned int region, const double* value);
+/* System.regions (wrap_system.cpp:49-50): ncells indices (zeros without regions); the sorted
+ * distinct indices (InterParameter.unique_regions, wrap_parameter.cpp:114-115): *count of them,
+ * the first min(cap, *count) written to out (out may be NULL). */
+int mpa_fm_regions(const mpa_ferromagnet* m, unsigned int* out /* ncells */);
+int mpa_fm_unique_regions(const mpa_ferromagnet* m, unsigned int* out, int cap, int* count);
+/* InterParameter (wrap_parameter.cpp:109-125, inter_parameter.cu): key inter_exchange or
+ * scale_exchange (ferromagnet.cpp:26-27).  size = N (N - 1) / 2 values, N = largest region index
+ * + 1 (at least 1), in the order i > j at i (i - 1) / 2 + j (inter_parameter.hpp:77-81).  set:
+ * one value for every pair; set_between / get_between: the pair (i, j), both regions of the
+ * magnet and i != j, else MPA_ERR_INVALID.  eval writes the size values. */
+int mpa_fm_inter_info(const mpa_ferromagnet* m, const char* key, int* is_uniform, double* uniform_value,
+                      long long* size, char* name, int namelen, char* unit, int unitlen);
+int mpa_fm_inter_set(mpa_ferromagnet* m, const char* key, double value);
+int mpa_fm_inter_set_between(mpa_ferromagnet* m, const char* key, unsigned int i, unsigned int j, double value);
+int mpa_fm_inter_get_between(const mpa_ferromagnet* m, const char* key, unsigned int i, unsigned int j,
+                             double* out);
+int mpa_fm_inter_eval(const mpa_ferromagnet* m, const char* key, float* out /* size */);
 /* Variable.set / get of the magnetisation (wrap_variable.cpp:8-16, wrap_field.cpp:19-68,
  * NormalizedVariable::set variable.cpp:76-79): count 3 (uniform) or 3*ncells. */
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count);
@@ -198,6 +221,20 @@ int mpa_afm_param_add_time_term(mpa_antiferromagnet* a, const char* key, mpa_tim
                                 const float* mask);
 int mpa_afm_param_remove_time_terms(mpa_antiferromagnet* a, const char* key);
 int mpa_afm_param_is_dynamic(const mpa_antiferromagnet* a, const char* key, int* out);
+/* Regions of an antiferromagnet (shared by its sublattices) and its inter-region parameters
+ * inter_afmex_nn / scale_afmex_nn (antiferromagnet.cpp:22-23), as the mpa_fm_ ones. */
+int mpa_afm_param_set_in_region(mpa_antiferromagnet* a, const char* key, unsigned int region,
+                                const double* value);
+int mpa_afm_regions(const mpa_antiferromagnet* a, unsigned int* out);
+int mpa_afm_unique_regions(const mpa_antiferromagnet* a, unsigned int* out, int cap, int* count);
+int mpa_afm_inter_info(const mpa_antiferromagnet* a, const char* key, int* is_uniform, double* uniform_value,
+                       long long* size, char* name, int namelen, char* unit, int unitlen);
+int mpa_afm_inter_set(mpa_antiferromagnet* a, const char* key, double value);
+int mpa_afm_inter_set_between(mpa_antiferromagnet* a, const char* key, unsigned int i, unsigned int j,
+                              double value);
+int mpa_afm_inter_get_between(const mpa_antiferromagnet* a, const char* key, unsigned int i, unsigned int j,
+                              double* out);
+int mpa_afm_inter_eval(const mpa_antiferromagnet* a, const char* key, float* out);
 /* Elastic displacement / velocity of the antiferromagnet, as mpa_fm_set_variable. */
 int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count);
 int mpa_afm_get_variable(mpa_antiferromagnet* a, const char* which, float* out);

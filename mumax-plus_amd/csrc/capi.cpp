@@ -1,6 +1,7 @@
 // extern "C" boundary of the library; declarations in include/mumaxplus_amd.h.
 #include "../../include/mumaxplus_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -83,6 +84,11 @@ void param_get_uniform(const MagnetCore* m, const char* key, double* out);
 void param_add_time_term(MagnetCore* m, const char* key, TimeFn fn, void* ctx, const float* mask);
 void param_remove_time_terms(MagnetCore* m, const char* key);
 void param_is_dynamic(const MagnetCore* m, const char* key, int* out);
+void magnet_regions(const MagnetCore* m, unsigned int* out);
+void magnet_unique_regions(const MagnetCore* m, unsigned int* out, int cap, int* count);
+void inter_info(const MagnetCore* m, const char* key, int* is_uniform, double* uniform_value, long long* size,
+                char* name, int namelen, char* unit, int unitlen);
+void inter_eval(const MagnetCore* m, const char* key, float* out);
 void magnet_grid(const MagnetCore* m, int size[3], int origin[3]) {
   for (int a = 0; a < 3; ++a) {
     if (size) size[a] = m->n[a];
@@ -176,6 +182,37 @@ void param_remove_time_terms(MagnetCore* m, const char* key) {
   m->world->touch();
 }
 
+void magnet_regions(const MagnetCore* m, unsigned int* out) {
+  need(out, "out");
+  if (m->regions_host.empty()) std::memset(out, 0, (size_t)m->N * sizeof(unsigned int));
+  else std::memcpy(out, m->regions_host.data(), (size_t)m->N * sizeof(unsigned int));
+}
+
+void magnet_unique_regions(const MagnetCore* m, unsigned int* out, int cap, int* count) {
+  need(count, "count");
+  *count = (int)m->unique_regions.size();
+  if (out)
+    for (int i = 0; i < std::min(cap, *count); ++i) out[i] = m->unique_regions[i];
+}
+
+void inter_info(const MagnetCore* m, const char* key, int* is_uniform, double* uniform_value, long long* size,
+                char* name, int namelen, char* unit, int unitlen) {
+  need(key, "key");
+  const InterParam& p = m->inter(key);
+  if (is_uniform) *is_uniform = p.is_uniform() ? 1 : 0;
+  if (uniform_value) *uniform_value = p.uniform;
+  if (size) *size = (long long)p.limit;
+  copy_str(p.name, name, namelen);
+  copy_str(p.unit, unit, unitlen);
+}
+
+void inter_eval(const MagnetCore* m, const char* key, float* out) {
+  need(key, "key");
+  need(out, "out");
+  const InterParam& p = m->inter(key);
+  for (size_t k = 0; k < p.limit; ++k) out[k] = p.is_uniform() ? p.uniform : p.host[k];
+}
+
 void param_is_dynamic(const MagnetCore* m, const char* key, int* out) {
   need(key, "key");
   need(out, "out");
@@ -264,12 +301,12 @@ int mpa_world_get_bias_magnetic_field(const mpa_world* w, double out[3]) {
 }
 
 int mpa_world_add_ferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
-                              const char* name, mpa_ferromagnet** out) {
+                              const unsigned int* regions, const char* name, mpa_ferromagnet** out) {
   return guard([&] {
     need(w, "world");
     need(size, "size");
     const int zero[3] = {0, 0, 0};
-    Ferromagnet* f = w->w.add_ferromagnet(size, origin ? origin : zero, geometry, name);
+    Ferromagnet* f = w->w.add_ferromagnet(size, origin ? origin : zero, geometry, regions, name);
     if (out) *out = H(f);
   });
 }
@@ -298,12 +335,12 @@ int mpa_world_find_ferromagnet(mpa_world* w, const char* name, mpa_ferromagnet**
 }
 
 int mpa_world_add_antiferromagnet(mpa_world* w, const int size[3], const int origin[3], const uint8_t* geometry,
-                                  const char* name, mpa_antiferromagnet** out) {
+                                  const unsigned int* regions, const char* name, mpa_antiferromagnet** out) {
   return guard([&] {
     need(w, "world");
     need(size, "size");
     const int zero[3] = {0, 0, 0};
-    Antiferromagnet* a = w->w.add_antiferromagnet(size, origin ? origin : zero, geometry, name);
+    Antiferromagnet* a = w->w.add_antiferromagnet(size, origin ? origin : zero, geometry, regions, name);
     if (out) *out = HA(a);
   });
 }
@@ -464,6 +501,53 @@ int mpa_fm_param_remove_time_terms(mpa_ferromagnet* m, const char* key) {
 
 int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out) {
   return guard([&] { param_is_dynamic(FM(m), key, out); });
+}
+
+int mpa_fm_param_set_in_region(mpa_ferromagnet* m, const char* key, unsigned int region, const double* value) {
+  return guard([&] {
+    need(key, "key");
+    need(value, "value");
+    FM(m)->param_set_in_region(key, region, value);
+  });
+}
+
+int mpa_fm_regions(const mpa_ferromagnet* m, unsigned int* out) {
+  return guard([&] { magnet_regions(FM(m), out); });
+}
+
+int mpa_fm_unique_regions(const mpa_ferromagnet* m, unsigned int* out, int cap, int* count) {
+  return guard([&] { magnet_unique_regions(FM(m), out, cap, count); });
+}
+
+int mpa_fm_inter_info(const mpa_ferromagnet* m, const char* key, int* is_uniform, double* uniform_value,
+                       long long* size, char* name, int namelen, char* unit, int unitlen) {
+  return guard([&] { inter_info(FM(m), key, is_uniform, uniform_value, size, name, namelen, unit, unitlen); });
+}
+
+int mpa_fm_inter_set(mpa_ferromagnet* m, const char* key, double value) {
+  return guard([&] {
+    need(key, "key");
+    FM(m)->inter_set(key, value);
+  });
+}
+
+int mpa_fm_inter_set_between(mpa_ferromagnet* m, const char* key, unsigned int i, unsigned int j, double value) {
+  return guard([&] {
+    need(key, "key");
+    FM(m)->inter_set_between(key, i, j, value);
+  });
+}
+
+int mpa_fm_inter_get_between(const mpa_ferromagnet* m, const char* key, unsigned int i, unsigned int j, double* out) {
+  return guard([&] {
+    need(key, "key");
+    need(out, "out");
+    *out = FM(m)->inter_get_between(key, i, j);
+  });
+}
+
+int mpa_fm_inter_eval(const mpa_ferromagnet* m, const char* key, float* out) {
+  return guard([&] { inter_eval(FM(m), key, out); });
 }
 
 int mpa_fm_set_magnetization(mpa_ferromagnet* m, const float* values, long long count) {
@@ -644,6 +728,53 @@ int mpa_afm_param_remove_time_terms(mpa_antiferromagnet* a, const char* key) {
 
 int mpa_afm_param_is_dynamic(const mpa_antiferromagnet* a, const char* key, int* out) {
   return guard([&] { param_is_dynamic(AFM(a), key, out); });
+}
+
+int mpa_afm_param_set_in_region(mpa_antiferromagnet* m, const char* key, unsigned int region, const double* value) {
+  return guard([&] {
+    need(key, "key");
+    need(value, "value");
+    AFM(m)->param_set_in_region(key, region, value);
+  });
+}
+
+int mpa_afm_regions(const mpa_antiferromagnet* m, unsigned int* out) {
+  return guard([&] { magnet_regions(AFM(m), out); });
+}
+
+int mpa_afm_unique_regions(const mpa_antiferromagnet* m, unsigned int* out, int cap, int* count) {
+  return guard([&] { magnet_unique_regions(AFM(m), out, cap, count); });
+}
+
+int mpa_afm_inter_info(const mpa_antiferromagnet* m, const char* key, int* is_uniform, double* uniform_value,
+                       long long* size, char* name, int namelen, char* unit, int unitlen) {
+  return guard([&] { inter_info(AFM(m), key, is_uniform, uniform_value, size, name, namelen, unit, unitlen); });
+}
+
+int mpa_afm_inter_set(mpa_antiferromagnet* m, const char* key, double value) {
+  return guard([&] {
+    need(key, "key");
+    AFM(m)->inter_set(key, value);
+  });
+}
+
+int mpa_afm_inter_set_between(mpa_antiferromagnet* m, const char* key, unsigned int i, unsigned int j, double value) {
+  return guard([&] {
+    need(key, "key");
+    AFM(m)->inter_set_between(key, i, j, value);
+  });
+}
+
+int mpa_afm_inter_get_between(const mpa_antiferromagnet* m, const char* key, unsigned int i, unsigned int j, double* out) {
+  return guard([&] {
+    need(key, "key");
+    need(out, "out");
+    *out = AFM(m)->inter_get_between(key, i, j);
+  });
+}
+
+int mpa_afm_inter_eval(const mpa_antiferromagnet* m, const char* key, float* out) {
+  return guard([&] { inter_eval(AFM(m), key, out); });
 }
 
 int mpa_afm_set_variable(mpa_antiferromagnet* a, const char* which, const float* values, long long count) {

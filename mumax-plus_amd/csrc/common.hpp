@@ -79,6 +79,18 @@ struct PV {
   }
 };
 
+// Value between two regions i != j (CuInterParameter::valueBetween,
+// inter_parameter.hpp:71-90): uniform, or a lookup table over the pairs in
+// lower-triangle order, i > j at i (i - 1) / 2 + j.
+struct IP {
+  const float* p;  // nullptr -> uniform
+  float v;
+  __host__ __device__ __forceinline__ static size_t index(unsigned i, unsigned j) {
+    return i > j ? (size_t)i * (i - 1) / 2 + j : (size_t)j * (j - 1) / 2 + i;
+  }
+  __device__ __forceinline__ float between(unsigned i, unsigned j) const { return p ? p[index(i, j)] : v; }
+};
+
 // i mod d for i within a few periods of [0, d) (stencil offsets): repeated
 // add / subtract instead of an integer division.
 __host__ __device__ __forceinline__ int wrap_near(int i, int d) {

@@ -61,6 +61,16 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   auto S = [&](const PS& p, int i) { return UNI ? p.v : p.at(i); };
   auto V = [&](const PV& p, int i) { return UNI ? make_float3(p.x, p.y, p.z) : p.at(i, N); };
   auto HM = [&](float u, float v) { return UNI ? u : harmonic_mean(u, v); };
+  // Exchange constant of the link to an in-geometry neighbour nb: hm, or
+  // the inter-region value when the regions differ and it is non-zero,
+  // times the inter-region scale (exchange.cu:79-99, afmexchange.cu:116-143).
+  auto link = [&](float hm, int nb, const IP& inter, const IP& scale) {
+    if (!d.regions) return hm;
+    const unsigned r = d.regions[idx], r_ = d.regions[nb];
+    if (r == r_) return hm;
+    const float in = inter.between(r, r_);
+    return (in != 0.f ? in : hm) * scale.between(r, r_);
+  };
   const bool ingeom = !d.geom || d.geom[idx];
   if (!ingeom) {
     if (mode == MODE_EDENS) out[idx] = 0.f;
@@ -130,6 +140,7 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
         bool use = true;
         float3 mn;
         float an_;
+        bool linked = false;  // an in-geometry neighbour: the region rules apply
         if (sub && !openbc) {
           // sublattice variant, exchange.cu:106-202 (no open boundaries)
           if (!nb_in) {
@@ -140,6 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
             if (S(d.msat, nb) == 0.f) use = false;
             mn = mn_[dir];
             an_ = S(d.aex, nb);
+            linked = true;
           }
         } else if (!nb_in) {
           // exchange.cu:61-101
@@ -150,9 +162,11 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
           if (openbc && S(d.msat, nb) == 0.f) use = false;
           mn = mn_[dir];
           an_ = S(d.aex, nb);
+          linked = true;
         }
         if (use) {
-          const float A = HM(a, an_);
+          float A = HM(a, an_);
+          if (linked) A = link(A, nb, d.inter_ex, d.scale_ex);
           hex = hex + (2 * A * w[axis]) * (mn - mc);
         }
       }
@@ -263,7 +277,9 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
           m2n = m2c + (delta / (2 * a)) * (ann * cross3(cross3(dm1, m2c), m2c) + g2);
           ann_ = ann;
         }
-        hafm_nn = hafm_nn + (HM(ann, ann_) * (idelta * idelta)) * (m2n - m2c);
+        float A = HM(ann, ann_);
+        if (nb_in) A = link(A, nb, d.inter_afm, d.scale_afm);
+        hafm_nn = hafm_nn + (A * (idelta * idelta)) * (m2n - m2c);
       }
       hafm_nn = hafm_nn / ms2;
     }
@@ -461,9 +477,10 @@ bool all_uniform(const FMDesc& d) {
   bool u = !d.msat.p && !d.aex.p && !d.alpha.p && !d.ku1.p && !d.ku2.p && !d.kc1.p && !d.kc2.p && !d.kc3.p &&
            !d.frozen.p && !d.anisU.p && !d.anisC1.p && !d.anisC2.p && !d.bias.p;
   for (int k = 0; k < D_COUNT; ++k) u = u && !d.dmi[k].p;
-  // sublattices, magnetoelastic coupling, spin-transfer torques and the
-  // thermal field use the per-cell kernel only
-  return u && !d.sublattice && !(d.terms & (TERM_ME | TERM_AFM_NN | TERM_AFM_SITE | TERM_STT | TERM_THERMAL));
+  // sublattices, inter-region exchange, magnetoelastic coupling,
+  // spin-transfer torques and the thermal field use the per-cell kernel only
+  return u && !d.sublattice && !d.regions &&
+         !(d.terms & (TERM_ME | TERM_AFM_NN | TERM_AFM_SITE | TERM_STT | TERM_THERMAL));
 }
 
 // Host-side coefficients of the uniform path, in double then rounded once.

@@ -61,6 +61,13 @@ struct FMDesc {
   PS pol, xi, lambda, epsp, fthick;
   PV jcur, fixed;
   int fixed_on_top;
+  // regions (system.hpp:79-99): per-cell region index, or nullptr when the
+  // regions change no term (no regions, or default inter-region values);
+  // exchange between cells of different regions (exchange.cu:79-99,
+  // afmexchange.cu:116-143)
+  const uint32_t* regions;
+  IP inter_ex, scale_ex;    // this ferromagnet's inter_exchange / scale_exchange
+  IP inter_afm, scale_afm;  // the host's inter_afmex_nn / scale_afmex_nn
   // thermal field (thermalnoise.cu:21-70): per cell sqrt(pref alpha T /
   // ((1 + alpha^2) Ms)) times three normal deviates of (seed, cell, purpose,
   // draw); added to the torque times noise_scale = 1/sqrt(dt) when `noise`

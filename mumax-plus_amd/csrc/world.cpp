@@ -242,6 +242,116 @@ const Param& MagnetCore::param(const std::string& key) const {
   return it->second;
 }
 
+void MagnetCore::add_inter(const std::string& key, float value, const char* unit) {
+  InterParam p;
+  p.name = name + ":" + key;
+  p.unit = unit;
+  p.uniform = value;
+  inter_params.emplace(key, std::move(p));
+}
+
+InterParam& MagnetCore::inter(const std::string& key) {
+  auto it = inter_params.find(key);
+  if (it == inter_params.end()) throw Error(MPA_ERR_INVALID, "unknown inter-region parameter '" + key + "'");
+  return it->second;
+}
+
+const InterParam& MagnetCore::inter(const std::string& key) const {
+  auto it = inter_params.find(key);
+  if (it == inter_params.end()) throw Error(MPA_ERR_INVALID, "unknown inter-region parameter '" + key + "'");
+  return it->second;
+}
+
+void MagnetCore::set_regions(const uint32_t* r) {
+  regions_host.clear();
+  unique_regions.clear();
+  regions.release();
+  if (r) {
+    regions_host.assign(r, r + N);
+    unique_regions = regions_host;
+    std::sort(unique_regions.begin(), unique_regions.end());
+    unique_regions.erase(std::unique(unique_regions.begin(), unique_regions.end()), unique_regions.end());
+    regions.allocate(N);
+    HIP_CHECK(hipMemcpy(regions.get(), regions_host.data(), (size_t)N * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  // inter_parameter.cu:7-20: N = largest index + 1, at least 1
+  const size_t nr = unique_regions.empty() ? 1 : (size_t)unique_regions.back() + 1;
+  for (auto& kv : inter_params) {
+    kv.second.limit = nr * (nr - 1) / 2;
+    kv.second.host.clear();
+    kv.second.lut.release();
+  }
+  world->touch();
+}
+
+void MagnetCore::check_region(unsigned idx) const {
+  if (!std::binary_search(unique_regions.begin(), unique_regions.end(), (uint32_t)idx))
+    throw Error(MPA_ERR_INVALID, "The region index " + std::to_string(idx) + " is not defined.");
+}
+
+void MagnetCore::param_set_in_region(const std::string& key, unsigned idx, const double* value) {
+  check_region(idx);
+  Param& p = param(key);
+  const size_t n = (size_t)p.ncomp * N;
+  std::vector<float> h(n);
+  if (p.cells.n == 0) {
+    bool same = true;
+    for (int c = 0; c < p.ncomp; ++c) same = same && (float)value[c] == p.uniform[c];
+    if (same) return;
+    for (int c = 0; c < p.ncomp; ++c) std::fill(h.begin() + (size_t)c * N, h.begin() + (size_t)(c + 1) * N, p.uniform[c]);
+    p.cells.allocate(n);
+    // the per-cell values now hold the static value; a time term adds to them
+    for (int c = 0; c < 3; ++c) p.uniform[c] = 0.f;
+  } else {
+    HIP_CHECK(hipMemcpy(h.data(), p.cells.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < N; ++i)
+    if (regions_host[i] == idx)
+      for (int c = 0; c < p.ncomp; ++c) h[(size_t)c * N + i] = (float)value[c];
+  HIP_CHECK(hipMemcpy(p.cells.get(), h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  p.changed();
+  world->touch();
+}
+
+void MagnetCore::inter_set(const std::string& key, double value) {
+  InterParam& p = inter(key);
+  p.uniform = (float)value;
+  p.host.clear();
+  p.lut.release();
+  world->touch();
+}
+
+void MagnetCore::inter_set_between(const std::string& key, unsigned i, unsigned j, double value) {
+  check_region(i);
+  check_region(j);
+  InterParam& p = inter(key);
+  if (i == j)
+    throw Error(MPA_ERR_INVALID,
+                "Can not set " + p.name + ", region indexes must be different: " + std::to_string(i) + ".");
+  if (p.is_uniform()) {
+    if ((float)value == p.uniform) return;
+    if (p.limit > ((size_t)1 << 28))
+      throw Error(MPA_ERR_INVALID, "Can not set " + p.name + ": region indices up to " +
+                                       std::to_string(unique_regions.back()) + " need too large a table.");
+    p.host.assign(p.limit, p.uniform);
+    p.lut.allocate(p.limit);
+  }
+  const size_t k = IP::index(i, j);
+  p.host[k] = (float)value;
+  HIP_CHECK(hipMemcpy(p.lut.get(), p.host.data(), p.limit * sizeof(float), hipMemcpyHostToDevice));
+  world->touch();
+}
+
+double MagnetCore::inter_get_between(const std::string& key, unsigned i, unsigned j) const {
+  check_region(i);
+  check_region(j);
+  const InterParam& p = inter(key);
+  if (i == j)
+    throw Error(MPA_ERR_INVALID,
+                "Can not get " + p.name + ", region indexes must be different: " + std::to_string(i) + ".");
+  return p.is_uniform() ? p.uniform : p.host[IP::index(i, j)];
+}
+
 // Magnet::setEnableElastodynamics (magnet.cpp:177-213): u and v start at zero.
 void MagnetCore::set_elastodynamics(bool on) {
   if (on == el.enabled) return;
@@ -481,6 +591,9 @@ Ferromagnet::Ferromagnet(World* w, const int size[3], const int org[3], const ui
                          Antiferromagnet* h)
     : MagnetCore(w, size, org, geometry, std::move(nm)), host(h) {
   add_params(kFmParams, (int)(sizeof(kFmParams) / sizeof(kFmParams[0])));
+  // ferromagnet.cpp:26-27
+  add_inter("inter_exchange", 0.f, "J/m");
+  add_inter("scale_exchange", 1.f, "");
   param("free_layer_thickness").uniform[0] = (float)n[2] * (float)w->cellsize[2];
   noise_seed = 0x9e3779b9u * (unsigned)(1 + w->magnets.size() + 7 * w->afms.size()) +
                (host ? (unsigned)(name.back() - '0') * 0x85ebca6bu : 0u);
@@ -514,8 +627,9 @@ unsigned Ferromagnet::active_terms() {
   auto az = [&](const char* k) { return params.at(k).assured_zero(); };
   const bool msat0 = az("msat");
   unsigned t = 0;
-  // exchange.cu:14-18 (no region-pair exchange in this build)
-  if (!(az("aex") || msat0)) t |= TERM_EXCH;
+  // exchange.cu:14-18
+  if (!((az("aex") && inter("inter_exchange").assured_zero()) || inter("scale_exchange").assured_zero() || msat0))
+    t |= TERM_EXCH;
   // anisotropy.cu:9-23 and the uniaxial-else-cubic choice of :123-139
   const bool uni0 = (az("ku1") && az("ku2")) || az("anisU") || msat0;
   const bool cub0 = (az("kc1") && az("kc2") && az("kc3")) || (az("anisC1") && az("anisC2")) || msat0;
@@ -615,6 +729,27 @@ FMDesc Ferromagnet::desc(unsigned terms) {
     d.latcon = host->param("latcon").ps();
     for (int k = 0; k < D_COUNT; ++k) d.hdmi[k] = host->param(kDmiKeys[k]).ps();
   }
+  // inter-region exchange: the regions matter when a pair value or scale is
+  // not the default (inter 0: the harmonic mean, scale 1)
+  auto trivial = [](const InterParam& in, const InterParam& sc) {
+    return in.assured_zero() && sc.is_uniform() && sc.uniform == 1.f;
+  };
+  const InterParam& ie = inter("inter_exchange");
+  const InterParam& se = inter("scale_exchange");
+  d.inter_ex = ie.ip();
+  d.scale_ex = se.ip();
+  bool matter = !trivial(ie, se);
+  if (host) {
+    const InterParam& ia = host->inter("inter_afmex_nn");
+    const InterParam& sa = host->inter("scale_afmex_nn");
+    d.inter_afm = ia.ip();
+    d.scale_afm = sa.ip();
+    matter = matter || !trivial(ia, sa);
+  } else {
+    d.inter_afm = IP{nullptr, 0.f};
+    d.scale_afm = IP{nullptr, 1.f};
+  }
+  d.regions = (matter && regions.n) ? regions.get() : nullptr;
   return d;
 }
 
@@ -947,11 +1082,20 @@ Antiferromagnet::Antiferromagnet(World* w, const int size[3], const int org[3], 
     : MagnetCore(w, size, org, geometry, std::move(nm)) {
   add_params(kAfmParams, (int)(sizeof(kAfmParams) / sizeof(kAfmParams[0])));
   add_params(kElasticParams, (int)(sizeof(kElasticParams) / sizeof(kElasticParams[0])));
+  // antiferromagnet.cpp:22-23
+  add_inter("inter_afmex_nn", 0.f, "J/m");
+  add_inter("scale_afmex_nn", 1.f, "");
   const uint8_t* g = geom_host.empty() ? nullptr : geom_host.data();
   sub[0] = std::make_unique<Ferromagnet>(w, size, org, g, name + ":sublattice_1", this);
   sub[1] = std::make_unique<Ferromagnet>(w, size, org, g, name + ":sublattice_2", this);
   hdemag.allocate((size_t)3 * N);
   scratch.allocate((size_t)3 * N);
+}
+
+void Antiferromagnet::set_regions(const uint32_t* r) {
+  MagnetCore::set_regions(r);
+  sub[0]->set_regions(r);
+  sub[1]->set_regions(r);
 }
 
 DemagPlan& Antiferromagnet::demag() {
@@ -1558,26 +1702,29 @@ void World::check_name(const std::string& name) {
   if (find(name) || find_afm(name)) throw Error(MPA_ERR_INVALID, "A magnet with the name '" + name + "' already exists");
 }
 
-Ferromagnet* World::add_ferromagnet(const int size[3], const int org[3], const uint8_t* geometry, const char* nm) {
+Ferromagnet* World::add_ferromagnet(const int size[3], const int org[3], const uint8_t* geometry,
+                                    const uint32_t* regions, const char* nm) {
   static int unnamed = 1;  // ferromagnet_<n> naming of mumaxworld.cpp:61-64
   std::string name = nm ? nm : "";
   if (name.empty()) name = "ferromagnet_" + std::to_string(unnamed++);
   check_name(name);
   check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
   magnets.push_back(std::make_unique<Ferromagnet>(this, size, org, geometry, name));
+  magnets.back()->set_regions(regions);
   touch();
   solver.equations_changed();
   return magnets.back().get();
 }
 
 Antiferromagnet* World::add_antiferromagnet(const int size[3], const int org[3], const uint8_t* geometry,
-                                            const char* nm) {
+                                            const uint32_t* regions, const char* nm) {
   static int unnamed = 1;  // antiferromagnet_<n> naming of mumaxworld.cpp:86-89
   std::string name = nm ? nm : "";
   if (name.empty()) name = "antiferromagnet_" + std::to_string(unnamed++);
   check_name(name);
   check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
   afms.push_back(std::make_unique<Antiferromagnet>(this, size, org, geometry, name));
+  afms.back()->set_regions(regions);
   touch();
   solver.equations_changed();
   return afms.back().get();

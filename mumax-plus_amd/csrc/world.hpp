@@ -90,6 +90,20 @@ struct Param {
   mutable double at_ = 0.0;
 };
 
+// A value between pairs of regions (InterParameter,
+// src/core/inter_parameter.hpp:13-55): uniform, or one value per pair i != j
+// of region indices below max + 1, in IP's lower-triangle order.
+struct InterParam {
+  std::string name, unit;
+  float uniform = 0.f;
+  size_t limit = 0;            // N (N - 1) / 2, N = largest region index + 1 (at least 1)
+  std::vector<float> host;     // the table, when non-uniform
+  DeviceArray<float> lut;
+  bool is_uniform() const { return lut.n == 0; }
+  bool assured_zero() const { return is_uniform() && uniform == 0.f; }
+  IP ip() const { return IP{lut.get(), uniform}; }
+};
+
 struct Tableau {
   std::vector<double> nodes;
   std::vector<std::vector<double>> A;
@@ -122,6 +136,12 @@ class MagnetCore {
   DeviceArray<uint8_t> geom;
   int cellsingeo;
   std::map<std::string, Param> params;
+  // regions (System::regions_, system.cpp:13-36): one index per cell, or
+  // empty (every cell in region 0, and no region is defined)
+  std::vector<uint32_t> regions_host;
+  DeviceArray<uint32_t> regions;
+  std::vector<uint32_t> unique_regions;  // sorted
+  std::map<std::string, InterParam> inter_params;
   ElasticState el;
   // Magnet::enableAsStrayFieldSource / Destination (magnet.cpp:23-24); a
   // sublattice follows its host's flags (zeeman.cu:13-22).
@@ -146,6 +166,20 @@ class MagnetCore {
   Param& param(const std::string& key);
   const Param& param(const std::string& key) const;
   bool has_param(const std::string& key) const { return params.count(key) != 0; }
+  InterParam& inter(const std::string& key);
+  const InterParam& inter(const std::string& key) const;
+  // Set once, right after construction (and passed on to sublattices);
+  // sizes the inter-region tables (inter_parameter.cu:7-20).
+  virtual void set_regions(const uint32_t* regions);
+  // System::checkIdxInRegions (system.cpp:73-77)
+  void check_region(unsigned idx) const;
+  // Parameter::setInRegion (parameter.cu:41-47, 144-150): the static value
+  // in the cells of region idx
+  void param_set_in_region(const std::string& key, unsigned idx, const double* value);
+  // InterParameter::set / setBetween / getBetween (inter_parameter.cu:38-82)
+  void inter_set(const std::string& key, double value);
+  void inter_set_between(const std::string& key, unsigned i, unsigned j, double value);
+  double inter_get_between(const std::string& key, unsigned i, unsigned j) const;
 
   // Elastodynamics of a magnet that is not a sublattice.
   void set_elastodynamics(bool on);
@@ -170,6 +204,7 @@ class MagnetCore {
 
  protected:
   void add_params(const void* specs, int count);
+  void add_inter(const std::string& key, float value, const char* unit);
 };
 
 class Ferromagnet : public MagnetCore {
@@ -246,6 +281,7 @@ class Antiferromagnet : public MagnetCore {
 
   int num_fm() const override { return 2; }
   Ferromagnet* fm_at(int i) override { return sub[i].get(); }
+  void set_regions(const uint32_t* regions) override;  // the sublattices share the host's regions
   bool stray_source_of(StraySource& out) override;
 
   // Quantities of the whole antiferromagnet (neel_vector, full_magnetization
@@ -317,9 +353,11 @@ class World {
   Profiler prof;
   unsigned long long version = 1;  // bumped by every state change (invalidates FSAL reuse)
 
-  Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry, const char* name);
+  // regions: one index per cell, or nullptr (mumaxworld.cpp:57-103)
+  Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry,
+                               const uint32_t* regions, const char* name);
   Antiferromagnet* add_antiferromagnet(const int size[3], const int origin[3], const uint8_t* geometry,
-                                       const char* name);
+                                       const uint32_t* regions, const char* name);
   Ferromagnet* find(const std::string& name);
   Antiferromagnet* find_afm(const std::string& name);
   // every ferromagnet with a magnetisation: independent ones and sublattices

@@ -16,6 +16,7 @@ from .timesolver import TimeSolver, TimeSolverOutput
 from .fieldquantity import FieldQuantity
 from .scalarquantity import ScalarQuantity
 from .parameter import Parameter
+from .interparameter import InterParameter
 from .variable import Variable
 from .dmitensor import DmiTensor, DmiTensorGroup
 from .strayfield import StrayField
@@ -25,4 +26,4 @@ from . import util
 
 __all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
            "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
-           "DmiTensorGroup", "StrayField", "Batch", "serving", "util"]
+           "DmiTensorGroup", "InterParameter", "StrayField", "Batch", "serving", "util"]

@@ -44,7 +44,7 @@ _SIGS = {
     "mpa_world_get_cellsize": ([_P, _P], _cint),
     "mpa_world_set_bias_magnetic_field": ([_P, _P], _cint),
     "mpa_world_get_bias_magnetic_field": ([_P, _P], _cint),
-    "mpa_world_add_ferromagnet": ([_P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_world_add_ferromagnet": ([_P, _P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
     "mpa_world_num_ferromagnets": ([_P, _ct.POINTER(_cint)], _cint),
     "mpa_world_ferromagnet_at": ([_P, _cint, _ct.POINTER(_P)], _cint),
     "mpa_world_find_ferromagnet": ([_P, _c_char_p, _ct.POINTER(_P)], _cint),
@@ -69,6 +69,15 @@ _SIGS = {
     "mpa_fm_param_add_time_term": ([_P, _c_char_p, _TIME_FN, _P, _P], _cint),
     "mpa_fm_param_remove_time_terms": ([_P, _c_char_p], _cint),
     "mpa_fm_param_is_dynamic": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_param_set_in_region": ([_P, _c_char_p, _ct.c_uint, _P], _cint),
+    "mpa_fm_regions": ([_P, _P], _cint),
+    "mpa_fm_unique_regions": ([_P, _P, _cint, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_inter_info": ([_P, _c_char_p, _ct.POINTER(_cint), _ct.POINTER(_cdouble), _ct.POINTER(_ct.c_longlong),
+                            _c_char_p, _cint, _c_char_p, _cint], _cint),
+    "mpa_fm_inter_set": ([_P, _c_char_p, _cdouble], _cint),
+    "mpa_fm_inter_set_between": ([_P, _c_char_p, _ct.c_uint, _ct.c_uint, _cdouble], _cint),
+    "mpa_fm_inter_get_between": ([_P, _c_char_p, _ct.c_uint, _ct.c_uint, _ct.POINTER(_cdouble)], _cint),
+    "mpa_fm_inter_eval": ([_P, _c_char_p, _P], _cint),
     "mpa_fm_set_magnetization": ([_P, _P, _ct.c_longlong], _cint),
     "mpa_fm_get_magnetization": ([_P, _P], _cint),
     "mpa_fm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
@@ -79,7 +88,7 @@ _SIGS = {
     "mpa_fm_host": ([_P, _ct.POINTER(_P)], _cint),
     "mpa_fm_set_variable": ([_P, _c_char_p, _P, _ct.c_longlong], _cint),
     "mpa_fm_get_variable": ([_P, _c_char_p, _P], _cint),
-    "mpa_world_add_antiferromagnet": ([_P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
+    "mpa_world_add_antiferromagnet": ([_P, _P, _P, _P, _P, _c_char_p, _ct.POINTER(_P)], _cint),
     "mpa_world_num_antiferromagnets": ([_P, _ct.POINTER(_cint)], _cint),
     "mpa_world_antiferromagnet_at": ([_P, _cint, _ct.POINTER(_P)], _cint),
     "mpa_world_find_antiferromagnet": ([_P, _c_char_p, _ct.POINTER(_P)], _cint),
@@ -99,6 +108,15 @@ _SIGS = {
     "mpa_afm_param_add_time_term": ([_P, _c_char_p, _TIME_FN, _P, _P], _cint),
     "mpa_afm_param_remove_time_terms": ([_P, _c_char_p], _cint),
     "mpa_afm_param_is_dynamic": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_afm_param_set_in_region": ([_P, _c_char_p, _ct.c_uint, _P], _cint),
+    "mpa_afm_regions": ([_P, _P], _cint),
+    "mpa_afm_unique_regions": ([_P, _P, _cint, _ct.POINTER(_cint)], _cint),
+    "mpa_afm_inter_info": ([_P, _c_char_p, _ct.POINTER(_cint), _ct.POINTER(_cdouble), _ct.POINTER(_ct.c_longlong),
+                            _c_char_p, _cint, _c_char_p, _cint], _cint),
+    "mpa_afm_inter_set": ([_P, _c_char_p, _cdouble], _cint),
+    "mpa_afm_inter_set_between": ([_P, _c_char_p, _ct.c_uint, _ct.c_uint, _cdouble], _cint),
+    "mpa_afm_inter_get_between": ([_P, _c_char_p, _ct.c_uint, _ct.c_uint, _ct.POINTER(_cdouble)], _cint),
+    "mpa_afm_inter_eval": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_set_variable": ([_P, _c_char_p, _P, _ct.c_longlong], _cint),
     "mpa_afm_get_variable": ([_P, _c_char_p, _P], _cint),
     "mpa_afm_quantity_ncomp": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
@@ -257,7 +275,17 @@ class System:
     @property
     def regions(self):
         nx, ny, nz = self.grid.size
-        return _np.zeros((nz, ny, nx), dtype=_np.uint32)
+        out = _np.empty(nx * ny * nz, dtype=_np.uint32)
+        _call(self._fm, "regions", _ptr(out))
+        return out.reshape(nz, ny, nx)
+
+    @property
+    def unique_regions(self):
+        n = _cint()
+        _call(self._fm, "unique_regions", None, 0, _ct.byref(n))
+        out = _np.empty(max(n.value, 1), dtype=_np.uint32)
+        _call(self._fm, "unique_regions", _ptr(out), n.value, _ct.byref(n))
+        return [int(v) for v in out[:n.value]]
 
     def cell_position(self, idx):
         cs = self.cellsize
@@ -469,6 +497,12 @@ class Parameter(FieldQuantity):
         a = _np.ascontiguousarray(a)
         _call(self._fm, "param_set_cells", self._key, _ptr(a))
 
+    def set_in_region(self, region_idx, value):
+        vals = _np.ascontiguousarray(_np.atleast_1d(_np.asarray(value, dtype=_np.float64)))
+        if vals.size != self.ncomp:
+            raise ValueError(f"expected {self.ncomp} components")
+        _call(self._fm, "param_set_in_region", self._key, int(region_idx), _ptr(vals))
+
     def add_time_term(self, term, mask=None):
         """term(t) -> ncomp values; mask None or an array of shape
         (ncomp, nz, ny, nx) (dynamic_parameter.hpp:20-48)."""
@@ -514,6 +548,72 @@ class VectorParameter(Parameter):
     pass
 
 
+# ------------------------------------------------------ InterParameter
+class InterParameter:
+    """Value between pairs of regions (wrap_parameter.cpp:109-125)."""
+
+    ncomp = 1
+
+    def __init__(self, magnet, key):
+        self._fm = magnet
+        self._key = _enc(key)
+
+    def _info(self):
+        uni, val, size = _cint(), _cdouble(), _ct.c_longlong()
+        nm = _ct.create_string_buffer(256)
+        un = _ct.create_string_buffer(64)
+        _call(self._fm, "inter_info", self._key, _ct.byref(uni), _ct.byref(val), _ct.byref(size), nm, 256, un, 64)
+        return bool(uni.value), val.value, size.value, nm.value.decode(), un.value.decode()
+
+    @property
+    def name(self):
+        return self._info()[3]
+
+    @property
+    def unit(self):
+        return self._info()[4]
+
+    @property
+    def is_uniform(self):
+        return self._info()[0]
+
+    @property
+    def uniform_value(self):
+        uni, val, _, name, _ = self._info()
+        if not uni:
+            raise ValueError(f"Cannot get uniform value of non-uniform InterParameter {name}.")
+        return val
+
+    @uniform_value.setter
+    def uniform_value(self, value):
+        self.set(value)
+
+    def set(self, value):
+        _call(self._fm, "inter_set", self._key, float(value))
+
+    def set_between(self, i, j, value):
+        _call(self._fm, "inter_set_between", self._key, int(i), int(j), float(value))
+
+    def get_between(self, i, j):
+        out = _cdouble()
+        _call(self._fm, "inter_get_between", self._key, int(i), int(j), _ct.byref(out))
+        return out.value
+
+    def eval(self):
+        size = self._info()[2]
+        out = _np.empty(max(size, 1), dtype=_np.float32)
+        _call(self._fm, "inter_eval", self._key, _ptr(out))
+        return [float(v) for v in out[:size]]
+
+    @property
+    def unique_regions(self):
+        return self._fm.system.unique_regions
+
+    @property
+    def number_of_regions(self):
+        return len(self.unique_regions)
+
+
 class DmiTensor:
     """The nine DMI parameters (wrap_dmitensor.cpp; dmitensor.hpp:12-22)."""
 
@@ -531,6 +631,7 @@ class _Magnet:
     _pfx = None
     _PARAMS = ()
     _VPARAMS = ()
+    _INTER = ()
     _ELASTIC_PARAMS = ("C11", "C12", "C44", "eta", "rho")
     _ELASTIC_VPARAMS = ("external_body_force", "rigid_norm_strain", "rigid_shear_strain")
 
@@ -539,6 +640,7 @@ class _Magnet:
         object.__setattr__(self, "_world", world)
         params = {k: Parameter(self, k) for k in self._PARAMS}
         params.update({k: VectorParameter(self, k) for k in self._VPARAMS})
+        params.update({k: InterParameter(self, k) for k in self._INTER})
         if self._has_elastic_params():
             params.update({k: Parameter(self, k) for k in self._ELASTIC_PARAMS})
             params.update({k: VectorParameter(self, k) for k in self._ELASTIC_VPARAMS})
@@ -651,6 +753,7 @@ class Ferromagnet(_Magnet):
     _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "B1", "B2",
                "temperature", "xi", "Lambda", "free_layer_thickness", "epsilon_prime", "pol")
     _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field", "fixed_layer", "jcur")
+    _INTER = ("inter_exchange", "scale_exchange")
 
     def __init__(self, handle, world):
         super().__init__(handle, world)
@@ -738,6 +841,7 @@ class Antiferromagnet(_Magnet):
 
     _pfx = "afm"
     _PARAMS = ("afmex_cell", "afmex_nn", "latcon")
+    _INTER = ("inter_afmex_nn", "scale_afmex_nn")
 
     def __init__(self, handle, world):
         super().__init__(handle, world)
@@ -933,9 +1037,19 @@ class World:
         _check(_lib.mpa_world_set_bias_magnetic_field(self._h, _as_d3(b, "bias_magnetic_field")))
 
     @staticmethod
+    def _region_words(grid, regions_array):
+        if regions_array is None:
+            return None
+        nx, ny, nz = grid.size
+        r = _np.asarray(regions_array)
+        if r.size != nx * ny * nz:
+            raise RuntimeError("The size of the region buffer does not match the size of the system.")
+        if _np.any(r < 0):
+            raise ValueError("region indices must be non-negative")
+        return _np.ascontiguousarray(r.reshape(-1).astype(_np.uint32))
+
+    @staticmethod
     def _geometry_bytes(grid, geometry_array, regions_array):
-        if regions_array is not None and _np.any(_np.asarray(regions_array) != 0):
-            raise NotImplementedError("regions are not supported by this build")
         if geometry_array is None:
             return None
         nx, ny, nz = grid.size
@@ -946,9 +1060,11 @@ class World:
 
     def add_ferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
         geo = self._geometry_bytes(grid, geometry_array, regions_array)
+        reg = self._region_words(grid, regions_array)
         h = _P()
         _check(_lib.mpa_world_add_ferromagnet(self._h, _i3(*grid.size), _i3(*grid.origin),
-                                              None if geo is None else _ptr(geo), _enc(name or ""),
+                                              None if geo is None else _ptr(geo),
+                                              None if reg is None else _ptr(reg), _enc(name or ""),
                                               _ct.byref(h)))
         fm = Ferromagnet(h, self)
         self._magnets[fm.name] = fm
@@ -956,9 +1072,11 @@ class World:
 
     def add_antiferromagnet(self, grid, geometry_array=None, regions_array=None, name=""):
         geo = self._geometry_bytes(grid, geometry_array, regions_array)
+        reg = self._region_words(grid, regions_array)
         h = _P()
         _check(_lib.mpa_world_add_antiferromagnet(self._h, _i3(*grid.size), _i3(*grid.origin),
-                                                  None if geo is None else _ptr(geo), _enc(name or ""),
+                                                  None if geo is None else _ptr(geo),
+                                                  None if reg is None else _ptr(reg), _enc(name or ""),
                                                   _ct.byref(h)))
         afm = Antiferromagnet(h, self)
         self._afms[afm.name] = afm

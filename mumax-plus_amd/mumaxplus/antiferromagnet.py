@@ -4,9 +4,12 @@ and inhomogeneous (afmex_nn) antiferromagnetic exchange and an
 inter-sublattice DMI tensor, sharing one demag field and one elastic
 displacement."""
 
+import warnings
+
 from .dmitensor import DmiTensor, DmiTensorGroup
 from .ferromagnet import Ferromagnet, _warn_positive
 from .fieldquantity import FieldQuantity
+from .interparameter import InterParameter
 from .magnet import Magnet
 from .parameter import Parameter
 from .scalarquantity import ScalarQuantity
@@ -106,6 +109,29 @@ class Antiferromagnet(Magnet):
         _warn_positive(self.afmex_nn, "The antiferromagnetic exchange constant afmex_nn is set to a "
                                       "positive value, instead of negative (or zero). Make sure this is "
                                       "intentional!")
+
+    @property
+    def inter_afmex_nn(self):
+        """Antiferromagnetic exchange constant (J/m) between different regions;
+        zero (the default) uses the harmonic mean of the two regions' afmex_nn
+        (antiferromagnet.py:228-250 of the reference)."""
+        return InterParameter(self._impl.inter_afmex_nn)
+
+    @inter_afmex_nn.setter
+    def inter_afmex_nn(self, value):
+        if value > 0:
+            warnings.warn("The antiferromagnetic exchange constant inter_afmex_nn is set to a positive value, "
+                          "instead of negative (or zero). Make sure this is intentional!", UserWarning)
+        self.inter_afmex_nn.set(value)
+
+    @property
+    def scale_afmex_nn(self):
+        """Factor on the antiferromagnetic exchange between different regions (default 1)."""
+        return InterParameter(self._impl.scale_afmex_nn)
+
+    @scale_afmex_nn.setter
+    def scale_afmex_nn(self, value):
+        self.scale_afmex_nn.set(value)
 
     @property
     def latcon(self):

@@ -5,6 +5,7 @@ import warnings
 import numpy as _np
 
 from .dmitensor import DmiTensor
+from .interparameter import InterParameter
 from .magnet import Magnet
 from .variable import Variable
 
@@ -172,6 +173,26 @@ class Ferromagnet(Magnet):
         self.B2.set(value)
         _warn_positive(self.B2, "The second magnetoelastic coupling constant B2 is set to a positive value, "
                                 "instead of negative (or zero). Make sure this is intentional!")
+
+    @property
+    def inter_exchange(self):
+        """Exchange constant (J/m) between different regions; zero (the default)
+        uses the harmonic mean of the two regions' aex (ferromagnet.py:248-270
+        of the reference)."""
+        return InterParameter(self._impl.inter_exchange)
+
+    @inter_exchange.setter
+    def inter_exchange(self, value):
+        self.inter_exchange.set(value)
+
+    @property
+    def scale_exchange(self):
+        """Factor on the exchange between different regions (default 1)."""
+        return InterParameter(self._impl.scale_exchange)
+
+    @scale_exchange.setter
+    def scale_exchange(self, value):
+        self.scale_exchange.set(value)
 
     def _lookup_param(self, key):
         from .parameter import Parameter

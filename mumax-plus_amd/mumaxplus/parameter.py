@@ -89,4 +89,8 @@ class Parameter(FieldQuantity):
             self._impl.set(value)
 
     def set_in_region(self, region_idx, value):
-        raise NotImplementedError("regions are not supported by this build")
+        """A uniform, static value in the cells of one region (parameter.py:216-224
+        of the reference)."""
+        assert (isinstance(value, (float, int)) or
+                (isinstance(value, tuple) and len(value) == 3)), "The value should be uniform and static."
+        self._impl.set_in_region(region_idx, value)

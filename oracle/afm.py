@@ -89,7 +89,7 @@ def exchange_field_sublattice(sub, m, m2):
         include = np.where(nb_in, ms[nb] != 0, a != 0)
         m_ = np.where(nb_in[None], m_nb, m_neu)
         a_ = np.where(nb_in, a[nb], a)
-        A = harmonic_mean(a, a_)
+        A = sub.region_link("inter_exchange", "scale_exchange", nb_in, nb, harmonic_mean(a, a_))
         h += np.where(include[None], 2 * A * w[axis] * (m_ - m), 0.0)
     act = sub.geom_mask() & (ms != 0)
     with np.errstate(divide="ignore", invalid="ignore"):
@@ -181,7 +181,7 @@ def afm_nn_field(sub, m1, m2):
             m2_neu = m2 + (ann * cross(cross(d_m1, m2), m2) + g2) * delta / (2 * a)
         m2_ = np.where(nb_in[None], m2[:, nb[0], nb[1], nb[2]], m2_neu)
         ann_ = np.where(nb_in, ann[nb], ann)
-        A = harmonic_mean(ann, ann_)
+        A = sub.host.region_link("inter_afmex_nn", "scale_afmex_nn", nb_in, nb, harmonic_mean(ann, ann_))
         h += np.where(include[None], np.nan_to_num(A * (m2_ - m2) / (delta * delta)), 0.0)
     ok = sub.geom_mask() & (ms2 != 0)
     with np.errstate(divide="ignore", invalid="ignore"):

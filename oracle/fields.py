@@ -111,7 +111,7 @@ def exchange_field(mag, m):
             include = nb_in | (a != 0)
         m_ = np.where(nb_in[None], m_nb, m_neu)
         a_ = np.where(nb_in, a_nb, a)
-        A = harmonic_mean(a, a_)
+        A = mag.region_link("inter_exchange", "scale_exchange", nb_in, nb, harmonic_mean(a, a_))
         contrib = 2 * A * w[axis] * (m_ - m)
         h += np.where(include[None], contrib, 0.0)
     act = _active(mag)

@@ -99,6 +99,41 @@ class _Params:
     def dmi_arrays(self):
         return {k: self.val(k) for k in DMI_KEYS}
 
+    # Regions and values between pairs of regions (System::regions_,
+    # system.cpp:13-36; InterParameter, inter_parameter.cu:7-82): `regions` is
+    # None (every cell in region 0) or an int array of the grid's shape;
+    # inter[name] is a scalar (every pair) or a symmetric matrix indexed by
+    # region.
+    def set_between(self, name, i, j, value):
+        if i == j:
+            raise ValueError("region indexes must be different")
+        v = self.inter[name]
+        if np.ndim(v) == 0:
+            n = int(np.max(self.regions)) + 1
+            v = np.full((n, n), float(v))
+        v[i, j] = v[j, i] = value
+        self.inter[name] = v
+
+    def inter_assured_zero(self, name):
+        v = self.inter[name]
+        return np.ndim(v) == 0 and v == 0
+
+    def region_link(self, inter_key, scale_key, nb_in, nb, hm):
+        """Exchange constant of the links to in-geometry neighbours in
+        different regions: the inter-region value when non-zero, else hm,
+        times the inter-region scale (exchange.cu:79-99)."""
+        if self.regions is None:
+            return hm
+        r = self.regions
+        r_ = r[nb]
+        diff = nb_in & (r != r_)
+
+        def pair(v):
+            return np.full(r.shape, float(v)) if np.ndim(v) == 0 else v[r, r_]
+
+        iv, sv = pair(self.inter[inter_key]), pair(self.inter[scale_key])
+        return np.where(diff, np.where(iv != 0, iv, hm) * sv, hm)
+
     # elasticity of the magnet that owns the displacement (magnet.cpp:26-34)
     def elasticity_assured_zero(self):
         """elastodynamics.cu:47-51."""
@@ -124,6 +159,8 @@ class Ferromagnet(_Params):
         self.params = dict(SCALAR_DEFAULTS)
         self.params.update(VECTOR_DEFAULTS)
         self.dmi = {k: 0.0 for k in DMI_KEYS}
+        self.regions = None
+        self.inter = {"inter_exchange": 0.0, "scale_exchange": 1.0}
         self.enable_demag = True
         self.openbc = False
         # free layer thickness defaults to the grid's thickness, an FP32 product
@@ -159,7 +196,10 @@ class Ferromagnet(_Params):
             self._plan = None
 
     def exchange_assured_zero(self):
-        return self.assured_zero("aex") or self.assured_zero("msat")
+        """exchange.cu:14-18."""
+        return ((self.assured_zero("aex") and self.inter_assured_zero("inter_exchange"))
+                or (np.ndim(self.inter["scale_exchange"]) == 0 and self.inter["scale_exchange"] == 0)
+                or self.assured_zero("msat"))
 
     def dmi_assured_zero(self):
         return self.assured_zero("msat") or all(self.assured_zero(k) for k in DMI_KEYS)
@@ -248,6 +288,7 @@ class Antiferromagnet(_Params):
         self.cellsize = self.sub1.cellsize
         self.params = dict(AFM_DEFAULTS)
         self.dmi = {k: 0.0 for k in DMI_KEYS}
+        self.inter = {"inter_afmex_nn": 0.0, "scale_afmex_nn": 1.0}
         self.enable_elastodynamics = False
         self.u = np.zeros((3,) + self.shape)
         self.v = np.zeros((3,) + self.shape)
@@ -260,6 +301,16 @@ class Antiferromagnet(_Params):
     @property
     def sublattices(self):
         return (self.sub1, self.sub2)
+
+    @property
+    def regions(self):
+        return self.sub1.regions
+
+    @regions.setter
+    def regions(self, r):
+        # the sublattices share the antiferromagnet's regions
+        for sub in (self.sub1, self.sub2):
+            sub.regions = None if r is None else np.asarray(r, dtype=int)
 
     def neighbor(self, d):
         return self.sub1.neighbor(d)

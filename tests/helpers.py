@@ -17,13 +17,14 @@ def random_unit(rng, shape):
 
 
 def make_pair(size, cellsize, params, m0, geometry=None, mastergrid=(0, 0, 0), pbc=(0, 0, 0), openbc=False,
-              demag=True, world_bias=(0.0, 0.0, 0.0), dmi=None):
+              demag=True, world_bias=(0.0, 0.0, 0.0), dmi=None, regions=None):
     """Return (product world, product magnet, oracle world, oracle magnet)."""
     nx, ny, nz = size
     world = mp.World(cellsize, pbc_repetitions=tuple(pbc), mastergrid=mp.Grid(tuple(mastergrid)))
-    mag = mp.Ferromagnet(world, mp.Grid(size), geometry=geometry)
+    mag = mp.Ferromagnet(world, mp.Grid(size), geometry=geometry, regions=regions)
     ow = omodel.World(cellsize, mastergrid=tuple(mastergrid), pbc=tuple(pbc))
     om = ow.add_ferromagnet(size, geometry=geometry)
+    om.regions = None if regions is None else np.asarray(regions, dtype=int)
     mag.enable_demag = demag
     om.enable_demag = demag
     mag.enable_openbc = openbc
