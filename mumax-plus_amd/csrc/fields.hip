@@ -576,6 +576,35 @@ void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t
   checkLaunch("k_field_uniform_b");
 }
 
+__global__ void k_max_angle(FMDesc d, const float* __restrict__ m, float* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  if ((d.geom && !d.geom[idx]) || d.msat.at(idx) == 0.f) {
+    out[idx] = 0.f;
+    return;
+  }
+  const int x = idx % d.nx, y = (idx / d.nx) % d.ny, z = idx / (d.nx * d.ny);
+  const float3 mc = load3(m, idx, N);
+  const float a = d.aex.at(idx);
+  float best = 0.f;
+  for (int dir = 0; dir < 6; ++dir) {
+    const int nb = neighbour(d, x, y, z, dir);
+    if (nb < 0 || (d.geom && !d.geom[nb]) || d.msat.at(nb) == 0.f) continue;
+    const float3 mn = load3(m, nb, N);
+    const bool same = mn.x == mc.x && mn.y == mc.y && mn.z == mc.z;
+    // acos of a dot product rounded above 1 is NaN and never the maximum
+    const float ang = same ? 0.f : acosf(dot3(mc, mn));
+    if (harmonic_mean(a, d.aex.at(nb)) != 0.f && ang > best) best = ang;
+  }
+  out[idx] = best;
+}
+
+void launch_max_angle(const FMDesc& d, const float* m, float* out, hipStream_t s) {
+  k_max_angle<<<(d.ncell + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(d, m, out);
+  checkLaunch("k_max_angle");
+}
+
 __global__ void k_param_fill(float* out, const float* base, const ParamFill f, int ncomp,
                              int N) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

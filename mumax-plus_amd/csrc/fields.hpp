@@ -91,6 +91,11 @@ enum FieldMode {
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
 
+// Per cell the largest angle (rad) between m and an in-geometry neighbour with
+// msat != 0 and a non-zero harmonic-mean exchange constant; 0 outside the
+// geometry and where msat = 0 (k_maxangle, exchange.cu:271-313).
+void launch_max_angle(const FMDesc& d, const float* m, float* out, hipStream_t s);
+
 // Per-cell values of a parameter with time-dependent terms:
 //   out[c N + i] = (base ? base[c N + i] : 0) + add[c] + sum_k coef[k][c] mask[k][c N + i]
 // for c < ncomp (dynamic_parameter.cpp:8-29 plus the static field).

@@ -196,6 +196,17 @@ __global__ void __launch_bounds__(BLOCK) k_combo_maxnorm(Combo co, int N, const 
 
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
+__global__ void __launch_bounds__(BLOCK) k_maxabs(const float* __restrict__ v, int N, const uint8_t* __restrict__ geom,
+                                                  unsigned* __restrict__ result) {
+  float best = 0.f;
+  for (int idx = blockIdx.x * BLOCK + threadIdx.x; idx < N; idx += gridDim.x * BLOCK) {
+    if (geom && !geom[idx]) continue;
+    best = fmaxf(best, fabsf(v[idx]));
+  }
+  best = block_max(best);
+  if (threadIdx.x == 0) atomicMax(result, __float_as_uint(best));
+}
+
 __global__ void __launch_bounds__(BLOCK) k_sums(const float* __restrict__ v, int ncomp, int N,
                                                 const uint8_t* __restrict__ geom, double* __restrict__ partials) {
   __shared__ double sh[BLOCK];
@@ -352,6 +363,18 @@ float Reducer::max_combo_norm(const Combo& co, int N, const uint8_t* geom, hipSt
   HIP_CHECK(hipStreamSynchronize(s));
   float r;
   static_assert(sizeof(r) == sizeof(bits), "float size");
+  __builtin_memcpy(&r, &bits, sizeof(r));
+  return r;
+}
+
+float Reducer::max_abs(const float* v, int N, const uint8_t* geom, hipStream_t s) {
+  k_zero_u32<<<1, 1, 0, s>>>(maxbits_.get());
+  k_maxabs<<<reduce_blocks(N), BLOCK, 0, s>>>(v, N, geom, maxbits_.get());
+  checkLaunch("k_maxabs");
+  unsigned bits = 0;
+  HIP_CHECK(hipMemcpyAsync(&bits, maxbits_.get(), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float r;
   __builtin_memcpy(&r, &bits, sizeof(r));
   return r;
 }

@@ -44,6 +44,8 @@ class Reducer {
   // max over geometry cells of |sum_j c[j] k[j]| (3 components), FP32 norms.
   float max_combo_norm(const Combo& co, int N, const uint8_t* geom, hipStream_t s);
   float max_vec_norm(const float* v, int N, const uint8_t* geom, hipStream_t s);
+  // max over geometry cells of |v| (one component; NaN cells are skipped)
+  float max_abs(const float* v, int N, const uint8_t* geom, hipStream_t s);
   // sum over geometry cells of each component, accumulated in FP64.
   void component_sums(const float* v, int ncomp, int N, const uint8_t* geom, double* out, hipStream_t s);
   // sum over geometry cells of |v|^2 (3 components), FP64 (dotSum, src/core/reduce.cu:153-195).

@@ -568,6 +568,21 @@ __global__ void k_afm_combine(const float* __restrict__ m1, PS ms1, const float*
         in ? sc * (a1 * s1 * m1[(size_t)c * N + i] + (m2 ? a2 * s2 * m2[(size_t)c * N + i] : 0.f)) : 0.f;
 }
 
+// k_angle (afmexchange.cu:260-281): acos(sign(afmex_cell) m1 . m2), 0 outside
+// the geometry and where msat1, msat2 or afmex_cell is 0.
+__global__ void k_afm_angle(const float* __restrict__ m1, const float* __restrict__ m2, PS afmex, PS ms1, PS ms2, int N,
+                            const uint8_t* __restrict__ geom, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float a = afmex.at(i);
+  if ((geom && !geom[i]) || ms1.at(i) == 0.f || ms2.at(i) == 0.f || a == 0.f) {
+    out[i] = 0.f;
+    return;
+  }
+  const float d = m1[i] * m2[i] + m1[N + i] * m2[N + i] + m1[2 * N + i] * m2[2 * N + i];
+  out[i] = (float)acos(copysign(1.0, (double)a) * (double)d);
+}
+
 __global__ void k_add2(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + b[i];
@@ -1021,6 +1036,11 @@ double Ferromagnet::scalar_quantity(const std::string& q, hipStream_t s) {
     eval_rhs(scratch.get(), s);
     return world->reducer.max_vec_norm(scratch.get(), N, geom.get(), s);
   }
+  if (q == "max_angle") {
+    // evalMaxAngle (exchange.cu:315-321)
+    launch_max_angle(desc(0), m.get(), scratch.get(), s);
+    return world->reducer.max_abs(scratch.get(), N, geom.get(), s);
+  }
   // energies: E = ncells * <e> * V (exchange.cu:247-256, energy.cu:78-92)
   static const std::map<std::string, std::string> dens = {
       {"exchange_energy", "exchange_energy_density"},
@@ -1121,7 +1141,7 @@ void Antiferromagnet::compute_demag(hipStream_t s, Profiler* prof) {
 
 int Antiferromagnet::quantity_ncomp(const std::string& q) const {
   if (q == "neel_vector" || q == "full_magnetization") return 3;
-  if (q == "total_energy_density") return 1;
+  if (q == "total_energy_density" || q == "angle_field") return 1;
   const int nc = magnet_quantity_ncomp(q);
   if (nc) return nc;
   throw Error(MPA_ERR_INVALID, "unknown quantity '" + q + "'");
@@ -1140,6 +1160,15 @@ const float* Antiferromagnet::eval_device(const std::string& q, hipStream_t s, i
     checkLaunch("k_afm_combine");
     return scratch.get();
   }
+  if (q == "angle_field") {
+    // evalAngleField (afmexchange.cu:283-294)
+    *ncomp = 1;
+    k_afm_angle<<<(N + 255) / 256, 256, 0, s>>>(sub[0]->m.get(), sub[1]->m.get(), param("afmex_cell").ps(),
+                                               sub[0]->param("msat").ps(), sub[1]->param("msat").ps(), N, geom.get(),
+                                               scratch.get());
+    checkLaunch("k_afm_angle");
+    return scratch.get();
+  }
   if (q == "total_energy_density") {
     // energy.cu:70-76: both sublattices, then the magnet's kinetic and elastic
     *ncomp = 1;
@@ -1156,6 +1185,12 @@ const float* Antiferromagnet::eval_device(const std::string& q, hipStream_t s, i
 
 double Antiferromagnet::scalar_quantity(const std::string& q, hipStream_t s) {
   if (q == "kinetic_energy" || q == "elastic_energy") return magnet_scalar(q, s);
+  if (q == "max_intracell_angle") {
+    // evalMaxAngle (afmexchange.cu:296-298)
+    int nc = 0;
+    const float* a = eval_device("angle_field", s, &nc);
+    return world->reducer.max_abs(a, N, geom.get(), s);
+  }
   if (q != "total_energy") throw Error(MPA_ERR_INVALID, "unknown scalar quantity '" + q + "'");
   double avg = 0.0;
   average_quantity("total_energy_density", &avg, s);

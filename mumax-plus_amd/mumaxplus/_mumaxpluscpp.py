@@ -314,7 +314,8 @@ class System:
 class FieldQuantity:
     """A field quantity of a magnet (wrap_fieldquantity.cpp:8-22)."""
 
-    _UNITS = {"magnetization": "", "full_magnetization": "A/m", "neel_vector": "", "exchange_field": "T",
+    _UNITS = {"magnetization": "", "full_magnetization": "A/m", "neel_vector": "", "angle_field": "rad",
+              "exchange_field": "T",
               "anisotropy_field": "T", "external_field": "T", "dmi_field": "T", "demag_field": "T",
               "magnetoelastic_field": "T", "homogeneous_exchange_field": "T",
               "inhomogeneous_exchange_field": "T", "effective_field": "T", "torque": "/s", "llg_torque": "/s",
@@ -364,7 +365,7 @@ class FieldQuantity:
 class ScalarQuantity:
     """Scalar quantity of a magnet (wrap_scalarquantity.cpp)."""
 
-    _UNITS = {"max_torque": "/s"}
+    _UNITS = {"max_torque": "/s", "max_angle": "rad", "max_intracell_angle": "rad"}
 
     def __init__(self, fm, qname):
         self._fm = fm

@@ -25,8 +25,8 @@ class Antiferromagnet(Magnet):
 
     _PARAMS = ()
     _MINIMIZE_SAMPLES = 20
-    _FIELD_QUANTITIES = ("neel_vector", "full_magnetization", "total_energy_density")
-    _SCALAR_QUANTITIES = ("total_energy",)
+    _FIELD_QUANTITIES = ("neel_vector", "full_magnetization", "total_energy_density", "angle_field")
+    _SCALAR_QUANTITIES = ("total_energy", "max_intracell_angle")
 
     def __init__(self, world, grid, name="", geometry=None, regions=None):
         self._create(world._impl.add_antiferromagnet, world, grid, name, geometry, regions)

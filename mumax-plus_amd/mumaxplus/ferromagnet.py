@@ -20,7 +20,7 @@ _FIELD_QUANTITIES = ("exchange_field", "anisotropy_field", "external_field", "dm
                      "zeeman_energy_density", "dmi_energy_density", "demag_energy_density",
                      "magnetoelastic_energy_density", "homogeneous_exchange_energy_density",
                      "inhomogeneous_exchange_energy_density", "total_energy_density")
-_SCALAR_QUANTITIES = ("max_torque", "exchange_energy", "anisotropy_energy", "zeeman_energy", "dmi_energy",
+_SCALAR_QUANTITIES = ("max_torque", "max_angle", "exchange_energy", "anisotropy_energy", "zeeman_energy", "dmi_energy",
                       "demag_energy", "magnetoelastic_energy", "homogeneous_exchange_energy",
                       "inhomogeneous_exchange_energy", "total_energy")
 
