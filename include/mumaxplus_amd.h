@@ -130,6 +130,25 @@ int mpa_fm_param_add_time_term(mpa_ferromagnet* m, const char* key, mpa_time_fn 
                                const float* mask /* ncomp*ncells or NULL */);
 int mpa_fm_param_remove_time_terms(mpa_ferromagnet* m, const char* key);
 int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out);
+/* Electrical potential (PoissonSystem, poissonsystem.cu:134-307; wrap_poissonsystem.cpp;
+ * LinSolver, linsolver.cpp and wrap_linsolver.cpp).  The parameters applied_potential (V, NaN
+ * where none), conductivity (S/m) and amr_ratio are mpa_fm_param_* keys; the quantities
+ * electrical_potential (1 component, solved on evaluation) and conductivity_tensor (6: xx yy zz xy
+ * xz yz) go through mpa_fm_eval.  init builds the rows (the solution and the stepper are reset only
+ * when the number of cells changes); solve = init + iterate, writing the potential (0 outside the
+ * geometry); step / restart / residual (max norm of b - A x) / solver_solve (iterate, then copy
+ * the FP64 solution when out is not NULL) act on the current system.  method: "jacobi"
+ * (default), "conjugategradient", "minimalresidual", "steepestdescent".  Options: "max_iter"
+ * (-1: no limit) and "tol" (default 1e-5). */
+int mpa_fm_poisson_init(mpa_ferromagnet* m);
+int mpa_fm_poisson_solve(mpa_ferromagnet* m, float* out /* ncells */);
+int mpa_fm_poisson_step(mpa_ferromagnet* m);
+int mpa_fm_poisson_restart(mpa_ferromagnet* m);
+int mpa_fm_poisson_solver_solve(mpa_ferromagnet* m, double* out /* ncells or NULL */);
+int mpa_fm_poisson_residual(mpa_ferromagnet* m, double* out);
+int mpa_fm_poisson_set_method(mpa_ferromagnet* m, const char* name);
+int mpa_fm_poisson_set_option(mpa_ferromagnet* m, const char* key, double value);
+int mpa_fm_poisson_get_option(const mpa_ferromagnet* m, const char* key, double* out);
 /* Parameter.set_in_region(idx, value) (wrap_parameter.cpp:37-38, 105-106; parameter.cu:41-47,
  * 144-150): the static value (ncomp doubles) in every cell of region idx; MPA_ERR_INVALID when idx
  * is not one of the magnet's regions (system.cpp:73-77). */

@@ -503,6 +503,88 @@ int mpa_fm_param_is_dynamic(const mpa_ferromagnet* m, const char* key, int* out)
   return guard([&] { param_is_dynamic(FM(m), key, out); });
 }
 
+int mpa_fm_poisson_init(mpa_ferromagnet* m) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->poisson.init(f->poisson_desc(), f->world->stream);
+  });
+}
+
+int mpa_fm_poisson_solve(mpa_ferromagnet* m, float* out) {
+  return guard([&] {
+    need(out, "out");
+    FM(m)->eval_quantity("electrical_potential", out, FM(m)->world->stream);
+  });
+}
+
+int mpa_fm_poisson_step(mpa_ferromagnet* m) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->poisson.step(f->world->stream);
+    HIP_CHECK(hipStreamSynchronize(f->world->stream));
+  });
+}
+
+int mpa_fm_poisson_restart(mpa_ferromagnet* m) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->poisson.restart(f->world->stream);
+    HIP_CHECK(hipStreamSynchronize(f->world->stream));
+  });
+}
+
+int mpa_fm_poisson_solver_solve(mpa_ferromagnet* m, double* out) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    hipStream_t s = f->world->stream;
+    f->poisson.solve(s);
+    if (out && f->poisson.ready())
+      HIP_CHECK(hipMemcpyAsync(out, f->poisson.x(), (size_t)f->poisson.rows() * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mpa_fm_poisson_residual(mpa_ferromagnet* m, double* out) {
+  return guard([&] {
+    need(out, "out");
+    Ferromagnet* f = FM(m);
+    *out = f->poisson.residual_max_norm(f->world->stream);
+  });
+}
+
+int mpa_fm_poisson_set_method(mpa_ferromagnet* m, const char* name) {
+  return guard([&] {
+    need(name, "name");
+    Ferromagnet* f = FM(m);
+    f->poisson.set_method(name);
+    // LinSolver::setMethod: a new stepper, restarted (linsolver.cpp)
+    f->poisson.restart(f->world->stream);
+    HIP_CHECK(hipStreamSynchronize(f->world->stream));
+  });
+}
+
+int mpa_fm_poisson_set_option(mpa_ferromagnet* m, const char* key, double value) {
+  return guard([&] {
+    need(key, "key");
+    const std::string k = key;
+    if (k == "max_iter") FM(m)->poisson.max_iter = (int)value;
+    else if (k == "tol") FM(m)->poisson.tol = value;
+    else throw Error(MPA_ERR_INVALID, "unknown poisson solver option '" + k + "'");
+  });
+}
+
+int mpa_fm_poisson_get_option(const mpa_ferromagnet* m, const char* key, double* out) {
+  return guard([&] {
+    need(key, "key");
+    need(out, "out");
+    const std::string k = key;
+    if (k == "max_iter") *out = FM(m)->poisson.max_iter;
+    else if (k == "tol") *out = FM(m)->poisson.tol;
+    else throw Error(MPA_ERR_INVALID, "unknown poisson solver option '" + k + "'");
+  });
+}
+
 int mpa_fm_param_set_in_region(mpa_ferromagnet* m, const char* key, unsigned int region, const double* value) {
   return guard([&] {
     need(key, "key");

@@ -101,6 +101,9 @@ const ParamSpec kFmParams[] = {
     {"Lambda", 1, {0, 0, 0}, "", ":lambda"},  {"free_layer_thickness", 1, {0, 0, 0}, "m"},
     {"epsilon_prime", 1, {0, 0, 0}, ""},      {"fixed_layer", 3, {0, 0, 0}, ""},
     {"pol", 1, {0, 0, 0}, ""},                {"jcur", 3, {0, 0, 0}, "A/m2"},
+    // electrical potential (ferromagnet.cpp:53-55): no applied potential is NaN
+    {"applied_potential", 1, {__builtin_nanf(""), 0, 0}, "V"},
+    {"conductivity", 1, {0, 0, 0}, "S/m"},    {"amr_ratio", 1, {0, 0, 0}, ""},
 };
 // The labels of ku2, kc3 (no colon) and B2 (":B1") are the reference's own
 // parameter names (ferromagnet.cpp:29,32,61), exposed as Parameter.name.
@@ -583,6 +586,13 @@ __global__ void k_afm_angle(const float* __restrict__ m1, const float* __restric
   out[i] = (float)acos(copysign(1.0, (double)a) * (double)d);
 }
 
+// the potential in the geometry (poissonsystem.cu:292-297), 0 outside
+__global__ void k_double_to_float(const double* __restrict__ x, const uint8_t* __restrict__ geom, int N,
+                                  float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) out[i] = (geom && !geom[i]) ? 0.f : (float)x[i];
+}
+
 __global__ void k_add2(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + b[i];
@@ -966,9 +976,24 @@ const QInfo* find_qinfo(const std::string& q) {
 }
 }  // namespace
 
+PoissonDesc Ferromagnet::poisson_desc() {
+  PoissonDesc d{};
+  d.nx = n[0]; d.ny = n[1]; d.nz = n[2];
+  d.cx = world->cellsize[0]; d.cy = world->cellsize[1]; d.cz = world->cellsize[2];
+  d.geom = geom.get();
+  d.sigma0 = param("conductivity").ps();
+  d.amr = param("amr_ratio").ps();
+  d.m = m.get();
+  d.aniso = param("amr_ratio").assured_zero() ? 0 : 1;  // poissonsystem.cu:262-263
+  d.pot = param("applied_potential").ps();
+  return d;
+}
+
 int Ferromagnet::quantity_ncomp(const std::string& q) const {
   if (const QInfo* qi = find_qinfo(q)) return qi->ncomp;
   if (q == "magnetoelastic_force" || q == "full_magnetization") return 3;
+  if (q == "conductivity_tensor") return 6;
+  if (q == "electrical_potential") return 1;
   if (!host) {
     const int nc = magnet_quantity_ncomp(q);
     if (nc) return nc;
@@ -993,6 +1018,24 @@ const float* Ferromagnet::eval_device(const std::string& q, hipStream_t s, int* 
       k_afm_combine<<<(N + 255) / 256, 256, 0, s>>>(m.get(), param("msat").ps(), nullptr, PS{nullptr, 0.f}, 1.f, 0.f,
                                                     0, N, geom.get(), scratch.get());
       checkLaunch("k_afm_combine");
+      return scratch.get();
+    }
+    if (q == "conductivity_tensor") {
+      // evalConductivityTensor (conductivitytensor.cu:34-41)
+      *ncomp = 6;
+      if (qbuf.n < (size_t)6 * N) qbuf.allocate((size_t)6 * N);
+      PoissonDesc pd = poisson_desc();
+      pd.aniso = 1;
+      launch_conductivity_tensor(pd, qbuf.get(), s);
+      return qbuf.get();
+    }
+    if (q == "electrical_potential") {
+      // evalElectricalPotential (electricalpotential.cu:9-19): PoissonSystem::solve
+      *ncomp = 1;
+      poisson.init(poisson_desc(), s);
+      poisson.solve(s);
+      k_double_to_float<<<(N + 255) / 256, 256, 0, s>>>(poisson.x(), geom.get(), N, scratch.get());
+      checkLaunch("k_double_to_float");
       return scratch.get();
     }
     if (!host && magnet_quantity_ncomp(q)) return magnet_eval_device(q, s, ncomp);

@@ -17,6 +17,7 @@
 #include "demag.hpp"
 #include "elastic.hpp"
 #include "fields.hpp"
+#include "poisson.hpp"
 #include "profiler.hpp"
 #include "rk.hpp"
 #include "stray.hpp"
@@ -228,6 +229,9 @@ class Ferromagnet : public MagnetCore {
   DeviceArray<float> x0;      // RK start state
   DeviceArray<float> hdemag;  // demag field scratch (independent magnets)
   DeviceArray<float> scratch; // quantity evaluation
+  // electrical potential (Ferromagnet::poissonSystem, ferromagnet.hpp:55)
+  PoissonSolver poisson;
+  PoissonDesc poisson_desc();
   std::vector<DeviceArray<float>> k;
 
   Ferromagnet* sister() const;

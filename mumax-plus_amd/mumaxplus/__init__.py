@@ -17,6 +17,7 @@ from .fieldquantity import FieldQuantity
 from .scalarquantity import ScalarQuantity
 from .parameter import Parameter
 from .interparameter import InterParameter
+from .poissonsystem import PoissonSystem
 from .variable import Variable
 from .dmitensor import DmiTensor, DmiTensorGroup
 from .strayfield import StrayField
@@ -26,4 +27,4 @@ from . import util
 
 __all__ = ["_cpp", "Grid", "World", "Magnet", "Ferromagnet", "Antiferromagnet", "TimeSolver",
            "TimeSolverOutput", "FieldQuantity", "ScalarQuantity", "Parameter", "Variable", "DmiTensor",
-           "DmiTensorGroup", "InterParameter", "StrayField", "Batch", "serving", "util"]
+           "DmiTensorGroup", "InterParameter", "PoissonSystem", "StrayField", "Batch", "serving", "util"]

@@ -69,6 +69,15 @@ _SIGS = {
     "mpa_fm_param_add_time_term": ([_P, _c_char_p, _TIME_FN, _P, _P], _cint),
     "mpa_fm_param_remove_time_terms": ([_P, _c_char_p], _cint),
     "mpa_fm_param_is_dynamic": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_poisson_init": ([_P], _cint),
+    "mpa_fm_poisson_solve": ([_P, _P], _cint),
+    "mpa_fm_poisson_step": ([_P], _cint),
+    "mpa_fm_poisson_restart": ([_P], _cint),
+    "mpa_fm_poisson_solver_solve": ([_P, _P], _cint),
+    "mpa_fm_poisson_residual": ([_P, _ct.POINTER(_cdouble)], _cint),
+    "mpa_fm_poisson_set_method": ([_P, _c_char_p], _cint),
+    "mpa_fm_poisson_set_option": ([_P, _c_char_p, _cdouble], _cint),
+    "mpa_fm_poisson_get_option": ([_P, _c_char_p, _ct.POINTER(_cdouble)], _cint),
     "mpa_fm_param_set_in_region": ([_P, _c_char_p, _ct.c_uint, _P], _cint),
     "mpa_fm_regions": ([_P, _P], _cint),
     "mpa_fm_unique_regions": ([_P, _P, _cint, _ct.POINTER(_cint)], _cint),
@@ -315,6 +324,7 @@ class FieldQuantity:
     """A field quantity of a magnet (wrap_fieldquantity.cpp:8-22)."""
 
     _UNITS = {"magnetization": "", "full_magnetization": "A/m", "neel_vector": "", "angle_field": "rad",
+              "electrical_potential": "V", "conductivity_tensor": "S/m",
               "exchange_field": "T",
               "anisotropy_field": "T", "external_field": "T", "dmi_field": "T", "demag_field": "T",
               "magnetoelastic_field": "T", "homogeneous_exchange_field": "T",
@@ -549,6 +559,75 @@ class VectorParameter(Parameter):
     pass
 
 
+# ------------------------------------------------------ PoissonSystem
+class LinSolver:
+    """Iterative solver of a ferromagnet's potential system (wrap_linsolver.cpp:4-15)."""
+
+    def __init__(self, fm):
+        self._fm = fm
+
+    def step(self):
+        _call(self._fm, "poisson_step")
+
+    def restart(self):
+        _call(self._fm, "poisson_restart")
+
+    def max_norm_residual(self):
+        out = _cdouble()
+        _call(self._fm, "poisson_residual", _ct.byref(out))
+        return out.value
+
+    def set_method(self, name):
+        _call(self._fm, "poisson_set_method", _enc(name))
+
+    def solve(self):
+        nx, ny, nz = self._fm.grid.size
+        out = _np.empty(nx * ny * nz, dtype=_np.float64)
+        _call(self._fm, "poisson_solver_solve", _ptr(out))
+        return out
+
+    def _opt(self, key):
+        out = _cdouble()
+        _call(self._fm, "poisson_get_option", _enc(key), _ct.byref(out))
+        return out.value
+
+    @property
+    def max_iter(self):
+        return int(self._opt("max_iter"))
+
+    @max_iter.setter
+    def max_iter(self, value):
+        _call(self._fm, "poisson_set_option", _enc("max_iter"), float(int(value)))
+
+    @property
+    def tol(self):
+        return self._opt("tol")
+
+    @tol.setter
+    def tol(self, value):
+        _call(self._fm, "poisson_set_option", _enc("tol"), float(value))
+
+
+class PoissonSystem:
+    """Potential system of a ferromagnet (wrap_poissonsystem.cpp:4-10)."""
+
+    def __init__(self, fm):
+        self._fm = fm
+
+    def init(self):
+        _call(self._fm, "poisson_init")
+
+    def solve(self):
+        nx, ny, nz = self._fm.grid.size
+        out = _np.empty((1, nz, ny, nx), dtype=_np.float32)
+        _call(self._fm, "poisson_solve", _ptr(out))
+        return out
+
+    @property
+    def solver(self):
+        return LinSolver(self._fm)
+
+
 # ------------------------------------------------------ InterParameter
 class InterParameter:
     """Value between pairs of regions (wrap_parameter.cpp:109-125)."""
@@ -752,9 +831,14 @@ class Ferromagnet(_Magnet):
 
     _pfx = "fm"
     _PARAMS = ("msat", "aex", "alpha", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "B1", "B2",
-               "temperature", "xi", "Lambda", "free_layer_thickness", "epsilon_prime", "pol")
+               "temperature", "xi", "Lambda", "free_layer_thickness", "epsilon_prime", "pol",
+               "applied_potential", "conductivity", "amr_ratio")
     _VPARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field", "fixed_layer", "jcur")
     _INTER = ("inter_exchange", "scale_exchange")
+
+    @property
+    def poisson_system(self):
+        return PoissonSystem(self)
 
     def __init__(self, handle, world):
         super().__init__(handle, world)

@@ -6,11 +6,12 @@ import numpy as _np
 
 from .dmitensor import DmiTensor
 from .interparameter import InterParameter
+from .poissonsystem import PoissonSystem
 from .magnet import Magnet
 from .variable import Variable
 
 _SCALAR_PARAMS = ("msat", "alpha", "aex", "ku1", "ku2", "kc1", "kc2", "kc3", "frozen_spins", "xi", "Lambda",
-                  "free_layer_thickness", "epsilon_prime", "pol")
+                  "free_layer_thickness", "epsilon_prime", "pol", "applied_potential", "conductivity", "amr_ratio")
 _VECTOR_PARAMS = ("anisU", "anisC1", "anisC2", "bias_magnetic_field", "fixed_layer", "jcur")
 _FIELD_QUANTITIES = ("exchange_field", "anisotropy_field", "external_field", "dmi_field", "demag_field",
                      "magnetoelastic_field", "homogeneous_exchange_field", "inhomogeneous_exchange_field",
@@ -19,7 +20,8 @@ _FIELD_QUANTITIES = ("exchange_field", "anisotropy_field", "external_field", "dm
                      "full_magnetization", "exchange_energy_density", "anisotropy_energy_density",
                      "zeeman_energy_density", "dmi_energy_density", "demag_energy_density",
                      "magnetoelastic_energy_density", "homogeneous_exchange_energy_density",
-                     "inhomogeneous_exchange_energy_density", "total_energy_density")
+                     "inhomogeneous_exchange_energy_density", "total_energy_density", "electrical_potential",
+                     "conductivity_tensor")
 _SCALAR_QUANTITIES = ("max_torque", "max_angle", "exchange_energy", "anisotropy_energy", "zeeman_energy", "dmi_energy",
                       "demag_energy", "magnetoelastic_energy", "homogeneous_exchange_energy",
                       "inhomogeneous_exchange_energy", "total_energy")
@@ -173,6 +175,11 @@ class Ferromagnet(Magnet):
         self.B2.set(value)
         _warn_positive(self.B2, "The second magnetoelastic coupling constant B2 is set to a positive value, "
                                 "instead of negative (or zero). Make sure this is intentional!")
+
+    @property
+    def poisson_system(self):
+        """The solver of the electrical potential (ferromagnet.py:664-673 of the reference)."""
+        return PoissonSystem(self._impl.poisson_system)
 
     @property
     def inter_exchange(self):

@@ -31,7 +31,9 @@ SCALAR_DEFAULTS = {"msat": 1.0, "aex": 0.0, "alpha": 0.0, "ku1": 0.0, "ku2": 0.0
                    # elastic parameters of an independent magnet (magnet.cpp:26-34)
                    "C11": 0.0, "C12": 0.0, "C44": 0.0, "eta": 0.0, "rho": 1.0,
                    # temperature and spin-transfer torque (ferromagnet.cpp:34-56)
-                   "temperature": 0.0, "xi": 0.0, "Lambda": 0.0, "epsilon_prime": 0.0, "pol": 0.0}
+                   "temperature": 0.0, "xi": 0.0, "Lambda": 0.0, "epsilon_prime": 0.0, "pol": 0.0,
+                   # electrical potential (ferromagnet.cpp:53-55)
+                   "applied_potential": float("nan"), "conductivity": 0.0, "amr_ratio": 0.0}
 VECTOR_DEFAULTS = {"anisU": (0.0, 0.0, 0.0), "anisC1": (0.0, 0.0, 0.0),
                    "anisC2": (0.0, 0.0, 0.0), "bias": (0.0, 0.0, 0.0),
                    "external_body_force": (0.0, 0.0, 0.0), "rigid_norm_strain": (0.0, 0.0, 0.0),
