@@ -50,11 +50,9 @@ __device__ __forceinline__ float3 normalized3(float3 v) {
 // UNI: every parameter is uniform, so each lookup is the scalar and the
 // harmonic mean of a cell and its neighbour is the value itself.
 template <bool UNI>
-__global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restrict__ m,
-                                                 const float* __restrict__ hdemag, float* __restrict__ out,
-                                                 int mode) {
-  const int nblk = gridDim.x;
-  const int bid = xcd_remap(blockIdx.x, nblk);
+__device__ __forceinline__ void field_body(const FMDesc& d, const float* __restrict__ m,
+                                           const float* __restrict__ hdemag, float* __restrict__ out, int mode,
+                                           int bid) {
   const int idx = bid * BLOCK + threadIdx.x;
   const int N = d.ncell;
   if (idx >= N) return;
@@ -415,6 +413,26 @@ __global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restri
   out[2 * N + idx] = r.z;
 }
 
+template <bool UNI>
+__global__ void __launch_bounds__(BLOCK) k_field(FMDesc d, const float* __restrict__ m,
+                                                 const float* __restrict__ hdemag, float* __restrict__ out,
+                                                 int mode) {
+  field_body<UNI>(d, m, hdemag, out, mode, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Both sublattices of an antiferromagnet in one launch: blockIdx.y selects
+// one.  Each branch reads its own kernel argument at fixed offsets (indexing
+// a descriptor array by blockIdx.y would copy it to scratch).
+template <bool UNI>
+__global__ void __launch_bounds__(BLOCK) k_field_pair(FMDesc d0, const float* __restrict__ m0,
+                                                      const float* __restrict__ h0, float* __restrict__ o0, FMDesc d1,
+                                                      const float* __restrict__ m1, const float* __restrict__ h1,
+                                                      float* __restrict__ o1, int mode) {
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (blockIdx.y == 0) field_body<UNI>(d0, m0, h0, o0, mode, bid);
+  else field_body<UNI>(d1, m1, h1, o1, mode, bid);
+}
+
 __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                          const float* __restrict__ hdemag, float* __restrict__ out,
                                                          int mode) {
@@ -623,6 +641,20 @@ void launch_param_fill(float* out, const float* base, const ParamFill& f, int nc
     throw Error(MPA_ERR_RUNTIME, "launch_param_fill: bad arguments");
   k_param_fill<<<(N + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(out, base, f, ncomp, N);
   checkLaunch("k_param_fill");
+}
+
+void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float* o0, const FMDesc& d1,
+                       const float* m1, const float* h1, float* o1, FieldMode mode, hipStream_t s) {
+  const bool u0 = params_uniform(d0), u1 = params_uniform(d1);
+  if (d0.ncell != d1.ncell || u0 != u1 || (mode != MODE_EDENS && (all_uniform(d0) || all_uniform(d1)))) {
+    launch_field(d0, m0, h0, o0, mode, s);
+    launch_field(d1, m1, h1, o1, mode, s);
+    return;
+  }
+  const dim3 grid((d0.ncell + BLOCK - 1) / BLOCK, 2);
+  if (u0) k_field_pair<true><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
+  else k_field_pair<false><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
+  checkLaunch("k_field_pair");
 }
 
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {

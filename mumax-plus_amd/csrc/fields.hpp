@@ -91,6 +91,11 @@ enum FieldMode {
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
 
+// launch_field for the two sublattices of an antiferromagnet in one launch
+// (two separate launches when their parameter layouts differ).
+void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float* o0, const FMDesc& d1,
+                       const float* m1, const float* h1, float* o1, FieldMode mode, hipStream_t s);
+
 // Per cell the largest angle (rad) between m and an in-geometry neighbour with
 // msat != 0 and a non-zero harmonic-mean exchange constant; 0 outside the
 // geometry and where msat = 0 (k_maxangle, exchange.cu:271-313).

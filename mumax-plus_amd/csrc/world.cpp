@@ -914,16 +914,17 @@ void MagnetCore::compute_stray(hipStream_t s, Profiler* prof) {
   }
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode, double noise_dt) {
-  const unsigned terms = active_terms();
-  Profiler* prof = &world->prof;
-  if (!prepared) prepare(terms, s, prof);
-  // algorithmic bytes: read m, h_demag (when present), the sister's m (for
-  // a sublattice) and the strain (dynamic magnetoelastic term); write the torque
+// algorithmic bytes of one right-hand side: read m, h_demag (when present),
+// the sister's m (for a sublattice) and the strain (dynamic magnetoelastic
+// term); write the torque
+double Ferromagnet::rhs_bytes(unsigned terms) {
   double bytes = 12.0 * N * ((terms & TERM_DEMAG) ? 3 : 2);
   if (host) bytes += 12.0 * N;
   if ((terms & TERM_ME) && !owner()->rigid_strain_applied()) bytes += 24.0 * N;
-  Timed t(prof, "k_field", bytes, s);
+  return bytes;
+}
+
+FMDesc Ferromagnet::rhs_desc(unsigned terms, double noise_dt) {
   FMDesc d = desc(terms);
   if (noise_dt > 0 && (terms & TERM_THERMAL)) {
     // k += noise / sqrt(dt) with the step's draw (rungekutta.cpp:95-106, in FP32)
@@ -932,7 +933,26 @@ void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode m
     d.noise_purpose = 0;
     d.noise_draw = step_draw;
   }
-  launch_field(d, m.get(), demag_field(), out, mode, s);
+  return d;
+}
+
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode, double noise_dt) {
+  const unsigned terms = active_terms();
+  Profiler* prof = &world->prof;
+  if (!prepared) prepare(terms, s, prof);
+  Timed t(prof, "k_field", rhs_bytes(terms), s);
+  launch_field(rhs_desc(terms, noise_dt), m.get(), demag_field(), out, mode, s);
+}
+
+// Both sublattices' right-hand sides in one launch (everything they read is
+// prepared).
+void Antiferromagnet::eval_rhs_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt) {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  const unsigned ta = a->active_terms(), tb = b->active_terms();
+  Timed t(&world->prof, "k_field_pair", a->rhs_bytes(ta) + b->rhs_bytes(tb), s);
+  launch_field_pair(a->rhs_desc(ta, noise_dt), a->m.get(), a->demag_field(), out0, b->rhs_desc(tb, noise_dt),
+                    b->m.get(), b->demag_field(), out1, mode, s);
 }
 
 namespace {
@@ -1303,12 +1323,20 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
     if ((a->sub[0]->active_terms() | a->sub[1]->active_terms()) & TERM_DEMAG) a->compute_demag(s, &prof);
     a->compute_stray(s, &prof);
   }
-  for (const auto& sv : vars) {
+  for (size_t vi = 0; vi < vars.size(); ++vi) {
+    const StateVar& sv = vars[vi];
     float* out = (*sv.k)[stage].get();
     if (sv.kind == StateVar::MAG) {
       Ferromagnet* fm = sv.fm;
       const FieldMode mode = relaxing_ ? MODE_DAMPING : MODE_LLG;
       if (fm->host) {
+        // the two sublattices follow each other: one launch for both
+        if (fm == fm->host->sub[0].get() && vi + 1 < vars.size() && vars[vi + 1].kind == StateVar::MAG &&
+            vars[vi + 1].fm == fm->host->sub[1].get()) {
+          fm->host->eval_rhs_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
+          ++vi;
+          continue;
+        }
         fm->eval_rhs(out, s, true, mode, noise_dt);
       } else {
         const unsigned t = fm->active_terms();

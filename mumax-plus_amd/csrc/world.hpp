@@ -251,6 +251,8 @@ class Ferromagnet : public MagnetCore {
   // noise_dt > 0: add the thermal field of the step in progress divided by
   // sqrt(noise_dt) (the stepper's noise term).
   void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG, double noise_dt = 0.0);
+  double rhs_bytes(unsigned terms);
+  FMDesc rhs_desc(unsigned terms, double noise_dt);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
   int quantity_ncomp(const std::string& q) const;
   void average_quantity(const std::string& q, double* out, hipStream_t s);
@@ -264,9 +266,10 @@ class Ferromagnet : public MagnetCore {
   bool stray_source_of(StraySource& out) override;
 
   const float* eval_device(const std::string& q, hipStream_t s, int* ncomp);
+  // the demag field this magnet reads (its own, or its antiferromagnet's)
+  const float* demag_field() const;
 
  private:
-  const float* demag_field() const;
   std::unique_ptr<DemagPlan> demag_;
 };
 
@@ -286,6 +289,7 @@ class Antiferromagnet : public MagnetCore {
   int num_fm() const override { return 2; }
   Ferromagnet* fm_at(int i) override { return sub[i].get(); }
   void set_regions(const uint32_t* regions) override;  // the sublattices share the host's regions
+  void eval_rhs_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt);
   bool stray_source_of(StraySource& out) override;
 
   // Quantities of the whole antiferromagnet (neel_vector, full_magnetization
