@@ -35,6 +35,9 @@ constexpr int BLOCK = 256;
 #ifndef MPA_FIELD_TILE_X
 #define MPA_FIELD_TILE_X 256
 #endif
+#ifndef MPA_FIELD_ZPER
+#define MPA_FIELD_ZPER 1
+#endif
 
 __device__ __forceinline__ float harmonic_mean(float a, float b) {
   if (a + b == 0.0f) return 0.0f;
@@ -446,10 +449,14 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   const int bxi = tile % nbx, rest = tile / nbx;
   const int x = bxi * blockDim.x + threadIdx.x;
   const int y = (rest % nby) * blockDim.y + threadIdx.y;
-  const int z = rest / nby;
   if (x >= d.nx || y >= d.ny) return;
   const int N = d.ncell;
-  field_uniform_cell(d, u, m, [&](int i) { return load3(hdemag, i, N); }, out, mode, x, y, z, N);
+  // MPA_FIELD_ZPER consecutive z planes per thread
+#pragma unroll
+  for (int k = 0; k < MPA_FIELD_ZPER; ++k) {
+    const int z = (rest / nby) * MPA_FIELD_ZPER + k;
+    if (z < d.nz) field_uniform_cell(d, u, m, [&](int i) { return load3(hdemag, i, N); }, out, mode, x, y, z, N);
+  }
 }
 
 // A member of a batched launch: its description, coefficients, first cell
@@ -663,7 +670,7 @@ void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* o
     int bx = 1;
     while (bx < d.nx && bx < MPA_FIELD_TILE_X) bx <<= 1;
     const int by = BLOCK / bx;
-    dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, d.nz);
+    dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, (d.nz + MPA_FIELD_ZPER - 1) / MPA_FIELD_ZPER);
     k_field_uniform<<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode);
     checkLaunch("k_field_uniform");
     return;
