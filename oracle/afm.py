@@ -26,11 +26,22 @@ Pinned by tests/test_oracle_afm.py: the numpy formulations of
 test/test_exchange_afm.py (sublattice, homogeneous and inhomogeneous
 exchange), the reduction to the ferromagnet fields without AFM coupling
 (the premise of test/test_DMI_*_afm.py), linearity in the inter-sublattice
-tensor, and the AFM stray field of the total magnetisation.  Parity
-unpinned: the exact form of the inter-sublattice DMI term (the signs of
-dmi.cu:231-233 are restated, no closed form or golden value fixes them) and
-the sister-derivative terms of the Neumann boundary when afmex_nn != 0
-(exchange.cu:181-186, dmi.cu:205-207, afmexchange.cu:122-132).
+tensor, and the AFM stray field of the total magnetisation.
+
+Pinned by tests/test_oracle_afm_pins.py, with planted mistakes that fail:
+* the sister-derivative terms of the Neumann ghost cells (exchange.cu:181-186,
+  dmi.cu:205-207, afmexchange.cu:122-132): locked antiparallel sublattices
+  equal a ferromagnet with A_eff = A - A_nn/2, edges included, for a state
+  whose edge difference obeys the boundary condition;
+* the yz elements of the inter-sublattice term (dmi.cu:229-231): locked
+  parallel sublattices see them as the same elements of an own tensor;
+* its structure: reciprocal for the xy and yz elements, anti-reciprocal for
+  the xz elements.
+Parity unpinned: the overall sign of the xy and xz elements of the
+inter-sublattice term.  They are restated from dmi.cu:229-231; the xy
+elements act with the sign opposite to the energy-derived one and the xz
+elements follow no energy, so no closed form fixes them, and no reference
+test or example has an observable that depends on them.
 """
 
 import numpy as np

@@ -120,7 +120,10 @@ def test_neel_canting_relax_chain():
     n = afm.sub1.val("msat") * afm.sub1.m - afm.sub2.val("msat") * afm.sub2.m
     cant = n[2] / np.linalg.norm(n, axis=0)
     k = math.sqrt(1 - (2 * (D / 2) / (math.pi * Dc)) ** 2)
-    want = math.tanh(math.atanh(k) + N // 2)
+    # analytical(x, D) of test_neel_canting.py:19-21 at the surface cell x = 0
+    x = 0
+    want = math.tanh(math.atanh(k) + N // 2 - abs(x - N // 2))
+    assert abs(want - k) < 1e-12
     assert abs(cant[0, 0, 0] - want) / want < 2e-2
     # the canting decays into the bulk
     assert cant[0, 0, 0] < cant[0, 0, 4] < 1.0
