@@ -105,6 +105,13 @@ int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsi
  * has no such setting) with the read-only "noise_draws" (thermal draws taken by time steps). */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
+/* The thermal-noise stream state of a ferromagnet or sublattice: the Philox key and the 64-bit
+ * draw counters (draws taken by time steps, draws taken by thermal_noise evaluations).  No
+ * counterpart in the reference (cuRAND seeded from the clock, ferromagnet.cpp:79-81, and no
+ * checkpoints); a snapshot saves it so that a restored run continues the uninterrupted one. */
+int mpa_fm_get_noise_state(const mpa_ferromagnet* m, uint32_t* seed, uint64_t* step_draws,
+                           uint64_t* quantity_draws);
+int mpa_fm_set_noise_state(mpa_ferromagnet* m, uint32_t seed, uint64_t step_draws, uint64_t quantity_draws);
 /* Parameter / VectorParameter (wrap_parameter.cpp:13-107, parameter.cu:19-39).  key: msat, aex,
  * alpha, ku1, ku2, kc1, kc2, kc3, frozen_spins, anisU, anisC1, anisC2, bias_magnetic_field,
  * dmi_xxy, dmi_xyz, dmi_xxz, dmi_yxy, dmi_yyz, dmi_yxz, dmi_zxy, dmi_zyz, dmi_zxz, B1, B2,

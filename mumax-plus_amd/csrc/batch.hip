@@ -368,6 +368,7 @@ int Batch::add(Ferromagnet* fm, int mode_flags, double arg) {
   sl.span = span;
   sl.mode = mode;
   sl.demag = (terms & TERM_DEMAG) != 0;
+  sl.desc = d;
   sl.used = true;
   HIP_CHECK(hipStreamSynchronize(w->stream));
   // state and mask into the packed arrays (the span's tail stays masked out)
@@ -438,7 +439,9 @@ void Batch::remove(int id) {
   HIP_CHECK(hipStreamSynchronize(stream));
   TimeSolver& ts = fm->world->solver;
   ts.time = S.time;
-  ts.dt = S.dt;
+  // a member taken off while the shortened final step of a run is pending
+  // gets the step it had before that piece, not the piece
+  ts.dt = (S.last && !S.done) ? S.dt_saved : S.dt;
   ts.nsteps += S.steps;
   ts.nrejected += S.rejected;
   fm->world->touch();
@@ -454,10 +457,11 @@ void Batch::rebuild() {
   std::vector<FieldBatchMember> fb;
   for (const Slot& sl : slots_) {
     if (!sl.used) continue;
+    // the parameters read on admission: a member's result never depends on
+    // when other members join or leave
     Ferromagnet* fm = sl.fm;
-    const unsigned terms = fm->active_terms();
-    FMDesc d = fm->desc(terms);
-    if (terms & TERM_DEMAG) dm.push_back({&fm->demag(), sl.off, PS{nullptr, d.msat.v}, fm->geom_ptr()});
+    const FMDesc& d = sl.desc;
+    if (sl.demag) dm.push_back({&fm->demag(), sl.off, PS{nullptr, d.msat.v}, fm->geom_ptr()});
     fb.push_back({d, (int)sl.off, sl.mode == BATCH_RELAX ? MODE_DAMPING : MODE_LLG});
   }
   demag_ = dm.empty() ? nullptr : std::make_unique<DemagBatch>(dm, cs_);

@@ -94,6 +94,7 @@ class Batch {
     int mode = 0;
     bool demag = false;
     bool used = false;
+    FMDesc desc{};  // the member's parameters as read on admission
   };
   void rebuild();
   void prime(const std::vector<int>& fresh);

@@ -470,6 +470,28 @@ int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
   });
 }
 
+int mpa_fm_get_noise_state(const mpa_ferromagnet* m, uint32_t* seed, uint64_t* step_draws,
+                           uint64_t* quantity_draws) {
+  return guard([&] {
+    const Ferromagnet* f = FM(m);
+    need(seed, "seed");
+    need(step_draws, "step_draws");
+    need(quantity_draws, "quantity_draws");
+    *seed = f->noise_seed;
+    *step_draws = f->noise_draws[0];
+    *quantity_draws = f->noise_draws[1];
+  });
+}
+
+int mpa_fm_set_noise_state(mpa_ferromagnet* m, uint32_t seed, uint64_t step_draws, uint64_t quantity_draws) {
+  return guard([&] {
+    Ferromagnet* f = FM(m);
+    f->noise_seed = seed;
+    f->noise_draws[0] = step_draws;
+    f->noise_draws[1] = quantity_draws;
+  });
+}
+
 int mpa_fm_param_info(const mpa_ferromagnet* m, const char* key, int* ncomp, int* is_uniform, char* name,
                       int namelen, char* unit, int unitlen) {
   return guard([&] { param_info(FM(m), key, ncomp, is_uniform, name, namelen, unit, unitlen); });

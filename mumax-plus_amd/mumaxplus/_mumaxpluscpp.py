@@ -60,6 +60,9 @@ _SIGS = {
     "mpa_fm_cellsingeo": ([_P, _ct.POINTER(_cint)], _cint),
     "mpa_fm_set_flag": ([_P, _c_char_p, _cint], _cint),
     "mpa_fm_get_flag": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
+    "mpa_fm_get_noise_state": ([_P, _ct.POINTER(_ct.c_uint32), _ct.POINTER(_ct.c_uint64),
+                                _ct.POINTER(_ct.c_uint64)], _cint),
+    "mpa_fm_set_noise_state": ([_P, _ct.c_uint32, _ct.c_uint64, _ct.c_uint64], _cint),
     "mpa_fm_param_info": ([_P, _c_char_p, _ct.POINTER(_cint), _ct.POINTER(_cint), _c_char_p, _cint,
                            _c_char_p, _cint], _cint),
     "mpa_fm_param_set_uniform": ([_P, _c_char_p, _P], _cint),
@@ -185,8 +188,6 @@ MPA_ERR_INVALID, MPA_ERR_RUNTIME, MPA_ERR_HIP, MPA_ERR_INDEX = 1, 2, 3, 4
 # The exception a time-dependent term raised inside the library; the failing
 # entry point re-raises it.
 _term_error = None
-# (magnet kind, handle, parameter key) -> the callbacks the library holds
-_TIME_FNS = {}
 
 
 def _check(rc):
@@ -543,13 +544,17 @@ class Parameter(FieldQuantity):
             a = _np.ascontiguousarray(a)
             mptr = _ptr(a)
         _call(self._fm, "param_add_time_term", self._key, fn, None, mptr)
-        # the library calls fn until the terms are removed; magnet wrappers
-        # come and go, so the callbacks are kept by handle
-        _TIME_FNS.setdefault(self._fn_key(), []).append(fn)
+        # the library calls fn until the terms are removed or the world is
+        # destroyed; magnet wrappers come and go, so the world that owns the
+        # magnet keeps the callbacks (and drops them with itself)
+        self._time_fns().setdefault(self._fn_key(), []).append(fn)
 
     def remove_time_terms(self):
         _call(self._fm, "param_remove_time_terms", self._key)
-        _TIME_FNS.pop(self._fn_key(), None)
+        self._time_fns().pop(self._fn_key(), None)
+
+    def _time_fns(self):
+        return self._fm._world._time_fns
 
     def _fn_key(self):
         return (self._fm._pfx, self._fm._h.value, self._q)
@@ -904,7 +909,19 @@ class Ferromagnet(_Magnet):
 
     @property
     def noise_draws(self):
-        return self._int_flag("noise_draws")
+        return self.noise_state[1]
+
+    @property
+    def noise_state(self):
+        """(seed, step draws, quantity draws) of the thermal-noise stream, 64-bit counters."""
+        seed, a, b = _ct.c_uint32(), _ct.c_uint64(), _ct.c_uint64()
+        _check(_lib.mpa_fm_get_noise_state(self._h, _ct.byref(seed), _ct.byref(a), _ct.byref(b)))
+        return int(seed.value), int(a.value), int(b.value)
+
+    @noise_state.setter
+    def noise_state(self, value):
+        seed, a, b = (int(v) for v in value)
+        _check(_lib.mpa_fm_set_noise_state(self._h, seed & 0xFFFFFFFF, a, b))
 
     @property
     def is_sublattice(self):
@@ -1097,6 +1114,8 @@ class World:
         self._h = h
         self._magnets = {}
         self._afms = {}
+        # (magnet kind, handle, parameter key) -> the time-term callbacks the library holds
+        self._time_fns = {}
         self.timesolver = TimeSolver(self)
 
     def __del__(self):

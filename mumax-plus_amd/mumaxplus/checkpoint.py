@@ -4,9 +4,11 @@ The reference writes tables of quantities while it runs (TimeSolverOutput,
 timesolver.py:7-46) but cannot save and resume a simulation.  A snapshot
 holds everything a later run depends on: the magnetisation of every
 ferromagnet and of both sublattices of every antiferromagnet, the elastic
-displacement and velocity of magnets with elastodynamics, and the time
-solver's method, time, step and controller settings.  Parameters are the
-script's business and are not part of it.
+displacement and velocity of magnets with elastodynamics, the thermal-noise
+stream of every ferromagnet and sublattice (its key and 64-bit draw counters,
+so that a restored run draws the noise the uninterrupted run would have drawn),
+and the time solver's method, time, step and controller settings.  Parameters
+are the script's business and are not part of it.
 
 A checkpoint file is a NumPy ``.npz`` of plain arrays plus the settings as
 a JSON string; it is read back with ``allow_pickle=False``.
@@ -36,13 +38,25 @@ def _state_arrays(world):
     return out
 
 
+def _noise_owners(world):
+    """{key: ferromagnet or sublattice} of every thermal-noise stream."""
+    out = {f"ferromagnet/{name}": fm for name, fm in world.ferromagnets.items()}
+    for name, afm in world.antiferromagnets.items():
+        out[f"antiferromagnet/{name}/sub1"] = afm.sub1
+        out[f"antiferromagnet/{name}/sub2"] = afm.sub2
+    return out
+
+
 def snapshot(world):
-    """The world's dynamic state: {"arrays": {key: ndarray}, "solver": {...}}."""
+    """The world's dynamic state: {"arrays": {key: ndarray}, "noise": {key: [seed, step draws,
+    quantity draws]}, "solver": {...}}."""
     ts = world.timesolver
     solver = {k: float(getattr(ts, k)) for k in _SOLVER_KEYS}
     solver["adaptive_timestep"] = bool(ts.adaptive_timestep)
     solver["method"] = ts._impl.method
-    return {"arrays": {k: np.array(v, copy=True) for k, v in _state_arrays(world).items()}, "solver": solver}
+    noise = {k: list(mag._impl.noise_state) for k, mag in _noise_owners(world).items()}
+    return {"arrays": {k: np.array(v, copy=True) for k, v in _state_arrays(world).items()}, "noise": noise,
+            "solver": solver}
 
 
 def restore(world, snap):
@@ -65,6 +79,11 @@ def restore(world, snap):
             mag.elastic_velocity = value
         else:
             raise KeyError(f"unknown state '{key}'")
+    owners = _noise_owners(world)
+    for key, state in snap.get("noise", {}).items():
+        if key not in owners:
+            raise KeyError(f"the snapshot has a noise stream '{key}' this world does not have")
+        owners[key]._impl.noise_state = state
     s = snap["solver"]
     ts = world.timesolver
     ts.set_method(s["method"])
@@ -76,12 +95,14 @@ def restore(world, snap):
 def save(world, path):
     """Write a checkpoint of the world to `path` (.npz)."""
     snap = snapshot(world)
-    np.savez(path, _solver=np.array(json.dumps(snap["solver"])), **snap["arrays"])
+    np.savez(path, _solver=np.array(json.dumps(snap["solver"])), _noise=np.array(json.dumps(snap["noise"])),
+             **snap["arrays"])
 
 
 def load(world, path):
     """Restore the world from a checkpoint written by `save`."""
     with np.load(path, allow_pickle=False) as f:
         solver = json.loads(str(f["_solver"]))
-        arrays = {k: f[k] for k in f.files if k != "_solver"}
-    restore(world, {"arrays": arrays, "solver": solver})
+        noise = json.loads(str(f["_noise"])) if "_noise" in f.files else {}
+        arrays = {k: f[k] for k in f.files if k not in ("_solver", "_noise")}
+    restore(world, {"arrays": arrays, "noise": noise, "solver": solver})
