@@ -243,8 +243,16 @@ class Grid:
     """Grid(size, origin=(0,0,0)) (wrap_grid.cpp:4-17)."""
 
     def __init__(self, size, origin=(0, 0, 0)):
-        self.size = tuple(int(x) for x in _as_i3(size, "size"))
-        self.origin = tuple(int(x) for x in _as_i3(origin, "origin"))
+        self._size = tuple(int(x) for x in _as_i3(size, "size"))
+        self._origin = tuple(int(x) for x in _as_i3(origin, "origin"))
+
+    @property
+    def size(self):
+        return self._size
+
+    @property
+    def origin(self):
+        return self._origin
 
     @property
     def ncells(self):
@@ -340,7 +348,11 @@ class FieldQuantity:
         self._q = qname
         nc = _cint()
         _call(fm, "quantity_ncomp", _enc(qname), _ct.byref(nc))
-        self.ncomp = nc.value
+        self._ncomp = nc.value
+
+    @property
+    def ncomp(self):
+        return self._ncomp
 
     @property
     def name(self):
@@ -371,6 +383,82 @@ class FieldQuantity:
         out = _np.zeros(self.ncomp, dtype=_np.float64)
         _call(self._fm, "average", _enc(self._q), _ptr(out))
         return [float(v) for v in out]
+
+    def get_rgb(self):
+        """The quantity's RGB colours as a Field (wrap_fieldquantity.cpp:20; fieldops.cu:316-385)."""
+        system = getattr(self, "system", None)
+        geom = system.geometry if system is not None else None
+        return Field(self.grid, 3, _rgb(self.eval(), geom))
+
+
+def _rgb(v, geom):
+    """RGB colours (3, nz, ny, nx) of a vector field (fieldops.cu:316-385 of
+    the reference): the field divided by its largest norm in the geometry,
+    then hue = in-plane angle, saturation = norm and lightness = (1 + z) / 2,
+    in FP32; grey outside the geometry.  A display helper on the host, not
+    part of a time step."""
+    v = _np.asarray(v, dtype=_np.float32)
+    if v.shape[0] != 3:
+        raise ValueError("getRGB can only operate on vector fields with 3 components.")
+    if geom is None:
+        geom = _np.ones(v.shape[1:], dtype=bool)
+    norm = _np.sqrt((v * v).sum(axis=0))
+    v = v * _np.float32(1.0 / norm[geom].max())
+    x, y, z = v
+    pi = _np.float32(3.1415926535897931)
+    H = _np.arctan2(y, x)
+    S = _np.sqrt(x * x + y * y + z * z)
+    L = _np.float32(0.5) + _np.float32(0.5) * z
+    Hp = _np.float32(3.0) * H / pi
+    Hp = _np.where(Hp < 0, Hp + 6, _np.where(Hp >= 6, Hp - 6, Hp)).astype(_np.float32)
+    C = _np.where(L <= 0.5, 2 * L * S, 2 * (1 - L) * S).astype(_np.float32)
+    X = (C * (1 - _np.abs(_np.fmod(Hp, _np.float32(2)) - 1))).astype(_np.float32)
+    m = L - C / 2
+    sector = _np.floor(Hp).astype(int).clip(0, 5)
+    # (R, G, B) additions of C and X per 60-degree sector of the hue
+    addC = _np.array([[1, 0, 0], [0, 1, 0], [0, 1, 0], [0, 0, 1], [0, 0, 1], [1, 0, 0]], dtype=_np.float32)
+    addX = _np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1], [0, 1, 0], [1, 0, 0], [0, 0, 1]], dtype=_np.float32)
+    rgb = m[None] + C[None] * _np.moveaxis(addC[sector], -1, 0) + X[None] * _np.moveaxis(addX[sector], -1, 0)
+    rgb = _np.clip(rgb, 0, 1).astype(_np.float32)
+    return _np.where(geom[None], rgb, _np.float32(0.5))
+
+
+class Field:
+    """A field of values on a grid (wrap_field.cpp:10-17), held on the host:
+    ``get()`` returns a copy as (ncomp, nz, ny, nx), ``set()`` takes that
+    shape or ncomp values for a uniform field."""
+
+    def __init__(self, grid, ncomp, data=None):
+        self._grid = Grid(grid.size, grid.origin)
+        self._ncomp = int(ncomp)
+        nx, ny, nz = self._grid.size
+        self._data = _np.zeros((self._ncomp, nz, ny, nx), dtype=_np.float32)
+        if data is not None:
+            self.set(data)
+
+    @property
+    def grid(self):
+        return self._grid
+
+    @property
+    def ncomp(self):
+        return self._ncomp
+
+    def get(self):
+        return self._data.copy()
+
+    def set(self, data):
+        a = _np.asarray(data, dtype=_np.float32)
+        if a.ndim == 1:
+            if a.shape[0] != self._ncomp:
+                raise ValueError(f"The number of components do not match, expected {a.shape[0]}, "
+                                 f"got {self._ncomp}.")
+            self._data[...] = a[:, None, None, None]
+        elif a.shape == self._data.shape:
+            self._data[...] = a
+        else:
+            raise ValueError(f"The shape of the data does not match the shape of the field, expected "
+                             f"{self._data.shape}, got {a.shape}.")
 
 
 class ScalarQuantity:
@@ -404,7 +492,7 @@ class Variable(FieldQuantity):
     def __init__(self, fm, which="magnetization"):
         self._fm = fm
         self._q = which
-        self.ncomp = 3
+        self._ncomp = 3
 
     def get(self):
         nx, ny, nz = self.grid.size
@@ -445,7 +533,7 @@ class Parameter(FieldQuantity):
         nm = _ct.create_string_buffer(256)
         un = _ct.create_string_buffer(64)
         _call(fm, "param_info", self._key, _ct.byref(nc), _ct.byref(uni), nm, 256, un, 64)
-        self.ncomp = nc.value
+        self._ncomp = nc.value
         self._name = nm.value.decode()
         self._unit = un.value.decode()
 
@@ -705,8 +793,15 @@ class DmiTensor:
     KEYS = ("xxy", "xyz", "xxz", "yxy", "yyz", "yxz", "zxy", "zyz", "zxz")
 
     def __init__(self, fm):
-        for k in self.KEYS:
-            setattr(self, k, Parameter(fm, "dmi_" + k))
+        self._elements = {k: Parameter(fm, "dmi_" + k) for k in self.KEYS}
+
+
+def _element_property(key):
+    return property(lambda self: self._elements[key], doc=f"DMI tensor element D_{key} (wrap_dmitensor.cpp)")
+
+
+for _k in DmiTensor.KEYS:
+    setattr(DmiTensor, _k, _element_property(_k))
 
 
 # -------------------------------------------------------------- Magnets
@@ -818,6 +913,13 @@ class _Magnet:
             raise ValueError("elasticVelocity Variable does not exist yet. Enable elastodynamics first.")
         return Variable(self, "elastic_velocity")
 
+    def stray_field_from_magnet(self, source):
+        """The stray field of `source` on this magnet's grid, zero outside this
+        magnet's geometry (wrap_magnet.cpp:45-53)."""
+        if source == self:
+            raise RuntimeError("Can not compute the magnetic field of the magnet")
+        return StrayField(source, self.grid, mask=self.system.geometry)
+
     def field_quantity(self, name):
         return FieldQuantity(self, name)
 
@@ -847,8 +949,16 @@ class Ferromagnet(_Magnet):
 
     def __init__(self, handle, world):
         super().__init__(handle, world)
-        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
-        object.__setattr__(self, "magnetization", Variable(self))
+        object.__setattr__(self, "_dmi", DmiTensor(self))
+        object.__setattr__(self, "_magnetization", Variable(self))
+
+    @property
+    def dmi_tensor(self):
+        return self._dmi
+
+    @property
+    def magnetization(self):
+        return self._magnetization
 
     def _has_elastic_params(self):
         return self._host_handle() is None
@@ -947,13 +1057,18 @@ class Antiferromagnet(_Magnet):
 
     def __init__(self, handle, world):
         super().__init__(handle, world)
-        object.__setattr__(self, "dmi_tensor", DmiTensor(self))
+        object.__setattr__(self, "_dmi", DmiTensor(self))
         subs = []
         for i in range(2):
             h = _P()
             _check(_lib.mpa_afm_sublattice(self._h, i, _ct.byref(h)))
             subs.append(Ferromagnet(h, world))
         object.__setattr__(self, "_subs", tuple(subs))
+
+    @property
+    def dmi_tensor(self):
+        """The inter-sublattice DMI tensor (wrap_antiferromagnet.cpp:29)."""
+        return self._dmi
 
     def sub1(self):
         return self._subs[0]
@@ -972,7 +1087,27 @@ class Antiferromagnet(_Magnet):
         raise IndexError("Sublattice not found in Antiferromagnet.")
 
 
-class StrayField:
+def _param_property(key):
+    def get(self):
+        params = object.__getattribute__(self, "_params")
+        if key not in params:
+            raise AttributeError(f"'{type(self).__name__}' object has no attribute '{key}'")
+        return params[key]
+    return property(get, doc=f"the '{key}' parameter (wrap_ferromagnet.cpp, wrap_antiferromagnet.cpp, "
+                             f"wrap_magnet.cpp)")
+
+
+for _k in _Magnet._ELASTIC_PARAMS + _Magnet._ELASTIC_VPARAMS:
+    setattr(_Magnet, _k, _param_property(_k))
+for _cls in (Ferromagnet, Antiferromagnet):
+    for _k in _cls._PARAMS + _cls._VPARAMS + _cls._INTER:
+        setattr(_cls, _k, _param_property(_k))
+
+# the reference registers the common base as "Magnet" (wrap_magnet.cpp:17)
+Magnet = _Magnet
+
+
+class StrayField(FieldQuantity):
     """StrayField(magnet, grid) (wrap_strayfield.cpp:10-25): the field of a
     magnet on a grid; set_method("fft" | "brute")."""
 
@@ -985,7 +1120,7 @@ class StrayField:
         self._grid = Grid(grid.size, grid.origin)
         self._method = 0
         self._mask = mask  # destination geometry (bool (nz, ny, nx)) or None
-        self.ncomp = 3
+        self._ncomp = 3
 
     def set_method(self, method):
         if method not in self._METHODS:
@@ -1056,21 +1191,17 @@ class TimeSolver:
         _check(_lib.mpa_ts_set(self._world._h, _enc(key), float(value)))
 
     def __getattr__(self, key):
-        if key in TimeSolver._KEYS or key in ("nsteps", "nrejected", "nrhs"):
+        if key in ("nsteps", "nrejected", "nrhs"):
             return self._get(key)
         raise AttributeError(key)
-
-    def __setattr__(self, key, value):
-        if key in TimeSolver._KEYS:
-            self._set(key, value)
-        elif key == "adaptive_timestep":
-            self._set(key, 1.0 if value else 0.0)
-        else:
-            object.__setattr__(self, key, value)
 
     @property
     def adaptive_timestep(self):
         return self._get("adaptive_timestep") != 0.0
+
+    @adaptive_timestep.setter
+    def adaptive_timestep(self, value):
+        self._set("adaptive_timestep", 1.0 if value else 0.0)
 
     @property
     def sensible_timestep(self):
@@ -1101,6 +1232,15 @@ class TimeSolver:
             self.step()
 
 
+def _solver_property(key):
+    return property(lambda self: self._get(key), lambda self, value: self._set(key, value),
+                    doc=f"TimeSolver.{key} (wrap_timesolver.cpp:15-44)")
+
+
+for _k in TimeSolver._KEYS:
+    setattr(TimeSolver, _k, _solver_property(_k))
+
+
 # --------------------------------------------------------------- World
 class World:
     """World(cellsize[, mastergrid, pbc_repetitions]) (wrap_world.cpp:36-119)."""
@@ -1116,7 +1256,11 @@ class World:
         self._afms = {}
         # (magnet kind, handle, parameter key) -> the time-term callbacks the library holds
         self._time_fns = {}
-        self.timesolver = TimeSolver(self)
+        self._timesolver = TimeSolver(self)
+
+    @property
+    def timesolver(self):
+        return self._timesolver
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -1398,3 +1542,82 @@ class DemagPlan:
         b, d = _cdouble(), _cdouble()
         _check(_lib.mpa_demag_plan_info(self._h, ks, _ct.byref(b), _ct.byref(d)))
         return tuple(ks), b.value, d.value
+
+
+# ------------------------------------------- module-level quantity factories
+# The reference registers one module function per quantity; each takes a
+# magnet and returns a quantity object bound to it (wrap_ferromagnet.cpp:75-128,
+# wrap_antiferromagnet.cpp:35-44, wrap_magnet.cpp:56-75).  The objects are the
+# same FieldQuantity / ScalarQuantity as the package's, in the subclasses the
+# reference registers (wrap_fieldquantity.cpp, wrap_scalarquantity.cpp).
+class MagnetFieldQuantity(FieldQuantity):
+    """M_FieldQuantity: a field quantity of any magnet."""
+
+
+class FerromagnetFieldQuantity(FieldQuantity):
+    """FM_FieldQuantity: a field quantity of a ferromagnet or sublattice."""
+
+
+class AntiferromagnetFieldQuantity(FieldQuantity):
+    """AFM_FieldQuantity: a field quantity of an antiferromagnet."""
+
+
+class MagnetScalarQuantity(ScalarQuantity):
+    """M_ScalarQuantity."""
+
+
+class FerromagnetScalarQuantity(ScalarQuantity):
+    """FM_ScalarQuantity."""
+
+
+class AntiferromagnetScalarQuantity(ScalarQuantity):
+    """AFM_ScalarQuantity."""
+
+
+# wrap_ferromagnet.cpp:75-128
+_FM_FIELD = ("torque", "llg_torque", "damping_torque", "spin_transfer_torque", "demag_energy_density",
+             "anisotropy_field", "anisotropy_energy_density", "exchange_field", "exchange_energy_density",
+             "inhomogeneous_exchange_field", "homogeneous_exchange_field", "inhomogeneous_exchange_energy_density",
+             "homogeneous_exchange_energy_density", "dmi_field", "dmi_energy_density", "external_field",
+             "zeeman_energy_density", "effective_field", "total_energy_density", "conductivity_tensor",
+             "electrical_potential", "thermal_noise", "full_magnetization", "magnetoelastic_field",
+             "magnetoelastic_energy_density", "magnetoelastic_force")
+_FM_SCALAR = ("max_torque", "demag_energy", "anisotropy_energy", "exchange_energy", "max_angle",
+              "inhomogeneous_exchange_energy", "homogeneous_exchange_energy", "dmi_energy", "zeeman_energy",
+              "total_energy", "magnetoelastic_energy")
+# wrap_antiferromagnet.cpp:35-44
+_AFM_FIELD = ("neel_vector", "full_magnetization", "angle_field", "total_energy_density")
+_AFM_SCALAR = ("max_intracell_angle", "total_energy")
+# wrap_magnet.cpp:61-75
+_MAGNET_FIELD = ("strain_tensor", "stress_tensor", "elastic_force", "effective_body_force", "elastic_damping",
+                 "elastic_acceleration", "kinetic_energy_density", "elastic_energy_density", "poynting_vector")
+_MAGNET_SCALAR = ("kinetic_energy", "elastic_energy")
+
+
+def _quantity_factory(name):
+    fm_cls = (FerromagnetFieldQuantity if name in _FM_FIELD else
+              FerromagnetScalarQuantity if name in _FM_SCALAR else None)
+    afm_cls = (AntiferromagnetFieldQuantity if name in _AFM_FIELD else
+               AntiferromagnetScalarQuantity if name in _AFM_SCALAR else None)
+    if name in _MAGNET_FIELD:
+        fm_cls = afm_cls = MagnetFieldQuantity
+    elif name in _MAGNET_SCALAR:
+        fm_cls = afm_cls = MagnetScalarQuantity
+    kinds = " or ".join(k for k, c in (("Ferromagnet", fm_cls), ("Antiferromagnet", afm_cls)) if c)
+
+    def factory(magnet):
+        if fm_cls is not None and isinstance(magnet, Ferromagnet):
+            return fm_cls(magnet, name)
+        if afm_cls is not None and isinstance(magnet, Antiferromagnet):
+            return afm_cls(magnet, name)
+        raise TypeError(f"{name}(): incompatible function arguments; expected a {kinds}, "
+                        f"got {type(magnet).__name__}")
+
+    factory.__name__ = factory.__qualname__ = name
+    factory.__doc__ = f"The {name} quantity of a {kinds}."
+    return factory
+
+
+for _name in dict.fromkeys(_FM_FIELD + _FM_SCALAR + _AFM_FIELD + _AFM_SCALAR + _MAGNET_FIELD + _MAGNET_SCALAR):
+    globals()[_name] = _quantity_factory(_name)
+del _name, _k, _cls

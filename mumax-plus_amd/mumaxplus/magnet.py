@@ -128,10 +128,7 @@ class Magnet:
         """The stray field of `source_magnet` on this magnet (zero outside
         this magnet's geometry)."""
         from .strayfield import StrayField
-        from . import _mumaxpluscpp as _cpp
-        if source_magnet == self:
-            raise RuntimeError("Can not compute the magnetic field of the magnet")
-        return StrayField._from_impl(_cpp.StrayField(source_magnet._impl, self._impl.grid, mask=self.geometry))
+        return StrayField._from_impl(self._impl.stray_field_from_magnet(source_magnet._impl))
 
     # ---------------------------------------------------- elastodynamics
     @property
