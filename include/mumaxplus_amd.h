@@ -339,6 +339,13 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
  * bytes each), the summed milliseconds, launch count and summed algorithmic bytes; *count
  * receives the number of kernel names (at most `cap` are written). */
 int mpa_world_set_profiling(mpa_world* w, int enable);
+/* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 1): a world whose
+ * only variable is one magnetisation on the uniform-parameter LLG path, stepped with a
+ * first-same-as-last tableau, computes each stage state inside the field launch that produces
+ * the last derivative it needs (no separate combination launch); 0 runs the separate
+ * combination, with the same arithmetic. */
+int mpa_world_set_option(mpa_world* w, const char* key, int value);
+int mpa_world_get_option(const mpa_world* w, const char* key, int* value);
 int mpa_world_profile_reset(mpa_world* w);
 int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);

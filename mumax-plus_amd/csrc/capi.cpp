@@ -1146,6 +1146,27 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
   });
 }
 
+int mpa_world_set_option(mpa_world* w, const char* key, int value) {
+  return guard([&] {
+    need(w, "world");
+    need(key, "key");
+    const std::string k = key;
+    if (k == "fuse_stages") w->w.solver.fuse_stages = value != 0;
+    else throw Error(MPA_ERR_INVALID, "unknown world option '" + k + "'");
+  });
+}
+
+int mpa_world_get_option(const mpa_world* w, const char* key, int* value) {
+  return guard([&] {
+    need(w, "world");
+    need(key, "key");
+    need(value, "value");
+    const std::string k = key;
+    if (k == "fuse_stages") *value = w->w.solver.fuse_stages ? 1 : 0;
+    else throw Error(MPA_ERR_INVALID, "unknown world option '" + k + "'");
+  });
+}
+
 // ---------------------------------------------------------------- Timing
 int mpa_world_set_profiling(mpa_world* w, int enable) {
   return guard([&] {

@@ -69,19 +69,26 @@ struct FMUniform {
 
 __device__ __forceinline__ float3 gamma_u(const FMUniform& u, int axis, float3 m) { return dmi_gamma(u.D, axis, m); }
 
+// What a field kernel does with a cell's result besides storing it: nothing,
+// or (NextStage) the next Runge-Kutta stage state of the cell.
+struct NoEpilogue {
+  __device__ __forceinline__ void operator()(int, float3, float3, bool) const {}
+};
+
 // One cell (x, y, z) of the uniform path; m and out have component stride N
 // (the magnet's cell count, or the packed length of a batch); hdem(idx) gives
 // the demag field of the cell (read from memory, or the x-inverse FFT pass's
 // own registers when the field is computed inside that pass).
-template <class HDem>
+template <class HDem, class Epi = NoEpilogue>
 __device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUniform& u, const float* __restrict__ m,
                                                    const HDem& hdem, float* __restrict__ out, int mode, int x, int y,
-                                                   int z, int N) {
+                                                   int z, int N, const Epi& epi = Epi()) {
   const int idx = (z * d.ny + y) * d.nx + x;
   if (d.geom && !d.geom[idx]) {
     out[idx] = 0.f;
     out[N + idx] = 0.f;
     out[2 * N + idx] = 0.f;
+    epi(idx, f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f), false);
     return;
   }
   const float3 mc = load3(m, idx, N);
@@ -181,6 +188,7 @@ __device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUnif
   out[idx] = r.x;
   out[N + idx] = r.y;
   out[2 * N + idx] = r.z;
+  epi(idx, mc, r, true);
 }
 
 

@@ -436,9 +436,10 @@ __global__ void __launch_bounds__(BLOCK) k_field_pair(FMDesc d0, const float* __
   else field_body<UNI>(d1, m1, h1, o1, mode, bid);
 }
 
+template <bool NEXT>
 __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                          const float* __restrict__ hdemag, float* __restrict__ out,
-                                                         int mode) {
+                                                         int mode, NextStage next) {
   // 2-D blocks: threadIdx.x along x, threadIdx.y along y, one z plane per
   // block row.  Workgroups are dealt round-robin to the 8 XCDs in dispatch
   // order; the remap gives each XCD a contiguous run of tiles, so the y and z
@@ -455,7 +456,11 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
 #pragma unroll
   for (int k = 0; k < MPA_FIELD_ZPER; ++k) {
     const int z = (rest / nby) * MPA_FIELD_ZPER + k;
-    if (z < d.nz) field_uniform_cell(d, u, m, [&](int i) { return load3(hdemag, i, N); }, out, mode, x, y, z, N);
+    if (z < d.nz) {
+      auto hd = [&](int i) { return load3(hdemag, i, N); };
+      if constexpr (NEXT) field_uniform_cell(d, u, m, hd, out, mode, x, y, z, N, next);
+      else field_uniform_cell(d, u, m, hd, out, mode, x, y, z, N);
+    }
   }
 }
 
@@ -664,15 +669,39 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
   checkLaunch("k_field_pair");
 }
 
+namespace {
+void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
+                          const NextStage* next, hipStream_t s) {
+  const FMUniform u = make_uniform(d);
+  int bx = 1;
+  while (bx < d.nx && bx < MPA_FIELD_TILE_X) bx <<= 1;
+  const int by = BLOCK / bx;
+  dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, (d.nz + MPA_FIELD_ZPER - 1) / MPA_FIELD_ZPER);
+  if (next) {
+    k_field_uniform<true><<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, *next);
+    checkLaunch("k_field_uniform<next>");
+  } else {
+    k_field_uniform<false><<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, NextStage{});
+    checkLaunch("k_field_uniform");
+  }
+}
+}  // namespace
+
+bool field_next_fusable(const FMDesc& d, FieldMode mode) {
+  return mode == MODE_LLG && all_uniform(d) && d.nz <= 65535;
+}
+
+void launch_field_next(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
+                       const NextStage& next, hipStream_t s) {
+  if (!field_next_fusable(d, mode)) throw Error(MPA_ERR_RUNTIME, "launch_field_next: not the uniform LLG path");
+  if (next.nk < 0 || next.nk > 7 || !next.out || next.out == m || next.out == out || next.out == next.x0)
+    throw Error(MPA_ERR_RUNTIME, "launch_field_next: bad stage combination");
+  launch_field_uniform(d, m, hdemag, out, mode, &next, s);
+}
+
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s) {
   if (mode != MODE_EDENS && all_uniform(d) && d.nz <= 65535) {
-    const FMUniform u = make_uniform(d);
-    int bx = 1;
-    while (bx < d.nx && bx < MPA_FIELD_TILE_X) bx <<= 1;
-    const int by = BLOCK / bx;
-    dim3 grid((d.nx + bx - 1) / bx, (d.ny + by - 1) / by, (d.nz + MPA_FIELD_ZPER - 1) / MPA_FIELD_ZPER);
-    k_field_uniform<<<grid, dim3(bx, by, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode);
-    checkLaunch("k_field_uniform");
+    launch_field_uniform(d, m, hdemag, out, mode, nullptr, s);
     return;
   }
   const int nblk = (d.ncell + BLOCK - 1) / BLOCK;

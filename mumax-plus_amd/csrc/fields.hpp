@@ -91,6 +91,44 @@ enum FieldMode {
 // out may alias nothing.  hdemag (3N) is added when TERM_DEMAG is set.
 void launch_field(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode, hipStream_t s);
 
+// x_next = normalise(x0 + sum_j c_j k_j + cself * k) at the cell, zero
+// outside the geometry, with k the derivative just computed: the stage
+// combination of the Runge-Kutta stepper fused into the field kernel that
+// produces the last derivative it needs.  Same arithmetic, term order and
+// normalisation as k_combine_batch.  x0 == nullptr takes the cell's own
+// state as x0 (the first stage of the next step, whose x0 is the state the
+// field was evaluated at).
+struct NextStage {
+  const float* x0;
+  const float* k[7];
+  float c[7];
+  float cself;
+  int nk;
+  float* out;
+  int N;
+  __device__ __forceinline__ void operator()(int idx, float3 mc, float3 r, bool in) const {
+    float v[3] = {mc.x, mc.y, mc.z};
+    const float rr[3] = {r.x, r.y, r.z};
+#pragma unroll
+    for (int c3 = 0; c3 < 3; ++c3) {
+      float a = x0 ? x0[(size_t)c3 * N + idx] : v[c3];
+      for (int j = 0; j < nk; ++j) a += c[j] * k[j][(size_t)c3 * N + idx];
+      a += cself * rr[c3];
+      v[c3] = a;
+    }
+    const float inv = rsqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+    for (int c3 = 0; c3 < 3; ++c3) out[(size_t)c3 * N + idx] = in ? v[c3] * inv : 0.f;
+  }
+};
+
+// launch_field on the uniform-parameter LLG path that also writes the next
+// Runge-Kutta stage state (NextStage above); field_next_fusable
+// tells whether a magnet is on that path.
+bool field_next_fusable(const FMDesc& d, FieldMode mode);
+void launch_field_next(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
+                       const NextStage& next, hipStream_t s);
+
 // launch_field for the two sublattices of an antiferromagnet in one launch
 // (two separate launches when their parameter layouts differ).
 void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float* o0, const FMDesc& d1,

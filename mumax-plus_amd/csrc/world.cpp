@@ -936,12 +936,28 @@ FMDesc Ferromagnet::rhs_desc(unsigned terms, double noise_dt) {
   return d;
 }
 
-void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode, double noise_dt) {
+void Ferromagnet::eval_rhs(float* out, hipStream_t s, bool prepared, FieldMode mode, double noise_dt,
+                           const NextStage* next) {
   const unsigned terms = active_terms();
   Profiler* prof = &world->prof;
   if (!prepared) prepare(terms, s, prof);
+  if (next) {
+    // plus the combination: read x0 (unless it is the cell's own state) and
+    // nk derivatives, write the next stage state
+    const double bytes = rhs_bytes(terms) + 12.0 * N * ((next->x0 ? 1 : 0) + next->nk + 1);
+    Timed t(prof, "k_field_next", bytes, s);
+    launch_field_next(rhs_desc(terms, noise_dt), m.get(), demag_field(), out, mode, *next, s);
+    return;
+  }
   Timed t(prof, "k_field", rhs_bytes(terms), s);
   launch_field(rhs_desc(terms, noise_dt), m.get(), demag_field(), out, mode, s);
+}
+
+bool Ferromagnet::next_fusable() {
+  if (host || el.enabled) return false;
+  const unsigned terms = active_terms();
+  if (terms & TERM_THERMAL) return false;
+  return field_next_fusable(desc(terms), MODE_LLG);
 }
 
 // Both sublattices' right-hand sides in one launch (everything they read is
@@ -1313,7 +1329,9 @@ std::vector<StateVar> World::state_vars() {
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
 void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready,
-                         double noise_dt) {
+                         double noise_dt, const NextStage* next) {
+  if (next && (vars.size() != 1 || vars[0].kind != StateVar::MAG || relaxing_))
+    throw Error(MPA_ERR_RUNTIME, "a fused stage combination needs a single magnetisation variable");
   for (auto& fm : magnets) {
     if (fm->el.enabled) fm->compute_strain(s);
     fm->compute_stray(s, &prof);
@@ -1342,7 +1360,7 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
         const unsigned t = fm->active_terms();
         if (t & TERM_DEMAG)
           fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
-        fm->eval_rhs(out, s, true, mode, noise_dt);
+        fm->eval_rhs(out, s, true, mode, noise_dt, next);
       }
     } else if (sv.kind == StateVar::DISP) {
       // du/dt = v (elastodynamics.cu:93-96); a stage's combination may have
@@ -1645,6 +1663,18 @@ void TimeSolver::step() {
       noisy = true;
     }
   bool k0_valid = !noisy && fsal_valid_ && fsal_version_ == world_->version && fsal_time_ == t0;
+  // One independent magnet on the uniform-parameter LLG path with a
+  // first-same-as-last tableau: each evaluation also writes the next stage
+  // state (the combination's last derivative is the one just computed), so
+  // no separate combination launch runs.  With a fixed step the step's last
+  // evaluation writes the next step's first stage state as well.
+  Ferromagnet* fused = nullptr;
+  if (fuse_stages && !noisy && tb.fsal && S > 1 && vars.size() == 1 && vars[0].kind == StateVar::MAG && !world_->relaxing() &&
+      vars[0].fm->next_fusable())
+    fused = vars[0].fm;
+  bool next_ready = fused && k0_valid && next_valid_ && next_h_ == dt;
+  next_valid_ = false;
+  if (fused && fused->xn.n == 0) fused->xn.allocate((size_t)3 * fused->N);
   // x0 = the state at the step start.  When the first stage's derivative is
   // reused, nothing reads the state before the second stage's combination
   // overwrites it, so the state array and x0 trade buffers instead of copying.
@@ -1700,11 +1730,43 @@ void TimeSolver::step() {
   };
   while (true) {
     const double h = dt;
+    bool have_next = next_ready;  // the magnet's xn holds the coming stage's state
+    next_ready = false;
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
-      if (st > 0) combine(tb.A[st], st, h, st);
+      if (st > 0) {
+        if (have_next) {
+          std::swap(fused->m, fused->xn);
+          vars[0].x = fused->m.get();
+          have_next = false;
+        } else {
+          combine(tb.A[st], st, h, st);
+        }
+      }
       if (st == 0 && k0_valid) continue;
-      world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0);
+      const bool cross = st + 1 == S;  // the next step's first stage
+      if (fused && !(cross && adaptive)) {
+        NextStage ns{};
+        const std::vector<double>& a = tb.A[cross ? 1 : st + 1];
+        ns.x0 = cross ? nullptr : vars[0].x0->get();
+        ns.nk = cross ? 0 : st;
+        for (int j = 0; j < ns.nk; ++j) {
+          ns.k[j] = (*vars[0].k)[j].get();
+          ns.c[j] = (float)(h * a[j]);
+        }
+        ns.cself = (float)(h * a[cross ? 0 : st]);
+        ns.out = fused->xn.get();
+        ns.N = fused->N;
+        world_->eval_rhs_all(vars, st, s, st > 0, 0.0, &ns);
+        if (cross) {
+          next_valid_ = true;
+          next_h_ = h;
+        } else {
+          have_next = true;
+        }
+      } else {
+        world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0);
+      }
       nrhs += (long long)vars.size();
     }
     time = t0 + h;

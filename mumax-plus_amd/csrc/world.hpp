@@ -227,6 +227,7 @@ class Ferromagnet : public MagnetCore {
 
   DeviceArray<float> m;       // magnetisation, 3N
   DeviceArray<float> x0;      // RK start state
+  DeviceArray<float> xn;      // next stage state written by a fused field evaluation
   DeviceArray<float> hdemag;  // demag field scratch (independent magnets)
   DeviceArray<float> scratch; // quantity evaluation
   // electrical potential (Ferromagnet::poissonSystem, ferromagnet.hpp:55)
@@ -250,7 +251,11 @@ class Ferromagnet : public MagnetCore {
   // field and strain are taken as already computed.
   // noise_dt > 0: add the thermal field of the step in progress divided by
   // sqrt(noise_dt) (the stepper's noise term).
-  void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG, double noise_dt = 0.0);
+  void eval_rhs(float* out, hipStream_t s, bool prepared = false, FieldMode mode = MODE_LLG, double noise_dt = 0.0,
+                const NextStage* next = nullptr);
+  // The right-hand side can also write the next stage state in the same
+  // launch (the uniform-parameter LLG path of an independent magnet).
+  bool next_fusable();
   double rhs_bytes(unsigned terms);
   FMDesc rhs_desc(unsigned terms, double noise_dt);
   int eval_quantity(const std::string& q, float* host, hipStream_t s);
@@ -329,6 +334,7 @@ class TimeSolver {
   double sensible_factor = 0.01, sensible_default = 1e-14;
   std::string method = "Fehlberg";
   long long nsteps = 0, nrejected = 0, nrhs = 0;
+  bool fuse_stages = true;  // stage combinations inside the field launch where possible
 
   void set_method(const std::string& name);
   double sensible_timestep();
@@ -343,6 +349,10 @@ class TimeSolver {
   bool fsal_valid_ = false;
   double fsal_time_ = 0.0;
   unsigned long long fsal_version_ = 0;
+  // the last step's final evaluation also wrote the next step's first stage
+  // state (fixed step h) into the magnet's xn
+  bool next_valid_ = false;
+  double next_h_ = 0.0;
 };
 
 class World {
@@ -383,8 +393,11 @@ class World {
   // disp_ready: the displacements' k[stage] (the stage velocity) is written.
   // noise_dt > 0: magnetisation right-hand sides include the thermal noise
   // term of the step in progress (rungekutta.cpp:95-106).
+  // next: the single magnetisation variable's evaluation also writes the
+  // next stage state (Ferromagnet::next_fusable).
   void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false,
-                    double noise_dt = 0.0);
+                    double noise_dt = 0.0, const NextStage* next = nullptr);
+  bool relaxing() const { return relaxing_; }
 
   // Field of `src` on the grid (size, origin); method 0 = FFT, 1 = direct sum.
   void stray_field_on_grid(MagnetCore* src, const int size[3], const int origin[3], int method, float* host_out);

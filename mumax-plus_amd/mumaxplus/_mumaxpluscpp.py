@@ -160,6 +160,8 @@ _SIGS = {
     "mpa_demag_plan_exec": ([_P, _P, _P, _ct.c_float, _P, _P, _P], _cint),
     "mpa_demag_plan_info": ([_P, _P, _ct.POINTER(_cdouble), _ct.POINTER(_cdouble)], _cint),
     "mpa_world_set_profiling": ([_P, _cint], _cint),
+    "mpa_world_set_option": ([_P, _c_char_p, _cint], _cint),
+    "mpa_world_get_option": ([_P, _c_char_p, _ct.POINTER(_cint)], _cint),
     "mpa_world_profile_reset": ([_P], _cint),
     "mpa_world_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
     "mpa_batch_create": ([_ct.c_longlong, _c_char_p, _ct.POINTER(_P)], _cint),
@@ -1387,6 +1389,15 @@ class World:
     def stream(self):
         v = _P()
         _check(_lib.mpa_world_stream(self._h, _ct.byref(v)))
+        return v.value
+
+    def set_option(self, key, value):
+        """Stepper switches of the library (mpa_world_set_option), e.g. "fuse_stages"."""
+        _check(_lib.mpa_world_set_option(self._h, _enc(key), int(value)))
+
+    def get_option(self, key):
+        v = _cint()
+        _check(_lib.mpa_world_get_option(self._h, _enc(key), _ct.byref(v)))
         return v.value
 
     def set_profiling(self, enable):
