@@ -339,11 +339,14 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
  * bytes each), the summed milliseconds, launch count and summed algorithmic bytes; *count
  * receives the number of kernel names (at most `cap` are written). */
 int mpa_world_set_profiling(mpa_world* w, int enable);
-/* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 1): a world whose
+/* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 0): a world whose
  * only variable is one magnetisation on the uniform-parameter LLG path, stepped with a
  * first-same-as-last tableau, computes each stage state inside the field launch that produces
  * the last derivative it needs (no separate combination launch); 0 runs the separate
- * combination, with the same arithmetic. */
+ * combination, with the same arithmetic.  "field_zchunk" (default 4): z planes per workgroup of the
+ * uniform-parameter field kernel's LDS halo-tile variant (magnets with no geometry mask or periodic
+ * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant.  Both give the
+ * same values bit for bit. */
 int mpa_world_set_option(mpa_world* w, const char* key, int value);
 int mpa_world_get_option(const mpa_world* w, const char* key, int* value);
 int mpa_world_profile_reset(mpa_world* w);

@@ -75,28 +75,22 @@ struct NoEpilogue {
   __device__ __forceinline__ void operator()(int, float3, float3, bool) const {}
 };
 
-// One cell (x, y, z) of the uniform path; m and out have component stride N
-// (the magnet's cell count, or the packed length of a batch); hdem(idx) gives
-// the demag field of the cell (read from memory, or the x-inverse FFT pass's
-// own registers when the field is computed inside that pass).
-template <class HDem, class Epi = NoEpilogue>
-__device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUniform& u, const float* __restrict__ m,
-                                                   const HDem& hdem, float* __restrict__ out, int mode, int x, int y,
-                                                   int z, int N, const Epi& epi = Epi()) {
-  const int idx = (z * d.ny + y) * d.nx + x;
-  if (d.geom && !d.geom[idx]) {
-    out[idx] = 0.f;
-    out[N + idx] = 0.f;
-    out[2 * N + idx] = 0.f;
-    epi(idx, f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f), false);
-    return;
-  }
-  const float3 mc = load3(m, idx, N);
+// The field (mode MODE_FIELD) or torque of one cell on the uniform path from
+// its own value mc and its six neighbour values mn[dir] (dir 0:-x 1:+x 2:-y
+// 3:+y 4:-z 5:+z).  ok[dir] is false for a missing neighbour (outside the
+// grid or the geometry), whose mn is not read.  Only the first ndir
+// directions are visited (4 skips the z axis of a flat layer).  hs: the stray
+// field of the other magnets (added when has_stray); hd: the demag field
+// (added when u.demag).  Term order: exchange, anisotropy, external, stray,
+// DMI, demag.
+__device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, float3 mc, const float3 (&mn)[6],
+                                                 const bool (&ok)[6], int ndir, bool has_stray, float3 hs,
+                                                 float3 hd) {
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
-  // DMI field of neighbour value mn along `axis`, before the direction sign
-  auto dmi_of = [&](int axis, float3 mn) {
-    return f3(u.dxy[axis] * mn.y + u.dxz[axis] * mn.z, -u.dxy[axis] * mn.x + u.dyz[axis] * mn.z,
-              -u.dxz[axis] * mn.x - u.dyz[axis] * mn.y);
+  // DMI field of neighbour value v along `axis`, before the direction sign
+  auto dmi_of = [&](int axis, float3 v) {
+    return f3(u.dxy[axis] * v.y + u.dxz[axis] * v.z, -u.dxy[axis] * v.x + u.dyz[axis] * v.z,
+              -u.dxz[axis] * v.x - u.dyz[axis] * v.y);
   };
   // Missing or outside-geometry neighbour: Neumann with the DMI correction,
   // or the open boundary.
@@ -106,29 +100,6 @@ __device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUnif
     if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
   };
   if (u.has_ex || u.has_dmi) {
-    // Every neighbour value is loaded before any is used (a missing
-    // neighbour reads the cell itself and is not used), so the six loads are
-    // in flight together; the wrap, geometry and boundary rules then act on
-    // registers.  A single layer without z-axis DMI entries skips z: its two
-    // terms cancel.
-    const int ndir = u.zflat ? 4 : 6;
-    int nbi[6];
-    bool ok[6];
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-      if (dir < ndir) {
-        const int nb = neighbour(d, x, y, z, dir);
-        ok[dir] = nb >= 0 && (!d.geom || d.geom[nb]);
-        nbi[dir] = nb >= 0 ? nb : idx;
-      } else {
-        ok[dir] = false;
-        nbi[dir] = idx;
-      }
-    }
-    float3 mn[6];
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir)
-      if (dir < ndir) mn[dir] = load3(m, nbi[dir], N);
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
       if (2 * axis >= ndir) break;
@@ -171,20 +142,67 @@ __device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUnif
   float3 h = hex;
   h = h + han;
   h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
-  if (d.hstray) h = h + load3(d.hstray, idx, d.ncell);
+  if (has_stray) h = h + hs;
   h = h + hdm;
-  if (u.demag) h = h + hdem(idx);
-  float3 r;
-  if (mode == MODE_FIELD) {
-    r = h;
-  } else if (u.frozen) {
-    r = f3(0.f, 0.f, 0.f);
-  } else if (mode == MODE_LLG) {
+  if (u.demag) h = h + hd;
+  if (mode == MODE_FIELD) return h;
+  if (u.frozen) return f3(0.f, 0.f, 0.f);
+  if (mode == MODE_LLG) {
     const float3 mxh = cross3(mc, h);
-    r = u.llg * (mxh + u.alpha * cross3(mc, mxh));
-  } else {
-    r = -GAMMALL_F * cross3(mc, cross3(mc, h));
+    return u.llg * (mxh + u.alpha * cross3(mc, mxh));
   }
+  return -GAMMALL_F * cross3(mc, cross3(mc, h));
+}
+
+// One cell (x, y, z) of the uniform path; m and out have component stride N
+// (the magnet's cell count, or the packed length of a batch); hdem(idx) gives
+// the demag field of the cell (read from memory, or the x-inverse FFT pass's
+// own registers when the field is computed inside that pass).
+template <class HDem, class Epi = NoEpilogue>
+__device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUniform& u, const float* __restrict__ m,
+                                                   const HDem& hdem, float* __restrict__ out, int mode, int x, int y,
+                                                   int z, int N, const Epi& epi = Epi()) {
+  const int idx = (z * d.ny + y) * d.nx + x;
+  if (d.geom && !d.geom[idx]) {
+    out[idx] = 0.f;
+    out[N + idx] = 0.f;
+    out[2 * N + idx] = 0.f;
+    epi(idx, f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f), false);
+    return;
+  }
+  const float3 mc = load3(m, idx, N);
+  // Every neighbour value is loaded before any is used (a missing neighbour
+  // reads the cell itself and is not used), so the six loads are in flight
+  // together; the wrap, geometry and boundary rules then act on registers.
+  // A single layer without z-axis DMI entries skips z: its two terms cancel.
+  const int ndir = u.zflat ? 4 : 6;
+  bool ok[6];
+  float3 mn[6];
+  if (u.has_ex || u.has_dmi) {
+    int nbi[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      if (dir < ndir) {
+        const int nb = neighbour(d, x, y, z, dir);
+        ok[dir] = nb >= 0 && (!d.geom || d.geom[nb]);
+        nbi[dir] = nb >= 0 ? nb : idx;
+      } else {
+        ok[dir] = false;
+        nbi[dir] = idx;
+      }
+    }
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) mn[dir] = dir < ndir ? load3(m, nbi[dir], N) : mc;
+  } else {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      ok[dir] = false;
+      mn[dir] = mc;
+    }
+  }
+  const float3 hs = d.hstray ? load3(d.hstray, idx, d.ncell) : f3(0.f, 0.f, 0.f);
+  const float3 hd = u.demag ? hdem(idx) : f3(0.f, 0.f, 0.f);
+  const float3 r = uniform_result(u, mode, mc, mn, ok, ndir, d.hstray != nullptr, hs, hd);
   out[idx] = r.x;
   out[N + idx] = r.y;
   out[2 * N + idx] = r.z;

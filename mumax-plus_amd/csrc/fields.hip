@@ -15,6 +15,7 @@
 // branch by branch.
 #include "fields.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 #include "field_cell.hpp"
@@ -464,6 +465,110 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
   }
 }
 
+// LDS halo-tile variant of the uniform path, for a magnet with no geometry
+// mask and no periodic wrap.  A block owns a TILE_X x TILE_Y column of cells
+// and marches it through `zc` consecutive z planes.  Per plane, the block's
+// centre values and a one-cell halo ring go into LDS (double-buffered, one
+// barrier per plane), the x and y neighbours are LDS reads at immediate
+// offsets, and the z neighbours are the thread's own centre values of the
+// previous and next planes, kept in registers.  Each plane's centre is thus
+// read from global memory once (plus the two planes beyond the chunk), and
+// the next plane's loads are in flight while the current plane is computed.
+// The arithmetic is uniform_result's, so the field equals k_field_uniform's
+// bit for bit.
+constexpr int TILE_X = 64, TILE_Y = 4;
+constexpr int TILE_PX = TILE_X + 2;  // LDS row pitch (halo column on each side)
+
+template <bool NEXT>
+__global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUniform u, const float* __restrict__ m,
+                                                                 const float* __restrict__ hdemag,
+                                                                 float* __restrict__ out, int mode, NextStage next,
+                                                                 int zc) {
+  __shared__ float s[2][3][(TILE_Y + 2) * TILE_PX];
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int tile = xcd_remap(blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z), nbx * nby * gridDim.z);
+  const int bxi = tile % nbx, rest = tile / nbx;
+  const int byi = rest % nby, bzi = rest / nby;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int nx = d.nx, ny = d.ny, nz = d.nz, N = d.ncell, P = nx * ny;
+  const int x0 = bxi * TILE_X, y0 = byi * TILE_Y;
+  const int x = x0 + tx, y = y0 + ty;
+  const int z0 = bzi * zc, z1 = min(z0 + zc, nz);
+  const bool inb = x < nx && y < ny;
+  const int col = y * nx + x;  // index within a plane
+  // Halo cell of this thread: rows y0-1 (ty 0) and y0+TILE_Y (ty 1) over the
+  // tile's x range, columns x0-1 and x0+TILE_X (ty 2, tx < 2 TILE_Y) over its
+  // y range; ty 3 has none.  hl: LDS slot, hc: plane index, hv: inside grid.
+  int hl = -1, hc = 0;
+  bool hv = false;
+  if (ty < 2) {
+    const int hy = ty == 0 ? y0 - 1 : y0 + TILE_Y;
+    hl = (ty == 0 ? 0 : TILE_Y + 1) * TILE_PX + tx + 1;
+    hv = x < nx && hy >= 0 && hy < ny;
+    hc = hy * nx + x;
+  } else if (ty == 2 && tx < 2 * TILE_Y) {
+    const int r = tx % TILE_Y, hx = tx < TILE_Y ? x0 - 1 : x0 + TILE_X, hy = y0 + r;
+    hl = (r + 1) * TILE_PX + (tx < TILE_Y ? 0 : TILE_X + 1);
+    hv = hx >= 0 && hx < nx && hy < ny;
+    hc = hy * nx + hx;
+  }
+  const int cl = (ty + 1) * TILE_PX + tx + 1;  // LDS slot of the thread's cell
+  auto ld = [&](int plane, int c) -> float3 { return load3(m, plane * P + c, N); };
+  auto put = [&](int b, int slot, float3 v) {
+    s[b][0][slot] = v.x;
+    s[b][1][slot] = v.y;
+    s[b][2][slot] = v.z;
+  };
+  const float3 zero = f3(0.f, 0.f, 0.f);
+  // planes z0 - 1 and z0 into registers, plane z0 into LDS buffer 0
+  float3 mzm = (inb && z0 > 0) ? ld(z0 - 1, col) : zero;
+  float3 mc = inb ? ld(z0, col) : zero;
+  put(0, cl, mc);
+  if (hl >= 0) put(0, hl, hv ? ld(z0, hc) : zero);
+  __syncthreads();
+  const int ndir = u.zflat ? 4 : 6;
+  const bool xm_ok = x > 0, xp_ok = x + 1 < nx, ym_ok = y > 0, yp_ok = y + 1 < ny;
+  for (int z = z0; z < z1; ++z) {
+    const int b = (z - z0) & 1;
+    // next plane's centre (the +z neighbour) and halo cell
+    const bool more = z + 1 < nz;
+    const float3 mzp = (inb && more) ? ld(z + 1, col) : zero;
+    const float3 hnext = (hl >= 0 && hv && z + 1 < z1) ? ld(z + 1, hc) : zero;
+    if (inb) {
+      const int idx = z * P + col;
+      const float3 hs = d.hstray ? load3(d.hstray, idx, N) : zero;
+      const float3 hd = u.demag ? load3(hdemag, idx, N) : zero;
+      float3 mn[6];
+      bool ok[6];
+      auto at = [&](int slot) { return f3(s[b][0][slot], s[b][1][slot], s[b][2][slot]); };
+      mn[0] = at(cl - 1);
+      mn[1] = at(cl + 1);
+      mn[2] = at(cl - TILE_PX);
+      mn[3] = at(cl + TILE_PX);
+      mn[4] = mzm;
+      mn[5] = mzp;
+      ok[0] = xm_ok;
+      ok[1] = xp_ok;
+      ok[2] = ym_ok;
+      ok[3] = yp_ok;
+      ok[4] = z > 0;
+      ok[5] = more;
+      const float3 r = uniform_result(u, mode, mc, mn, ok, ndir, d.hstray != nullptr, hs, hd);
+      out[idx] = r.x;
+      out[N + idx] = r.y;
+      out[2 * N + idx] = r.z;
+      if constexpr (NEXT) next(idx, mc, r, true);
+    }
+    if (z + 1 < z1) {
+      put(b ^ 1, cl, mzp);
+      if (hl >= 0) put(b ^ 1, hl, hnext);
+      __syncthreads();
+    }
+    mzm = mc;
+    mc = mzp;
+  }
+}
+
 // A member of a batched launch: its description, coefficients, first cell
 // in the packed arrays and first block of the launch.
 struct FBInst {
@@ -670,9 +775,28 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
 }
 
 namespace {
+// The halo-tile kernel covers magnets without a geometry mask or periodic
+// wrap that are at least one tile wide, unless the world switched it off.
+bool tile_eligible(const FMDesc& d) {
+  return d.tile_zc > 0 && !d.geom && d.mgx == 0 && d.mgy == 0 && d.mgz == 0 && d.nx >= TILE_X && d.ny >= TILE_Y;
+}
+
 void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
                           const NextStage* next, hipStream_t s) {
   const FMUniform u = make_uniform(d);
+  if (tile_eligible(d)) {
+    const int zc = std::min(d.tile_zc, d.nz);
+    dim3 grid((d.nx + TILE_X - 1) / TILE_X, (d.ny + TILE_Y - 1) / TILE_Y, (d.nz + zc - 1) / zc);
+    if (next) {
+      k_field_tile<true><<<grid, dim3(TILE_X, TILE_Y, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, *next, zc);
+      checkLaunch("k_field_tile<next>");
+    } else {
+      k_field_tile<false><<<grid, dim3(TILE_X, TILE_Y, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, NextStage{},
+                                                                   zc);
+      checkLaunch("k_field_tile");
+    }
+    return;
+  }
   int bx = 1;
   while (bx < d.nx && bx < MPA_FIELD_TILE_X) bx <<= 1;
   const int by = BLOCK / bx;

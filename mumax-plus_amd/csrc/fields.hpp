@@ -77,6 +77,9 @@ struct FMDesc {
   float noise_pref, noise_scale;
   unsigned noise_seed, noise_purpose;
   unsigned long long noise_draw;
+  // uniform path: z planes per block of the LDS halo-tile kernel, 0 for the
+  // one-cell-per-thread kernel
+  int tile_zc;
 };
 
 enum FieldMode {

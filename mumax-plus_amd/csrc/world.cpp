@@ -714,6 +714,7 @@ FMDesc Ferromagnet::desc(unsigned terms) {
   for (int c = 0; c < 3; ++c) d.wb[c] = (float)world->bias[c];
   d.openbc = openbc ? 1 : 0;
   d.terms = terms;
+  d.tile_zc = world->field_zchunk;
   d.hstray = nullptr;
   if ((terms & TERM_EXT) && owner()->stray_active()) {
     MagnetCore* own = owner();

@@ -334,7 +334,9 @@ class TimeSolver {
   double sensible_factor = 0.01, sensible_default = 1e-14;
   std::string method = "Fehlberg";
   long long nsteps = 0, nrejected = 0, nrhs = 0;
-  bool fuse_stages = true;  // stage combinations inside the field launch where possible
+  // stage combinations inside the field launch where possible; off by
+  // default: slower on MI355X at 256x256x64 (2.65 against 2.56 ms per step)
+  bool fuse_stages = false;
 
   void set_method(const std::string& name);
   double sensible_timestep();
@@ -370,6 +372,9 @@ class World {
   Reducer reducer;
   Profiler prof;
   unsigned long long version = 1;  // bumped by every state change (invalidates FSAL reuse)
+  // Uniform-parameter field kernel: z planes per block of the LDS halo-tile
+  // variant, 0 for the one-cell-per-thread variant.
+  int field_zchunk = 4;
 
   // regions: one index per cell, or nullptr (mumaxworld.cpp:57-103)
   Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry,

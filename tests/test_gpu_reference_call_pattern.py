@@ -203,7 +203,9 @@ def test_antiferromagnet_factories():
     assert np.allclose(_cpp.full_magnetization(afm).eval(), 5e5 * m1 + 3e5 * m2, rtol=1e-6, atol=1e-1)
     assert np.allclose(_cpp.full_magnetization(afm.sub1()).eval(), 5e5 * m1, rtol=1e-6, atol=1e-1)
     ang = _cpp.angle_field(afm).eval()[0]
-    want = np.arccos(np.clip(np.sum(m1 * m2, axis=0), -1, 1))
+    # acos(sign(afmex_cell) m1 . m2): the angle from antiparallel for a
+    # negative homogeneous exchange (afmexchange.cu:280-282)
+    want = np.arccos(np.clip(-np.sum(m1 * m2, axis=0), -1, 1))
     assert np.abs(ang - want).max() < 1e-3
     assert _cpp.max_intracell_angle(afm).eval() == pytest.approx(want.max(), abs=1e-3)
     parts = _cpp.total_energy(afm.sub1()).eval() + _cpp.total_energy(afm.sub2()).eval()
