@@ -327,6 +327,15 @@ int mpa_demag_plan_destroy(mpa_demag_plan* p);
  * stream: hipStream_t or NULL. */
 int mpa_demag_plan_exec(mpa_demag_plan* p, const float* m, const float* msat_cells, float msat_uniform,
                         const uint8_t* geometry, float* h, void* stream);
+/* No counterpart in the reference: mpa_demag_plan_exec for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result; the FFT intermediates live in a static 64 MiB device scratch array
+ * that each GPU holds until the process ends, instead of the plan's own buffers (which this call
+ * never allocates).  A plan whose intermediates exceed 64 MiB (3 Nz Hx Ny plus, for Nz > 1,
+ * 3 Hx Ky Nz complex FP32 values, Hx = Kx/2+1) is rejected with MPA_ERR_INVALID.  Calls share
+ * one scratch per GPU: calls that overlap on one GPU (other streams or threads) race. */
+int mpa_demag_plan_exec_noalloc(mpa_demag_plan* p, const float* m, const float* msat_cells, float msat_uniform,
+                                const uint8_t* geometry, float* h, void* stream);
 /* FFT lengths, algorithmic HBM bytes per exec and device bytes held by the plan. */
 int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algorithmic_bytes,
                         double* device_bytes);

@@ -1136,6 +1136,16 @@ int mpa_demag_plan_exec(mpa_demag_plan* p, const float* m, const float* msat_cel
   });
 }
 
+int mpa_demag_plan_exec_noalloc(mpa_demag_plan* p, const float* m, const float* msat_cells, float msat_uniform,
+                                const uint8_t* geometry, float* h, void* stream) {
+  return guard([&] {
+    need(p, "plan");
+    need(m, "m");
+    need(h, "h");
+    p->plan.exec_noalloc(m, PS{msat_cells, msat_uniform}, geometry, h, (hipStream_t)stream);
+  });
+}
+
 int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algorithmic_bytes, double* device_bytes) {
   return guard([&] {
     need(p, "plan");

@@ -37,6 +37,8 @@
 #include "newell.hpp"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -913,8 +915,6 @@ DemagPlan::DemagPlan(const DemagGeometry& g) : g_(g) {
   Hx_ = K_[0] == 1 ? 1 : K_[0] / 2 + 1;
   Hy_ = K_[1] / 2 + 1;
   Hz_ = K_[2] / 2 + 1;
-  A_.allocate((size_t)3 * g.n[2] * Hx_ * g.n[1]);
-  if (!is2d()) B_.allocate((size_t)3 * Hx_ * K_[1] * g.n[2]);
   for (int a = 0; a < 3; ++a) make_twiddles(tw_[a], log2K_[a]);
   build_kernel(nullptr);
 }
@@ -952,8 +952,68 @@ void DemagPlan::build_kernel(hipStream_t s) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+size_t DemagPlan::a_count() const { return (size_t)3 * g_.n[2] * Hx_ * g_.n[1]; }
+size_t DemagPlan::b_count() const { return is2d() ? 0 : (size_t)3 * Hx_ * K_[1] * g_.n[2]; }
+
+size_t DemagPlan::workspace_bytes() const {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  return up(a_count() * sizeof(float2)) + up(b_count() * sizeof(float2));
+}
+
+void DemagPlan::ensure_workspace() {
+  if (A_.n == 0) A_.allocate(a_count());
+  if (!is2d() && B_.n == 0) B_.allocate(b_count());
+}
+
 void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof,
                      const float* m2, PS msat2) {
+  ensure_workspace();
+  run(A_.get(), B_.get(), m, msat, geom, h, s, prof, m2, msat2);
+}
+
+// Scratch of the allocation-free exec: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// (on the host it names a stand-in, not device memory): the device address
+// comes from hipGetSymbolAddress, looked up once per device on its first use
+// with that device current.
+namespace {
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+void* scratch_address() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw Error(MPA_ERR_RUNTIME, "demag scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(MPA_ERR_RUNTIME, "demag scratch: no device address");
+    scratch_ptr[dev] = p;
+  }
+  return scratch_ptr[dev];
+}
+}  // namespace
+
+size_t DemagPlan::scratch_capacity() { return SCRATCH_BYTES; }
+
+void DemagPlan::exec_noalloc(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s) {
+  const size_t need = workspace_bytes();
+  if (need > SCRATCH_BYTES)
+    throw Error(MPA_ERR_INVALID, "demag: this grid needs " + std::to_string(need >> 20) +
+                                     " MiB of FFT scratch; the allocation-free exec holds " +
+                                     std::to_string(SCRATCH_BYTES >> 20) + " MiB");
+  unsigned char* base = static_cast<unsigned char*>(scratch_address());
+  float2* A = reinterpret_cast<float2*>(base);
+  float2* B = is2d() ? nullptr : reinterpret_cast<float2*>(base + (a_count() * sizeof(float2) + 255) / 256 * 256);
+  run(A, B, m, msat, geom, h, s, nullptr, nullptr, PS{nullptr, 0.f});
+}
+
+void DemagPlan::run(float2* A, float2* B, const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s,
+                    Profiler* prof, const float* m2, PS msat2) {
   DD d;
   d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
   d.Kx = K_[0]; d.Ky = K_[1]; d.Kz = K_[2];
@@ -964,28 +1024,28 @@ void DemagPlan::exec(const float* m, PS msat, const uint8_t* geom, float* h, hip
   {
     // the second sublattice's magnetisation is one more read of 12 B per cell
     Timed t(prof, "k_xfwd", pass_bytes(0) + (m2 ? 12.0 * d.nx * d.ny * d.nz : 0.0), s);
-    with_line<RunXfwd>(d.l2x, d, m, msat, m2, msat2, geom, A_.get(), tw_[0].get(), s);
+    with_line<RunXfwd>(d.l2x, d, m, msat, m2, msat2, geom, A, tw_[0].get(), s);
   }
   if (is2d()) {
     Timed t(prof, "k_ypass2d", pass_bytes(1), s);
-    with_line<RunYpass2d>(d.l2y, d, A_.get(), T_.get(), tw_[1].get(), s);
+    with_line<RunYpass2d>(d.l2y, d, A, T_.get(), tw_[1].get(), s);
   } else {
     {
       Timed t(prof, "k_yfwd", pass_bytes(1), s);
-      with_line<RunYfwd>(d.l2y, d, A_.get(), B_.get(), tw_[1].get(), s);
+      with_line<RunYfwd>(d.l2y, d, A, B, tw_[1].get(), s);
     }
     {
       Timed t(prof, "k_zpass", pass_bytes(2), s);
-      with_line<RunZpass>(d.l2z, d, B_.get(), T_.get(), tw_[2].get(), s);
+      with_line<RunZpass>(d.l2z, d, B, T_.get(), tw_[2].get(), s);
     }
     {
       Timed t(prof, "k_yinv", pass_bytes(3), s);
-      with_line<RunYinv>(d.l2y, d, B_.get(), A_.get(), tw_[1].get(), s);
+      with_line<RunYinv>(d.l2y, d, B, A, tw_[1].get(), s);
     }
   }
   {
     Timed t(prof, "k_xinv", pass_bytes(npasses() - 1), s);
-    with_line<RunXinv>(d.l2x, d, A_.get(), geom, h, tw_[0].get(), s);
+    with_line<RunXinv>(d.l2x, d, A, geom, h, tw_[0].get(), s);
   }
 }
 
@@ -1030,6 +1090,7 @@ DemagBatch::DemagBatch(const std::vector<Member>& members, long long cs) {
         size_t shm = 0;
         with_line<BatchShape>(l2_of(pass, j), pass, dd[j], &nb, &shm);
         shm_max = shm > shm_max ? shm : shm_max;
+        members[j].plan->ensure_workspace();
         const DemagPlan& p = *members[j].plan;
         BSeg sg{};
         sg.d = dd[j];

@@ -31,8 +31,21 @@ class DemagPlan {
   // prof (optional) times each pass with HIP events.
   // With m2 != nullptr the source is Ms m + Ms2 m2 (the two sublattices of an
   // antiferromagnet, strayfieldfft.cu:206-213).
+  // The FFT intermediates A and B are the plan's own, allocated on the
+  // first exec.
   void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof = nullptr,
             const float* m2 = nullptr, PS msat2 = PS{nullptr, 0.f});
+  // The same transform with A and B in the library's static device scratch
+  // array instead (scratch_capacity() bytes per device, shared by every call
+  // on that device): allocates nothing.  Throws when workspace_bytes()
+  // exceeds the scratch.  Concurrent calls on one device race on the scratch.
+  void exec_noalloc(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s);
+  static size_t scratch_capacity();
+  // Device bytes of the intermediates A (3 Nz Hx Ny complex) and, for a 3-D
+  // grid, B (3 Hx Ky Nz complex), each 256-byte aligned.
+  size_t workspace_bytes() const;
+  // Allocates A and B if the plan does not hold them yet.
+  void ensure_workspace();
   // Pass names and algorithmic HBM bytes per pass.
   int npasses() const { return is2d() ? 3 : 5; }
   const char* pass_name(int i) const;
@@ -52,6 +65,10 @@ class DemagPlan {
  private:
   friend class DemagBatch;
   void build_kernel(hipStream_t s);
+  void run(float2* A, float2* B, const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s,
+           Profiler* prof, const float* m2, PS msat2);
+  size_t a_count() const;
+  size_t b_count() const;
   DemagGeometry g_;
   int N_ = 0;
   int K_[3];

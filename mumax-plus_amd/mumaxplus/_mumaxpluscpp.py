@@ -158,6 +158,7 @@ _SIGS = {
     "mpa_demag_plan_create": ([_P, _P, _P, _P, _ct.POINTER(_P)], _cint),
     "mpa_demag_plan_destroy": ([_P], _cint),
     "mpa_demag_plan_exec": ([_P, _P, _P, _ct.c_float, _P, _P, _P], _cint),
+    "mpa_demag_plan_exec_noalloc": ([_P, _P, _P, _ct.c_float, _P, _P, _P], _cint),
     "mpa_demag_plan_info": ([_P, _P, _ct.POINTER(_cdouble), _ct.POINTER(_cdouble)], _cint),
     "mpa_world_set_profiling": ([_P, _cint], _cint),
     "mpa_world_set_option": ([_P, _c_char_p, _cint], _cint),
@@ -1547,6 +1548,11 @@ class DemagPlan:
 
     def exec(self, m_ptr, h_ptr, msat_uniform=1.0, msat_ptr=None, geom_ptr=None, stream=None):
         _check(_lib.mpa_demag_plan_exec(self._h, m_ptr, msat_ptr, float(msat_uniform), geom_ptr, h_ptr, stream))
+
+    def exec_noalloc(self, m_ptr, h_ptr, msat_uniform=1.0, msat_ptr=None, geom_ptr=None, stream=None):
+        """exec without any device allocation (the library's static scratch)."""
+        _check(_lib.mpa_demag_plan_exec_noalloc(self._h, m_ptr, msat_ptr, float(msat_uniform), geom_ptr, h_ptr,
+                                                stream))
 
     def info(self):
         ks = _i3()
