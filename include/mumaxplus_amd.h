@@ -352,7 +352,10 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
  * only variable is one magnetisation on the uniform-parameter LLG path, stepped with a
  * first-same-as-last tableau, computes each stage state inside the field launch that produces
  * the last derivative it needs (no separate combination launch); 0 runs the separate
- * combination, with the same arithmetic.  "field_zchunk" (default 4): z planes per workgroup of the
+ * combination, with the same arithmetic.  "fuse_combine" (default 0): when the only variable is one
+ * independent magnetisation with demag, each stage state is formed inside the demag's first FFT
+ * pass, which reads the step start and the derivatives instead of the state (no separate
+ * combination launch); 0 runs the separate combination.  "field_zchunk" (default 4): z planes per workgroup of the
  * uniform-parameter field kernel's LDS halo-tile variant (magnets with no geometry mask or periodic
  * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant.  Both give the
  * same values bit for bit. */

@@ -35,6 +35,7 @@
 #include "demag.hpp"
 #include "fft.hpp"
 #include "newell.hpp"
+#include "rk.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -107,16 +108,22 @@ struct Line {
   static constexpr int E = L2 >= 3 ? 8 : 1;   // values per thread
   static constexpr int EH = L2 >= 3 ? 4 : 1;  // values covering the first L/2 points
   static constexpr int H = L2 >= 3 ? L / 2 + 1 : 1;
-  // LDS stride of one line: room for lpad(L-1), rounded up to `target`
-  // modulo 32 float2 so that the lines a half-wave touches at once fall on
-  // different banks (target < 0: no rounding).
+  // LDS stride of one line: room for every pad of the transform and of the
+  // staging (at most L + L/8 - 2 slots), rounded up to `target` modulo 32
+  // float2 (target < 0: no rounding).
   static constexpr int stride(int target) {
     const int base = L2 >= 3 ? L + L / 8 - 1 : 1;
     return target < 0 ? base : base + (((target - base) % 32) + 32) % 32;
   }
-  // bank target for lines used only by the transform: T lines of T < 32
-  // threads share a half-wave
-  static constexpr int fft_target() { return (T >= 8 && T < 32) ? T : -1; }
+  // Line stride for the transform's own exchanges: lines of fewer than 16
+  // threads share a group of 16 lanes, and these residues modulo 32 keep
+  // their stores and loads on different bank pairs (2 and 3 leave one 2-way
+  // conflict per exchange); longer lines need none.
+  static constexpr int fft_target() { return L2 == 4 ? 2 : (L2 == 5 ? 3 : (L2 == 6 ? 8 : -1)); }
+  // Line stride when a block also reads its lines across (one element of
+  // each of up to 16 lines per lane group): lines of 16 threads or more
+  // take `across`, shorter ones the transform's own residue.
+  static constexpr int target(int across) { return T >= 16 ? across : fft_target(); }
 };
 
 // ---------------------------------------------------------------- x forward
@@ -127,7 +134,8 @@ struct XfwdCfg {
   static constexpr int PB = clampc(MPA_XFWD_THREADS / C::T, 1, 32);  // line pairs per block
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
-  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (PB < 16 ? PB : 16));
+  // the split reads one element of PB lines per lane group: odd stride
+  static constexpr int LP = C::stride(C::target(1));
   static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
   static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xfwd LDS");
 };
@@ -140,10 +148,18 @@ struct XfwdCfgB {
   static constexpr int PB = 256 / C::T;
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
-  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (PB < 16 ? PB : 16));
+  static constexpr int LP = C::stride(C::target(1));
   static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
   static_assert(THREADS == 256 && SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "batched k_xfwd");
 };
+
+// Slot of element i of a staged line (the stores after a transform and the
+// loads before one go to consecutive elements): the pad of the transform's
+// first exchange with one slot per group.
+template <int L2>
+__device__ __forceinline__ int iopad(int i) {
+  return padsm<fft_pad_shift(L2 < 3 ? 3 : L2, 0), 1>(i);
+}
 
 // Offset of x-line l (c, z, y) in A[c][z][kx][y] at kx = 0; -1 past the end.
 __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
@@ -187,7 +203,7 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
   float2* line = lds + g * K::LP;
   line_fft<L2, false, 1>(v, t, line, 0, tw);
 #pragma unroll
-  for (int q = 0; q < C::E; ++q) line[lpad(t + C::T * q)] = v[0][q];
+  for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
   __syncthreads();
   constexpr int L2NL = K::L2PB + 1;
   for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
@@ -195,8 +211,8 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
     const long long base = abase[li];
     if (base < 0) continue;
     const float2* pl = lds + (li >> 1) * K::LP;
-    const float2 z1 = pl[lpad(kx)];
-    const float2 z2 = pl[lpad((C::L - kx) & (C::L - 1))];
+    const float2 z1 = pl[iopad<L2>(kx)];
+    const float2 z2 = pl[iopad<L2>((C::L - kx) & (C::L - 1))];
     float2 X;
     if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
     else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
@@ -211,6 +227,97 @@ __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
   xfwd_body<L2>(d, m, msat, m2, msat2, geom, A, tw, blockIdx.x);
 }
 
+// x forward with the Runge-Kutta stage combination in front.  The rows of
+// the magnet are r = z ny + y; thread group g of block blk takes rows
+// ra = 2 (PB blk + g) and ra + 1.  For its cells it forms the stage state
+// x = normalise(x0 + sum_j c_j k_j), zero outside the geometry (the
+// arithmetic of k_combine_batch), writes it as the new state and keeps it in
+// registers; then it transforms Ms x one component at a time (two real rows
+// per complex transform, as k_xfwd) and the block stores each component's
+// half spectra into A.  The state is thus read once for both the combination
+// and the transform.
+template <int L2, class K = XfwdCfg<L2>>
+__device__ __forceinline__ void xfwd_combine_body(const DD& d, const float* __restrict__ x0, const Combo& co,
+                                                  float* __restrict__ xout, PS msat, const uint8_t* __restrict__ geom,
+                                                  float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
+  using C = Line<L2>;
+  extern __shared__ float2 lds[];
+  __shared__ long long abase[2 * K::PB];
+  const int nrows = d.ny * d.nz;
+  const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
+  const int r0 = 2 * K::PB * blk;
+  for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) {
+    const int r = r0 + i;
+    abase[i] = r < nrows ? (long long)(r / d.ny) * d.Hx * d.ny + (r % d.ny) : -1;
+  }
+  const size_t N = (size_t)d.cs;
+  const int ra = r0 + 2 * g;
+  // stage state of the group's cells: s[row][c][q]
+  float sv[2][3][C::EH];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int row = ra + h;
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t + C::T * q;
+      float v3[3] = {0.f, 0.f, 0.f};
+      if (row < nrows && x < d.nx) {
+        const size_t idx = (size_t)row * d.nx + x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float a = x0[c * N + idx];
+          for (int j = 0; j < co.nk; ++j) a += co.c[j] * co.k[j][c * N + idx];
+          v3[c] = a;
+        }
+        const float inv = rsqrtf(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v3[c] *= inv;
+        if (geom && !geom[idx]) v3[0] = v3[1] = v3[2] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xout[c * N + idx] = v3[c];
+        const float ms = msat.at(idx);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v3[c] *= ms;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sv[h][c][q] = v3[c];
+    }
+  }
+  float2* line = lds + g * K::LP;
+  const long long cstride = (long long)d.nz * d.Hx * d.ny;  // component stride of A
+  constexpr int L2NL = K::L2PB + 1;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float2 v[1][8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[0][q] = q < C::EH ? make_float2(sv[0][c][q], sv[1][c][q]) : make_float2(0.f, 0.f);
+    line_fft<L2, false, 1>(v, t, line, 0, tw);
+#pragma unroll
+    for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
+    __syncthreads();
+    for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
+      const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
+      const long long base = abase[li];
+      if (base < 0) continue;
+      const float2* pl = lds + (li >> 1) * K::LP;
+      const float2 z1 = pl[iopad<L2>(kx)];
+      const float2 z2 = pl[iopad<L2>((C::L - kx) & (C::L - 1))];
+      float2 X;
+      if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+      else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+      A[c * cstride + base + (long long)kx * d.ny] = X;
+    }
+    __syncthreads();
+  }
+}
+
+template <int L2>
+__global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
+    k_xfwd_combine(DD d, const float* __restrict__ x0, Combo co, float* __restrict__ xout, PS msat,
+                   const uint8_t* __restrict__ geom, float2* __restrict__ A, const float2* __restrict__ tw) {
+  xfwd_combine_body<L2>(d, x0, co, xout, msat, geom, A, tw, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- y forward / inverse (3-D)
 
 template <int L2>
@@ -221,7 +328,9 @@ struct YCfg {
   static constexpr int LB = C::T >= 64 ? clampc(1024 / C::T, 1, 16) : clampc(512 / C::T, 1, 32);
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
-  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (LB < 32 ? LB : 32));
+  // the transposed store reads one element of each of 16 lines per lane
+  // group: odd stride
+  static constexpr int LP = C::stride(C::target(1));
   static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
   static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
 };
@@ -233,7 +342,9 @@ struct YinvCfg {
   static constexpr int LB = clampc(MPA_YINV_THREADS / C::T, 1, 32);
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
-  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / LB);
+  // the staging writes 8 lines x 2 consecutive elements per lane group:
+  // stride 2 modulo 16
+  static constexpr int LP = C::stride(C::target(2));
   static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
   static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yinv LDS");
 };
@@ -244,7 +355,7 @@ struct YCfgB {
   static constexpr int LB = 256 / C::T;
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
-  static constexpr int LP = C::stride(C::T < 32 ? C::fft_target() : 32 / (LB < 32 ? LB : 32));
+  static constexpr int LP = C::stride(C::target(1));
   static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
   static_assert(THREADS == 256 && SHM + LB * sizeof(long long) <= LDS_LIMIT, "batched k_yfwd");
 };
@@ -288,13 +399,13 @@ __device__ __forceinline__ void yfwd_body(const DD& d, const float2* __restrict_
   float2* line = lds + g * K::LP;
   line_fft<L2, false, 1>(v, t, line, 0, tw);
 #pragma unroll
-  for (int q = 0; q < C::E; ++q) line[lpad(t + C::T * q)] = v[0][q];
+  for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
   __syncthreads();
   for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
     const int ky = e >> K::L2LB, li = e & (K::LB - 1);
     const long long base = bbase[li];
     if (base < 0) continue;
-    B[base + (long long)ky * d.nz] = lds[li * K::LP + lpad(ky)];
+    B[base + (long long)ky * d.nz] = lds[li * K::LP + iopad<L2>(ky)];
   }
 }
 
@@ -317,14 +428,14 @@ __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict_
   for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
     const int ky = e >> K::L2LB, li = e & (K::LB - 1);
     const long long base = bbase[li];
-    lds[li * K::LP + lpad(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
+    lds[li * K::LP + iopad<L2>(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   float2* line = lds + g * K::LP;
   float2 v[1][8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[0][q] = q < C::E ? line[lpad(t + C::T * q)] : make_float2(0.f, 0.f);
+  for (int q = 0; q < 8; ++q) v[0][q] = q < C::E ? line[iopad<L2>(t + C::T * q)] : make_float2(0.f, 0.f);
   __syncthreads();
   line_fft<L2, true, 1>(v, t, line, 0, tw);
   const int ql = q0 + g;
@@ -520,7 +631,7 @@ struct XinvCfg {
   static constexpr int PB = clampc(MPA_XINV_THREADS / C::T, 1, 32);
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
-  static constexpr int XS = C::H + 1;  // stride of a staged half spectrum
+  static constexpr int XS = C::H;  // stride of a staged half spectrum (odd: no bank conflicts)
   static constexpr int LP = C::stride(C::fft_target());
   // the staged half spectra and the transform buffers share the LDS
   static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
@@ -534,7 +645,7 @@ struct XinvCfgB {
   static constexpr int PB = 256 / C::T;
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
-  static constexpr int XS = C::H + 1;
+  static constexpr int XS = C::H;
   static constexpr int LP = C::stride(C::fft_target());
   static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
   static constexpr size_t SHM = (XB > ZB ? XB : ZB) * sizeof(float2);
@@ -709,6 +820,17 @@ struct RunXfwd {
     const int npairs = (d.nlines + 1) / 2;
     k_xfwd<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
     checkLaunch("k_xfwd");
+  }
+};
+
+template <int L2>
+struct RunXfwdCombine {
+  static void run(const DD& d, const float* x0, const Combo& co, float* xout, PS msat, const uint8_t* geom,
+                  float2* A, const float2* tw, hipStream_t s) {
+    using K = XfwdCfg<L2>;
+    const int npairs = (d.ny * d.nz + 1) / 2;
+    k_xfwd_combine<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, x0, co, xout, msat, geom, A, tw);
+    checkLaunch("k_xfwd_combine");
   }
 };
 
@@ -1012,8 +1134,19 @@ void DemagPlan::exec_noalloc(const float* m, PS msat, const uint8_t* geom, float
   run(A, B, m, msat, geom, h, s, nullptr, nullptr, PS{nullptr, 0.f});
 }
 
+void DemagPlan::exec_combine(const float* x0, const Combo& co, float* x, PS msat, const uint8_t* geom, float* h,
+                             hipStream_t s, Profiler* prof) {
+  if (co.nk < 0 || co.nk > MAX_STAGES || !x0 || !x || x == x0)
+    throw Error(MPA_ERR_RUNTIME, "demag: bad stage combination");
+  for (int j = 0; j < co.nk; ++j)
+    if (co.k[j] == x) throw Error(MPA_ERR_RUNTIME, "demag: stage combination writes a derivative it reads");
+  ensure_workspace();
+  const StageIn in{x0, &co, x};
+  run(A_.get(), B_.get(), x, msat, geom, h, s, prof, nullptr, PS{nullptr, 0.f}, &in);
+}
+
 void DemagPlan::run(float2* A, float2* B, const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s,
-                    Profiler* prof, const float* m2, PS msat2) {
+                    Profiler* prof, const float* m2, PS msat2, const StageIn* in) {
   DD d;
   d.nx = g_.n[0]; d.ny = g_.n[1]; d.nz = g_.n[2];
   d.Kx = K_[0]; d.Ky = K_[1]; d.Kz = K_[2];
@@ -1021,7 +1154,13 @@ void DemagPlan::run(float2* A, float2* B, const float* m, PS msat, const uint8_t
   d.l2x = log2K_[0]; d.l2y = log2K_[1]; d.l2z = log2K_[2];
   d.nlines = 3 * d.ny * d.nz;
   d.cs = (long long)d.nx * d.ny * d.nz;
-  {
+  if (in) {
+    // read x0 and nk derivatives, write the state (12 B per cell each) instead
+    // of reading it
+    const double n = (double)d.nx * d.ny * d.nz;
+    Timed t(prof, "k_xfwd_combine", pass_bytes(0) + 12.0 * n * (in->co->nk + 1), s);
+    with_line<RunXfwdCombine>(d.l2x, d, in->x0, *in->co, in->x, msat, geom, A, tw_[0].get(), s);
+  } else {
     // the second sublattice's magnetisation is one more read of 12 B per cell
     Timed t(prof, "k_xfwd", pass_bytes(0) + (m2 ? 12.0 * d.nx * d.ny * d.nz : 0.0), s);
     with_line<RunXfwd>(d.l2x, d, m, msat, m2, msat2, geom, A, tw_[0].get(), s);

@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "profiler.hpp"
+#include "rk.hpp"
 
 namespace mpa {
 
@@ -35,6 +36,11 @@ class DemagPlan {
   // first exec.
   void exec(const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s, Profiler* prof = nullptr,
             const float* m2 = nullptr, PS msat2 = PS{nullptr, 0.f});
+  // exec of the Runge-Kutta stage state formed in the first pass:
+  // x = normalise(x0 + sum_j co.c[j] co.k[j]), zero outside geom, is written
+  // to x (3N floats, not aliasing x0 or any co.k[j]) and its field to h.
+  void exec_combine(const float* x0, const Combo& co, float* x, PS msat, const uint8_t* geom, float* h,
+                    hipStream_t s, Profiler* prof = nullptr);
   // The same transform with A and B in the library's static device scratch
   // array instead (scratch_capacity() bytes per device, shared by every call
   // on that device): allocates nothing.  Throws when workspace_bytes()
@@ -65,8 +71,13 @@ class DemagPlan {
  private:
   friend class DemagBatch;
   void build_kernel(hipStream_t s);
+  struct StageIn {
+    const float* x0;
+    const Combo* co;
+    float* x;
+  };
   void run(float2* A, float2* B, const float* m, PS msat, const uint8_t* geom, float* h, hipStream_t s,
-           Profiler* prof, const float* m2, PS msat2);
+           Profiler* prof, const float* m2, PS msat2, const StageIn* in = nullptr);
   size_t a_count() const;
   size_t b_count() const;
   DemagGeometry g_;

@@ -13,9 +13,15 @@
 // radix-4 pass when L2 is not a multiple of 3, then radix-8 passes.
 //
 // L2 is a template parameter, so every pass's radix, Ns and twiddle-table
-// offset are compile-time constants.  LDS uses one pad slot every 8
-// elements, which keeps all Stockham read and write offsets affine in the
-// per-thread base (immediate offsets in the LDS instructions).  Twiddles come
+// offset are compile-time constants.  The exchange after pass p stores
+// element i at slot i + PM (i >> PS) (fft_pad_*), an affine function of the
+// per-thread base, so every LDS access has an immediate offset.  PS and PM
+// are chosen per exchange so that, with 32 banks of 4 bytes and 8-byte
+// accesses served 16 lanes at a time, the 16 lanes of a group touch 16
+// different bank pairs both when the butterflies store (element stride 8,
+// or blocks of Ns consecutive elements) and when the next pass loads
+// (consecutive elements): no bank conflicts for lines of 128 points and
+// more.  Twiddles come
 // from a per-pass table (stage_twiddles) holding, for each k < Ns, the R-1
 // factors w^(r k) consecutively.  The transform is unnormalised; L is 1 (no
 // work) or a power of two >= 8.
@@ -39,8 +45,9 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
   return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
-// LDS index with one pad slot every 8 elements.
-__host__ __device__ constexpr int lpad(int i) { return i + (i >> 3); }
+// LDS slot i + M (i >> S): M pad slots every 2^S elements.
+template <int S, int M>
+__host__ __device__ constexpr int padsm(int i) { return i + M * (i >> S); }
 
 // Synchronisation of an exchange between the T threads of one line.  Lines
 // are aligned to wavefronts (thread t of line g is threadIdx.x = g*T + t), so
@@ -76,6 +83,16 @@ __host__ __device__ constexpr int fft_twoff(int L2, int p) {
   return o;
 }
 __host__ __device__ constexpr int fft_twsize(int L2) { return fft_twoff(L2, fft_npass(L2)); }
+
+// Pad of the exchange after pass p (slot = i + PM (i >> PS)).  Lines of 128
+// points and more: one slot per 16 elements after a pass with Ns = 1 or
+// Ns >= 16, two after Ns = 2, 4 or 8.  Shorter lines share a lane group with
+// other lines; their pads and line strides (Line::fft_target) come from the
+// same bank model.
+__host__ __device__ constexpr int fft_pad_shift(int L2, int p) { return (L2 == 4 || L2 == 6) ? 3 : 4; }
+__host__ __device__ constexpr int fft_pad_mul(int L2, int p) {
+  return (L2 >= 7 && fft_l2ns(L2, p) >= 1 && fft_l2ns(L2, p) <= 3) ? 2 : 1;
+}
 
 // Host: per-pass twiddle table of an L = 2^L2 transform (FP64 -> FP32).
 inline std::vector<float2> stage_twiddles(int L2) {
@@ -139,6 +156,7 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[NC][8], int c, int m, 
   constexpr int R = fft_radix(L2, P), M = 8 / R;
   constexpr int NS = 1 << fft_l2ns(L2, P);
   constexpr bool LAST = P == fft_npass(L2) - 1;
+  constexpr int PS = fft_pad_shift(L2, P), PM = fft_pad_mul(L2, P);
   float2 a[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) a[r] = v[c][m + r * M];
@@ -152,7 +170,7 @@ __device__ __forceinline__ void fft_butterfly(float2 (&v)[NC][8], int c, int m, 
     for (int r = 0; r < R; ++r) v[c][m + r * M] = a[r];
   } else {
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[base + r * NS + ((r * NS) >> 3)] = a[r];
+    for (int r = 0; r < R; ++r) buf[base + r * NS + PM * ((r * NS) >> PS)] = a[r];
   }
 }
 
@@ -172,6 +190,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds,
   constexpr int L2NS = fft_l2ns(L2, P), NS = 1 << L2NS;
   constexpr int TWOFF = fft_twoff(L2, P);
   constexpr bool LAST = P == fft_npass(L2) - 1;
+  constexpr int PS = fft_pad_shift(L2, P), PM = fft_pad_mul(L2, P);
   float2 w[M][R];
   int base[M];
 #pragma unroll
@@ -186,9 +205,9 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds,
         if (INV) w[m][r].y = -w[m][r].y;
       }
     }
-    base[m] = lpad(((j >> L2NS) << (L2NS + LOG2R)) + k);
+    base[m] = padsm<PS, PM>(((j >> L2NS) << (L2NS + LOG2R)) + k);
   }
-  const int rb = lpad(t);
+  const int rb = padsm<PS, PM>(t);
   if constexpr (LAST) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -203,7 +222,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds,
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[c][q] = lds[c * bstride + rb + T * q + ((T * q) >> 3)];
+      for (int q = 0; q < 8; ++q) v[c][q] = lds[c * bstride + rb + T * q + PM * ((T * q) >> PS)];
     line_sync<T>();
   } else {
 #pragma unroll
@@ -213,7 +232,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[NC][8], int t, float2* lds,
       for (int m = 0; m < M; ++m) fft_butterfly<L2, P, INV, NC>(v, c, m, w[m], base[m], buf);
       line_sync<T>();
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[c][q] = buf[rb + T * q + ((T * q) >> 3)];
+      for (int q = 0; q < 8; ++q) v[c][q] = buf[rb + T * q + PM * ((T * q) >> PS)];
       if constexpr (NBUF == 1) line_sync<T>();
     }
   }

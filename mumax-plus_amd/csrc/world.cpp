@@ -1330,9 +1330,12 @@ std::vector<StateVar> World::state_vars() {
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
 void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready,
-                         double noise_dt, const NextStage* next) {
+                         double noise_dt, const NextStage* next, const StageIn* stage_in) {
   if (next && (vars.size() != 1 || vars[0].kind != StateVar::MAG || relaxing_))
     throw Error(MPA_ERR_RUNTIME, "a fused stage combination needs a single magnetisation variable");
+  if (stage_in && (next || vars.size() != 1 || vars[0].kind != StateVar::MAG || vars[0].fm->host ||
+                   !(vars[0].fm->active_terms() & TERM_DEMAG) || vars[0].x != vars[0].fm->m.get()))
+    throw Error(MPA_ERR_RUNTIME, "a stage state formed in the demag pass needs one magnetisation with demag");
   for (auto& fm : magnets) {
     if (fm->el.enabled) fm->compute_strain(s);
     fm->compute_stray(s, &prof);
@@ -1359,7 +1362,10 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
         fm->eval_rhs(out, s, true, mode, noise_dt);
       } else {
         const unsigned t = fm->active_terms();
-        if (t & TERM_DEMAG)
+        if (stage_in)
+          fm->demag().exec_combine(stage_in->x0, stage_in->co, fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(),
+                                   fm->hdemag.get(), s, &prof);
+        else if (t & TERM_DEMAG)
           fm->demag().exec(fm->m.get(), fm->param("msat").ps(), fm->geom_ptr(), fm->hdemag.get(), s, &prof);
         fm->eval_rhs(out, s, true, mode, noise_dt, next);
       }
@@ -1673,6 +1679,11 @@ void TimeSolver::step() {
   if (fuse_stages && !noisy && tb.fsal && S > 1 && vars.size() == 1 && vars[0].kind == StateVar::MAG && !world_->relaxing() &&
       vars[0].fm->next_fusable())
     fused = vars[0].fm;
+  // One independent magnetisation with demag: each stage state is formed in
+  // the demag's first FFT pass, which reads x0 and the derivatives instead of
+  // the state (no combination launch).
+  const bool xfuse = !fused && fuse_combine && vars.size() == 1 && vars[0].kind == StateVar::MAG &&
+                     !vars[0].fm->host && vars[0].normalize && (vars[0].fm->active_terms() & TERM_DEMAG);
   bool next_ready = fused && k0_valid && next_valid_ && next_h_ == dt;
   next_valid_ = false;
   if (fused && fused->xn.n == 0) fused->xn.allocate((size_t)3 * fused->N);
@@ -1735,11 +1746,21 @@ void TimeSolver::step() {
     next_ready = false;
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
+      World::StageIn sin{};
+      const World::StageIn* stage_in = nullptr;
       if (st > 0) {
         if (have_next) {
           std::swap(fused->m, fused->xn);
           vars[0].x = fused->m.get();
           have_next = false;
+        } else if (xfuse) {
+          sin.x0 = vars[0].x0->get();
+          sin.co.nk = st;
+          for (int j = 0; j < st; ++j) {
+            sin.co.k[j] = (*vars[0].k)[j].get();
+            sin.co.c[j] = (float)(h * tb.A[st][j]);
+          }
+          stage_in = &sin;
         } else {
           combine(tb.A[st], st, h, st);
         }
@@ -1766,7 +1787,7 @@ void TimeSolver::step() {
           have_next = true;
         }
       } else {
-        world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0);
+        world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0, nullptr, stage_in);
       }
       nrhs += (long long)vars.size();
     }

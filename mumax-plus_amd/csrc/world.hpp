@@ -337,6 +337,11 @@ class TimeSolver {
   // stage combinations inside the field launch where possible; off by
   // default: slower on MI355X at 256x256x64 (2.65 against 2.56 ms per step)
   bool fuse_stages = false;
+  // the stage combination of a lone magnetisation with demag inside the
+  // demag's first FFT pass (no combination launch, one read of the state);
+  // off by default: slower on MI355X at 256x256x64 (132 us for the fused pass
+  // against 54 + 38 us for the combination and the x-forward pass)
+  bool fuse_combine = false;
 
   void set_method(const std::string& name);
   double sensible_timestep();
@@ -400,8 +405,14 @@ class World {
   // term of the step in progress (rungekutta.cpp:95-106).
   // next: the single magnetisation variable's evaluation also writes the
   // next stage state (Ferromagnet::next_fusable).
+  // stage_in: the single magnetisation variable's stage state is formed inside
+  // its demag's first FFT pass (x = normalise(x0 + sum_j c_j k_j)).
+  struct StageIn {
+    const float* x0;
+    Combo co;
+  };
   void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false,
-                    double noise_dt = 0.0, const NextStage* next = nullptr);
+                    double noise_dt = 0.0, const NextStage* next = nullptr, const StageIn* stage_in = nullptr);
   bool relaxing() const { return relaxing_; }
 
   // Field of `src` on the grid (size, origin); method 0 = FFT, 1 = direct sum.
