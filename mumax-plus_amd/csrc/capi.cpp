@@ -408,7 +408,7 @@ int mpa_world_get_pbc(const mpa_world* w, int mastergrid_size[3], int mastergrid
 int mpa_world_bounding_grid(const mpa_world* w, int size[3], int origin[3]) {
   return guard([&] {
     need(w, "world");
-    w->w.bounding_grid(size, origin);
+    w->w.bounding_grid(size, origin, true);
   });
 }
 

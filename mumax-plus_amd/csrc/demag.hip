@@ -64,6 +64,9 @@ constexpr size_t LDS_LIMIT = 163840;  // LDS a gfx950 workgroup may allocate
 #ifndef MPA_YINV_THREADS
 #define MPA_YINV_THREADS 512
 #endif
+#ifndef MPA_YFWD_THREADS
+#define MPA_YFWD_THREADS 1024
+#endif
 
 int ilog2(int v) {
   int l = 0;
@@ -161,6 +164,50 @@ __device__ __forceinline__ int iopad(int i) {
   return padsm<fft_pad_shift(L2 < 3 ? 3 : L2, 0), 1>(i);
 }
 
+// Block-wide transfer of TOTAL elements in two phases: each thread first
+// issues all its reads (elements e = threadIdx.x + i THREADS), then all its
+// writes, so its global loads (or LDS loads) are in flight together instead
+// of one at a time.
+template <int TOTAL, int THREADS, class Rd, class Wr>
+__device__ __forceinline__ void staged_copy(const Rd& rd, const Wr& wr) {
+  constexpr int NIT = (TOTAL + THREADS - 1) / THREADS;
+  float2 buf[NIT];
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int e = threadIdx.x + i * THREADS;
+    buf[i] = e < TOTAL ? rd(e) : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int e = threadIdx.x + i * THREADS;
+    if (e < TOTAL) wr(e, buf[i]);
+  }
+}
+
+// The split of PB complex lines (two real lines packed each) into their half
+// spectra, stored to A[...][kx][y] at abase[li] + c_off: elements e = (kx, li)
+// with li (the real line) fastest.
+template <int L2, class K>
+__device__ __forceinline__ void xfwd_split_store(const DD& d, const float2* lds, const long long* abase,
+                                                 long long c_off, float2* __restrict__ A) {
+  using C = Line<L2>;
+  constexpr int L2NL = K::L2PB + 1;
+  staged_copy<(C::H << L2NL), K::THREADS>(
+      [&](int e) {
+        const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
+        const float2* pl = lds + (li >> 1) * K::LP;
+        const float2 z1 = pl[iopad<L2>(kx)];
+        const float2 z2 = pl[iopad<L2>((C::L - kx) & (C::L - 1))];
+        return (li & 1) ? make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x))
+                        : make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+      },
+      [&](int e, float2 X) {
+        const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
+        const long long base = abase[li];
+        if (base >= 0) A[c_off + base + (long long)kx * d.ny] = X;
+      });
+}
+
 // Offset of x-line l (c, z, y) in A[c][z][kx][y] at kx = 0; -1 past the end.
 __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
   if (l >= d.nlines) return -1;
@@ -205,19 +252,7 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
 #pragma unroll
   for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
   __syncthreads();
-  constexpr int L2NL = K::L2PB + 1;
-  for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
-    const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
-    const long long base = abase[li];
-    if (base < 0) continue;
-    const float2* pl = lds + (li >> 1) * K::LP;
-    const float2 z1 = pl[iopad<L2>(kx)];
-    const float2 z2 = pl[iopad<L2>((C::L - kx) & (C::L - 1))];
-    float2 X;
-    if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
-    else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-    A[base + (long long)kx * d.ny] = X;
-  }
+  xfwd_split_store<L2, K>(d, lds, abase, 0, A);
 }
 
 template <int L2>
@@ -285,7 +320,6 @@ __device__ __forceinline__ void xfwd_combine_body(const DD& d, const float* __re
   }
   float2* line = lds + g * K::LP;
   const long long cstride = (long long)d.nz * d.Hx * d.ny;  // component stride of A
-  constexpr int L2NL = K::L2PB + 1;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float2 v[1][8];
@@ -295,18 +329,7 @@ __device__ __forceinline__ void xfwd_combine_body(const DD& d, const float* __re
 #pragma unroll
     for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
     __syncthreads();
-    for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
-      const int kx = e >> L2NL, li = e & ((1 << L2NL) - 1);
-      const long long base = abase[li];
-      if (base < 0) continue;
-      const float2* pl = lds + (li >> 1) * K::LP;
-      const float2 z1 = pl[iopad<L2>(kx)];
-      const float2 z2 = pl[iopad<L2>((C::L - kx) & (C::L - 1))];
-      float2 X;
-      if (!(li & 1)) X = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
-      else X = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-      A[c * cstride + base + (long long)kx * d.ny] = X;
-    }
+    xfwd_split_store<L2, K>(d, lds, abase, c * cstride, A);
     __syncthreads();
   }
 }
@@ -325,7 +348,7 @@ struct YCfg {
   using C = Line<L2>;
   // lines (c, kx, z) per block, z fastest: at least 16 (128-byte runs of z on
   // the strided B side) while a block stays within 1024 threads
-  static constexpr int LB = C::T >= 64 ? clampc(1024 / C::T, 1, 16) : clampc(512 / C::T, 1, 32);
+  static constexpr int LB = C::T >= 64 ? clampc(MPA_YFWD_THREADS / C::T, 1, 16) : clampc(512 / C::T, 1, 32);
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
   // the transposed store reads one element of each of 16 lines per lane
@@ -401,12 +424,16 @@ __device__ __forceinline__ void yfwd_body(const DD& d, const float2* __restrict_
 #pragma unroll
   for (int q = 0; q < C::E; ++q) line[iopad<L2>(t + C::T * q)] = v[0][q];
   __syncthreads();
-  for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
-    const int ky = e >> K::L2LB, li = e & (K::LB - 1);
-    const long long base = bbase[li];
-    if (base < 0) continue;
-    B[base + (long long)ky * d.nz] = lds[li * K::LP + iopad<L2>(ky)];
-  }
+  staged_copy<(C::L << K::L2LB), K::THREADS>(
+      [&](int e) {
+        const int ky = e >> K::L2LB, li = e & (K::LB - 1);
+        return lds[li * K::LP + iopad<L2>(ky)];
+      },
+      [&](int e, float2 v) {
+        const int ky = e >> K::L2LB, li = e & (K::LB - 1);
+        const long long base = bbase[li];
+        if (base >= 0) B[base + (long long)ky * d.nz] = v;
+      });
 }
 
 template <int L2>
@@ -425,11 +452,16 @@ __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict_
   const int q0 = blk * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
   __syncthreads();
-  for (int e = threadIdx.x; e < (C::L << K::L2LB); e += K::THREADS) {
-    const int ky = e >> K::L2LB, li = e & (K::LB - 1);
-    const long long base = bbase[li];
-    lds[li * K::LP + iopad<L2>(ky)] = base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
-  }
+  staged_copy<(C::L << K::L2LB), K::THREADS>(
+      [&](int e) {
+        const int ky = e >> K::L2LB, li = e & (K::LB - 1);
+        const long long base = bbase[li];
+        return base >= 0 ? B[base + (long long)ky * d.nz] : make_float2(0.f, 0.f);
+      },
+      [&](int e, float2 v) {
+        const int ky = e >> K::L2LB, li = e & (K::LB - 1);
+        lds[li * K::LP + iopad<L2>(ky)] = v;
+      });
   __syncthreads();
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   float2* line = lds + g * K::LP;
@@ -666,11 +698,16 @@ __device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict_
   const int l0 = 2 * blk * K::PB;
   for (int i = threadIdx.x; i < NL; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   __syncthreads();
-  for (int e = threadIdx.x; e < (C::H << L2NL); e += K::THREADS) {
-    const int kx = e >> L2NL, li = e & (NL - 1);
-    const long long base = abase[li];
-    X[li * K::XS + kx] = base >= 0 ? A[base + (long long)kx * d.ny] : make_float2(0.f, 0.f);
-  }
+  staged_copy<(C::H << L2NL), K::THREADS>(
+      [&](int e) {
+        const int kx = e >> L2NL, li = e & (NL - 1);
+        const long long base = abase[li];
+        return base >= 0 ? A[base + (long long)kx * d.ny] : make_float2(0.f, 0.f);
+      },
+      [&](int e, float2 v) {
+        const int kx = e >> L2NL, li = e & (NL - 1);
+        X[li * K::XS + kx] = v;
+      });
   __syncthreads();
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   constexpr int HALF = C::L >> 1;

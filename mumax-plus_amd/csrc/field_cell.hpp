@@ -83,9 +83,17 @@ struct NoEpilogue {
 // field of the other magnets (added when has_stray); hd: the demag field
 // (added when u.demag).  Term order: exchange, anisotropy, external, stray,
 // DMI, demag.
+// FAST: the configuration uniform_fast() accepts, with its switches fixed at
+// compile time (exchange and DMI on, Neumann boundaries, uniaxial
+// anisotropy, external field, demag, no stray field, LLG torque, not
+// frozen); zero DMI or anisotropy constants then add exact zeros.
+template <bool FAST = false>
 __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, float3 mc, const float3 (&mn)[6],
                                                  const bool (&ok)[6], int ndir, bool has_stray, float3 hs,
                                                  float3 hd) {
+  const bool has_ex = FAST || u.has_ex, has_dmi = FAST || u.has_dmi;
+  const bool openbc = !FAST && u.openbc, openbc_dmi = !FAST && u.openbc_dmi;
+  const bool uni = FAST || u.uni, cubic = !FAST && u.cubic, demag = FAST || u.demag;
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
   // DMI field of neighbour value v along `axis`, before the direction sign
   auto dmi_of = [&](int axis, float3 v) {
@@ -96,28 +104,28 @@ __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, f
   // or the open boundary.
   auto boundary = [&](int axis, float s) {
     const float3 gam = gamma_u(u, axis, mc);
-    if (u.has_ex && !u.openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
-    if (u.has_dmi && !u.openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
+    if (has_ex && !openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
+    if (has_dmi && !openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
   };
-  if (u.has_ex || u.has_dmi) {
+  if (has_ex || has_dmi) {
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
       if (2 * axis >= ndir) break;
       const float3 mm = mn[2 * axis], mp = mn[2 * axis + 1];
       if (ok[2 * axis] && ok[2 * axis + 1]) {
         // both neighbours plain: kex (m+ + m- - 2m) and D (m+ - m-)
-        if (u.has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
-        if (u.has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
+        if (has_ex) hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
+        if (has_dmi) hdm = hdm + dmi_of(axis, mp - mm);
       } else {
         if (ok[2 * axis]) {
-          if (u.has_ex) hex = hex + u.kex[axis] * (mm - mc);
-          if (u.has_dmi) hdm = hdm - dmi_of(axis, mm);
+          if (has_ex) hex = hex + u.kex[axis] * (mm - mc);
+          if (has_dmi) hdm = hdm - dmi_of(axis, mm);
         } else {
           boundary(axis, -1.f);
         }
         if (ok[2 * axis + 1]) {
-          if (u.has_ex) hex = hex + u.kex[axis] * (mp - mc);
-          if (u.has_dmi) hdm = hdm + dmi_of(axis, mp);
+          if (has_ex) hex = hex + u.kex[axis] * (mp - mc);
+          if (has_dmi) hdm = hdm + dmi_of(axis, mp);
         } else {
           boundary(axis, 1.f);
         }
@@ -125,11 +133,11 @@ __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, f
     }
   }
   float3 han = f3(0.f, 0.f, 0.f);
-  if (u.uni) {
+  if (uni) {
     const float3 ua = f3(u.u[0], u.u[1], u.u[2]);
     const float mu = dot3(mc, ua);
     han = (u.ku1 * mu + u.ku2 * mu * mu * mu) * ua;
-  } else if (u.cubic) {
+  } else if (cubic) {
     const float3 c1 = f3(u.c1[0], u.c1[1], u.c1[2]), c2 = f3(u.c2[0], u.c2[1], u.c2[2]),
                  c3 = f3(u.c3[0], u.c3[1], u.c3[2]);
     const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
@@ -142,12 +150,12 @@ __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, f
   float3 h = hex;
   h = h + han;
   h = h + f3(u.hext[0], u.hext[1], u.hext[2]);
-  if (has_stray) h = h + hs;
+  if (!FAST && has_stray) h = h + hs;
   h = h + hdm;
-  if (u.demag) h = h + hd;
-  if (mode == MODE_FIELD) return h;
-  if (u.frozen) return f3(0.f, 0.f, 0.f);
-  if (mode == MODE_LLG) {
+  if (demag) h = h + hd;
+  if (!FAST && mode == MODE_FIELD) return h;
+  if (!FAST && u.frozen) return f3(0.f, 0.f, 0.f);
+  if (FAST || mode == MODE_LLG) {
     const float3 mxh = cross3(mc, h);
     return u.llg * (mxh + u.alpha * cross3(mc, mxh));
   }

@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
 constexpr int TILE_X = 64, TILE_Y = 4;
 constexpr int TILE_PX = TILE_X + 2;  // LDS row pitch (halo column on each side)
 
-template <bool NEXT>
+template <bool NEXT, bool FAST>
 __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                                  const float* __restrict__ hdemag,
                                                                  float* __restrict__ out, int mode, NextStage next,
@@ -536,8 +536,8 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
     const float3 hnext = (hl >= 0 && hv && z + 1 < z1) ? ld(z + 1, hc) : zero;
     if (inb) {
       const int idx = z * P + col;
-      const float3 hs = d.hstray ? load3(d.hstray, idx, N) : zero;
-      const float3 hd = u.demag ? load3(hdemag, idx, N) : zero;
+      const float3 hs = (!FAST && d.hstray) ? load3(d.hstray, idx, N) : zero;
+      const float3 hd = (FAST || u.demag) ? load3(hdemag, idx, N) : zero;
       float3 mn[6];
       bool ok[6];
       auto at = [&](int slot) { return f3(s[b][0][slot], s[b][1][slot], s[b][2][slot]); };
@@ -553,7 +553,7 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
       ok[3] = yp_ok;
       ok[4] = z > 0;
       ok[5] = more;
-      const float3 r = uniform_result(u, mode, mc, mn, ok, ndir, d.hstray != nullptr, hs, hd);
+      const float3 r = uniform_result<FAST>(u, mode, mc, mn, ok, ndir, d.hstray != nullptr, hs, hd);
       out[idx] = r.x;
       out[N + idx] = r.y;
       out[2 * N + idx] = r.z;
@@ -775,6 +775,21 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
 }
 
 namespace {
+// The configuration uniform_result<true> fixes at compile time: exchange on
+// with Neumann boundaries (A and Ms non-zero, no open boundaries), DMI on or
+// all nine constants zero, no cubic anisotropy, demag on, no stray field
+// from other magnets, not frozen, LLG torque.
+bool uniform_fast(const FMDesc& d, const FMUniform& u, FieldMode mode) {
+  bool dmi_ok = u.has_dmi != 0;
+  if (!dmi_ok) {
+    dmi_ok = true;
+    for (int k = 0; k < D_COUNT; ++k) dmi_ok = dmi_ok && u.D[k] == 0.f;
+    for (int a = 0; a < 3; ++a) dmi_ok = dmi_ok && u.dxz[a] == 0.f && u.dxy[a] == 0.f && u.dyz[a] == 0.f;
+  }
+  return mode == MODE_LLG && u.has_ex && !u.openbc && !u.openbc_dmi && !u.cubic && u.demag && !u.frozen &&
+         !d.hstray && dmi_ok;
+}
+
 // The halo-tile kernel covers magnets without a geometry mask or periodic
 // wrap that are at least one tile wide, unless the world switched it off.
 bool tile_eligible(const FMDesc& d) {
@@ -787,14 +802,14 @@ void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, 
   if (tile_eligible(d)) {
     const int zc = std::min(d.tile_zc, d.nz);
     dim3 grid((d.nx + TILE_X - 1) / TILE_X, (d.ny + TILE_Y - 1) / TILE_Y, (d.nz + zc - 1) / zc);
-    if (next) {
-      k_field_tile<true><<<grid, dim3(TILE_X, TILE_Y, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, *next, zc);
-      checkLaunch("k_field_tile<next>");
-    } else {
-      k_field_tile<false><<<grid, dim3(TILE_X, TILE_Y, 1), 0, s>>>(d, u, m, hdemag, out, (int)mode, NextStage{},
-                                                                   zc);
-      checkLaunch("k_field_tile");
-    }
+    const dim3 blk(TILE_X, TILE_Y, 1);
+    const NextStage ns = next ? *next : NextStage{};
+    const bool fast = uniform_fast(d, u, mode);
+    if (next && fast) k_field_tile<true, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
+    else if (next) k_field_tile<true, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
+    else if (fast) k_field_tile<false, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
+    else k_field_tile<false, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
+    checkLaunch("k_field_tile");
     return;
   }
   int bx = 1;

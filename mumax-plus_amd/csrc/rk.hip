@@ -94,6 +94,10 @@ struct CombineBatch {
 
 // One job per blockIdx.y; four consecutive cells per thread when N is a
 // multiple of 4, one otherwise.  Same arithmetic as k_combine / k_combine4.
+// NK >= 0: every job combines NK derivatives, a compile-time count, so the
+// loop unrolls and all of a thread's loads are in flight together; NK < 0
+// reads each job's count.
+template <int NK>
 __global__ void __launch_bounds__(BLOCK) k_combine_batch(CombineBatch b) {
   const CombineJob& jb = b.job[blockIdx.y];
   const int N = jb.N;
@@ -105,7 +109,9 @@ __global__ void __launch_bounds__(BLOCK) k_combine_batch(CombineBatch b) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       float4 a = reinterpret_cast<const float4*>(jb.x0 + (size_t)c * N)[i4];
-      for (int j = 0; j < co.nk; ++j) {
+#pragma unroll
+      for (int j = 0; j < (NK >= 0 ? NK : MAX_STAGES); ++j) {
+        if (NK < 0 && j >= co.nk) break;
         const float4 kv = reinterpret_cast<const float4*>(co.k[j] + (size_t)c * N)[i4];
         const float cj = co.c[j];
         a.x += cj * kv.x;
@@ -151,7 +157,11 @@ __global__ void __launch_bounds__(BLOCK) k_combine_batch(CombineBatch b) {
   const bool in = !jb.geom || jb.geom[idx];
   for (int c = 0; c < 3; ++c) {
     float a = jb.x0[(size_t)c * N + idx];
-    for (int j = 0; j < co.nk; ++j) a += co.c[j] * co.k[j][(size_t)c * N + idx];
+#pragma unroll
+    for (int j = 0; j < (NK >= 0 ? NK : MAX_STAGES); ++j) {
+      if (NK < 0 && j >= co.nk) break;
+      a += co.c[j] * co.k[j][(size_t)c * N + idx];
+    }
     v[c] = a;
   }
   if (jb.normalize) {
@@ -348,7 +358,20 @@ void launch_combine_batch(const CombineJob* jobs, int njobs, hipStream_t s) {
     nmax = per > nmax ? per : nmax;
   }
   const dim3 grid((unsigned)((nmax + BLOCK - 1) / BLOCK), (unsigned)njobs);
-  k_combine_batch<<<grid, BLOCK, 0, s>>>(b);
+  int nk = jobs[0].co.nk;
+  for (int i = 1; i < njobs; ++i)
+    if (jobs[i].co.nk != nk) nk = -1;
+  switch (nk) {
+    case 0: k_combine_batch<0><<<grid, BLOCK, 0, s>>>(b); break;
+    case 1: k_combine_batch<1><<<grid, BLOCK, 0, s>>>(b); break;
+    case 2: k_combine_batch<2><<<grid, BLOCK, 0, s>>>(b); break;
+    case 3: k_combine_batch<3><<<grid, BLOCK, 0, s>>>(b); break;
+    case 4: k_combine_batch<4><<<grid, BLOCK, 0, s>>>(b); break;
+    case 5: k_combine_batch<5><<<grid, BLOCK, 0, s>>>(b); break;
+    case 6: k_combine_batch<6><<<grid, BLOCK, 0, s>>>(b); break;
+    case 7: k_combine_batch<7><<<grid, BLOCK, 0, s>>>(b); break;
+    default: k_combine_batch<-1><<<grid, BLOCK, 0, s>>>(b); break;
+  }
   checkLaunch("k_combine_batch");
 }
 

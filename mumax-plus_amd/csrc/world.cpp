@@ -1879,17 +1879,32 @@ void World::check_pbc(const int mg_size[3], const int reps[3]) const {
       throw Error(MPA_ERR_INVALID, "0 in size of mastergrid should match 0 in pbcRepetitions.");
 }
 
-void World::check_in_mastergrid(const int size[3], const int org[3], const int mg_size[3],
-                                const int mg_origin[3]) const {
+// MumaxWorld::checkAddibility / checkAllMagnetsInMastergrid (mumaxworld.cpp:35-46, :228-236):
+// std::out_of_range in the reference, an index error at this boundary.
+void World::check_in_mastergrid(const int size[3], const int org[3], const int mg_size[3], const int mg_origin[3],
+                                bool adding) const {
   for (int a = 0; a < 3; ++a) {
     if (mg_size[a] == 0) continue;
     if (org[a] < mg_origin[a] || org[a] + size[a] > mg_origin[a] + mg_size[a])
-      throw Error(MPA_ERR_INVALID, "The grid of the magnet does not fit inside the mastergrid.");
+      throw Error(MPA_ERR_INDEX, adding ? "Can not add magnet because the grid does not fit in the mastergrid"
+                                        : "Not all magnets of the world fit inside the mastergrid.");
   }
 }
 
+void World::check_no_overlap(const int size[3], const int org[3]) const {
+  auto overlaps = [&](const MagnetCore& m) {
+    for (int a = 0; a < 3; ++a)
+      if (org[a] >= m.origin[a] + m.n[a] || m.origin[a] >= org[a] + size[a]) return false;
+    return true;
+  };
+  for (auto& m : magnets)
+    if (overlaps(*m)) throw Error(MPA_ERR_INDEX, "Can not add magnet because it overlaps with another magnet.");
+  for (auto& a : afms)
+    if (overlaps(*a)) throw Error(MPA_ERR_INDEX, "Can not add magnet because it overlaps with another magnet.");
+}
+
 void World::check_name(const std::string& name) {
-  if (find(name) || find_afm(name)) throw Error(MPA_ERR_INVALID, "A magnet with the name '" + name + "' already exists");
+  if (find(name) || find_afm(name)) throw Error(MPA_ERR_RUNTIME, "A magnet with the name '" + name + "' already exists");
 }
 
 Ferromagnet* World::add_ferromagnet(const int size[3], const int org[3], const uint8_t* geometry,
@@ -1897,8 +1912,9 @@ Ferromagnet* World::add_ferromagnet(const int size[3], const int org[3], const u
   static int unnamed = 1;  // ferromagnet_<n> naming of mumaxworld.cpp:61-64
   std::string name = nm ? nm : "";
   if (name.empty()) name = "ferromagnet_" + std::to_string(unnamed++);
+  check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin, true);
+  check_no_overlap(size, org);
   check_name(name);
-  check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
   magnets.push_back(std::make_unique<Ferromagnet>(this, size, org, geometry, name));
   magnets.back()->set_regions(regions);
   touch();
@@ -1911,8 +1927,9 @@ Antiferromagnet* World::add_antiferromagnet(const int size[3], const int org[3],
   static int unnamed = 1;  // antiferromagnet_<n> naming of mumaxworld.cpp:86-89
   std::string name = nm ? nm : "";
   if (name.empty()) name = "antiferromagnet_" + std::to_string(unnamed++);
+  check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin, true);
+  check_no_overlap(size, org);
   check_name(name);
-  check_in_mastergrid(size, org, mastergrid_size, mastergrid_origin);
   afms.push_back(std::make_unique<Antiferromagnet>(this, size, org, geometry, name));
   afms.back()->set_regions(regions);
   touch();
@@ -1942,10 +1959,12 @@ std::vector<Ferromagnet*> World::all_ferromagnets() {
   return v;
 }
 
-void World::bounding_grid(int size[3], int org[3]) const {
+void World::bounding_grid(int size[3], int org[3], bool require_magnets) const {
   std::vector<const MagnetCore*> all;
   for (auto& m : magnets) all.push_back(m.get());
   for (auto& a : afms) all.push_back(a.get());
+  if (all.empty() && require_magnets)  // MumaxWorld::boundingGrid (mumaxworld.cpp:264-267)
+    throw Error(MPA_ERR_INDEX, "Cannot find minimum bounding box if there are no magnets in the world.");
   if (all.empty()) {
     for (int a = 0; a < 3; ++a) size[a] = org[a] = 0;
     return;
@@ -1981,8 +2000,8 @@ void World::pbc_changed() {
 // MumaxWorld::setPBC(mastergrid, reps) (mumaxworld.cpp:311-320)
 void World::set_pbc(const int mg_size[3], const int mg_origin[3], const int reps[3]) {
   check_pbc(mg_size, reps);
-  for (auto& m : magnets) check_in_mastergrid(m->n, m->origin, mg_size, mg_origin);
-  for (auto& a : afms) check_in_mastergrid(a->n, a->origin, mg_size, mg_origin);
+  for (auto& m : magnets) check_in_mastergrid(m->n, m->origin, mg_size, mg_origin, false);
+  for (auto& a : afms) check_in_mastergrid(a->n, a->origin, mg_size, mg_origin, false);
   for (int a = 0; a < 3; ++a) {
     mastergrid_size[a] = mg_size[a];
     mastergrid_origin[a] = mg_origin[a];
@@ -1996,7 +2015,7 @@ void World::set_pbc_auto(const int reps[3]) {
   for (int a = 0; a < 3; ++a)
     if (reps[a] < 0) throw Error(MPA_ERR_INVALID, "Number of pbcRepetitions should not be negative.");
   int size[3], org[3];
-  bounding_grid(size, org);
+  bounding_grid(size, org, true);
   for (int a = 0; a < 3; ++a)
     if (reps[a] == 0) size[a] = 0;
   set_pbc(size, org, reps);

@@ -390,7 +390,8 @@ class World {
   Antiferromagnet* find_afm(const std::string& name);
   // every ferromagnet with a magnetisation: independent ones and sublattices
   std::vector<Ferromagnet*> all_ferromagnets();
-  void bounding_grid(int size[3], int origin[3]) const;
+  // require_magnets: an index error for a world without magnets (else all zeros)
+  void bounding_grid(int size[3], int origin[3], bool require_magnets = false) const;
   void set_pbc(const int mg_size[3], const int mg_origin[3], const int reps[3]);
   void set_pbc_auto(const int reps[3]);
   void unset_pbc();
@@ -428,7 +429,9 @@ class World {
 
  private:
   void check_pbc(const int mg_size[3], const int reps[3]) const;
-  void check_in_mastergrid(const int size[3], const int origin[3], const int mg_size[3], const int mg_origin[3]) const;
+  void check_in_mastergrid(const int size[3], const int origin[3], const int mg_size[3], const int mg_origin[3],
+                           bool adding) const;
+  void check_no_overlap(const int size[3], const int origin[3]) const;
   void check_name(const std::string& name);
   void pbc_changed();
   double total_energy(MagnetCore* m);

@@ -50,9 +50,13 @@ class Parameter(FieldQuantity):
         return _np.array(_np.vectorize(mask, otypes=[float] * ncomp)(X, Y, Z))
 
     def _check_mask_shape(self, mask):
-        """(nz, ny, nx), (1, nz, ny, nx) or (ncomp, nz, ny, nx) -> (ncomp, nz, ny, nx)."""
+        """(nz, ny, nx), (1, nz, ny, nx) or (ncomp, nz, ny, nx) -> (ncomp, nz, ny, nx);
+        a mask with another number of components is rejected (ValueError), as
+        the reference's component-wise copy rejects it (parameter.py:125-144)."""
         if mask.ndim == 4 and mask.shape[0] == self.ncomp:
             return mask
+        if mask.ndim == 4 and mask.shape[0] != 1:
+            raise ValueError(f"The mask has {mask.shape[0]} components; the parameter has {self.ncomp}.")
         core = mask if mask.ndim != 4 else mask[0]
         return _np.broadcast_to(core, (self.ncomp, *core.shape)).copy()
 

@@ -435,7 +435,7 @@ def test_full_magnetization_sums_sublattices():
 def test_is_sublattice_and_names():
     world = mp.World(cellsize=(3e-9, 2e-9, 1e-9))
     fm = mp.Ferromagnet(world, mp.Grid((16, 8, 2)))
-    afm = mp.Antiferromagnet(world, mp.Grid((8, 16, 2)), name="afm")
+    afm = mp.Antiferromagnet(world, mp.Grid((8, 16, 2), origin=(0, 8, 0)), name="afm")
     fm.enable_as_stray_field_source = False
     afm.enable_as_stray_field_source = False
     assert not fm.is_sublattice
@@ -447,8 +447,10 @@ def test_is_sublattice_and_names():
         world.get_antiferromagnet("nope")
     with pytest.raises(ValueError):
         afm.sub1.enable_elastodynamics = True
-    with pytest.raises(ValueError):
-        mp.Antiferromagnet(world, mp.Grid((8, 16, 2)), name="afm")
+    with pytest.raises(IndexError):  # overlaps (mumaxworld.cpp:42-47)
+        mp.Antiferromagnet(world, mp.Grid((8, 16, 2)), name="other")
+    with pytest.raises(RuntimeError):  # the name is taken (mumaxworld.cpp:50-53)
+        mp.Antiferromagnet(world, mp.Grid((8, 16, 2), origin=(40, 0, 0)), name="afm")
 
 
 def test_afm_magnetoelastic_sum_rules():

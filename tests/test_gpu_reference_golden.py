@@ -152,3 +152,50 @@ def test_edge_canting_matches_mumax3(kind, x_dir):
     if kind != "neumann":
         want = _analytic_canting()
         assert abs(cur - want) / want < CANT_RTOL
+
+
+# ---------------------------------------------------------------- frozen spins
+# test/test_frozenspins.py:7-53: the standard problem 4 film (128 x 32 x 2
+# cells of 500/128 x 125/32 x 1.5 nm, Ms 800e3, A 13e-12, alpha 0.02).
+SP4_CS = (500e-9 / 128, 125e-9 / 32, 3e-9 / 2)
+SP4_FIELD = (-24.6e-3, 4.3e-3, 0.0)
+
+
+def _sp4_film():
+    world = mp.World(SP4_CS)
+    magnet = mp.Ferromagnet(world, mp.Grid((128, 32, 2)))
+    magnet.msat = 800e3
+    magnet.aex = 13e-12
+    magnet.alpha = 0.02
+    return world, magnet
+
+
+def test_frozen_spins_do_not_move():
+    """All spins frozen: 1 ns in the switching field leaves <m> unchanged
+    (test/test_frozenspins.py:7-23, tolerance 1e-7)."""
+    world, magnet = _sp4_film()
+    magnet.bias_magnetic_field = SP4_FIELD
+    magnet.frozen_spins = 1
+    magnet.magnetization = (1, 0.1, 0)
+    avg0 = np.array(magnet.magnetization.average())
+    world.timesolver.run(1e-9)
+    assert np.all(np.abs(np.array(magnet.magnetization.average()) - avg0) < 1e-7)
+
+
+def test_half_frozen_minimize_and_reversal_golden():
+    """Right half frozen: <m> after minimize and after 1 ns of reversal
+    equal the values the reference records (test/test_frozenspins.py:26-53,
+    tolerance 1e-4)."""
+    world, magnet = _sp4_film()
+    magnet.magnetization = (1, 0.1, 0)
+    cx = magnet.center[0]
+    magnet.frozen_spins = lambda x, y, z: 1 if x > cx else 0
+    magnet.magnetization = (1, 0.1, 0)
+    magnet.minimize()
+    got = np.array(magnet.magnetization.average())
+    assert np.all(np.abs(got - (0.9809795618057251, 0.11742201447486877, 0.0)) < 1e-4), got
+    magnet.bias_magnetic_field = SP4_FIELD
+    world.timesolver.run(1e-9)
+    got = np.array(magnet.magnetization.average())
+    want = (0.14499470591545105, 0.24905619025230408, 0.0021978835575282574)
+    assert np.all(np.abs(got - want) < 1e-4), got
