@@ -348,7 +348,7 @@ int mpa_demag_plan_info(const mpa_demag_plan* p, int fft_size[3], double* algori
  * bytes each), the summed milliseconds, launch count and summed algorithmic bytes; *count
  * receives the number of kernel names (at most `cap` are written). */
 int mpa_world_set_profiling(mpa_world* w, int enable);
-/* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 0): a world whose
+/* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 1): a world whose
  * only variable is one magnetisation on the uniform-parameter LLG path, stepped with a
  * first-same-as-last tableau, computes each stage state inside the field launch that produces
  * the last derivative it needs (no separate combination launch); 0 runs the separate

@@ -86,7 +86,9 @@ struct NoEpilogue {
 // FAST: the configuration uniform_fast() accepts, with its switches fixed at
 // compile time (exchange and DMI on, Neumann boundaries, uniaxial
 // anisotropy, external field, demag, no stray field, LLG torque, not
-// frozen); zero DMI or anisotropy constants then add exact zeros.
+// frozen); zero DMI or anisotropy constants then add exact zeros, and the
+// boundary terms are taken through ghost neighbours (same terms, rounded in
+// another order).
 template <bool FAST = false>
 __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, float3 mc, const float3 (&mn)[6],
                                                  const bool (&ok)[6], int ndir, bool has_stray, float3 hs,
@@ -100,6 +102,30 @@ __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, f
     return f3(u.dxy[axis] * v.y + u.dxz[axis] * v.z, -u.dxy[axis] * v.x + u.dyz[axis] * v.z,
               -u.dxz[axis] * v.x - u.dyz[axis] * v.y);
   };
+  if constexpr (FAST) {
+    // Neumann boundaries as ghost values: the boundary terms below (exchange
+    // s Gamma/(Ms c), DMI s D.(m + s c Gamma/(2A))) are the interior formula
+    // with the missing neighbour replaced by m + s (c/(2A)) Gamma(m).  Only
+    // the lanes at a boundary compute a ghost; then every cell takes the
+    // same branch-free two-sided formula.
+    float3 nb[6];
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      nb[dir] = mn[dir];
+      if (dir < ndir && !ok[dir]) {
+        const int axis = dir >> 1;
+        const float sg = (dir & 1) ? 1.f : -1.f;
+        nb[dir] = mc + (sg * u.ncoef[axis]) * gamma_u(u, axis, mc);
+      }
+    }
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+      if (2 * axis >= ndir) break;
+      const float3 mm = nb[2 * axis], mp = nb[2 * axis + 1];
+      hex = hex + u.kex[axis] * (mm + mp - 2.f * mc);
+      hdm = hdm + dmi_of(axis, mp - mm);
+    }
+  }
   // Missing or outside-geometry neighbour: Neumann with the DMI correction,
   // or the open boundary.
   auto boundary = [&](int axis, float s) {
@@ -107,7 +133,7 @@ __device__ __forceinline__ float3 uniform_result(const FMUniform& u, int mode, f
     if (has_ex && !openbc) hex = hex + (s * u.inv_msc[axis]) * gam;
     if (has_dmi && !openbc_dmi) hdm = hdm + s * dmi_of(axis, mc + (s * u.ncoef[axis]) * gam);
   };
-  if (has_ex || has_dmi) {
+  if (!FAST && (has_ex || has_dmi)) {
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
       if (2 * axis >= ndir) break;

@@ -39,6 +39,9 @@ constexpr int BLOCK = 256;
 #ifndef MPA_FIELD_ZPER
 #define MPA_FIELD_ZPER 1
 #endif
+#ifndef MPA_FIELD_NT_STORE
+#define MPA_FIELD_NT_STORE 1
+#endif
 
 __device__ __forceinline__ float harmonic_mean(float a, float b) {
   if (a + b == 0.0f) return 0.0f;
@@ -479,7 +482,7 @@ __global__ void __launch_bounds__(BLOCK) k_field_uniform(FMDesc d, FMUniform u, 
 constexpr int TILE_X = 64, TILE_Y = 4;
 constexpr int TILE_PX = TILE_X + 2;  // LDS row pitch (halo column on each side)
 
-template <bool NEXT, bool FAST>
+template <bool NEXT, bool FAST, int NK = -1>
 __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUniform u, const float* __restrict__ m,
                                                                  const float* __restrict__ hdemag,
                                                                  float* __restrict__ out, int mode, NextStage next,
@@ -554,10 +557,21 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
       ok[4] = z > 0;
       ok[5] = more;
       const float3 r = uniform_result<FAST>(u, mode, mc, mn, ok, ndir, d.hstray != nullptr, hs, hd);
-      out[idx] = r.x;
-      out[N + idx] = r.y;
-      out[2 * N + idx] = r.z;
-      if constexpr (NEXT) next(idx, mc, r, true);
+#if MPA_FIELD_NT_STORE
+      if constexpr (NEXT) {
+        // the derivative is read again only by later stages: stream it past
+        // the caches, which keep the stage state the next demag pass reads
+        __builtin_nontemporal_store(r.x, out + idx);
+        __builtin_nontemporal_store(r.y, out + N + idx);
+        __builtin_nontemporal_store(r.z, out + 2 * N + idx);
+      } else
+#endif
+      {
+        out[idx] = r.x;
+        out[N + idx] = r.y;
+        out[2 * N + idx] = r.z;
+      }
+      if constexpr (NEXT) next.template apply<NK>(idx, mc, r, true);
     }
     if (z + 1 < z1) {
       put(b ^ 1, cl, mzp);
@@ -805,8 +819,17 @@ void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, 
     const dim3 blk(TILE_X, TILE_Y, 1);
     const NextStage ns = next ? *next : NextStage{};
     const bool fast = uniform_fast(d, u, mode);
-    if (next && fast) k_field_tile<true, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
-    else if (next) k_field_tile<true, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
+    if (next && fast) {
+      // derivative counts of the first-same-as-last tableaus' stages
+      switch (ns.x0 ? ns.nk : -2) {
+        case 1: k_field_tile<true, true, 1><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+        case 2: k_field_tile<true, true, 2><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+        case 3: k_field_tile<true, true, 3><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+        case 4: k_field_tile<true, true, 4><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+        case 5: k_field_tile<true, true, 5><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+        default: k_field_tile<true, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc); break;
+      }
+    } else if (next) k_field_tile<true, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
     else if (fast) k_field_tile<false, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
     else k_field_tile<false, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
     checkLaunch("k_field_tile");

@@ -110,12 +110,22 @@ struct NextStage {
   float* out;
   int N;
   __device__ __forceinline__ void operator()(int idx, float3 mc, float3 r, bool in) const {
+    apply<-1>(idx, mc, r, in);
+  }
+  // NK >= 0: nk == NK known at compile time, so every load is issued before
+  // the sums (NK < 0: nk read at run time).
+  template <int NK>
+  __device__ __forceinline__ void apply(int idx, float3 mc, float3 r, bool in) const {
     float v[3] = {mc.x, mc.y, mc.z};
     const float rr[3] = {r.x, r.y, r.z};
 #pragma unroll
     for (int c3 = 0; c3 < 3; ++c3) {
       float a = x0 ? x0[(size_t)c3 * N + idx] : v[c3];
-      for (int j = 0; j < nk; ++j) a += c[j] * k[j][(size_t)c3 * N + idx];
+#pragma unroll
+      for (int j = 0; j < (NK >= 0 ? NK : 7); ++j) {
+        if (NK < 0 && j >= nk) break;
+        a += c[j] * k[j][(size_t)c3 * N + idx];
+      }
       a += cself * rr[c3];
       v[c3] = a;
     }

@@ -2,7 +2,8 @@
 
 When a world's only variable is one independent magnetisation with demag,
 each stage's combination x = normalise(x0 + sum_j c_j k_j) runs inside the
-x-forward FFT pass (world option "fuse_combine", off by default), which reads
+x-forward FFT pass (world option "fuse_combine", off by default, with the
+combination inside the field launch switched off), which reads
 the step start and the derivatives once for both the new state and its
 transform.  The arithmetic is the separate combination's; the compiler may
 fuse its multiply-adds differently in the two kernels, so the runs agree to
@@ -37,6 +38,10 @@ def _film(fuse, size=(24, 16, 3), seed=3, geometry=True, per_cell=True):
     mag.dmi_tensor.set_interfacial_dmi(2e-3)
     world.bias_magnetic_field = (0.01, 0.0, 0.02)
     mag.magnetization = random_unit(np.random.default_rng(seed), (nz, ny, nx)).astype(np.float32)
+    # the combination inside the field launch (on by default) takes
+    # precedence; this test compares the demag-pass placement with the
+    # separate launch
+    world._impl.set_option("fuse_stages", 0)
     world._impl.set_option("fuse_combine", int(fuse))
     assert world._impl.get_option("fuse_combine") == int(fuse)
     return world, mag

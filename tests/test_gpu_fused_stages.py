@@ -7,9 +7,9 @@ state inside the field launch that produces the last derivative it needs
 last evaluation).  The arithmetic is that of the separate combination, so
 the fused run must equal the unfused one ("fuse_stages" off) bit for bit:
 fixed and adaptive steps, rejections, a step change and a state change
-between calls.  The unfused path (the default) is compared with the FP64
-oracle by the other time-stepping tests, so equality carries that parity
-over to the fused one.
+between calls.  The fused path is the default, so the time-stepping tests
+against the FP64 oracle cover it directly; equality with the separate
+combination carries the same parity over to the other path.
 """
 
 import numpy as np
