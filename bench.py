@@ -601,6 +601,9 @@ def main():
     ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve"), default="ferromagnet")
     ap.add_argument("--no-sub", action="store_true",
                     help="default workload only: leave out the batched and served sub-measurements")
+    ap.add_argument("--unfused", action="store_true",
+                    help="run the stage combination as its own launch (world option fuse_stages 0), e.g. to "
+                         "calibrate counters on a kernel of known traffic")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.gpus != 1:
@@ -622,6 +625,8 @@ def main():
     world, mag = build_world(args.seed)
     ts = world.timesolver
     impl = world._impl
+    if args.unfused:
+        impl.set_option("fuse_stages", 0)
     N = SIZE[0] * SIZE[1] * SIZE[2]
 
     ts.steps(args.warmup)
