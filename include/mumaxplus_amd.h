@@ -397,8 +397,20 @@ int mpa_batch_member(mpa_batch* b, int id, double* time, double* dt, long long* 
 int mpa_batch_info(mpa_batch* b, long long* capacity, long long* free_cells, int* members, double* step_bytes,
                    int* launches_per_step);
 /* "graph": replay a captured HIP graph per step (default 1); "profiling": per-kernel HIP-event
- * timing (steps then run without the graph). */
+ * timing (steps then run without the graph, one launch per pass);
+ * "step_mode": 0 one launch per pass (default), 1 one persistent kernel for all n steps of an
+ * advance (FFT lines up to 256 points), 2 whichever of these the autotuner measures faster for the
+ * batch's size class; "persist_blocks_per_cu": resident blocks per compute unit of the persistent
+ * kernel (0: as many as fit); "persist_spin": the persistent kernel's grid-barrier poll limit
+ * (0: its first barrier fails, for tests of the error path). */
 int mpa_batch_set_option(mpa_batch* b, const char* key, int value);
+/* The step path of the last advance (0 per-pass launches, 1 persistent kernel) and the grid of the
+ * persistent kernel for the current members (0 when it cannot run them). */
+int mpa_batch_step_path(mpa_batch* b, int* last_path, int* persist_grid);
+/* The step-path autotuner's record (process-wide), one line per size class: the class, its choice
+ * and the best measured ms per step of each candidate.  At most cap bytes (NUL-terminated) are
+ * written to text; *len = the full length. */
+int mpa_batch_tune_report(mpa_batch* b, int cap, char* text, int* len);
 int mpa_batch_profile_report(mpa_batch* b, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);
 

@@ -19,228 +19,41 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <sstream>
 
+#include "batch_kernels.hpp"
 #include "world.hpp"
 
 namespace mpa {
 
 namespace {
 
-constexpr int CH = 256;  // cells per chunk; members start on chunk boundaries
-
-struct StageArgs {
-  const float* k[MAX_STAGES];
-  double a[MAX_STAGES];
-  int nk;
-  int commit;
-};
-
-struct ErrArgs {
-  const float* k[MAX_STAGES];
-  double e[MAX_STAGES];
-  int nk;
-  const float* tq;
-};
-
-struct CtlArgs {
-  int order1, order2;
-  int prime;
-};
-
-__device__ __forceinline__ void axpy4(float4& a, float c, const float4& k) {
-  a.x += c * k.x;
-  a.y += c * k.y;
-  a.z += c * k.z;
-  a.w += c * k.w;
-}
+using namespace bk;
 
 // One wavefront per chunk (64 threads x 4 consecutive cells), four chunks
-// per block; a chunk belongs to one member, so every branch on the member is
-// uniform over the wavefront.
+// per block.
 __global__ void __launch_bounds__(256) k_bstage(StageArgs sa, float* __restrict__ X, float* __restrict__ X0,
                                                 float* __restrict__ K0, const float* __restrict__ KL,
                                                 const uint8_t* __restrict__ G, const int* __restrict__ c2m,
                                                 const BatchMember* __restrict__ st, int cs, int nch) {
-  const int chunk = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (chunk >= nch) return;
-  const int m = c2m[chunk];
-  if (m < 0) return;
-  const BatchMember& S = st[m];
-  if (!S.active) return;
-  const int q = chunk * (CH / 4) + (threadIdx.x & 63);  // float4 index within a component
-  const int cs4 = cs >> 2;
-  const bool take = sa.commit && S.accept;
-  float c[MAX_STAGES];
-#pragma unroll
-  for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < sa.nk ? (float)(S.dt * sa.a[j]) : 0.f;
-  float4* X4 = reinterpret_cast<float4*>(X);
-  float4* X04 = reinterpret_cast<float4*>(X0);
-  float4* K04 = reinterpret_cast<float4*>(K0);
-  const float4* KL4 = reinterpret_cast<const float4*>(KL);
-  float4 v[3];
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    const size_t i = (size_t)comp * cs4 + q;
-    float4 x0, k0;
-    if (take) {
-      x0 = X4[i];
-      k0 = KL4[i];
-      X04[i] = x0;
-      K04[i] = k0;
-    } else {
-      x0 = X04[i];
-      k0 = K04[i];
-    }
-    float4 a = x0;
-    axpy4(a, c[0], k0);
-    for (int j = 1; j < sa.nk; ++j) axpy4(a, c[j], reinterpret_cast<const float4*>(sa.k[j])[i]);
-    v[comp] = a;
-  }
-  const float i0 = rsqrtf(v[0].x * v[0].x + v[1].x * v[1].x + v[2].x * v[2].x);
-  const float i1 = rsqrtf(v[0].y * v[0].y + v[1].y * v[1].y + v[2].y * v[2].y);
-  const float i2 = rsqrtf(v[0].z * v[0].z + v[1].z * v[1].z + v[2].z * v[2].z);
-  const float i3 = rsqrtf(v[0].w * v[0].w + v[1].w * v[1].w + v[2].w * v[2].w);
-  const uchar4 g = reinterpret_cast<const uchar4*>(G)[q];
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    float4 o;
-    o.x = g.x ? v[comp].x * i0 : 0.f;
-    o.y = g.y ? v[comp].y * i1 : 0.f;
-    o.z = g.z ? v[comp].z * i2 : 0.f;
-    o.w = g.w ? v[comp].w * i3 : 0.f;
-    X4[(size_t)comp * cs4 + q] = o;
-  }
-}
-
-__device__ __forceinline__ float block_max(float v, float* red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) red[w] = v;
-  __syncthreads();
-  float r = 0.f;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < CH / 64; ++i) r = fmaxf(r, red[i]);
-  return r;
+  bstage_block(sa, X, X0, K0, KL, G, c2m, st, cs, nch, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(CH) k_berr(ErrArgs ea, const uint8_t* __restrict__ G, const int* __restrict__ c2m,
                                              const BatchMember* __restrict__ st, unsigned* __restrict__ errbits,
                                              unsigned* __restrict__ tqbits, int cs) {
   __shared__ float red[2][CH / 64];
-  const int chunk = blockIdx.x;
-  const int m = c2m[chunk];
-  if (m < 0) return;
-  const BatchMember& S = st[m];
-  if (!S.active) return;
-  const bool want_err = S.adaptive && ea.nk > 0;
-  const bool want_tq = ea.tq && (S.mode == BATCH_RELAX || S.accept == 2);
-  if (!want_err && !want_tq) return;
-  const int g = chunk * CH + threadIdx.x;
-  float err = 0.f, tq = 0.f;
-  if (G[g]) {
-    if (want_err) {
-      float c[MAX_STAGES];
-#pragma unroll
-      for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < ea.nk ? (float)(S.dt * ea.e[j]) : 0.f;
-      float e[3];
-#pragma unroll
-      for (int comp = 0; comp < 3; ++comp) {
-        float a = 0.f;
-        for (int j = 0; j < ea.nk; ++j) a += c[j] * ea.k[j][(size_t)comp * cs + g];
-        e[comp] = a;
-      }
-      err = fmaxf(0.f, sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]));
-    }
-    if (want_tq) {
-      const float t0 = ea.tq[g], t1 = ea.tq[(size_t)cs + g], t2 = ea.tq[2 * (size_t)cs + g];
-      tq = fmaxf(0.f, sqrtf(t0 * t0 + t1 * t1 + t2 * t2));
-    }
-  }
-  const float be = block_max(err, red[0]);
-  const float bt = block_max(tq, red[1]);
-  if (threadIdx.x == 0) {
-    if (want_err) atomicMax(errbits + m, __float_as_uint(be));
-    if (want_tq) atomicMax(tqbits + m, __float_as_uint(bt));
-  }
+  berr_block(ea, G, c2m, st, errbits, tqbits, cs, blockIdx.x, red);
 }
 
 // One thread per member slot.
 __global__ void k_bcontrol(BatchMember* __restrict__ st, unsigned* __restrict__ errbits,
                            unsigned* __restrict__ tqbits, int nslots, CtlArgs c) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nslots) return;
-  BatchMember S = st[m];
-  if (!S.active) return;
-  const float err = __uint_as_float(errbits[m]);
-  const float tq = __uint_as_float(tqbits[m]);
-  errbits[m] = 0u;
-  tqbits[m] = 0u;
-  if (c.prime) {
-    if (S.accept != 2) return;
-    // the initial step of TimeSolver.run / steps (timesolver.py:42-47,
-    // timesolver.cpp:37-47): the sensible step when it is smaller
-    if (S.adaptive && !S.keep_step) {
-      const double sens = tq == 0.f ? S.sdefault : S.sfactor / (double)tq;
-      if (S.dt == 0.0 || S.dt > sens) S.dt = sens;
-    }
-    S.last_torque = tq;
-    S.accept = 1;  // the first combination takes x0 = x and k_0 = rhs(x)
-    bool done = false;
-    if (S.mode == BATCH_STEPS) done = S.target <= 0;
-    else if (S.mode == BATCH_RELAX) done = tq <= S.threshold;
-    else if (!(S.time < S.stop - S.dt)) {
-      S.dt_saved = S.dt;
-      S.dt = S.stop - S.time;
-      S.last = 1;
-    }
-    if (done) {
-      S.active = 0;
-      S.done = 1;
-    }
-    st[m] = S;
-    return;
-  }
-  const double h = S.dt;
-  bool ok = true;
-  if (S.adaptive) {
-    ok = err < S.max_error;
-    const double ratio = S.max_error / (double)err;
-    double corr = ok ? pow(ratio, 1.0 / (c.order2 + 1)) : pow(ratio, 1.0 / (c.order1 + 1));
-    if (isnan(corr)) corr = 1.0;
-    corr *= S.headroom;
-    corr = corr > S.upper ? S.upper : corr;
-    corr = corr < S.lower ? S.lower : corr;
-    S.dt = h * corr;
-    S.last_error = err;
-  }
-  S.accept = ok ? 1 : 0;
-  if (!ok) {
-    ++S.rejected;
-    st[m] = S;
-    return;
-  }
-  S.time = S.time + h;
-  ++S.steps;
-  S.last_torque = tq;
-  bool done = false;
-  if (S.last) {
-    done = true;
-    if (!S.adaptive) S.dt = S.dt_saved;
-  } else if (S.mode == BATCH_STEPS) {
-    done = S.steps >= S.target;
-  } else if (S.mode == BATCH_RELAX) {
-    done = tq <= S.threshold || S.steps >= S.target;  // the target caps a relaxation that stalls
-  } else if (!(S.time < S.stop - S.dt)) {
-    S.dt_saved = S.dt;
-    S.dt = S.stop - S.time;
-    S.last = 1;
-  }
-  if (done) {
-    S.active = 0;
-    S.done = 1;
-  }
-  st[m] = S;
+  bcontrol_member(st, errbits, tqbits, nslots, c, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 long long round_chunk(long long n) { return (n + CH - 1) / CH * CH; }
@@ -429,8 +242,12 @@ void Batch::remove(int id) {
   Slot& sl = slots_[id];
   BatchMember S{};
   HIP_CHECK(hipMemcpyAsync(&S, st_.get() + id, sizeof(S), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
   Ferromagnet* fm = sl.fm;
-  HIP_CHECK(hipMemcpy2DAsync(fm->m.get(), (size_t)fm->N * sizeof(float), X_.get() + sl.off,
+  // after a rejected attempt the state is the attempt's start, X0 (X holds
+  // the rejected attempt's last stage)
+  const float* src = S.accept == 0 ? X0_.get() : X_.get();
+  HIP_CHECK(hipMemcpy2DAsync(fm->m.get(), (size_t)fm->N * sizeof(float), src + sl.off,
                              (size_t)cs_ * sizeof(float), (size_t)fm->N * sizeof(float), 3, hipMemcpyDeviceToDevice,
                              stream));
   BatchMember Z{};
@@ -470,6 +287,7 @@ void Batch::rebuild() {
                            stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   dirty_ = false;
+  pdirty_ = true;
 }
 
 void Batch::eval_rhs(float* out) {
@@ -542,18 +360,257 @@ void Batch::record_step() {
   checkLaunch("k_bcontrol");
 }
 
-void Batch::advance(long long n) {
-  if (n <= 0) return;
-  std::vector<int> fresh;
-  if (dirty_) {
-    rebuild();
+namespace {
+
+// Autotuner of the batch step path: per size class, every candidate path is
+// timed on real steps (wall clock per step over an advance, the advances
+// that capture a graph or build a plan left out) until each has
+// TUNE_SAMPLES samples; the fastest by its best sample is then used for
+// every later advance of that class in the process.
+constexpr int TUNE_SAMPLES = 2;
+constexpr int PERSIST_MAX_GRID = 4096;
+constexpr int TUNE_CANDIDATES = 3;  // passes, persistent (blocks per CU as fit), persistent (1 block per CU)
+const char* const TUNE_NAMES[TUNE_CANDIDATES] = {"passes", "persistent", "persistent_1pcu"};
+
+struct TuneEntry {
+  int count[TUNE_CANDIDATES] = {0, 0, 0};
+  double best[TUNE_CANDIDATES] = {0.0, 0.0, 0.0};  // ms per step
+  bool usable[TUNE_CANDIDATES] = {true, true, true};
+  int choice = -1;
+};
+
+std::mutex tune_mu;
+std::map<std::string, TuneEntry>& tune_table() {
+  static std::map<std::string, TuneEntry> t;
+  return t;
+}
+
+}  // namespace
+
+std::string Batch::size_class() {
+  long long cells = 0;
+  int members = 0;
+  bool has3d = false;
+  int maxline = 0;
+  for (const Slot& sl : slots_) {
+    if (!sl.used) continue;
+    ++members;
+    cells += sl.ncell;
+    has3d = has3d || sl.desc.nz > 1;
+    maxline = std::max({maxline, sl.desc.nx, sl.desc.ny, sl.desc.nz});
+  }
+  auto lg = [](long long v) {
+    int k = 0;
+    while ((1LL << (k + 1)) <= v) ++k;
+    return k;
+  };
+  std::ostringstream o;
+  o << "cells<2^" << lg(cells) + 1 << " members<2^" << lg(members) + 1 << " line<=2^" << lg(maxline) + (maxline & (maxline - 1) ? 1 : 0)
+    << (has3d ? " 3d" : " 2d") << " " << method_;
+  return o.str();
+}
+
+std::string Batch::tune_report() const {
+  std::lock_guard<std::mutex> lk(tune_mu);
+  std::ostringstream o;
+  for (const auto& kv : tune_table()) {
+    const TuneEntry& e = kv.second;
+    o << kv.first << " | " << (e.choice >= 0 ? TUNE_NAMES[e.choice] : "undecided");
+    for (int c = 0; c < TUNE_CANDIDATES; ++c) {
+      o << " | " << TUNE_NAMES[c] << " ";
+      if (!e.usable[c]) o << "n/a";
+      else if (e.count[c] == 0) o << "-";
+      else o << e.best[c] << " ms";
+    }
+    o << "\n";
+  }
+  return o.str();
+}
+
+bool Batch::persist_supported() {
+  if (dirty_) rebuild();
+  for (const Slot& sl : slots_) {
+    if (!sl.used || !sl.demag) continue;
+    for (int a = 0; a < 3; ++a) {
+      const int n = a == 0 ? sl.desc.nx : (a == 1 ? sl.desc.ny : sl.desc.nz);
+      if (demag_fft_length(n) > 256) return false;
+    }
+  }
+  return nmembers() > 0;
+}
+
+int Batch::persist_grid() {
+  if (!persist_supported()) return 0;
+  if (pdirty_) build_persist();
+  return grid_for(persist_blocks_per_cu);
+}
+
+int Batch::grid_for(int blocks_per_cu) const {
+  const int per = blocks_per_cu > 0 ? std::min(blocks_per_cu, pfit_) : pfit_;
+  return std::min({per * pcus_, pmaxnvb_, PERSIST_MAX_GRID});
+}
+
+void Batch::build_persist() {
+  const Tableau& tb = tableau_by_name(method_);
+  const int S = (int)tb.nodes.size();
+  const int nch = (int)(cs_ / CH);
+  const int ns = (int)slots_.size();
+  std::vector<bk::Phase> ph;
+  // the demag passes as phases: the two groups' chains side by side
+  // (members with a z transform: xfwd, yfwd, zpass, yinv, xinv; single
+  // layers: xfwd, ypass2d, xinv, whose x inverse shares the z pass's phase)
+  bk::Phase dph[5];
+  for (auto& d : dph) {
+    d = bk::Phase{};
+    d.kind = bk::PH_DEMAG;
+  }
+  size_t shm = 0;
+  if (demag_) {
+    for (const DemagBatch::PassTable& t : demag_->pass_tables()) {
+      static const int slot0[6] = {0, 1, 2, 3, -1, 4};
+      static const int slot1[6] = {0, -1, -1, -1, 1, 2};
+      const int slot = t.group == 0 ? slot0[t.pass] : slot1[t.pass];
+      if (slot < 0) throw Error(MPA_ERR_RUNTIME, "batch: unexpected demag pass");
+      bk::Phase& d = dph[slot];
+      if (d.npass >= 2) throw Error(MPA_ERR_RUNTIME, "batch: two passes of one group in a phase");
+      d.pass[d.npass++] = bk::PhasePass{t.pass, (int)t.nblocks, t.segs, t.b2s};
+      d.nvb += (int)t.nblocks;
+      shm = std::max(shm, t.shm);
+    }
+  }
+  for (int s = 1; s < S; ++s) {
+    bk::Phase st{};
+    st.kind = bk::PH_STAGE;
+    st.nvb = (nch + 3) / 4;
+    st.sa.nk = s;
+    st.sa.commit = s == 1;
+    for (int j = 0; j < s; ++j) {
+      st.sa.k[j] = K_[j].get();
+      st.sa.a[j] = tb.A[s][j];
+    }
+    ph.push_back(st);
+    for (const bk::Phase& d : dph)
+      if (d.npass > 0) ph.push_back(d);
+    bk::Phase f{};
+    f.kind = bk::PH_FIELD;
+    f.nvb = (int)field_->nblocks();
+    f.out = K_[s].get();
+    ph.push_back(f);
+  }
+  bk::Phase e{};
+  e.kind = bk::PH_ERR;
+  e.nvb = nch;
+  e.ea.nk = S;
+  for (int j = 0; j < S; ++j) {
+    e.ea.k[j] = K_[j].get();
+    e.ea.e[j] = tb.w1[j] - tb.w2[j];
+  }
+  e.ea.tq = K_[S - 1].get();
+  ph.push_back(e);
+  bk::Phase c{};
+  c.kind = bk::PH_CONTROL;
+  c.nvb = (ns + 255) / 256;
+  c.ca = bk::CtlArgs{tb.order1, tb.order2, 0};
+  ph.push_back(c);
+  int maxnvb = 1;
+  for (const bk::Phase& p : ph) maxnvb = std::max(maxnvb, p.nvb);
+  phases_.allocate(ph.size() * sizeof(bk::Phase));
+  HIP_CHECK(hipMemcpy(phases_.get(), ph.data(), ph.size() * sizeof(bk::Phase), hipMemcpyHostToDevice));
+  nph_ = (int)ph.size();
+  pshm_ = shm;
+  // grid: every block resident at once
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&pcus_, hipDeviceAttributeMultiprocessorCount, dev));
+  pfit_ = batch_persist_blocks_per_cu(shm);
+  if (pfit_ < 1) throw Error(MPA_ERR_RUNTIME, "batch: the persistent step kernel does not fit a compute unit");
+  pmaxnvb_ = maxnvb;
+  if (pflags_.n == 0) {
+    pflags_.allocate(PERSIST_MAX_GRID + 1);
+    perr_.allocate(1);
+    HIP_CHECK(hipMemset(pflags_.get(), 0, pflags_.bytes()));
+    HIP_CHECK(hipMemset(perr_.get(), 0, perr_.bytes()));
+    pgen_ = 1;
+  }
+  bk::PersistArgs a{};
+  a.ph = reinterpret_cast<const bk::Phase*>(phases_.get());
+  a.nph = nph_;
+  a.X = X_.get();
+  a.X0 = X0_.get();
+  a.K0 = K_[0].get();
+  a.KL = K_[S - 1].get();
+  a.HD = HD_.get();
+  a.G = G_.get();
+  a.c2m = c2m_.get();
+  a.st = st_.get();
+  a.errbits = errbits_.get();
+  a.tqbits = tqbits_.get();
+  a.ftab = field_->table();
+  a.fb2i = field_->block_members();
+  a.cs = (int)cs_;
+  a.nch = nch;
+  a.nslots = ns;
+  a.flags = pflags_.get();
+  a.go = pflags_.get() + PERSIST_MAX_GRID;
+  a.err = perr_.get();
+  a.spin = persist_spin;
+  pargs_.allocate(sizeof(a));
+  HIP_CHECK(hipMemcpy(pargs_.get(), &a, sizeof(a), hipMemcpyHostToDevice));
+  pdirty_ = false;
+}
+
+void Batch::persist_recover(int err) {
+  // the barrier words back to their initial state
+  HIP_CHECK(hipMemsetAsync(pflags_.get(), 0, pflags_.bytes(), stream));
+  HIP_CHECK(hipMemsetAsync(perr_.get(), 0, perr_.bytes(), stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  pgen_ = 1;
+  if (err < 0) {
+    broken_ = true;
+    throw Error(MPA_ERR_RUNTIME, "batch: a block of the persistent step kernel gave up at a grid barrier; "
+                                 "the members' state is lost");
+  }
+  // Barrier `ordinal` failed: every block finished that phase and none
+  // started the next.  After the controller phase (the last of a step) the
+  // state is that of a completed step; after any other phase the step start
+  // is intact in X0 and K0 (the first phase takes it over), so the step is
+  // retried from there: accept = 0, error norms cleared.
+  const int phase = (err - 1) % nph_;
+  if (phase != nph_ - 1) {
     std::vector<BatchMember> host(slots_.size());
     HIP_CHECK(hipMemcpy(host.data(), st_.get(), host.size() * sizeof(BatchMember), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < slots_.size(); ++i)
-      if (slots_[i].used && host[i].active && host[i].accept == 2) fresh.push_back((int)i);
+    for (size_t i = 0; i < host.size(); ++i)
+      if (slots_[i].used && host[i].active && host[i].accept == 1) host[i].accept = 0;
+    HIP_CHECK(hipMemcpy(st_.get(), host.data(), host.size() * sizeof(BatchMember), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(errbits_.get(), 0, errbits_.bytes()));
+    HIP_CHECK(hipMemset(tqbits_.get(), 0, tqbits_.bytes()));
   }
-  prime(fresh);
-  if (nmembers() == 0) return;
+  throw Error(MPA_ERR_RUNTIME, "batch: a grid barrier of the persistent step kernel timed out at phase " +
+                                   std::to_string(phase) + "; the batch is back at the start of that step");
+}
+
+void Batch::advance_persistent(long long n, int blocks_per_cu) {
+  if (pdirty_) build_persist();
+  const int grid = grid_for(blocks_per_cu);
+  const bk::PersistArgs* pa = reinterpret_cast<const bk::PersistArgs*>(pargs_.get());
+  while (n > 0) {
+    const int k = (int)std::min<long long>(n, 1 << 16);
+    const unsigned long long need = (unsigned long long)k * (unsigned)nph_;
+    if (pgen_ + need >= 0xFFFFFF00ull) {
+      HIP_CHECK(hipMemsetAsync(pflags_.get(), 0, pflags_.bytes(), stream));
+      pgen_ = 1;
+    }
+    batch_persist_launch(pa, k, pgen_, grid, pshm_, stream);
+    pgen_ += (unsigned)need;
+    int err = 0;
+    HIP_CHECK(hipMemcpyAsync(&err, perr_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (err != 0) persist_recover(err);
+    n -= k;
+  }
+}
+
+void Batch::advance_passes(long long n) {
   const bool graph = use_graph && !prof.enabled;
   if (graph && !exec_) {
     HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -574,6 +631,76 @@ void Batch::advance(long long n) {
   }
   HIP_CHECK(hipStreamSynchronize(stream));
   prof.flush_sync(stream);
+}
+
+void Batch::advance(long long n) {
+  if (n <= 0) return;
+  if (broken_) throw Error(MPA_ERR_RUNTIME, "batch: the members' state was lost at a failed grid barrier");
+  std::vector<int> fresh;
+  if (dirty_) {
+    rebuild();
+    std::vector<BatchMember> host(slots_.size());
+    HIP_CHECK(hipMemcpy(host.data(), st_.get(), host.size() * sizeof(BatchMember), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < slots_.size(); ++i)
+      if (slots_[i].used && host[i].active && host[i].accept == 2) fresh.push_back((int)i);
+  }
+  prime(fresh);
+  if (nmembers() == 0) return;
+  // the per-kernel profile times the per-pass launches
+  int mode = prof.enabled ? STEP_PASSES : step_mode;
+  if (mode == STEP_PERSISTENT && !persist_supported())
+    throw Error(MPA_ERR_INVALID, "batch: the persistent step kernel takes FFT lines of up to 256 points");
+  if (mode == STEP_PERSISTENT) {
+    last_path_ = 1;
+    advance_persistent(n, persist_blocks_per_cu);
+    return;
+  }
+  if (mode == STEP_PASSES) {
+    last_path_ = 0;
+    advance_passes(n);
+    return;
+  }
+  // STEP_AUTO
+  const std::string key = size_class();
+  int cand = -1;
+  const bool ps = persist_supported();
+  if (ps && pdirty_) build_persist();
+  // one block per compute unit is a candidate of its own only when it
+  // gives a smaller grid
+  bool usable[TUNE_CANDIDATES] = {true, ps, ps && grid_for(1) < grid_for(0)};
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    TuneEntry& e = tune_table()[key];
+    for (int c = 0; c < TUNE_CANDIDATES; ++c) e.usable[c] = e.usable[c] && usable[c];
+    if (e.choice >= 0 && e.usable[e.choice]) {
+      cand = e.choice;
+    } else {
+      e.choice = -1;
+      for (int c = 0; c < TUNE_CANDIDATES; ++c)
+        if (e.usable[c] && (cand < 0 || e.count[c] < e.count[cand])) cand = c;
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(stream));
+  const bool cold = cand == 0 && use_graph && !exec_;
+  const auto t0 = std::chrono::steady_clock::now();
+  last_path_ = cand == 0 ? 0 : 1;
+  if (cand == 0) advance_passes(n);
+  else advance_persistent(n, cand == 1 ? 0 : 1);
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / (double)n;
+  if (cold) return;
+  std::lock_guard<std::mutex> lk(tune_mu);
+  TuneEntry& e = tune_table()[key];
+  if (e.choice >= 0) return;
+  e.best[cand] = e.count[cand] == 0 ? ms : std::min(e.best[cand], ms);
+  ++e.count[cand];
+  bool done = true;
+  int best = -1;
+  for (int c = 0; c < TUNE_CANDIDATES; ++c) {
+    if (!e.usable[c]) continue;
+    if (e.count[c] < TUNE_SAMPLES) done = false;
+    else if (best < 0 || e.best[c] < e.best[best]) best = c;
+  }
+  if (done && best >= 0) e.choice = best;
 }
 
 std::vector<int> Batch::finished() {

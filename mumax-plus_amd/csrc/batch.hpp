@@ -82,6 +82,26 @@ class Batch {
   int launches_per_step();
 
   bool use_graph = true;
+  // How a batch step runs: STEP_PASSES, one launch per pass (a captured
+  // graph when use_graph); STEP_PERSISTENT, one persistent kernel for all n
+  // steps of an advance; STEP_AUTO, whichever of these measures faster for
+  // the batch's size class (see tune_report).
+  enum StepMode { STEP_PASSES = 0, STEP_PERSISTENT = 1, STEP_AUTO = 2 };
+  int step_mode = STEP_PASSES;
+  int persist_blocks_per_cu = 0;    // resident blocks per compute unit (0: as many as fit)
+  unsigned persist_spin = 1u << 22;  // grid-barrier poll limit (0: fail the first barrier)
+  // Whether the persistent kernel can run this batch (line lengths up to 256
+  // points), and the grid it launches (0 when it cannot).
+  bool persist_supported();
+  int persist_grid();
+  // The step path the last advance took (0 passes, 1 persistent).
+  int last_path() const { return last_path_; }
+  // One line per size class the autotuner has seen: the class, the chosen
+  // path and the measured ms per step of each candidate.
+  std::string tune_report() const;
+  // Rebuild the persistent kernel's tables and arguments before its next
+  // launch (after a change of persist_spin).
+  void invalidate_persist() { pdirty_ = true; }
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the single layers' FFT chain
   hipEvent_t fork_ = nullptr, join_ = nullptr;
@@ -99,6 +119,12 @@ class Batch {
   void rebuild();
   void prime(const std::vector<int>& fresh);
   void record_step();
+  void advance_passes(long long n);
+  void advance_persistent(long long n, int blocks_per_cu);
+  void build_persist();
+  int grid_for(int blocks_per_cu) const;
+  void persist_recover(int err);
+  std::string size_class();
   void eval_rhs(float* out);
   void drop_graph();
   long long alloc_range(long long span);
@@ -120,6 +146,16 @@ class Batch {
   bool dirty_ = true;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
+  // persistent step kernel: phase table, arguments, barrier words
+  DeviceArray<char> phases_, pargs_;
+  DeviceArray<unsigned> pflags_;  // arrival words, then the release word
+  DeviceArray<int> perr_;
+  int nph_ = 0, pfit_ = 0, pcus_ = 0, pmaxnvb_ = 0;
+  size_t pshm_ = 0;
+  unsigned pgen_ = 1;
+  bool pdirty_ = true;
+  bool broken_ = false;
+  int last_path_ = 0;
 };
 
 }  // namespace mpa

@@ -1304,9 +1304,38 @@ int mpa_batch_set_option(mpa_batch* b, const char* key, int value) {
       b->b.prof.flush_sync(b->b.stream);
       b->b.prof.enabled = value != 0;
       if (value) b->b.prof.reset();
+    } else if (k == "step_mode") {
+      if (value < Batch::STEP_PASSES || value > Batch::STEP_AUTO)
+        throw Error(MPA_ERR_INVALID, "step_mode must be 0 (passes), 1 (persistent) or 2 (auto)");
+      b->b.step_mode = value;
+    } else if (k == "persist_blocks_per_cu") {
+      if (value < 0) throw Error(MPA_ERR_INVALID, "persist_blocks_per_cu must be >= 0");
+      b->b.persist_blocks_per_cu = value;
+    } else if (k == "persist_spin") {
+      if (value < 0) throw Error(MPA_ERR_INVALID, "persist_spin must be >= 0");
+      b->b.persist_spin = (unsigned)value;
+      b->b.invalidate_persist();
     } else {
       throw Error(MPA_ERR_INVALID, "unknown batch option '" + k + "'");
     }
+  });
+}
+
+int mpa_batch_step_path(mpa_batch* b, int* last_path, int* persist_grid) {
+  return guard([&] {
+    need(b, "batch");
+    if (last_path) *last_path = b->b.last_path();
+    if (persist_grid) *persist_grid = b->b.nmembers() ? b->b.persist_grid() : 0;
+  });
+}
+
+int mpa_batch_tune_report(mpa_batch* b, int cap, char* text, int* len) {
+  return guard([&] {
+    need(b, "batch");
+    need(len, "len");
+    const std::string r = b->b.tune_report();
+    *len = (int)r.size();
+    if (text && cap > 0) copy_str(r, text, cap);
   });
 }
 

@@ -33,6 +33,8 @@
 // sums of the FP64 octant of the real-space kernel, pre-scaled by
 // -mu0/(Kx Ky Kz) (the unnormalised inverse), and rounded to FP32 once.
 #include "demag.hpp"
+#include "batch_kernels.hpp"
+#include "field_cell.hpp"
 #include "fft.hpp"
 #include "newell.hpp"
 #include "rk.hpp"
@@ -139,8 +141,10 @@ struct XfwdCfg {
   static constexpr int THREADS = PB * C::T;
   // the split reads one element of PB lines per lane group: odd stride
   static constexpr int LP = C::stride(C::target(1));
-  static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
-  static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xfwd LDS");
+  // the lines, then the A offsets of the block's real lines
+  static constexpr size_t DATA = (size_t)PB * LP * sizeof(float2);
+  static constexpr size_t SHM = DATA + 2 * PB * sizeof(long long);
+  static_assert(SHM <= LDS_LIMIT, "k_xfwd LDS");
 };
 
 // Batched launches run every line length with 256 threads per block, so
@@ -152,8 +156,9 @@ struct XfwdCfgB {
   static constexpr int L2PB = ilog2c(PB);
   static constexpr int THREADS = PB * C::T;
   static constexpr int LP = C::stride(C::target(1));
-  static constexpr size_t SHM = (size_t)PB * LP * sizeof(float2);
-  static_assert(THREADS == 256 && SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "batched k_xfwd");
+  static constexpr size_t DATA = (size_t)PB * LP * sizeof(float2);
+  static constexpr size_t SHM = DATA + 2 * PB * sizeof(long long);
+  static_assert(THREADS == 256 && SHM <= LDS_LIMIT, "batched k_xfwd");
 };
 
 // Slot of element i of a staged line (the stores after a transform and the
@@ -221,7 +226,7 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
                                           float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long abase[2 * K::PB];
+  long long* abase = reinterpret_cast<long long*>(lds + K::DATA / sizeof(float2));
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   const int l0 = 2 * K::PB * blk;
   for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
@@ -277,7 +282,7 @@ __device__ __forceinline__ void xfwd_combine_body(const DD& d, const float* __re
                                                   float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long abase[2 * K::PB];
+  long long* abase = reinterpret_cast<long long*>(lds + K::DATA / sizeof(float2));
   const int nrows = d.ny * d.nz;
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   const int r0 = 2 * K::PB * blk;
@@ -354,8 +359,10 @@ struct YCfg {
   // the transposed store reads one element of each of 16 lines per lane
   // group: odd stride
   static constexpr int LP = C::stride(C::target(1));
-  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
-  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yfwd LDS");
+  // the lines, then the B offsets of the block's lines
+  static constexpr size_t DATA = (size_t)LB * LP * sizeof(float2);
+  static constexpr size_t SHM = DATA + LB * sizeof(long long);
+  static_assert(SHM <= LDS_LIMIT, "k_yfwd LDS");
 };
 
 // The y inverse reads B with LB consecutive z per (c, kx) and writes A lines.
@@ -368,8 +375,9 @@ struct YinvCfg {
   // the staging writes 8 lines x 2 consecutive elements per lane group:
   // stride 2 modulo 16
   static constexpr int LP = C::stride(C::target(2));
-  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
-  static_assert(SHM + LB * sizeof(long long) <= LDS_LIMIT, "k_yinv LDS");
+  static constexpr size_t DATA = (size_t)LB * LP * sizeof(float2);
+  static constexpr size_t SHM = DATA + LB * sizeof(long long);
+  static_assert(SHM <= LDS_LIMIT, "k_yinv LDS");
 };
 
 template <int L2>
@@ -379,8 +387,9 @@ struct YCfgB {
   static constexpr int L2LB = ilog2c(LB);
   static constexpr int THREADS = LB * C::T;
   static constexpr int LP = C::stride(C::target(1));
-  static constexpr size_t SHM = (size_t)LB * LP * sizeof(float2);
-  static_assert(THREADS == 256 && SHM + LB * sizeof(long long) <= LDS_LIMIT, "batched k_yfwd");
+  static constexpr size_t DATA = (size_t)LB * LP * sizeof(float2);
+  static constexpr size_t SHM = DATA + LB * sizeof(long long);
+  static_assert(THREADS == 256 && SHM <= LDS_LIMIT, "batched k_yfwd");
 };
 
 // Offset of y-line q = (c, kx, z) in B[c][kx][ky][z] at ky = 0; -1 past the end.
@@ -402,7 +411,7 @@ __device__ __forceinline__ void yfwd_body(const DD& d, const float2* __restrict_
                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long bbase[K::LB];
+  long long* bbase = reinterpret_cast<long long*>(lds + K::DATA / sizeof(float2));
   const int nq = 3 * d.Hx * d.nz;
   const int g = threadIdx.x >> C::L2T, t = threadIdx.x & (C::T - 1);
   const int q0 = blk * K::LB;
@@ -447,7 +456,7 @@ __device__ __forceinline__ void yinv_body(const DD& d, const float2* __restrict_
                                           const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long bbase[K::LB];
+  long long* bbase = reinterpret_cast<long long*>(lds + K::DATA / sizeof(float2));
   const int nq = 3 * d.Hx * d.nz;
   const int q0 = blk * K::LB;
   for (int i = threadIdx.x; i < K::LB; i += K::THREADS) bbase[i] = b_line_base(d, q0 + i);
@@ -667,8 +676,9 @@ struct XinvCfg {
   static constexpr int LP = C::stride(C::fft_target());
   // the staged half spectra and the transform buffers share the LDS
   static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
-  static constexpr size_t SHM = (XB > ZB ? XB : ZB) * sizeof(float2);
-  static_assert(SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "k_xinv LDS");
+  static constexpr size_t DATA = (XB > ZB ? XB : ZB) * sizeof(float2);
+  static constexpr size_t SHM = DATA + 2 * PB * sizeof(long long);  // then the A offsets of the lines
+  static_assert(SHM <= LDS_LIMIT, "k_xinv LDS");
 };
 
 template <int L2>
@@ -680,8 +690,9 @@ struct XinvCfgB {
   static constexpr int XS = C::H;
   static constexpr int LP = C::stride(C::fft_target());
   static constexpr size_t XB = (size_t)2 * PB * XS, ZB = (size_t)PB * LP;
-  static constexpr size_t SHM = (XB > ZB ? XB : ZB) * sizeof(float2);
-  static_assert(THREADS == 256 && SHM + 2 * PB * sizeof(long long) <= LDS_LIMIT, "batched k_xinv");
+  static constexpr size_t DATA = (XB > ZB ? XB : ZB) * sizeof(float2);
+  static constexpr size_t SHM = DATA + 2 * PB * sizeof(long long);
+  static_assert(THREADS == 256 && SHM <= LDS_LIMIT, "batched k_xinv");
 };
 
 // PB line pairs per block; C2R of two lines at a time.
@@ -690,7 +701,7 @@ __device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict_
                                           float* __restrict__ h, const float2* __restrict__ tw, int blk) {
   using C = Line<L2>;
   extern __shared__ float2 lds[];
-  __shared__ long long abase[2 * K::PB];
+  long long* abase = reinterpret_cast<long long*>(lds + K::DATA / sizeof(float2));
   constexpr int L2NL = K::L2PB + 1;
   constexpr int NL = 2 * K::PB;
   float2* X = lds;
@@ -920,75 +931,243 @@ struct RunXinv {
 // Batched passes: one launch of 256-thread blocks per pass over magnets of
 // every size; a block finds its magnet through b2s and runs that magnet's
 // line length.
-#define MPA_L2_CASES(l2, CALL)                                       \
-  switch (l2) {                                                       \
-    case 0: CALL(0); break;                                           \
-    case 3: CALL(3); break;                                           \
-    case 4: CALL(4); break;                                           \
-    case 5: CALL(5); break;                                           \
-    case 6: CALL(6); break;                                           \
-    case 7: CALL(7); break;                                           \
-    case 8: CALL(8); break;                                           \
-    case 9: CALL(9); break;                                           \
-    case 10: CALL(10); break;                                         \
-    case 11: CALL(11); break;                                         \
-    default: break;                                                   \
+// Block b of batched pass PASS (0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d,
+// 5 xinv): the member b2s[b] at its own line length.  X: packed state (pass
+// 0), H: packed field (pass 5).
+// Lines of up to 2^PERSIST_MAXL2 points in the persistent step kernel (its
+// register budget is that of its largest body).
+constexpr int PERSIST_MAXL2 = 8;
+
+template <int PASS, int LMAX = MAXL2>
+__device__ __forceinline__ void batch_pass_block(const BSeg* __restrict__ segs, const int* __restrict__ b2s, int b,
+                                                 const float* __restrict__ X, float* __restrict__ H) {
+  const BSeg s = segs[b2s[b]];
+  const int lb = b - s.first;
+#define MPA_L2_CASES(l2, CALL)                       \
+  switch (l2) {                                       \
+    case 0: CALL(0); break;                           \
+    case 3: CALL(3); break;                           \
+    case 4: CALL(4); break;                           \
+    case 5: CALL(5); break;                           \
+    case 6: CALL(6); break;                           \
+    case 7: CALL(7); break;                           \
+    case 8: CALL(8); break;                           \
+    case 9: if constexpr (LMAX >= 9) CALL(9); break;  \
+    case 10: if constexpr (LMAX >= 10) CALL(10); break; \
+    case 11: if constexpr (LMAX >= 11) CALL(11); break; \
+    default: break;                                   \
   }
+  if constexpr (PASS == 0) {
+#define MPA_CALL(L) xfwd_body<L, XfwdCfgB<L>>(s.d, X + s.off, s.msat, nullptr, PS{nullptr, 0.f}, s.geom, s.A, s.tw[0], lb)
+    MPA_L2_CASES(s.d.l2x, MPA_CALL)
+#undef MPA_CALL
+  } else if constexpr (PASS == 1) {
+#define MPA_CALL(L) yfwd_body<L, YCfgB<L>>(s.d, s.A, s.B, s.tw[1], lb)
+    MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+  } else if constexpr (PASS == 2) {
+#define MPA_CALL(L) zpass_body<L, PointCfgB<L>>(s.d, s.B, s.T, s.tw[2], lb)
+    MPA_L2_CASES(s.d.l2z, MPA_CALL)
+#undef MPA_CALL
+  } else if constexpr (PASS == 3) {
+#define MPA_CALL(L) yinv_body<L, YCfgB<L>>(s.d, s.B, s.A, s.tw[1], lb)
+    MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+  } else if constexpr (PASS == 4) {
+#define MPA_CALL(L) ypass2d_body<L, PointCfgB<L>>(s.d, s.A, s.T, s.tw[1], lb)
+    MPA_L2_CASES(s.d.l2y, MPA_CALL)
+#undef MPA_CALL
+  } else {
+#define MPA_CALL(L) xinv_body<L, XinvCfgB<L>>(s.d, s.A, s.geom, H + s.off, s.tw[0], lb)
+    MPA_L2_CASES(s.d.l2x, MPA_CALL)
+#undef MPA_CALL
+  }
+}
+#undef MPA_L2_CASES
 
 __global__ void __launch_bounds__(256) k_xfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s,
                                                 const float* __restrict__ X) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) xfwd_body<L, XfwdCfgB<L>>(s.d, X + s.off, s.msat, nullptr, PS{nullptr, 0.f}, s.geom, s.A, s.tw[0], lb)
-  MPA_L2_CASES(s.d.l2x, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<0>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), X, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_yfwd_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) yfwd_body<L, YCfgB<L>>(s.d, s.A, s.B, s.tw[1], lb)
-  MPA_L2_CASES(s.d.l2y, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<1>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_zpass_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) zpass_body<L, PointCfgB<L>>(s.d, s.B, s.T, s.tw[2], lb)
-  MPA_L2_CASES(s.d.l2z, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<2>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_yinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) yinv_body<L, YCfgB<L>>(s.d, s.B, s.A, s.tw[1], lb)
-  MPA_L2_CASES(s.d.l2y, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<3>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_ypass2d_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) ypass2d_body<L, PointCfgB<L>>(s.d, s.A, s.T, s.tw[1], lb)
-  MPA_L2_CASES(s.d.l2y, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<4>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(256) k_xinv_b(const BSeg* __restrict__ segs, const int* __restrict__ b2s,
                                                 float* __restrict__ H) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const BSeg s = segs[b2s[b]];
-  const int lb = b - s.first;
-#define MPA_CALL(L) xinv_body<L, XinvCfgB<L>>(s.d, s.A, s.geom, H + s.off, s.tw[0], lb)
-  MPA_L2_CASES(s.d.l2x, MPA_CALL)
-#undef MPA_CALL
+  batch_pass_block<5>(segs, b2s, xcd_remap(blockIdx.x, gridDim.x), nullptr, H);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batch step: every launch of the per-pass step (stage
+// combination, demag passes, field, error norms, controller) is a phase of
+// one kernel; the phases' blocks are shared out over a grid that is resident
+// as a whole (a cooperative launch), and a grid barrier separates phases.
+
+__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier number `gen` (generations increase by one per barrier and
+// never repeat between launches).  Every block publishes its writes (agent-
+// scope release by every thread), then stores gen into its arrival word;
+// block 0 waits for every arrival word and stores gen into the release word,
+// which the other blocks wait for.  Block 0 is the only one that decides:
+// when an arrival does not come within a.spin polls it writes the error word
+// instead of releasing, so either every block passes barrier gen or none
+// does.  A block that waits far longer than that for the release (64 a.spin
+// polls) gives up too.  Returns false when the kernel must end.
+__device__ bool grid_barrier(const bk::PersistArgs& a, unsigned gen, int ordinal, int& fail) {
+  if (threadIdx.x == 0) fail = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    bool ok = a.spin != 0;
+    for (int i = 1 + (int)threadIdx.x; ok && i < (int)gridDim.x; i += (int)blockDim.x) {
+      unsigned n = 0;
+      while (ld_agent(a.flags + i) != gen) {
+        if (++n > a.spin || ld_agent(a.err) != 0) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (!ok) fail = 1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (fail) {
+        int expect = 0;
+        __hip_atomic_compare_exchange_strong(a.err, &expect, 1 + ordinal, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(a.go, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  } else if (threadIdx.x == 0) {
+    __hip_atomic_store(a.flags + blockIdx.x, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long limit = 64ull * (a.spin ? a.spin : 1u);
+    unsigned long long n = 0;
+    while (ld_agent(a.go) != gen) {
+      if (ld_agent(a.err) != 0) {
+        fail = 1;
+        break;
+      }
+      if (++n > limit) {
+        int expect = 0;
+        __hip_atomic_compare_exchange_strong(a.err, &expect, -1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  const bool ok = fail == 0;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+
+// Each phase body is its own (not inlined) function, so the kernel's
+// registers are those of its largest body, not of all of them inlined into
+// one loop.
+// Arguments of a called function arrive in vector registers; these put the
+// block-uniform ones back into scalar registers, so the body's member table
+// loads and address arithmetic stay scalar as in the per-pass kernels.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T* uni(T* p) {
+  const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+
+template <int PASS>
+__device__ __noinline__ void persist_pass(const BSeg* segs, const int* b2s, int b, const float* X, float* H) {
+  batch_pass_block<PASS, PERSIST_MAXL2>(uni(segs), uni(b2s), uni(b), uni(X), uni(H));
+}
+
+__device__ __forceinline__ void persist_demag(const bk::PhasePass& P, int b, const float* X, float* H) {
+  const BSeg* segs = static_cast<const BSeg*>(P.segs);
+  switch (P.pass) {
+    case 0: persist_pass<0>(segs, P.b2s, b, X, H); break;
+    case 1: persist_pass<1>(segs, P.b2s, b, X, H); break;
+    case 2: persist_pass<2>(segs, P.b2s, b, X, H); break;
+    case 3: persist_pass<3>(segs, P.b2s, b, X, H); break;
+    case 4: persist_pass<4>(segs, P.b2s, b, X, H); break;
+    default: persist_pass<5>(segs, P.b2s, b, X, H); break;
+  }
+}
+
+__device__ __noinline__ void persist_stage(const bk::PersistArgs* a, const bk::StageArgs* sa, int v) {
+  a = uni(a);
+  bk::bstage_block(*uni(sa), a->X, a->X0, a->K0, a->KL, a->G, a->c2m, a->st, a->cs, a->nch, uni(v));
+}
+
+__device__ __noinline__ void persist_field(const bk::PersistArgs* a, float* out, int v) {
+  a = uni(a);
+  field_uniform_b_block(static_cast<const FBInst*>(a->ftab), a->fb2i, a->X, a->HD, uni(out), a->cs, uni(v));
+}
+
+__device__ __noinline__ void persist_err(const bk::PersistArgs* a, const bk::ErrArgs* ea, int v,
+                                         float (*red)[bk::CH / 64]) {
+  a = uni(a);
+  bk::berr_block(*uni(ea), a->G, a->c2m, a->st, a->errbits, a->tqbits, a->cs, uni(v), red);
+}
+
+__device__ __noinline__ void persist_control(const bk::PersistArgs* a, const bk::CtlArgs* ca, int v) {
+  a = uni(a);
+  bk::bcontrol_member(a->st, a->errbits, a->tqbits, a->nslots, *uni(ca), uni(v) * 256 + (int)threadIdx.x);
+}
+
+// The launch's arguments are in device memory (pa), so the called phase
+// bodies read them through a pointer instead of a private copy.
+__global__ void __launch_bounds__(256, 3) k_bstep_persist(const bk::PersistArgs* __restrict__ pa, int nsteps,
+                                                          unsigned gen0) {
+  __shared__ float red[2][bk::CH / 64];
+  __shared__ int fail;
+  const bk::PersistArgs& a = *pa;
+  int ordinal = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    for (int p = 0; p < a.nph; ++p) {
+      const bk::Phase& P = a.ph[p];
+      const int kind = P.kind, nvb = P.nvb;
+      for (int v = blockIdx.x; v < nvb; v += gridDim.x) {
+        switch (kind) {
+          case bk::PH_STAGE: persist_stage(pa, &P.sa, v); break;
+          case bk::PH_DEMAG:
+            if (v < P.pass[0].nblocks) persist_demag(P.pass[0], v, a.X, a.HD);
+            else persist_demag(P.pass[1], v - P.pass[0].nblocks, a.X, a.HD);
+            break;
+          case bk::PH_FIELD: persist_field(pa, P.out, v); break;
+          case bk::PH_ERR: persist_err(pa, &P.ea, v, red); break;
+          default: persist_control(pa, &P.ca, v); break;
+        }
+        __syncthreads();  // the next virtual block reuses the LDS
+      }
+      if (!grid_barrier(a, gen0 + (unsigned)ordinal, ordinal, fail)) return;
+      ++ordinal;
+    }
+  }
 }
 
 // Blocks a magnet needs in a batched pass, and the pass's LDS bytes at its
@@ -1318,6 +1497,26 @@ void DemagBatch::exec(const float* X, float* H, hipStream_t s, Profiler* prof, h
     HIP_CHECK(hipEventRecord(join, s2));
     HIP_CHECK(hipStreamWaitEvent(s, join, 0));
   }
+}
+
+std::vector<DemagBatch::PassTable> DemagBatch::pass_tables() const {
+  std::vector<PassTable> out;
+  for (const Launch& L : launches_) out.push_back({L.pass, L.group, L.nblocks, L.shm, L.segs.get(), L.b2s.get()});
+  return out;
+}
+
+int batch_persist_blocks_per_cu(size_t shm) {
+  int nb = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_bstep_persist), 256,
+                                                         shm));
+  return nb;
+}
+
+void batch_persist_launch(const bk::PersistArgs* a, int nsteps, unsigned gen0, int grid, size_t shm, hipStream_t s) {
+  const bk::PersistArgs* args = a;
+  void* params[] = {&args, &nsteps, &gen0};
+  HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_bstep_persist), dim3((unsigned)grid),
+                                       dim3(256), params, (unsigned)shm, s));
 }
 
 double DemagBatch::algorithmic_bytes() const {

@@ -113,6 +113,17 @@ class DemagBatch {
             hipEvent_t fork = nullptr, hipEvent_t join = nullptr) const;
   double algorithmic_bytes() const;
   int nlaunches() const { return (int)launches_.size(); }
+  // The launches' block tables, in launch order (each group's passes in
+  // pass order), for the persistent step kernel.
+  struct PassTable {
+    int pass;   // 0 xfwd, 1 yfwd, 2 zpass, 3 yinv, 4 ypass2d, 5 xinv
+    int group;  // 0: members with a z transform, 1: single layers
+    unsigned nblocks;
+    size_t shm;
+    const void* segs;
+    const int* b2s;
+  };
+  std::vector<PassTable> pass_tables() const;
 
  private:
   struct Launch {

@@ -244,6 +244,32 @@ __device__ __forceinline__ void field_uniform_cell(const FMDesc& d, const FMUnif
 }
 
 
+// A member of a batched field launch: its description, coefficients, first
+// cell in the packed arrays and first block of the launch.
+struct FBInst {
+  FMDesc d;
+  FMUniform u;
+  int off;
+  int first;
+  int mode;
+};
+
+// Block b of a batched field launch: 256 consecutive cells of the member
+// b2i[b] (every member starts on a multiple of 256 cells of the packed
+// arrays X, HD and OUT, whose component stride is cs).
+__device__ __forceinline__ void field_uniform_b_block(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
+                                                      const float* __restrict__ X, const float* __restrict__ HD,
+                                                      float* __restrict__ OUT, int cs, int b) {
+  const FBInst& I = tab[b2i[b]];
+  const int j = (b - I.first) * 256 + threadIdx.x;
+  const int nx = I.d.nx, ny = I.d.ny;
+  if (j >= I.d.ncell) return;
+  const int x = j % nx, yz = j / nx, y = yz % ny, z = yz / ny;
+  const float* hd = HD + I.off;
+  field_uniform_cell(I.d, I.u, X + I.off, [&](int i) { return load3(hd, i, cs); }, OUT + I.off, I.mode, x, y, z, cs);
+}
+
+
 // Every parameter uniform and only the terms the uniform path has.
 bool all_uniform(const FMDesc& d);
 // Host-side coefficients of the uniform path.

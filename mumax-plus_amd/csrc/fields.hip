@@ -583,29 +583,12 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
   }
 }
 
-// A member of a batched launch: its description, coefficients, first cell
-// in the packed arrays and first block of the launch.
-struct FBInst {
-  FMDesc d;
-  FMUniform u;
-  int off;
-  int first;
-  int mode;
-};
-
 // Batched uniform path: 256 consecutive cells of one magnet per block (every
 // magnet starts on a multiple of 256 cells of the packed arrays).
 __global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
                                                            const float* __restrict__ X, const float* __restrict__ HD,
                                                            float* __restrict__ OUT, int cs) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const FBInst& I = tab[b2i[b]];
-  const int j = (b - I.first) * BLOCK + threadIdx.x;
-  const int nx = I.d.nx, ny = I.d.ny;
-  if (j >= I.d.ncell) return;
-  const int x = j % nx, yz = j / nx, y = yz % ny, z = yz / ny;
-  const float* hd = HD + I.off;
-  field_uniform_cell(I.d, I.u, X + I.off, [&](int i) { return load3(hd, i, cs); }, OUT + I.off, I.mode, x, y, z, cs);
+  field_uniform_b_block(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Every parameter the per-cell kernel reads is uniform.

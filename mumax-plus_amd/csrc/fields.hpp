@@ -181,6 +181,9 @@ class FieldBatch {
   // member has TERM_DEMAG, for every member.
   void launch(const float* X, const float* HD, float* OUT, hipStream_t s) const;
   unsigned nblocks() const { return nblocks_; }
+  // The device member table (FBInst entries) and block -> member map.
+  const void* table() const { return tab_.get(); }
+  const int* block_members() const { return b2i_.get(); }
 
  private:
   int cs_;

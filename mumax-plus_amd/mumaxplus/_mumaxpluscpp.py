@@ -178,6 +178,8 @@ _SIGS = {
                         _ct.POINTER(_cdouble), _ct.POINTER(_cint)], _cint),
     "mpa_batch_set_option": ([_P, _c_char_p, _cint], _cint),
     "mpa_batch_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
+    "mpa_batch_step_path": ([_P, _ct.POINTER(_cint), _ct.POINTER(_cint)], _cint),
+    "mpa_batch_tune_report": ([_P, _cint, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -1529,6 +1531,18 @@ class Batch:
 
     def profile_report(self):
         return _profile_report(_lib.mpa_batch_profile_report, self._h)
+
+    def step_path(self):
+        path, grid = _cint(), _cint()
+        _check(_lib.mpa_batch_step_path(self._h, _ct.byref(path), _ct.byref(grid)))
+        return {"last_path": ("passes", "persistent")[path.value], "persist_grid": grid.value}
+
+    def tune_report(self):
+        n = _cint()
+        _check(_lib.mpa_batch_tune_report(self._h, 0, None, _ct.byref(n)))
+        buf = _ct.create_string_buffer(n.value + 1)
+        _check(_lib.mpa_batch_tune_report(self._h, n.value + 1, buf, _ct.byref(n)))
+        return buf.value.decode()
 
 
 class DemagPlan:

@@ -118,6 +118,44 @@ class Batch:
         self._impl.set_option("graph", bool(value))
         self._graph = bool(value)
 
+    _STEP_MODES = ("passes", "persistent", "auto")
+
+    @property
+    def step_mode(self):
+        """How a batch step runs: "passes" (one launch per pass, replayed
+        from a captured graph), "persistent" (one resident kernel for all the
+        steps of an advance, a grid barrier between passes; FFT lines of up
+        to 256 points) or "auto" (whichever of these the autotuner measures
+        faster for the batch's size class, see :meth:`tune_report`)."""
+        return getattr(self, "_step_mode", "passes")
+
+    @step_mode.setter
+    def step_mode(self, value):
+        if value not in self._STEP_MODES:
+            raise ValueError("step_mode must be one of " + ", ".join(self._STEP_MODES))
+        self._impl.set_option("step_mode", self._STEP_MODES.index(value))
+        self._step_mode = value
+
+    def step_path(self):
+        """{"last_path": the path the last advance took, "persist_grid": the
+        persistent kernel's grid for the current members (0: cannot run)}."""
+        return self._impl.step_path()
+
+    def tune_report(self):
+        """The step-path autotuner's record in this process (every batch's
+        size classes): {size class: {"choice": path or None, candidate: best
+        ms per step, or None when not measured}}."""
+        text = self._impl.tune_report()
+        out = {}
+        for line in text.splitlines():
+            parts = [p.strip() for p in line.split("|")]
+            entry = {"choice": None if parts[1] == "undecided" else parts[1]}
+            for cand in parts[2:]:
+                name, _, val = cand.partition(" ")
+                entry[name] = float(val.split()[0]) if val.endswith("ms") else None
+            out[parts[0]] = entry
+        return out
+
     def set_profiling(self, enable):
         self._impl.set_option("profiling", bool(enable))
 
