@@ -302,15 +302,28 @@ def measure_batch(args, steps):
     wb.run_until_done(poll=2)
     for _, mg in warm:
         _, _ = mg, mg.magnetization.average()
+    # the step path: under "auto" the autotuner first times the candidates
+    # on the full membership (its size class), then the timed run uses its
+    # choice
+    mode = args.batch_step_mode
+    prof = [build_batch_member(sp) for sp in specs]
+    pb = mp.Batch(span)
+    pb.step_mode = mode
+    for _, mg in prof:
+        pb.add(mg, steps=10 ** 9)
+    for _ in range(12 if mode == "auto" else 1):
+        pb.advance(5)
     # batched path
     members = [build_batch_member(sp) for sp in specs]
     batch = mp.Batch(span)
+    batch.step_mode = mode
     t0 = time.perf_counter()
     for _, mg in members:
         if batch.add(mg, steps=steps) is None:
             raise RuntimeError("batch capacity too small")
     batch.run_until_done(poll=steps)
     wall = time.perf_counter() - t0
+    path = batch.step_path()
     # plain sequential path: the same instances one World at a time
     wall_seq, dm = None, None
     if not args.no_plain:
@@ -323,12 +336,8 @@ def measure_batch(args, steps):
         wall_seq = time.perf_counter() - t0
         dm = max(float(np.abs(a.magnetization.get() - b.magnetization.get()).max())
                  for (_, a), (_, b) in zip(members, plain))
-    # steady-state step time and per-kernel roofline on the full membership
-    prof = [build_batch_member(sp) for sp in specs]
-    pb = mp.Batch(span)
-    for _, mg in prof:
-        pb.add(mg, steps=10 ** 9)
-    pb.advance(5)
+    # steady-state step time on the full membership (the chosen path), and
+    # the per-kernel roofline of the per-pass launches
     t0 = time.perf_counter()
     pb.advance(args.profile_steps)
     t_step = (time.perf_counter() - t0) / args.profile_steps
@@ -375,6 +384,8 @@ def measure_batch(args, steps):
                               "frac": b_step / t_step / HBM_PEAK,
                               "launches": info["launches_per_step"]}},
         "kernels": kernels,
+        "step_path": {"mode": mode, "taken": path["last_path"], "persist_grid": path["persist_grid"],
+                      "tuning": pb.tune_report() if mode == "auto" else None},
         "cpu_baseline": cpu,
     }
     return out
@@ -438,11 +449,11 @@ def measure_serve(args):
     for r in warm:
         r.duration = 2e-12 if r.kind == "switch" else None
         r.threshold = 1e11 if r.kind == "relax" else None
-    serving.Server(SERVE_CAP).serve(warm, realtime=False)
+    serving.Server(SERVE_CAP, step_mode=args.batch_step_mode).serve(warm, realtime=False)
     for r in warm[:1]:
         r.solve_plain()
     # served stream
-    server = serving.Server(SERVE_CAP, poll_steps=16)
+    server = serving.Server(SERVE_CAP, poll_steps=16, step_mode=args.batch_step_mode)
     t0 = time.perf_counter()
     res = server.serve(reqs, realtime=True)
     wall = time.perf_counter() - t0
@@ -468,7 +479,8 @@ def measure_serve(args):
                                     - np.array(byid[r.rid].magnet.magnetization.average())).max()))
     wall_plain = time.perf_counter() - t0
     # the same first requests served, for a like-for-like latency
-    res_sub = serving.Server(SERVE_CAP, poll_steps=16).serve(sub, realtime=True) if sub else []
+    res_sub = serving.Server(SERVE_CAP, poll_steps=16, step_mode=args.batch_step_mode).serve(sub, realtime=True) \
+        if sub else []
     sub_prof = reqs[:SERVE_PLAIN]
     # per-kernel roofline of a profiled pass over the first requests
     pserver = serving.Server(SERVE_CAP, poll_steps=16)
@@ -508,6 +520,8 @@ def measure_serve(args):
         "latency_s": {"p50": _pct(lat, 50), "p99": _pct(lat, 99), "max": max(lat)},
         "wall_s": wall,
         "server": dict(server.stats, ms_per_batch_step=1e3 * server.stats["advance_s"] / max(server.stats["batch_steps"], 1)),
+        "step_path": {"mode": args.batch_step_mode,
+                      "tuning": server.batch.tune_report() if args.batch_step_mode == "auto" else None},
         "plain": None if not sub else {"requests": len(sub), "value": len(sub) / wall_plain, "unit": "solves/s",
                   "mean_service_s": float(np.mean(service)),
                   "saturated_value": 1.0 / float(np.mean(service)),
@@ -601,6 +615,8 @@ def main():
     ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve"), default="ferromagnet")
     ap.add_argument("--no-sub", action="store_true",
                     help="default workload only: leave out the batched and served sub-measurements")
+    ap.add_argument("--batch-step-mode", choices=("passes", "persistent", "auto"), default="passes",
+                    help="step path of the batched and served workloads (Batch.step_mode)")
     ap.add_argument("--unfused", action="store_true",
                     help="run the stage combination as its own launch (world option fuse_stages 0), e.g. to "
                          "calibrate counters on a kernel of known traffic")

@@ -401,7 +401,7 @@ int mpa_batch_info(mpa_batch* b, long long* capacity, long long* free_cells, int
  * "step_mode": 0 one launch per pass (default), 1 one persistent kernel for all n steps of an
  * advance (FFT lines up to 256 points), 2 whichever of these the autotuner measures faster for the
  * batch's size class; "persist_blocks_per_cu": resident blocks per compute unit of the persistent
- * kernel (0: as many as fit); "persist_spin": the persistent kernel's grid-barrier poll limit
+ * kernel (0: as many as fit); "persist_max_grid": a cap on its grid (0: none); "persist_spin": the persistent kernel's grid-barrier poll limit
  * (0: its first barrier fails, for tests of the error path). */
 int mpa_batch_set_option(mpa_batch* b, const char* key, int value);
 /* The step path of the last advance (0 per-pass launches, 1 persistent kernel) and the grid of the

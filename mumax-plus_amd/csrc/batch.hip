@@ -369,13 +369,19 @@ namespace {
 // every later advance of that class in the process.
 constexpr int TUNE_SAMPLES = 2;
 constexpr int PERSIST_MAX_GRID = 4096;
-constexpr int TUNE_CANDIDATES = 3;  // passes, persistent (blocks per CU as fit), persistent (1 block per CU)
-const char* const TUNE_NAMES[TUNE_CANDIDATES] = {"passes", "persistent", "persistent_1pcu"};
+// Candidates: the per-pass launches, and the persistent kernel with as many
+// blocks per compute unit as fit, with one per compute unit, and with 64
+// blocks (a grid barrier costs more the more blocks take part, a phase's
+// work runs faster the more blocks share it).
+constexpr int TUNE_CANDIDATES = 4;
+const char* const TUNE_NAMES[TUNE_CANDIDATES] = {"passes", "persistent", "persistent_1pcu", "persistent_64"};
+const int TUNE_BPC[TUNE_CANDIDATES] = {0, 0, 1, 1};
+const int TUNE_CAP[TUNE_CANDIDATES] = {0, 0, 0, 64};
 
 struct TuneEntry {
-  int count[TUNE_CANDIDATES] = {0, 0, 0};
-  double best[TUNE_CANDIDATES] = {0.0, 0.0, 0.0};  // ms per step
-  bool usable[TUNE_CANDIDATES] = {true, true, true};
+  int count[TUNE_CANDIDATES] = {0, 0, 0, 0};
+  double best[TUNE_CANDIDATES] = {0.0, 0.0, 0.0, 0.0};  // ms per step
+  bool usable[TUNE_CANDIDATES] = {true, true, true, true};
   int choice = -1;
 };
 
@@ -442,12 +448,13 @@ bool Batch::persist_supported() {
 int Batch::persist_grid() {
   if (!persist_supported()) return 0;
   if (pdirty_) build_persist();
-  return grid_for(persist_blocks_per_cu);
+  return grid_for(persist_blocks_per_cu, persist_max_grid);
 }
 
-int Batch::grid_for(int blocks_per_cu) const {
+int Batch::grid_for(int blocks_per_cu, int cap) const {
   const int per = blocks_per_cu > 0 ? std::min(blocks_per_cu, pfit_) : pfit_;
-  return std::min({per * pcus_, pmaxnvb_, PERSIST_MAX_GRID});
+  const int g = std::min({per * pcus_, pmaxnvb_, PERSIST_MAX_GRID});
+  return cap > 0 ? std::min(g, cap) : g;
 }
 
 void Batch::build_persist() {
@@ -589,9 +596,9 @@ void Batch::persist_recover(int err) {
                                    std::to_string(phase) + "; the batch is back at the start of that step");
 }
 
-void Batch::advance_persistent(long long n, int blocks_per_cu) {
+void Batch::advance_persistent(long long n, int blocks_per_cu, int cap) {
   if (pdirty_) build_persist();
-  const int grid = grid_for(blocks_per_cu);
+  const int grid = grid_for(blocks_per_cu, cap);
   const bk::PersistArgs* pa = reinterpret_cast<const bk::PersistArgs*>(pargs_.get());
   while (n > 0) {
     const int k = (int)std::min<long long>(n, 1 << 16);
@@ -652,7 +659,7 @@ void Batch::advance(long long n) {
     throw Error(MPA_ERR_INVALID, "batch: the persistent step kernel takes FFT lines of up to 256 points");
   if (mode == STEP_PERSISTENT) {
     last_path_ = 1;
-    advance_persistent(n, persist_blocks_per_cu);
+    advance_persistent(n, persist_blocks_per_cu, persist_max_grid);
     return;
   }
   if (mode == STEP_PASSES) {
@@ -667,7 +674,7 @@ void Batch::advance(long long n) {
   if (ps && pdirty_) build_persist();
   // one block per compute unit is a candidate of its own only when it
   // gives a smaller grid
-  bool usable[TUNE_CANDIDATES] = {true, ps, ps && grid_for(1) < grid_for(0)};
+  bool usable[TUNE_CANDIDATES] = {true, ps, ps && grid_for(1, 0) < grid_for(0, 0), ps && 64 < grid_for(1, 0)};
   {
     std::lock_guard<std::mutex> lk(tune_mu);
     TuneEntry& e = tune_table()[key];
@@ -685,7 +692,7 @@ void Batch::advance(long long n) {
   const auto t0 = std::chrono::steady_clock::now();
   last_path_ = cand == 0 ? 0 : 1;
   if (cand == 0) advance_passes(n);
-  else advance_persistent(n, cand == 1 ? 0 : 1);
+  else advance_persistent(n, TUNE_BPC[cand], TUNE_CAP[cand]);
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / (double)n;
   if (cold) return;
   std::lock_guard<std::mutex> lk(tune_mu);

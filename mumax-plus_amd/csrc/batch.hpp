@@ -89,6 +89,7 @@ class Batch {
   enum StepMode { STEP_PASSES = 0, STEP_PERSISTENT = 1, STEP_AUTO = 2 };
   int step_mode = STEP_PASSES;
   int persist_blocks_per_cu = 0;    // resident blocks per compute unit (0: as many as fit)
+  int persist_max_grid = 0;         // cap on the persistent grid (0: none)
   unsigned persist_spin = 1u << 22;  // grid-barrier poll limit (0: fail the first barrier)
   // Whether the persistent kernel can run this batch (line lengths up to 256
   // points), and the grid it launches (0 when it cannot).
@@ -120,9 +121,9 @@ class Batch {
   void prime(const std::vector<int>& fresh);
   void record_step();
   void advance_passes(long long n);
-  void advance_persistent(long long n, int blocks_per_cu);
+  void advance_persistent(long long n, int blocks_per_cu, int cap);
   void build_persist();
-  int grid_for(int blocks_per_cu) const;
+  int grid_for(int blocks_per_cu, int cap) const;
   void persist_recover(int err);
   std::string size_class();
   void eval_rhs(float* out);

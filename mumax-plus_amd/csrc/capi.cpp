@@ -1311,6 +1311,9 @@ int mpa_batch_set_option(mpa_batch* b, const char* key, int value) {
     } else if (k == "persist_blocks_per_cu") {
       if (value < 0) throw Error(MPA_ERR_INVALID, "persist_blocks_per_cu must be >= 0");
       b->b.persist_blocks_per_cu = value;
+    } else if (k == "persist_max_grid") {
+      if (value < 0) throw Error(MPA_ERR_INVALID, "persist_max_grid must be >= 0");
+      b->b.persist_max_grid = value;
     } else if (k == "persist_spin") {
       if (value < 0) throw Error(MPA_ERR_INVALID, "persist_spin must be >= 0");
       b->b.persist_spin = (unsigned)value;

@@ -1025,17 +1025,22 @@ __device__ __forceinline__ int ld_agent(const int* p) {
 }
 
 // Grid barrier number `gen` (generations increase by one per barrier and
-// never repeat between launches).  Every block publishes its writes (agent-
-// scope release by every thread), then stores gen into its arrival word;
-// block 0 waits for every arrival word and stores gen into the release word,
-// which the other blocks wait for.  Block 0 is the only one that decides:
-// when an arrival does not come within a.spin polls it writes the error word
-// instead of releasing, so either every block passes barrier gen or none
-// does.  A block that waits far longer than that for the release (64 a.spin
-// polls) gives up too.  Returns false when the kernel must end.
+// never repeat between launches).  After the block's own barrier (its
+// stores have then reached the L2), one thread per block publishes them
+// (agent-scope release: the L2 written back) and stores gen into the
+// block's arrival word; block 0 waits for every arrival word and stores gen
+// into the release word, which the other blocks wait for; one thread per
+// block then acquires (agent scope: the L1 and L2 invalidated) before the
+// block's barrier.  The agent-scope fences act on the compute unit's and
+// the XCD's caches, so one thread per block suffices, and is several times
+// cheaper at a full grid than a fence in every thread.  Block 0 is the only
+// one that decides: when an arrival does not come within a.spin polls it
+// writes the error word instead of releasing, so either every block passes
+// barrier gen or none does.  A block that waits far longer than that for
+// the release (64 a.spin polls) gives up too.  Returns false when the
+// kernel must end.
 __device__ bool grid_barrier(const bk::PersistArgs& a, unsigned gen, int ordinal, int& fail) {
   if (threadIdx.x == 0) fail = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (blockIdx.x == 0) {
     bool ok = a.spin != 0;
@@ -1050,7 +1055,6 @@ __device__ bool grid_barrier(const bk::PersistArgs& a, unsigned gen, int ordinal
       }
     }
     if (!ok) fail = 1;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
       if (fail) {
@@ -1058,10 +1062,12 @@ __device__ bool grid_barrier(const bk::PersistArgs& a, unsigned gen, int ordinal
         __hip_atomic_compare_exchange_strong(a.err, &expect, 1 + ordinal, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_store(a.go, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __hip_atomic_store(a.go, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   } else if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_store(a.flags + blockIdx.x, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long limit = 64ull * (a.spin ? a.spin : 1u);
     unsigned long long n = 0;
@@ -1079,16 +1085,12 @@ __device__ bool grid_barrier(const bk::PersistArgs& a, unsigned gen, int ordinal
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  const bool ok = fail == 0;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok;
+  return fail == 0;
 }
 
-// Each phase body is its own (not inlined) function, so the kernel's
-// registers are those of its largest body, not of all of them inlined into
-// one loop.
 // Arguments of a called function arrive in vector registers; these put the
 // block-uniform ones back into scalar registers, so the body's member table
 // loads and address arithmetic stay scalar as in the per-pass kernels.

@@ -66,13 +66,15 @@ def _build(spec, fixed_dt=None, bias=(0.01, 0.0, 0.02), dmi=0.0):
     return world, mag
 
 
-def _run(mode, rules, fixed_dt=None, poll=8, spin=None, blocks_per_cu=None):
+def _run(mode, rules, fixed_dt=None, poll=8, blocks_per_cu=None, options=None):
     """Every spec with its rule ("steps" or "duration"), stepped by a batch in
     the given step mode; returns [(m, time, steps, rejected)]."""
     batch = mp.Batch(80000)
     batch.step_mode = mode
     if blocks_per_cu is not None:
         batch._impl.set_option("persist_blocks_per_cu", blocks_per_cu)
+    for k, v in (options or {}).items():
+        batch._impl.set_option(k, v)
     out = []
     for i, (spec, (how, arg)) in enumerate(zip(SPECS, rules)):
         w, m = _build(spec, fixed_dt=fixed_dt, dmi=1e-3 if i == 0 else 0.0)
@@ -129,6 +131,13 @@ def test_persistent_with_one_block_per_compute_unit():
     _same(got, want)
 
 
+def test_persistent_with_a_grid_of_64_blocks():
+    want, _ = _run("passes", FIXED, fixed_dt=1e-13)
+    batch_opts = {"persist_max_grid": 64}
+    got, _ = _run("persistent", FIXED, fixed_dt=1e-13, options=batch_opts)
+    _same(got, want)
+
+
 def test_auto_mode_decides_and_records():
     want, _ = _run("passes", FIXED, fixed_dt=1e-13, poll=4)
     got, path = _run("auto", FIXED, fixed_dt=1e-13, poll=2)
@@ -139,7 +148,7 @@ def test_auto_mode_decides_and_records():
     decided = [v for v in report.values() if v["choice"] is not None]
     assert decided, report
     for v in decided:
-        assert v["choice"] in ("passes", "persistent", "persistent_1pcu")
+        assert v["choice"] in ("passes", "persistent", "persistent_1pcu", "persistent_64")
         assert v[v["choice"]] is not None and v[v["choice"]] > 0
 
 
