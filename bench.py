@@ -151,6 +151,9 @@ def run_afm(args):
     world, afm = build_afm_world(args.seed)
     ts = world.timesolver
     impl = world._impl
+    if args.serial_elastic:
+        impl.set_option("split_elastic", 0)
+        impl.set_option("split_demag", 0)
     nx, ny, nz = AFM_SIZE
     N = nx * ny * nz
     steps = args.steps if args.steps_given else 500
@@ -617,6 +620,9 @@ def main():
                     help="default workload only: leave out the batched and served sub-measurements")
     ap.add_argument("--batch-step-mode", choices=("passes", "persistent", "auto"), default="passes",
                     help="step path of the batched and served workloads (Batch.step_mode)")
+    ap.add_argument("--serial-elastic", action="store_true",
+                    help="afm workload: strains, accelerations and sublattice fields on the world's stream, "
+                         "not beside the demag passes")
     ap.add_argument("--unfused", action="store_true",
                     help="run the stage combination as its own launch (world option fuse_stages 0), e.g. to "
                          "calibrate counters on a kernel of known traffic")

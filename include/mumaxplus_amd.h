@@ -358,7 +358,12 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
  * combination launch); 0 runs the separate combination.  "field_zchunk" (default 4): z planes per workgroup of the
  * uniform-parameter field kernel's LDS halo-tile variant (magnets with no geometry mask or periodic
  * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant.  Both give the
- * same values bit for bit. */
+ * same values bit for bit.  "split_elastic" (default 1): with elastodynamics, the strain and the
+ * acceleration of each right-hand side run on other streams beside the demag passes; 0 runs
+ * them on the world's stream (the same values).  "split_demag" (default 0): 1 computes the sublattice
+ * fields without demag of an antiferromagnet with demag and uniform parameters beside the demag
+ * passes and forms its torques after them (the same values bit for bit); 0 computes fields and
+ * torques in one launch after the demag passes. */
 int mpa_world_set_option(mpa_world* w, const char* key, int value);
 int mpa_world_get_option(const mpa_world* w, const char* key, int* value);
 int mpa_world_profile_reset(mpa_world* w);

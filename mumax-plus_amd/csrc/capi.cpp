@@ -1163,6 +1163,8 @@ int mpa_world_set_option(mpa_world* w, const char* key, int value) {
     const std::string k = key;
     if (k == "fuse_stages") w->w.solver.fuse_stages = value != 0;
     else if (k == "fuse_combine") w->w.solver.fuse_combine = value != 0;
+    else if (k == "split_elastic") w->w.split_elastic = value != 0;
+    else if (k == "split_demag") w->w.split_demag = value != 0;
     else if (k == "field_zchunk") {
       if (value < 0) throw Error(MPA_ERR_INVALID, "field_zchunk must be >= 0");
       w->w.field_zchunk = value;
@@ -1178,6 +1180,8 @@ int mpa_world_get_option(const mpa_world* w, const char* key, int* value) {
     const std::string k = key;
     if (k == "fuse_stages") *value = w->w.solver.fuse_stages ? 1 : 0;
     else if (k == "fuse_combine") *value = w->w.solver.fuse_combine ? 1 : 0;
+    else if (k == "split_elastic") *value = w->w.split_elastic ? 1 : 0;
+    else if (k == "split_demag") *value = w->w.split_demag ? 1 : 0;
     else if (k == "field_zchunk") *value = w->w.field_zchunk;
     else throw Error(MPA_ERR_INVALID, "unknown world option '" + k + "'");
   });

@@ -331,15 +331,17 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
     out[idx] = edens;
     return;
   }
-  // effectivefield.cpp:15-28 order
+  // the terms of effectivefield.cpp:15-28, the demag field added last (so a
+  // field evaluated without demag plus the demag field is this sum bit for
+  // bit: k_torque_pair)
   float3 h = hex;
   h = h + han;
   h = h + hext;
   h = h + hdm;
-  h = h + hdem;
   h = h + hme;
   h = h + hafm_nn;
   h = h + hafm_site;
+  h = h + hdem;
 
   // spin-transfer torque (evalSpinTransferTorque, stt.cu:152-195): Slonczewski
   // when it is not assured zero, else Zhang-Li; zero for frozen spins
@@ -438,6 +440,46 @@ __global__ void __launch_bounds__(BLOCK) k_field_pair(FMDesc d0, const float* __
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   if (blockIdx.y == 0) field_body<UNI>(d0, m0, h0, o0, mode, bid);
   else field_body<UNI>(d1, m1, h1, o1, mode, bid);
+}
+
+// The torque of a sublattice cell from its field without demag (hl) and the
+// demag field (hd), as field_body forms it for uniform parameters in MODE_LLG
+// or MODE_DAMPING (no spin-transfer torque, no thermal noise, not frozen).
+__device__ __forceinline__ void torque_cell(const FMDesc& d, const float* __restrict__ m,
+                                            const float* __restrict__ hl, const float* __restrict__ hd,
+                                            float* __restrict__ out, int mode, int idx) {
+  const int N = d.ncell;
+  if (idx >= N) return;
+  if (d.geom && !d.geom[idx]) {
+    out[idx] = 0.f;
+    out[N + idx] = 0.f;
+    out[2 * N + idx] = 0.f;
+    return;
+  }
+  const float3 mc = load3(m, idx, N);
+  const float3 h = load3(hl, idx, N) + load3(hd, idx, N);
+  float3 r;
+  if (mode == MODE_LLG) {
+    const float al = d.alpha.v;
+    const float3 mxh = cross3(mc, h);
+    const float3 mxmxh = cross3(mc, mxh);
+    r = (-GAMMALL / (1 + al * al)) * (mxh + al * mxmxh);
+  } else {
+    r = -GAMMALL * cross3(mc, cross3(mc, h));
+  }
+  out[idx] = r.x;
+  out[N + idx] = r.y;
+  out[2 * N + idx] = r.z;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_torque_pair(FMDesc d0, const float* __restrict__ m0,
+                                                       const float* __restrict__ hl0, const float* __restrict__ hd0,
+                                                       float* __restrict__ o0, FMDesc d1, const float* __restrict__ m1,
+                                                       const float* __restrict__ hl1, const float* __restrict__ hd1,
+                                                       float* __restrict__ o1, int mode) {
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (blockIdx.y == 0) torque_cell(d0, m0, hl0, hd0, o0, mode, idx);
+  else torque_cell(d1, m1, hl1, hd1, o1, mode, idx);
 }
 
 template <bool NEXT>
@@ -769,6 +811,23 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
   if (u0) k_field_pair<true><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
   else k_field_pair<false><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
   checkLaunch("k_field_pair");
+}
+
+bool torque_pair_splittable(const FMDesc& d0, const FMDesc& d1, FieldMode mode) {
+  auto one = [&](const FMDesc& d) {
+    return params_uniform(d) && !all_uniform(d) && (d.terms & TERM_DEMAG) && !(d.terms & TERM_STT) && !d.noise &&
+           d.frozen.v == 0.f && !d.hstray;
+  };
+  return (mode == MODE_LLG || mode == MODE_DAMPING) && d0.ncell == d1.ncell && one(d0) && one(d1);
+}
+
+void launch_torque_pair(const FMDesc& d0, const float* m0, const float* hl0, const float* hd0, float* o0,
+                        const FMDesc& d1, const float* m1, const float* hl1, const float* hd1, float* o1,
+                        FieldMode mode, hipStream_t s) {
+  if (!torque_pair_splittable(d0, d1, mode)) throw Error(MPA_ERR_RUNTIME, "launch_torque_pair: not splittable");
+  const dim3 grid((d0.ncell + BLOCK - 1) / BLOCK, 2);
+  k_torque_pair<<<grid, BLOCK, 0, s>>>(d0, m0, hl0, hd0, o0, d1, m1, hl1, hd1, o1, (int)mode);
+  checkLaunch("k_torque_pair");
 }
 
 namespace {

@@ -147,6 +147,19 @@ void launch_field_next(const FMDesc& d, const float* m, const float* hdemag, flo
 void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float* o0, const FMDesc& d1,
                        const float* m1, const float* h1, float* o1, FieldMode mode, hipStream_t s);
 
+// The torques launch_field_pair gives in `mode` (MODE_LLG or MODE_DAMPING)
+// with demag, from the sublattices' fields without demag (hl0, hl1:
+// launch_field_pair in MODE_FIELD with TERM_DEMAG taken out of the
+// descriptors' terms) and the demag fields hd0, hd1, for descriptors that
+// torque_pair_splittable accepts (uniform parameters on the general
+// sublattice path, demag on, no spin-transfer torque, no thermal noise, not
+// frozen, no stray field of other magnets).  Lets the field without demag
+// run beside the demag passes.
+bool torque_pair_splittable(const FMDesc& d0, const FMDesc& d1, FieldMode mode);
+void launch_torque_pair(const FMDesc& d0, const float* m0, const float* hl0, const float* hd0, float* o0,
+                        const FMDesc& d1, const float* m1, const float* hl1, const float* hd1, float* o1,
+                        FieldMode mode, hipStream_t s);
+
 // Per cell the largest angle (rad) between m and an in-geometry neighbour with
 // msat != 0 and a non-zero harmonic-mean exchange constant; 0 outside the
 // geometry and where msat = 0 (k_maxangle, exchange.cu:271-313).

@@ -972,6 +972,34 @@ void Antiferromagnet::eval_rhs_pair(float* out0, float* out1, hipStream_t s, Fie
                     b->m.get(), b->demag_field(), out1, mode, s);
 }
 
+bool Antiferromagnet::pair_split(FieldMode mode, double noise_dt) {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  return torque_pair_splittable(a->rhs_desc(a->active_terms(), noise_dt), b->rhs_desc(b->active_terms(), noise_dt),
+                                mode);
+}
+
+void Antiferromagnet::eval_local_pair(hipStream_t s, double noise_dt) {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  const unsigned ta = a->active_terms() & ~TERM_DEMAG, tb = b->active_terms() & ~TERM_DEMAG;
+  for (auto& h : hloc)
+    if (h.n == 0) h.allocate((size_t)3 * N);
+  Timed t(&world->prof, "k_field_pair", a->rhs_bytes(ta) + b->rhs_bytes(tb), s);
+  launch_field_pair(a->rhs_desc(ta, noise_dt), a->m.get(), nullptr, hloc[0].get(), b->rhs_desc(tb, noise_dt),
+                    b->m.get(), nullptr, hloc[1].get(), MODE_FIELD, s);
+}
+
+void Antiferromagnet::eval_torque_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt) {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  // read m, the field without demag and the demag field; write the torque
+  Timed t(&world->prof, "k_torque_pair", 2 * 48.0 * N, s);
+  launch_torque_pair(a->rhs_desc(a->active_terms(), noise_dt), a->m.get(), hloc[0].get(), a->demag_field(), out0,
+                     b->rhs_desc(b->active_terms(), noise_dt), b->m.get(), hloc[1].get(), b->demag_field(), out1,
+                     mode, s);
+}
+
 namespace {
 struct QInfo {
   const char* name;
@@ -1336,15 +1364,48 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
   if (stage_in && (next || vars.size() != 1 || vars[0].kind != StateVar::MAG || vars[0].fm->host ||
                    !(vars[0].fm->active_terms() & TERM_DEMAG) || vars[0].x != vars[0].fm->m.get()))
     throw Error(MPA_ERR_RUNTIME, "a stage state formed in the demag pass needs one magnetisation with demag");
-  for (auto& fm : magnets) {
-    if (fm->el.enabled) fm->compute_strain(s);
-    fm->compute_stray(s, &prof);
+  // The strains and accelerations depend on the displacements, velocities
+  // and magnetisations of the stage state only, so with elastodynamics they
+  // run on stream2 beside the demag passes: the field kernels (the
+  // magnetoelastic field reads the strain) wait for the strains, and the
+  // evaluation ends when both streams are done.
+  // Likewise the sublattice fields without demag of an antiferromagnet
+  // (after the strains, on stream2), whose torques a launch after the demag
+  // passes completes.  Accelerations go to stream3.
+  bool elastic = false;
+  for (auto& fm : magnets) elastic = elastic || fm->el.enabled;
+  for (auto& a : afms) elastic = elastic || a->el.enabled;
+  const FieldMode fmode = relaxing_ ? MODE_DAMPING : MODE_LLG;
+  std::vector<Antiferromagnet*> dsplit;
+  if (split_demag && !next && !stage_in)
+    for (auto& a : afms)
+      if (a->pair_split(fmode, noise_dt)) dsplit.push_back(a.get());
+  auto in_split = [&](const Antiferromagnet* a) {
+    return std::find(dsplit.begin(), dsplit.end(), a) != dsplit.end();
+  };
+  const bool split = elastic && split_elastic;
+  const bool fork = split || !dsplit.empty();
+  hipStream_t se = fork ? stream2 : s;  // the strains
+  hipStream_t sa = split ? stream3 : s;  // the accelerations
+  if (fork) {
+    HIP_CHECK(hipEventRecord(ev_fork, s));
+    HIP_CHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+    if (split) HIP_CHECK(hipStreamWaitEvent(stream3, ev_fork, 0));
   }
+  for (auto& fm : magnets)
+    if (fm->el.enabled) fm->compute_strain(se);
+  for (auto& a : afms)
+    if (a->el.enabled) a->compute_strain(se);
+  if (fork) HIP_CHECK(hipEventRecord(ev_strain, stream2));
+  if (split) HIP_CHECK(hipStreamWaitEvent(stream3, ev_strain, 0));
+  for (Antiferromagnet* a : dsplit) a->eval_local_pair(stream2, noise_dt);
+  if (!dsplit.empty()) HIP_CHECK(hipEventRecord(ev_local, stream2));
+  for (auto& fm : magnets) fm->compute_stray(s, &prof);
   for (auto& a : afms) {
-    if (a->el.enabled) a->compute_strain(s);
     if ((a->sub[0]->active_terms() | a->sub[1]->active_terms()) & TERM_DEMAG) a->compute_demag(s, &prof);
     a->compute_stray(s, &prof);
   }
+  if (fork) HIP_CHECK(hipStreamWaitEvent(s, dsplit.empty() ? ev_strain : ev_local, 0));
   for (size_t vi = 0; vi < vars.size(); ++vi) {
     const StateVar& sv = vars[vi];
     float* out = (*sv.k)[stage].get();
@@ -1355,7 +1416,8 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
         // the two sublattices follow each other: one launch for both
         if (fm == fm->host->sub[0].get() && vi + 1 < vars.size() && vars[vi + 1].kind == StateVar::MAG &&
             vars[vi + 1].fm == fm->host->sub[1].get()) {
-          fm->host->eval_rhs_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
+          if (in_split(fm->host)) fm->host->eval_torque_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
+          else fm->host->eval_rhs_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
           ++vi;
           continue;
         }
@@ -1379,10 +1441,18 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
     } else {
       MagnetCore* own = sv.owner;
       // read stress (24 B), v (12 B) and each coupled magnetisation (12 B); write a (12 B)
-      Timed t(&prof, "k_elastic", (48.0 + 12.0 * own->num_fm()) * sv.N, s);
+      Timed t(&prof, "k_elastic", (48.0 + 12.0 * own->num_fm()) * sv.N, sa);
       launch_elastic(own->eldesc(), own->el.stress.get(), own->el.v.get(), own->me_force_of(0),
-                     own->num_fm() > 1 ? own->me_force_of(1) : MEForce{}, out, EL_ACCEL, s);
+                     own->num_fm() > 1 ? own->me_force_of(1) : MEForce{}, out, EL_ACCEL, sa);
     }
+  }
+  if (fork) {
+    HIP_CHECK(hipEventRecord(ev_join, stream2));
+    HIP_CHECK(hipStreamWaitEvent(s, ev_join, 0));
+  }
+  if (split) {
+    HIP_CHECK(hipEventRecord(ev_join3, stream3));
+    HIP_CHECK(hipStreamWaitEvent(s, ev_join3, 0));
   }
 }
 
@@ -1862,11 +1932,19 @@ World::World(const double cs[3], const int mg_size[3], const int mg_origin[3], c
   }
   check_pbc(mastergrid_size, pbc);
   HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&ev_fork, &ev_strain, &ev_local, &ev_join, &ev_join3})
+    HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
 }
 
 World::~World() {
   magnets.clear();
   afms.clear();
+  for (hipEvent_t e : {ev_fork, ev_strain, ev_local, ev_join, ev_join3})
+    if (e) (void)hipEventDestroy(e);
+  if (stream3) (void)hipStreamDestroy(stream3);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

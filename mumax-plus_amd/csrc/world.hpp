@@ -295,6 +295,14 @@ class Antiferromagnet : public MagnetCore {
   Ferromagnet* fm_at(int i) override { return sub[i].get(); }
   void set_regions(const uint32_t* regions) override;  // the sublattices share the host's regions
   void eval_rhs_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt);
+  // eval_rhs_pair in two parts, so that the first can run beside the demag
+  // passes: the sublattices' fields without demag (into hloc), then their
+  // torques from those and the demag field.  pair_split tells whether the
+  // two parts give what eval_rhs_pair gives (torque_pair_splittable).
+  bool pair_split(FieldMode mode, double noise_dt);
+  void eval_local_pair(hipStream_t s, double noise_dt);
+  void eval_torque_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt);
+  DeviceArray<float> hloc[2];
   bool stray_source_of(StraySource& out) override;
 
   // Quantities of the whole antiferromagnet (neel_vector, full_magnetization
@@ -375,12 +383,25 @@ class World {
   std::vector<std::unique_ptr<Antiferromagnet>> afms;
   TimeSolver solver;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // strains and fields without demag of a right-hand side
+  hipStream_t stream3 = nullptr;  // accelerations of a right-hand side
+  hipEvent_t ev_fork = nullptr, ev_strain = nullptr, ev_local = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   Reducer reducer;
   Profiler prof;
   unsigned long long version = 1;  // bumped by every state change (invalidates FSAL reuse)
   // Uniform-parameter field kernel: z planes per block of the LDS halo-tile
   // variant, 0 for the one-cell-per-thread variant.
   int field_zchunk = 4;
+  // Elastodynamics: the strain and the acceleration of a right-hand side run
+  // on a second stream, concurrently with the demag passes (joined before
+  // the magnetoelastic field and at the end of the evaluation).
+  bool split_elastic = true;
+  // Antiferromagnets with demag: the sublattice fields without demag run on
+  // a second stream beside the demag passes; a torque launch after the x
+  // inverse adds the demag field (the same values bit for bit).  Off by
+  // default: at 512 x 512 the field launch fills every compute unit and the
+  // demag passes beside it slow down more than the overlap gains.
+  bool split_demag = false;
 
   // regions: one index per cell, or nullptr (mumaxworld.cpp:57-103)
   Ferromagnet* add_ferromagnet(const int size[3], const int origin[3], const uint8_t* geometry,

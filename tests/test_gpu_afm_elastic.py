@@ -332,6 +332,38 @@ def test_coupled_fixed_steps_match_oracle(method):
     assert abs(ts.time - p.ow.solver.time) < 1e-25
 
 
+@pytest.mark.parametrize("options", [{"split_elastic": 0}, {"split_demag": 1}, {"split_elastic": 0, "split_demag": 1}])
+def test_streams_beside_the_demag_passes_change_nothing(options):
+    """The strain and acceleration launches (world option split_elastic) and
+    the sublattice fields without demag (split_demag) run on other streams
+    beside the demag passes: adaptive Dormand-Prince steps of the coupled
+    antiferromagnet give the same m1, m2, u, v and time bit for bit with the
+    options on and off."""
+    out = []
+    for opts in ({}, options):
+        rng = np.random.default_rng(22)
+        nx, ny, nz = 32, 16, 1
+        shape = (nz, ny, nx)
+        sub = {"msat": 5.66e5, "aex": 2.48e-12, "alpha": 0.05, "B1": -8.8e6, "B2": -8.8e6}
+        p = AfmPair(rng, size=(nx, ny, nz), cs=(2e-9, 2e-9, 2e-9), mastergrid=(nx, ny, 0), pbc=(1, 1, 0),
+                    afm={"afmex_cell": -4e-12}, subs=(dict(sub), dict(sub)), inter_dmi={"xxz": 5e-4, "yyz": 5e-4},
+                    elastic={"C11": 283e9, "C12": 166e9, "C44": 58e9, "rho": 8900.0, "eta": 1e10,
+                             "u": 1e-11 * rng.normal(size=(3,) + shape), "v": 10 * rng.normal(size=(3,) + shape)})
+        for k, v in opts.items():
+            p.world._impl.set_option(k, v)
+            assert p.world._impl.get_option(k) == v
+        ts = p.world.timesolver
+        ts.set_method("DormandPrince")
+        ts.steps(25)
+        out.append(([s_.magnetization.get() for s_ in p.subs], p.afm.elastic_displacement.get(),
+                    p.afm.elastic_velocity.get(), ts.time))
+    (m_a, u_a, v_a, t_a), (m_b, u_b, v_b, t_b) = out
+    for x, y in zip(m_a, m_b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(u_a, u_b) and np.array_equal(v_a, v_b)
+    assert t_a == t_b
+
+
 def test_sensible_timestep_counts_elastic_equations():
     """The initial time step is sensible_factor / the largest rhs norm over
     all equations, the elastic accelerations included (timesolver.cpp:37-47)."""
