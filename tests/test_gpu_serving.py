@@ -6,7 +6,9 @@
   mix where it is admitted while others are running;
 * a served switching run against the FP64 oracle: <m> within 1e-4;
 * a served relaxation stops below its torque threshold, at the state the
-  plain relax() reaches (<m> within 1e-2: different step rules);
+  plain relax() reaches (<m> within 1e-2: different step rules), and within
+  1e-4 in <m> of the FP64 oracle stepping the same damping-only dynamics to
+  the same threshold;
 * admission by footprint: the running requests never exceed the capacity.
 """
 
@@ -85,6 +87,46 @@ def test_served_relaxation_reaches_the_plain_relaxed_state():
     assert res.max_torque <= 1e9
     w, m = req.solve_plain()
     assert np.abs(np.array(res.magnet.magnetization.average()) - np.array(m.magnetization.average())).max() < 1e-2
+
+
+def test_served_relaxation_against_fp64_oracle():
+    """A served relaxation (damping-only Dormand-Prince with the adaptive step,
+    until the max damping torque is below the threshold) against the FP64
+    oracle stepping the same damping-only dynamics from the same state with
+    the same stopping rule: <m> within 1e-4."""
+    from oracle import relax as orelax
+
+    nx, ny, nz = 20, 12, 1
+    x = (np.arange(nx) + 0.5) / nx * 2 - 1
+    y = (np.arange(ny) + 0.5) / ny * 2 - 1
+    geom = np.broadcast_to((x[None, :] ** 2 + y[:, None] ** 2) <= 1.0, (nz, ny, nx)).copy()
+    v = np.array([np.cos(0.6), np.sin(0.6), 0.15])
+    m0 = np.empty((3, nz, ny, nx), dtype=np.float32)
+    m0[:] = (v / np.linalg.norm(v))[:, None, None, None]
+    params = {"msat": 8e5, "aex": 1.3e-11, "ku1": 3e5, "anisU": (1.0, 0.0, 0.0), "alpha": 0.02}
+    thr = 2e7
+    # a field across the easy axis: the relaxed state is canted
+    req = serving.Request("relax", (nx, ny, nz), (2e-9,) * 3, params, m0, geometry=geom, bias=(0.0, 0.12, 0.03),
+                          threshold=thr, rid=0)
+    res = serving.Server(1024).serve([req], realtime=False)[0]
+    assert res.max_torque <= thr
+    world, mag, ow, om = make_pair(req.size, req.cellsize, req.params, req.m0, geometry=req.geometry,
+                                   world_bias=req.bias)
+    ts = ow.solver
+    ts.set_method("DormandPrince")
+    ts.adaptive = True
+    ow.relax_fms = [om]
+    ts.equations_changed()
+    g = om.geom_mask()
+    steps = 0
+    while orelax._max_norm(orelax.damping_torque(ow, om), g) > thr:
+        ts.step()
+        steps += 1
+        assert steps < 200000
+    ow.relax_fms = None
+    got = np.array(res.magnet.magnetization.average())
+    want = om.m[:, g].mean(axis=1)
+    assert np.abs(got - want).max() < 1e-4
 
 
 def test_admission_respects_capacity_and_every_request_finishes():
