@@ -36,11 +36,12 @@ using namespace bk;
 
 // One wavefront per chunk (64 threads x 4 consecutive cells), four chunks
 // per block.
-__global__ void __launch_bounds__(256) k_bstage(StageArgs sa, float* __restrict__ X, float* __restrict__ X0,
-                                                float* __restrict__ K0, const float* __restrict__ KL,
-                                                const uint8_t* __restrict__ G, const int* __restrict__ c2m,
-                                                const BatchMember* __restrict__ st, int cs, int nch) {
-  bstage_block(sa, X, X0, K0, KL, G, c2m, st, cs, nch, blockIdx.x);
+__global__ void __launch_bounds__(256) k_bstage(StageArgs sa, const float* X, float* XO,
+                                                float* __restrict__ X0, float* __restrict__ K0,
+                                                const float* __restrict__ KL, const uint8_t* __restrict__ G,
+                                                const int* __restrict__ c2m, const BatchMember* __restrict__ st,
+                                                int cs, int nch) {
+  bstage_block(sa, X, XO, X0, K0, KL, G, c2m, st, cs, nch, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(CH) k_berr(ErrArgs ea, const uint8_t* __restrict__ G, const int* __restrict__ c2m,
@@ -73,12 +74,13 @@ Batch::Batch(long long capacity, const std::string& method) : method_(method) {
   HIP_CHECK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
   const size_t n3 = (size_t)3 * cs_;
   X_.allocate(n3);
+  XO_.allocate(n3);
   X0_.allocate(n3);
   HD_.allocate(n3);
   const int S = (int)tb.nodes.size();
   for (int s = 0; s < S; ++s) K_.emplace_back(n3);
   G_.allocate((size_t)cs_);
-  for (DeviceArray<float>* a : {&X_, &X0_, &HD_})
+  for (DeviceArray<float>* a : {&X_, &XO_, &X0_, &HD_})
     HIP_CHECK(hipMemsetAsync(a->get(), 0, a->bytes(), stream));
   for (auto& k : K_) HIP_CHECK(hipMemsetAsync(k.get(), 0, k.bytes(), stream));
   HIP_CHECK(hipMemsetAsync(G_.get(), 0, G_.bytes(), stream));
@@ -272,14 +274,15 @@ void Batch::rebuild() {
   drop_graph();
   std::vector<DemagBatch::Member> dm;
   std::vector<FieldBatchMember> fb;
-  for (const Slot& sl : slots_) {
+  for (size_t id = 0; id < slots_.size(); ++id) {
+    const Slot& sl = slots_[id];
     if (!sl.used) continue;
     // the parameters read on admission: a member's result never depends on
     // when other members join or leave
     Ferromagnet* fm = sl.fm;
     const FMDesc& d = sl.desc;
     if (sl.demag) dm.push_back({&fm->demag(), sl.off, PS{nullptr, d.msat.v}, fm->geom_ptr()});
-    fb.push_back({d, (int)sl.off, sl.mode == BATCH_RELAX ? MODE_DAMPING : MODE_LLG});
+    fb.push_back({d, (int)sl.off, sl.mode == BATCH_RELAX ? MODE_DAMPING : MODE_LLG, (int)id});
   }
   demag_ = dm.empty() ? nullptr : std::make_unique<DemagBatch>(dm, cs_);
   field_ = std::make_unique<FieldBatch>(fb, (int)cs_);
@@ -290,13 +293,36 @@ void Batch::rebuild() {
   pdirty_ = true;
 }
 
-void Batch::eval_rhs(float* out) {
-  if (demag_) demag_->exec(X_.get(), HD_.get(), stream, &prof, stream2, fork_, join_);
+void Batch::eval_rhs(const float* x, float* out, const BatchNext* next) {
+  if (demag_) demag_->exec(x, HD_.get(), stream, &prof, stream2, fork_, join_);
   double bytes = 0.0;
   for (const Slot& sl : slots_)
-    if (sl.used) bytes += 12.0 * sl.ncell * (sl.demag ? 3 : 2);
+    if (sl.used) bytes += 12.0 * sl.ncell * ((sl.demag ? 3 : 2) + (next ? next->nk + 2 : 0));
   Timed t(&prof, "k_field_uniform_b", bytes, stream);
-  field_->launch(X_.get(), HD_.get(), out, stream);
+  field_->launch(x, HD_.get(), out, stream, next);
+}
+
+float* Batch::stage_state(int s) {
+  // the last stage's state is the new state, in X_; the stages alternate
+  // between X_ and XO_ backwards from there, so a field launch reads one and
+  // writes the next stage state into the other
+  const int S = (int)tableau_by_name(method_).nodes.size();
+  return ((S - 1 - s) % 2 == 0) ? X_.get() : XO_.get();
+}
+
+BatchNext Batch::next_stage(int s) {
+  // the state of stage s + 1, formed in the field launch of stage s
+  const Tableau& tb = tableau_by_name(method_);
+  BatchNext n{};
+  n.x0 = X0_.get();
+  n.nk = s;
+  for (int j = 0; j <= s; ++j) {
+    if (j < s) n.k[j] = K_[j].get();
+    n.a[j] = tb.A[s + 1][j];
+  }
+  n.xout = stage_state(s + 1);
+  n.st = st_.get();
+  return n;
 }
 
 void Batch::prime(const std::vector<int>& fresh) {
@@ -306,7 +332,7 @@ void Batch::prime(const std::vector<int>& fresh) {
   // k_(S-1) = rhs(x) for every member: unchanged for members whose last
   // attempt was accepted (x is that attempt's last stage state), unused by
   // the others, and the fresh members' first derivative.
-  eval_rhs(K_[S - 1].get());
+  eval_rhs(X_.get(), K_[S - 1].get(), nullptr);
   ErrArgs ea{};
   ea.nk = 0;
   ea.tq = K_[S - 1].get();
@@ -326,21 +352,26 @@ void Batch::record_step() {
   double used = 0.0;
   for (const Slot& sl : slots_)
     if (sl.used) used += (double)sl.ncell;
-  for (int s = 1; s < S; ++s) {
+  // stage 1: take over the accepted state and form the first stage state;
+  // every later stage state is formed in the previous stage's field launch
+  {
     StageArgs sa{};
-    sa.nk = s;
-    sa.commit = s == 1;
-    for (int j = 0; j < s; ++j) {
-      sa.k[j] = K_[j].get();
-      sa.a[j] = tb.A[s][j];
+    sa.nk = 1;
+    sa.commit = 1;
+    sa.k[0] = K_[0].get();
+    sa.a[0] = tb.A[1][0];
+    Timed t(&prof, "k_bstage", 12.0 * used * 3 + 24.0 * used, stream);
+    k_bstage<<<(nch + 3) / 4, 256, 0, stream>>>(sa, X_.get(), stage_state(1), X0_.get(), K_[0].get(),
+                                                K_[S - 1].get(), G_.get(), c2m_.get(), st_.get(), (int)cs_, (int)nch);
+    checkLaunch("k_bstage");
+  }
+  for (int s = 1; s < S; ++s) {
+    if (s + 1 < S) {
+      const BatchNext n = next_stage(s);
+      eval_rhs(stage_state(s), K_[s].get(), &n);
+    } else {
+      eval_rhs(stage_state(s), K_[s].get(), nullptr);
     }
-    {
-      Timed t(&prof, "k_bstage", 12.0 * used * (2 + s) + (s == 1 ? 24.0 * used : 0.0), stream);
-      k_bstage<<<(nch + 3) / 4, 256, 0, stream>>>(sa, X_.get(), X0_.get(), K_[0].get(), K_[S - 1].get(), G_.get(),
-                                                  c2m_.get(), st_.get(), (int)cs_, (int)nch);
-      checkLaunch("k_bstage");
-    }
-    eval_rhs(K_[s].get());
   }
   ErrArgs ea{};
   ea.nk = S;
@@ -485,23 +516,30 @@ void Batch::build_persist() {
       shm = std::max(shm, t.shm);
     }
   }
+  // as the per-pass step: the first stage state by the combination, every
+  // later one by the previous stage's field phase
+  bk::Phase st{};
+  st.kind = bk::PH_STAGE;
+  st.nvb = (nch + 3) / 4;
+  st.sa.nk = 1;
+  st.sa.commit = 1;
+  st.sa.k[0] = K_[0].get();
+  st.sa.a[0] = tb.A[1][0];
+  st.out = stage_state(1);
+  ph.push_back(st);
   for (int s = 1; s < S; ++s) {
-    bk::Phase st{};
-    st.kind = bk::PH_STAGE;
-    st.nvb = (nch + 3) / 4;
-    st.sa.nk = s;
-    st.sa.commit = s == 1;
-    for (int j = 0; j < s; ++j) {
-      st.sa.k[j] = K_[j].get();
-      st.sa.a[j] = tb.A[s][j];
-    }
-    ph.push_back(st);
-    for (const bk::Phase& d : dph)
-      if (d.npass > 0) ph.push_back(d);
+    for (bk::Phase d : dph)
+      if (d.npass > 0) {
+        d.x = stage_state(s);
+        ph.push_back(d);
+      }
     bk::Phase f{};
     f.kind = bk::PH_FIELD;
     f.nvb = (int)field_->nblocks();
+    f.x = stage_state(s);
     f.out = K_[s].get();
+    f.has_next = s + 1 < S;
+    if (f.has_next) f.next = next_stage(s);
     ph.push_back(f);
   }
   bk::Phase e{};
@@ -736,9 +774,12 @@ double Batch::step_bytes() {
   double used = 0.0;
   for (const Slot& sl : slots_)
     if (sl.used) used += (double)sl.ncell;
-  double b = 0.0;
-  for (int s = 1; s < S; ++s) b += 12.0 * used * (2 + s);
-  b += 24.0 * used;                            // the take-over at the first stage
+  // stage state 1 by the combination (x0 and k0 read, the state written;
+  // the take-over copies x and k into x0 and k0), stage state q >= 2 in the
+  // field launch of stage q - 1 (x0 and k_0 .. k_(q-2) read, the state
+  // written; k_(q-1) is in registers)
+  double b = 12.0 * used * 3 + 24.0 * used;
+  for (int q = 2; q < S; ++q) b += 12.0 * used * (q + 1);
   for (const Slot& sl : slots_)
     if (sl.used) b += (S - 1) * 12.0 * sl.ncell * (sl.demag ? 3 : 2);  // field kernel: m, h_demag, k
   if (demag_) b += (S - 1) * demag_->algorithmic_bytes();
@@ -749,7 +790,7 @@ int Batch::launches_per_step() {
   if (dirty_) rebuild();
   const Tableau& tb = tableau_by_name(method_);
   const int S = (int)tb.nodes.size();
-  return (S - 1) * (2 + (demag_ ? demag_->nlaunches() : 0)) + 2;
+  return (S - 1) * (1 + (demag_ ? demag_->nlaunches() : 0)) + 3;
 }
 
 }  // namespace mpa

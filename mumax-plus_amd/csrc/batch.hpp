@@ -126,7 +126,9 @@ class Batch {
   int grid_for(int blocks_per_cu, int cap) const;
   void persist_recover(int err);
   std::string size_class();
-  void eval_rhs(float* out);
+  void eval_rhs(const float* x, float* out, const BatchNext* next);
+  float* stage_state(int s);
+  BatchNext next_stage(int s);
   void drop_graph();
   long long alloc_range(long long span);
   void free_range(long long off, long long span);
@@ -135,7 +137,7 @@ class Batch {
   long long cs_;
   std::vector<Slot> slots_;
   std::vector<std::pair<long long, long long>> free_;  // (off, span), sorted
-  DeviceArray<float> X_, X0_, HD_;
+  DeviceArray<float> X_, XO_, X0_, HD_;  // state (and stage states), the other stage states, step start, demag
   std::vector<DeviceArray<float>> K_;
   DeviceArray<uint8_t> G_;
   DeviceArray<int> c2m_;  // 256-cell chunk -> member id, -1 when free

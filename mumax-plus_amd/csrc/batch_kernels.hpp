@@ -42,8 +42,10 @@ __device__ __forceinline__ void axpy4(float4& a, float c, const float4& k) {
 
 // One wavefront per chunk (64 threads x 4 consecutive cells), four chunks
 // per block; a chunk belongs to one member, so every branch on the member is
-// uniform over the wavefront.
-__device__ __forceinline__ void bstage_block(const StageArgs& sa, float* __restrict__ X, float* __restrict__ X0,
+// uniform over the wavefront.  X: the state (taken over as x0 when the
+// member's last attempt was accepted and sa.commit), XO: the stage state
+// (may be X: every element is read before it is written, by its thread).
+__device__ __forceinline__ void bstage_block(const StageArgs& sa, const float* X, float* XO, float* __restrict__ X0,
                                              float* __restrict__ K0, const float* __restrict__ KL,
                                              const uint8_t* __restrict__ G, const int* __restrict__ c2m,
                                              const BatchMember* __restrict__ st, int cs, int nch, int blk) {
@@ -59,7 +61,8 @@ __device__ __forceinline__ void bstage_block(const StageArgs& sa, float* __restr
   float c[MAX_STAGES];
 #pragma unroll
   for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < sa.nk ? (float)(S.dt * sa.a[j]) : 0.f;
-  float4* X4 = reinterpret_cast<float4*>(X);
+  const float4* X4 = reinterpret_cast<const float4*>(X);
+  float4* XO4 = reinterpret_cast<float4*>(XO);
   float4* X04 = reinterpret_cast<float4*>(X0);
   float4* K04 = reinterpret_cast<float4*>(K0);
   const float4* KL4 = reinterpret_cast<const float4*>(KL);
@@ -94,7 +97,7 @@ __device__ __forceinline__ void bstage_block(const StageArgs& sa, float* __restr
     o.y = g.y ? v[comp].y * i1 : 0.f;
     o.z = g.z ? v[comp].z * i2 : 0.f;
     o.w = g.w ? v[comp].w * i3 : 0.f;
-    X4[(size_t)comp * cs4 + q] = o;
+    XO4[(size_t)comp * cs4 + q] = o;
   }
 }
 
@@ -250,7 +253,10 @@ struct Phase {
   int nvb;             // virtual blocks
   int npass;           // PH_DEMAG: independent passes run side by side
   PhasePass pass[2];
-  float* out;          // PH_FIELD: the stage derivative
+  const float* x;      // PH_DEMAG, PH_FIELD: the stage state
+  float* out;          // PH_FIELD: the stage derivative; PH_STAGE: the stage state
+  int has_next;        // PH_FIELD: also the next stage state (next)
+  BatchNext next;
   StageArgs sa;        // PH_STAGE
   ErrArgs ea;          // PH_ERR
   CtlArgs ca;          // PH_CONTROL

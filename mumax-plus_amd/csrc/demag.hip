@@ -1120,14 +1120,18 @@ __device__ __forceinline__ void persist_demag(const bk::PhasePass& P, int b, con
   }
 }
 
-__device__ __noinline__ void persist_stage(const bk::PersistArgs* a, const bk::StageArgs* sa, int v) {
+__device__ __noinline__ void persist_stage(const bk::PersistArgs* a, const bk::Phase* P, int v) {
   a = uni(a);
-  bk::bstage_block(*uni(sa), a->X, a->X0, a->K0, a->KL, a->G, a->c2m, a->st, a->cs, a->nch, uni(v));
+  P = uni(P);
+  bk::bstage_block(P->sa, a->X, P->out, a->X0, a->K0, a->KL, a->G, a->c2m, a->st, a->cs, a->nch, uni(v));
 }
 
-__device__ __noinline__ void persist_field(const bk::PersistArgs* a, float* out, int v) {
+__device__ __noinline__ void persist_field(const bk::PersistArgs* a, const bk::Phase* P, int v) {
   a = uni(a);
-  field_uniform_b_block(static_cast<const FBInst*>(a->ftab), a->fb2i, a->X, a->HD, uni(out), a->cs, uni(v));
+  P = uni(P);
+  const FBInst* tab = static_cast<const FBInst*>(a->ftab);
+  if (P->has_next) field_uniform_b_block<true>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next);
+  else field_uniform_b_block<false>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next);
 }
 
 __device__ __noinline__ void persist_err(const bk::PersistArgs* a, const bk::ErrArgs* ea, int v,
@@ -1155,12 +1159,12 @@ __global__ void __launch_bounds__(256, 3) k_bstep_persist(const bk::PersistArgs*
       const int kind = P.kind, nvb = P.nvb;
       for (int v = blockIdx.x; v < nvb; v += gridDim.x) {
         switch (kind) {
-          case bk::PH_STAGE: persist_stage(pa, &P.sa, v); break;
+          case bk::PH_STAGE: persist_stage(pa, &P, v); break;
           case bk::PH_DEMAG:
-            if (v < P.pass[0].nblocks) persist_demag(P.pass[0], v, a.X, a.HD);
-            else persist_demag(P.pass[1], v - P.pass[0].nblocks, a.X, a.HD);
+            if (v < P.pass[0].nblocks) persist_demag(P.pass[0], v, P.x, a.HD);
+            else persist_demag(P.pass[1], v - P.pass[0].nblocks, P.x, a.HD);
             break;
-          case bk::PH_FIELD: persist_field(pa, P.out, v); break;
+          case bk::PH_FIELD: persist_field(pa, &P, v); break;
           case bk::PH_ERR: persist_err(pa, &P.ea, v, red); break;
           default: persist_control(pa, &P.ca, v); break;
         }

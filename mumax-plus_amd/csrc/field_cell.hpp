@@ -2,6 +2,7 @@
 // the uniform-parameter field and torque of one cell, and its helpers.
 #pragma once
 
+#include "batch.hpp"
 #include "fields.hpp"
 
 namespace mpa {
@@ -252,21 +253,60 @@ struct FBInst {
   int off;
   int first;
   int mode;
+  int slot;
+};
+
+// Epilogue of the batched field kernel: the member's next stage state at
+// the cell (BatchNext), from the derivative r in registers.  c[j] = dt a_j.
+struct BatchNextCell {
+  const BatchNext& n;
+  const float* c;
+  int off, cs;
+  __device__ __forceinline__ void operator()(int idx, float3 mc, float3 r, bool in) const {
+    const size_t p = (size_t)off + idx;
+    const float rr[3] = {r.x, r.y, r.z};
+    float v[3];
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const size_t i = (size_t)comp * cs + p;
+      float a = n.x0[i];
+      for (int j = 0; j < n.nk; ++j) a += c[j] * n.k[j][i];
+      a += c[n.nk] * rr[comp];
+      v[comp] = a;
+    }
+    const float inv = rsqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) n.xout[(size_t)comp * cs + p] = in ? v[comp] * inv : 0.f;
+  }
 };
 
 // Block b of a batched field launch: 256 consecutive cells of the member
 // b2i[b] (every member starts on a multiple of 256 cells of the packed
-// arrays X, HD and OUT, whose component stride is cs).
+// arrays X, HD and OUT, whose component stride is cs).  NEXT: also the next
+// stage state of an active member (BatchNext).
+template <bool NEXT>
 __device__ __forceinline__ void field_uniform_b_block(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
                                                       const float* __restrict__ X, const float* __restrict__ HD,
-                                                      float* __restrict__ OUT, int cs, int b) {
+                                                      float* __restrict__ OUT, int cs, int b, const BatchNext& nx) {
   const FBInst& I = tab[b2i[b]];
   const int j = (b - I.first) * 256 + threadIdx.x;
-  const int nx = I.d.nx, ny = I.d.ny;
+  const int nxc = I.d.nx, ny = I.d.ny;
   if (j >= I.d.ncell) return;
-  const int x = j % nx, yz = j / nx, y = yz % ny, z = yz / ny;
+  const int x = j % nxc, yz = j / nxc, y = yz % ny, z = yz / ny;
   const float* hd = HD + I.off;
-  field_uniform_cell(I.d, I.u, X + I.off, [&](int i) { return load3(hd, i, cs); }, OUT + I.off, I.mode, x, y, z, cs);
+  auto hdem = [&](int i) { return load3(hd, i, cs); };
+  if constexpr (NEXT) {
+    const BatchMember& S = nx.st[I.slot];
+    if (S.active) {
+      float c[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) c[q] = q <= nx.nk ? (float)(S.dt * nx.a[q]) : 0.f;
+      field_uniform_cell(I.d, I.u, X + I.off, hdem, OUT + I.off, I.mode, x, y, z, cs,
+                         BatchNextCell{nx, c, I.off, cs});
+      return;
+    }
+  }
+  field_uniform_cell(I.d, I.u, X + I.off, hdem, OUT + I.off, I.mode, x, y, z, cs);
 }
 
 

@@ -627,10 +627,11 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
 
 // Batched uniform path: 256 consecutive cells of one magnet per block (every
 // magnet starts on a multiple of 256 cells of the packed arrays).
+template <bool NEXT>
 __global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
                                                            const float* __restrict__ X, const float* __restrict__ HD,
-                                                           float* __restrict__ OUT, int cs) {
-  field_uniform_b_block(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x));
+                                                           float* __restrict__ OUT, int cs, BatchNext nx) {
+  field_uniform_b_block<NEXT>(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x), nx);
 }
 
 // Every parameter the per-cell kernel reads is uniform.
@@ -731,6 +732,7 @@ FieldBatch::FieldBatch(const std::vector<FieldBatchMember>& members, int cs) : c
     I.off = mb.off;
     I.first = (int)b2i.size();
     I.mode = mb.mode;
+    I.slot = mb.slot;
     const int nb = (mb.d.ncell + BLOCK - 1) / BLOCK;
     b2i.insert(b2i.end(), nb, (int)tab.size());
     tab.push_back(I);
@@ -743,10 +745,15 @@ FieldBatch::FieldBatch(const std::vector<FieldBatchMember>& members, int cs) : c
   HIP_CHECK(hipMemcpy(b2i_.get(), b2i.data(), b2i.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
-void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t s) const {
+void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchNext* next) const {
   if (nblocks_ == 0) return;
-  k_field_uniform_b<<<nblocks_, BLOCK, 0, s>>>(reinterpret_cast<const FBInst*>(tab_.get()), b2i_.get(), X, HD, OUT,
-                                                cs_);
+  const FBInst* tab = reinterpret_cast<const FBInst*>(tab_.get());
+  if (next) {
+    if (next->nk < 0 || next->nk > 6) throw Error(MPA_ERR_RUNTIME, "batched field: bad stage count");
+    k_field_uniform_b<true><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, *next);
+  } else {
+    k_field_uniform_b<false><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, BatchNext{});
+  }
   checkLaunch("k_field_uniform_b");
 }
 

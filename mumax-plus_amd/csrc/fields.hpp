@@ -186,13 +186,32 @@ struct FieldBatchMember {
   FMDesc d;  // d.geom: the member's own mask or nullptr
   int off;
   int mode;  // MODE_FIELD, MODE_LLG or MODE_DAMPING
+  int slot;  // the member's record in BatchNext::st
+};
+
+struct BatchMember;
+// The next Runge-Kutta stage state of every active batch member, formed in
+// the batched field launch from the derivative k it has just computed:
+//   x_next = normalise(x0 + sum_{j < nk} (dt a_j) k_j + (dt a_nk) k),
+// zero outside the member's geometry, dt the member's own step (each
+// coefficient rounded once from double); inactive members are left alone.
+// The arithmetic, term order and normalisation of the batch's stage
+// combination kernel.
+struct BatchNext {
+  const float* x0;
+  const float* k[7];
+  double a[7];  // a[0..nk]: a[nk] multiplies the derivative just computed
+  int nk;
+  float* xout;
+  const BatchMember* st;
 };
 class FieldBatch {
  public:
   FieldBatch(const std::vector<FieldBatchMember>& members, int cs);
   // OUT = field or torque (each member's mode) of X, with HD where the
-  // member has TERM_DEMAG, for every member.
-  void launch(const float* X, const float* HD, float* OUT, hipStream_t s) const;
+  // member has TERM_DEMAG, for every member; with `next`, also the next
+  // stage state of every active member (BatchNext).
+  void launch(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchNext* next = nullptr) const;
   unsigned nblocks() const { return nblocks_; }
   // The device member table (FBInst entries) and block -> member map.
   const void* table() const { return tab_.get(); }
