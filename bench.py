@@ -542,6 +542,43 @@ def measure_serve(args):
     return out
 
 
+def run_dft8(args):
+    """One radix-8 FFT stage, 32 rounds in LDS over 2^20 lines, by vector-ALU
+    butterflies and as a matrix-core product: the measurement behind the FFT
+    passes' choice of butterflies."""
+    import mumaxplus as mp
+    from mumaxplus import _mumaxpluscpp as cpp
+    if not mp._cpp.device_available():
+        raise RuntimeError("no HIP device visible")
+    nlines, rounds = 1 << 20, 32
+    rng = np.random.default_rng(args.seed)
+    lines = ((rng.random((nlines, 8)) - 0.5) + 1j * (rng.random((nlines, 8)) - 0.5)).astype(np.complex64) * 0.1
+    res = {}
+    for v, name in ((0, "valu_butterflies"), (1, "mfma_16x16x4_f32")):
+        best = None
+        for _ in range(3):
+            _, ms = cpp._bench_dft8(v, lines, rounds)
+            best = ms if best is None else min(best, ms)
+        # per 8-point DFT: butterflies 64 real adds and 8 real multiplies by
+        # sqrt(1/2) (plus the twiddles a pass adds); the matrix form 16 x 16
+        # multiply-adds
+        res[name] = {"ms": best, "dfts_per_s": nlines * rounds / (best * 1e-3)}
+    out = {
+        "metric": "8-point DFTs/s of one FFT stage in LDS (vector-ALU butterflies vs matrix cores)",
+        "value": res["valu_butterflies"]["dfts_per_s"],
+        "unit": "DFT/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{nlines} lines of 8 complex points, {rounds} rounds each in LDS, block barrier "
+                               "between rounds", "lines": nlines, "rounds": rounds},
+        "variants": res,
+        "mfma_over_valu": res["valu_butterflies"]["ms"] / res["mfma_16x16x4_f32"]["ms"],
+    }
+    print(json.dumps(out))
+
+
 def run_serve(args):
     print(json.dumps(measure_serve(args)))
 
@@ -615,7 +652,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="batch / serve: skip the one-at-a-time comparison (for profiler runs)")
-    ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve"), default="ferromagnet")
+    ap.add_argument("--workload", choices=("ferromagnet", "afm", "batch", "serve", "dft8"), default="ferromagnet")
     ap.add_argument("--no-sub", action="store_true",
                     help="default workload only: leave out the batched and served sub-measurements")
     ap.add_argument("--batch-step-mode", choices=("passes", "persistent", "auto"), default="passes",
@@ -639,6 +676,9 @@ def main():
         return
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "dft8":
+        run_dft8(args)
         return
 
     import mumaxplus as mp

@@ -419,6 +419,14 @@ int mpa_batch_tune_report(mpa_batch* b, int cap, char* text, int* len);
 int mpa_batch_profile_report(mpa_batch* b, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);
 
+/* ------------------------------------------------------- Measurement -- */
+/* No counterpart in the reference: the measurement behind the FFT passes' choice of vector-ALU
+ * butterflies over matrix-core products.  One radix-8 stage (8-point forward DFTs of `nlines`
+ * lines, a multiple of 256, of 16 floats each: re/im interleaved) applied `rounds` (1..40) times
+ * in LDS, by butterflies (variant 0) or as a 16 x 16 real matrix product on v_mfma_f32_16x16x4f32
+ * (variant 1); in / out are host arrays of 16 nlines floats, *ms the kernel's time. */
+int mpa_bench_dft8(int variant, int nlines, int rounds, const float* in, float* out, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1328,6 +1328,15 @@ int mpa_batch_set_option(mpa_batch* b, const char* key, int value) {
   });
 }
 
+int mpa_bench_dft8(int variant, int nlines, int rounds, const float* in, float* out, double* ms) {
+  return guard([&] {
+    need(in, "in");
+    need(out, "out");
+    const double t = bench_dft8(variant, nlines, rounds, in, out);
+    if (ms) *ms = t;
+  });
+}
+
 int mpa_batch_step_path(mpa_batch* b, int* last_path, int* persist_grid) {
   return guard([&] {
     need(b, "batch");

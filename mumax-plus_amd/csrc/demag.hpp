@@ -138,6 +138,12 @@ class DemagBatch {
   std::vector<Launch> launches_;
 };
 
+// One radix-8 stage of the FFT passes (8-point DFTs of nlines lines of 16
+// floats, re/im interleaved) done `rounds` times in LDS, by vector-ALU
+// butterflies (variant 0) or as a 16 x 16 matrix product on the matrix cores
+// (variant 1); returns the kernel's milliseconds, the results in host_out.
+double bench_dft8(int variant, int nlines, int rounds, const float* host_in, float* host_out);
+
 // The real-space kernel exactly as the reference lays it out for a magnet
 // (wrap_magnet.cpp:56-59 `_demag_kernel`): shape (6, Kz, Ky, Kx) on the
 // reference kernel grid (strayfieldkernel.cu:80-99), computed in FP64 on the

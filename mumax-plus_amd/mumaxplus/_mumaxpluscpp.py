@@ -179,6 +179,7 @@ _SIGS = {
     "mpa_batch_set_option": ([_P, _c_char_p, _cint], _cint),
     "mpa_batch_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
     "mpa_batch_step_path": ([_P, _ct.POINTER(_cint), _ct.POINTER(_cint)], _cint),
+    "mpa_bench_dft8": ([_cint, _cint, _cint, _P, _P, _ct.POINTER(_cdouble)], _cint),
     "mpa_batch_tune_report": ([_P, _cint, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
@@ -1165,6 +1166,19 @@ class StrayField(FieldQuantity):
             n = max(int(self._mask.sum()), 1)
             return [float(v[c][self._mask].sum() / n) for c in range(3)]
         return [float(v[c].mean()) for c in range(3)]
+
+
+def _bench_dft8(variant, lines, rounds):
+    """One radix-8 FFT stage applied `rounds` times in LDS to `lines` (n x 8
+    complex64) by vector-ALU butterflies (variant 0) or a matrix-core product
+    (variant 1): (output lines, kernel milliseconds)."""
+    lines = _np.ascontiguousarray(lines, dtype=_np.complex64)
+    if lines.ndim != 2 or lines.shape[1] != 8:
+        raise ValueError("lines must have shape (n, 8)")
+    out = _np.empty_like(lines)
+    ms = _cdouble()
+    _check(_lib.mpa_bench_dft8(int(variant), lines.shape[0], int(rounds), _ptr(lines), _ptr(out), _ct.byref(ms)))
+    return out, ms.value
 
 
 def _demag_kernel(fm):
