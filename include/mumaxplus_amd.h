@@ -416,6 +416,12 @@ int mpa_batch_step_path(mpa_batch* b, int* last_path, int* persist_grid);
  * and the best measured ms per step of each candidate.  At most cap bytes (NUL-terminated) are
  * written to text; *len = the full length. */
 int mpa_batch_tune_report(mpa_batch* b, int cap, char* text, int* len);
+/* The autotuner's record written to a text file (one size class a line), merged from one (its
+ * entries replace those of the same classes), or cleared; a process that loads a record skips the
+ * timing of the classes it holds decisions for. */
+int mpa_batch_tune_save(const char* path);
+int mpa_batch_tune_load(const char* path);
+int mpa_batch_tune_clear(void);
 int mpa_batch_profile_report(mpa_batch* b, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);
 

@@ -2,14 +2,17 @@
 //
 // One batch step, for a tableau whose last stage is the new state
 // (first-same-as-last), with S stages:
+//   k_bstage     x_1 = normalise(x0 + dt_m a_10 k_0) per member m, after a
+//                member whose previous attempt was accepted takes x0 = x and
+//                k_0 = k_(S-1) (the reference swaps these buffers,
+//                rungekutta.cpp:25-84; here the take-over is fused into the
+//                first combination, so no pointer changes between steps and
+//                the step can be replayed from a graph)
 //   for s = 1 .. S-1:
-//     k_bstage   x = normalise(x0 + dt_m sum_j<s a_sj k_j) per member m; at
-//                s = 1 a member whose previous attempt was accepted first
-//                takes x0 = x and k_0 = k_(S-1) (the reference swaps these
-//                buffers, rungekutta.cpp:25-84; here the take-over is fused
-//                into the first combination, so no pointer changes between
-//                steps and the step can be replayed from a graph)
-//     demag passes and field kernel of every member -> k_s
+//     demag passes and field kernel of every member at x_s -> k_s; for
+//                s < S-1 the field kernel also forms
+//                x_(s+1) = normalise(x0 + dt_m sum_j<=s a_(s+1)j k_j)
+//                (the stage states alternate between two buffers)
 //   k_berr       per-member max |dt sum_j (w1_j - w2_j) k_j| (getError,
 //                rungekutta.cpp:140-151) and max |k_(S-1)| (the torque at
 //                the new state, for relaxation)
@@ -21,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <fstream>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -423,6 +427,57 @@ std::map<std::string, TuneEntry>& tune_table() {
 }
 
 }  // namespace
+
+// The record as text, one size class a line: the class, then tab-separated
+// the chosen candidate (-1: undecided) and, per candidate, "usable count
+// best_ms".
+void batch_tune_save(const std::string& path) {
+  std::lock_guard<std::mutex> lk(tune_mu);
+  const std::string tmp = path + ".tmp";
+  {
+    std::ofstream f(tmp);
+    if (!f) throw Error(MPA_ERR_RUNTIME, "batch tuning: cannot write " + tmp);
+    f.precision(17);
+    for (const auto& kv : tune_table()) {
+      const TuneEntry& e = kv.second;
+      f << kv.first << '\t' << e.choice;
+      for (int c = 0; c < TUNE_CANDIDATES; ++c) f << '\t' << (e.usable[c] ? 1 : 0) << ' ' << e.count[c] << ' ' << e.best[c];
+      f << '\n';
+    }
+    if (!f) throw Error(MPA_ERR_RUNTIME, "batch tuning: cannot write " + tmp);
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) throw Error(MPA_ERR_RUNTIME, "batch tuning: cannot write " + path);
+}
+
+void batch_tune_load(const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw Error(MPA_ERR_RUNTIME, "batch tuning: cannot read " + path);
+  std::map<std::string, TuneEntry> read;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty()) continue;
+    const size_t tab = line.find('\t');
+    if (tab == std::string::npos) throw Error(MPA_ERR_INVALID, "batch tuning: malformed line in " + path);
+    std::istringstream in(line.substr(tab + 1));
+    TuneEntry e;
+    in >> e.choice;
+    for (int c = 0; c < TUNE_CANDIDATES; ++c) {
+      int u = 0;
+      in >> u >> e.count[c] >> e.best[c];
+      e.usable[c] = u != 0;
+    }
+    if (!in || e.choice < -1 || e.choice >= TUNE_CANDIDATES)
+      throw Error(MPA_ERR_INVALID, "batch tuning: malformed line in " + path);
+    read[line.substr(0, tab)] = e;
+  }
+  std::lock_guard<std::mutex> lk(tune_mu);
+  for (auto& kv : read) tune_table()[kv.first] = kv.second;
+}
+
+void batch_tune_clear() {
+  std::lock_guard<std::mutex> lk(tune_mu);
+  tune_table().clear();
+}
 
 std::string Batch::size_class() {
   long long cells = 0;

@@ -161,4 +161,10 @@ class Batch {
   int last_path_ = 0;
 };
 
+// The step-path autotuner's record (process-wide): written to and merged
+// from a text file, or cleared.
+void batch_tune_save(const std::string& path);
+void batch_tune_load(const std::string& path);
+void batch_tune_clear();
+
 }  // namespace mpa

@@ -1328,6 +1328,24 @@ int mpa_batch_set_option(mpa_batch* b, const char* key, int value) {
   });
 }
 
+int mpa_batch_tune_save(const char* path) {
+  return guard([&] {
+    need(path, "path");
+    batch_tune_save(path);
+  });
+}
+
+int mpa_batch_tune_load(const char* path) {
+  return guard([&] {
+    need(path, "path");
+    batch_tune_load(path);
+  });
+}
+
+int mpa_batch_tune_clear(void) {
+  return guard([&] { batch_tune_clear(); });
+}
+
 int mpa_bench_dft8(int variant, int nlines, int rounds, const float* in, float* out, double* ms) {
   return guard([&] {
     need(in, "in");

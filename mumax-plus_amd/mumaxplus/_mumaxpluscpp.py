@@ -180,6 +180,9 @@ _SIGS = {
     "mpa_batch_profile_report": ([_P, _cint, _P, _P, _P, _P, _ct.POINTER(_cint)], _cint),
     "mpa_batch_step_path": ([_P, _ct.POINTER(_cint), _ct.POINTER(_cint)], _cint),
     "mpa_bench_dft8": ([_cint, _cint, _cint, _P, _P, _ct.POINTER(_cdouble)], _cint),
+    "mpa_batch_tune_save": ([_ct.c_char_p], _cint),
+    "mpa_batch_tune_load": ([_ct.c_char_p], _cint),
+    "mpa_batch_tune_clear": ([], _cint),
     "mpa_batch_tune_report": ([_P, _cint, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
@@ -1166,6 +1169,18 @@ class StrayField(FieldQuantity):
             n = max(int(self._mask.sum()), 1)
             return [float(v[c][self._mask].sum() / n) for c in range(3)]
         return [float(v[c].mean()) for c in range(3)]
+
+
+def _batch_tune_save(path):
+    _check(_lib.mpa_batch_tune_save(_enc(str(path))))
+
+
+def _batch_tune_load(path):
+    _check(_lib.mpa_batch_tune_load(_enc(str(path))))
+
+
+def _batch_tune_clear():
+    _check(_lib.mpa_batch_tune_clear())
 
 
 def _bench_dft8(variant, lines, rounds):

@@ -156,6 +156,22 @@ class Batch:
             out[parts[0]] = entry
         return out
 
+    @staticmethod
+    def save_tuning(path):
+        """Write the autotuner's record (every size class of this process) to
+        a text file."""
+        _cpp._batch_tune_save(path)
+
+    @staticmethod
+    def load_tuning(path):
+        """Merge a record written by save_tuning: its decided size classes
+        are used without timing them again."""
+        _cpp._batch_tune_load(path)
+
+    @staticmethod
+    def clear_tuning():
+        _cpp._batch_tune_clear()
+
     def set_profiling(self, enable):
         self._impl.set_option("profiling", bool(enable))
 

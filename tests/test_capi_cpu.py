@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "mumaxplus_amd.h")
@@ -38,3 +40,33 @@ def test_error_path_without_device():
     rc = b._lib.mpa_ts_step(None)
     assert rc == b.MPA_ERR_INVALID
     assert b"NULL" in b._lib.mpa_last_error()
+
+
+def test_batch_tuning_record_round_trip(tmp_path):
+    """The step-path autotuner's record: a file merged into the (process-wide)
+    record and written back gives the same entries (no device needed)."""
+    from mumaxplus import _mumaxpluscpp as cpp
+
+    src = tmp_path / "tune.txt"
+    src.write_text("cells<2^12 members<2^2 line<=2^6 3d DormandPrince\t0\t1 2 0.19\t1 2 0.73\t1 2 0.41\t1 2 0.31\n"
+                   "cells<2^21 members<2^9 line<=2^7 3d DormandPrince\t-1\t1 1 1.3\t0 0 0\t0 0 0\t0 0 0\n")
+    cpp._batch_tune_clear()
+    cpp._batch_tune_load(src)
+    out = tmp_path / "out.txt"
+    cpp._batch_tune_save(out)
+    rows = sorted(line.split("\t") for line in out.read_text().splitlines())
+    want = sorted(line.split("\t") for line in src.read_text().splitlines())
+    assert [r[0] for r in rows] == [w[0] for w in want]
+    for r, w in zip(rows, want):
+        assert int(r[1]) == int(w[1])
+        for a, b in zip(r[2:], w[2:]):
+            ua, ca, ba = a.split()
+            ub, cb, bb = b.split()
+            assert (ua, ca) == (ub, cb) and float(ba) == float(bb)
+    cpp._batch_tune_clear()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("no tab here\n")
+    with pytest.raises(ValueError, match="malformed"):
+        cpp._batch_tune_load(bad)
+    with pytest.raises(RuntimeError, match="cannot read"):
+        cpp._batch_tune_load(tmp_path / "missing.txt")
