@@ -87,6 +87,9 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
   const int z = idx / (d.nx * d.ny);
   const float3 mc = load3(m, idx, N);
   const float ms = S(d.msat, idx);
+  // one reciprocal each for the divisions by msat and the cell sizes (the
+  // field is VALU-bound and an IEEE division is about ten instructions)
+  const float ims = 1.f / ms;
   float edens = 0.f;
 
   float3 hex = f3(0.f, 0.f, 0.f), hdm = f3(0.f, 0.f, 0.f);
@@ -123,7 +126,8 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
     float D[D_COUNT];
 #pragma unroll
     for (int k = 0; k < D_COUNT; ++k) D[k] = S(d.dmi[k], idx);
-    const float w[3] = {1.f / (d.cx * d.cx), 1.f / (d.cy * d.cy), 1.f / (d.cz * d.cz)};
+    const float w[3] = {icsz[0] * icsz[0], icsz[1] * icsz[1], icsz[2] * icsz[2]};
+    const float h2a = 0.5f / a;  // delta / (2 a) = delta h2a at a Neumann boundary
     const bool openbc = d.openbc != 0;
     const bool open_dmi = openbc || a == 0.f;
 #pragma unroll
@@ -150,7 +154,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
           // sublattice variant, exchange.cu:106-202 (no open boundaries)
           if (!nb_in) {
             if (a == 0.f) use = false;
-            mn = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
+            mn = mc + (delta * h2a) * (an * cross3(cross3(dm2, mc), mc) + gamma);
             an_ = a;
           } else {
             if (S(d.msat, nb) == 0.f) use = false;
@@ -161,7 +165,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
         } else if (!nb_in) {
           // exchange.cu:61-101
           if (openbc || a == 0.f) use = false;
-          mn = mc + (delta / (2 * a)) * gamma;
+          mn = mc + (delta * h2a) * gamma;
           an_ = a;
         } else {
           if (openbc && S(d.msat, nb) == 0.f) use = false;
@@ -191,7 +195,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
           const bool skip = open_dmi && (!nb_in || S(d.msat, nbi) == 0.f);
           if (!skip) {
             float3 mn;
-            if (!nb_in && !open_dmi) mn = mc + (delta / (2 * a)) * gamma;
+            if (!nb_in && !open_dmi) mn = mc + (delta * h2a) * gamma;
             else mn = mn_[dir];
             hdm.x += (Dxy * mn.y + Dxz * mn.z) * idelta;
             hdm.y += (-Dxy * mn.x + Dyz * mn.z) * idelta;
@@ -208,8 +212,8 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
             if (!nb_in) {
               // Neumann boundary; both sublattices use this sublattice's own tensor
               const float3 g2 = dmi_gamma(D, axis, m2c);
-              m1n = mc + (delta / (2 * a)) * (an * cross3(cross3(dm2, mc), mc) + gamma);
-              m2n = m2c + (delta / (2 * a)) * (an * cross3(cross3(idelta * (m1n - mc), m2c), m2c) + g2);
+              m1n = mc + (delta * h2a) * (an * cross3(cross3(dm2, mc), mc) + gamma);
+              m2n = m2c + (delta * h2a) * (an * cross3(cross3(idelta * (m1n - mc), m2c), m2c) + g2);
             } else {
               m1n = mn_[dir];
               m2n = m2n_[dir];
@@ -223,7 +227,6 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
         }
       }
     }
-    const float ims = 1.f / ms;
     if (do_ex) hex = ims * hex;
     if (do_dmi) hdm = ims * hdm;
   }
@@ -239,7 +242,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
 #pragma unroll
       for (int c = 0; c < 6; ++c) e[c] = d.strain[(size_t)c * N + idx];
     }
-    const float b1 = S(d.B1, idx), b2 = S(d.B2, idx), f = -2.f / ms;
+    const float b1 = S(d.B1, idx), b2 = S(d.B2, idx), f = -2.f * ims;
     hme = f3(f * (b1 * e[0] * mc.x + b2 * (e[3] * mc.y + e[4] * mc.z)),
              f * (b1 * e[1] * mc.y + b2 * (e[5] * mc.z + e[3] * mc.x)),
              f * (b1 * e[2] * mc.z + b2 * (e[4] * mc.x + e[5] * mc.y)));
@@ -252,10 +255,11 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
     const float ms2 = S(d.msat2, idx);
     if (ms2 != 0.f && (d.terms & TERM_AFM_SITE)) {
       const float l = S(d.latcon, idx);
-      hafm_site = (4.f * S(d.afmex_cell, idx) / (l * l) / ms2) * m2c;
+      hafm_site = (4.f * S(d.afmex_cell, idx) / (l * l * ms2)) * m2c;
     }
     if (ms2 != 0.f && (d.terms & TERM_AFM_NN)) {
       const float a = S(d.aex, idx);
+      const float h2a = 0.5f / a;
       const float ann = S(d.afmex_nn, idx);
       const bool open_nn = d.openbc != 0 || a == 0.f;
       float D[D_COUNT];
@@ -279,14 +283,14 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
           const float3 g2 = dmi_gamma(D, axis, m2c);
           float3 dm1 = f3(0.f, 0.f, 0.f);
           if (nin[dir ^ 1]) dm1 = idelta * (mc - mn_[dir ^ 1]);
-          m2n = m2c + (delta / (2 * a)) * (ann * cross3(cross3(dm1, m2c), m2c) + g2);
+          m2n = m2c + (delta * h2a) * (ann * cross3(cross3(dm1, m2c), m2c) + g2);
           ann_ = ann;
         }
         float A = HM(ann, ann_);
         if (nb_in) A = link(A, nb, d.inter_afm, d.scale_afm);
         hafm_nn = hafm_nn + (A * (idelta * idelta)) * (m2n - m2c);
       }
-      hafm_nn = hafm_nn / ms2;
+      hafm_nn = (1.f / ms2) * hafm_nn;
     }
   }
 
@@ -296,7 +300,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
       const float3 u = normalized3(V(d.anisU, idx));
       const float k1 = S(d.ku1, idx), k2 = S(d.ku2, idx);
       const float mu = dot3(mc, u);
-      han = ((2 * k1 * mu + 4 * k2 * mu * mu * mu) / ms) * u;
+      han = ((2 * k1 * mu + 4 * k2 * mu * mu * mu) * ims) * u;
       if (mode == MODE_EDENS) edens += -k1 * mu * mu - k2 * mu * mu * mu * mu;
     } else if (d.terms & TERM_CUBIC) {
       const float3 c1 = normalized3(V(d.anisC1, idx));
@@ -305,9 +309,9 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
       const float kc1 = S(d.kc1, idx), kc2 = S(d.kc2, idx), kc3 = S(d.kc3, idx);
       const float a1 = dot3(c1, mc), a2 = dot3(c2, mc), a3 = dot3(c3, mc);
       const float s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
-      han = (-2 * kc1 / ms) * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
-            (-2 * kc2 / ms) * ((s2 * s3) * a1 * c1 + (s1 * s3) * a2 * c2 + (s1 * s2) * a3 * c3) +
-            (-4 * kc3 / ms) * ((s2 * s2 + s3 * s3) * (s1 * a1) * c1 + (s1 * s1 + s3 * s3) * (s2 * a2) * c2 +
+      han = (-2 * kc1 * ims) * ((s2 + s3) * a1 * c1 + (s1 + s3) * a2 * c2 + (s1 + s2) * a3 * c3) +
+            (-2 * kc2 * ims) * ((s2 * s3) * a1 * c1 + (s1 * s3) * a2 * c2 + (s1 * s2) * a3 * c3) +
+            (-4 * kc3 * ims) * ((s2 * s2 + s3 * s3) * (s1 * a1) * c1 + (s1 * s1 + s3 * s3) * (s2 * a2) * c2 +
                                (s1 * s1 + s2 * s2) * (s3 * a3) * c3);
       if (mode == MODE_EDENS)
         edens += kc1 * (s1 * s2 + s1 * s3 + s2 * s3) + kc2 * s1 * s2 * s3 +
