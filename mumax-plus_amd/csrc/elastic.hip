@@ -30,14 +30,33 @@ __device__ __forceinline__ int shifted(const ElDesc& d, int x, int y, int z, int
 
 __device__ __forceinline__ bool in_geo(const ElDesc& d, int i) { return i >= 0 && (!d.geom || d.geom[i]); }
 
+// The cells -2, -1, +1, +2 steps along an axis from a cell (after the
+// mastergrid wrap; -1 outside the magnet grid) and whether each is in the
+// geometry: computed once per cell and axis, shared by every derivative and
+// divergence the cell takes along that axis.
+struct AxisNb {
+  int i[4];  // -2, -1, +1, +2
+  bool g[4];
+};
+
+__device__ __forceinline__ AxisNb axis_nb(const ElDesc& d, int x, int y, int z, int axis) {
+  AxisNb a;
+  const int k[4] = {-2, -1, 1, 2};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    a.i[q] = shifted(d, x, y, z, axis, k[q]);
+    a.g[q] = in_geo(d, a.i[q]);
+  }
+  return a;
+}
+
 // d f / d axis of a 3-vector field with the stencil chain of
 // straintensor.cu:45-79 (shared by magnetoelasticforce.cu:42-76).  Every
 // branch reads only cells it has checked to be in the geometry.
-__device__ float3 deriv(const ElDesc& d, const float* f, int x, int y, int z, int axis, float3 f0) {
+__device__ __forceinline__ float3 deriv(const ElDesc& d, const float* f, const AxisNb& nb, int axis, float3 f0) {
   const int N = d.ncell;
-  const int im2 = shifted(d, x, y, z, axis, -2), im1 = shifted(d, x, y, z, axis, -1);
-  const int ip1 = shifted(d, x, y, z, axis, 1), ip2 = shifted(d, x, y, z, axis, 2);
-  const bool gm2 = in_geo(d, im2), gm1 = in_geo(d, im1), gp1 = in_geo(d, ip1), gp2 = in_geo(d, ip2);
+  const int im2 = nb.i[0], im1 = nb.i[1], ip1 = nb.i[2], ip2 = nb.i[3];
+  const bool gm2 = nb.g[0], gm1 = nb.g[1], gp1 = nb.g[2], gp2 = nb.g[3];
   float3 r;
   if (!gm1 && !gp1) {
     r = f3(0.f, 0.f, 0.f);                                                   // --1--
@@ -73,9 +92,9 @@ __global__ void __launch_bounds__(BLOCK) k_strain(ElDesc d, const float* __restr
   }
   const int x = idx % d.nx, y = (idx / d.nx) % d.ny, z = idx / (d.nx * d.ny);
   const float3 u0 = ld3(u, idx, N);
-  const float3 dx = deriv(d, u, x, y, z, 0, u0);  // d_x u
-  const float3 dy = deriv(d, u, x, y, z, 1, u0);
-  const float3 dz = deriv(d, u, x, y, z, 2, u0);
+  const float3 dx = deriv(d, u, axis_nb(d, x, y, z, 0), 0, u0);  // d_x u
+  const float3 dy = deriv(d, u, axis_nb(d, x, y, z, 1), 1, u0);
+  const float3 dz = deriv(d, u, axis_nb(d, x, y, z, 2), 2, u0);
   // straintensor.cu:81-97: e_ii = d_i u_i, e_ij = (d_i u_j + d_j u_i)/2
   const float e[6] = {dx.x, dy.y, dz.z, 0.5f * (dx.y + dy.x), 0.5f * (dx.z + dz.x), 0.5f * (dy.z + dz.y)};
   for (int c = 0; c < 6; ++c) strain[(size_t)c * N + idx] = e[c];
@@ -98,15 +117,15 @@ __device__ __forceinline__ float3 stress_row(const float* s, int j, int N, int i
 }
 
 // f_j = d_i sigma_ij with traction-free boundaries, elasticforce.cu:47-98.
-__device__ float3 stress_divergence(const ElDesc& d, const float* s, int idx, int x, int y, int z) {
+__device__ __forceinline__ float3 stress_divergence(const ElDesc& d, const float* s, int idx, const AxisNb (&nb)[3]) {
   const int N = d.ncell;
   const float c[3] = {d.cx, d.cy, d.cz};
   float3 f = f3(0.f, 0.f, 0.f);
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float w = 1.0f / c[i];
-    const int im2 = shifted(d, x, y, z, i, -2), im1 = shifted(d, x, y, z, i, -1);
-    const int ip1 = shifted(d, x, y, z, i, 1), ip2 = shifted(d, x, y, z, i, 2);
-    const bool gm2 = in_geo(d, im2), gm1 = in_geo(d, im1), gp1 = in_geo(d, ip1), gp2 = in_geo(d, ip2);
+    const int im2 = nb[i].i[0], im1 = nb[i].i[1], ip1 = nb[i].i[2], ip2 = nb[i].i[3];
+    const bool gm2 = nb[i].g[0], gm1 = nb[i].g[1], gp1 = nb[i].g[2], gp2 = nb[i].g[3];
     if (!gm1 && !gp1) continue;
     if (!gm1) {
       f = f + w * (stress_row(s, idx, N, i) + (1.f / 3.f) * stress_row(s, ip1, N, i));
@@ -123,10 +142,10 @@ __device__ float3 stress_divergence(const ElDesc& d, const float* s, int idx, in
 }
 
 // magnetoelasticforce.cu:82-91 with der[k] = d_k m.
-__device__ float3 me_force(const ElDesc& d, const MEForce& a, int idx, int x, int y, int z) {
+__device__ __forceinline__ float3 me_force(const ElDesc& d, const MEForce& a, int idx, const AxisNb (&nb)[3]) {
   const int N = d.ncell;
   const float3 m = ld3(a.m, idx, N);
-  const float3 d0 = deriv(d, a.m, x, y, z, 0, m), d1 = deriv(d, a.m, x, y, z, 1, m), d2 = deriv(d, a.m, x, y, z, 2, m);
+  const float3 d0 = deriv(d, a.m, nb[0], 0, m), d1 = deriv(d, a.m, nb[1], 1, m), d2 = deriv(d, a.m, nb[2], 2, m);
   const float b1 = a.B1.at(idx), b2 = a.B2.at(idx);
   return f3(2.f * b1 * m.x * d0.x + b2 * m.x * (d1.y + d2.z) + b2 * (m.y * d1.x + m.z * d2.x),
             2.f * b1 * m.y * d1.y + b2 * m.y * (d2.z + d0.x) + b2 * (m.z * d2.y + m.x * d0.y),
@@ -142,11 +161,14 @@ __global__ void __launch_bounds__(BLOCK) k_elastic(ElDesc d, const float* __rest
   float3 f = f3(0.f, 0.f, 0.f);
   if (in_geo(d, idx)) {
     const int x = idx % d.nx, y = (idx / d.nx) % d.ny, z = idx / (d.nx * d.ny);
+    AxisNb nb[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) nb[ax] = axis_nb(d, x, y, z, ax);
     const bool body = mode == EL_BODY || mode == EL_ACCEL;
-    if (mode == EL_FORCE || (body && d.elastic)) f = f + stress_divergence(d, stress, idx, x, y, z);
+    if (mode == EL_FORCE || (body && d.elastic)) f = f + stress_divergence(d, stress, idx, nb);
     if (body && d.ext) f = f + d.fext.at(idx, N);
-    if ((mode == EL_MEFORCE || body) && a.m) f = f + me_force(d, a, idx, x, y, z);
-    if (body && b.m) f = f + me_force(d, b, idx, x, y, z);
+    if ((mode == EL_MEFORCE || body) && a.m) f = f + me_force(d, a, idx, nb);
+    if (body && b.m) f = f + me_force(d, b, idx, nb);
     if (mode == EL_DAMPING || (mode == EL_ACCEL && d.damping)) f = f + (-d.eta.at(idx)) * ld3(v, idx, N);
     if (mode == EL_ACCEL && d.divrho) {
       const float p = d.rho.at(idx);
