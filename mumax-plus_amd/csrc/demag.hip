@@ -1130,8 +1130,18 @@ __device__ __noinline__ void persist_field(const bk::PersistArgs* a, const bk::P
   a = uni(a);
   P = uni(P);
   const FBInst* tab = static_cast<const FBInst*>(a->ftab);
-  if (P->has_next) field_uniform_b_block<true>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next);
-  else field_uniform_b_block<false>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next);
+  if (!P->has_next) {
+    field_uniform_b_block<0>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next);
+    return;
+  }
+  switch (P->next.nk) {
+    case 1: field_uniform_b_block<1>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+    case 2: field_uniform_b_block<2>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+    case 3: field_uniform_b_block<3>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+    case 4: field_uniform_b_block<4>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+    case 5: field_uniform_b_block<5>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+    default: field_uniform_b_block<-1>(tab, a->fb2i, P->x, a->HD, P->out, a->cs, uni(v), P->next); break;
+  }
 }
 
 __device__ __noinline__ void persist_err(const bk::PersistArgs* a, const bk::ErrArgs* ea, int v,

@@ -258,6 +258,9 @@ struct FBInst {
 
 // Epilogue of the batched field kernel: the member's next stage state at
 // the cell (BatchNext), from the derivative r in registers.  c[j] = dt a_j.
+// NK > 0: n.nk == NK known at compile time, so every load is issued before
+// the sums (NK < 0: n.nk read at run time).
+template <int NK>
 struct BatchNextCell {
   const BatchNext& n;
   const float* c;
@@ -265,13 +268,18 @@ struct BatchNextCell {
   __device__ __forceinline__ void operator()(int idx, float3 mc, float3 r, bool in) const {
     const size_t p = (size_t)off + idx;
     const float rr[3] = {r.x, r.y, r.z};
+    const int nk = NK > 0 ? NK : n.nk;
     float v[3];
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
       const size_t i = (size_t)comp * cs + p;
       float a = n.x0[i];
-      for (int j = 0; j < n.nk; ++j) a += c[j] * n.k[j][i];
-      a += c[n.nk] * rr[comp];
+#pragma unroll
+      for (int j = 0; j < (NK > 0 ? NK : 7); ++j) {
+        if (NK < 0 && j >= nk) break;
+        a += c[j] * n.k[j][i];
+      }
+      a += c[nk] * rr[comp];
       v[comp] = a;
     }
     const float inv = rsqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -282,9 +290,9 @@ struct BatchNextCell {
 
 // Block b of a batched field launch: 256 consecutive cells of the member
 // b2i[b] (every member starts on a multiple of 256 cells of the packed
-// arrays X, HD and OUT, whose component stride is cs).  NEXT: also the next
-// stage state of an active member (BatchNext).
-template <bool NEXT>
+// arrays X, HD and OUT, whose component stride is cs).  NK != 0: also the
+// next stage state of an active member (BatchNext; NK > 0 its nk, -1 any).
+template <int NK>
 __device__ __forceinline__ void field_uniform_b_block(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
                                                       const float* __restrict__ X, const float* __restrict__ HD,
                                                       float* __restrict__ OUT, int cs, int b, const BatchNext& nx) {
@@ -295,14 +303,15 @@ __device__ __forceinline__ void field_uniform_b_block(const FBInst* __restrict__
   const int x = j % nxc, yz = j / nxc, y = yz % ny, z = yz / ny;
   const float* hd = HD + I.off;
   auto hdem = [&](int i) { return load3(hd, i, cs); };
-  if constexpr (NEXT) {
+  if constexpr (NK != 0) {
     const BatchMember& S = nx.st[I.slot];
     if (S.active) {
+      const int nk = NK > 0 ? NK : nx.nk;
       float c[7];
 #pragma unroll
-      for (int q = 0; q < 7; ++q) c[q] = q <= nx.nk ? (float)(S.dt * nx.a[q]) : 0.f;
+      for (int q = 0; q < 7; ++q) c[q] = q <= nk ? (float)(S.dt * nx.a[q]) : 0.f;
       field_uniform_cell(I.d, I.u, X + I.off, hdem, OUT + I.off, I.mode, x, y, z, cs,
-                         BatchNextCell{nx, c, I.off, cs});
+                         BatchNextCell<NK>{nx, c, I.off, cs});
       return;
     }
   }

@@ -631,11 +631,11 @@ __global__ void __launch_bounds__(TILE_X* TILE_Y) k_field_tile(FMDesc d, FMUnifo
 
 // Batched uniform path: 256 consecutive cells of one magnet per block (every
 // magnet starts on a multiple of 256 cells of the packed arrays).
-template <bool NEXT>
+template <int NK>
 __global__ void __launch_bounds__(BLOCK) k_field_uniform_b(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
                                                            const float* __restrict__ X, const float* __restrict__ HD,
                                                            float* __restrict__ OUT, int cs, BatchNext nx) {
-  field_uniform_b_block<NEXT>(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x), nx);
+  field_uniform_b_block<NK>(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x), nx);
 }
 
 // Every parameter the per-cell kernel reads is uniform.
@@ -754,9 +754,17 @@ void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t
   const FBInst* tab = reinterpret_cast<const FBInst*>(tab_.get());
   if (next) {
     if (next->nk < 0 || next->nk > 6) throw Error(MPA_ERR_RUNTIME, "batched field: bad stage count");
-    k_field_uniform_b<true><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, *next);
+    const BatchNext& n = *next;
+    switch (n.nk) {
+      case 1: k_field_uniform_b<1><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+      case 2: k_field_uniform_b<2><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+      case 3: k_field_uniform_b<3><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+      case 4: k_field_uniform_b<4><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+      case 5: k_field_uniform_b<5><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+      default: k_field_uniform_b<-1><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, n); break;
+    }
   } else {
-    k_field_uniform_b<false><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, BatchNext{});
+    k_field_uniform_b<0><<<nblocks_, BLOCK, 0, s>>>(tab, b2i_.get(), X, HD, OUT, cs_, BatchNext{});
   }
   checkLaunch("k_field_uniform_b");
 }
