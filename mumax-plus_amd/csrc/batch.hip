@@ -277,7 +277,8 @@ void Batch::remove(int id) {
 void Batch::rebuild() {
   drop_graph();
   std::vector<DemagBatch::Member> dm;
-  std::vector<FieldBatchMember> fb;
+  std::vector<FieldBatchMember> fb, fb0, fb1;
+  fbytes0_ = fbytes1_ = 0.0;
   for (size_t id = 0; id < slots_.size(); ++id) {
     const Slot& sl = slots_[id];
     if (!sl.used) continue;
@@ -287,14 +288,24 @@ void Batch::rebuild() {
     const FMDesc& d = sl.desc;
     if (sl.demag) dm.push_back({&fm->demag(), sl.off, PS{nullptr, d.msat.v}, fm->geom_ptr()});
     fb.push_back({d, (int)sl.off, sl.mode == BATCH_RELAX ? MODE_DAMPING : MODE_LLG, (int)id});
+    // single layers with demag form the demag batch's second group
+    const bool g1 = sl.demag && d.nz == 1;
+    (g1 ? fb1 : fb0).push_back(fb.back());
+    (g1 ? fbytes1_ : fbytes0_) += 12.0 * sl.ncell * (sl.demag ? 3 : 2);
   }
   demag_ = dm.empty() ? nullptr : std::make_unique<DemagBatch>(dm, cs_);
   field_ = std::make_unique<FieldBatch>(fb, (int)cs_);
+  field0_ = std::make_unique<FieldBatch>(fb0, (int)cs_);
+  field1_ = std::make_unique<FieldBatch>(fb1, (int)cs_);
   HIP_CHECK(hipMemcpyAsync(c2m_.get(), c2m_host_.data(), c2m_host_.size() * sizeof(int), hipMemcpyHostToDevice,
                            stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   dirty_ = false;
   pdirty_ = true;
+}
+
+bool Batch::two_chains() const {
+  return demag_ && demag_->has_group(1) && field0_->nblocks() > 0 && field1_->nblocks() > 0;
 }
 
 void Batch::eval_rhs(const float* x, float* out, const BatchNext* next) {
@@ -369,13 +380,40 @@ void Batch::record_step() {
                                                 K_[S - 1].get(), G_.get(), c2m_.get(), st_.get(), (int)cs_, (int)nch);
     checkLaunch("k_bstage");
   }
-  for (int s = 1; s < S; ++s) {
-    if (s + 1 < S) {
-      const BatchNext n = next_stage(s);
-      eval_rhs(stage_state(s), K_[s].get(), &n);
-    } else {
-      eval_rhs(stage_state(s), K_[s].get(), nullptr);
+  if (!two_chains()) {
+    for (int s = 1; s < S; ++s) {
+      if (s + 1 < S) {
+        const BatchNext n = next_stage(s);
+        eval_rhs(stage_state(s), K_[s].get(), &n);
+      } else {
+        eval_rhs(stage_state(s), K_[s].get(), nullptr);
+      }
     }
+  } else {
+    // The single layers with demag depend on the other members nowhere in
+    // the stages (their cells, spectra and stage states are their own), so
+    // their whole chain of stages runs on the second stream, forked after
+    // the stage combination and joined before the error norms.
+    HIP_CHECK(hipEventRecord(fork_, stream));
+    HIP_CHECK(hipStreamWaitEvent(stream2, fork_, 0));
+    for (int s = 1; s < S; ++s) {
+      const bool has_next = s + 1 < S;
+      const BatchNext n = has_next ? next_stage(s) : BatchNext{};
+      const double nb = has_next ? 12.0 * (n.nk + 2) : 0.0;
+      const float* x = stage_state(s);
+      for (int g = 0; g < 2; ++g) {
+        hipStream_t st = g == 0 ? stream : stream2;
+        const FieldBatch& fbat = g == 0 ? *field0_ : *field1_;
+        demag_->exec_group(x, HD_.get(), g, st, &prof);
+        double cells = 0.0;
+        for (const Slot& sl : slots_)
+          if (sl.used && ((sl.demag && sl.desc.nz == 1) == (g == 1))) cells += (double)sl.ncell;
+        Timed t(&prof, "k_field_uniform_b", (g == 0 ? fbytes0_ : fbytes1_) + nb * cells, st);
+        fbat.launch(x, HD_.get(), K_[s].get(), st, has_next ? &n : nullptr);
+      }
+    }
+    HIP_CHECK(hipEventRecord(join_, stream2));
+    HIP_CHECK(hipStreamWaitEvent(stream, join_, 0));
   }
   ErrArgs ea{};
   ea.nk = S;

@@ -127,6 +127,7 @@ class Batch {
   void persist_recover(int err);
   std::string size_class();
   void eval_rhs(const float* x, float* out, const BatchNext* next);
+  bool two_chains() const;
   float* stage_state(int s);
   BatchNext next_stage(int s);
   void drop_graph();
@@ -145,7 +146,12 @@ class Batch {
   DeviceArray<unsigned> errbits_, tqbits_;
   std::vector<int> c2m_host_;
   std::unique_ptr<DemagBatch> demag_;
-  std::unique_ptr<FieldBatch> field_;
+  std::unique_ptr<FieldBatch> field_;   // every member
+  // the same split by demag group: single layers with demag (field1_) and
+  // the others (field0_); the step runs the two groups' whole chains on two
+  // streams
+  std::unique_ptr<FieldBatch> field0_, field1_;
+  double fbytes0_ = 0.0, fbytes1_ = 0.0;  // field launch bytes per group (without the next stage)
   bool dirty_ = true;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;

@@ -1535,6 +1535,21 @@ void batch_persist_launch(const bk::PersistArgs* a, int nsteps, unsigned gen0, i
                                        dim3(256), params, (unsigned)shm, s));
 }
 
+void DemagBatch::exec_group(const float* X, float* H, int group, hipStream_t s, Profiler* prof) const {
+  static const char* names[] = {"k_xfwd_b", "k_yfwd_b", "k_zpass_b", "k_yinv_b", "k_ypass2d_b", "k_xinv_b"};
+  for (const Launch& L : launches_) {
+    if (L.group != group) continue;
+    Timed t(prof, names[L.pass], L.bytes, s);
+    run_batch_pass(L.pass, L.nblocks, L.shm, reinterpret_cast<const BSeg*>(L.segs.get()), L.b2s.get(), X, H, s);
+  }
+}
+
+bool DemagBatch::has_group(int group) const {
+  for (const Launch& L : launches_)
+    if (L.group == group) return true;
+  return false;
+}
+
 double DemagBatch::algorithmic_bytes() const {
   double b = 0.0;
   for (const Launch& L : launches_) b += L.bytes;

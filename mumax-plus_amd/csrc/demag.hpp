@@ -111,6 +111,10 @@ class DemagBatch {
   // through the events; works under graph capture).
   void exec(const float* X, float* H, hipStream_t s, Profiler* prof, hipStream_t s2 = nullptr,
             hipEvent_t fork = nullptr, hipEvent_t join = nullptr) const;
+  // The passes of one group only (0: members with a z transform, 1: single
+  // layers), on stream s.
+  void exec_group(const float* X, float* H, int group, hipStream_t s, Profiler* prof) const;
+  bool has_group(int group) const;
   double algorithmic_bytes() const;
   int nlaunches() const { return (int)launches_.size(); }
   // The launches' block tables, in launch order (each group's passes in
