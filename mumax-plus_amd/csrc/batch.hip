@@ -12,10 +12,12 @@
 //     demag passes and field kernel of every member at x_s -> k_s; for
 //                s < S-1 the field kernel also forms
 //                x_(s+1) = normalise(x0 + dt_m sum_j<=s a_(s+1)j k_j)
-//                (the stage states alternate between two buffers)
-//   k_berr       per-member max |dt sum_j (w1_j - w2_j) k_j| (getError,
+//                (the stage states alternate between two buffers); at
+//                s = S-1 it forms the per-member max
+//                |dt sum_j (w1_j - w2_j) k_j| (getError,
 //                rungekutta.cpp:140-151) and max |k_(S-1)| (the torque at
-//                the new state, for relaxation)
+//                the new state, for relaxation); k_berr does the same from
+//                memory when a member is primed
 //   k_bcontrol   per-member step controller (rungekutta.cpp:69-74,
 //                timesolver.cpp:85-96), accept / reject, time, and the
 //                stopping rules (TimeSolver::run, timesolver.cpp:97-111)
@@ -308,13 +310,30 @@ bool Batch::two_chains() const {
   return demag_ && demag_->has_group(1) && field0_->nblocks() > 0 && field1_->nblocks() > 0;
 }
 
-void Batch::eval_rhs(const float* x, float* out, const BatchNext* next) {
+void Batch::eval_rhs(const float* x, float* out, const BatchNext* next, const BatchErr* err) {
   if (demag_) demag_->exec(x, HD_.get(), stream, &prof, stream2, fork_, join_);
   double bytes = 0.0;
   for (const Slot& sl : slots_)
     if (sl.used) bytes += 12.0 * sl.ncell * ((sl.demag ? 3 : 2) + (next ? next->nk + 2 : 0));
   Timed t(&prof, "k_field_uniform_b", bytes, stream);
-  field_->launch(x, HD_.get(), out, stream, next);
+  if (err) field_->launch_err(x, HD_.get(), out, stream, *err);
+  else field_->launch(x, HD_.get(), out, stream, next);
+}
+
+BatchErr Batch::step_error() {
+  // the error norms of the step, formed in the last stage's field launch
+  const Tableau& tb = tableau_by_name(method_);
+  const int S = (int)tb.nodes.size();
+  BatchErr e{};
+  e.nk = S - 1;
+  for (int j = 0; j < S; ++j) {
+    if (j < S - 1) e.k[j] = K_[j].get();
+    e.e[j] = tb.w1[j] - tb.w2[j];
+  }
+  e.errbits = errbits_.get();
+  e.tqbits = tqbits_.get();
+  e.st = st_.get();
+  return e;
 }
 
 float* Batch::stage_state(int s) {
@@ -347,7 +366,7 @@ void Batch::prime(const std::vector<int>& fresh) {
   // k_(S-1) = rhs(x) for every member: unchanged for members whose last
   // attempt was accepted (x is that attempt's last stage state), unused by
   // the others, and the fresh members' first derivative.
-  eval_rhs(X_.get(), K_[S - 1].get(), nullptr);
+  eval_rhs(X_.get(), K_[S - 1].get(), nullptr, nullptr);
   ErrArgs ea{};
   ea.nk = 0;
   ea.tq = K_[S - 1].get();
@@ -380,13 +399,14 @@ void Batch::record_step() {
                                                 K_[S - 1].get(), G_.get(), c2m_.get(), st_.get(), (int)cs_, (int)nch);
     checkLaunch("k_bstage");
   }
+  const BatchErr err = step_error();
   if (!two_chains()) {
     for (int s = 1; s < S; ++s) {
       if (s + 1 < S) {
         const BatchNext n = next_stage(s);
-        eval_rhs(stage_state(s), K_[s].get(), &n);
+        eval_rhs(stage_state(s), K_[s].get(), &n, nullptr);
       } else {
-        eval_rhs(stage_state(s), K_[s].get(), nullptr);
+        eval_rhs(stage_state(s), K_[s].get(), nullptr, &err);
       }
     }
   } else {
@@ -409,23 +429,12 @@ void Batch::record_step() {
         for (const Slot& sl : slots_)
           if (sl.used && ((sl.demag && sl.desc.nz == 1) == (g == 1))) cells += (double)sl.ncell;
         Timed t(&prof, "k_field_uniform_b", (g == 0 ? fbytes0_ : fbytes1_) + nb * cells, st);
-        fbat.launch(x, HD_.get(), K_[s].get(), st, has_next ? &n : nullptr);
+        if (has_next) fbat.launch(x, HD_.get(), K_[s].get(), st, &n);
+        else fbat.launch_err(x, HD_.get(), K_[s].get(), st, err);
       }
     }
     HIP_CHECK(hipEventRecord(join_, stream2));
     HIP_CHECK(hipStreamWaitEvent(stream, join_, 0));
-  }
-  ErrArgs ea{};
-  ea.nk = S;
-  for (int j = 0; j < S; ++j) {
-    ea.k[j] = K_[j].get();
-    ea.e[j] = tb.w1[j] - tb.w2[j];
-  }
-  ea.tq = K_[S - 1].get();
-  {
-    Timed t(&prof, "k_berr", 0.0, stream);
-    k_berr<<<nch, CH, 0, stream>>>(ea, G_.get(), c2m_.get(), st_.get(), errbits_.get(), tqbits_.get(), (int)cs_);
-    checkLaunch("k_berr");
   }
   CtlArgs c{tb.order1, tb.order2, 0};
   const int ns = (int)slots_.size();
@@ -883,7 +892,7 @@ int Batch::launches_per_step() {
   if (dirty_) rebuild();
   const Tableau& tb = tableau_by_name(method_);
   const int S = (int)tb.nodes.size();
-  return (S - 1) * (1 + (demag_ ? demag_->nlaunches() : 0)) + 3;
+  return (S - 1) * (1 + (demag_ ? demag_->nlaunches() : 0)) + 2;
 }
 
 }  // namespace mpa

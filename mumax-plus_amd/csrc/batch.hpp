@@ -126,7 +126,8 @@ class Batch {
   int grid_for(int blocks_per_cu, int cap) const;
   void persist_recover(int err);
   std::string size_class();
-  void eval_rhs(const float* x, float* out, const BatchNext* next);
+  void eval_rhs(const float* x, float* out, const BatchNext* next, const BatchErr* err);
+  BatchErr step_error();
   bool two_chains() const;
   float* stage_state(int s);
   BatchNext next_stage(int s);

@@ -319,6 +319,84 @@ __device__ __forceinline__ void field_uniform_b_block(const FBInst* __restrict__
 }
 
 
+// Epilogue of the last stage's batched field launch: the cell's error norm
+// and torque norm (BatchErr) into err and tq.  c[j] = dt e_j.
+struct BatchErrCell {
+  const BatchErr& e;
+  const float* c;
+  int off, cs;
+  bool want_err, want_tq;
+  float& err;
+  float& tq;
+  __device__ __forceinline__ void operator()(int idx, float3 mc, float3 r, bool in) const {
+    if (!in) return;
+    const size_t p = (size_t)off + idx;
+    if (want_err) {
+      const float rr[3] = {r.x, r.y, r.z};
+      float ev[3];
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {
+        const size_t i = (size_t)comp * cs + p;
+        float a = 0.f;
+        for (int j = 0; j < e.nk; ++j) a += c[j] * e.k[j][i];
+        a += c[e.nk] * rr[comp];
+        ev[comp] = a;
+      }
+      err = fmaxf(0.f, sqrtf(ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2]));
+    }
+    if (want_tq) tq = fmaxf(0.f, sqrtf(r.x * r.x + r.y * r.y + r.z * r.z));
+  }
+};
+
+// Block maximum of v over the 256 threads (every thread calls it; red: 4
+// floats of LDS; the result in thread 0).
+__device__ __forceinline__ float field_block_max(float v, float* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float r = 0.f;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 4; ++i) r = fmaxf(r, red[i]);
+  return r;
+}
+
+// Block b of the last stage's batched field launch, with the error norms of
+// the member's cells merged into errbits / tqbits (every thread of the block
+// takes part in the reductions).
+__device__ __forceinline__ void field_uniform_b_block_err(const FBInst* __restrict__ tab, const int* __restrict__ b2i,
+                                                          const float* __restrict__ X, const float* __restrict__ HD,
+                                                          float* __restrict__ OUT, int cs, int b, const BatchErr& be) {
+  __shared__ float red[2][4];
+  const FBInst& I = tab[b2i[b]];
+  const int j = (b - I.first) * 256 + threadIdx.x;
+  const bool valid = j < I.d.ncell;
+  const int nxc = I.d.nx, ny = I.d.ny;
+  const int x = j % nxc, yz = j / nxc, y = yz % ny, z = yz / ny;
+  const float* hd = HD + I.off;
+  auto hdem = [&](int i) { return load3(hd, i, cs); };
+  const BatchMember& S = be.st[I.slot];
+  // uniform over the block: one member
+  const bool want_err = S.active && S.adaptive && be.nk > 0;
+  const bool want_tq = S.active && (S.mode == BATCH_RELAX || S.accept == 2);
+  if (!want_err && !want_tq) {
+    if (valid) field_uniform_cell(I.d, I.u, X + I.off, hdem, OUT + I.off, I.mode, x, y, z, cs);
+    return;
+  }
+  float c[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) c[q] = q <= be.nk ? (float)(S.dt * be.e[q]) : 0.f;
+  float err = 0.f, tq = 0.f;
+  if (valid)
+    field_uniform_cell(I.d, I.u, X + I.off, hdem, OUT + I.off, I.mode, x, y, z, cs,
+                       BatchErrCell{be, c, I.off, cs, want_err, want_tq, err, tq});
+  const float bm = field_block_max(err, red[0]);
+  const float bt = field_block_max(tq, red[1]);
+  if (threadIdx.x == 0) {
+    if (want_err) atomicMax(be.errbits + I.slot, __float_as_uint(bm));
+    if (want_tq) atomicMax(be.tqbits + I.slot, __float_as_uint(bt));
+  }
+}
+
 // Every parameter uniform and only the terms the uniform path has.
 bool all_uniform(const FMDesc& d);
 // Host-side coefficients of the uniform path.

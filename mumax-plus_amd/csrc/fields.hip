@@ -749,6 +749,22 @@ FieldBatch::FieldBatch(const std::vector<FieldBatchMember>& members, int cs) : c
   HIP_CHECK(hipMemcpy(b2i_.get(), b2i.data(), b2i.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
+__global__ void __launch_bounds__(BLOCK) k_field_uniform_b_err(const FBInst* __restrict__ tab,
+                                                               const int* __restrict__ b2i,
+                                                               const float* __restrict__ X,
+                                                               const float* __restrict__ HD, float* __restrict__ OUT,
+                                                               int cs, BatchErr be) {
+  field_uniform_b_block_err(tab, b2i, X, HD, OUT, cs, xcd_remap(blockIdx.x, gridDim.x), be);
+}
+
+void FieldBatch::launch_err(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchErr& err) const {
+  if (nblocks_ == 0) return;
+  if (err.nk < 0 || err.nk > 6) throw Error(MPA_ERR_RUNTIME, "batched field: bad stage count");
+  k_field_uniform_b_err<<<nblocks_, BLOCK, 0, s>>>(reinterpret_cast<const FBInst*>(tab_.get()), b2i_.get(), X, HD,
+                                                    OUT, cs_, err);
+  checkLaunch("k_field_uniform_b_err");
+}
+
 void FieldBatch::launch(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchNext* next) const {
   if (nblocks_ == 0) return;
   const FBInst* tab = reinterpret_cast<const FBInst*>(tab_.get());

@@ -205,6 +205,21 @@ struct BatchNext {
   float* xout;
   const BatchMember* st;
 };
+// The step's error norms, formed in the batched field launch of the last
+// stage from its derivative k in registers (the arithmetic of the batch's
+// error-norm kernel): per cell of an active member |sum_{j < nk} (dt e_j)
+// k_j + (dt e_nk) k| when the member steps adaptively, and |k| (the torque at
+// the new state) when it relaxes; each block's maxima merged into the
+// member's errbits / tqbits (float bits, atomic max).
+struct BatchErr {
+  const float* k[7];
+  double e[7];  // e[0..nk]: e[nk] multiplies the derivative just computed
+  int nk;
+  unsigned* errbits;
+  unsigned* tqbits;
+  const BatchMember* st;
+};
+
 class FieldBatch {
  public:
   FieldBatch(const std::vector<FieldBatchMember>& members, int cs);
@@ -212,6 +227,8 @@ class FieldBatch {
   // member has TERM_DEMAG, for every member; with `next`, also the next
   // stage state of every active member (BatchNext).
   void launch(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchNext* next = nullptr) const;
+  // launch without a next stage, forming the error norms (BatchErr).
+  void launch_err(const float* X, const float* HD, float* OUT, hipStream_t s, const BatchErr& err) const;
   unsigned nblocks() const { return nblocks_; }
   // The device member table (FBInst entries) and block -> member map.
   const void* table() const { return tab_.get(); }
