@@ -1371,7 +1371,7 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
   // evaluation ends when both streams are done.
   // Likewise the sublattice fields without demag of an antiferromagnet
   // (after the strains, on stream2), whose torques a launch after the demag
-  // passes completes.  Accelerations go to stream3.
+  // passes completes; the accelerations then go to stream3.
   bool elastic = false;
   for (auto& fm : magnets) elastic = elastic || fm->el.enabled;
   for (auto& a : afms) elastic = elastic || a->el.enabled;
