@@ -674,6 +674,9 @@ def main():
     ap.add_argument("--serial-elastic", action="store_true",
                     help="afm workload: strains, accelerations and sublattice fields on the world's stream, "
                          "not beside the demag passes")
+    ap.add_argument("--field-zchunk", type=int, default=None,
+                    help="default workload: z planes per workgroup of the field kernel's halo-tile variant "
+                         "(world option field_zchunk; -1: the tile-schedule autotuner picks per grid)")
     ap.add_argument("--unfused", action="store_true",
                     help="run the stage combination as its own launch (world option fuse_stages 0), e.g. to "
                          "calibrate counters on a kernel of known traffic")
@@ -703,6 +706,8 @@ def main():
     impl = world._impl
     if args.unfused:
         impl.set_option("fuse_stages", 0)
+    if args.field_zchunk is not None:
+        impl.set_option("field_zchunk", args.field_zchunk)
     N = SIZE[0] * SIZE[1] * SIZE[2]
 
     ts.steps(args.warmup)
@@ -752,7 +757,10 @@ def main():
         "config": {"workload": "ferromagnet 256x256x64, 2 nm cells, exchange + interfacial DMI + uniaxial "
                                "anisotropy + demag, Dormand-Prince fixed dt 5e-14 s",
                    "grid": list(SIZE), "cellsize_m": CELL, "msat": MSAT, "aex": AEX, "dmi": DMI, "ku1": KU1,
-                   "alpha": 0.0, "dt": DT, "method": "DormandPrince", "seed": args.seed, "fft": [kx, ky, kz]},
+                   "alpha": 0.0, "dt": DT, "method": "DormandPrince", "seed": args.seed, "fft": [kx, ky, kz],
+                   "field_zchunk": impl.get_option("field_zchunk"),
+                   "field_tile": {f"{k[0]}x{k[1]}x{k[2]}/{k[3]}": v["choice"]
+                                  for k, v in mp._cpp._field_tile_report().items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "bytes_per_launch": d_b / d_n,

@@ -357,8 +357,10 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
  * pass, which reads the step start and the derivatives instead of the state (no separate
  * combination launch); 0 runs the separate combination.  "field_zchunk" (default 4): z planes per workgroup of the
  * uniform-parameter field kernel's LDS halo-tile variant (magnets with no geometry mask or periodic
- * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant.  Both give the
- * same values bit for bit.  "split_elastic" (default 1): with elastodynamics, the strain and the
+ * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant; -1 lets the
+ * tile-schedule autotuner pick 1, 2, 4 or 8 per grid size and kernel variant by timing each on the
+ * first launch (mpa_field_tile_report).  Every chunk gives the same values bit for bit; the
+ * one-cell-per-thread variant agrees with them to a few ulps.  "split_elastic" (default 1): with elastodynamics, the strain and the
  * acceleration of each right-hand side run on other streams beside the demag passes; 0 runs
  * them on the world's stream (the same values).  "split_demag" (default 0): 1 computes the sublattice
  * fields without demag of an antiferromagnet with demag and uniform parameters beside the demag
@@ -366,6 +368,12 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
  * torques in one launch after the demag passes. */
 int mpa_world_set_option(mpa_world* w, const char* key, int value);
 int mpa_world_get_option(const mpa_world* w, const char* key, int* value);
+/* The tile-schedule autotuner's record (process-wide; no counterpart in the reference), one line per
+ * grid and field-kernel variant: "nx x ny x nz variant v | choice | 1 t1 us | 2 t2 us | 4 t4 us | 8 t8 us"
+ * with "-" for a chunk not timed (not below nz).  At most cap bytes (NUL-terminated) are written to
+ * text; *len = the full length.  mpa_field_tile_clear forgets every decision. */
+int mpa_field_tile_report(int cap, char* text, int* len);
+int mpa_field_tile_clear(void);
 int mpa_world_profile_reset(mpa_world* w);
 int mpa_world_profile_report(mpa_world* w, int cap, char* names, double* total_ms, long long* launches,
                              double* bytes, int* count);

@@ -1166,7 +1166,7 @@ int mpa_world_set_option(mpa_world* w, const char* key, int value) {
     else if (k == "split_elastic") w->w.split_elastic = value != 0;
     else if (k == "split_demag") w->w.split_demag = value != 0;
     else if (k == "field_zchunk") {
-      if (value < 0) throw Error(MPA_ERR_INVALID, "field_zchunk must be >= 0");
+      if (value < -1) throw Error(MPA_ERR_INVALID, "field_zchunk must be >= -1");
       w->w.field_zchunk = value;
     } else throw Error(MPA_ERR_INVALID, "unknown world option '" + k + "'");
   });
@@ -1344,6 +1344,19 @@ int mpa_batch_tune_load(const char* path) {
 
 int mpa_batch_tune_clear(void) {
   return guard([&] { batch_tune_clear(); });
+}
+
+int mpa_field_tile_report(int cap, char* text, int* len) {
+  return guard([&] {
+    need(len, "len");
+    const std::string r = field_tile_report();
+    *len = (int)r.size();
+    if (text && cap > 0) copy_str(r, text, cap);
+  });
+}
+
+int mpa_field_tile_clear(void) {
+  return guard([&] { field_tile_clear(); });
 }
 
 int mpa_bench_dft8(int variant, int nlines, int rounds, const float* in, float* out, double* ms) {

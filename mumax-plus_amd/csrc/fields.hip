@@ -16,7 +16,12 @@
 #include "fields.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
 
 #include "field_cell.hpp"
 
@@ -884,18 +889,39 @@ bool uniform_fast(const FMDesc& d, const FMUniform& u, FieldMode mode) {
 // The halo-tile kernel covers magnets without a geometry mask or periodic
 // wrap that are at least one tile wide, unless the world switched it off.
 bool tile_eligible(const FMDesc& d) {
-  return d.tile_zc > 0 && !d.geom && d.mgx == 0 && d.mgy == 0 && d.mgz == 0 && d.nx >= TILE_X && d.ny >= TILE_Y;
+  return d.tile_zc != 0 && !d.geom && d.mgx == 0 && d.mgy == 0 && d.mgz == 0 && d.nx >= TILE_X && d.ny >= TILE_Y;
+}
+
+// Tile-schedule autotuner of the halo-tile field kernel: per grid size and
+// kernel variant, the z planes per workgroup (TILE_ZC) are timed on the
+// first launch (three launches each, the kernel's own outputs, which every
+// candidate writes with the same values) and the fastest is kept for the
+// process.
+constexpr int TILE_NZC = 4;
+constexpr int TILE_ZC[TILE_NZC] = {1, 2, 4, 8};
+constexpr int TILE_ZC_DEFAULT = 4;
+
+struct TileTune {
+  int zc = 0;
+  float us[TILE_NZC] = {0.f, 0.f, 0.f, 0.f};  // per launch; 0: not timed (beyond nz)
+};
+
+std::mutex tile_mu;
+std::map<std::array<int, 4>, TileTune>& tile_table() {
+  static std::map<std::array<int, 4>, TileTune> t;
+  return t;
 }
 
 void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
                           const NextStage* next, hipStream_t s) {
   const FMUniform u = make_uniform(d);
   if (tile_eligible(d)) {
-    const int zc = std::min(d.tile_zc, d.nz);
-    dim3 grid((d.nx + TILE_X - 1) / TILE_X, (d.ny + TILE_Y - 1) / TILE_Y, (d.nz + zc - 1) / zc);
-    const dim3 blk(TILE_X, TILE_Y, 1);
     const NextStage ns = next ? *next : NextStage{};
     const bool fast = uniform_fast(d, u, mode);
+    auto launch_tile = [&](int zc_req) {
+    const int zc = std::min(zc_req, d.nz);
+    dim3 grid((d.nx + TILE_X - 1) / TILE_X, (d.ny + TILE_Y - 1) / TILE_Y, (d.nz + zc - 1) / zc);
+    const dim3 blk(TILE_X, TILE_Y, 1);
     if (next && fast) {
       // derivative counts of the first-same-as-last tableaus' stages
       switch (ns.x0 ? ns.nk : -2) {
@@ -910,6 +936,49 @@ void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, 
     else if (fast) k_field_tile<false, true><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
     else k_field_tile<false, false><<<grid, blk, 0, s>>>(d, u, m, hdemag, out, (int)mode, ns, zc);
     checkLaunch("k_field_tile");
+    };
+    if (d.tile_zc > 0) {
+      launch_tile(d.tile_zc);
+      return;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_CHECK(hipStreamIsCapturing(s, &cap));
+    const std::array<int, 4> key{d.nx, d.ny, d.nz, (next ? 2 : 0) + (fast ? 1 : 0) + (next && ns.x0 ? 4 * ns.nk : 0)};
+    int zc = 0;
+    {
+      std::lock_guard<std::mutex> lk(tile_mu);
+      auto it = tile_table().find(key);
+      if (it != tile_table().end()) zc = it->second.zc;
+    }
+    if (zc == 0 && cap != hipStreamCaptureStatusNone) zc = TILE_ZC_DEFAULT;  // no timing inside a capture
+    if (zc == 0) {
+      TileTune tt;
+      hipEvent_t e0, e1;
+      HIP_CHECK(hipEventCreate(&e0));
+      HIP_CHECK(hipEventCreate(&e1));
+      float best = 0.f;
+      for (int c = 0; c < TILE_NZC; ++c) {
+        if (c > 0 && TILE_ZC[c - 1] >= d.nz) break;  // the same schedule as the previous one
+        launch_tile(TILE_ZC[c]);  // warm-up
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch_tile(TILE_ZC[c]);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        tt.us[c] = ms * 1e3f / 3.f;
+        if (tt.zc == 0 || tt.us[c] < best) {
+          tt.zc = TILE_ZC[c];
+          best = tt.us[c];
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      std::lock_guard<std::mutex> lk(tile_mu);
+      tile_table()[key] = tt;
+      zc = tt.zc;
+    }
+    launch_tile(zc);
     return;
   }
   int bx = 1;
@@ -925,6 +994,27 @@ void launch_field_uniform(const FMDesc& d, const float* m, const float* hdemag, 
   }
 }
 }  // namespace
+
+std::string field_tile_report() {
+  std::lock_guard<std::mutex> lk(tile_mu);
+  std::ostringstream o;
+  for (const auto& kv : tile_table()) {
+    o << kv.first[0] << "x" << kv.first[1] << "x" << kv.first[2] << " variant " << kv.first[3] << " | "
+      << kv.second.zc;
+    for (int c = 0; c < TILE_NZC; ++c) {
+      o << " | " << TILE_ZC[c] << " ";
+      if (kv.second.us[c] > 0.f) o << kv.second.us[c] << " us";
+      else o << "-";
+    }
+    o << "\n";
+  }
+  return o.str();
+}
+
+void field_tile_clear() {
+  std::lock_guard<std::mutex> lk(tile_mu);
+  tile_table().clear();
+}
 
 bool field_next_fusable(const FMDesc& d, FieldMode mode) {
   return mode == MODE_LLG && all_uniform(d) && d.nz <= 65535;

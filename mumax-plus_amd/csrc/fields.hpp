@@ -142,6 +142,13 @@ bool field_next_fusable(const FMDesc& d, FieldMode mode);
 void launch_field_next(const FMDesc& d, const float* m, const float* hdemag, float* out, FieldMode mode,
                        const NextStage& next, hipStream_t s);
 
+// The tile-schedule autotuner's record of the uniform field kernel (z planes
+// per workgroup chosen per grid size and kernel variant when FMDesc.tile_zc
+// is -1): one line per grid, the choice and the microseconds per launch of
+// each candidate; and its reset.
+std::string field_tile_report();
+void field_tile_clear();
+
 // launch_field for the two sublattices of an antiferromagnet in one launch
 // (two separate launches when their parameter layouts differ).
 void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float* o0, const FMDesc& d1,

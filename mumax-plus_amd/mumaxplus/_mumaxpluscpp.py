@@ -183,6 +183,8 @@ _SIGS = {
     "mpa_batch_tune_save": ([_ct.c_char_p], _cint),
     "mpa_batch_tune_load": ([_ct.c_char_p], _cint),
     "mpa_batch_tune_clear": ([], _cint),
+    "mpa_field_tile_report": ([_cint, _P, _ct.POINTER(_cint)], _cint),
+    "mpa_field_tile_clear": ([], _cint),
     "mpa_batch_tune_report": ([_P, _cint, _P, _ct.POINTER(_cint)], _cint),
 }
 for _name, (_args, _res) in _SIGS.items():
@@ -1181,6 +1183,30 @@ def _batch_tune_load(path):
 
 def _batch_tune_clear():
     _check(_lib.mpa_batch_tune_clear())
+
+
+def _field_tile_report():
+    """The tile-schedule autotuner's record: {(nx, ny, nz, variant): {"choice":
+    zc, zc_candidate: microseconds per launch or None}}."""
+    n = _cint(0)
+    _check(_lib.mpa_field_tile_report(0, None, _ct.byref(n)))
+    buf = _ct.create_string_buffer(n.value + 1)
+    _check(_lib.mpa_field_tile_report(n.value + 1, buf, _ct.byref(n)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        parts = [p.strip() for p in line.split("|")]
+        dims, var = parts[0].split(" variant ")
+        key = tuple(int(v) for v in dims.split("x")) + (int(var),)
+        rec = {"choice": int(parts[1])}
+        for p in parts[2:]:
+            zc, t = p.split(" ", 1)
+            rec[int(zc)] = None if t == "-" else float(t.split()[0])
+        out[key] = rec
+    return out
+
+
+def _field_tile_clear():
+    _check(_lib.mpa_field_tile_clear())
 
 
 def _bench_dft8(variant, lines, rounds):

@@ -92,5 +92,46 @@ def test_runs_equal_per_cell_kernel(method):
 def test_zchunk_option():
     world, _ = _magnet(8, (64, 8, 2))
     assert world._impl.get_option("field_zchunk") == 8
+    world._impl.set_option("field_zchunk", -1)
+    assert world._impl.get_option("field_zchunk") == -1
     with pytest.raises(ValueError):
-        world._impl.set_option("field_zchunk", -1)
+        world._impl.set_option("field_zchunk", -2)
+
+
+@pytest.mark.parametrize("size", [(96, 10, 7), (64, 8, 1)])
+def test_autotuned_chunk_equals_every_fixed_chunk(size):
+    """field_zchunk -1: the autotuner times chunks 1, 2, 4, 8 (those below nz)
+    on the first launch for the grid and records its choice; every chunk runs
+    the same cell code, so fields, torques and a run equal each fixed chunk's
+    bit for bit."""
+    mp._cpp._field_tile_clear()
+    res = {}
+    for zc in (-1, 1, 2, 4, 8):
+        world, mag = _magnet(zc, size, seed=2)
+        h = mag.effective_field.eval()
+        t = mag.torque.eval()
+        ts = world.timesolver
+        ts.set_method("DormandPrince")
+        ts.adaptive_timestep = False
+        ts.timestep = 4e-14
+        ts.steps(6)
+        res[zc] = (h, t, mag.magnetization.get())
+    for zc in (1, 2, 4, 8):
+        for a, b in zip(res[-1], res[zc]):
+            assert np.array_equal(a, b), zc
+    rep = mp._cpp._field_tile_report()
+    mine = {k: v for k, v in rep.items() if k[:3] == size}
+    assert mine, rep
+    for rec in mine.values():
+        assert rec["choice"] in (1, 2, 4, 8)
+        assert rec["choice"] == 1 or rec["choice"] < size[2] * 2
+        timed = [zc for zc in (1, 2, 4, 8) if rec[zc] is not None]
+        assert timed and timed[0] == 1
+        assert all(rec[zc] > 0 for zc in timed)
+        assert rec[rec["choice"]] == min(rec[zc] for zc in timed)
+        # chunks at or beyond nz beyond the first such one are not timed
+        for zc in (2, 4, 8):
+            if zc // 2 >= size[2]:
+                assert rec[zc] is None
+    mp._cpp._field_tile_clear()
+    assert not mp._cpp._field_tile_report()
