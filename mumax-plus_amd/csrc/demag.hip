@@ -220,7 +220,11 @@ __device__ __forceinline__ long long a_line_base(const DD& d, int l) {
   return (long long)yz * d.Hx * d.ny + (l - yz * d.ny);
 }
 
-template <int L2, class K = XfwdCfg<L2>>
+// PLAIN: no geometry mask and uniform Ms of both sublattices (geom, msat.p
+// and msat2.p null; PLAIN 1: no second sublattice, PLAIN 2: m2 set), so the
+// rows are loaded and scaled with nothing else live in registers; PLAIN 0
+// takes any magnet.
+template <int L2, class K = XfwdCfg<L2>, int PLAIN = 0>
 __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__ m, PS msat,
                                           const float* __restrict__ m2, PS msat2, const uint8_t* __restrict__ geom,
                                           float2* __restrict__ A, const float2* __restrict__ tw, int blk) {
@@ -232,26 +236,100 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
   for (int i = threadIdx.x; i < 2 * K::PB; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
   const int la = l0 + 2 * g, lb = la + 1;
   const int nyz = d.ny * d.nz;
+  const bool va = la < d.nlines, vb = lb < d.nlines;
   const int cella = (la % nyz) * d.nx, cellb = (lb % nyz) * d.nx;
+  // element offsets of the two rows; a row past the end reads row 0
+  const size_t ia = va ? (size_t)(la / nyz) * d.cs + cella : 0, ib = vb ? (size_t)(lb / nyz) * d.cs + cellb : 0;
+  const int ca = va ? cella : 0, cb = vb ? cellb : 0;
   // M = Ms m, or Ms1 m1 + Ms2 m2 for the two sublattices of an antiferromagnet
-  // (k_pad, strayfieldfft.cu:51-73)
-  auto M = [&](int line, int cell) {
-    const size_t i = (size_t)(line / nyz) * d.cs + cell;
-    float v = msat.at(cell) * m[i];
-    if (m2) v += msat2.at(cell) * m2[i];
-    return v;
-  };
+  // (k_pad, strayfieldfft.cu:51-73), zero outside the geometry.  Every load
+  // is issued unconditionally from an in-range address (x clamped to 0 past
+  // nx) and the row, x and geometry tests select afterwards, so all of a
+  // thread's loads are in flight together.
+  float ma[C::EH], mb[C::EH], m2a[C::EH], m2b[C::EH];
+  if (d.nx >= C::T * C::EH) {
+    // rows as long as the half line: fixed offsets from one address per row
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      ma[q] = m[ia + t + C::T * q];
+      mb[q] = m[ib + t + C::T * q];
+      if constexpr (PLAIN == 2) {
+        m2a[q] = m2[ia + t + C::T * q];
+        m2b[q] = m2[ib + t + C::T * q];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t + C::T * q, xs = x < d.nx ? x : 0;
+      ma[q] = m[ia + xs];
+      mb[q] = m[ib + xs];
+      if constexpr (PLAIN == 2) {
+        m2a[q] = m2[ia + xs];
+        m2b[q] = m2[ib + xs];
+      }
+    }
+  }
+  bool ina[C::EH], inb[C::EH];
+  if constexpr (PLAIN != 0) {
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t + C::T * q;
+      ina[q] = va && x < d.nx;
+      inb[q] = vb && x < d.nx;
+      ma[q] *= msat.v;
+      mb[q] *= msat.v;
+      if constexpr (PLAIN == 2) {
+        ma[q] += msat2.v * m2a[q];
+        mb[q] += msat2.v * m2b[q];
+      }
+    }
+  } else {
+    // geometry flags as bits (q: row a, 8 + q: row b)
+    unsigned gm = 0xFFFFu;
+    if (geom) {
+      gm = 0;
+#pragma unroll
+      for (int q = 0; q < C::EH; ++q) {
+        const int x = t + C::T * q, xs = x < d.nx ? x : 0;
+        gm |= (geom[ca + xs] ? 1u : 0u) << q;
+        gm |= (geom[cb + xs] ? 1u : 0u) << (8 + q);
+      }
+    }
+    if (msat.p) {
+#pragma unroll
+      for (int q = 0; q < C::EH; ++q) {
+        const int x = t + C::T * q, xs = x < d.nx ? x : 0;
+        ma[q] *= msat.p[ca + xs];
+        mb[q] *= msat.p[cb + xs];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < C::EH; ++q) {
+        ma[q] *= msat.v;
+        mb[q] *= msat.v;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t + C::T * q;
+      ina[q] = va && x < d.nx && ((gm >> q) & 1u);
+      inb[q] = vb && x < d.nx && ((gm >> (8 + q)) & 1u);
+    }
+  }
+  if (PLAIN == 0 && m2) {
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t + C::T * q, xs = x < d.nx ? x : 0;
+      ma[q] += msat2.at(ca + xs) * m2[ia + xs];
+      mb[q] += msat2.at(cb + xs) * m2[ib + xs];
+    }
+  }
   float2 v[1][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[0][q] = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int q = 0; q < C::EH; ++q) {
-    const int x = t + C::T * q;
-    if (x < d.nx) {
-      if (la < d.nlines && (!geom || geom[cella + x])) v[0][q].x = M(la, cella + x);
-      if (lb < d.nlines && (!geom || geom[cellb + x])) v[0][q].y = M(lb, cellb + x);
-    }
-  }
+  for (int q = 0; q < C::EH; ++q) v[0][q] = make_float2(ina[q] ? ma[q] : 0.f, inb[q] ? mb[q] : 0.f);
   float2* line = lds + g * K::LP;
   line_fft<L2, false, 1>(v, t, line, 0, tw);
 #pragma unroll
@@ -260,11 +338,11 @@ __device__ __forceinline__ void xfwd_body(const DD& d, const float* __restrict__
   xfwd_split_store<L2, K>(d, lds, abase, 0, A);
 }
 
-template <int L2>
+template <int L2, int PLAIN>
 __global__ void __launch_bounds__(XfwdCfg<L2>::THREADS)
     k_xfwd(DD d, const float* __restrict__ m, PS msat, const float* __restrict__ m2, PS msat2,
            const uint8_t* __restrict__ geom, float2* __restrict__ A, const float2* __restrict__ tw) {
-  xfwd_body<L2>(d, m, msat, m2, msat2, geom, A, tw, blockIdx.x);
+  xfwd_body<L2, XfwdCfg<L2>, PLAIN>(d, m, msat, m2, msat2, geom, A, tw, blockIdx.x);
 }
 
 // x forward with the Runge-Kutta stage combination in front.  The rows of
@@ -708,6 +786,22 @@ __device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict_
   float2* Z = lds;  // reused once the half spectra are in registers
   const int l0 = 2 * blk * K::PB;
   for (int i = threadIdx.x; i < NL; i += K::THREADS) abase[i] = a_line_base(d, l0 + i);
+  // geometry flags of the thread's output cells (bit q: row a, bit 8 + q:
+  // row b), loaded together up front so their latency hides behind the
+  // transform (x clamped to 0 past nx: those cells are not stored)
+  unsigned gmask = 0xFFFFu;
+  if (geom) {
+    const int g0 = threadIdx.x >> C::L2T, t0 = threadIdx.x & (C::T - 1);
+    const int nyz0 = d.ny * d.nz;
+    const int ca = ((l0 + 2 * g0) % nyz0) * d.nx, cb = ((l0 + 2 * g0 + 1) % nyz0) * d.nx;
+    gmask = 0;
+#pragma unroll
+    for (int q = 0; q < C::EH; ++q) {
+      const int x = t0 + C::T * q, xs = x < d.nx ? x : 0;
+      gmask |= (geom[ca + xs] ? 1u : 0u) << q;
+      gmask |= (geom[cb + xs] ? 1u : 0u) << (8 + q);
+    }
+  }
   __syncthreads();
   staged_copy<(C::H << L2NL), K::THREADS>(
       [&](int e) {
@@ -748,8 +842,8 @@ __device__ __forceinline__ void xinv_body(const DD& d, const float2* __restrict_
   for (int q = 0; q < C::EH; ++q) {
     const int x = t + C::T * q;
     if (x >= d.nx) continue;
-    if (la < d.nlines) h[(size_t)(la / nyz) * d.cs + cella + x] = (geom && !geom[cella + x]) ? 0.f : v[0][q].x;
-    if (lb < d.nlines) h[(size_t)(lb / nyz) * d.cs + cellb + x] = (geom && !geom[cellb + x]) ? 0.f : v[0][q].y;
+    if (la < d.nlines) h[(size_t)(la / nyz) * d.cs + cella + x] = (gmask >> q) & 1u ? v[0][q].x : 0.f;
+    if (lb < d.nlines) h[(size_t)(lb / nyz) * d.cs + cellb + x] = (gmask >> (8 + q)) & 1u ? v[0][q].y : 0.f;
   }
 }
 
@@ -866,7 +960,11 @@ struct RunXfwd {
                   const float2* tw, hipStream_t s) {
     using K = XfwdCfg<L2>;
     const int npairs = (d.nlines + 1) / 2;
-    k_xfwd<L2><<<grid_for(npairs, K::PB), K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
+    const dim3 grid = grid_for(npairs, K::PB);
+    if (!geom && !msat.p && !m2) k_xfwd<L2, 1><<<grid, K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
+    else if (!geom && !msat.p && !msat2.p)
+      k_xfwd<L2, 2><<<grid, K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
+    else k_xfwd<L2, 0><<<grid, K::THREADS, K::SHM, s>>>(d, m, msat, m2, msat2, geom, A, tw);
     checkLaunch("k_xfwd");
   }
 };
