@@ -154,6 +154,8 @@ def run_afm(args):
     if args.serial_elastic:
         impl.set_option("split_elastic", 0)
         impl.set_option("split_demag", 0)
+    if args.split_elastic:
+        impl.set_option("split_elastic", 1)
     nx, ny, nz = AFM_SIZE
     N = nx * ny * nz
     steps = args.steps if args.steps_given else 500
@@ -195,7 +197,8 @@ def run_afm(args):
                    "grid": list(AFM_SIZE), "cellsize_m": CELL, "pbc": [2, 2, 0], "msat": 566e3, "aex": 2.48e-12,
                    "afmex_cell": -4e-12, "bulk_dmi": 1e-3, "alpha": 0.004, "B1": -8.8e6, "B2": -8.8e6,
                    "C11": 283e9, "C12": 166e9, "C44": 58e9, "rho": 8900.0, "eta": 1e10, "dt": AFM_DT,
-                   "method": "DormandPrince", "seed": args.seed},
+                   "method": "DormandPrince", "seed": args.seed,
+                   "split_elastic": impl.get_option("split_elastic"), "split_demag": impl.get_option("split_demag")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_launch": d_b / d_n,
@@ -673,7 +676,10 @@ def main():
                     help="step path of the batched and served workloads (Batch.step_mode)")
     ap.add_argument("--serial-elastic", action="store_true",
                     help="afm workload: strains, accelerations and sublattice fields on the world's stream, "
-                         "not beside the demag passes")
+                         "not beside the demag passes (the default; also switches split_demag off)")
+    ap.add_argument("--split-elastic", action="store_true",
+                    help="afm workload: strains and accelerations on a second stream beside the demag passes "
+                         "(world option split_elastic 1)")
     ap.add_argument("--field-zchunk", type=int, default=None,
                     help="default workload: z planes per workgroup of the field kernel's halo-tile variant "
                          "(world option field_zchunk; -1: the tile-schedule autotuner picks per grid)")

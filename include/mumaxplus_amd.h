@@ -360,8 +360,8 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
  * wrap, at least 64 x 4 cells per plane); 0 selects the one-cell-per-thread variant; -1 lets the
  * tile-schedule autotuner pick 1, 2, 4 or 8 per grid size and kernel variant by timing each on the
  * first launch (mpa_field_tile_report).  Every chunk gives the same values bit for bit; the
- * one-cell-per-thread variant agrees with them to a few ulps.  "split_elastic" (default 1): with elastodynamics, the strain and the
- * acceleration of each right-hand side run on other streams beside the demag passes; 0 runs
+ * one-cell-per-thread variant agrees with them to a few ulps.  "split_elastic" (default 0): 1 runs, with elastodynamics, the strain
+ * and the acceleration of each right-hand side on another stream beside the demag passes; 0 runs
  * them on the world's stream (the same values).  "split_demag" (default 0): 1 computes the sublattice
  * fields without demag of an antiferromagnet with demag and uniform parameters beside the demag
  * passes and forms its torques after them (the same values bit for bit); 0 computes fields and

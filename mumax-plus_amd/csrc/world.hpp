@@ -394,8 +394,10 @@ class World {
   int field_zchunk = 4;
   // Elastodynamics: the strain and the acceleration of a right-hand side run
   // on a second stream, concurrently with the demag passes (joined before
-  // the magnetoelastic field and at the end of the evaluation).
-  bool split_elastic = true;
+  // the magnetoelastic field and at the end of the evaluation).  Off by
+  // default: on a 512 x 512 antiferromagnet the overlap varied from step
+  // to step between a little faster and a good deal slower than one stream.
+  bool split_elastic = false;
   // Antiferromagnets with demag: the sublattice fields without demag run on
   // a second stream beside the demag passes; a torque launch after the x
   // inverse adds the demag field (the same values bit for bit).  Off by
