@@ -144,6 +144,14 @@ def afm_step_bytes(N, demag_bytes):
     return 6 * per_eval + 4 * 384.0 * N + 6 * 12.0 * N
 
 
+def set_world_options(impl, args):
+    """--world-option KEY=VALUE (repeatable): a world switch of the C ABI
+    (mpa_world_set_option), e.g. split_demag=1."""
+    for kv in args.world_option or []:
+        k, v = kv.split("=", 1)
+        impl.set_option(k, int(v))
+
+
 def run_afm(args):
     import mumaxplus as mp
     if not mp._cpp.device_available():
@@ -156,6 +164,7 @@ def run_afm(args):
         impl.set_option("split_demag", 0)
     if args.split_elastic:
         impl.set_option("split_elastic", 1)
+    set_world_options(impl, args)
     nx, ny, nz = AFM_SIZE
     N = nx * ny * nz
     steps = args.steps if args.steps_given else 500
@@ -677,6 +686,8 @@ def main():
     ap.add_argument("--serial-elastic", action="store_true",
                     help="afm workload: strains, accelerations and sublattice fields on the world's stream, "
                          "not beside the demag passes (the default; also switches split_demag off)")
+    ap.add_argument("--world-option", action="append", metavar="KEY=VALUE",
+                    help="default and afm workloads: set a world option (mpa_world_set_option), repeatable")
     ap.add_argument("--split-elastic", action="store_true",
                     help="afm workload: strains and accelerations on a second stream beside the demag passes "
                          "(world option split_elastic 1)")
@@ -714,6 +725,7 @@ def main():
         impl.set_option("fuse_stages", 0)
     if args.field_zchunk is not None:
         impl.set_option("field_zchunk", args.field_zchunk)
+    set_world_options(impl, args)
     N = SIZE[0] * SIZE[1] * SIZE[2]
 
     ts.steps(args.warmup)

@@ -332,7 +332,7 @@ def test_coupled_fixed_steps_match_oracle(method):
     assert abs(ts.time - p.ow.solver.time) < 1e-25
 
 
-@pytest.mark.parametrize("options", [{"split_elastic": 0}, {"split_demag": 1}, {"split_elastic": 0, "split_demag": 1}])
+@pytest.mark.parametrize("options", [{"split_elastic": 1}, {"split_demag": 1}, {"split_elastic": 1, "split_demag": 1}])
 def test_streams_beside_the_demag_passes_change_nothing(options):
     """The strain and acceleration launches (world option split_elastic) and
     the sublattice fields without demag (split_demag) run on other streams
