@@ -58,30 +58,41 @@ __device__ __forceinline__ void bstage_block(const StageArgs& sa, const float* X
   const int q = chunk * (CH / 4) + (threadIdx.x & 63);  // float4 index within a component
   const int cs4 = cs >> 2;
   const bool take = sa.commit && S.accept;
+  const double dt = S.dt;
   float c[MAX_STAGES];
 #pragma unroll
-  for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < sa.nk ? (float)(S.dt * sa.a[j]) : 0.f;
+  for (int j = 0; j < MAX_STAGES; ++j) c[j] = j < sa.nk ? (float)(dt * sa.a[j]) : 0.f;
   const float4* X4 = reinterpret_cast<const float4*>(X);
   float4* XO4 = reinterpret_cast<float4*>(XO);
   float4* X04 = reinterpret_cast<float4*>(X0);
   float4* K04 = reinterpret_cast<float4*>(K0);
   const float4* KL4 = reinterpret_cast<const float4*>(KL);
+  // every component's loads first (all in flight together), then the
+  // take-over stores, then the combination
+  float4 x0[3], k0[3];
+  const float4* xs = take ? X4 : X04;
+  const float4* ks = take ? KL4 : K04;
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const size_t i = (size_t)comp * cs4 + q;
+    x0[comp] = xs[i];
+    k0[comp] = ks[i];
+  }
+  const uchar4 g = reinterpret_cast<const uchar4*>(G)[q];
+  if (take) {
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const size_t i = (size_t)comp * cs4 + q;
+      X04[i] = x0[comp];
+      K04[i] = k0[comp];
+    }
+  }
   float4 v[3];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     const size_t i = (size_t)comp * cs4 + q;
-    float4 x0, k0;
-    if (take) {
-      x0 = X4[i];
-      k0 = KL4[i];
-      X04[i] = x0;
-      K04[i] = k0;
-    } else {
-      x0 = X04[i];
-      k0 = K04[i];
-    }
-    float4 a = x0;
-    axpy4(a, c[0], k0);
+    float4 a = x0[comp];
+    axpy4(a, c[0], k0[comp]);
     for (int j = 1; j < sa.nk; ++j) axpy4(a, c[j], reinterpret_cast<const float4*>(sa.k[j])[i]);
     v[comp] = a;
   }
@@ -89,7 +100,6 @@ __device__ __forceinline__ void bstage_block(const StageArgs& sa, const float* X
   const float i1 = rsqrtf(v[0].y * v[0].y + v[1].y * v[1].y + v[2].y * v[2].y);
   const float i2 = rsqrtf(v[0].z * v[0].z + v[1].z * v[1].z + v[2].z * v[2].z);
   const float i3 = rsqrtf(v[0].w * v[0].w + v[1].w * v[1].w + v[2].w * v[2].w);
-  const uchar4 g = reinterpret_cast<const uchar4*>(G)[q];
 #pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     float4 o;
