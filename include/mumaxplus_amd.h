@@ -102,7 +102,8 @@ int mpa_fm_cellsingeo(const mpa_ferromagnet* m, int* out);     /* System::cellsi
  * destination switch).  Also "enable_zhang_li_torque", "enable_slonczewski_torque",
  * "fixed_layer_on_top" (wrap_ferromagnet.cpp:35-39), and "noise_seed" (the 32-bit key of the
  * magnet's thermal field; the reference seeds cuRAND from the clock, ferromagnet.cpp:79-81, so it
- * has no such setting) with the read-only "noise_draws" (thermal draws taken by time steps). */
+ * has no such setting) with the read-only "noise_draws" (thermal draws taken by time steps;
+ * MPA_ERR_INDEX once the 64-bit count exceeds an int: mpa_fm_get_noise_state reads it whole). */
 int mpa_fm_set_flag(mpa_ferromagnet* m, const char* flag, int value);
 int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out);
 /* The thermal-noise stream state of a ferromagnet or sublattice: the Philox key and the 64-bit

@@ -2,6 +2,7 @@
 #include "../../include/mumaxplus_amd.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -465,7 +466,12 @@ int mpa_fm_get_flag(const mpa_ferromagnet* m, const char* flag, int* out) {
     else if (k == "enable_slonczewski_torque") *out = f->enable_slonc;
     else if (k == "fixed_layer_on_top") *out = f->fixed_on_top;
     else if (k == "noise_seed") *out = (int)f->noise_seed;
-    else if (k == "noise_draws") *out = (int)f->noise_draws[0];
+    else if (k == "noise_draws") {
+      // the 64-bit counter as an int flag; mpa_fm_get_noise_state gives it whole
+      if (f->noise_draws[0] > (unsigned long long)INT_MAX)
+        throw Error(MPA_ERR_INDEX, "noise_draws exceeds an int: read it with mpa_fm_get_noise_state");
+      *out = (int)f->noise_draws[0];
+    }
     else if (!magnet_get_flag(f, k, out)) throw Error(MPA_ERR_INVALID, "unknown flag '" + k + "'");
   });
 }
