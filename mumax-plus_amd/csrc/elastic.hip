@@ -49,6 +49,20 @@ __global__ void __launch_bounds__(BLOCK) k_elastic(ElDesc d, const float* __rest
   elastic_cell(d, stress, v, a, b, out, mode, idx);
 }
 
+template <int NK>
+__global__ void __launch_bounds__(BLOCK) k_elastic_next(ElDesc d, const float* __restrict__ stress,
+                                                        const float* __restrict__ v, MEForce a, MEForce b,
+                                                        float* __restrict__ out, ElNext nx) {
+  const int idx = xcd_remap(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  const int N = d.ncell;
+  if (idx >= N) return;
+  const float3 f = elastic_term(d, stress, v, a, b, EL_ACCEL, idx);
+  out[idx] = f.x;
+  out[(size_t)N + idx] = f.y;
+  out[(size_t)2 * N + idx] = f.z;
+  elastic_next<NK>(d, nx, f, idx);
+}
+
 __global__ void __launch_bounds__(BLOCK) k_elastic_local(ElDesc d, const float* __restrict__ stress,
                                                          const float* __restrict__ strain,
                                                          const float* __restrict__ v, float* __restrict__ out,
@@ -93,6 +107,22 @@ void launch_elastic(const ElDesc& d, const float* stress, const float* v, const 
     throw Error(MPA_ERR_RUNTIME, "elastic force needs the stress tensor");
   k_elastic<<<blocks(d.ncell), BLOCK, 0, s>>>(d, stress, v, a, b, out, (int)mode);
   checkLaunch("k_elastic");
+}
+
+void launch_elastic_next(const ElDesc& d, const float* stress, const float* v, const MEForce& a, const MEForce& b,
+                         float* out, const ElNext& nx, hipStream_t s) {
+  if (d.elastic && !stress) throw Error(MPA_ERR_RUNTIME, "elastic force needs the stress tensor");
+  const unsigned g = blocks(d.ncell);
+  switch (nx.nk) {
+    case 0: k_elastic_next<0><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    case 1: k_elastic_next<1><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    case 2: k_elastic_next<2><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    case 3: k_elastic_next<3><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    case 4: k_elastic_next<4><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    case 5: k_elastic_next<5><<<g, BLOCK, 0, s>>>(d, stress, v, a, b, out, nx); break;
+    default: throw Error(MPA_ERR_RUNTIME, "launch_elastic_next: 0 to 5 earlier derivatives");
+  }
+  checkLaunch("k_elastic_next");
 }
 
 void launch_elastic_local(const ElDesc& d, const float* stress, const float* strain, const float* v, float* out,

@@ -42,12 +42,37 @@ enum ElMode {
   EL_ACCEL = 4,    // (body force + damping) / rho
 };
 
+// The next Runge-Kutta stage state of (u, v), formed in the acceleration
+// launch of stage s (k_combine_batch's arithmetic and term order):
+// v' = v0 + sum_{j<nk} c_j a_j + cself a_s with a_s the acceleration just
+// computed, u' = u0 + sum_{j<nk} c_j w_j + cself w_s with w_j the stage
+// velocities (the displacement's derivatives, ku[0..nk]), both zero outside
+// the geometry; v' goes to vout and to kdisp (the displacement's derivative
+// of the next stage).
+struct ElNext {
+  const float* v0;
+  const float* u0;
+  const float* ka[7];
+  const float* ku[8];
+  float c[7];
+  float cself;
+  int nk;
+  float* vout;
+  float* uout;
+  float* kdisp;
+};
+
 // strain (6N: xx yy zz xy xz yz) of displacement u; stress (6N) as well when
 // stress != nullptr.
 void launch_strain(const ElDesc& d, const float* u, float* strain, float* stress, hipStream_t s);
 // out (3N) = the selected term.  stress may be nullptr when d.elastic == 0.
 void launch_elastic(const ElDesc& d, const float* stress, const float* v, const MEForce& a, const MEForce& b,
                     float* out, ElMode mode, hipStream_t s);
+
+// launch_elastic in EL_ACCEL mode that also writes the next stage state of
+// (u, v) (ElNext).
+void launch_elastic_next(const ElDesc& d, const float* stress, const float* v, const MEForce& a, const MEForce& b,
+                         float* out, const ElNext& nx, hipStream_t s);
 
 enum ElLocal {
   EL_KINETIC_EDENS = 0,  // 0.5 rho |v|^2 (elasticenergies.cu:7-23), 1 component

@@ -124,11 +124,10 @@ __device__ __forceinline__ float3 me_force(const ElDesc& d, const MEForce& a, in
             2.f * b1 * m.z * d2.z + b2 * m.z * (d0.x + d1.y) + b2 * (m.x * d0.z + m.y * d1.z));
 }
 
-// The selected elastic term at cell idx (< ncell), written to out
-// (k_elastic's cell; see ElMode).
-__device__ __forceinline__ void elastic_cell(const ElDesc& d, const float* __restrict__ stress,
-                                             const float* __restrict__ v, const MEForce& a, const MEForce& b,
-                                             float* __restrict__ out, int mode, int idx) {
+// The selected elastic term at cell idx (< ncell; see ElMode).
+__device__ __forceinline__ float3 elastic_term(const ElDesc& d, const float* __restrict__ stress,
+                                               const float* __restrict__ v, const MEForce& a, const MEForce& b,
+                                               int mode, int idx) {
   const int N = d.ncell;
   float3 f = f3(0.f, 0.f, 0.f);
   if (in_geo(d, idx)) {
@@ -147,9 +146,45 @@ __device__ __forceinline__ void elastic_cell(const ElDesc& d, const float* __res
       f = p != 0.f ? (1.f / p) * f : f3(0.f, 0.f, 0.f);
     }
   }
+  return f;
+}
+
+// The selected elastic term at cell idx (< ncell), written to out
+// (k_elastic's cell).
+__device__ __forceinline__ void elastic_cell(const ElDesc& d, const float* __restrict__ stress,
+                                             const float* __restrict__ v, const MEForce& a, const MEForce& b,
+                                             float* __restrict__ out, int mode, int idx) {
+  const int N = d.ncell;
+  const float3 f = elastic_term(d, stress, v, a, b, mode, idx);
   out[idx] = f.x;
   out[(size_t)N + idx] = f.y;
   out[(size_t)2 * N + idx] = f.z;
+}
+
+// The next stage state of (u, v) at cell idx from the acceleration f just
+// computed there (ElNext); NK = nx.nk, known at compile time so that every
+// load is issued before the sums.
+template <int NK>
+__device__ __forceinline__ void elastic_next(const ElDesc& d, const ElNext& nx, float3 f, int idx) {
+  const int N = d.ncell;
+  const bool in = in_geo(d, idx);
+  const float fc[3] = {f.x, f.y, f.z};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = (size_t)c * N + idx;
+    float av = nx.v0[i];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) av += nx.c[j] * nx.ka[j][i];
+    av += nx.cself * fc[c];
+    float au = nx.u0[i];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) au += nx.c[j] * nx.ku[j][i];
+    au += nx.cself * nx.ku[NK][i];
+    const float rv = in ? av : 0.f, ru = in ? au : 0.f;
+    nx.vout[i] = rv;
+    nx.kdisp[i] = rv;
+    nx.uout[i] = ru;
+  }
 }
 
 }  // namespace elc

@@ -60,11 +60,13 @@ __device__ __forceinline__ float3 normalized3(float3 v) {
 }
 
 // UNI: every parameter is uniform, so each lookup is the scalar and the
-// harmonic mean of a cell and its neighbour is the value itself.
-template <bool UNI>
+// harmonic mean of a cell and its neighbour is the value itself.  NK >= 0:
+// the cell also writes the next Runge-Kutta stage state (nx, NK derivatives
+// from memory and the one just computed; x0 set).
+template <bool UNI, int NK = -2>
 __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restrict__ m,
                                            const float* __restrict__ hdemag, float* __restrict__ out, int mode,
-                                           int bid) {
+                                           int bid, const NextStage* nx = nullptr) {
   const int idx = bid * BLOCK + threadIdx.x;
   const int N = d.ncell;
   if (idx >= N) return;
@@ -85,6 +87,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
   if (!ingeom) {
     if (mode == MODE_EDENS) out[idx] = 0.f;
     else { out[idx] = 0.f; out[N + idx] = 0.f; out[2 * N + idx] = 0.f; }
+    if constexpr (NK >= 0) nx->template apply<NK>(idx, f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f), false);
     return;
   }
   const int x = idx % d.nx;
@@ -429,6 +432,7 @@ __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restr
   out[idx] = r.x;
   out[N + idx] = r.y;
   out[2 * N + idx] = r.z;
+  if constexpr (NK >= 0) nx->template apply<NK>(idx, mc, r, true);
 }
 
 template <bool UNI>
@@ -449,6 +453,18 @@ __global__ void __launch_bounds__(BLOCK) k_field_pair(FMDesc d0, const float* __
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   if (blockIdx.y == 0) field_body<UNI>(d0, m0, h0, o0, mode, bid);
   else field_body<UNI>(d1, m1, h1, o1, mode, bid);
+}
+
+// k_field_pair that also writes both sublattices' next stage states (n0, n1
+// with NK derivatives from memory each).
+template <bool UNI, int NK>
+__global__ void __launch_bounds__(BLOCK)
+    k_field_pair_next(FMDesc d0, const float* __restrict__ m0, const float* __restrict__ h0, float* __restrict__ o0,
+                      NextStage n0, FMDesc d1, const float* __restrict__ m1, const float* __restrict__ h1,
+                      float* __restrict__ o1, NextStage n1, int mode) {
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (blockIdx.y == 0) field_body<UNI, NK>(d0, m0, h0, o0, mode, bid, &n0);
+  else field_body<UNI, NK>(d1, m1, h1, o1, mode, bid, &n1);
 }
 
 // The torque of a sublattice cell from its field without demag (hl) and the
@@ -851,6 +867,35 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
   if (u0) k_field_pair<true><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
   else k_field_pair<false><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, d1, m1, h1, o1, (int)mode);
   checkLaunch("k_field_pair");
+}
+
+bool field_pair_next_fusable(const FMDesc& d0, const FMDesc& d1, FieldMode mode) {
+  const bool u0 = params_uniform(d0), u1 = params_uniform(d1);
+  return mode == MODE_LLG && d0.ncell == d1.ncell && u0 == u1 && !all_uniform(d0) && !all_uniform(d1) &&
+         !d0.noise && !d1.noise;
+}
+
+void launch_field_pair_next(const FMDesc& d0, const float* m0, const float* h0, float* o0, const NextStage& n0,
+                            const FMDesc& d1, const float* m1, const float* h1, float* o1, const NextStage& n1,
+                            FieldMode mode, hipStream_t s) {
+  if (!field_pair_next_fusable(d0, d1, mode) || n0.nk != n1.nk || !n0.x0 || !n1.x0)
+    throw Error(MPA_ERR_RUNTIME, "launch_field_pair_next: not a fusable sublattice pair");
+  const dim3 grid((d0.ncell + BLOCK - 1) / BLOCK, 2);
+  const bool u = params_uniform(d0);
+#define MPA_PAIR_NEXT(K)                                                                                        \
+  if (u) k_field_pair_next<true, K><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, n0, d1, m1, h1, o1, n1, (int)mode);  \
+  else k_field_pair_next<false, K><<<grid, BLOCK, 0, s>>>(d0, m0, h0, o0, n0, d1, m1, h1, o1, n1, (int)mode);
+  switch (n0.nk) {
+    case 0: MPA_PAIR_NEXT(0) break;
+    case 1: MPA_PAIR_NEXT(1) break;
+    case 2: MPA_PAIR_NEXT(2) break;
+    case 3: MPA_PAIR_NEXT(3) break;
+    case 4: MPA_PAIR_NEXT(4) break;
+    case 5: MPA_PAIR_NEXT(5) break;
+    default: throw Error(MPA_ERR_RUNTIME, "launch_field_pair_next: 0 to 5 earlier derivatives");
+  }
+#undef MPA_PAIR_NEXT
+  checkLaunch("k_field_pair_next");
 }
 
 bool torque_pair_splittable(const FMDesc& d0, const FMDesc& d1, FieldMode mode) {

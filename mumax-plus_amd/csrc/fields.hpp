@@ -162,6 +162,15 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
 // sublattice path, demag on, no spin-transfer torque, no thermal noise, not
 // frozen, no stray field of other magnets).  Lets the field without demag
 // run beside the demag passes.
+// The two sublattices' fields and torques in one launch (launch_field_pair's
+// pair kernel) that also writes each sublattice's next Runge-Kutta stage
+// state (NextStage, x0 set, 0 to 5 earlier derivatives); fusable: the pair
+// takes the one-launch pair kernel in MODE_LLG without thermal noise.
+bool field_pair_next_fusable(const FMDesc& d0, const FMDesc& d1, FieldMode mode);
+void launch_field_pair_next(const FMDesc& d0, const float* m0, const float* h0, float* o0, const NextStage& n0,
+                            const FMDesc& d1, const float* m1, const float* h1, float* o1, const NextStage& n1,
+                            FieldMode mode, hipStream_t s);
+
 bool torque_pair_splittable(const FMDesc& d0, const FMDesc& d1, FieldMode mode);
 void launch_torque_pair(const FMDesc& d0, const float* m0, const float* hl0, const float* hd0, float* o0,
                         const FMDesc& d1, const float* m1, const float* hl1, const float* hd1, float* o1,

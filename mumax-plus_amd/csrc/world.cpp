@@ -972,6 +972,26 @@ void Antiferromagnet::eval_rhs_pair(float* out0, float* out1, hipStream_t s, Fie
                     b->m.get(), b->demag_field(), out1, mode, s);
 }
 
+bool Antiferromagnet::next_fusable() {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  const unsigned ta = a->active_terms(), tb = b->active_terms();
+  if ((ta | tb) & TERM_THERMAL) return false;
+  return field_pair_next_fusable(a->desc(ta), b->desc(tb), MODE_LLG);
+}
+
+void Antiferromagnet::eval_rhs_pair_next(float* out0, float* out1, hipStream_t s, const NextStage& n0,
+                                         const NextStage& n1) {
+  Ferromagnet* a = sub[0].get();
+  Ferromagnet* b = sub[1].get();
+  const unsigned ta = a->active_terms(), tb = b->active_terms();
+  // plus each sublattice's combination: read x0 and nk derivatives, write the next stage state
+  const double bytes = a->rhs_bytes(ta) + b->rhs_bytes(tb) + 2 * 12.0 * N * (n0.nk + 2);
+  Timed t(&world->prof, "k_field_pair_next", bytes, s);
+  launch_field_pair_next(a->rhs_desc(ta, 0.0), a->m.get(), a->demag_field(), out0, n0, b->rhs_desc(tb, 0.0),
+                         b->m.get(), b->demag_field(), out1, n1, MODE_LLG, s);
+}
+
 bool Antiferromagnet::pair_split(FieldMode mode, double noise_dt) {
   Ferromagnet* a = sub[0].get();
   Ferromagnet* b = sub[1].get();
@@ -1358,7 +1378,7 @@ std::vector<StateVar> World::state_vars() {
 // with elastodynamics and the stray field of every antiferromagnet once,
 // then the torques, velocities and accelerations.
 void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready,
-                         double noise_dt, const NextStage* next, const StageIn* stage_in) {
+                         double noise_dt, const NextStage* next, const StageIn* stage_in, const PairNext* pair_next) {
   if (next && (vars.size() != 1 || vars[0].kind != StateVar::MAG || relaxing_))
     throw Error(MPA_ERR_RUNTIME, "a fused stage combination needs a single magnetisation variable");
   if (stage_in && (next || vars.size() != 1 || vars[0].kind != StateVar::MAG || vars[0].fm->host ||
@@ -1421,6 +1441,8 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
         if (fm == fm->host->sub[0].get() && vi + 1 < vars.size() && vars[vi + 1].kind == StateVar::MAG &&
             vars[vi + 1].fm == fm->host->sub[1].get()) {
           if (in_split(fm->host)) fm->host->eval_torque_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
+          else if (pair_next && pair_next->afm == fm->host)
+            fm->host->eval_rhs_pair_next(out, (*vars[vi + 1].k)[stage].get(), s, pair_next->m[0], pair_next->m[1]);
           else fm->host->eval_rhs_pair(out, (*vars[vi + 1].k)[stage].get(), s, mode, noise_dt);
           ++vi;
           continue;
@@ -1444,6 +1466,15 @@ void World::eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream
                                hipMemcpyDeviceToDevice, s));
     } else {
       MagnetCore* own = sv.owner;
+      if (pair_next && pair_next->has_el && pair_next->afm == own) {
+        // plus the (u, v) combinations: read u0, v0, nk accelerations and nk + 1
+        // velocities; write u, v and the next stage velocity
+        const int nk = pair_next->el.nk;
+        Timed t(&prof, "k_elastic_next", (48.0 + 12.0 * own->num_fm() + 12.0 * (2 * nk + 6)) * sv.N, sa);
+        launch_elastic_next(own->eldesc(), own->el.stress.get(), own->el.v.get(), own->me_force_of(0),
+                            own->num_fm() > 1 ? own->me_force_of(1) : MEForce{}, out, pair_next->el, sa);
+        continue;
+      }
       // read stress (24 B), v (12 B) and each coupled magnetisation (12 B); write a (12 B)
       Timed t(&prof, "k_elastic", (48.0 + 12.0 * own->num_fm()) * sv.N, sa);
       launch_elastic(own->eldesc(), own->el.stress.get(), own->el.v.get(), own->me_force_of(0),
@@ -1761,6 +1792,30 @@ void TimeSolver::step() {
   bool next_ready = fused && k0_valid && next_valid_ && next_h_ == dt;
   next_valid_ = false;
   if (fused && fused->xn.n == 0) fused->xn.allocate((size_t)3 * fused->N);
+  // One antiferromagnet on the sublattice-pair kernel (with its displacement
+  // and velocity when it has elastodynamics) and nothing else: each
+  // evaluation but the last also writes the next stage state of every
+  // variable, the sublattices' in the field launch and (u, v) in the
+  // acceleration launch, with the combination's arithmetic.
+  Antiferromagnet* afused = nullptr;
+  if (fuse_stages && !fused && !xfuse && !noisy && S > 1 && !world_->relaxing() && world_->magnets.empty() &&
+      world_->afms.size() == 1 && !world_->split_elastic && !world_->split_demag) {
+    Antiferromagnet* a = world_->afms[0].get();
+    const bool el = a->el.enabled;
+    const bool layout = vars.size() == (el ? 4u : 2u) && vars[0].fm == a->sub[0].get() &&
+                        vars[1].fm == a->sub[1].get() &&
+                        (!el || (vars[2].kind == StateVar::DISP && vars[3].kind == StateVar::VEL &&
+                                 vars[2].owner == a && vars[3].owner == a));
+    if (layout && a->next_fusable()) afused = a;
+  }
+  if (afused) {
+    for (auto& f : afused->sub)
+      if (f->xn.n == 0) f->xn.allocate((size_t)3 * f->N);
+    if (afused->el.enabled && afused->el.un.n == 0) {
+      afused->el.un.allocate((size_t)3 * afused->N);
+      afused->el.vn.allocate((size_t)3 * afused->N);
+    }
+  }
   // x0 = the state at the step start.  When the first stage's derivative is
   // reused, nothing reads the state before the second stage's combination
   // overwrites it, so the state array and x0 trade buffers instead of copying.
@@ -1818,6 +1873,7 @@ void TimeSolver::step() {
     const double h = dt;
     bool have_next = next_ready;  // the magnet's xn holds the coming stage's state
     next_ready = false;
+    bool have_pair_next = false;  // the antiferromagnet's xn, un, vn hold it
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
       World::StageIn sin{};
@@ -1827,6 +1883,14 @@ void TimeSolver::step() {
           std::swap(fused->m, fused->xn);
           vars[0].x = fused->m.get();
           have_next = false;
+        } else if (have_pair_next) {
+          for (auto& f : afused->sub) std::swap(f->m, f->xn);
+          if (afused->el.enabled) {
+            std::swap(afused->el.u, afused->el.un);
+            std::swap(afused->el.v, afused->el.vn);
+          }
+          for (auto& sv : vars) sv.x = sv.xa->get();
+          have_pair_next = false;
         } else if (xfuse) {
           sin.x0 = vars[0].x0->get();
           sin.co.nk = st;
@@ -1860,6 +1924,41 @@ void TimeSolver::step() {
         } else {
           have_next = true;
         }
+      } else if (afused && st + 1 < S) {
+        World::PairNext pn{};
+        pn.afm = afused;
+        const std::vector<double>& a = tb.A[st + 1];
+        for (int i = 0; i < 2; ++i) {
+          NextStage& ns = pn.m[i];
+          ns.x0 = vars[i].x0->get();
+          ns.nk = st;
+          for (int j = 0; j < st; ++j) {
+            ns.k[j] = (*vars[i].k)[j].get();
+            ns.c[j] = (float)(h * a[j]);
+          }
+          ns.cself = (float)(h * a[st]);
+          ns.out = afused->sub[i]->xn.get();
+          ns.N = vars[i].N;
+        }
+        if (afused->el.enabled) {
+          pn.has_el = true;
+          ElNext& e = pn.el;
+          const StateVar &du = vars[2], &dv = vars[3];
+          e.u0 = du.x0->get();
+          e.v0 = dv.x0->get();
+          e.nk = st;
+          for (int j = 0; j < st; ++j) {
+            e.ka[j] = (*dv.k)[j].get();
+            e.c[j] = (float)(h * a[j]);
+          }
+          for (int j = 0; j <= st; ++j) e.ku[j] = (*du.k)[j].get();
+          e.cself = (float)(h * a[st]);
+          e.uout = afused->el.un.get();
+          e.vout = afused->el.vn.get();
+          e.kdisp = (*du.k)[st + 1].get();
+        }
+        world_->eval_rhs_all(vars, st, s, st > 0, 0.0, nullptr, nullptr, &pn);
+        have_pair_next = true;
       } else {
         world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0, nullptr, stage_in);
       }

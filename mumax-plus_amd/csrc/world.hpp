@@ -120,6 +120,7 @@ struct ElasticState {
   bool enabled = false;
   DeviceArray<float> u, v;    // 3N each
   DeviceArray<float> u0, v0;  // step start
+  DeviceArray<float> un, vn;  // next stage state formed in the acceleration launch
   std::vector<DeviceArray<float>> ku, kv;
   DeviceArray<float> strain, stress;  // 6N each
 };
@@ -295,6 +296,11 @@ class Antiferromagnet : public MagnetCore {
   Ferromagnet* fm_at(int i) override { return sub[i].get(); }
   void set_regions(const uint32_t* regions) override;  // the sublattices share the host's regions
   void eval_rhs_pair(float* out0, float* out1, hipStream_t s, FieldMode mode, double noise_dt);
+  // eval_rhs_pair that also writes both sublattices' next stage states;
+  // next_fusable tells whether the pair takes it (uniform-or-not alike,
+  // MODE_LLG, no thermal noise).
+  bool next_fusable();
+  void eval_rhs_pair_next(float* out0, float* out1, hipStream_t s, const NextStage& n0, const NextStage& n1);
   // eval_rhs_pair in two parts, so that the first can run beside the demag
   // passes: the sublattices' fields without demag (into hloc), then their
   // torques from those and the demag field.  pair_split tells whether the
@@ -436,8 +442,18 @@ class World {
     const float* x0;
     Combo co;
   };
+  // pair_next: an antiferromagnet's evaluation also writes the next stage
+  // state of both sublattices (in its field launch) and, with elastodynamics,
+  // of its displacement and velocity (in its acceleration launch).
+  struct PairNext {
+    Antiferromagnet* afm;
+    NextStage m[2];
+    bool has_el;
+    ElNext el;
+  };
   void eval_rhs_all(const std::vector<StateVar>& vars, int stage, hipStream_t s, bool disp_ready = false,
-                    double noise_dt = 0.0, const NextStage* next = nullptr, const StageIn* stage_in = nullptr);
+                    double noise_dt = 0.0, const NextStage* next = nullptr, const StageIn* stage_in = nullptr,
+                    const PairNext* pair_next = nullptr);
   bool relaxing() const { return relaxing_; }
 
   // Field of `src` on the grid (size, origin); method 0 = FFT, 1 = direct sum.

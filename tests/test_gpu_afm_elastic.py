@@ -528,3 +528,54 @@ def test_afm_magnetoelastic_sum_rules():
     assert semirel(magnet.total_energy_density.eval(), edens) < 1e-5
     assert magnet.total_energy.eval() == pytest.approx(
         magnet.total_energy_density.average()[0] * nx * ny * nz * 1.5e-9 * 2e-9 * 2.5e-9, rel=1e-6)
+
+
+@pytest.mark.parametrize("method,adaptive,elastic,geometry",
+                         [("DormandPrince", True, True, False), ("DormandPrince", False, True, True),
+                          ("Fehlberg", True, True, False), ("DormandPrince", True, False, False)])
+def test_stage_states_formed_in_the_field_and_acceleration_launches(method, adaptive, elastic, geometry):
+    """World option fuse_stages (default on): for a lone antiferromagnet the
+    field launch of each stage but the last also writes both sublattices'
+    next stage states, and the acceleration launch those of u and v (the
+    displacement's next derivative included), with the combination launch's
+    arithmetic and term order; m1, m2, u, v and the time equal the separate
+    combination's bit for bit, over fixed and adaptive steps, with an FSAL
+    tableau and without, with a geometry mask, and without elastodynamics."""
+    out = []
+    for fuse in (0, 1):
+        rng = np.random.default_rng(31)
+        nx, ny, nz = 24, 20, 2
+        shape = (nz, ny, nx)
+        geom = None
+        if geometry:
+            x = (np.arange(nx) + 0.5) / nx - 0.5
+            y = (np.arange(ny) + 0.5) / ny - 0.5
+            geom = np.broadcast_to((x[None, :] ** 2 + y[:, None] ** 2) <= 0.25, shape).copy()
+        sub = {"msat": 5.66e5, "aex": 2.48e-12, "alpha": 0.05, "B1": -8.8e6, "B2": -8.8e6}
+        el = None
+        if elastic:
+            el = {"C11": 283e9, "C12": 166e9, "C44": 58e9, "rho": 8900.0, "eta": 1e10,
+                  "u": 1e-11 * rng.normal(size=(3,) + shape), "v": 10 * rng.normal(size=(3,) + shape)}
+        p = AfmPair(rng, size=(nx, ny, nz), cs=(2e-9, 2e-9, 2e-9), geometry=geom, afm={"afmex_cell": -4e-12},
+                    subs=(dict(sub), dict(sub)), inter_dmi={"xxz": 5e-4, "yyz": 5e-4}, elastic=el)
+        p.world._impl.set_option("fuse_stages", fuse)
+        ts = p.world.timesolver
+        ts.set_method(method)
+        if not adaptive:
+            ts.adaptive_timestep = False
+            ts.timestep = 2e-14
+        p.world._impl.set_profiling(True)
+        ts.steps(12)
+        launches = p.world._impl.profile_report()
+        p.world._impl.set_profiling(False)
+        assert ("k_field_pair_next" in launches) == bool(fuse), launches.keys()
+        if elastic:
+            assert ("k_elastic_next" in launches) == bool(fuse), launches.keys()
+        res = [s_.magnetization.get() for s_ in p.subs]
+        if elastic:
+            res += [p.afm.elastic_displacement.get(), p.afm.elastic_velocity.get()]
+        out.append((res, ts.time, ts.timestep))
+    (ra, ta, da), (rb, tb, db) = out
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+    assert ta == tb and da == db
