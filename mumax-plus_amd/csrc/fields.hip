@@ -62,7 +62,7 @@ __device__ __forceinline__ float3 normalized3(float3 v) {
 // UNI: every parameter is uniform, so each lookup is the scalar and the
 // harmonic mean of a cell and its neighbour is the value itself.  NK >= 0:
 // the cell also writes the next Runge-Kutta stage state (nx, NK derivatives
-// from memory and the one just computed; x0 set).
+// from memory and the one just computed; x0 == nullptr: the cell's state).
 template <bool UNI, int NK = -2>
 __device__ __forceinline__ void field_body(const FMDesc& d, const float* __restrict__ m,
                                            const float* __restrict__ hdemag, float* __restrict__ out, int mode,
@@ -878,7 +878,7 @@ bool field_pair_next_fusable(const FMDesc& d0, const FMDesc& d1, FieldMode mode)
 void launch_field_pair_next(const FMDesc& d0, const float* m0, const float* h0, float* o0, const NextStage& n0,
                             const FMDesc& d1, const float* m1, const float* h1, float* o1, const NextStage& n1,
                             FieldMode mode, hipStream_t s) {
-  if (!field_pair_next_fusable(d0, d1, mode) || n0.nk != n1.nk || !n0.x0 || !n1.x0)
+  if (!field_pair_next_fusable(d0, d1, mode) || n0.nk != n1.nk || (n0.nk > 0 && (!n0.x0 || !n1.x0)))
     throw Error(MPA_ERR_RUNTIME, "launch_field_pair_next: not a fusable sublattice pair");
   const dim3 grid((d0.ncell + BLOCK - 1) / BLOCK, 2);
   const bool u = params_uniform(d0);

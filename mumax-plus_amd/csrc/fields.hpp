@@ -164,7 +164,8 @@ void launch_field_pair(const FMDesc& d0, const float* m0, const float* h0, float
 // run beside the demag passes.
 // The two sublattices' fields and torques in one launch (launch_field_pair's
 // pair kernel) that also writes each sublattice's next Runge-Kutta stage
-// state (NextStage, x0 set, 0 to 5 earlier derivatives); fusable: the pair
+// state (NextStage, 0 to 5 earlier derivatives, x0 == nullptr only with
+// none: the next step's first stage); fusable: the pair
 // takes the one-launch pair kernel in MODE_LLG without thermal noise.
 bool field_pair_next_fusable(const FMDesc& d0, const FMDesc& d1, FieldMode mode);
 void launch_field_pair_next(const FMDesc& d0, const float* m0, const float* h0, float* o0, const NextStage& n0,

@@ -1789,7 +1789,8 @@ void TimeSolver::step() {
   // the state (no combination launch).
   const bool xfuse = !fused && fuse_combine && vars.size() == 1 && vars[0].kind == StateVar::MAG &&
                      !vars[0].fm->host && vars[0].normalize && (vars[0].fm->active_terms() & TERM_DEMAG);
-  bool next_ready = fused && k0_valid && next_valid_ && next_h_ == dt;
+  const bool next_kept = k0_valid && next_valid_ && next_h_ == dt;
+  bool next_ready = fused && next_kept;
   next_valid_ = false;
   if (fused && fused->xn.n == 0) fused->xn.allocate((size_t)3 * fused->N);
   // One antiferromagnet on the sublattice-pair kernel (with its displacement
@@ -1808,6 +1809,9 @@ void TimeSolver::step() {
                                  vars[2].owner == a && vars[3].owner == a));
     if (layout && a->next_fusable()) afused = a;
   }
+  // with a fixed step the previous step's last evaluation formed this step's
+  // first stage state (xn, un, vn and the displacement's k[1])
+  bool pair_ready = afused && next_kept;
   if (afused) {
     for (auto& f : afused->sub)
       if (f->xn.n == 0) f->xn.allocate((size_t)3 * f->N);
@@ -1873,7 +1877,8 @@ void TimeSolver::step() {
     const double h = dt;
     bool have_next = next_ready;  // the magnet's xn holds the coming stage's state
     next_ready = false;
-    bool have_pair_next = false;  // the antiferromagnet's xn, un, vn hold it
+    bool have_pair_next = pair_ready;  // the antiferromagnet's xn, un, vn hold it
+    pair_ready = false;
     for (int st = 0; st < S; ++st) {
       time = t0 + h * tb.nodes[st];
       World::StageIn sin{};
@@ -1924,19 +1929,25 @@ void TimeSolver::step() {
         } else {
           have_next = true;
         }
-      } else if (afused && st + 1 < S) {
+      } else if (afused && (st + 1 < S || (st + 1 == S && !adaptive && tb.fsal))) {
+        // st + 1 == S: the next step's first stage state, from this step's
+        // final state (the stage state itself) and its derivative, for the
+        // same fixed step
+        const bool cross = st + 1 == S;
         World::PairNext pn{};
         pn.afm = afused;
-        const std::vector<double>& a = tb.A[st + 1];
+        const std::vector<double>& a = tb.A[cross ? 1 : st + 1];
+        const int nk = cross ? 0 : st;
+        const float cself = (float)(h * a[cross ? 0 : st]);
         for (int i = 0; i < 2; ++i) {
           NextStage& ns = pn.m[i];
-          ns.x0 = vars[i].x0->get();
-          ns.nk = st;
-          for (int j = 0; j < st; ++j) {
+          ns.x0 = cross ? nullptr : vars[i].x0->get();
+          ns.nk = nk;
+          for (int j = 0; j < nk; ++j) {
             ns.k[j] = (*vars[i].k)[j].get();
             ns.c[j] = (float)(h * a[j]);
           }
-          ns.cself = (float)(h * a[st]);
+          ns.cself = cself;
           ns.out = afused->sub[i]->xn.get();
           ns.N = vars[i].N;
         }
@@ -1944,21 +1955,29 @@ void TimeSolver::step() {
           pn.has_el = true;
           ElNext& e = pn.el;
           const StateVar &du = vars[2], &dv = vars[3];
-          e.u0 = du.x0->get();
-          e.v0 = dv.x0->get();
-          e.nk = st;
-          for (int j = 0; j < st; ++j) {
+          // across steps: x0 = the current state; the displacement's first
+          // derivative is the current velocity
+          e.u0 = cross ? du.x : du.x0->get();
+          e.v0 = cross ? dv.x : dv.x0->get();
+          e.nk = nk;
+          for (int j = 0; j < nk; ++j) {
             e.ka[j] = (*dv.k)[j].get();
             e.c[j] = (float)(h * a[j]);
           }
-          for (int j = 0; j <= st; ++j) e.ku[j] = (*du.k)[j].get();
-          e.cself = (float)(h * a[st]);
+          for (int j = 0; j <= nk; ++j) e.ku[j] = (*du.k)[j].get();
+          if (cross) e.ku[0] = dv.x;
+          e.cself = cself;
           e.uout = afused->el.un.get();
           e.vout = afused->el.vn.get();
-          e.kdisp = (*du.k)[st + 1].get();
+          e.kdisp = (*du.k)[cross ? 1 : st + 1].get();
         }
         world_->eval_rhs_all(vars, st, s, st > 0, 0.0, nullptr, nullptr, &pn);
-        have_pair_next = true;
+        if (cross) {
+          next_valid_ = true;
+          next_h_ = h;
+        } else {
+          have_pair_next = true;
+        }
       } else {
         world_->eval_rhs_all(vars, st, s, st > 0, noisy ? h : 0.0, nullptr, stage_in);
       }
