@@ -352,8 +352,10 @@ int mpa_world_set_profiling(mpa_world* w, int enable);
 /* Stepper switches (no counterpart in the reference).  "fuse_stages" (default 1): a world whose
  * only variable is one magnetisation on the uniform-parameter LLG path, stepped with a
  * first-same-as-last tableau, computes each stage state inside the field launch that produces
- * the last derivative it needs (no separate combination launch); 0 runs the separate
- * combination, with the same arithmetic.  "fuse_combine" (default 0): when the only variable is one
+ * the last derivative it needs (no separate combination launch); so does a world whose only magnet
+ * is one antiferromagnet on the sublattice-pair field kernel (any tableau; its displacement and
+ * velocity, with elastodynamics, inside the acceleration launch); 0 runs the separate
+ * combination, with the same arithmetic (the same values bit for bit).  "fuse_combine" (default 0): when the only variable is one
  * independent magnetisation with demag, each stage state is formed inside the demag's first FFT
  * pass, which reads the step start and the derivatives instead of the state (no separate
  * combination launch); 0 runs the separate combination.  "field_zchunk" (default 4): z planes per workgroup of the
