@@ -482,7 +482,7 @@ def measure_serve(args):
     for r in warm[:1]:
         r.solve_plain()
     # served stream
-    server = serving.Server(SERVE_CAP, poll_steps=16, step_mode=args.batch_step_mode)
+    server = serving.Server(SERVE_CAP, poll_steps=args.serve_poll, step_mode=args.batch_step_mode)
     t0 = time.perf_counter()
     res = server.serve(reqs, realtime=True)
     wall = time.perf_counter() - t0
@@ -508,11 +508,11 @@ def measure_serve(args):
                                     - np.array(byid[r.rid].magnet.magnetization.average())).max()))
     wall_plain = time.perf_counter() - t0
     # the same first requests served, for a like-for-like latency
-    res_sub = serving.Server(SERVE_CAP, poll_steps=16, step_mode=args.batch_step_mode).serve(sub, realtime=True) \
-        if sub else []
+    sub_server = serving.Server(SERVE_CAP, poll_steps=args.serve_poll, step_mode=args.batch_step_mode)
+    res_sub = sub_server.serve(sub, realtime=True) if sub else []
     sub_prof = reqs[:SERVE_PLAIN]
     # per-kernel roofline of a profiled pass over the first requests
-    pserver = serving.Server(SERVE_CAP, poll_steps=16)
+    pserver = serving.Server(SERVE_CAP, poll_steps=args.serve_poll)
     pserver.batch.set_profiling(True)
     pserver.serve(sub_prof, realtime=False)
     rep = pserver.batch.profile_report()
@@ -545,7 +545,7 @@ def measure_serve(args):
                                "(damping-only until max torque <= 1e9 /s), half switching runs (10-200 mT "
                                "pulses against the easy axis for 0.1-2 ns), adaptive Dormand-Prince",
                    "requests": SERVE_N, "rate_per_s": SERVE_RATE, "capacity_cells": SERVE_CAP,
-                   "poll_steps": 16, "seed": args.seed},
+                   "poll_steps": args.serve_poll, "seed": args.seed},
         "latency_s": {"p50": _pct(lat, 50), "p99": _pct(lat, 99), "max": max(lat)},
         "wall_s": wall,
         "server": dict(server.stats, ms_per_batch_step=1e3 * server.stats["advance_s"] / max(server.stats["batch_steps"], 1)),
@@ -686,6 +686,8 @@ def main():
     ap.add_argument("--serial-elastic", action="store_true",
                     help="afm workload: strains, accelerations and sublattice fields on the world's stream, "
                          "not beside the demag passes (the default; also switches split_demag off)")
+    ap.add_argument("--serve-poll", type=int, default=32,
+                    help="served workload: batch steps per server poll (admissions and retirements between polls)")
     ap.add_argument("--world-option", action="append", metavar="KEY=VALUE",
                     help="default and afm workloads: set a world option (mpa_world_set_option), repeatable")
     ap.add_argument("--split-elastic", action="store_true",

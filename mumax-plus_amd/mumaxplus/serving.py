@@ -97,12 +97,12 @@ class Result:
 
 
 class Server:
-    """Server(capacity, poll_steps=16, step_mode="passes"): continuous
+    """Server(capacity, poll_steps=32, step_mode="passes"): continuous
     batching of requests in a Batch of `capacity` packed cells; finished
     requests are retired and waiting ones admitted every `poll_steps` batch
     steps.  `step_mode` is the batch's step path (Batch.step_mode)."""
 
-    def __init__(self, capacity, poll_steps=16, method="DormandPrince", step_mode="passes"):
+    def __init__(self, capacity, poll_steps=32, method="DormandPrince", step_mode="passes"):
         self.batch = Batch(capacity, method)
         self.batch.step_mode = step_mode
         self.poll_steps = int(poll_steps)
